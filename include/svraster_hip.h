@@ -1,0 +1,164 @@
+/*
+ * C-ABI of the MI355X (gfx950) sparse-voxel SDF rasterizer library, libsvraster_hip.so.
+ *
+ * Plain pointers and sizes only.  Every pointer argument named like a tensor is DEVICE memory
+ * (float32 / int64 / int32 / uint8 as stated); `stream` is a hipStream_t (NULL = the null
+ * stream).  Functions return 0 on success and a nonzero code on failure, with a message
+ * available from svr_last_error() (thread-local).  Inputs are read-only unless documented as
+ * outputs or in-place; "zeroed by the caller" outputs are accumulated into.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * reference's cuda/ directory).  The reference binds these through pybind11
+ * (binding.cpp:28-58); here the Python module svraster_amd/_C.py binds them with ctypes.
+ */
+#ifndef SVRASTER_HIP_H_
+#define SVRASTER_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failed call on this thread ("" when none). */
+const char* svr_last_error(void);
+/* ABI version of this header (bumped on any signature change). */
+int svr_abi_version(void);
+
+/* HIP-event timing of the compositor launches on the caller's stream (no counterpart in the
+ * reference; for benchmarks).  kind 0 = forward compositor, 1 = backward compositor (also the
+ * batched launches).  svr_timing_read synchronises on the recorded events and returns their
+ * summed duration; reset != 0 clears the record. */
+void svr_timing_enable(int on);
+int svr_timing_read(int kind, int reset, double* total_ms, long long* launches);
+
+/* ---- opaque state buffer sizes (the reference's raster_state.h:41-78 roles) ---------------- */
+size_t svr_geom_state_bytes(int n_views, int P);
+size_t svr_binning_state_bytes(long long K);
+size_t svr_image_state_bytes(int n_views, long long n_pixels, long long n_tiles);
+
+/* Replaces PREPROCESS::rasterize_preprocess (src/preprocess.cu:158-228).
+ * w2c, c2w: device float[16] row-major.  out_n_dup: int32[P].  geom_buffer: device bytes of
+ * svr_geom_state_bytes(1, P).  cam_mode must be CAM_PERSP (0). */
+int svr_rasterize_preprocess(int W, int H, float tanfovx, float tanfovy, float cx, float cy,
+                             const float* w2c, const float* c2w, int cam_mode, float near_plane,
+                             const int64_t* octree_paths, const float* vox_centers,
+                             const float* vox_lengths, const uint8_t* is_leaf, int P,
+                             int32_t* out_n_dup, void* geom_buffer, void* stream);
+
+/* Replaces the scan + num_rendered readback of FORWARD::rasterize_voxels_procedure
+ * (src/forward.cu:881-897).  Synchronises `stream`.  K_out: number of (tile, rank) keys;
+ * rank_shift_out: low rank bits that are zero in every visible voxel's octpath (a multiple of
+ * 3), passed on to svr_rasterize_forward. */
+int svr_rasterize_count(void* geom_buffer, int P, long long* K_out, int* rank_shift_out, void* stream);
+
+/* Replaces FORWARD::rasterize_voxels (src/forward.cu:986-1096) after the count: duplication,
+ * sort, tile ranges and compositing.  Only density_mode = SDF (1) with vox_geo_mode =
+ * triinterp3 (3), cam_mode = 0 and need_distortion = 0 are supported.
+ * Outputs (zeroed by the caller): out_color [3,H,W], out_depth [4,H,W] (when need_depth),
+ * out_normal [3,H,W] (when need_normal), out_T [H,W], max_w [P] (may be NULL).
+ * binning_buffer: svr_binning_state_bytes(K); image_buffer: svr_image_state_bytes(1, W*H, tiles). */
+int svr_rasterize_forward(int vox_geo_mode, int density_mode, int W, int H, float tanfovx,
+                          float tanfovy, float cx, float cy, const float* w2c, const float* c2w,
+                          const float* background, int cam_mode, int need_depth,
+                          int need_distortion, int need_normal, const int64_t* octree_paths,
+                          const float* vox_centers, const float* vox_lengths, const float* geos,
+                          const float* rgbs, int P, void* geom_buffer, long long K, int rank_shift,
+                          void* binning_buffer, void* image_buffer, float* out_color,
+                          float* out_depth, float* out_normal, float* out_T, float* max_w,
+                          const float* s_val, void* stream);
+
+/* Replaces BACKWARD::rasterize_voxels_backward (src/backward.cu:986-1109).
+ * dL_dvox [P,12] = [dL/dgeos(8) | dL/drgbs(3) | subdivision priority(1)] and dL_ds [1] are
+ * zeroed by the caller.  dL_ddepth / dL_dnormal / gt_color may be NULL when unused.
+ * lambda_ascending and lambda_dist must be 0. */
+int svr_rasterize_backward(long long K, int vox_geo_mode, int density_mode, int W, int H,
+                           float tanfovx, float tanfovy, float cx, float cy, const float* w2c,
+                           const float* c2w, const float* background, const float* vox_centers,
+                           const float* vox_lengths, const float* geos, const float* rgbs, int P,
+                           void* geom_buffer, void* binning_buffer, void* image_buffer,
+                           const float* out_T, const float* dL_dcolor, const float* dL_ddepth,
+                           const float* dL_dnormal, const float* dL_dT, float lambda_R_concen,
+                           const float* gt_color, float lambda_ascending, float lambda_dist,
+                           int need_depth, int need_normal, const float* s_val, float* dL_dvox,
+                           float* dL_ds, void* stream);
+
+/* Replaces SH_COMPUTE::sh_compute (src/sh_compute.cu:244-270).  idx: int64[n_idx] or NULL
+ * (all P voxels).  shs: [P, M-1, 3].  rgbs [P,3] zeroed by the caller. */
+int svr_sh_compute(int degree, const int64_t* idx, int n_idx, const float* vox_centers, int P,
+                   const float* cam_pos, const float* sh0, const float* shs, int M, float* rgbs,
+                   void* stream);
+
+/* Replaces SH_COMPUTE::sh_compute_bw (src/sh_compute.cu:272-299).  dL_dsh0 [P,3] and
+ * dL_dshs [P,M-1,3] zeroed by the caller. */
+int svr_sh_compute_bw(int degree, int M, const int64_t* idx, int n_idx, const float* vox_centers,
+                      int P, const float* cam_pos, const float* dL_drgbs, float* dL_dsh0,
+                      float* dL_dshs, void* stream);
+
+/* Replaces GEO_PARAMS_GATHER::gather_triinterp_geo_params (src/geo_params_gather.cu:134-152).
+ * geos [P,8]: rows not in care_idx are left untouched. */
+int svr_gather_triinterp_geo_params(const int64_t* vox_key, int P, const int64_t* care_idx,
+                                    int n_care, const float* grid_pts, float* geos, void* stream);
+
+/* Replaces gather_triinterp_geo_params_bw (src/geo_params_gather.cu:154-173).
+ * dL_dgrid_pts [M] zeroed by the caller. */
+int svr_gather_triinterp_geo_params_bw(const int64_t* vox_key, int P, const int64_t* care_idx,
+                                       int n_care, const float* dL_dgeos, float* dL_dgrid_pts,
+                                       void* stream);
+
+/* Replaces ADAM_STEP::unbiased_adam_step (src/adam_step.cu:106-135).  In place. */
+int svr_unbiased_adam_step(int sparse, float* param, const float* grad, float* exp_avg,
+                           float* exp_avg_sq, long long numel, double step, double lr,
+                           double beta1, double beta2, float eps, void* stream);
+
+/* Replaces ADAM_STEP::biased_adam_step (src/adam_step.cu:137-158).  In place. */
+int svr_biased_adam_step(int sparse, float* param, const float* grad, float* exp_avg,
+                         float* exp_avg_sq, long long numel, float lr, float beta1, float beta2,
+                         float eps, void* stream);
+
+/* Replace GE_COMPUTE::grid_eikonal_bw (src/ge_compute.cu:120-161) and
+ * LS_COMPUTE::laplacian_smoothness_bw (src/ls_compute.cu:112-155).  Accumulate into
+ * grid_pts_grad [n_pts].  grid_mask: uint8[res^3]; grid_keys/grid2voxel: int32[G] sorted by
+ * key; active_list: int32[A].  The per-key random offset is randv[A,3] when given, else a PCG
+ * hash of (seed, key index). */
+int svr_grid_eikonal_bw(const float* grid_pts, long long n_pts, const int64_t* vox_key, int N,
+                        const float* grid_voxel_coord, const float* grid_voxel_size, int grid_res,
+                        const uint8_t* grid_mask, const int32_t* grid_keys,
+                        const int32_t* grid2voxel, int G, const int32_t* active_list, int A,
+                        float weight, float vox_size_inv, unsigned int seed, const float* randv,
+                        float* grid_pts_grad, void* stream);
+int svr_laplacian_smoothness_bw(const float* grid_pts, long long n_pts, const int64_t* vox_key,
+                                int N, const float* grid_voxel_coord,
+                                const float* grid_voxel_size, int grid_res,
+                                const uint8_t* grid_mask, const int32_t* grid_keys,
+                                const int32_t* grid2voxel, int G, const int32_t* active_list,
+                                int A, float weight, float vox_size_inv, unsigned int seed,
+                                const float* randv, float* grid_pts_grad, void* stream);
+
+/* Replace UTILS::ijk_2_octpath / octpath_2_ijk (src/utils.cu:129-189, 329-361). */
+int svr_ijk_2_octpath(const int64_t* ijk, const int8_t* octlevel, int n, int64_t* octpath,
+                      void* stream);
+int svr_octpath_2_ijk(const int64_t* octpath, const int8_t* octlevel, int n, int64_t* ijk,
+                      void* stream);
+
+/* Replaces UTILS::valid_gradient_table (src/utils.cu:190-252, 362-408) in two calls: count
+ * (synchronises `stream`; workspace of svr_valid_gradient_table_workspace_bytes(N)), then
+ * fill.  grid_mask uint8[(2^grid_res_pow2)^3] zeroed by the caller; keys / voxel ids int32[count]
+ * in voxel order (deterministic). */
+size_t svr_valid_gradient_table_workspace_bytes(int N);
+int svr_valid_gradient_table_count(const float* vox_center, const float* vox_size,
+                                   const float* scene_center, float inside_extent,
+                                   int grid_res_pow2, const uint8_t* is_leaf, int N,
+                                   void* workspace, long long* count_out, void* stream);
+int svr_valid_gradient_table_fill(const float* vox_center, const float* vox_size,
+                                  const float* scene_center, float inside_extent,
+                                  int grid_res_pow2, const uint8_t* is_leaf, int N,
+                                  void* workspace, uint8_t* grid_mask, int32_t* grid_keys,
+                                  int32_t* grid2voxel, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVRASTER_HIP_H_ */

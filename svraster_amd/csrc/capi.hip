@@ -1,0 +1,478 @@
+// C-ABI entry points of libsvraster_hip.so (declared in include/svraster_hip.h).
+//
+// Host orchestration of one rasterizer call, restating the steps of the reference's host
+// functions (cuda/src/preprocess.cu:158-228, forward.cu:832-982 and 986-1096,
+// backward.cu:986-1109, sh_compute.cu:244-299, geo_params_gather.cu:134-173,
+// adam_step.cu:106-158, ge_compute.cu:120-161, ls_compute.cu:112-155, utils.cu:329-408) on
+// top of this library's kernels.  All launches go to the caller's stream.
+#include <stdio.h>
+#include <string.h>
+#include <math.h>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/svraster_hip.h"
+#include "common.h"
+#include "ops.h"
+#include "state.h"
+
+using namespace svr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, const char* a = "", long long b = 0) {
+  char buf[512];
+  snprintf(buf, sizeof(buf), fmt, a, b);
+  g_err = buf;
+  return code;
+}
+
+int fail_hip(hipError_t e, const char* where) {
+  char buf[512];
+  snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), where);
+  g_err = buf;
+  return 2;
+}
+
+#define SVR_TRY(expr)                                      \
+  do {                                                     \
+    hipError_t _e = (expr);                                \
+    if (_e != hipSuccess) return fail_hip(_e, #expr);      \
+  } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Optional HIP-event timing of the compositor launches (enabled by svr_timing_enable).
+constexpr int kTimedKinds = 2;  // 0 = forward compositor, 1 = backward compositor
+struct TimedLaunch { hipEvent_t a, b; };
+std::mutex g_timing_mu;
+bool g_timing_on = false;
+std::vector<TimedLaunch> g_timing[kTimedKinds];
+
+struct LaunchTimer {
+  int kind; hipStream_t st; TimedLaunch ev{nullptr, nullptr}; bool on = false;
+  LaunchTimer(int k, hipStream_t s) : kind(k), st(s) {
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    on = g_timing_on;
+    if (on && hipEventCreate(&ev.a) == hipSuccess && hipEventCreate(&ev.b) == hipSuccess) {
+      (void)hipEventRecord(ev.a, st);
+    } else {
+      on = false;
+    }
+  }
+  ~LaunchTimer() {
+    if (!on) return;
+    (void)hipEventRecord(ev.b, st);
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    g_timing[kind].push_back(ev);
+  }
+};
+
+int tile_bits_for(long long n_tiles) {
+  int b = 0;
+  while ((1LL << b) < n_tiles) ++b;
+  return b;
+}
+
+ViewDesc make_view(int W, int H, float tanfovx, float tanfovy, float cx, float cy, float near_plane) {
+  ViewDesc v;
+  memset(&v, 0, sizeof(v));
+  v.W = W; v.H = H;
+  v.tiles_x = ceil_div(W, kTile);
+  v.tiles_y = ceil_div(H, kTile);
+  v.tanfovx = tanfovx; v.tanfovy = tanfovy; v.cx = cx; v.cy = cy;
+  // Focal lengths in FP32 with the reference's operation order (preprocess.cu:197-198).
+  v.focal_x = (0.5f * (float)W) / tanfovx;
+  v.focal_y = (0.5f * (float)H) / tanfovy;
+  v.near_ = near_plane;
+  v.tile_base = 0;
+  v.pix_base = 0;
+  return v;
+}
+
+int check_modes(int vox_geo_mode, int density_mode, int cam_mode) {
+  if (density_mode != kDensitySDF)
+    return fail(1, "density_mode %s%lld is not supported (only SDF = 1)", "", density_mode);
+  if (vox_geo_mode != kVoxGeoTriinterp3)
+    return fail(1, "vox_geo_mode %s%lld is not supported (only triinterp3 = 3)", "", vox_geo_mode);
+  if (cam_mode != kCamPersp)
+    return fail(1, "cam_mode %s%lld is not supported (only perspective = 0)", "", cam_mode);
+  return 0;
+}
+
+int check_image(int W, int H, int P) {
+  if (W <= 0 || H <= 0 || W > 65535 || H > 65535)
+    return fail(1, "image size must be within 1..65535 (got W=%s%lld)", "", W);
+  if ((long long)ceil_div(W, kTile) * ceil_div(H, kTile) > 65536)
+    return fail(1, "at most 65536 tiles per view%s (got %lld)", "", (long long)ceil_div(W, kTile) * ceil_div(H, kTile));
+  if (P < 0 || P >= (1 << 29)) return fail(1, "voxel count must be below 2^29%s (got %lld)", "", P);
+  return 0;
+}
+
+struct KeyLayout {
+  int shift, qb, rank_bits, nbits;
+};
+
+KeyLayout key_layout(int rank_shift, long long n_tiles) {
+  KeyLayout k;
+  k.shift = rank_shift;
+  k.qb = rank_shift >= 3 ? 3 : 0;
+  k.rank_bits = kRankBits - rank_shift + k.qb;
+  k.nbits = k.rank_bits + tile_bits_for(n_tiles);
+  return k;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* svr_last_error(void) { return g_err.c_str(); }
+
+void svr_timing_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  g_timing_on = on != 0;
+}
+
+int svr_timing_read(int kind, int reset, double* total_ms, long long* launches) {
+  g_err.clear();
+  *total_ms = 0.0;
+  *launches = 0;
+  if (kind < 0 || kind >= kTimedKinds) return fail(1, "timing kind out of range%s (%lld)", "", kind);
+  std::vector<TimedLaunch> evs;
+  {
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    evs = g_timing[kind];
+    if (reset) g_timing[kind].clear();
+  }
+  for (auto& e : evs) {
+    SVR_TRY(hipEventSynchronize(e.b));
+    float ms = 0.f;
+    SVR_TRY(hipEventElapsedTime(&ms, e.a, e.b));
+    *total_ms += ms;
+    *launches += 1;
+    if (reset) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+  }
+  return 0;
+}
+int svr_abi_version(void) { return 1; }
+
+size_t svr_geom_state_bytes(int n_views, int P) { return GeomLayout(n_views, P).total; }
+size_t svr_binning_state_bytes(long long K) { return BinLayout(K).total; }
+size_t svr_image_state_bytes(int n_views, long long n_pixels, long long n_tiles) {
+  return ImageLayout(n_views, n_pixels, n_tiles).total;
+}
+
+int svr_rasterize_preprocess(int W, int H, float tanfovx, float tanfovy, float cx, float cy,
+                             const float* w2c, const float* c2w, int cam_mode, float near_plane,
+                             const int64_t* octree_paths, const float* vox_centers,
+                             const float* vox_lengths, const uint8_t* is_leaf, int P,
+                             int32_t* out_n_dup, void* geom_buffer, void* stream) {
+  g_err.clear();
+  if (cam_mode != kCamPersp) return fail(1, "cam_mode %s%lld is not supported (only perspective = 0)", "", cam_mode);
+  if (int e = check_image(W, H, P)) return e;
+  if (P == 0) return 0;
+  hipStream_t st = as_stream(stream);
+  GeomPtrs gp(geom_buffer, 1, P);
+  const ViewDesc v = make_view(W, H, tanfovx, tanfovy, cx, cy, near_plane);
+  SVR_TRY(launch_init_view(gp.views, v, w2c, c2w, nullptr, st));
+  SVR_TRY(launch_fill_u32(gp.misc, 1, (uint32_t)kRankBits, st));
+  SVR_TRY(launch_preprocess(gp.views, 1, P, vox_centers, vox_lengths, is_leaf, octree_paths, out_n_dup,
+                            gp.n_dup, gp.bbox, gp.quad, gp.misc, st));
+  return 0;
+}
+
+int svr_rasterize_count(void* geom_buffer, int P, long long* K_out, int* rank_shift_out, void* stream) {
+  g_err.clear();
+  *K_out = 0;
+  *rank_shift_out = 0;
+  if (P <= 0) return 0;
+  hipStream_t st = as_stream(stream);
+  GeomPtrs gp(geom_buffer, 1, P);
+  SVR_TRY(launch_scan(gp.n_dup, nullptr, P, gp.partial, gp.scan, true, st));
+  uint32_t host[2] = {0, 0};
+  SVR_TRY(hipMemcpyAsync(&host[0], gp.scan + (P - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  SVR_TRY(hipMemcpyAsync(&host[1], gp.misc, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  SVR_TRY(hipStreamSynchronize(st));
+  *K_out = (long long)host[0];
+  *rank_shift_out = (int)(host[1] > (uint32_t)kRankBits ? kRankBits : host[1]);
+  return 0;
+}
+
+int svr_rasterize_forward(int vox_geo_mode, int density_mode, int W, int H, float tanfovx,
+                          float tanfovy, float cx, float cy, const float* w2c, const float* c2w,
+                          const float* background, int cam_mode, int need_depth,
+                          int need_distortion, int need_normal, const int64_t* octree_paths,
+                          const float* vox_centers, const float* vox_lengths, const float* geos,
+                          const float* rgbs, int P, void* geom_buffer, long long K, int rank_shift,
+                          void* binning_buffer, void* image_buffer, float* out_color,
+                          float* out_depth, float* out_normal, float* out_T, float* max_w,
+                          const float* s_val, void* stream) {
+  g_err.clear();
+  if (int e = check_modes(vox_geo_mode, density_mode, cam_mode)) return e;
+  if (int e = check_image(W, H, P)) return e;
+  if (need_distortion) return fail(1, "distortion output is not supported%s%lld", "", 0);
+  if (rank_shift < 0 || rank_shift > kRankBits || rank_shift % 3) return fail(1, "bad rank_shift %s%lld", "", rank_shift);
+  if (K < 0 || K >= (1LL << 31)) return fail(1, "key count out of range%s (%lld)", "", K);
+  if (P == 0) return 0;
+  hipStream_t st = as_stream(stream);
+  const long long n_tiles = (long long)ceil_div(W, kTile) * ceil_div(H, kTile);
+  GeomPtrs gp(geom_buffer, 1, P);
+  BinPtrs bp(binning_buffer, K);
+  ImagePtrs ip(image_buffer, 1, (long long)W * H, n_tiles);
+  const ViewDesc v = make_view(W, H, tanfovx, tanfovy, cx, cy, 0.f);
+  SVR_TRY(launch_init_view(ip.views, v, w2c, c2w, background, st));
+  const uint32_t* n_dev = gp.scan + (P - 1);
+
+  const KeyLayout kl = key_layout(rank_shift, n_tiles);
+  const int passes = (kl.nbits + 7) / 8;
+  // Duplicate into the buffer pair that makes the last sort pass land in the primary pair.
+  const bool start_alt = (passes % 2) == 1;
+  uint64_t* k0 = start_alt ? bp.keys_alt : bp.keys;
+  uint32_t* v0 = start_alt ? bp.vals_alt : bp.vals;
+  uint64_t* k1 = start_alt ? bp.keys : bp.keys_alt;
+  uint32_t* v1 = start_alt ? bp.vals : bp.vals_alt;
+  if (K > 0) {
+    SVR_TRY(launch_duplicate(ip.views, 1, P, octree_paths, gp.n_dup, gp.scan, gp.bbox, gp.quad, kl.shift, kl.qb,
+                             kl.rank_bits, k0, v0, st));
+    bool in_alt = false;
+    SVR_TRY(launch_radix_sort(k0, v0, k1, v1, n_dev, K, kl.nbits, bp.hist, bp.partial, &in_alt, st));
+  }
+  SVR_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(ip.ranges), 2 * n_tiles, 0u, st));
+  if (K > 0) SVR_TRY(launch_tile_ranges(bp.keys, n_dev, K, kl.rank_bits, ip.ranges, st));
+
+  FwdArgs a;
+  memset(&a, 0, sizeof(a));
+  a.views = ip.views; a.n_views = 1; a.total_tiles = (int)n_tiles;
+  a.ranges = ip.ranges; a.vals = bp.vals;
+  a.bboxes = gp.bbox; a.P = P;
+  a.centers = vox_centers; a.lengths = vox_lengths; a.geos = geos; a.rgbs = rgbs;
+  a.s_val = s_val;
+  a.tile_last = ip.tile_last; a.n_contrib = ip.n_contrib;
+  a.color = out_color; a.depth = out_depth; a.normal = out_normal; a.T = out_T; a.max_w = max_w;
+  a.work_counter = nullptr;
+  a.need_depth = need_depth; a.need_normal = need_normal;
+  {
+    LaunchTimer timer(0, st);
+    SVR_TRY(launch_render_forward(a, st));
+  }
+  return 0;
+}
+
+int svr_rasterize_backward(long long K, int vox_geo_mode, int density_mode, int W, int H,
+                           float tanfovx, float tanfovy, float cx, float cy, const float* w2c,
+                           const float* c2w, const float* background, const float* vox_centers,
+                           const float* vox_lengths, const float* geos, const float* rgbs, int P,
+                           void* geom_buffer, void* binning_buffer, void* image_buffer,
+                           const float* out_T, const float* dL_dcolor, const float* dL_ddepth,
+                           const float* dL_dnormal, const float* dL_dT, float lambda_R_concen,
+                           const float* gt_color, float lambda_ascending, float lambda_dist,
+                           int need_depth, int need_normal, const float* s_val, float* dL_dvox,
+                           float* dL_ds, void* stream) {
+  g_err.clear();
+  if (int e = check_modes(vox_geo_mode, density_mode, kCamPersp)) return e;
+  if (int e = check_image(W, H, P)) return e;
+  if (lambda_ascending != 0.f) return fail(1, "lambda_ascending > 0 is not supported%s%lld", "", 0);
+  if (lambda_dist != 0.f) return fail(1, "lambda_dist > 0 is not supported%s%lld", "", 0);
+  if (need_depth && !dL_ddepth) return fail(1, "need_depth requires dL_ddepth%s%lld", "", 0);
+  if (need_normal && !dL_dnormal) return fail(1, "need_normal requires dL_dnormal%s%lld", "", 0);
+  if (lambda_R_concen > 0.f && !gt_color) return fail(1, "lambda_R_concen > 0 requires gt_color%s%lld", "", 0);
+  if (P == 0 || K <= 0) return 0;
+  hipStream_t st = as_stream(stream);
+  const long long n_tiles = (long long)ceil_div(W, kTile) * ceil_div(H, kTile);
+  GeomPtrs gp(geom_buffer, 1, P);
+  BinPtrs bp(binning_buffer, K);
+  ImagePtrs ip(image_buffer, 1, (long long)W * H, n_tiles);
+  const ViewDesc v = make_view(W, H, tanfovx, tanfovy, cx, cy, 0.f);
+  SVR_TRY(launch_init_view(ip.views, v, w2c, c2w, background, st));
+  BwdArgs a;
+  memset(&a, 0, sizeof(a));
+  a.views = ip.views; a.n_views = 1; a.total_tiles = (int)n_tiles;
+  a.ranges = ip.ranges; a.tile_last = ip.tile_last; a.vals = bp.vals;
+  a.bboxes = gp.bbox; a.P = P;
+  a.centers = vox_centers; a.lengths = vox_lengths; a.geos = geos; a.rgbs = rgbs;
+  a.s_val = s_val;
+  a.T = out_T; a.n_contrib = ip.n_contrib;
+  a.dL_dcolor = dL_dcolor; a.dL_ddepth = dL_ddepth; a.dL_dnormal = dL_dnormal; a.dL_dT = dL_dT;
+  a.gt_color = gt_color; a.lambda_R_concen = lambda_R_concen;
+  a.dL_dvox = dL_dvox; a.dL_ds = dL_ds;
+  a.work_counter = nullptr;
+  a.need_depth = need_depth; a.need_normal = need_normal;
+  {
+    LaunchTimer timer(1, st);
+    SVR_TRY(launch_render_backward(a, st));
+  }
+  return 0;
+}
+
+int svr_sh_compute(int degree, const int64_t* idx, int n_idx, const float* vox_centers, int P,
+                   const float* cam_pos, const float* sh0, const float* shs, int M, float* rgbs,
+                   void* stream) {
+  g_err.clear();
+  if (degree < 0 || degree > 3) return fail(1, "SH degree must be 0..3%s (got %lld)", "", degree);
+  if (M < (degree + 1) * (degree + 1)) return fail(1, "shs has too few coefficients%s (M=%lld)", "", M);
+  const int n = idx ? n_idx : P;
+  SVR_TRY(launch_sh_forward(n, idx, degree, M, vox_centers, cam_pos, sh0, shs, rgbs, as_stream(stream)));
+  return 0;
+}
+
+int svr_sh_compute_bw(int degree, int M, const int64_t* idx, int n_idx, const float* vox_centers,
+                      int P, const float* cam_pos, const float* dL_drgbs, float* dL_dsh0,
+                      float* dL_dshs, void* stream) {
+  g_err.clear();
+  if (degree < 0 || degree > 3) return fail(1, "SH degree must be 0..3%s (got %lld)", "", degree);
+  const int n = idx ? n_idx : P;
+  SVR_TRY(launch_sh_backward(n, idx, degree, M, vox_centers, cam_pos, dL_drgbs, dL_dsh0, dL_dshs,
+                             as_stream(stream)));
+  return 0;
+}
+
+int svr_gather_triinterp_geo_params(const int64_t* vox_key, int P, const int64_t* care_idx,
+                                    int n_care, const float* grid_pts, float* geos, void* stream) {
+  g_err.clear();
+  (void)P;
+  SVR_TRY(launch_gather_geo(n_care, vox_key, care_idx, grid_pts, geos, as_stream(stream)));
+  return 0;
+}
+
+int svr_gather_triinterp_geo_params_bw(const int64_t* vox_key, int P, const int64_t* care_idx,
+                                       int n_care, const float* dL_dgeos, float* dL_dgrid_pts,
+                                       void* stream) {
+  g_err.clear();
+  (void)P;
+  SVR_TRY(launch_gather_geo_bw(n_care, vox_key, care_idx, dL_dgeos, dL_dgrid_pts, as_stream(stream)));
+  return 0;
+}
+
+int svr_unbiased_adam_step(int sparse, float* param, const float* grad, float* exp_avg,
+                           float* exp_avg_sq, long long numel, double step, double lr,
+                           double beta1, double beta2, float eps, void* stream) {
+  g_err.clear();
+  // Bias corrections in double on the host (adam_step.cu:117-122).
+  const double bc1 = 1.0 - pow(beta1, step);
+  const double bc2 = 1.0 - pow(beta2, step);
+  const double step_size = lr / bc1;
+  const double rsqrt_bc2 = 1.0 / sqrt(bc2);
+  SVR_TRY(launch_adam(sparse != 0, false, numel, param, grad, exp_avg, exp_avg_sq, (float)step_size,
+                      (float)beta1, (float)beta2, (float)rsqrt_bc2, eps, as_stream(stream)));
+  return 0;
+}
+
+int svr_biased_adam_step(int sparse, float* param, const float* grad, float* exp_avg,
+                         float* exp_avg_sq, long long numel, float lr, float beta1, float beta2,
+                         float eps, void* stream) {
+  g_err.clear();
+  SVR_TRY(launch_adam(sparse != 0, true, numel, param, grad, exp_avg, exp_avg_sq, lr, beta1, beta2, 1.f, eps,
+                      as_stream(stream)));
+  return 0;
+}
+
+static int grid_loss_common(bool lap, const float* grid_pts, long long n_pts, const int64_t* vox_key, int N,
+                            const float* coord, const float* size, int res, const uint8_t* mask,
+                            const int32_t* keys, const int32_t* key2vox, int G, const int32_t* active, int A,
+                            float weight, float inv, unsigned seed, const float* randv, float* grad,
+                            void* stream) {
+  g_err.clear();
+  if (res <= 2) return fail(1, "grid_res must exceed 2%s (got %lld)", "", res);
+  GridTable T;
+  T.grid_pts = grid_pts; T.vox_key = vox_key; T.keys = keys; T.key2vox = key2vox;
+  T.vox_coord = coord; T.vox_size = size; T.n_keys = G; T.res = res; T.n_vox = N; T.n_pts = n_pts;
+  SVR_TRY(launch_grid_loss(lap, T, reinterpret_cast<const bool*>(mask), active, A, weight, inv, seed, randv, grad,
+                           as_stream(stream)));
+  return 0;
+}
+
+int svr_grid_eikonal_bw(const float* grid_pts, long long n_pts, const int64_t* vox_key, int N,
+                        const float* grid_voxel_coord, const float* grid_voxel_size, int grid_res,
+                        const uint8_t* grid_mask, const int32_t* grid_keys,
+                        const int32_t* grid2voxel, int G, const int32_t* active_list, int A,
+                        float weight, float vox_size_inv, unsigned int seed, const float* randv,
+                        float* grid_pts_grad, void* stream) {
+  return grid_loss_common(false, grid_pts, n_pts, vox_key, N, grid_voxel_coord, grid_voxel_size, grid_res,
+                          grid_mask, grid_keys, grid2voxel, G, active_list, A, weight, vox_size_inv, seed, randv,
+                          grid_pts_grad, stream);
+}
+
+int svr_laplacian_smoothness_bw(const float* grid_pts, long long n_pts, const int64_t* vox_key,
+                                int N, const float* grid_voxel_coord,
+                                const float* grid_voxel_size, int grid_res,
+                                const uint8_t* grid_mask, const int32_t* grid_keys,
+                                const int32_t* grid2voxel, int G, const int32_t* active_list,
+                                int A, float weight, float vox_size_inv, unsigned int seed,
+                                const float* randv, float* grid_pts_grad, void* stream) {
+  return grid_loss_common(true, grid_pts, n_pts, vox_key, N, grid_voxel_coord, grid_voxel_size, grid_res,
+                          grid_mask, grid_keys, grid2voxel, G, active_list, A, weight, vox_size_inv, seed, randv,
+                          grid_pts_grad, stream);
+}
+
+int svr_ijk_2_octpath(const int64_t* ijk, const int8_t* octlevel, int n, int64_t* octpath, void* stream) {
+  g_err.clear();
+  SVR_TRY(launch_ijk_to_octpath(n, ijk, octlevel, octpath, as_stream(stream)));
+  return 0;
+}
+
+int svr_octpath_2_ijk(const int64_t* octpath, const int8_t* octlevel, int n, int64_t* ijk, void* stream) {
+  g_err.clear();
+  SVR_TRY(launch_octpath_to_ijk(n, octpath, octlevel, ijk, as_stream(stream)));
+  return 0;
+}
+
+size_t svr_valid_gradient_table_workspace_bytes(int N) {
+  Carver c;
+  c.take<uint32_t>(N);                                    // per-voxel cell counts
+  c.take<uint32_t>(N);                                    // inclusive scan
+  c.take<uint32_t>((long long)scan_partial_count(N) + 1); // scan scratch
+  return c.off;
+}
+
+static void table_ptrs(void* ws, int N, uint32_t** count, uint32_t** scan, uint32_t** partial) {
+  Carver c;
+  const size_t o0 = c.take<uint32_t>(N);
+  const size_t o1 = c.take<uint32_t>(N);
+  const size_t o2 = c.take<uint32_t>((long long)scan_partial_count(N) + 1);
+  *count = (uint32_t*)((char*)ws + o0);
+  *scan = (uint32_t*)((char*)ws + o1);
+  *partial = (uint32_t*)((char*)ws + o2);
+}
+
+int svr_valid_gradient_table_count(const float* vox_center, const float* vox_size,
+                                   const float* scene_center, float inside_extent,
+                                   int grid_res_pow2, const uint8_t* is_leaf, int N,
+                                   void* workspace, long long* count_out, void* stream) {
+  g_err.clear();
+  *count_out = 0;
+  if (grid_res_pow2 < 0 || grid_res_pow2 > 10) return fail(1, "grid_res_pow2 must be 0..10%s (got %lld)", "", grid_res_pow2);
+  if (N <= 0) return 0;
+  hipStream_t st = as_stream(stream);
+  uint32_t *cnt, *scan, *part;
+  table_ptrs(workspace, N, &cnt, &scan, &part);
+  const int res = 1 << grid_res_pow2;
+  SVR_TRY(launch_table_count(N, vox_center, vox_size, scene_center, inside_extent, res,
+                             reinterpret_cast<const bool*>(is_leaf), cnt, st));
+  SVR_TRY(launch_scan(cnt, nullptr, N, part, scan, true, st));
+  uint32_t total = 0;
+  SVR_TRY(hipMemcpyAsync(&total, scan + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  SVR_TRY(hipStreamSynchronize(st));
+  *count_out = total;
+  return 0;
+}
+
+int svr_valid_gradient_table_fill(const float* vox_center, const float* vox_size,
+                                  const float* scene_center, float inside_extent,
+                                  int grid_res_pow2, const uint8_t* is_leaf, int N,
+                                  void* workspace, uint8_t* grid_mask, int32_t* grid_keys,
+                                  int32_t* grid2voxel, void* stream) {
+  g_err.clear();
+  if (N <= 0) return 0;
+  uint32_t *cnt, *scan, *part;
+  table_ptrs(workspace, N, &cnt, &scan, &part);
+  SVR_TRY(launch_table_write(N, vox_center, vox_size, scene_center, inside_extent, 1 << grid_res_pow2,
+                             reinterpret_cast<const bool*>(is_leaf), scan, reinterpret_cast<bool*>(grid_mask),
+                             grid_keys, grid2voxel, as_stream(stream)));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,224 @@
+// Shared device helpers for the gfx950 sparse-voxel SDF rasterizer.
+//
+// Conventions restated from the reference (cuda/src/config.h:14-39, cuda/src/auxiliary.h):
+//   * 16x16 pixel tiles; a tile is one 256-thread workgroup = four wave64s, each wave an
+//     8x8 pixel quadrant of the tile.
+//   * octpath: 48 rank bits (16 levels x 3 bits), x = bit2, y = bit1, z = bit0.
+//   * sorted value = octant << 29 | voxel id.
+//   * corner index of the 8 corner SDFs = x<<2 | y<<1 | z.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace svr {
+
+constexpr int kTile = 16;
+constexpr int kTilePixels = kTile * kTile;
+constexpr int kMaxLevels = 16;
+constexpr int kRankBits = 3 * kMaxLevels;
+constexpr float kMinAlpha = 1e-5f;
+constexpr float kEarlyStopT = 1e-4f;
+constexpr float kPhiEps = 1e-6f;
+constexpr float kThird = 0.3333333f;       // depth sub-step factor used by the reference
+constexpr uint32_t kVoxMask = (1u << 29) - 1u;
+
+constexpr int kVoxGeoTriinterp = 2, kVoxGeoTriinterp1 = 1, kVoxGeoTriinterp3 = 3;
+constexpr int kDensityExpLinear11 = 0, kDensitySDF = 1;
+constexpr int kCamPersp = 0, kCamOrtho = 10;
+
+// One camera view as the kernels see it.  Built on the device from the caller's matrices so
+// that no host synchronisation is needed, and stored in the per-view state buffers.
+struct alignas(16) ViewDesc {
+  int W, H, tiles_x, tiles_y;
+  float tanfovx, tanfovy, cx, cy;
+  float focal_x, focal_y, near_, pad0;
+  float w2c[12];
+  float c2w[12];
+  float bg[3];
+  int tile_base;          // first global tile index of this view (batched launches)
+  long long pix_base;     // first pixel of this view in concatenated per-pixel buffers
+  long long pad1;
+};
+
+__host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// Octant of a direction / of a corner relative to the camera (auxiliary.h:76-84).
+__device__ inline uint32_t ray_octant(float x, float y, float z) {
+  return (uint32_t(x < 0.f) << 2) | (uint32_t(y < 0.f) << 1) | uint32_t(z < 0.f);
+}
+
+// Pixel-centre ray in camera space (auxiliary.h:87-95).
+__device__ inline void camera_ray(const ViewDesc& v, int px, int py, float& x, float& y) {
+#pragma clang fp contract(off)
+  x = ((float)px + 0.5f - v.cx) * 2.f * v.tanfovx / (float)v.W;
+  y = ((float)py + 0.5f - v.cy) * 2.f * v.tanfovy / (float)v.H;
+}
+
+struct Ray {
+  float ox, oy, oz;
+  float dx, dy, dz;
+  float ix, iy, iz;     // 1/d
+  float norm_inv;       // 1/|cam_rd|
+  uint32_t octant;
+};
+
+// World ray of pixel (px, py): rotate the camera ray by c2w and divide by its camera-space
+// length (forward.cu:124-133).
+__device__ inline Ray pixel_ray(const ViewDesc& v, int px, int py) {
+#pragma clang fp contract(off)
+  float cx_, cy_;
+  camera_ray(v, px, py, cx_, cy_);
+  const float n = sqrtf(cx_ * cx_ + cy_ * cy_ + 1.f);
+  Ray r;
+  r.norm_inv = 1.f / n;
+  const float* m = v.c2w;
+  r.dx = (m[0] * cx_ + m[1] * cy_ + m[2]) * r.norm_inv;
+  r.dy = (m[4] * cx_ + m[5] * cy_ + m[6]) * r.norm_inv;
+  r.dz = (m[8] * cx_ + m[9] * cy_ + m[10]) * r.norm_inv;
+  r.ox = m[3]; r.oy = m[7]; r.oz = m[11];
+  r.ix = 1.f / r.dx; r.iy = 1.f / r.dy; r.iz = 1.f / r.dz;
+  r.octant = ray_octant(r.dx, r.dy, r.dz);
+  return r;
+}
+
+// Slab test of a ray against a cube of centre c and half size h (auxiliary.h:251-264).
+__device__ inline void ray_box(const Ray& r, float cx, float cy, float cz, float h,
+                               float& a, float& b) {
+  const float dx = cx - r.ox, dy = cy - r.oy, dz = cz - r.oz;
+  const float x0 = (dx - h) * r.ix, x1 = (dx + h) * r.ix;
+  const float y0 = (dy - h) * r.iy, y1 = (dy + h) * r.iy;
+  const float z0 = (dz - h) * r.iz, z1 = (dz + h) * r.iz;
+  a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
+  b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+}
+
+__device__ inline bool pixel_in_bbox(int px, int py, uint32_t b0, uint32_t b1) {
+  return px >= int(b0 >> 16) && py >= int(b0 & 0xFFFFu) &&
+         px <= int(b1 >> 16) && py <= int(b1 & 0xFFFFu);
+}
+
+// Trilinear interpolation of 8 corner values at local coordinate q in [0,1]^3, as nested
+// lerps (7 lerps).  Forward and backward both call this so alpha is bit-identical in the two
+// passes, which keeps the T / (1 - alpha) recovery exact.
+__device__ inline float trilerp(const float g[8], float x, float y, float z) {
+#pragma clang fp contract(off)
+  const float c00 = fmaf(z, g[1] - g[0], g[0]);
+  const float c01 = fmaf(z, g[3] - g[2], g[2]);
+  const float c10 = fmaf(z, g[5] - g[4], g[4]);
+  const float c11 = fmaf(z, g[7] - g[6], g[6]);
+  const float c0 = fmaf(y, c01 - c00, c00);
+  const float c1 = fmaf(y, c11 - c10, c10);
+  return fmaf(x, c1 - c0, c0);
+}
+
+__device__ inline void trilinear_weights(float x, float y, float z, float w[8]) {
+  const float x0 = 1.f - x, y0 = 1.f - y, z0 = 1.f - z;
+  const float a00 = x0 * y0, a01 = x0 * y, a10 = x * y0, a11 = x * y;
+  w[0] = a00 * z0; w[1] = a00 * z; w[2] = a01 * z0; w[3] = a01 * z;
+  w[4] = a10 * z0; w[5] = a10 * z; w[6] = a11 * z0; w[7] = a11 * z;
+}
+
+__device__ inline float fast_sigmoid(float x) {
+#pragma clang fp contract(off)
+  return __builtin_amdgcn_rcpf(1.f + __expf(-x));
+}
+
+// The SDF segment of one (ray, voxel) pair: SDF at the entry point and three equal steps to
+// the exit point, their logistic CDFs, and the NeuS alpha (forward.cu:255-306).
+struct Segment {
+  float qx, qy, qz;        // entry point in voxel-local [0,1]^3
+  float sx, sy, sz;        // local step between samples
+  float sdf[4];
+  float phi[4];
+  float alpha;
+};
+
+__device__ inline void eval_segment(const Ray& r, float a, float b, float minx, float miny,
+                                    float minz, float l_inv, const float g[8], float s,
+                                    Segment& seg) {
+#pragma clang fp contract(off)
+  seg.qx = (fmaf(a, r.dx, r.ox) - minx) * l_inv;
+  seg.qy = (fmaf(a, r.dy, r.oy) - miny) * l_inv;
+  seg.qz = (fmaf(a, r.dz, r.oz) - minz) * l_inv;
+  const float st = ((b - a) * (1.f / 3.f)) * l_inv;
+  seg.sx = st * r.dx; seg.sy = st * r.dy; seg.sz = st * r.dz;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float fk = (float)k;
+    seg.sdf[k] = trilerp(g, fmaf(seg.sx, fk, seg.qx), fmaf(seg.sy, fk, seg.qy), fmaf(seg.sz, fk, seg.qz));
+    seg.phi[k] = fast_sigmoid(s * seg.sdf[k]);
+  }
+  const float raw = (seg.phi[0] - seg.phi[3]) * __builtin_amdgcn_rcpf(seg.phi[0] + kPhiEps);
+  seg.alpha = fmaxf(raw, 0.f);
+}
+
+// Depth of one segment from its three sub-interval alphas (forward.cu:349-357).
+__device__ inline void sub_alphas(const Segment& seg, float sub[3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float raw = (seg.phi[k] - seg.phi[k + 1]) * __builtin_amdgcn_rcpf(seg.phi[k] + kPhiEps);
+    sub[k] = fmaxf(raw, 0.f);
+  }
+}
+
+__device__ inline float depth_value(const float sub[3], float a, float b) {
+  const float step = kThird * (b - a);
+  const float t0 = a + 0.5f * step, t1 = a + 1.5f * step, t2 = a + 2.5f * step;
+  return sub[0] * t0 + (1.f - sub[0]) * sub[1] * t1 + (1.f - sub[0]) * (1.f - sub[1]) * sub[2] * t2;
+}
+
+// Unnormalised "linear" normal from corner sums and its reciprocal norm (forward.cu:422-433).
+__device__ inline void lin_normal(const float g[8], float& nx, float& ny, float& nz, float& rn) {
+  nx = (g[4] + g[5] + g[6] + g[7]) - (g[0] + g[1] + g[2] + g[3]);
+  ny = (g[2] + g[3] + g[6] + g[7]) - (g[0] + g[1] + g[4] + g[5]);
+  nz = (g[1] + g[3] + g[5] + g[7]) - (g[0] + g[2] + g[4] + g[6]);
+  rn = rsqrtf(nx * nx + ny * ny + nz * nz + 1e-15f);
+}
+
+// Tile rectangle of a packed pixel bbox, clamped to the view (auxiliary.h:97-111).
+__device__ inline void bbox_tiles(uint32_t b0, uint32_t b1, int tiles_x, int tiles_y,
+                                  int& tx0, int& ty0, int& tx1, int& ty1) {
+  tx0 = max(0, min(tiles_x - 1, int(b0 >> 16) / kTile));
+  ty0 = max(0, min(tiles_y - 1, int(b0 & 0xFFFFu) / kTile));
+  tx1 = max(0, min(tiles_x - 1, int(b1 >> 16) / kTile));
+  ty1 = max(0, min(tiles_y - 1, int(b1 & 0xFFFFu) / kTile));
+}
+
+__device__ inline uint64_t rep3(uint32_t q) {
+  uint64_t v = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxLevels; ++i) v = (v << 3) | q;
+  return v;
+}
+
+// Wave64 helpers.
+//
+// Sum over the 64 lanes (all lanes must be active), returned in every lane: DPP quad swaps and
+// row mirrors give each 16-lane row its sum, then the four row sums are read back as scalars.
+__device__ inline float wave_sum_uniform(float x) {
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x141, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x140, 0xF, 0xF, false));
+  const int v = __float_as_int(x);
+  const float r0 = __int_as_float(__builtin_amdgcn_readlane(v, 0));
+  const float r1 = __int_as_float(__builtin_amdgcn_readlane(v, 16));
+  const float r2 = __int_as_float(__builtin_amdgcn_readlane(v, 32));
+  const float r3 = __int_as_float(__builtin_amdgcn_readlane(v, 48));
+  return (r0 + r1) + (r2 + r3);
+}
+
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
+}  // namespace svr

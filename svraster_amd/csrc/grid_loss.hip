@@ -1,0 +1,247 @@
+// Grid smoothness backward on gfx950: eikonal and Laplacian losses sampled on the dense
+// valid-gradient grid table, plus the table build itself.
+//
+// References:
+//   grid eikonal   cuda/src/ge_compute.cu:14-117 (host 120-161)
+//   grid Laplacian cuda/src/ls_compute.cu:14-109 (host 112-155)
+//   table helpers  cuda/src/auxiliary.h:360-455 (neighbour validity, key search, sampling,
+//                  corner scatter)
+//   table build    cuda/src/utils.cu:190-252, 362-408
+//
+// Each active grid key samples the SDF at the cell (Laplacian) and its six face neighbours,
+// all offset by one random vector per key, by locating the covering voxel with a binary search
+// over the sorted key table and interpolating its 8 corners; the loss gradient is scattered to
+// those corners with atomics.  The reference seeds the offset from the clock; here it comes
+// from a seeded PCG hash of (seed, key index) or from an explicit per-key array, so results are
+// reproducible.
+#include "common.h"
+#include "ops.h"
+
+namespace svr {
+
+__device__ inline uint32_t pcg_hash(uint32_t x) {
+  const uint32_t state = x * 747796405u + 2891336453u;
+  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+
+__device__ inline float hash_unit(uint32_t x) { return (float)(pcg_hash(x) >> 8) * (1.0f / 16777216.0f); }
+
+__device__ inline void random_offset(const float* randv, uint32_t seed, int tid, float& rx, float& ry, float& rz) {
+  if (randv) {
+    rx = randv[3 * tid]; ry = randv[3 * tid + 1]; rz = randv[3 * tid + 2];
+  } else {
+    const uint32_t b = pcg_hash(seed) ^ (uint32_t)tid * 3u;
+    rx = hash_unit(b); ry = hash_unit(b + 0x9E3779B9u); rz = hash_unit(b + 0x3C6EF372u);
+  }
+}
+
+__device__ inline bool neighbours_valid(int x, int y, int z, int res, const bool* __restrict__ mask) {
+  if (x <= 0 || x >= res - 1 || y <= 0 || y >= res - 1 || z <= 0 || z >= res - 1) return false;
+  const long long rr = (long long)res * res;
+  const long long k = x + (long long)y * res + z * rr;
+  return mask[k + 1] && mask[k - 1] && mask[k + res] && mask[k - res] && mask[k + rr] && mask[k - rr];
+}
+
+// Same midpoint rule as the reference's search, so duplicate keys resolve to the same entry.
+__device__ inline int key_search(const int32_t* __restrict__ keys, int size, int key) {
+  int l = 0, r = size - 1;
+  while (l <= r) {
+    const int m = (l + r) / 2;
+    const int km = keys[m];
+    if (km == key) return m;
+    if (km < key) l = m + 1; else r = m - 1;
+  }
+  return -1;
+}
+
+
+
+// SDF at grid point (gx+rx, gy+ry, gz+rz); returns voxel id (-1 when not covered) and weights.
+__device__ inline float grid_sample(const GridTable& G, int gx, int gy, int gz, float rx, float ry, float rz,
+                                    float w[8], int& vox) {
+  const int key = gx + gy * G.res + gz * G.res * G.res;
+  const int idx = key_search(G.keys, G.n_keys, key);
+  vox = -1;
+  if (idx < 0) return 0.f;
+  const int v = G.key2vox[idx];
+  if (v < 0 || v >= G.n_vox) return 0.f;
+  vox = v;
+  const float size = G.vox_size[v];
+  const float qx = ((float)gx + rx - G.vox_coord[3 * v]) / size;
+  const float qy = ((float)gy + ry - G.vox_coord[3 * v + 1]) / size;
+  const float qz = ((float)gz + rz - G.vox_coord[3 * v + 2]) / size;
+  trilinear_weights(qx, qy, qz, w);
+  float sdf = 0.f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    w[i] = fminf(fmaxf(w[i], 0.f), 1.f);
+    sdf += G.grid_pts[G.vox_key[(long long)v * 8 + i]] * w[i];
+  }
+  return sdf;
+}
+
+__device__ inline void grid_scatter(const GridTable& G, int vox, const float w[8], float g, float* __restrict__ out) {
+  if (vox < 0) return;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) atomicAdd(out + G.vox_key[(long long)vox * 8 + i], g * w[i]);
+}
+
+__global__ __launch_bounds__(256) void grid_eikonal_kernel(GridTable G, const bool* __restrict__ mask,
+                                                           const int32_t* __restrict__ active, int A,
+                                                           float weight, float inv, uint32_t seed,
+                                                           const float* __restrict__ randv,
+                                                           float* __restrict__ grad) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= A) return;
+  const int gk = G.keys[active[tid]];
+  const int x = gk % G.res, y = (gk / G.res) % G.res, z = gk / (G.res * G.res);
+  if (!mask[gk] || !neighbours_valid(x, y, z, G.res, mask)) return;
+  float rx, ry, rz;
+  random_offset(randv, seed, tid, rx, ry, rz);
+  const int dx[6] = {1, -1, 0, 0, 0, 0}, dy[6] = {0, 0, 1, -1, 0, 0}, dz[6] = {0, 0, 0, 0, 1, -1};
+  float w[6][8], sd[6];
+  int vox[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) sd[i] = grid_sample(G, x + dx[i], y + dy[i], z + dz[i], rx, ry, rz, w[i], vox[i]);
+  const float gx = (sd[0] - sd[1]) * 0.5f, gy = (sd[2] - sd[3]) * 0.5f, gz = (sd[4] - sd[5]) * 0.5f;
+  const float gn = sqrtf(gx * gx + gy * gy + gz * gz + 1e-6f);
+  const float gw = gn * 0.5f * inv;
+  const float dLdg = 2.f * (gw - 1.f) * weight;
+  const float lx = dLdg * gx / gn * 0.5f * inv;
+  const float ly = dLdg * gy / gn * 0.5f * inv;
+  const float lz = dLdg * gz / gn * 0.5f * inv;
+  grid_scatter(G, vox[0], w[0], lx, grad);
+  grid_scatter(G, vox[1], w[1], -lx, grad);
+  grid_scatter(G, vox[2], w[2], ly, grad);
+  grid_scatter(G, vox[3], w[3], -ly, grad);
+  grid_scatter(G, vox[4], w[4], lz, grad);
+  grid_scatter(G, vox[5], w[5], -lz, grad);
+}
+
+__global__ __launch_bounds__(256) void grid_laplacian_kernel(GridTable G, const bool* __restrict__ mask,
+                                                             const int32_t* __restrict__ active, int A,
+                                                             float weight, float inv, uint32_t seed,
+                                                             const float* __restrict__ randv,
+                                                             float* __restrict__ grad) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= A) return;
+  const int gk = G.keys[active[tid]];
+  const int x = gk % G.res, y = (gk / G.res) % G.res, z = gk / (G.res * G.res);
+  if (!mask[gk] || !neighbours_valid(x, y, z, G.res, mask)) return;
+  float rx, ry, rz;
+  random_offset(randv, seed, tid, rx, ry, rz);
+  const int dx[7] = {0, 1, -1, 0, 0, 0, 0}, dy[7] = {0, 0, 0, 1, -1, 0, 0}, dz[7] = {0, 0, 0, 0, 0, 1, -1};
+  float w[7][8], sd[7];
+  int vox[7];
+#pragma unroll
+  for (int i = 0; i < 7; ++i) sd[i] = grid_sample(G, x + dx[i], y + dy[i], z + dz[i], rx, ry, rz, w[i], vox[i]);
+  const float r0 = sd[1] + sd[2] - 2.f * sd[0];
+  const float r1 = sd[3] + sd[4] - 2.f * sd[0];
+  const float r2 = sd[5] + sd[6] - 2.f * sd[0];
+  const float lr = sqrtf(r0 * r0 + r1 * r1 + r2 * r2 + 1e-10f);
+  const float sz = weight * 0.5f * inv * inv;
+  const float lx = r0 / lr * sz, ly = r1 / lr * sz, lz = r2 / lr * sz;
+  const float lc = -2.f * (lx + ly + lz);
+  grid_scatter(G, vox[0], w[0], lc, grad);
+  grid_scatter(G, vox[1], w[1], lx, grad);
+  grid_scatter(G, vox[2], w[2], lx, grad);
+  grid_scatter(G, vox[3], w[3], ly, grad);
+  grid_scatter(G, vox[4], w[4], ly, grad);
+  grid_scatter(G, vox[5], w[5], lz, grad);
+  grid_scatter(G, vox[6], w[6], lz, grad);
+}
+
+// Valid-gradient table: voxels (leaves with >= 0.9 cells, non-leaves with 0.9..1.5 cells) whose
+// centre lies inside the region are rasterised into the res^3 grid.  Two passes (count, then
+// write at the scanned offset) make the key order deterministic: voxel order, then x, y, z.
+__device__ inline bool table_cell_range(int i, const float* __restrict__ center, const float* __restrict__ size,
+                                        const float* __restrict__ scene_center, float extent, int res,
+                                        const bool* __restrict__ is_leaf, int lo[3], int hi[3]) {
+#pragma clang fp contract(off)
+  const float cells = (size[i] / extent) * (float)res;
+  const bool leaf = is_leaf[i];
+  if (!leaf && (cells > 1.5f || cells < 0.9f)) return false;
+  if (leaf && cells < 0.9f) return false;
+  for (int d = 0; d < 3; ++d) {
+    const float mn = scene_center[d] - 0.5f * extent, mx = scene_center[d] + 0.5f * extent;
+    const float c = center[3 * i + d];
+    if (c < mn || c > mx) return false;
+  }
+  for (int d = 0; d < 3; ++d) {
+    const float mn = scene_center[d] - 0.5f * extent, mx = scene_center[d] + 0.5f * extent;
+    const float c = center[3 * i + d];
+    const float a = c - 0.5f * size[i], b = c + 0.5f * size[i];
+    lo[d] = (int)roundf((a - mn) / (mx - mn) * (float)res);
+    hi[d] = (int)roundf((b - mn) / (mx - mn) * (float)res);
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(256) void table_count_kernel(int N, const float* __restrict__ center,
+                                                          const float* __restrict__ size,
+                                                          const float* __restrict__ scene_center, float extent,
+                                                          int res, const bool* __restrict__ is_leaf,
+                                                          uint32_t* __restrict__ count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int lo[3], hi[3];
+  uint32_t c = 0;
+  if (table_cell_range(i, center, size, scene_center, extent, res, is_leaf, lo, hi)) {
+    const int nx = max(hi[0] - lo[0], 0), ny = max(hi[1] - lo[1], 0), nz = max(hi[2] - lo[2], 0);
+    c = (uint32_t)(nx * ny * nz);
+  }
+  count[i] = c;
+}
+
+__global__ __launch_bounds__(256) void table_write_kernel(int N, const float* __restrict__ center,
+                                                          const float* __restrict__ size,
+                                                          const float* __restrict__ scene_center, float extent,
+                                                          int res, const bool* __restrict__ is_leaf,
+                                                          const uint32_t* __restrict__ scan, bool* __restrict__ mask,
+                                                          int32_t* __restrict__ keys, int32_t* __restrict__ key2vox) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int lo[3], hi[3];
+  if (!table_cell_range(i, center, size, scene_center, extent, res, is_leaf, lo, hi)) return;
+  uint32_t off = i ? scan[i - 1] : 0u;
+  for (int x = lo[0]; x < hi[0]; ++x)
+    for (int y = lo[1]; y < hi[1]; ++y)
+      for (int z = lo[2]; z < hi[2]; ++z) {
+        const int k = x + res * (y + res * z);
+        mask[k] = true;
+        keys[off] = k;
+        key2vox[off] = i;
+        ++off;
+      }
+}
+
+// ------------------------------------------------------------------------------------------
+hipError_t launch_grid_loss(bool laplacian, const GridTable& G, const bool* mask, const int32_t* active, int A,
+                            float weight, float inv, uint32_t seed, const float* randv, float* grad,
+                            hipStream_t st) {
+  if (A <= 0) return hipSuccess;
+  const unsigned nb = (unsigned)((A + 255) / 256);
+  if (laplacian) grid_laplacian_kernel<<<nb, 256, 0, st>>>(G, mask, active, A, weight, inv, seed, randv, grad);
+  else grid_eikonal_kernel<<<nb, 256, 0, st>>>(G, mask, active, A, weight, inv, seed, randv, grad);
+  return hipGetLastError();
+}
+
+hipError_t launch_table_count(int N, const float* center, const float* size, const float* scene_center,
+                              float extent, int res, const bool* is_leaf, uint32_t* count, hipStream_t st) {
+  if (N <= 0) return hipSuccess;
+  table_count_kernel<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(N, center, size, scene_center, extent, res,
+                                                                   is_leaf, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_table_write(int N, const float* center, const float* size, const float* scene_center,
+                              float extent, int res, const bool* is_leaf, const uint32_t* scan, bool* mask,
+                              int32_t* keys, int32_t* key2vox, hipStream_t st) {
+  if (N <= 0) return hipSuccess;
+  table_write_kernel<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(N, center, size, scene_center, extent, res,
+                                                                   is_leaf, scan, mask, keys, key2vox);
+  return hipGetLastError();
+}
+
+}  // namespace svr

@@ -1,0 +1,410 @@
+// Integer stage of the rasterizer on gfx950: voxel preprocess, prefix scan, (tile, rank) key
+// duplication, in-house stable LSD radix sort and tile-range extraction.
+//
+// Reference semantics:
+//   preprocess        cuda/src/preprocess.cu:35-154 (+ focal length at 195-198)
+//   scan + count      cuda/src/forward.cu:881-897
+//   duplicateWithKeys cuda/src/forward.cu:749-802, key codec cuda/src/auxiliary.h:40-74
+//   radix sort        cuda/src/forward.cu:917-926 (CUB SortPairs over the key bits)
+//   tile ranges       cuda/src/forward.cu:807-829, 928
+//
+// Every kernel here takes a view table and a (view, voxel) index space so that one launch can
+// serve one view (the drop-in path) or many views (the batched path).  Counts that only the
+// device knows (the number of duplicated keys) are read from device memory, so the batched
+// path can run without host synchronisation.
+#include "common.h"
+#include "state.h"
+
+namespace svr {
+
+// ------------------------------------------------------------------------------------------
+// View table construction (no host synchronisation: matrices are device pointers).
+// ------------------------------------------------------------------------------------------
+__global__ void init_view_kernel(ViewDesc* out, ViewDesc base, const float* w2c, const float* c2w,
+                                 const float* bg) {
+  if (threadIdx.x != 0) return;
+  ViewDesc v = base;
+  for (int i = 0; i < 12; ++i) { v.w2c[i] = w2c[i]; v.c2w[i] = c2w[i]; }
+  if (bg) { v.bg[0] = bg[0]; v.bg[1] = bg[1]; v.bg[2] = bg[2]; }
+  *out = v;
+}
+
+// ------------------------------------------------------------------------------------------
+// Preprocess.  The floating-point steps are evaluated without contraction so that the packed
+// bounding boxes are bit-identical to an IEEE FP32 restatement with the same operation order.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void preprocess_kernel(
+    const ViewDesc* __restrict__ views, int P, const float* __restrict__ centers,
+    const float* __restrict__ lengths, const uint8_t* __restrict__ is_leaf,
+    const int64_t* __restrict__ octpaths, int32_t* __restrict__ out_n_dup,
+    uint32_t* __restrict__ n_dup, uint2* __restrict__ bboxes, uint8_t* __restrict__ quads,
+    uint32_t* __restrict__ tz_min) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int view = blockIdx.y;
+  if (i >= P) return;
+  const ViewDesc& v = views[view];
+  const long long o = (long long)view * P + i;
+  uint32_t nd = 0;
+  uint2 bb = make_uint2(0u, 0u);
+  uint32_t qbits = 0;
+  if (is_leaf[i] != 0) {
+    const float vcx = centers[3 * i + 0], vcy = centers[3 * i + 1], vcz = centers[3 * i + 2];
+    const float r = 0.5f * lengths[i];
+    const float rox = v.c2w[3], roy = v.c2w[7], roz = v.c2w[11];
+    const float rx = vcx - rox, ry = vcy - roy, rz = vcz - roz;
+    const float d2 = rx * rx + ry * ry + rz * rz;
+    if (!(d2 < v.near_ * v.near_)) {
+      float mnx = 1e9f, mny = 1e9f, mxx = -1e9f, mxy = -1e9f;
+      const float* m = v.w2c;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const float wx = vcx + r * (float)(((c & 4) >> 2) * 2 - 1);
+        const float wy = vcy + r * (float)(((c & 2) >> 1) * 2 - 1);
+        const float wz = vcz + r * (float)((c & 1) * 2 - 1);
+        const float cx = m[0] * wx + m[1] * wy + m[2] * wz + m[3];
+        const float cy = m[4] * wx + m[5] * wy + m[6] * wz + m[7];
+        const float cz = m[8] * wx + m[9] * wy + m[10] * wz + m[11];
+        if (cz < v.near_) continue;
+        const float inv_z = 1.0f / cz;
+        const float ux = cx * inv_z, uy = cy * inv_z;
+        mnx = fminf(mnx, ux); mny = fminf(mny, uy);
+        mxx = fmaxf(mxx, ux); mxy = fmaxf(mxy, uy);
+        qbits |= 1u << ((uint32_t(wx < rox) << 2) | (uint32_t(wy < roy) << 1) | uint32_t(wz < roz));
+      }
+      const float cxh = v.cx - 0.5f, cyh = v.cy - 0.5f;
+      const float bx0 = fmaxf(v.focal_x * mnx + cxh, 0.0f);
+      const float by0 = fmaxf(v.focal_y * mny + cyh, 0.0f);
+      const float bx1 = fminf(v.focal_x * mxx + cxh, (float)v.W);
+      const float by1 = fminf(v.focal_y * mxy + cyh, (float)v.H);
+      if (!(bx0 > bx1 || by0 > by1)) {
+        bb.x = ((uint32_t)__builtin_rintf(bx0) << 16) | (uint32_t)__builtin_rintf(by0);
+        bb.y = ((uint32_t)__builtin_rintf(bx1) << 16) | (uint32_t)__builtin_rintf(by1);
+        int tx0, ty0, tx1, ty1;
+        bbox_tiles(bb.x, bb.y, v.tiles_x, v.tiles_y, tx0, ty0, tx1, ty1);
+        nd = (uint32_t)((1 + ty1 - ty0) * (1 + tx1 - tx0)) * (uint32_t)__popc(qbits);
+      }
+    }
+  }
+  if (nd == 0) { bb = make_uint2(0u, 0u); qbits = 0; }
+  if (out_n_dup) out_n_dup[o] = (int32_t)nd;
+  n_dup[o] = nd;
+  bboxes[o] = bb;
+  quads[o] = (uint8_t)qbits;
+  if (nd && tz_min) {
+    // Number of low rank bits (a multiple of 3) that are zero in this voxel's octpath.
+    const uint64_t p = (uint64_t)octpaths[i] & ((1ull << kRankBits) - 1ull);
+    uint32_t tz = p ? (uint32_t)__builtin_ctzll(p) : (uint32_t)kRankBits;
+    tz = min(tz - tz % 3u, (uint32_t)kRankBits);
+    atomicMin(tz_min, tz);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Reduce-then-scan over u32 (tile = 256 threads x 16 items).
+// ------------------------------------------------------------------------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t& total, uint32_t* lds_w) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = (uint32_t)__shfl_up((int)x, o, 64);
+    if (lane >= o) x += t;
+  }
+  if (lane == 63) lds_w[w] = x;
+  __syncthreads();
+  uint32_t wbase = 0, tot = 0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+    const uint32_t c = lds_w[k];
+    if (k < w) wbase += c;
+    tot += c;
+  }
+  __syncthreads();
+  total = tot;
+  return wbase + x - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(
+    const uint32_t* __restrict__ in, const uint32_t* __restrict__ n_dev, long long n_cap,
+    uint32_t* __restrict__ partial) {
+  __shared__ uint32_t lw[4];
+  const long long n = n_dev ? min((long long)*n_dev, n_cap) : n_cap;
+  const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const long long i = base + k;
+    if (i < n) s += in[i];
+  }
+  uint32_t tot;
+  block_exclusive_scan(s, tot, lw);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// Single workgroup: exclusive scan of the per-tile partial sums in place.
+__global__ __launch_bounds__(1024) void scan_partials_kernel(uint32_t* __restrict__ partial, int nb) {
+  __shared__ uint32_t lw[16];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 1024) {
+    const int i = base + threadIdx.x;
+    const uint32_t v = i < nb ? partial[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan(v, tot, lw);
+    const uint32_t c = carry;
+    if (i < nb) partial[i] = c + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry = c + tot;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(
+    const uint32_t* __restrict__ in, const uint32_t* __restrict__ n_dev, long long n_cap,
+    const uint32_t* __restrict__ partial, uint32_t* __restrict__ out, int inclusive) {
+  __shared__ uint32_t lw[4];
+  const long long n = n_dev ? min((long long)*n_dev, n_cap) : n_cap;
+  const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const long long i = base + k;
+    v[k] = i < n ? in[i] : 0u;
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = block_exclusive_scan(s, tot, lw) + partial[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const long long i = base + k;
+    const uint32_t nx = run + v[k];
+    if (i < n) out[i] = inclusive ? nx : run;
+    run = nx;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Duplication: one (compressed key, value) pair per (octant, tile) of each visible voxel, in
+// the reference's order (octants ascending, then tile rows, then tile columns).
+//
+// Compressed key = global tile << rank_bits | (rank >> shift) << qb | (qb ? octant : 0), where
+// rank = octpath ^ repeat3(octant), `shift` is a multiple of 3 below which every visible
+// voxel's octpath is zero, and qb = 3 when shift >= 3.  Below `shift` the rank bits are the
+// octant pattern, which orders like the octant itself, so sorting the compressed key gives the
+// same order as sorting the reference's (tile << 48 | rank) key.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void duplicate_kernel(
+    const ViewDesc* __restrict__ views, int P, const int64_t* __restrict__ octpaths,
+    const uint32_t* __restrict__ n_dup, const uint32_t* __restrict__ scan,
+    const uint2* __restrict__ bboxes, const uint8_t* __restrict__ quads, int shift, int qb,
+    int rank_bits, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int view = blockIdx.y;
+  if (i >= P) return;
+  const long long o = (long long)view * P + i;
+  const uint32_t nd = n_dup[o];
+  if (nd == 0) return;
+  const ViewDesc& v = views[view];
+  uint32_t off = scan[o] - nd;
+  const uint2 bb = bboxes[o];
+  int tx0, ty0, tx1, ty1;
+  bbox_tiles(bb.x, bb.y, v.tiles_x, v.tiles_y, tx0, ty0, tx1, ty1);
+  const uint64_t path = (uint64_t)octpaths[i];
+  const uint32_t qbits = quads[o];
+  for (uint32_t q = 0; q < 8; ++q) {
+    if (!(qbits & (1u << q))) continue;
+    const uint64_t rank = ((path ^ rep3(q)) & ((1ull << kRankBits) - 1ull)) >> shift;
+    const uint64_t rk = (rank << qb) | (qb ? (uint64_t)q : 0ull);
+    const uint32_t val = (q << 29) | (uint32_t)i;
+    for (int y = ty0; y <= ty1; ++y)
+      for (int x = tx0; x <= tx1; ++x) {
+        const uint64_t tile = (uint64_t)(v.tile_base + y * v.tiles_x + x);
+        keys[off] = (tile << rank_bits) | rk;
+        vals[off] = val;
+        ++off;
+      }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Stable LSD radix sort, 8-bit digits.  Per pass: per-tile digit histograms, an exclusive scan
+// of the digit-major histogram table, then a stable scatter in which each wave64 ranks its 64
+// items with eight ballots (items with equal digits get consecutive slots in lane order).
+// ------------------------------------------------------------------------------------------
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+
+__global__ __launch_bounds__(kSortThreads) void radix_histogram_kernel(
+    const uint64_t* __restrict__ keys, const uint32_t* __restrict__ n_dev, long long n_cap,
+    int shift, uint32_t* __restrict__ hist, int nb) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const long long n = n_dev ? min((long long)*n_dev, n_cap) : n_cap;
+  const long long base = (long long)blockIdx.x * kSortTile;
+#pragma unroll 4
+  for (int r = 0; r < kSortItems; ++r) {
+    const long long i = base + r * kSortThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 0xFFu], 1u);
+  }
+  __syncthreads();
+  hist[(long long)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
+    const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    const uint32_t* __restrict__ n_dev, long long n_cap, int shift,
+    const uint32_t* __restrict__ offsets, int nb, uint64_t* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out) {
+  __shared__ uint32_t running[256];
+  __shared__ uint32_t wcnt[4][257];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  running[t] = offsets[(long long)t * nb + blockIdx.x];
+  const long long n = n_dev ? min((long long)*n_dev, n_cap) : n_cap;
+  const long long base = (long long)blockIdx.x * kSortTile;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  for (int r = 0; r < kSortItems; ++r) {
+    if (base + (long long)r * kSortThreads >= n) break;       // uniform across the block
+    const long long i = base + r * kSortThreads + t;
+    const bool valid = i < n;
+    uint64_t key = 0;
+    uint32_t val = 0, d = 256;
+    if (valid) {
+      key = keys_in[i];
+      val = vals_in[i];
+      d = (uint32_t)(key >> shift) & 0xFFu;
+    }
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+    for (int k = lane; k < 257; k += 64) wcnt[w][k] = 0;
+    __syncthreads();
+    if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    uint32_t dst = 0;
+    if (valid) {
+      dst = running[d] + rank;
+      for (int k = 0; k < w; ++k) dst += wcnt[k][d];
+    }
+    __syncthreads();
+    running[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+    if (valid) {
+      keys_out[dst] = key;
+      vals_out[dst] = val;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Tile ranges of the sorted list (ranges must be zero-filled first).
+// ------------------------------------------------------------------------------------------
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, long long n, uint32_t v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void tile_ranges_kernel(
+    const uint64_t* __restrict__ keys, const uint32_t* __restrict__ n_dev, long long n_cap,
+    int rank_bits, uint2* __restrict__ ranges) {
+  const long long n = n_dev ? min((long long)*n_dev, n_cap) : n_cap;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t tile = (uint32_t)(keys[i] >> rank_bits);
+  if (i == 0 || (uint32_t)(keys[i - 1] >> rank_bits) != tile) ranges[tile].x = (uint32_t)i;
+  if (i == n - 1 || (uint32_t)(keys[i + 1] >> rank_bits) != tile) ranges[tile].y = (uint32_t)(i + 1);
+}
+
+// ------------------------------------------------------------------------------------------
+// Host-side launch helpers (used by the C-ABI entry points).
+// ------------------------------------------------------------------------------------------
+hipError_t launch_init_view(ViewDesc* out, const ViewDesc& base, const float* w2c,
+                            const float* c2w, const float* bg, hipStream_t st) {
+  init_view_kernel<<<1, 64, 0, st>>>(out, base, w2c, c2w, bg);
+  return hipGetLastError();
+}
+
+hipError_t launch_preprocess(const ViewDesc* views, int n_views, int P, const float* centers,
+                             const float* lengths, const uint8_t* is_leaf, const int64_t* octpaths,
+                             int32_t* out_n_dup, uint32_t* n_dup, uint2* bboxes, uint8_t* quads,
+                             uint32_t* tz_min, hipStream_t st) {
+  if (P <= 0 || n_views <= 0) return hipSuccess;
+  dim3 grid(ceil_div(P, 256), n_views);
+  preprocess_kernel<<<grid, 256, 0, st>>>(views, P, centers, lengths, is_leaf, octpaths, out_n_dup,
+                                           n_dup, bboxes, quads, tz_min);
+  return hipGetLastError();
+}
+
+size_t scan_partial_count(long long n) { return (size_t)((n + kScanTile - 1) / kScanTile); }
+
+hipError_t launch_scan(const uint32_t* in, const uint32_t* n_dev, long long n_cap, uint32_t* partial,
+                       uint32_t* out, bool inclusive, hipStream_t st) {
+  if (n_cap <= 0) return hipSuccess;
+  const int nb = (int)scan_partial_count(n_cap);
+  scan_reduce_kernel<<<nb, kScanThreads, 0, st>>>(in, n_dev, n_cap, partial);
+  scan_partials_kernel<<<1, 1024, 0, st>>>(partial, nb);
+  scan_apply_kernel<<<nb, kScanThreads, 0, st>>>(in, n_dev, n_cap, partial, out, inclusive ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_duplicate(const ViewDesc* views, int n_views, int P, const int64_t* octpaths,
+                            const uint32_t* n_dup, const uint32_t* scan, const uint2* bboxes,
+                            const uint8_t* quads, int shift, int qb, int rank_bits, uint64_t* keys,
+                            uint32_t* vals, hipStream_t st) {
+  if (P <= 0 || n_views <= 0) return hipSuccess;
+  dim3 grid(ceil_div(P, 256), n_views);
+  duplicate_kernel<<<grid, 256, 0, st>>>(views, P, octpaths, n_dup, scan, bboxes, quads, shift, qb,
+                                          rank_bits, keys, vals);
+  return hipGetLastError();
+}
+
+size_t sort_tile_count(long long n) { return (size_t)((n + kSortTile - 1) / kSortTile); }
+
+// Sorts (keys, vals) by bits [0, nbits).  Buffers ping-pong; returns in `*result_in_alt`
+// whether the sorted data ended in the alt buffers.
+hipError_t launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                             const uint32_t* n_dev, long long n_cap, int nbits, uint32_t* hist,
+                             uint32_t* scan_partial, bool* result_in_alt, hipStream_t st) {
+  *result_in_alt = false;
+  if (n_cap <= 0) return hipSuccess;
+  const int nb = (int)sort_tile_count(n_cap);
+  const long long hn = 256LL * nb;
+  uint64_t* ki = keys; uint32_t* vi = vals;
+  uint64_t* ko = keys_alt; uint32_t* vo = vals_alt;
+  for (int shift = 0; shift < nbits; shift += 8) {
+    radix_histogram_kernel<<<nb, kSortThreads, 0, st>>>(ki, n_dev, n_cap, shift, hist, nb);
+    hipError_t e = launch_scan(hist, nullptr, hn, scan_partial, hist, false, st);
+    if (e != hipSuccess) return e;
+    radix_scatter_kernel<<<nb, kSortThreads, 0, st>>>(ki, vi, n_dev, n_cap, shift, hist, nb, ko, vo);
+    std::swap(ki, ko);
+    std::swap(vi, vo);
+    *result_in_alt = !*result_in_alt;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, long long n, uint32_t v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  fill_u32_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_ranges(const uint64_t* keys, const uint32_t* n_dev, long long n_cap, int rank_bits,
+                              uint2* ranges, hipStream_t st) {
+  if (n_cap <= 0) return hipSuccess;
+  tile_ranges_kernel<<<(unsigned)((n_cap + 255) / 256), 256, 0, st>>>(keys, n_dev, n_cap, rank_bits, ranges);
+  return hipGetLastError();
+}
+
+}  // namespace svr

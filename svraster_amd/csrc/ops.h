@@ -1,0 +1,36 @@
+// Launch helpers of the per-voxel operators and grid losses (voxel_ops.hip, grid_loss.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace svr {
+
+hipError_t launch_sh_forward(int n, const int64_t* idx, int deg, int M, const float* centers,
+                             const float* cam, const float* sh0, const float* shs, float* rgbs,
+                             hipStream_t st);
+hipError_t launch_sh_backward(int n, const int64_t* idx, int deg, int M, const float* centers,
+                              const float* cam, const float* dL_drgb, float* dsh0, float* dshs,
+                              hipStream_t st);
+hipError_t launch_gather_geo(int n, const int64_t* vox_key, const int64_t* care, const float* grid_pts,
+                             float* geos, hipStream_t st);
+hipError_t launch_gather_geo_bw(int n, const int64_t* vox_key, const int64_t* care, const float* dgeos,
+                                float* dgrid, hipStream_t st);
+hipError_t launch_adam(bool sparse, bool biased, long long n, float* p, const float* g, float* m, float* v,
+                       float step_size, float b1, float b2, float rsqrt_bc2, float eps, hipStream_t st);
+hipError_t launch_ijk_to_octpath(int n, const int64_t* ijk, const int8_t* lv, int64_t* out, hipStream_t st);
+hipError_t launch_octpath_to_ijk(int n, const int64_t* path, const int8_t* lv, int64_t* ijk, hipStream_t st);
+
+struct GridTable {
+  const float* grid_pts; const int64_t* vox_key; const int32_t* keys; const int32_t* key2vox;
+  const float* vox_coord; const float* vox_size; int n_keys; int res; int n_vox; long long n_pts;
+};
+hipError_t launch_grid_loss(bool laplacian, const GridTable& G, const bool* mask, const int32_t* active, int A,
+                            float weight, float inv, uint32_t seed, const float* randv, float* grad,
+                            hipStream_t st);
+hipError_t launch_table_count(int N, const float* center, const float* size, const float* scene_center,
+                              float extent, int res, const bool* is_leaf, uint32_t* count, hipStream_t st);
+hipError_t launch_table_write(int N, const float* center, const float* size, const float* scene_center,
+                              float extent, int res, const bool* is_leaf, const uint32_t* scan, bool* mask,
+                              int32_t* keys, int32_t* key2vox, hipStream_t st);
+
+}  // namespace svr

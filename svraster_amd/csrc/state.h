@@ -1,0 +1,186 @@
+// Layout of the three opaque per-call state buffers (uint8 tensors on the Python side).
+// The reference keeps the same three roles (cuda/src/raster_state.h:41-78) with its own private
+// layout; this layout is private to this library and every sub-array is 256-byte aligned.
+//
+//   geometry state  (V views x P voxels): view table, counters, n_dup, inclusive scan,
+//                   packed pixel bboxes, octant bitsets, scan scratch
+//   binning state   (K keys): sorted keys/values, the ping-pong pair, radix-sort scratch
+//   image state     (pixels, tiles): tile ranges, last-contributor bound, per-pixel contributor
+//                   count, work-queue counter, view table copy
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+#include "common.h"
+
+namespace svr {
+
+size_t scan_partial_count(long long n);
+size_t sort_tile_count(long long n);
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct Carver {
+  size_t off = 0;
+  template <typename T>
+  size_t take(long long count) {
+    const size_t o = off;
+    off = align256(off + sizeof(T) * (size_t)(count > 0 ? count : 0));
+    return o;
+  }
+};
+
+// Geometry state.
+struct GeomLayout {
+  size_t views, misc, n_dup, scan, bbox, quad, partial, total;
+  GeomLayout(long long n_views, long long P) {
+    Carver c;
+    const long long VP = n_views * P;
+    views = c.take<ViewDesc>(n_views);
+    misc = c.take<uint32_t>(64);
+    n_dup = c.take<uint32_t>(VP);
+    scan = c.take<uint32_t>(VP);
+    bbox = c.take<uint2>(VP);
+    quad = c.take<uint8_t>(VP);
+    partial = c.take<uint32_t>((long long)scan_partial_count(VP) + 1);
+    total = c.off;
+  }
+};
+
+struct GeomPtrs {
+  ViewDesc* views;
+  uint32_t* misc;      // [0] = low zero rank bits shared by all visible voxels
+  uint32_t* n_dup;
+  uint32_t* scan;
+  uint2* bbox;
+  uint8_t* quad;
+  uint32_t* partial;
+  GeomPtrs(void* base, long long n_views, long long P) {
+    GeomLayout L(n_views, P);
+    char* b = (char*)base;
+    views = (ViewDesc*)(b + L.views);
+    misc = (uint32_t*)(b + L.misc);
+    n_dup = (uint32_t*)(b + L.n_dup);
+    scan = (uint32_t*)(b + L.scan);
+    bbox = (uint2*)(b + L.bbox);
+    quad = (uint8_t*)(b + L.quad);
+    partial = (uint32_t*)(b + L.partial);
+  }
+};
+
+// Binning state.
+struct BinLayout {
+  size_t keys, vals, keys_alt, vals_alt, hist, partial, total;
+  explicit BinLayout(long long K) {
+    Carver c;
+    keys = c.take<uint64_t>(K);
+    vals = c.take<uint32_t>(K);
+    keys_alt = c.take<uint64_t>(K);
+    vals_alt = c.take<uint32_t>(K);
+    const long long hn = 256LL * (long long)sort_tile_count(K);
+    hist = c.take<uint32_t>(hn);
+    partial = c.take<uint32_t>((long long)scan_partial_count(hn) + 1);
+    total = c.off;
+  }
+};
+
+struct BinPtrs {
+  uint64_t* keys;
+  uint32_t* vals;
+  uint64_t* keys_alt;
+  uint32_t* vals_alt;
+  uint32_t* hist;
+  uint32_t* partial;
+  BinPtrs(void* base, long long K) {
+    BinLayout L(K);
+    char* b = (char*)base;
+    keys = (uint64_t*)(b + L.keys);
+    vals = (uint32_t*)(b + L.vals);
+    keys_alt = (uint64_t*)(b + L.keys_alt);
+    vals_alt = (uint32_t*)(b + L.vals_alt);
+    hist = (uint32_t*)(b + L.hist);
+    partial = (uint32_t*)(b + L.partial);
+  }
+};
+
+// Image state.
+struct ImageLayout {
+  size_t views, misc, ranges, tile_last, n_contrib, total;
+  ImageLayout(long long n_views, long long n_pix, long long n_tiles) {
+    Carver c;
+    views = c.take<ViewDesc>(n_views);
+    misc = c.take<uint32_t>(64);
+    ranges = c.take<uint2>(n_tiles);
+    tile_last = c.take<uint32_t>(n_tiles);
+    n_contrib = c.take<uint32_t>(n_pix);
+    total = c.off;
+  }
+};
+
+struct ImagePtrs {
+  ViewDesc* views;
+  uint32_t* misc;      // [0] = forward work counter, [1] = backward work counter
+  uint2* ranges;
+  uint32_t* tile_last;
+  uint32_t* n_contrib;
+  ImagePtrs(void* base, long long n_views, long long n_pix, long long n_tiles) {
+    ImageLayout L(n_views, n_pix, n_tiles);
+    char* b = (char*)base;
+    views = (ViewDesc*)(b + L.views);
+    misc = (uint32_t*)(b + L.misc);
+    ranges = (uint2*)(b + L.ranges);
+    tile_last = (uint32_t*)(b + L.tile_last);
+    n_contrib = (uint32_t*)(b + L.n_contrib);
+  }
+};
+
+// Launch helpers implemented in integer_stage.hip.
+hipError_t launch_init_view(ViewDesc* out, const ViewDesc& base, const float* w2c, const float* c2w,
+                            const float* bg, hipStream_t st);
+hipError_t launch_preprocess(const ViewDesc* views, int n_views, int P, const float* centers,
+                             const float* lengths, const uint8_t* is_leaf, const int64_t* octpaths,
+                             int32_t* out_n_dup, uint32_t* n_dup, uint2* bboxes, uint8_t* quads,
+                             uint32_t* tz_min, hipStream_t st);
+hipError_t launch_scan(const uint32_t* in, const uint32_t* n_dev, long long n_cap, uint32_t* partial,
+                       uint32_t* out, bool inclusive, hipStream_t st);
+hipError_t launch_duplicate(const ViewDesc* views, int n_views, int P, const int64_t* octpaths,
+                            const uint32_t* n_dup, const uint32_t* scan, const uint2* bboxes,
+                            const uint8_t* quads, int shift, int qb, int rank_bits, uint64_t* keys,
+                            uint32_t* vals, hipStream_t st);
+hipError_t launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                             const uint32_t* n_dev, long long n_cap, int nbits, uint32_t* hist,
+                             uint32_t* scan_partial, bool* result_in_alt, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, long long n, uint32_t v, hipStream_t st);
+hipError_t launch_tile_ranges(const uint64_t* keys, const uint32_t* n_dev, long long n_cap, int rank_bits,
+                              uint2* ranges, hipStream_t st);
+
+// Render launch helpers implemented in render_fwd.hip / render_bwd.hip.
+struct FwdArgs {
+  const ViewDesc* views; int n_views; int total_tiles;
+  const uint2* ranges; const uint32_t* vals;
+  const uint2* bboxes; long long P;
+  const float* centers; const float* lengths; const float* geos; const float* rgbs;
+  const float* s_val;
+  uint32_t* tile_last; uint32_t* n_contrib;
+  float* color; float* depth; float* normal; float* T; float* max_w;
+  uint32_t* work_counter;          // null: one workgroup per tile
+  int need_depth, need_normal;
+};
+hipError_t launch_render_forward(const FwdArgs& a, hipStream_t st);
+
+struct BwdArgs {
+  const ViewDesc* views; int n_views; int total_tiles;
+  const uint2* ranges; const uint32_t* tile_last; const uint32_t* vals;
+  const uint2* bboxes; long long P;
+  const float* centers; const float* lengths; const float* geos; const float* rgbs;
+  const float* s_val;
+  const float* T; const uint32_t* n_contrib;
+  const float* dL_dcolor; const float* dL_ddepth; const float* dL_dnormal; const float* dL_dT;
+  const float* gt_color; float lambda_R_concen;
+  float* dL_dvox;                  // [P,12]
+  float* dL_ds;                    // [1]
+  uint32_t* work_counter;
+  int need_depth, need_normal;
+};
+hipError_t launch_render_backward(const BwdArgs& a, hipStream_t st);
+
+}  // namespace svr

@@ -1,0 +1,241 @@
+// Per-voxel operators around the compositor on gfx950: spherical-harmonics colour and its
+// transpose, corner-SDF gather and scatter-add, the Adam step, and octree index conversions.
+//
+// References:
+//   SH            cuda/src/sh_compute.cu:33-299 (degree <= 3, direction = voxel centre minus
+//                 camera position per voxel, +0.5 offset, clamp >= 0 in forward only)
+//   corner gather cuda/src/geo_params_gather.cu:22-72, 134-173
+//   Adam          cuda/src/adam_step.cu:21-158
+//   octree utils  cuda/src/utils.cu:129-189
+#include "common.h"
+#include "ops.h"
+
+namespace svr {
+
+// Real SH basis constants, degrees 0..3 (textbook values).
+__constant__ float kC0 = 0.28209479177387814f;
+__constant__ float kC1 = 0.4886025119029199f;
+__constant__ float kC2[5] = {1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f,
+                             -1.0925484305920792f, 0.5462742152960396f};
+__constant__ float kC3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f,
+                             0.3731763325901154f, -0.4570457994644658f, 1.445305721320277f,
+                             -0.5900435899266435f};
+
+// Basis values b[0..15] for direction (x, y, z); returns the number used at degree `deg`.
+__device__ inline int sh_basis(int deg, float x, float y, float z, float b[16]) {
+  b[0] = kC0;
+  if (deg < 1) return 1;
+  b[1] = -kC1 * y; b[2] = kC1 * z; b[3] = -kC1 * x;
+  if (deg < 2) return 4;
+  const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+  b[4] = kC2[0] * xy;
+  b[5] = kC2[1] * yz;
+  b[6] = kC2[2] * (2.f * zz - xx - yy);
+  b[7] = kC2[3] * xz;
+  b[8] = kC2[4] * (xx - yy);
+  if (deg < 3) return 9;
+  b[9] = kC3[0] * y * (3.f * xx - yy);
+  b[10] = kC3[1] * xy * z;
+  b[11] = kC3[2] * y * (4.f * zz - xx - yy);
+  b[12] = kC3[3] * z * (2.f * zz - 3.f * xx - 3.f * yy);
+  b[13] = kC3[4] * x * (4.f * zz - xx - yy);
+  b[14] = kC3[5] * z * (xx - yy);
+  b[15] = kC3[6] * x * (xx - 3.f * yy);
+  return 16;
+}
+
+__device__ inline void sh_dir(const float* __restrict__ centers, const float* __restrict__ cam, int v,
+                              float& x, float& y, float& z) {
+  x = centers[3 * v] - cam[0];
+  y = centers[3 * v + 1] - cam[1];
+  z = centers[3 * v + 2] - cam[2];
+  const float rn = rsqrtf(x * x + y * y + z * z);
+  x *= rn; y *= rn; z *= rn;
+}
+
+__global__ __launch_bounds__(256) void sh_forward_kernel(int n, const int64_t* __restrict__ idx, int deg,
+                                                         int M, const float* __restrict__ centers,
+                                                         const float* __restrict__ cam,
+                                                         const float* __restrict__ sh0,
+                                                         const float* __restrict__ shs,
+                                                         float* __restrict__ rgbs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int v = idx ? (int)idx[t] : t;
+  float x, y, z;
+  sh_dir(centers, cam, v, x, y, z);
+  float b[16];
+  const int nb = sh_basis(deg, x, y, z, b);
+  float r0 = b[0] * sh0[3 * v], r1 = b[0] * sh0[3 * v + 1], r2 = b[0] * sh0[3 * v + 2];
+  const float* c = shs + (long long)v * (M - 1) * 3;
+  for (int k = 1; k < nb; ++k) {
+    r0 += b[k] * c[3 * (k - 1)];
+    r1 += b[k] * c[3 * (k - 1) + 1];
+    r2 += b[k] * c[3 * (k - 1) + 2];
+  }
+  r0 += 0.5f; r1 += 0.5f; r2 += 0.5f;
+  rgbs[3 * v] = fmaxf(r0, 0.f);
+  rgbs[3 * v + 1] = fmaxf(r1, 0.f);
+  rgbs[3 * v + 2] = fmaxf(r2, 0.f);
+}
+
+// Transpose of the linear part; the forward clamp is not masked (sh_compute.cu:118-121).
+__global__ __launch_bounds__(256) void sh_backward_kernel(int n, const int64_t* __restrict__ idx, int deg,
+                                                          int M, const float* __restrict__ centers,
+                                                          const float* __restrict__ cam,
+                                                          const float* __restrict__ dL_drgb,
+                                                          float* __restrict__ dsh0,
+                                                          float* __restrict__ dshs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int v = idx ? (int)idx[t] : t;
+  float x, y, z;
+  sh_dir(centers, cam, v, x, y, z);
+  float b[16];
+  const int nb = sh_basis(deg, x, y, z, b);
+  const float g0 = dL_drgb[3 * v], g1 = dL_drgb[3 * v + 1], g2 = dL_drgb[3 * v + 2];
+  dsh0[3 * v] = b[0] * g0; dsh0[3 * v + 1] = b[0] * g1; dsh0[3 * v + 2] = b[0] * g2;
+  float* c = dshs + (long long)v * (M - 1) * 3;
+  for (int k = 1; k < nb && k < M; ++k) {
+    c[3 * (k - 1)] = b[k] * g0;
+    c[3 * (k - 1) + 1] = b[k] * g1;
+    c[3 * (k - 1) + 2] = b[k] * g2;
+  }
+}
+
+__global__ __launch_bounds__(256) void gather_geo_kernel(int n, const int64_t* __restrict__ vox_key,
+                                                         const int64_t* __restrict__ care,
+                                                         const float* __restrict__ grid_pts,
+                                                         float* __restrict__ geos) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const long long v = care[t];
+  const int64_t* k = vox_key + v * 8;
+  float4 a, b;
+  a.x = grid_pts[k[0]]; a.y = grid_pts[k[1]]; a.z = grid_pts[k[2]]; a.w = grid_pts[k[3]];
+  b.x = grid_pts[k[4]]; b.y = grid_pts[k[5]]; b.z = grid_pts[k[6]]; b.w = grid_pts[k[7]];
+  reinterpret_cast<float4*>(geos)[2 * v] = a;
+  reinterpret_cast<float4*>(geos)[2 * v + 1] = b;
+}
+
+__global__ __launch_bounds__(256) void gather_geo_bw_kernel(int n, const int64_t* __restrict__ vox_key,
+                                                            const int64_t* __restrict__ care,
+                                                            const float* __restrict__ dgeos,
+                                                            float* __restrict__ dgrid) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const long long v = care[t];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const float g = dgeos[v * 8 + i];
+    if (g != 0.f) atomicAdd(dgrid + vox_key[v * 8 + i], g);
+  }
+}
+
+// Adam (adam_step.cu:21-102).  lerp(v0, v1, t) = fma(t, v1, fma(-t, v0, v0)).
+__device__ inline float adam_lerp(float v0, float v1, float t) { return fmaf(t, v1, fmaf(-t, v0, v0)); }
+
+template <bool kSparse, bool kBiased>
+__global__ __launch_bounds__(256) void adam_kernel(long long n, float* __restrict__ p,
+                                                   const float* __restrict__ g, float* __restrict__ m,
+                                                   float* __restrict__ v, float step_size, float b1,
+                                                   float b2, float rsqrt_bc2, float eps) {
+#pragma clang fp contract(off)
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gv = g[i];
+  if (kSparse && gv == 0.0f) return;
+  const float mv = adam_lerp(gv, m[i], b1);
+  const float vv = adam_lerp(gv * gv, v[i], b2);
+  const float denom = kBiased ? sqrtf(vv) + eps : fmaf(sqrtf(vv), rsqrt_bc2, eps);
+  p[i] -= step_size * (mv / denom);
+  m[i] = mv;
+  v[i] = vv;
+}
+
+__global__ void ijk_to_octpath_kernel(int n, const int64_t* __restrict__ ijk, const int8_t* __restrict__ lv,
+                                      int64_t* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int i = (int)ijk[3 * t], j = (int)ijk[3 * t + 1], k = (int)ijk[3 * t + 2];
+  int64_t path = 0;
+  for (int l = 0; l < kMaxLevels; ++l) {
+    path |= (int64_t)(((i & 1) << 2) | ((j & 1) << 1) | (k & 1)) << (3 * l);
+    i >>= 1; j >>= 1; k >>= 1;
+  }
+  out[t] = path << (3 * (kMaxLevels - lv[t]));
+}
+
+__global__ void octpath_to_ijk_kernel(int n, const int64_t* __restrict__ path_in, const int8_t* __restrict__ lv,
+                                      int64_t* __restrict__ ijk) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int64_t path = path_in[t] >> (3 * (kMaxLevels - lv[t]));
+  int i = 0, j = 0, k = 0;
+  for (int l = 0; l < kMaxLevels; ++l) {
+    const int b = (int)(path & 7);
+    i |= ((b >> 2) & 1) << l;
+    j |= ((b >> 1) & 1) << l;
+    k |= (b & 1) << l;
+    path >>= 3;
+  }
+  ijk[3 * t] = i; ijk[3 * t + 1] = j; ijk[3 * t + 2] = k;
+}
+
+// ------------------------------------------------------------------------------------------
+static inline unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
+
+hipError_t launch_sh_forward(int n, const int64_t* idx, int deg, int M, const float* centers,
+                             const float* cam, const float* sh0, const float* shs, float* rgbs,
+                             hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  sh_forward_kernel<<<blocks_for(n), 256, 0, st>>>(n, idx, deg, M, centers, cam, sh0, shs, rgbs);
+  return hipGetLastError();
+}
+
+hipError_t launch_sh_backward(int n, const int64_t* idx, int deg, int M, const float* centers,
+                              const float* cam, const float* dL_drgb, float* dsh0, float* dshs,
+                              hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  sh_backward_kernel<<<blocks_for(n), 256, 0, st>>>(n, idx, deg, M, centers, cam, dL_drgb, dsh0, dshs);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_geo(int n, const int64_t* vox_key, const int64_t* care, const float* grid_pts,
+                             float* geos, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  gather_geo_kernel<<<blocks_for(n), 256, 0, st>>>(n, vox_key, care, grid_pts, geos);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_geo_bw(int n, const int64_t* vox_key, const int64_t* care, const float* dgeos,
+                                float* dgrid, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  gather_geo_bw_kernel<<<blocks_for(n), 256, 0, st>>>(n, vox_key, care, dgeos, dgrid);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(bool sparse, bool biased, long long n, float* p, const float* g, float* m, float* v,
+                       float step_size, float b1, float b2, float rsqrt_bc2, float eps, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const unsigned nb = blocks_for(n);
+  if (sparse && biased) adam_kernel<true, true><<<nb, 256, 0, st>>>(n, p, g, m, v, step_size, b1, b2, rsqrt_bc2, eps);
+  else if (sparse) adam_kernel<true, false><<<nb, 256, 0, st>>>(n, p, g, m, v, step_size, b1, b2, rsqrt_bc2, eps);
+  else if (biased) adam_kernel<false, true><<<nb, 256, 0, st>>>(n, p, g, m, v, step_size, b1, b2, rsqrt_bc2, eps);
+  else adam_kernel<false, false><<<nb, 256, 0, st>>>(n, p, g, m, v, step_size, b1, b2, rsqrt_bc2, eps);
+  return hipGetLastError();
+}
+
+hipError_t launch_ijk_to_octpath(int n, const int64_t* ijk, const int8_t* lv, int64_t* out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  ijk_to_octpath_kernel<<<blocks_for(n), 256, 0, st>>>(n, ijk, lv, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_octpath_to_ijk(int n, const int64_t* path, const int8_t* lv, int64_t* ijk, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  octpath_to_ijk_kernel<<<blocks_for(n), 256, 0, st>>>(n, path, lv, ijk);
+  return hipGetLastError();
+}
+
+}  // namespace svr

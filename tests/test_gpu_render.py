@@ -1,0 +1,135 @@
+"""Compositor forward/backward on the GPU vs the FP64 oracle.
+
+Tolerances (FP32 kernel vs FP64 oracle): image, T and depth channel 0 max-abs <= 1e-4; normal
+max-abs <= 1e-4; n_contrib equal on >= 99.9 % of pixels (a saturation or 1e-5 alpha cut landing
+within FP32 rounding of the threshold may move it); gradients relative L2 <= 1e-3."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import raster as orc
+from svraster_amd import _C
+from svraster_amd.synthetic import make_workload
+from tests._util import hip_raster, oracle_binning, oracle_render, oracle_rgbs, rel_l2, say
+
+
+def _setup(name, s_mult=1.0):
+    w = make_workload(name, device="cuda")
+    sc, cam = w.scene, w.cameras[0]
+    rgbs64 = oracle_rgbs(sc, cam)
+    geos32 = sc.geo_grid_pts.reshape(-1)[sc.vox_key].float().contiguous()
+    s = w.s_val * s_mult
+    return w, sc, cam, geos32, rgbs64, s
+
+
+def _compare_forward(name, window=None, s_mult=1.0):
+    w, sc, cam, geos32, rgbs64, s = _setup(name, s_mult)
+    bg = torch.ones(3, device="cuda")
+    s32 = torch.tensor([s], dtype=torch.float32, device="cuda")
+    out = hip_raster(sc, cam, geos32, rgbs64.float().cuda(), s32, bg, track_max_w=True)
+    bn = oracle_binning(sc, cam)
+    geos64 = geos32.cpu().double()
+    s64 = torch.tensor(float(s32.item()), dtype=torch.float64)
+    ref = oracle_render(sc, cam, bn, geos64, rgbs64, s64, [1.0, 1.0, 1.0], window=window)
+    x0, y0, x1, y1 = ref.window
+    sl = (slice(None), slice(y0, y1), slice(x0, x1))
+    col = out["color"][sl].cpu().double()
+    err_c = (col - ref.color.detach()).abs().max().item()
+    err_t = (out["T"][sl].cpu().double() - ref.T.detach()).abs().max().item()
+    err_d = (out["depth"][0, y0:y1, x0:x1].cpu().double() - ref.depth[0].detach()).abs().max().item()
+    err_n = (out["normal"][sl].cpu().double() - ref.normal.detach()).abs().max().item()
+    _, _, ncon = _C.unpack_ImageState(cam.image_width, cam.image_height, out["img"])
+    same = (ncon[y0:y1, x0:x1].cpu().long() == ref.n_contrib).double().mean().item()
+    dmed_ok = ((out["depth"][2, y0:y1, x0:x1].cpu().double() - ref.depth[2]).abs() <= 1e-3).double().mean().item()
+    say(f"  color {err_c:.2e} T {err_t:.2e} depth {err_d:.2e} normal {err_n:.2e} "
+        f"n_contrib-match {same:.5f} median-depth-match {dmed_ok:.5f}")
+    assert err_c <= 1e-4 and err_t <= 1e-4 and err_d <= 1e-4 and err_n <= 1e-4
+    assert same >= 0.999
+    assert dmed_ok >= 0.999
+    if window is None:
+        mw = out["max_w"].reshape(-1).cpu().double()
+        assert (mw - ref.max_w).abs().max().item() <= 1e-4
+    return out, ref
+
+
+@pytest.mark.gpu
+def test_forward_small_128_full_frame():
+    say("[forward] small_128 full frame")
+    _compare_forward("small_128")
+
+
+@pytest.mark.gpu
+def test_forward_small_128_sharp_s():
+    say("[forward] small_128, s x 33 (late-training sharpness)")
+    _compare_forward("small_128", s_mult=33.0)
+
+
+@pytest.mark.gpu
+def test_forward_single_1237_crop():
+    say("[forward] single_1237, 64x64 crop at the image centre")
+    w = make_workload("single_1237", device="cpu")
+    cx, cy = w.cameras[0].image_width // 2, w.cameras[0].image_height // 2
+    _compare_forward("single_1237", window=(cx - 32 - cx % 16, cy - 32 - cy % 16,
+                                            cx + 32 - cx % 16, cy + 32 - cy % 16))
+
+
+def _compare_backward(name, window=None, concen=0.0, s_mult=1.0):
+    w, sc, cam, geos32, rgbs64, s = _setup(name, s_mult)
+    bg = torch.ones(3, device="cuda")
+    s32 = torch.tensor([s], dtype=torch.float32, device="cuda")
+    rgbs32 = rgbs64.float().cuda()
+    out = hip_raster(sc, cam, geos32, rgbs32, s32, bg)
+    H, W = cam.image_height, cam.image_width
+    g = torch.Generator().manual_seed(1)
+    dC = torch.rand(3, H, W, generator=g) - 0.5
+    dD = torch.zeros(4, H, W)
+    dD[0] = torch.rand(H, W, generator=g) - 0.5
+    dN = torch.rand(3, H, W, generator=g) - 0.5
+    dT = torch.rand(1, H, W, generator=g) - 0.5
+    gt = torch.rand(3, H, W, generator=g)
+    if window is not None:
+        x0, y0, x1, y1 = window
+        m = torch.zeros(1, H, W)
+        m[:, y0:y1, x0:x1] = 1
+        dC, dD, dN, dT = dC * m, dD * m, dN * m, dT * m
+    dgeo, drgb, dsub, ds = _C.rasterize_voxels_backward(
+        out["K"], 3, 1, W, H, cam.tanfovx, cam.tanfovy, cam.cx, cam.cy, cam.w2c, cam.c2w, bg,
+        sc.octpath, sc.vox_center, sc.vox_size, geos32, rgbs32, out["geom"], out["binning"], out["img"],
+        out["T"], dC.cuda(), dD.cuda(), dN.cuda(), dT.cuda(), concen, gt.cuda(), 0.0, 0.0, True, True,
+        out["depth"], out["normal"], False, s32, torch.empty(0))
+    bn = oracle_binning(sc, cam)
+    geos64 = geos32.cpu().double().requires_grad_(True)
+    r64 = rgbs32.cpu().double().requires_grad_(True)
+    s64 = torch.tensor(float(s32.item()), dtype=torch.float64, requires_grad=True)
+    ref = oracle_render(sc, cam, bn, geos64, r64, s64, [1.0, 1.0, 1.0], window=window,
+                        lambda_R_concen=concen, gt_color=gt)
+    x0, y0, x1, y1 = ref.window
+    sl = (slice(None), slice(y0, y1), slice(x0, x1))
+    gg, gr, gs, gsd = orc.backward(ref, geos64, r64, s64, dC[sl], dD[sl], dN[sl], dT[sl])
+    e_geo = rel_l2(dgeo, gg)
+    e_rgb = rel_l2(drgb, gr)
+    e_sub = rel_l2(dsub.reshape(-1), gs)
+    e_s = abs(ds.item() - gsd.item()) / max(abs(gsd.item()), 1e-12)
+    say(f"  rel-L2 geo {e_geo:.2e} rgb {e_rgb:.2e} subdiv {e_sub:.2e} ds {e_s:.2e}")
+    assert e_geo <= 1e-3 and e_rgb <= 1e-3 and e_sub <= 1e-3 and e_s <= 1e-3
+
+
+@pytest.mark.gpu
+def test_backward_small_128_full_frame():
+    say("[backward] small_128 full frame, color+depth+normal+T cotangents")
+    _compare_backward("small_128")
+
+
+@pytest.mark.gpu
+def test_backward_small_128_color_concentration():
+    say("[backward] small_128 with lambda_R_concen = 0.01")
+    _compare_backward("small_128", concen=0.01)
+
+
+@pytest.mark.gpu
+def test_backward_single_1237_crop():
+    say("[backward] single_1237 64x64 crop")
+    w = make_workload("single_1237", device="cpu")
+    cx, cy = w.cameras[0].image_width // 2, w.cameras[0].image_height // 2
+    _compare_backward("single_1237", window=(cx - 32 - cx % 16, cy - 32 - cy % 16,
+                                             cx + 32 - cx % 16, cy + 32 - cy % 16))

@@ -1,0 +1,148 @@
+"""Pins of the FP64 compositor oracle (CPU only): closed forms, the scalar reference-control-flow
+loop, and central finite differences of its autograd backward."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import binning as ob
+from oracle import raster as orc
+from oracle import sh as osh
+from svraster_amd.synthetic import look_at_camera, make_workload
+
+
+def _one_voxel_scene(geo_fn, size=0.5, center=(0.0, 0.0, 0.0)):
+    """Single voxel; corner SDFs from geo_fn(corner xyz)."""
+    c = np.array(center, dtype=np.float64)
+    corners = np.array([[(i >> 2) & 1, (i >> 1) & 1, i & 1] for i in range(8)], dtype=np.float64)
+    xyz = c - size / 2 + corners * size
+    geos = torch.tensor([[geo_fn(p) for p in xyz]], dtype=torch.float64)
+    return (torch.tensor([c], dtype=torch.float32), torch.tensor([[size]], dtype=torch.float32),
+            torch.zeros(1, 1, dtype=torch.int64), geos)
+
+
+def _render_single(geos, centers, sizes, s, W=32, H=32, rgb=(1.0, 1.0, 1.0), bg=(0.0, 0.0, 0.0)):
+    cam = look_at_camera(np.array([0.0, 0.0, -2.0]), W, H, 0.8 * W, 0.8 * W)
+    bn = ob.binning(W, H, cam.tanfovx, cam.tanfovy, cam.cx, cam.cy, cam.w2c.numpy(), cam.c2w.numpy(),
+                    0.02, np.zeros(1, np.int64), centers.numpy(), sizes.numpy(), np.ones(1))
+    rgbs = torch.tensor([rgb], dtype=torch.float64)
+    out = orc.render(orc.camera_from(cam), list(bg), bn.vals, bn.ranges, bn.pre.bbox, centers, sizes,
+                     geos, rgbs, torch.tensor(s, dtype=torch.float64))
+    return cam, out
+
+
+def test_constant_sdf_is_transparent():
+    centers, sizes, _, geos = _one_voxel_scene(lambda p: 0.3)
+    _, out = _render_single(geos, centers, sizes, 50.0, bg=(0.2, 0.4, 0.6))
+    assert torch.allclose(out.T, torch.ones_like(out.T))
+    assert torch.allclose(out.color[0], torch.full_like(out.color[0], 0.2))
+
+
+def test_linear_sdf_alpha_closed_form():
+    """Linear SDF along z: trilinear interpolation is exact, so along the central ray (which
+    enters at z = -0.25 and exits at z = +0.25) alpha = (sig(s f_a) - sig(s f_b)) / (sig(s f_a) + 1e-6)."""
+    s = 20.0
+    centers, sizes, _, geos = _one_voxel_scene(lambda p: -p[2] + 0.05)
+    cam, out = _render_single(geos, centers, sizes, s)
+    sig = lambda x: 1.0 / (1.0 + np.exp(-x))
+    # pixel (15, 15)/(16,16) rays are off-centre; use the ray through the image centre instead
+    rc = orc.camera_from(cam)
+    ro, rd, _ = orc.pixel_rays(rc, torch.tensor([16]), torch.tensor([16]))
+    a, b = orc.ray_aabb(centers.double(), sizes.double().reshape(-1), ro, (1.0 / rd[0])[None])
+    fa = -(ro[2] + a * rd[0, 2]).item() + 0.05
+    fb = -(ro[2] + b * rd[0, 2]).item() + 0.05
+    alpha = (sig(s * fa) - sig(s * fb)) / (sig(s * fa) + orc.EPS_PHI)
+    assert abs((1.0 - out.T[0, 16, 16]).item() - alpha) < 1e-12
+    # with rgb = 1 and black background, colour = 1 - T (sum of weights + T = 1)
+    assert abs(out.color[0, 16, 16].item() - alpha) < 1e-12
+
+
+def test_weights_plus_T_sum_to_one_and_subalpha_telescoping():
+    w = make_workload("small_128")
+    sc, cam = w.scene, w.cameras[0]
+    bn = ob.binning_for_camera(cam, sc)
+    geos = sc.geo_grid_pts.reshape(-1)[sc.vox_key].double()
+    ones = torch.ones(sc.num_voxels, 3, dtype=torch.float64)
+    out = orc.render(orc.camera_from(cam), [0.0, 0.0, 0.0], bn.vals, bn.ranges, bn.pre.bbox, sc.vox_center,
+                     sc.vox_size, geos, ones, torch.tensor(w.s_val, dtype=torch.float64),
+                     window=(40, 40, 88, 88), need_depth=False, need_normal=False)
+    assert torch.allclose(out.color[0] + out.T[0], torch.ones_like(out.T[0]), atol=1e-12)
+    # telescoping of the sub-interval alphas: 1 - prod(1 - a_k) == a up to the 1e-6 epsilon
+    phi = torch.sigmoid(torch.tensor([3.0, 1.0, -0.5, -2.0], dtype=torch.float64))
+    sdf = torch.logit(phi).reshape(1, 4)
+    a, sub = orc.alphas_from_sdf(sdf, torch.tensor(1.0, dtype=torch.float64))
+    assert abs((1 - torch.prod(1 - sub)).item() - a.item()) < 1e-5
+
+
+def test_vectorised_matches_scalar_reference_loop():
+    w = make_workload("small_128")
+    sc, cam = w.scene, w.cameras[0]
+    bn = ob.binning_for_camera(cam, sc)
+    geos = sc.geo_grid_pts.reshape(-1)[sc.vox_key].double()
+    rgbs = osh.sh_rgb(0, None, sc.vox_center, cam.c2w[:3, 3], sc.sh0.double(), sc.shs.double()).detach()
+    win = (56, 56, 72, 72)
+    out = orc.render(orc.camera_from(cam), [1.0, 1.0, 1.0], bn.vals, bn.ranges, bn.pre.bbox, sc.vox_center,
+                     sc.vox_size, geos, rgbs, torch.tensor(w.s_val, dtype=torch.float64), window=win)
+    pix = [(x, y) for x in range(56, 72, 3) for y in range(56, 72, 3)]
+    ref = orc.render_scalar(orc.camera_from(cam), [1.0, 1.0, 1.0], bn.vals, bn.ranges, bn.pre.bbox,
+                            sc.vox_center.numpy(), sc.vox_size.numpy(), geos.numpy(), rgbs.numpy(), w.s_val, pix)
+    for (x, y), r in ref.items():
+        assert np.allclose(out.color[:, y - 56, x - 56].numpy(), r["color"], atol=1e-12)
+        assert abs(out.T[0, y - 56, x - 56].item() - r["T"]) < 1e-12
+        assert out.n_contrib[y - 56, x - 56].item() == r["n_contrib"]
+        assert abs(out.depth[0, y - 56, x - 56].item() - r["depth0"]) < 1e-12
+
+
+@pytest.mark.parametrize("what", ["geo", "rgb", "s"])
+def test_backward_matches_central_finite_differences(what):
+    """Central differences (h = 1e-4, FP64) of L = <g_c, color> + <g_t, T> (+ depth and normal
+    for corner SDFs / colours; s only through colour and T: the reference's dL/ds ignores the
+    depth sub-alphas)."""
+    w = make_workload("small_128")
+    sc, cam = w.scene, w.cameras[0]
+    bn = ob.binning_for_camera(cam, sc)
+    rc = orc.camera_from(cam)
+    win = (60, 60, 76, 76)
+    geos0 = sc.geo_grid_pts.reshape(-1)[sc.vox_key].double()
+    rgbs0 = osh.sh_rgb(0, None, sc.vox_center, cam.c2w[:3, 3], sc.sh0.double(), sc.shs.double()).detach()
+    s0 = torch.tensor(w.s_val, dtype=torch.float64)
+    g = torch.Generator().manual_seed(3)
+    gc = torch.rand(3, 16, 16, generator=g, dtype=torch.float64) - 0.5
+    gd = torch.rand(4, 16, 16, generator=g, dtype=torch.float64) - 0.5
+    gn = torch.rand(3, 16, 16, generator=g, dtype=torch.float64) - 0.5
+    gt = torch.rand(1, 16, 16, generator=g, dtype=torch.float64) - 0.5
+    with_dn = what != "s"
+
+    def loss(geos, rgbs, s):
+        o = orc.render(rc, [1.0, 1.0, 1.0], bn.vals, bn.ranges, bn.pre.bbox, sc.vox_center, sc.vox_size,
+                       geos, rgbs, s, window=win, need_depth=with_dn, need_normal=with_dn)
+        L = (o.color * gc).sum() + (o.T * gt).sum()
+        if with_dn:
+            L = L + (o.depth[0] * gd[0]).sum() + (o.normal * gn).sum()
+        return L, o
+
+    geos = geos0.clone().requires_grad_(True)
+    rgbs = rgbs0.clone().requires_grad_(True)
+    s = s0.clone().requires_grad_(True)
+    L, o = loss(geos, rgbs, s)
+    dgeo, drgb, _, ds = orc.backward(o, geos, rgbs, s, gc, gd if with_dn else None,
+                                     gn if with_dn else None, gt)
+    h = 1e-4
+    if what == "s":
+        num = (loss(geos0, rgbs0, s0 + h)[0] - loss(geos0, rgbs0, s0 - h)[0]).item() / (2 * h)
+        assert abs(num - ds.item()) <= 1e-4 * max(1.0, abs(num))
+        return
+    grad = dgeo if what == "geo" else drgb
+    base = geos0 if what == "geo" else rgbs0
+    flat = grad.reshape(-1).abs()
+    idx = torch.topk(flat, 12).indices.tolist()
+    for i in idx:
+        e = torch.zeros_like(base).reshape(-1)
+        e[i] = h
+        e = e.reshape(base.shape)
+        if what == "geo":
+            lp, lm = loss(geos0 + e, rgbs0, s0)[0], loss(geos0 - e, rgbs0, s0)[0]
+        else:
+            lp, lm = loss(geos0, rgbs0 + e, s0)[0], loss(geos0, rgbs0 - e, s0)[0]
+        num = (lp - lm).item() / (2 * h)
+        ana = grad.reshape(-1)[i].item()
+        assert abs(num - ana) <= 1e-4 * max(1.0, abs(num)), (i, num, ana)
