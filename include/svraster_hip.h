@@ -157,6 +157,50 @@ int svr_valid_gradient_table_fill(const float* vox_center, const float* vox_size
                                   void* workspace, uint8_t* grid_mask, int32_t* grid_keys,
                                   int32_t* grid2voxel, void* stream);
 
+/* ---- batched views (no counterpart in the reference: its API renders one view per call) ----
+ * One workspace of svr_batch_workspace_bytes(V, P, n_pixels, n_tiles, K_cap) bytes serves V
+ * views of one voxel set.  Per-pixel outputs are concatenated per view, each view [C, H, W].
+ * Per-view colours: rgbs_views [V, P, 3].  No call below synchronises except
+ * svr_batch_key_count; keys beyond K_cap are dropped (check with svr_batch_key_count).
+ * svr_batch_setup: host arrays W, H, tanfovx, tanfovy, cx, cy, near_plane [V]; device
+ * w2c, c2w [V,16] and background [3]. */
+size_t svr_batch_workspace_bytes(int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap);
+int svr_batch_setup(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                    const int* W, const int* H, const float* tanfovx, const float* tanfovy,
+                    const float* cx, const float* cy, const float* near_plane, const float* w2c,
+                    const float* c2w, const float* background, void* stream);
+/* Preprocess (bbox, octants, n_dup) of every (view, voxel) and the inclusive scan. */
+int svr_batch_preprocess(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                         const int64_t* octree_paths, const float* vox_centers, const float* vox_lengths,
+                         const uint8_t* is_leaf, void* stream);
+/* SH colour of every (view, visible voxel) pair into rgbs_views. */
+int svr_batch_sh_forward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                         const float* vox_centers, const float* sh0, const float* shs, int degree, int M,
+                         float* rgbs_views, void* stream);
+/* Duplication, sort, tile ranges and one persistent compositor launch over all (view, tile)
+ * pairs.  rank_shift: 3 * (16 - deepest octree level present).  Outputs zeroed by the caller. */
+int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                        const int64_t* octree_paths, const float* vox_centers, const float* vox_lengths,
+                        const float* geos, const float* rgbs_views, const float* s_val, int rank_shift,
+                        int need_depth, int need_normal, float* out_color, float* out_depth,
+                        float* out_normal, float* out_T, void* stream);
+/* One persistent reverse compositor launch over all (view, tile) pairs.  dL_dvox [P,12]
+ * (colour slots unused), dL_drgb_views [V,P,3] and dL_ds [1] zeroed by the caller. */
+int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                       const float* vox_centers, const float* vox_lengths, const float* geos,
+                       const float* rgbs_views, const float* s_val, const float* out_T,
+                       const float* dL_dcolor, const float* dL_ddepth, const float* dL_dnormal,
+                       const float* dL_dT, int need_depth, int need_normal, float* dL_dvox,
+                       float* dL_drgb_views, float* dL_ds, void* stream);
+/* SH transpose summed over the views, one voxel per thread; writes every row of dL_dsh0 [P,3]
+ * and dL_dshs [P,M-1,3]. */
+int svr_batch_sh_backward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                          const float* vox_centers, const float* dL_drgb_views, int degree, int M,
+                          float* dL_dsh0, float* dL_dshs, void* stream);
+/* Total key count of the last svr_batch_preprocess (synchronises `stream`). */
+int svr_batch_key_count(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                        long long* K_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -159,7 +159,7 @@ class OracleOut:
     n_contrib: torch.Tensor   # [h,w] int64
     max_w: torch.Tensor       # [P]
     reg_loss: torch.Tensor    # scalar: colour-concentration surrogate (0 when off)
-    pair_alpha: list          # per tile: (alpha tensor [npix, n] with grad, pc mask, vox ids)
+    pair_alpha: list          # per tile: (alpha [npix, n] with grad, pc mask, vox ids, dalpha/ds)
     window: tuple             # (x0, y0, x1, y1)
 
 
@@ -266,7 +266,12 @@ def render(cam: RasterCamera, bg, bin_vals: np.ndarray, bin_ranges: np.ndarray,
                 if need_depth:
                     dmed = median_depth(T_excl.detach(), pc, vid, centers, lengths,
                                         geos.detach(), float(s.detach()), ro, rd)
-                pair_alpha.append((alpha, pc, vid))
+                with torch.no_grad():
+                    p0 = torch.sigmoid(s * sdf4[..., 0])
+                    p3 = torch.sigmoid(s * sdf4[..., 3])
+                    dads = ((p0 * (1 - p0) * sdf4[..., 0] * (1 - alpha) - p3 * (1 - p3) * sdf4[..., 3])
+                            / (p0 + EPS_PHI))
+                pair_alpha.append((alpha, pc, vid, dads))
             lx, ly = px - x0, py - y0
             col_parts.append((ly, lx, Cpix + Tfin[:, None] * bg[None]))
             T_parts.append((ly, lx, Tfin))
@@ -340,9 +345,10 @@ def median_depth(T_excl, pc, vid, centers, lengths, geos, s, ro, rd):
 
 
 def backward(out: OracleOut, geos, rgbs, s, dL_dcolor, dL_ddepth=None, dL_dnormal=None,
-             dL_dT=None):
+             dL_dT=None, return_ds_abs=False):
     """Gradients the reference's backward returns: (dL_dgeos, dL_drgbs, subdiv_p_bw, dL_ds)
-    for the given output cotangents (each [C,h,w] over the rendered window)."""
+    for the given output cotangents (each [C,h,w] over the rendered window).  With return_ds_abs,
+    also the sum over pairs of |dL/dalpha * dalpha/ds| -- the scale of the cancelling sum dL/ds."""
     loss = (out.color * dL_dcolor.double()).sum()
     if dL_ddepth is not None:
         loss = loss + (out.depth[0] * dL_ddepth.double()[0]).sum()
@@ -356,13 +362,17 @@ def backward(out: OracleOut, geos, rgbs, s, dL_dcolor, dL_ddepth=None, dL_dnorma
     g_geo, g_rgb, g_s = gs[0], gs[1], gs[2]
     P = geos.shape[0]
     subdiv = torch.zeros(P, dtype=torch.float64)
-    for (alpha, pc, vid), ga in zip(out.pair_alpha, gs[3:]):
+    ds_abs = 0.0
+    for (alpha, pc, vid, dads), ga in zip(out.pair_alpha, gs[3:]):
         if ga is None:
             continue
-        v = (ga * alpha.detach()).abs() * pc.to(torch.float64)
+        pcf = pc.to(torch.float64)
+        v = (ga * alpha.detach()).abs() * pcf
         subdiv.index_add_(0, vid, v.sum(0))
+        ds_abs += float(((ga * dads).abs() * pcf).sum())
     z = lambda t, ref: torch.zeros_like(ref) if t is None else t
-    return z(g_geo, geos), z(g_rgb, rgbs), subdiv, z(g_s, s)
+    res = (z(g_geo, geos), z(g_rgb, rgbs), subdiv, z(g_s, s))
+    return res + (ds_abs,) if return_ds_abs else res
 
 
 # ----------------------------------------------------------------------------------------------

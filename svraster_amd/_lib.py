@@ -48,6 +48,14 @@ _SIGS = {
     "svr_valid_gradient_table_workspace_bytes": (SZ, [I]),
     "svr_valid_gradient_table_count": (I, [P, P, P, F, I, P, I, P, ctypes.POINTER(LL), P]),
     "svr_valid_gradient_table_fill": (I, [P, P, P, F, I, P, I, P, P, P, P, P]),
+    "svr_batch_workspace_bytes": (SZ, [I, I, LL, LL, LL]),
+    "svr_batch_setup": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, P, P, P, P, P]),
+    "svr_batch_preprocess": (I, [P, I, I, LL, LL, LL, P, P, P, P, P]),
+    "svr_batch_sh_forward": (I, [P, I, I, LL, LL, LL, P, P, P, I, I, P, P]),
+    "svr_batch_rasterize": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, I, I, I, P, P, P, P, P]),
+    "svr_batch_backward": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, P, P, P, P, I, I, P, P, P, P]),
+    "svr_batch_sh_backward": (I, [P, I, I, LL, LL, LL, P, P, I, I, P, P, P]),
+    "svr_batch_key_count": (I, [P, I, I, LL, LL, LL, ctypes.POINTER(LL), P]),
 }
 
 _lib = None

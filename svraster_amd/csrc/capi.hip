@@ -237,7 +237,7 @@ int svr_rasterize_forward(int vox_geo_mode, int density_mode, int W, int H, floa
   uint32_t* v1 = start_alt ? bp.vals : bp.vals_alt;
   if (K > 0) {
     SVR_TRY(launch_duplicate(ip.views, 1, P, octree_paths, gp.n_dup, gp.scan, gp.bbox, gp.quad, kl.shift, kl.qb,
-                             kl.rank_bits, k0, v0, st));
+                             kl.rank_bits, k0, v0, K, st));
     bool in_alt = false;
     SVR_TRY(launch_radix_sort(k0, v0, k1, v1, n_dev, K, kl.nbits, bp.hist, bp.partial, &in_alt, st));
   }
@@ -417,6 +417,195 @@ int svr_ijk_2_octpath(const int64_t* ijk, const int8_t* octlevel, int n, int64_t
 int svr_octpath_2_ijk(const int64_t* octpath, const int8_t* octlevel, int n, int64_t* ijk, void* stream) {
   g_err.clear();
   SVR_TRY(launch_octpath_to_ijk(n, octpath, octlevel, ijk, as_stream(stream)));
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Batched views: one workspace holding the geometry state of V views x P voxels, a binning
+// state of fixed capacity and the image state of all views' tiles and pixels.  Every launch
+// covers all views; key counts stay on the device, so a whole step can be captured as a graph.
+// ------------------------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+struct BatchPtrs {
+  GeomPtrs g;
+  BinPtrs b;
+  ImagePtrs im;
+  BatchPtrs(void* ws, int V, int P, long long n_pix, long long n_tiles, long long cap)
+      : g(ws, V, P),
+        b((char*)ws + GeomLayout(V, P).total, cap),
+        im((char*)ws + GeomLayout(V, P).total + BinLayout(cap).total, V, n_pix, n_tiles) {}
+};
+
+int check_batch(int V, int P, long long n_pix, long long n_tiles, long long cap) {
+  if (V <= 0 || V > 4096) return fail(1, "view count must be 1..4096%s (got %lld)", "", V);
+  if (P <= 0 || P >= (1 << 29)) return fail(1, "voxel count must be 1..2^29%s (got %lld)", "", P);
+  if ((long long)V * P >= (1LL << 32)) return fail(1, "views x voxels must stay below 2^32%s (%lld)", "", (long long)V * P);
+  if (n_tiles <= 0 || n_tiles >= (1LL << 31) || n_pix <= 0) return fail(1, "bad tile/pixel totals%s%lld", "", n_tiles);
+  if (cap <= 0 || cap >= (1LL << 31)) return fail(1, "key capacity must be 1..2^31%s (got %lld)", "", cap);
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+size_t svr_batch_workspace_bytes(int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap) {
+  return GeomLayout(n_views, P).total + BinLayout(K_cap).total + ImageLayout(n_views, n_pixels, n_tiles).total;
+}
+
+int svr_batch_setup(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                    const int* W, const int* H, const float* tanfovx, const float* tanfovy,
+                    const float* cx, const float* cy, const float* near_plane, const float* w2c,
+                    const float* c2w, const float* background, void* stream) {
+  g_err.clear();
+  if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
+  hipStream_t st = as_stream(stream);
+  BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
+  long long tile_base = 0, pix_base = 0;
+  for (int v = 0; v < n_views; ++v) {
+    if (int e = check_image(W[v], H[v], P)) return e;
+    ViewDesc d = make_view(W[v], H[v], tanfovx[v], tanfovy[v], cx[v], cy[v], near_plane[v]);
+    d.tile_base = (int)tile_base;
+    d.pix_base = pix_base;
+    SVR_TRY(launch_init_view(bp.g.views + v, d, w2c + 16 * v, c2w + 16 * v, background, st));
+    tile_base += (long long)d.tiles_x * d.tiles_y;
+    pix_base += (long long)W[v] * H[v];
+  }
+  if (tile_base != n_tiles || pix_base != n_pixels)
+    return fail(1, "tile/pixel totals do not match the views%s (tiles %lld)", "", tile_base);
+  return 0;
+}
+
+int svr_batch_preprocess(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                         const int64_t* octree_paths, const float* vox_centers, const float* vox_lengths,
+                         const uint8_t* is_leaf, void* stream) {
+  g_err.clear();
+  if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
+  hipStream_t st = as_stream(stream);
+  BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
+  const long long VP = (long long)n_views * P;
+  SVR_TRY(launch_preprocess(bp.g.views, n_views, P, vox_centers, vox_lengths, is_leaf, octree_paths, nullptr,
+                            bp.g.n_dup, bp.g.bbox, bp.g.quad, nullptr, st));
+  SVR_TRY(launch_scan(bp.g.n_dup, nullptr, VP, bp.g.partial, bp.g.scan, true, st));
+  return 0;
+}
+
+int svr_batch_sh_forward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                         const float* vox_centers, const float* sh0, const float* shs, int degree, int M,
+                         float* rgbs_views, void* stream) {
+  g_err.clear();
+  if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
+  if (degree < 0 || degree > 3 || M < (degree + 1) * (degree + 1)) return fail(1, "bad SH degree%s %lld", "", degree);
+  BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
+  SVR_TRY(launch_sh_views_forward(P, n_views, bp.g.n_dup, bp.g.views, degree, M, vox_centers, sh0, shs, rgbs_views,
+                                  as_stream(stream)));
+  return 0;
+}
+
+int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                        const int64_t* octree_paths, const float* vox_centers, const float* vox_lengths,
+                        const float* geos, const float* rgbs_views, const float* s_val, int rank_shift,
+                        int need_depth, int need_normal, float* out_color, float* out_depth, float* out_normal,
+                        float* out_T, void* stream) {
+  g_err.clear();
+  if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
+  if (rank_shift < 0 || rank_shift > kRankBits || rank_shift % 3) return fail(1, "bad rank_shift %s%lld", "", rank_shift);
+  const KeyLayout kl = key_layout(rank_shift, n_tiles);
+  if (kl.nbits > 64) return fail(1, "tile id + rank bits exceed 64%s (%lld)", "", kl.nbits);
+  hipStream_t st = as_stream(stream);
+  BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
+  const long long VP = (long long)n_views * P;
+  const uint32_t* n_dev = bp.g.scan + (VP - 1);
+  const int passes = (kl.nbits + 7) / 8;
+  const bool start_alt = (passes % 2) == 1;
+  uint64_t* k0 = start_alt ? bp.b.keys_alt : bp.b.keys;
+  uint32_t* v0 = start_alt ? bp.b.vals_alt : bp.b.vals;
+  uint64_t* k1 = start_alt ? bp.b.keys : bp.b.keys_alt;
+  uint32_t* v1 = start_alt ? bp.b.vals : bp.b.vals_alt;
+  SVR_TRY(launch_duplicate(bp.g.views, n_views, P, octree_paths, bp.g.n_dup, bp.g.scan, bp.g.bbox, bp.g.quad,
+                           kl.shift, kl.qb, kl.rank_bits, k0, v0, K_cap, st));
+  bool in_alt = false;
+  SVR_TRY(launch_radix_sort(k0, v0, k1, v1, n_dev, K_cap, kl.nbits, bp.b.hist, bp.b.partial, &in_alt, st));
+  SVR_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(bp.im.ranges), 2 * n_tiles, 0u, st));
+  SVR_TRY(launch_tile_ranges(bp.b.keys, n_dev, K_cap, kl.rank_bits, bp.im.ranges, st));
+  SVR_TRY(launch_fill_u32(bp.im.misc, 2, 0u, st));
+  FwdArgs a;
+  memset(&a, 0, sizeof(a));
+  a.views = bp.g.views; a.n_views = n_views; a.total_tiles = (int)n_tiles;
+  a.ranges = bp.im.ranges; a.vals = bp.b.vals;
+  a.bboxes = bp.g.bbox; a.P = P;
+  a.centers = vox_centers; a.lengths = vox_lengths; a.geos = geos; a.rgbs = rgbs_views;
+  a.rgb_view_stride = 3LL * P;
+  a.s_val = s_val;
+  a.tile_last = bp.im.tile_last; a.n_contrib = bp.im.n_contrib;
+  a.color = out_color; a.depth = out_depth; a.normal = out_normal; a.T = out_T; a.max_w = nullptr;
+  a.work_counter = bp.im.misc;
+  a.need_depth = need_depth; a.need_normal = need_normal;
+  {
+    LaunchTimer timer(0, st);
+    SVR_TRY(launch_render_forward(a, st));
+  }
+  return 0;
+}
+
+int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                       const float* vox_centers, const float* vox_lengths, const float* geos,
+                       const float* rgbs_views, const float* s_val, const float* out_T, const float* dL_dcolor,
+                       const float* dL_ddepth, const float* dL_dnormal, const float* dL_dT, int need_depth,
+                       int need_normal, float* dL_dvox, float* dL_drgb_views, float* dL_ds, void* stream) {
+  g_err.clear();
+  if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
+  if (need_depth && !dL_ddepth) return fail(1, "need_depth requires dL_ddepth%s%lld", "", 0);
+  if (need_normal && !dL_dnormal) return fail(1, "need_normal requires dL_dnormal%s%lld", "", 0);
+  hipStream_t st = as_stream(stream);
+  BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
+  SVR_TRY(launch_fill_u32(bp.im.misc + 1, 1, 0u, st));
+  BwdArgs a;
+  memset(&a, 0, sizeof(a));
+  a.views = bp.g.views; a.n_views = n_views; a.total_tiles = (int)n_tiles;
+  a.ranges = bp.im.ranges; a.tile_last = bp.im.tile_last; a.vals = bp.b.vals;
+  a.bboxes = bp.g.bbox; a.P = P;
+  a.centers = vox_centers; a.lengths = vox_lengths; a.geos = geos; a.rgbs = rgbs_views;
+  a.rgb_view_stride = 3LL * P;
+  a.s_val = s_val;
+  a.T = out_T; a.n_contrib = bp.im.n_contrib;
+  a.dL_dcolor = dL_dcolor; a.dL_ddepth = dL_ddepth; a.dL_dnormal = dL_dnormal; a.dL_dT = dL_dT;
+  a.gt_color = nullptr; a.lambda_R_concen = 0.f;
+  a.dL_dvox = dL_dvox; a.dL_drgb_views = dL_drgb_views; a.dL_ds = dL_ds;
+  a.work_counter = bp.im.misc + 1;
+  a.need_depth = need_depth; a.need_normal = need_normal;
+  {
+    LaunchTimer timer(1, st);
+    SVR_TRY(launch_render_backward(a, st));
+  }
+  return 0;
+}
+
+int svr_batch_sh_backward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                          const float* vox_centers, const float* dL_drgb_views, int degree, int M,
+                          float* dL_dsh0, float* dL_dshs, void* stream) {
+  g_err.clear();
+  if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
+  if (degree < 0 || degree > 3 || M < (degree + 1) * (degree + 1)) return fail(1, "bad SH degree%s %lld", "", degree);
+  BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
+  SVR_TRY(launch_sh_views_backward(P, n_views, bp.g.n_dup, bp.g.views, degree, M, vox_centers, dL_drgb_views,
+                                   dL_dsh0, dL_dshs, as_stream(stream)));
+  return 0;
+}
+
+int svr_batch_key_count(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                        long long* K_out, void* stream) {
+  g_err.clear();
+  *K_out = 0;
+  if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
+  hipStream_t st = as_stream(stream);
+  BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
+  uint32_t k = 0;
+  const long long VP = (long long)n_views * P;
+  SVR_TRY(hipMemcpyAsync(&k, bp.g.scan + (VP - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  SVR_TRY(hipStreamSynchronize(st));
+  *K_out = k;
   return 0;
 }
 

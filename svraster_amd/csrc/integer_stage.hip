@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void duplicate_kernel(
     const ViewDesc* __restrict__ views, int P, const int64_t* __restrict__ octpaths,
     const uint32_t* __restrict__ n_dup, const uint32_t* __restrict__ scan,
     const uint2* __restrict__ bboxes, const uint8_t* __restrict__ quads, int shift, int qb,
-    int rank_bits, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    int rank_bits, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, long long cap) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int view = blockIdx.y;
   if (i >= P) return;
@@ -225,8 +225,10 @@ __global__ __launch_bounds__(256) void duplicate_kernel(
     for (int y = ty0; y <= ty1; ++y)
       for (int x = tx0; x <= tx1; ++x) {
         const uint64_t tile = (uint64_t)(v.tile_base + y * v.tiles_x + x);
-        keys[off] = (tile << rank_bits) | rk;
-        vals[off] = val;
+        if ((long long)off < cap) {          // capacity guard for the preallocated batched path
+          keys[off] = (tile << rank_bits) | rk;
+          vals[off] = val;
+        }
         ++off;
       }
   }
@@ -361,11 +363,11 @@ hipError_t launch_scan(const uint32_t* in, const uint32_t* n_dev, long long n_ca
 hipError_t launch_duplicate(const ViewDesc* views, int n_views, int P, const int64_t* octpaths,
                             const uint32_t* n_dup, const uint32_t* scan, const uint2* bboxes,
                             const uint8_t* quads, int shift, int qb, int rank_bits, uint64_t* keys,
-                            uint32_t* vals, hipStream_t st) {
+                            uint32_t* vals, long long cap, hipStream_t st) {
   if (P <= 0 || n_views <= 0) return hipSuccess;
   dim3 grid(ceil_div(P, 256), n_views);
   duplicate_kernel<<<grid, 256, 0, st>>>(views, P, octpaths, n_dup, scan, bboxes, quads, shift, qb,
-                                          rank_bits, keys, vals);
+                                          rank_bits, keys, vals, cap);
   return hipGetLastError();
 }
 

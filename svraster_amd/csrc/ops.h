@@ -11,6 +11,13 @@ hipError_t launch_sh_forward(int n, const int64_t* idx, int deg, int M, const fl
 hipError_t launch_sh_backward(int n, const int64_t* idx, int deg, int M, const float* centers,
                               const float* cam, const float* dL_drgb, float* dsh0, float* dshs,
                               hipStream_t st);
+struct ViewDesc;
+hipError_t launch_sh_views_forward(int P, int V, const uint32_t* n_dup, const ViewDesc* views, int deg, int M,
+                                   const float* centers, const float* sh0, const float* shs, float* rgbs_views,
+                                   hipStream_t st);
+hipError_t launch_sh_views_backward(int P, int V, const uint32_t* n_dup, const ViewDesc* views, int deg, int M,
+                                    const float* centers, const float* drgb_views, float* dsh0, float* dshs,
+                                    hipStream_t st);
 hipError_t launch_gather_geo(int n, const int64_t* vox_key, const int64_t* care, const float* grid_pts,
                              float* geos, hipStream_t st);
 hipError_t launch_gather_geo_bw(int n, const int64_t* vox_key, const int64_t* care, const float* dgeos,

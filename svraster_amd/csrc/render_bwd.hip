@@ -105,7 +105,8 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
         const float4 g1 = reinterpret_cast<const float4*>(a.geos)[2 * vox + 1];
         s_g0[t] = g0;
         s_g1[t] = g1;
-        s_rgb[t] = make_float4(a.rgbs[3 * vox], a.rgbs[3 * vox + 1], a.rgbs[3 * vox + 2], __uint_as_float(val));
+        const float* rgbv = a.rgbs + (long long)vi * a.rgb_view_stride;
+        s_rgb[t] = make_float4(rgbv[3 * vox], rgbv[3 * vox + 1], rgbv[3 * vox + 2], __uint_as_float(val));
         if (kNormal) {
           const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
           float nx, ny, nz, rn;
@@ -242,7 +243,10 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
         }
         if (lane < 12) {
           const uint32_t vox = val & kVoxMask;
-          atomicAdd(a.dL_dvox + (long long)vox * 12 + lane, mine);
+          float* dst = a.dL_dvox + (long long)vox * 12 + lane;
+          if (a.dL_drgb_views && lane >= 8 && lane < 11)
+            dst = a.dL_drgb_views + ((long long)vi * a.P + vox) * 3 + (lane - 8);
+          atomicAdd(dst, mine);
         }
       }
     }

@@ -101,8 +101,9 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
     for (int base = 0; base < n; base += 256) {
       if (__syncthreads_count(done) == 256) break;
       if (base + t < n)
-        stage_entry(a.vals, rg.x + base + t, vi, a.P, a.bboxes, a.centers, a.lengths, a.geos, a.rgbs,
-                    kNormal, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t);
+        stage_entry(a.vals, rg.x + base + t, vi, a.P, a.bboxes, a.centers, a.lengths, a.geos,
+                    a.rgbs + (long long)vi * a.rgb_view_stride, kNormal, s_ch, s_ml, s_g0, s_g1, s_rgb,
+                    s_nrm, s_bb, t);
       __syncthreads();
       const int cnt = min(256, n - base);
       for (int j = 0; j < cnt; ++j) {

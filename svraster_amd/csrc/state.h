@@ -145,7 +145,7 @@ hipError_t launch_scan(const uint32_t* in, const uint32_t* n_dev, long long n_ca
 hipError_t launch_duplicate(const ViewDesc* views, int n_views, int P, const int64_t* octpaths,
                             const uint32_t* n_dup, const uint32_t* scan, const uint2* bboxes,
                             const uint8_t* quads, int shift, int qb, int rank_bits, uint64_t* keys,
-                            uint32_t* vals, hipStream_t st);
+                            uint32_t* vals, long long cap, hipStream_t st);
 hipError_t launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                              const uint32_t* n_dev, long long n_cap, int nbits, uint32_t* hist,
                              uint32_t* scan_partial, bool* result_in_alt, hipStream_t st);
@@ -159,6 +159,7 @@ struct FwdArgs {
   const uint2* ranges; const uint32_t* vals;
   const uint2* bboxes; long long P;
   const float* centers; const float* lengths; const float* geos; const float* rgbs;
+  long long rgb_view_stride;       // floats between the colour arrays of consecutive views (0: shared)
   const float* s_val;
   uint32_t* tile_last; uint32_t* n_contrib;
   float* color; float* depth; float* normal; float* T; float* max_w;
@@ -172,11 +173,13 @@ struct BwdArgs {
   const uint2* ranges; const uint32_t* tile_last; const uint32_t* vals;
   const uint2* bboxes; long long P;
   const float* centers; const float* lengths; const float* geos; const float* rgbs;
+  long long rgb_view_stride;
   const float* s_val;
   const float* T; const uint32_t* n_contrib;
   const float* dL_dcolor; const float* dL_ddepth; const float* dL_dnormal; const float* dL_dT;
   const float* gt_color; float lambda_R_concen;
   float* dL_dvox;                  // [P,12]
+  float* dL_drgb_views;            // [V,P,3] per-view colour gradients (null: into dL_dvox)
   float* dL_ds;                    // [1]
   uint32_t* work_counter;
   int need_depth, need_normal;

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void gather_geo_kernel(int n, const int64_t* _
                                                          float* __restrict__ geos) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const long long v = care[t];
+  const long long v = care ? care[t] : t;
   const int64_t* k = vox_key + v * 8;
   float4 a, b;
   a.x = grid_pts[k[0]]; a.y = grid_pts[k[1]]; a.z = grid_pts[k[2]]; a.w = grid_pts[k[3]];
@@ -124,11 +124,81 @@ __global__ __launch_bounds__(256) void gather_geo_bw_kernel(int n, const int64_t
                                                             float* __restrict__ dgrid) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const long long v = care[t];
+  const long long v = care ? care[t] : t;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const float g = dgeos[v * 8 + i];
     if (g != 0.f) atomicAdd(dgrid + vox_key[v * 8 + i], g);
+  }
+}
+
+// Batched views: colour of every (view, visible voxel) pair, direction from that view's camera.
+__global__ __launch_bounds__(256) void sh_views_forward_kernel(int P, const uint32_t* __restrict__ n_dup,
+                                                               const ViewDesc* __restrict__ views, int deg, int M,
+                                                               const float* __restrict__ centers,
+                                                               const float* __restrict__ sh0,
+                                                               const float* __restrict__ shs,
+                                                               float* __restrict__ rgbs_views) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int view = blockIdx.y;
+  if (i >= P) return;
+  const long long o = (long long)view * P + i;
+  if (n_dup[o] == 0) return;
+  const float* c2w = views[view].c2w;
+  const float cam[3] = {c2w[3], c2w[7], c2w[11]};
+  float x, y, z;
+  sh_dir(centers, cam, i, x, y, z);
+  float b[16];
+  const int nb = sh_basis(deg, x, y, z, b);
+  float r0 = b[0] * sh0[3 * i], r1 = b[0] * sh0[3 * i + 1], r2 = b[0] * sh0[3 * i + 2];
+  const float* c = shs + (long long)i * (M - 1) * 3;
+  for (int k = 1; k < nb; ++k) {
+    r0 += b[k] * c[3 * (k - 1)];
+    r1 += b[k] * c[3 * (k - 1) + 1];
+    r2 += b[k] * c[3 * (k - 1) + 2];
+  }
+  rgbs_views[3 * o] = fmaxf(r0 + 0.5f, 0.f);
+  rgbs_views[3 * o + 1] = fmaxf(r1 + 0.5f, 0.f);
+  rgbs_views[3 * o + 2] = fmaxf(r2 + 0.5f, 0.f);
+}
+
+// Transpose summed over the views, one voxel per thread: no atomics, and every output row is
+// written (zeros for voxels no view saw), so the outputs need no separate clearing.
+template <int kDeg>
+__global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, const uint32_t* __restrict__ n_dup,
+                                                                const ViewDesc* __restrict__ views, int M,
+                                                                const float* __restrict__ centers,
+                                                                const float* __restrict__ drgb_views,
+                                                                float* __restrict__ dsh0,
+                                                                float* __restrict__ dshs) {
+  constexpr int NB = (kDeg + 1) * (kDeg + 1);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  float acc[NB][3];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.f;
+  for (int v = 0; v < V; ++v) {
+    const long long o = (long long)v * P + i;
+    if (n_dup[o] == 0) continue;
+    const float g0 = drgb_views[3 * o], g1 = drgb_views[3 * o + 1], g2 = drgb_views[3 * o + 2];
+    const float* c2w = views[v].c2w;
+    const float cam[3] = {c2w[3], c2w[7], c2w[11]};
+    float x, y, z;
+    sh_dir(centers, cam, i, x, y, z);
+    float b[16];
+    sh_basis(kDeg, x, y, z, b);
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      acc[k][0] += b[k] * g0; acc[k][1] += b[k] * g1; acc[k][2] += b[k] * g2;
+    }
+  }
+  dsh0[3 * i] = acc[0][0]; dsh0[3 * i + 1] = acc[0][1]; dsh0[3 * i + 2] = acc[0][2];
+  float* c = dshs + (long long)i * (M - 1) * 3;
+  for (int k = 1; k < M; ++k) {
+    const bool on = k < NB;
+    c[3 * (k - 1)] = on ? acc[k < NB ? k : 0][0] : 0.f;
+    c[3 * (k - 1) + 1] = on ? acc[k < NB ? k : 0][1] : 0.f;
+    c[3 * (k - 1) + 2] = on ? acc[k < NB ? k : 0][2] : 0.f;
   }
 }
 
@@ -198,6 +268,29 @@ hipError_t launch_sh_backward(int n, const int64_t* idx, int deg, int M, const f
                               hipStream_t st) {
   if (n <= 0) return hipSuccess;
   sh_backward_kernel<<<blocks_for(n), 256, 0, st>>>(n, idx, deg, M, centers, cam, dL_drgb, dsh0, dshs);
+  return hipGetLastError();
+}
+
+hipError_t launch_sh_views_forward(int P, int V, const uint32_t* n_dup, const ViewDesc* views, int deg, int M,
+                                   const float* centers, const float* sh0, const float* shs, float* rgbs_views,
+                                   hipStream_t st) {
+  if (P <= 0 || V <= 0) return hipSuccess;
+  dim3 grid(blocks_for(P), V);
+  sh_views_forward_kernel<<<grid, 256, 0, st>>>(P, n_dup, views, deg, M, centers, sh0, shs, rgbs_views);
+  return hipGetLastError();
+}
+
+hipError_t launch_sh_views_backward(int P, int V, const uint32_t* n_dup, const ViewDesc* views, int deg, int M,
+                                    const float* centers, const float* drgb_views, float* dsh0, float* dshs,
+                                    hipStream_t st) {
+  if (P <= 0) return hipSuccess;
+  const unsigned nb = blocks_for(P);
+  switch (deg) {
+    case 0: sh_views_backward_kernel<0><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, drgb_views, dsh0, dshs); break;
+    case 1: sh_views_backward_kernel<1><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, drgb_views, dsh0, dshs); break;
+    case 2: sh_views_backward_kernel<2><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, drgb_views, dsh0, dshs); break;
+    default: sh_views_backward_kernel<3><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, drgb_views, dsh0, dshs); break;
+  }
   return hipGetLastError();
 }
 

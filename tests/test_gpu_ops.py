@@ -1,0 +1,134 @@
+"""Per-voxel operators on the GPU vs the oracle: SH colour and transpose, corner gather and
+scatter, Adam, octree codec, valid-gradient table, grid eikonal / Laplacian backward."""
+import numpy as np
+import pytest
+import torch
+
+import svraster_cuda
+from oracle import adam as oad
+from oracle import grid_loss as ogl
+from oracle import octree as oot
+from oracle import sh as osh
+from svraster_amd import _C
+from svraster_amd.synthetic import make_workload
+from tests._util import rel_l2, say
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("deg", [0, 1, 2, 3])
+def test_sh_forward_backward(deg):
+    say(f"[sh] degree {deg}")
+    rng = np.random.default_rng(deg)
+    P = 5000
+    centers = torch.from_numpy(rng.normal(size=(P, 3))).float()
+    cam = torch.tensor([2.0, -1.0, 0.5])
+    sh0 = torch.from_numpy(rng.normal(size=(P, 3))).float()
+    shs = torch.from_numpy(rng.normal(scale=0.3, size=(P, 15, 3))).float()
+    idx = torch.from_numpy(rng.choice(P, 3000, replace=False)).long()
+    got = _C.sh_compute(deg, idx.cuda(), centers.cuda(), cam.cuda(), sh0.cuda(), shs.cuda())
+    s0 = sh0.double().requires_grad_(True)
+    ss = shs.double().requires_grad_(True)
+    ref = osh.sh_rgb(deg, idx, centers.double(), cam.double(), s0, ss)
+    assert (got.cpu().double() - ref.detach()).abs().max().item() <= 1e-5
+    g = torch.from_numpy(rng.normal(size=(P, 3))).float()
+    d0, dss = _C.sh_compute_bw(deg, 16, idx.cuda(), centers.cuda(), cam.cuda(), got, g.cuda())
+    r0, rss = torch.autograd.grad((ref * g.double()).sum(), [s0, ss])
+    assert rel_l2(d0, r0) <= 1e-6 and rel_l2(dss, rss) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_gather_and_scatter():
+    say("[gather] corner SDF gather / scatter-add")
+    w = make_workload("fd_256", device="cuda")
+    sc = w.scene
+    care = torch.arange(0, sc.num_voxels, 3, device="cuda")
+    geos = _C.gather_triinterp_geo_params(sc.vox_key, care, sc.geo_grid_pts)
+    ref = sc.geo_grid_pts.reshape(-1)[sc.vox_key[care]]
+    assert torch.equal(geos[care], ref)
+    g = torch.randn(sc.num_voxels, 8, device="cuda")
+    dgrid = _C.gather_triinterp_geo_params_bw(sc.vox_key, care, sc.geo_grid_pts.shape[0], g)
+    refg = torch.zeros(sc.geo_grid_pts.shape[0], dtype=torch.float64)
+    refg.index_add_(0, sc.vox_key[care].reshape(-1).cpu(), g[care].reshape(-1).double().cpu())
+    assert rel_l2(dgrid.reshape(-1), refg) <= 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("biased,sparse", [(False, False), (False, True), (True, False), (True, True)])
+def test_adam(biased, sparse):
+    say(f"[adam] biased={biased} sparse={sparse}")
+    rng = np.random.default_rng(5)
+    n = 100_000
+    p = rng.normal(size=n).astype(np.float32)
+    m = np.zeros(n, np.float32)
+    v = np.zeros(n, np.float32)
+    pt, mt, vt = (torch.from_numpy(x.copy()).cuda() for x in (p, m, v))
+    ref = (p.astype(np.float64), m.astype(np.float64), v.astype(np.float64))
+    for step in range(1, 4):
+        g = rng.normal(size=n).astype(np.float32)
+        g[rng.random(n) < 0.3] = 0.0
+        gt = torch.from_numpy(g).cuda()
+        if biased:
+            _C.biased_adam_step(sparse, pt, gt, mt, vt, 1e-2, 0.1, 0.99, 1e-15)
+        else:
+            _C.unbiased_adam_step(sparse, pt, gt, mt, vt, step, 1e-2, 0.1, 0.99, 1e-15)
+        ref = oad.adam_step(ref[0], g, ref[1], ref[2], step, 1e-2, 0.1, 0.99, 1e-15, biased=biased, sparse=sparse)
+    assert np.abs(pt.cpu().numpy() - ref[0]).max() <= 1e-5
+    assert np.abs(mt.cpu().numpy() - ref[1]).max() <= 1e-6
+
+
+@pytest.mark.gpu
+def test_octree_codec():
+    say("[octree] ijk <-> octpath")
+    rng = np.random.default_rng(6)
+    lv = rng.integers(1, 17, 5000).astype(np.int8)
+    ijk = np.stack([rng.integers(0, 1 << int(l), 3) for l in lv]).astype(np.int64)
+    path = svraster_cuda.utils.ijk_2_octpath(torch.from_numpy(ijk).cuda(), torch.from_numpy(lv).cuda())
+    ref = np.array([oot.ijk_to_octpath(int(a), int(b), int(c), int(l)) for (a, b, c), l in zip(ijk, lv)])
+    assert np.array_equal(path.reshape(-1).cpu().numpy(), ref)
+    back = svraster_cuda.utils.octpath_2_ijk(path, torch.from_numpy(lv).cuda())
+    assert np.array_equal(back.cpu().numpy(), ijk)
+
+
+def _grid_setup(res_pow2=6):
+    w = make_workload("fd_256", device="cuda")
+    sc = w.scene
+    mask, keys, g2v = svraster_cuda.utils.valid_gradient_table(sc.vox_center, sc.vox_size, sc.scene_center, 2.0,
+                                                               res_pow2, sc.is_leaf.bool())
+    sk, order = keys.sort(stable=True)
+    g2v = g2v[order]
+    res = 1 << res_pow2
+    coord = ((sc.vox_center - sc.vox_size * 0.5) - (sc.scene_center - 1.0)) / 2.0 * res
+    size = (sc.vox_size / 2.0 * res).reshape(-1)
+    return sc, mask, keys, sk, g2v, coord, size, res
+
+
+@pytest.mark.gpu
+def test_valid_gradient_table():
+    say("[table] valid gradient table")
+    sc, mask, keys, sk, g2v, coord, size, res = _grid_setup()
+    omask, okeys, oown = ogl.valid_gradient_table(sc.vox_center.cpu().numpy(), sc.vox_size.cpu().numpy(),
+                                                  np.zeros(3), 2.0, 6, sc.is_leaf.cpu().numpy())
+    assert np.array_equal(mask.cpu().numpy(), omask)
+    assert np.array_equal(keys.cpu().numpy(), okeys)       # both in voxel order
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["ge", "ls"])
+def test_grid_losses(kind):
+    say(f"[grid loss] {kind}")
+    sc, mask, keys, sk, g2v, coord, size, res = _grid_setup()
+    G = sk.numel()
+    rng = np.random.default_rng(8)
+    active = torch.from_numpy(rng.permutation(G)[: G // 2].astype(np.int32)).cuda()
+    randv = torch.from_numpy(rng.uniform(size=(active.numel(), 3)).astype(np.float32)).cuda()
+    grad = torch.zeros_like(sc.geo_grid_pts)
+    fn = svraster_cuda.grid_loss_bw.grid_eikonal if kind == "ge" else svraster_cuda.grid_loss_bw.laplacian_smoothness
+    fn(grid_pts=sc.geo_grid_pts, vox_key=sc.vox_key, grid_voxel_coord=coord, grid_voxel_size=size, grid_res=res,
+       grid_mask=mask, grid_keys=sk, grid2voxel=g2v, active_list=active, weight=1e-3, vox_size_inv=float(res / 2.0),
+       no_tv_s=True, tv_sparse=False, grid_pts_grad=grad, randv=randv)
+    ref = ogl.grid_loss_grad(kind, sc.geo_grid_pts.cpu().numpy(), sc.vox_key.cpu().numpy(), coord.cpu().numpy(),
+                             size.cpu().numpy(), res, mask.cpu().numpy(), sk.cpu().numpy(), g2v.cpu().numpy(),
+                             active.cpu().numpy(), 1e-3, res / 2.0, randv.cpu().numpy())
+    e = rel_l2(grad.reshape(-1), torch.from_numpy(ref))
+    say(f"  rel-L2 {e:.2e}, |ref| {np.linalg.norm(ref):.3e}")
+    assert np.linalg.norm(ref) > 0 and e <= 1e-4

@@ -105,13 +105,19 @@ def _compare_backward(name, window=None, concen=0.0, s_mult=1.0):
                         lambda_R_concen=concen, gt_color=gt)
     x0, y0, x1, y1 = ref.window
     sl = (slice(None), slice(y0, y1), slice(x0, x1))
-    gg, gr, gs, gsd = orc.backward(ref, geos64, r64, s64, dC[sl], dD[sl], dN[sl], dT[sl])
+    gg, gr, gs, gsd, ds_abs = orc.backward(ref, geos64, r64, s64, dC[sl], dD[sl], dN[sl], dT[sl],
+                                           return_ds_abs=True)
     e_geo = rel_l2(dgeo, gg)
     e_rgb = rel_l2(drgb, gr)
     e_sub = rel_l2(dsub.reshape(-1), gs)
-    e_s = abs(ds.item() - gsd.item()) / max(abs(gsd.item()), 1e-12)
-    say(f"  rel-L2 geo {e_geo:.2e} rgb {e_rgb:.2e} subdiv {e_sub:.2e} ds {e_s:.2e}")
-    assert e_geo <= 1e-3 and e_rgb <= 1e-3 and e_sub <= 1e-3 and e_s <= 1e-3
+    # dL/ds is one scalar summed over every (pixel, voxel) pair with mixed signs: its FP32 error
+    # is bounded relative to the sum of the magnitudes of its terms, not to the cancelled total.
+    e_s = abs(ds.item() - gsd.item()) / max(abs(gsd.item()), 1e-30)
+    e_s_scaled = abs(ds.item() - gsd.item()) / max(ds_abs, 1e-30)
+    say(f"  rel-L2 geo {e_geo:.2e} rgb {e_rgb:.2e} subdiv {e_sub:.2e} ds {e_s:.2e} "
+        f"(|ds err| / sum|terms| {e_s_scaled:.2e}, sum|terms|/|ds| {ds_abs / max(abs(gsd.item()), 1e-30):.1f})")
+    assert e_geo <= 1e-3 and e_rgb <= 1e-3 and e_sub <= 1e-3
+    assert e_s <= 1e-3 or e_s_scaled <= 1e-5
 
 
 @pytest.mark.gpu
