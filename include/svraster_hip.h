@@ -33,6 +33,12 @@ int svr_abi_version(void);
 void svr_timing_enable(int on);
 int svr_timing_read(int kind, int reset, double* total_ms, long long* launches);
 
+/* Runtime options (no counterpart in the reference).  "compositor_variant": 0 = pair-compacted
+ * compositor kernels (default), 1 = one-entry-at-a-time kernels; both give the same images.
+ * svr_get_option returns -1 for an unknown name. */
+int svr_set_option(const char* name, int value);
+int svr_get_option(const char* name);
+
 /* ---- opaque state buffer sizes (the reference's raster_state.h:41-78 roles) ---------------- */
 size_t svr_geom_state_bytes(int n_views, int P);
 size_t svr_binning_state_bytes(long long K);

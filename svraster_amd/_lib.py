@@ -25,6 +25,8 @@ _SIGS = {
     "svr_last_error": (ctypes.c_char_p, []),
     "svr_abi_version": (I, []),
     "svr_timing_enable": (None, [I]),
+    "svr_set_option": (I, [ctypes.c_char_p, I]),
+    "svr_get_option": (I, [ctypes.c_char_p]),
     "svr_timing_read": (I, [I, I, ctypes.POINTER(D), ctypes.POINTER(LL)]),
     "svr_geom_state_bytes": (SZ, [I, I]),
     "svr_binning_state_bytes": (SZ, [LL]),
@@ -81,6 +83,14 @@ def load():
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def set_option(name: str, value: int):
+    check(load().svr_set_option(name.encode(), int(value)), f"set_option({name})")
+
+
+def get_option(name: str) -> int:
+    return int(load().svr_get_option(name.encode()))
 
 
 def check(rc: int, what: str = ""):

@@ -72,6 +72,9 @@ struct LaunchTimer {
   }
 };
 
+// Runtime options (svr_set_option): compositor variant, 0 = pair-compacted, 1 = one entry at a time.
+int g_compositor_variant = 0;
+
 int tile_bits_for(long long n_tiles) {
   int b = 0;
   while ((1LL << b) < n_tiles) ++b;
@@ -159,6 +162,21 @@ int svr_timing_read(int kind, int reset, double* total_ms, long long* launches) 
   return 0;
 }
 int svr_abi_version(void) { return 1; }
+
+int svr_set_option(const char* name, int value) {
+  g_err.clear();
+  if (name && strcmp(name, "compositor_variant") == 0) {
+    if (value < 0 || value > 1) return fail(1, "compositor_variant must be 0 or 1%s (got %lld)", "", value);
+    g_compositor_variant = value;
+    return 0;
+  }
+  return fail(1, "unknown option %s%lld", name ? name : "(null)", 0);
+}
+
+int svr_get_option(const char* name) {
+  if (name && strcmp(name, "compositor_variant") == 0) return g_compositor_variant;
+  return -1;
+}
 
 size_t svr_geom_state_bytes(int n_views, int P) { return GeomLayout(n_views, P).total; }
 size_t svr_binning_state_bytes(long long K) { return BinLayout(K).total; }
@@ -255,6 +273,7 @@ int svr_rasterize_forward(int vox_geo_mode, int density_mode, int W, int H, floa
   a.color = out_color; a.depth = out_depth; a.normal = out_normal; a.T = out_T; a.max_w = max_w;
   a.work_counter = nullptr;
   a.need_depth = need_depth; a.need_normal = need_normal;
+  a.variant = g_compositor_variant;
   {
     LaunchTimer timer(0, st);
     SVR_TRY(launch_render_forward(a, st));
@@ -301,6 +320,7 @@ int svr_rasterize_backward(long long K, int vox_geo_mode, int density_mode, int 
   a.dL_dvox = dL_dvox; a.dL_ds = dL_ds;
   a.work_counter = nullptr;
   a.need_depth = need_depth; a.need_normal = need_normal;
+  a.variant = g_compositor_variant;
   {
     LaunchTimer timer(1, st);
     SVR_TRY(launch_render_backward(a, st));
@@ -542,6 +562,7 @@ int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long l
   a.color = out_color; a.depth = out_depth; a.normal = out_normal; a.T = out_T; a.max_w = nullptr;
   a.work_counter = bp.im.misc;
   a.need_depth = need_depth; a.need_normal = need_normal;
+  a.variant = g_compositor_variant;
   {
     LaunchTimer timer(0, st);
     SVR_TRY(launch_render_forward(a, st));
@@ -575,6 +596,7 @@ int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long lo
   a.dL_dvox = dL_dvox; a.dL_drgb_views = dL_drgb_views; a.dL_ds = dL_ds;
   a.work_counter = bp.im.misc + 1;
   a.need_depth = need_depth; a.need_normal = need_normal;
+  a.variant = g_compositor_variant;
   {
     LaunchTimer timer(1, st);
     SVR_TRY(launch_render_backward(a, st));

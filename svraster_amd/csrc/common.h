@@ -215,6 +215,17 @@ __device__ inline float wave_sum(float v) {
   return v;
 }
 
+// Minimum over the currently active lanes of the wave (inactive lanes contribute nothing).
+__device__ inline uint32_t wave_min_active_u32(uint32_t v) {
+  const uint64_t act = __ballot(1);
+  uint32_t m = v;
+  for (uint64_t b = act; b; b &= b - 1) {
+    const int l = __ffsll((long long)b) - 1;
+    m = min(m, (uint32_t)__builtin_amdgcn_readlane((int)v, l));
+  }
+  return m;
+}
+
 __device__ inline uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));

@@ -91,12 +91,17 @@ __global__ __launch_bounds__(256) void preprocess_kernel(
   n_dup[o] = nd;
   bboxes[o] = bb;
   quads[o] = (uint8_t)qbits;
-  if (nd && tz_min) {
-    // Number of low rank bits (a multiple of 3) that are zero in this voxel's octpath.
-    const uint64_t p = (uint64_t)octpaths[i] & ((1ull << kRankBits) - 1ull);
-    uint32_t tz = p ? (uint32_t)__builtin_ctzll(p) : (uint32_t)kRankBits;
-    tz = min(tz - tz % 3u, (uint32_t)kRankBits);
-    atomicMin(tz_min, tz);
+  if (tz_min) {
+    // Number of low rank bits (a multiple of 3) that are zero in this voxel's octpath; reduced
+    // across the wave so that one atomic per wave reaches memory.
+    uint32_t tz = (uint32_t)kRankBits;
+    if (nd) {
+      const uint64_t p = (uint64_t)octpaths[i] & ((1ull << kRankBits) - 1ull);
+      tz = p ? (uint32_t)__builtin_ctzll(p) : (uint32_t)kRankBits;
+      tz = min(tz - tz % 3u, (uint32_t)kRankBits);
+    }
+    tz = wave_min_active_u32(tz);
+    if (__lane_id() == __ffsll((long long)__ballot(1)) - 1 && tz < (uint32_t)kRankBits) atomicMin(tz_min, tz);
   }
 }
 

@@ -27,6 +27,80 @@ __device__ inline int find_view_b(const ViewDesc* views, int n_views, int g) {
   return lo;
 }
 
+__device__ inline void stage_entry_b(const BwdArgs& a, uint32_t e, int vi, bool need_normal, float4* s_ch,
+                                     float4* s_ml, float4* s_g0, float4* s_g1, float4* s_rgb, float4* s_nrm,
+                                     uint2* s_bb, int slot) {
+  const uint32_t val = a.vals[e];
+  const uint32_t vox = val & kVoxMask;
+  s_bb[slot] = a.bboxes[(long long)vi * a.P + vox];
+  const float cx = a.centers[3 * vox], cy = a.centers[3 * vox + 1], cz = a.centers[3 * vox + 2];
+  const float l = a.lengths[vox];
+  s_ch[slot] = make_float4(cx, cy, cz, 0.5f * l);
+  s_ml[slot] = make_float4(cx - 0.5f * l, cy - 0.5f * l, cz - 0.5f * l, 1.f / l);
+  const float4 g0 = reinterpret_cast<const float4*>(a.geos)[2 * vox];
+  const float4 g1 = reinterpret_cast<const float4*>(a.geos)[2 * vox + 1];
+  s_g0[slot] = g0;
+  s_g1[slot] = g1;
+  const float* rgbv = a.rgbs + (long long)vi * a.rgb_view_stride;
+  s_rgb[slot] = make_float4(rgbv[3 * vox], rgbv[3 * vox + 1], rgbv[3 * vox + 2], __uint_as_float(val));
+  if (need_normal) {
+    const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+    float nx, ny, nz, rn;
+    lin_normal(gg, nx, ny, nz, rn);
+    s_nrm[slot] = make_float4(nx * rn, ny * rn, nz * rn, rn);
+  }
+}
+
+__device__ inline int select_set_bit_b(uint64_t m, int k) {
+  int pos = 0;
+  const int c32 = __popc((uint32_t)m);
+  if (k >= c32) { k -= c32; m >>= 32; pos += 32; }
+  uint32_t x = (uint32_t)m;
+  int c = __popc(x & 0xFFFFu);
+  if (k >= c) { k -= c; x >>= 16; pos += 16; }
+  c = __popc(x & 0xFFu);
+  if (k >= c) { k -= c; x >>= 8; pos += 8; }
+  c = __popc(x & 0xFu);
+  if (k >= c) { k -= c; x >>= 4; pos += 4; }
+  c = __popc(x & 0x3u);
+  if (k >= c) { k -= c; x >>= 2; pos += 2; }
+  if (k >= (int)(x & 1u)) pos += 1;
+  return pos;
+}
+
+__device__ inline uint64_t bbox_block_mask_b(uint32_t b0, uint32_t b1, int qx, int qy) {
+  const int x0 = max((int)(b0 >> 16) - qx, 0), x1 = min((int)(b1 >> 16) - qx, 7);
+  const int y0 = max((int)(b0 & 0xFFFFu) - qy, 0), y1 = min((int)(b1 & 0xFFFFu) - qy, 7);
+  if (x0 > x1 || y0 > y1) return 0ull;
+  const uint64_t cols = (uint64_t)(((1u << (x1 + 1)) - 1u) & ~((1u << x0) - 1u));
+  const uint64_t hi = (y1 == 7) ? ~0ull : ((1ull << (8 * (y1 + 1))) - 1ull);
+  const uint64_t lo = (1ull << (8 * y0)) - 1ull;
+  return cols * (0x0101010101010101ull & hi & ~lo);
+}
+
+__device__ inline void wave_lds_sync_b() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ inline uint32_t wave_incl_scan_b(uint32_t x, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = (uint32_t)__shfl_up((int)x, o, 64);
+    if (lane >= o) x += t;
+  }
+  return x;
+}
+
+// Ray of another pixel of the wave: same origin, direction from LDS, inverse recomputed
+// exactly as pixel_ray computes it.
+__device__ inline Ray lane_ray(const Ray& r, const float4 d) {
+  Ray pr = r;
+  pr.dx = d.x; pr.dy = d.y; pr.dz = d.z;
+  pr.ix = 1.f / d.x; pr.iy = 1.f / d.y; pr.iz = 1.f / d.z;
+  return pr;
+}
+
 template <bool kDepth, bool kNormal, bool kConcen>
 __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
   __shared__ float4 s_ch[256], s_ml[256], s_g0[256], s_g1[256], s_rgb[256];
@@ -258,10 +332,345 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
   if (t == 0) atomicAdd(a.dL_ds, (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]));
 }
 
+// ------------------------------------------------------------------------------------------
+// Pair-compacted variant of the backward.  Per staged batch (entries in back-to-front order):
+//   1. per wave, pixel masks of every entry (bbox, octant, pixels whose contributor bound
+//      reaches the batch) and the prefix of their popcounts;
+//   2. alpha (and depth value) of every (entry, pixel) pair, one pair per lane;
+//   3. the sequential reverse walk per pixel over the stored pair alphas: T <- T / (1 - a),
+//      dL/dalpha, the running dL/dT -- the only sequential part, a few FMAs per pair;
+//   4. per pair again, one pair per lane: the SDF-segment derivatives, accumulated per entry in
+//      LDS by all four waves; after the batch, one global atomic per (entry, gradient slot).
+// The per-voxel global atomics fall from 12 per (pixel, entry) in the reference to 12 per
+// (tile, entry).
+// ------------------------------------------------------------------------------------------
+constexpr int kBwdStage = 128;
+constexpr int kBwdCap = 128;
+
+template <bool kDepth, bool kNormal, bool kConcen>
+__global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
+  __shared__ float4 s_ch[kBwdStage], s_ml[kBwdStage], s_g0[kBwdStage], s_g1[kBwdStage], s_rgb[kBwdStage];
+  __shared__ float4 s_nrm[kNormal ? kBwdStage : 1];
+  __shared__ uint2 s_bb[kBwdStage];
+  __shared__ float s_acc[kBwdStage][12];
+  __shared__ uint32_t s_used[kBwdStage];
+  __shared__ uint64_t s_mask[4][kBwdStage];
+  __shared__ uint32_t s_pref[4][kBwdStage + 1];
+  __shared__ float4 s_rd[4][64];           // ray direction xyz, dL/ddepth
+  __shared__ float4 s_dc[4][64];           // dL/dcolor xyz, contributor bound (bits)
+  __shared__ float4 s_dn[4][kNormal ? 64 : 1];
+  __shared__ float4 s_gt[4][kConcen ? 64 : 1];
+  __shared__ float s_pa[4][kBwdCap];       // pair alpha (0 if skipped)
+  __shared__ float s_pd[4][kDepth ? kBwdCap : 1];
+  __shared__ float s_pl[4][kBwdCap];       // dL/dalpha
+  __shared__ float s_pt[4][kBwdCap];       // T before the entry (negative: pair does not contribute)
+  __shared__ float s_red[4];
+  __shared__ int s_tile;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const float s = *a.s_val;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  float ds_acc = 0.f;
+
+  for (int iter = 0;; ++iter) {
+    int g;
+    if (a.work_counter) {
+      if (t == 0) s_tile = (int)atomicAdd(a.work_counter, 1u);
+      __syncthreads();
+      g = s_tile;
+      __syncthreads();
+      if (g >= a.total_tiles) break;
+    } else {
+      if (iter > 0) break;
+      g = blockIdx.x;
+    }
+    const int vi = find_view_b(a.views, a.n_views, g);
+    const ViewDesc& V = a.views[vi];
+    const int W = V.W, H = V.H;
+    const int lt = g - V.tile_base;
+    const int tx = lt % V.tiles_x, ty = lt / V.tiles_x;
+    const int qx = tx * kTile + (w & 1) * 8, qy = ty * kTile + (w >> 1) * 8;
+    const int px = qx + (lane & 7), py = qy + (lane >> 3);
+    const bool inside = px < W && py < H;
+    const Ray r = pixel_ray(V, px, py);
+    const uint32_t start = a.ranges[g].x;
+    const uint32_t end = a.tile_last[g];
+    const int n = (int)(end - start);
+
+    const long long HW = (long long)W * H;
+    const long long pid = (long long)py * W + px;
+    const long long pb = V.pix_base;
+    float T = 0.f, dC0 = 0.f, dC1 = 0.f, dC2 = 0.f, last_dT = 0.f, dD = 0.f;
+    float dN0 = 0.f, dN1 = 0.f, dN2 = 0.f, gt0 = 0.f, gt1 = 0.f, gt2 = 0.f;
+    uint32_t last = 0;
+    if (inside) {
+      T = a.T[pb + pid];
+      last = a.n_contrib[pb + pid];
+      dC0 = a.dL_dcolor[pb * 3 + 0 * HW + pid];
+      dC1 = a.dL_dcolor[pb * 3 + 1 * HW + pid];
+      dC2 = a.dL_dcolor[pb * 3 + 2 * HW + pid];
+      last_dT = a.dL_dT[pb + pid] + (V.bg[0] * dC0 + V.bg[1] * dC1 + V.bg[2] * dC2);
+      if (kDepth) dD = a.dL_ddepth[pb * 4 + pid] * r.norm_inv;
+      if (kNormal) {
+        dN0 = a.dL_dnormal[pb * 3 + 0 * HW + pid];
+        dN1 = a.dL_dnormal[pb * 3 + 1 * HW + pid];
+        dN2 = a.dL_dnormal[pb * 3 + 2 * HW + pid];
+      }
+      if (kConcen) {
+        gt0 = a.gt_color[pb * 3 + 0 * HW + pid];
+        gt1 = a.gt_color[pb * 3 + 1 * HW + pid];
+        gt2 = a.gt_color[pb * 3 + 2 * HW + pid];
+      }
+    }
+    const float wR = kConcen ? a.lambda_R_concen / (float)(W * H) : 0.f;
+    s_rd[w][lane] = make_float4(r.dx, r.dy, r.dz, dD);
+    s_dc[w][lane] = make_float4(dC0, dC1, dC2, __uint_as_float(last));
+    if (kNormal) s_dn[w][lane] = make_float4(dN0, dN1, dN2, 0.f);
+    if (kConcen) s_gt[w][lane] = make_float4(gt0, gt1, gt2, 0.f);
+    uint64_t oct_mask[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) oct_mask[q] = __ballot(inside && r.octant == (uint32_t)q);
+
+    for (int base = 0; base < n; base += kBwdStage) {
+      __syncthreads();
+      const int cnt = min(kBwdStage, n - base);
+      if (t < cnt) {
+        const uint32_t e = end - 1 - (uint32_t)(base + t);
+        stage_entry_b(a, e, vi, kNormal, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t);
+        s_used[t] = 0u;
+      }
+      for (int k = t; k < kBwdStage * 12; k += 256) (&s_acc[0][0])[k] = 0.f;
+      __syncthreads();
+      // Slot j holds list position rel_j = n - 1 - base - j (relative to the tile start).
+      const uint32_t rel_low = (uint32_t)(n - base - cnt);
+      const uint64_t alive = __ballot(inside && last > rel_low);
+
+      if (alive != 0ull) {
+        uint64_t m2[2];
+        uint32_t c2[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int j = 2 * lane + u;
+          uint64_t m = 0ull;
+          if (j < cnt) {
+            const uint2 bb = s_bb[j];
+            const uint32_t q = __float_as_uint(s_rgb[j].w) >> 29;
+            uint64_t om = oct_mask[0];
+#pragma unroll
+            for (int qq = 1; qq < 8; ++qq) om = (q == (uint32_t)qq) ? oct_mask[qq] : om;
+            m = bbox_block_mask_b(bb.x, bb.y, qx, qy) & alive & om;
+          }
+          m2[u] = m;
+          c2[u] = (uint32_t)__popcll(m);
+        }
+        const uint32_t incl = wave_incl_scan_b(c2[0] + c2[1], lane);
+        const uint32_t excl = incl - c2[0] - c2[1];
+        s_mask[w][2 * lane] = m2[0];
+        s_mask[w][2 * lane + 1] = m2[1];
+        s_pref[w][2 * lane] = excl;
+        s_pref[w][2 * lane + 1] = excl + c2[0];
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        if (lane == 0) s_pref[w][kBwdStage] = total;
+        wave_lds_sync_b();
+
+        int j0 = 0;
+        while (total != 0 && j0 < cnt) {
+          const uint32_t p0 = s_pref[w][j0];
+          int lo = j0 + 1, hi = cnt;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_pref[w][mid] - p0 <= (uint32_t)kBwdCap) lo = mid; else hi = mid - 1;
+          }
+          const int j1 = lo;
+          const uint32_t np = s_pref[w][j1] - p0;
+
+          // 2. pair alphas
+          for (uint32_t pp = lane; pp < np; pp += 64) {
+            const uint32_t p = p0 + pp;
+            int el = j0, eh = j1 - 1;
+            while (el < eh) {
+              const int mid = (el + eh + 1) >> 1;
+              if (s_pref[w][mid] <= p) el = mid; else eh = mid - 1;
+            }
+            const int j = el;
+            const int bit = select_set_bit_b(s_mask[w][j], (int)(p - s_pref[w][j]));
+            const uint32_t rel = (uint32_t)(n - 1 - base - j);
+            float alpha = 0.f, dval = 0.f;
+            if (rel < __float_as_uint(s_dc[w][bit].w)) {
+              const Ray pr = lane_ray(r, s_rd[w][bit]);
+              const float4 ch = s_ch[j];
+              float ta, tb;
+              ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
+              if (!(ta > tb)) {
+                const float4 ml = s_ml[j];
+                const float4 g0 = s_g0[j], g1 = s_g1[j];
+                const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+                Segment seg;
+                eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
+                alpha = seg.alpha;
+                if (kDepth) {
+                  float sub[3];
+                  sub_alphas(seg, sub);
+                  dval = depth_value(sub, ta, tb);
+                }
+              }
+            }
+            s_pa[w][pp] = alpha;
+            if (kDepth) s_pd[w][pp] = dval;
+            s_pt[w][pp] = -1.f;
+          }
+          wave_lds_sync_b();
+
+          // 3. sequential reverse walk per pixel
+          for (int j = j0; j < j1; ++j) {
+            const uint64_t m = s_mask[w][j];
+            if (!((m >> lane) & 1ull)) continue;
+            const uint32_t rel = (uint32_t)(n - 1 - base - j);
+            if (rel >= last) continue;
+            const uint32_t pp = s_pref[w][j] - p0 + (uint32_t)__popcll(m & lt_mask);
+            const float alpha = s_pa[w][pp];
+            if (alpha < kMinAlpha) continue;
+            T = T / (1.f - alpha);
+            const float4 rgb = s_rgb[j];
+            float dptw = dC0 * rgb.x + dC1 * rgb.y + dC2 * rgb.z;
+            if (kNormal) {
+              const float4 nn = s_nrm[j];
+              dptw += dN0 * nn.x + dN1 * nn.y + dN2 * nn.z;
+            }
+            const float dLda = T * (dptw - last_dT);
+            last_dT += alpha * (dptw - last_dT);
+            if (kDepth) last_dT += dD * s_pd[w][pp];
+            s_pl[w][pp] = dLda;
+            s_pt[w][pp] = T;
+          }
+          wave_lds_sync_b();
+
+          // 4. per-pair derivatives, accumulated per entry in LDS
+          for (uint32_t pp = lane; pp < np; pp += 64) {
+            const float Tp = s_pt[w][pp];
+            if (Tp < 0.f) continue;
+            const uint32_t p = p0 + pp;
+            int el = j0, eh = j1 - 1;
+            while (el < eh) {
+              const int mid = (el + eh + 1) >> 1;
+              if (s_pref[w][mid] <= p) el = mid; else eh = mid - 1;
+            }
+            const int j = el;
+            const int bit = select_set_bit_b(s_mask[w][j], (int)(p - s_pref[w][j]));
+            const float4 rdd = s_rd[w][bit];
+            const Ray pr = lane_ray(r, rdd);
+            const float4 ch = s_ch[j];
+            float ta, tb;
+            ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
+            const float4 ml = s_ml[j];
+            const float4 g0 = s_g0[j], g1 = s_g1[j];
+            const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+            Segment seg;
+            eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
+            const float alpha = seg.alpha;
+            const float dLda = s_pl[w][pp];
+            const float ptw = alpha * Tp;
+            float gv[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) gv[k] = 0.f;
+            const float p0v = seg.phi[0], p3v = seg.phi[3];
+            const float rden = 1.f / (p0v + kPhiEps);
+            const float q0 = p0v * (1.f - p0v), q3 = p3v * (1.f - p3v);
+            float c[4];
+            c[0] = dLda * s * q0 * (1.f - alpha) * rden;
+            c[1] = 0.f;
+            c[2] = 0.f;
+            c[3] = -dLda * s * q3 * rden;
+            ds_acc += dLda * (q0 * seg.sdf[0] * (1.f - alpha) - q3 * seg.sdf[3]) * rden;
+            if (kDepth) {
+              float sub[3];
+              sub_alphas(seg, sub);
+              const float step = kThird * (tb - ta);
+              const float t0 = ta + 0.5f * step, t1 = ta + 1.5f * step, t2 = ta + 2.5f * step;
+              const float a0 = sub[0], a1 = sub[1], a2 = sub[2];
+              const float kk = rdd.w * Tp;
+              const float gk[3] = {kk * (t0 + a1 * a2 * t2 - a1 * t1 - a2 * t2),
+                                   kk * (t1 + a0 * a2 * t2 - a0 * t1 - a2 * t2),
+                                   kk * (t2 + a0 * a1 * t2 - a0 * t2 - a1 * t2)};
+#pragma unroll
+              for (int mm = 0; mm < 3; ++mm) {
+                const float pk = seg.phi[mm], pk1 = seg.phi[mm + 1];
+                const float dk = 1.f / (pk + kPhiEps);
+                if ((pk - pk1) * dk > 0.f) {
+                  c[mm] += gk[mm] * pk1 * dk * dk * s * pk * (1.f - pk);
+                  c[mm + 1] += -gk[mm] * dk * s * pk1 * (1.f - pk1);
+                }
+              }
+            }
+#pragma unroll
+            for (int mm = 0; mm < 4; ++mm) {
+              const float fm = (float)mm;
+              float wts[8];
+              trilinear_weights(seg.qx + seg.sx * fm, seg.qy + seg.sy * fm, seg.qz + seg.sz * fm, wts);
+#pragma unroll
+              for (int i = 0; i < 8; ++i) gv[i] += c[mm] * wts[i];
+            }
+            const float4 dcv = s_dc[w][bit];
+            if (kNormal) {
+              const float4 nn = s_nrm[j];
+              const float4 dnv = s_dn[w][bit];
+              const float sx = ptw * dnv.x, sy = ptw * dnv.y, sz = ptw * dnv.z;
+              const float pr2 = sx * nn.x + sy * nn.y + sz * nn.z;
+              const float lx = nn.w * (sx - pr2 * nn.x);
+              const float ly = nn.w * (sy - pr2 * nn.y);
+              const float lz = nn.w * (sz - pr2 * nn.z);
+#pragma unroll
+              for (int i = 0; i < 8; ++i)
+                gv[i] += ((i & 4) ? lx : -lx) + ((i & 2) ? ly : -ly) + ((i & 1) ? lz : -lz);
+            }
+            gv[8] = ptw * dcv.x;
+            gv[9] = ptw * dcv.y;
+            gv[10] = ptw * dcv.z;
+            if (kConcen) {
+              const float4 gtv = s_gt[w][bit];
+              const float4 rgb = s_rgb[j];
+              gv[8] += wR * ptw * 2.f * (rgb.x - gtv.x);
+              gv[9] += wR * ptw * 2.f * (rgb.y - gtv.y);
+              gv[10] += wR * ptw * 2.f * (rgb.z - gtv.z);
+            }
+            gv[11] = fabsf(dLda * alpha);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) atomicAdd(&s_acc[j][k], gv[k]);
+            s_used[j] = 1u;
+          }
+          wave_lds_sync_b();
+          j0 = j1;
+        }
+      }
+      __syncthreads();
+      // Flush the batch's per-entry sums: one global atomic per (entry, slot) that received any.
+      for (int k = t; k < cnt * 12; k += 256) {
+        const int j = k / 12, slot = k - 12 * j;
+        if (!s_used[j]) continue;
+        const float v = s_acc[j][slot];
+        const uint32_t vox = __float_as_uint(s_rgb[j].w) & kVoxMask;
+        float* dst = a.dL_dvox + (long long)vox * 12 + slot;
+        if (a.dL_drgb_views && slot >= 8 && slot < 11)
+          dst = a.dL_drgb_views + ((long long)vi * a.P + vox) * 3 + (slot - 8);
+        atomicAdd(dst, v);
+      }
+    }
+  }
+  const float wsum = wave_sum_uniform(ds_acc);
+  if (lane == 0) s_red[w] = wsum;
+  __syncthreads();
+  if (t == 0) atomicAdd(a.dL_ds, (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]));
+}
+
 template <bool D, bool N>
 static hipError_t launch_bwd_c(const BwdArgs& a, hipStream_t st, unsigned grid) {
-  if (a.lambda_R_concen > 0.f && a.gt_color) render_bwd_kernel<D, N, true><<<grid, 256, 0, st>>>(a);
-  else render_bwd_kernel<D, N, false><<<grid, 256, 0, st>>>(a);
+  const bool concen = a.lambda_R_concen > 0.f && a.gt_color;
+  if (a.variant == 1) {
+    if (concen) render_bwd_kernel<D, N, true><<<grid, 256, 0, st>>>(a);
+    else render_bwd_kernel<D, N, false><<<grid, 256, 0, st>>>(a);
+  } else {
+    if (concen) render_bwd_pairs_kernel<D, N, true><<<grid, 256, 0, st>>>(a);
+    else render_bwd_pairs_kernel<D, N, false><<<grid, 256, 0, st>>>(a);
+  }
   return hipGetLastError();
 }
 

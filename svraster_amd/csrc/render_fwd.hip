@@ -58,6 +58,57 @@ __device__ inline void stage_entry(const uint32_t* __restrict__ vals, uint32_t e
   }
 }
 
+// Median depth refined with 16 sub-steps inside the median voxel (forward.cu:450-521).
+__device__ inline float median_depth(const FwdArgs& a, const Ray& r, int v, float dmed_T, float s) {
+  const float cx = a.centers[3 * v], cy = a.centers[3 * v + 1], cz = a.centers[3 * v + 2];
+  const float l = a.lengths[v];
+  float gg[8];
+  for (int k = 0; k < 8; ++k) gg[k] = a.geos[8 * v + k];
+  float ta, tb;
+  ray_box(r, cx, cy, cz, 0.5f * l, ta, tb);
+  const float l_inv = 1.f / l;
+  const float step = (tb - ta) * (1.f / 16.f);
+  float Dmed = ta - 0.5f * step;
+  float qx = (r.ox + ta * r.dx - (cx - 0.5f * l)) * l_inv;
+  float qy = (r.oy + ta * r.dy - (cy - 0.5f * l)) * l_inv;
+  float qz = (r.oz + ta * r.dz - (cz - 0.5f * l)) * l_inv;
+  const float sx = step * r.dx * l_inv, sy = step * r.dy * l_inv, sz = step * r.dz * l_inv;
+  float phi_prev = fast_sigmoid(s * trilerp(gg, qx, qy, qz));
+  float Tm = dmed_T;
+  for (int k = 0; k < 16 && Tm > 0.5f; ++k) {
+    Dmed += step;
+    qx += sx; qy += sy; qz += sz;
+    const float phi = fast_sigmoid(s * trilerp(gg, qx, qy, qz));
+    Tm *= 1.f - (phi_prev - phi) / (phi_prev + kPhiEps);
+    phi_prev = phi;
+  }
+  return Dmed;
+}
+
+template <bool kDepth, bool kNormal>
+__device__ inline void write_pixel(const FwdArgs& a, const ViewDesc& V, const Ray& r, bool inside, int px, int py,
+                                   float T, float C0, float C1, float C2, float D, float Dmed, float N0,
+                                   float N1, float N2, uint32_t last) {
+  if (!inside) return;
+  const long long HW = (long long)V.W * V.H;
+  const long long pid = (long long)py * V.W + px;
+  const long long pb = V.pix_base;
+  a.color[pb * 3 + 0 * HW + pid] = C0 + T * V.bg[0];
+  a.color[pb * 3 + 1 * HW + pid] = C1 + T * V.bg[1];
+  a.color[pb * 3 + 2 * HW + pid] = C2 + T * V.bg[2];
+  a.T[pb + pid] = T;
+  a.n_contrib[pb + pid] = last;
+  if (kDepth) {
+    a.depth[pb * 4 + 0 * HW + pid] = D * r.norm_inv;
+    a.depth[pb * 4 + 2 * HW + pid] = Dmed * r.norm_inv;
+  }
+  if (kNormal) {
+    a.normal[pb * 3 + 0 * HW + pid] = N0;
+    a.normal[pb * 3 + 1 * HW + pid] = N1;
+    a.normal[pb * 3 + 2 * HW + pid] = N2;
+  }
+}
+
 template <bool kDepth, bool kNormal, bool kMaxW>
 __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
   __shared__ float4 s_ch[256], s_ml[256], s_g0[256], s_g1[256], s_rgb[256];
@@ -143,53 +194,245 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
     }
 
     float Dmed = 0.f;
-    if (kDepth && inside && dmed_vox >= 0) {
-      // Median depth refined with 16 sub-steps inside the median voxel (forward.cu:450-521).
-      const int v = dmed_vox;
-      const float cx = a.centers[3 * v], cy = a.centers[3 * v + 1], cz = a.centers[3 * v + 2];
-      const float l = a.lengths[v];
-      float gg[8];
-      for (int k = 0; k < 8; ++k) gg[k] = a.geos[8 * v + k];
-      float ta, tb;
-      ray_box(r, cx, cy, cz, 0.5f * l, ta, tb);
-      const float l_inv = 1.f / l;
-      const float step = (tb - ta) * (1.f / 16.f);
-      Dmed = ta - 0.5f * step;
-      float qx = (r.ox + ta * r.dx - (cx - 0.5f * l)) * l_inv;
-      float qy = (r.oy + ta * r.dy - (cy - 0.5f * l)) * l_inv;
-      float qz = (r.oz + ta * r.dz - (cz - 0.5f * l)) * l_inv;
-      const float sx = step * r.dx * l_inv, sy = step * r.dy * l_inv, sz = step * r.dz * l_inv;
-      float phi_prev = fast_sigmoid(s * trilerp(gg, qx, qy, qz));
-      float Tm = dmed_T;
-      for (int k = 0; k < 16 && Tm > 0.5f; ++k) {
-        Dmed += step;
-        qx += sx; qy += sy; qz += sz;
-        const float phi = fast_sigmoid(s * trilerp(gg, qx, qy, qz));
-        Tm *= 1.f - (phi_prev - phi) / (phi_prev + kPhiEps);
-        phi_prev = phi;
+    if (kDepth && inside && dmed_vox >= 0) Dmed = median_depth(a, r, dmed_vox, dmed_T, s);
+    write_pixel<kDepth, kNormal>(a, V, r, inside, px, py, T, C0, C1, C2, D, Dmed, N0, N1, N2, last);
+    // tile_last = max over the tile's pixels of (range start + contributors) (forward.cu:561).
+    const uint32_t tl = wave_max_u32(rg.x + (inside ? last : 0u));
+    if (lane == 0) s_red[w] = tl;
+    __syncthreads();
+    if (t == 0) a.tile_last[g] = max(max(s_red[0], s_red[1]), max(s_red[2], s_red[3]));
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Pair-compacted variant.  Sub-pixel voxels cover a handful of a wave's 64 pixels, so a loop in
+// which each lane tests every entry leaves most lanes idle during the expensive SDF segment.
+// Here each wave, per staged batch:
+//   1. builds for every entry the 64-bit mask of its pixels that lie in the entry's bbox, match
+//      its octant and are still unsaturated, and prefix-sums the popcounts;
+//   2. evaluates the (entry, pixel) pairs 64 at a time, one pair per lane, and stores alpha (and
+//      the depth value) per pair in LDS;
+//   3. walks the entries in order per pixel and composites from the stored pair results.
+// Compositing order and arithmetic per pixel are unchanged, so the output equals the
+// one-entry-at-a-time kernel bit for bit.
+// ------------------------------------------------------------------------------------------
+constexpr int kPairStage = 128;   // entries staged per workgroup round
+constexpr int kPairCap = 256;     // pair results held per wave at once
+
+__device__ inline int select_set_bit(uint64_t m, int k) {
+  int pos = 0;
+  const int c32 = __popc((uint32_t)m);
+  if (k >= c32) { k -= c32; m >>= 32; pos += 32; }
+  uint32_t x = (uint32_t)m;
+  int c = __popc(x & 0xFFFFu);
+  if (k >= c) { k -= c; x >>= 16; pos += 16; }
+  c = __popc(x & 0xFFu);
+  if (k >= c) { k -= c; x >>= 8; pos += 8; }
+  c = __popc(x & 0xFu);
+  if (k >= c) { k -= c; x >>= 4; pos += 4; }
+  c = __popc(x & 0x3u);
+  if (k >= c) { k -= c; x >>= 2; pos += 2; }
+  if (k >= (int)(x & 1u)) pos += 1;
+  return pos;
+}
+
+// Pixels of the wave's 8x8 block (bit = 8 * row + column) inside a packed bbox.
+__device__ inline uint64_t bbox_block_mask(uint32_t b0, uint32_t b1, int qx, int qy) {
+  const int x0 = max((int)(b0 >> 16) - qx, 0), x1 = min((int)(b1 >> 16) - qx, 7);
+  const int y0 = max((int)(b0 & 0xFFFFu) - qy, 0), y1 = min((int)(b1 & 0xFFFFu) - qy, 7);
+  if (x0 > x1 || y0 > y1) return 0ull;
+  const uint64_t cols = (uint64_t)(((1u << (x1 + 1)) - 1u) & ~((1u << x0) - 1u));
+  const uint64_t hi = (y1 == 7) ? ~0ull : ((1ull << (8 * (y1 + 1))) - 1ull);
+  const uint64_t lo = (1ull << (8 * y0)) - 1ull;
+  const uint64_t rows = 0x0101010101010101ull & hi & ~lo;
+  return cols * rows;
+}
+
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Inclusive wave64 scan of u32.
+__device__ inline uint32_t wave_incl_scan_u32(uint32_t x, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = (uint32_t)__shfl_up((int)x, o, 64);
+    if (lane >= o) x += t;
+  }
+  return x;
+}
+
+template <bool kDepth, bool kNormal, bool kMaxW>
+__global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
+  __shared__ float4 s_ch[kPairStage], s_ml[kPairStage], s_g0[kPairStage], s_g1[kPairStage], s_rgb[kPairStage];
+  __shared__ float4 s_nrm[kNormal ? kPairStage : 1];
+  __shared__ uint2 s_bb[kPairStage];
+  __shared__ uint64_t s_mask[4][kPairStage];
+  __shared__ uint32_t s_pref[4][kPairStage + 1];
+  __shared__ float4 s_rd[4][64];          // direction xyz
+  __shared__ float4 s_ri[4][64];          // 1/direction xyz
+  __shared__ float s_pa[4][kPairCap];     // pair alpha (0 when the ray misses the box)
+  __shared__ float s_pd[4][kDepth ? kPairCap : 1];
+  __shared__ uint32_t s_red[4];
+  __shared__ int s_tile;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const float s = *a.s_val;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+
+  for (int iter = 0;; ++iter) {
+    int g;
+    if (a.work_counter) {
+      if (t == 0) s_tile = (int)atomicAdd(a.work_counter, 1u);
+      __syncthreads();
+      g = s_tile;
+      __syncthreads();
+      if (g >= a.total_tiles) return;
+    } else {
+      if (iter > 0) return;
+      g = blockIdx.x;
+    }
+    const int vi = find_view(a.views, a.n_views, g);
+    const ViewDesc& V = a.views[vi];
+    const int W = V.W, H = V.H;
+    const int lt = g - V.tile_base;
+    const int tx = lt % V.tiles_x, ty = lt / V.tiles_x;
+    const int qx = tx * kTile + (w & 1) * 8, qy = ty * kTile + (w >> 1) * 8;
+    const int px = qx + (lane & 7), py = qy + (lane >> 3);
+    const bool inside = px < W && py < H;
+    const Ray r = pixel_ray(V, px, py);
+    s_rd[w][lane] = make_float4(r.dx, r.dy, r.dz, 0.f);
+    s_ri[w][lane] = make_float4(r.ix, r.iy, r.iz, 0.f);
+    uint64_t oct_mask[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) oct_mask[q] = __ballot(inside && r.octant == (uint32_t)q);
+    const uint2 rg = a.ranges[g];
+    const int n = (int)(rg.y - rg.x);
+
+    float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f, D = 0.f, N0 = 0.f, N1 = 0.f, N2 = 0.f;
+    uint32_t last = 0;
+    int dmed_vox = -1;
+    float dmed_T = 0.f;
+    bool done = !inside;
+
+    for (int base = 0; base < n; base += kPairStage) {
+      if (__syncthreads_count(done) == 256) break;
+      if (t < kPairStage && base + t < n)
+        stage_entry(a.vals, rg.x + base + t, vi, a.P, a.bboxes, a.centers, a.lengths, a.geos,
+                    a.rgbs + (long long)vi * a.rgb_view_stride, kNormal, s_ch, s_ml, s_g0, s_g1, s_rgb,
+                    s_nrm, s_bb, t);
+      __syncthreads();
+      const int cnt = min(kPairStage, n - base);
+      const uint64_t alive = __ballot(!done);
+      if (alive == 0) continue;
+
+      // 1. masks and prefix of pair counts (lane l owns entries 2l and 2l+1).
+      uint64_t m2[2];
+      uint32_t c2[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int j = 2 * lane + u;
+        uint64_t m = 0ull;
+        if (j < cnt) {
+          const uint2 bb = s_bb[j];
+          const uint32_t q = __float_as_uint(s_rgb[j].w) >> 29;
+          uint64_t om = oct_mask[0];
+#pragma unroll
+          for (int qq = 1; qq < 8; ++qq) om = (q == (uint32_t)qq) ? oct_mask[qq] : om;
+          m = bbox_block_mask(bb.x, bb.y, qx, qy) & alive & om;
+        }
+        m2[u] = m;
+        c2[u] = (uint32_t)__popcll(m);
+      }
+      const uint32_t incl = wave_incl_scan_u32(c2[0] + c2[1], lane);
+      const uint32_t excl = incl - c2[0] - c2[1];
+      s_mask[w][2 * lane] = m2[0];
+      s_mask[w][2 * lane + 1] = m2[1];
+      s_pref[w][2 * lane] = excl;
+      s_pref[w][2 * lane + 1] = excl + c2[0];
+      const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+      if (lane == 0) s_pref[w][kPairStage] = total;
+      wave_lds_sync();
+      if (total == 0) continue;
+
+      // 2. + 3. over windows of whole entries holding at most kPairCap pairs.
+      int j0 = 0;
+      while (j0 < cnt) {
+        const uint32_t p0 = s_pref[w][j0];
+        int lo = j0 + 1, hi = cnt;           // largest j1 in (j0, cnt] with pref[j1] - p0 <= cap
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_pref[w][mid] - p0 <= (uint32_t)kPairCap) lo = mid; else hi = mid - 1;
+        }
+        const int j1 = lo;
+        const uint32_t np = s_pref[w][j1] - p0;
+        for (uint32_t pp = lane; pp < np; pp += 64) {
+          const uint32_t p = p0 + pp;
+          int el = j0, eh = j1 - 1;              // last entry with pref <= p
+          while (el < eh) {
+            const int mid = (el + eh + 1) >> 1;
+            if (s_pref[w][mid] <= p) el = mid; else eh = mid - 1;
+          }
+          const int j = el;
+          const int bit = select_set_bit(s_mask[w][j], (int)(p - s_pref[w][j]));
+          Ray pr = r;
+          const float4 d = s_rd[w][bit];
+          const float4 iv = s_ri[w][bit];
+          pr.dx = d.x; pr.dy = d.y; pr.dz = d.z;
+          pr.ix = iv.x; pr.iy = iv.y; pr.iz = iv.z;
+          const float4 ch = s_ch[j];
+          float ta, tb;
+          ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
+          float alpha = 0.f, dval = 0.f;
+          if (!(ta > tb)) {
+            const float4 ml = s_ml[j];
+            const float4 g0 = s_g0[j], g1 = s_g1[j];
+            const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+            Segment seg;
+            eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
+            alpha = seg.alpha;
+            if (kDepth) {
+              float sub[3];
+              sub_alphas(seg, sub);
+              dval = depth_value(sub, ta, tb);
+            }
+          }
+          s_pa[w][pp] = alpha;
+          if (kDepth) s_pd[w][pp] = dval;
+        }
+        wave_lds_sync();
+        for (int j = j0; j < j1; ++j) {
+          const uint64_t m = s_mask[w][j];
+          if (!((m >> lane) & 1ull) || done) continue;
+          const uint32_t pp = s_pref[w][j] - p0 + (uint32_t)__popcll(m & lt_mask);
+          const float alpha = s_pa[w][pp];
+          if (alpha < kMinAlpha) continue;
+          const float4 rgb = s_rgb[j];
+          const float wgt = T * alpha;
+          C0 += wgt * rgb.x; C1 += wgt * rgb.y; C2 += wgt * rgb.z;
+          if (kDepth) {
+            D += T * s_pd[w][pp];
+            if (T > 0.5f) { dmed_vox = (int)(__float_as_uint(rgb.w) & kVoxMask); dmed_T = T; }
+          }
+          if (kNormal) {
+            const float4 nn = s_nrm[j];
+            N0 += wgt * nn.x; N1 += wgt * nn.y; N2 += wgt * nn.z;
+          }
+          T *= (1.f - alpha);
+          done = T < kEarlyStopT;
+          last = (uint32_t)(base + j + 1);
+          if (kMaxW)
+            atomicMax(reinterpret_cast<uint32_t*>(a.max_w) + (__float_as_uint(rgb.w) & kVoxMask), __float_as_uint(wgt));
+        }
+        wave_lds_sync();
+        j0 = j1;
       }
     }
 
-    if (inside) {
-      const long long HW = (long long)W * H;
-      const long long pid = (long long)py * W + px;
-      const long long pb = V.pix_base;
-      a.color[pb * 3 + 0 * HW + pid] = C0 + T * V.bg[0];
-      a.color[pb * 3 + 1 * HW + pid] = C1 + T * V.bg[1];
-      a.color[pb * 3 + 2 * HW + pid] = C2 + T * V.bg[2];
-      a.T[pb + pid] = T;
-      a.n_contrib[pb + pid] = last;
-      if (kDepth) {
-        a.depth[pb * 4 + 0 * HW + pid] = D * r.norm_inv;
-        a.depth[pb * 4 + 2 * HW + pid] = Dmed * r.norm_inv;
-      }
-      if (kNormal) {
-        a.normal[pb * 3 + 0 * HW + pid] = N0;
-        a.normal[pb * 3 + 1 * HW + pid] = N1;
-        a.normal[pb * 3 + 2 * HW + pid] = N2;
-      }
-    }
-    // tile_last = max over the tile's pixels of (range start + contributors) (forward.cu:561).
+    float Dmed = 0.f;
+    if (kDepth && inside && dmed_vox >= 0) Dmed = median_depth(a, r, dmed_vox, dmed_T, s);
+    write_pixel<kDepth, kNormal>(a, V, r, inside, px, py, T, C0, C1, C2, D, Dmed, N0, N1, N2, last);
     const uint32_t tl = wave_max_u32(rg.x + (inside ? last : 0u));
     if (lane == 0) s_red[w] = tl;
     __syncthreads();
@@ -200,8 +443,13 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
 
 template <bool D, bool N>
 static hipError_t launch_fwd_mw(const FwdArgs& a, hipStream_t st, unsigned grid) {
-  if (a.max_w) render_fwd_kernel<D, N, true><<<grid, 256, 0, st>>>(a);
-  else render_fwd_kernel<D, N, false><<<grid, 256, 0, st>>>(a);
+  if (a.variant == 1) {
+    if (a.max_w) render_fwd_kernel<D, N, true><<<grid, 256, 0, st>>>(a);
+    else render_fwd_kernel<D, N, false><<<grid, 256, 0, st>>>(a);
+  } else {
+    if (a.max_w) render_fwd_pairs_kernel<D, N, true><<<grid, 256, 0, st>>>(a);
+    else render_fwd_pairs_kernel<D, N, false><<<grid, 256, 0, st>>>(a);
+  }
   return hipGetLastError();
 }
 

@@ -165,6 +165,7 @@ struct FwdArgs {
   float* color; float* depth; float* normal; float* T; float* max_w;
   uint32_t* work_counter;          // null: one workgroup per tile
   int need_depth, need_normal;
+  int variant;                     // 0: pair-compacted (default), 1: one entry at a time
 };
 hipError_t launch_render_forward(const FwdArgs& a, hipStream_t st);
 
@@ -183,6 +184,7 @@ struct BwdArgs {
   float* dL_ds;                    // [1]
   uint32_t* work_counter;
   int need_depth, need_normal;
+  int variant;                     // 0: pair-compacted (default), 1: one entry at a time
 };
 hipError_t launch_render_backward(const BwdArgs& a, hipStream_t st);
 

@@ -133,33 +133,42 @@ __global__ __launch_bounds__(256) void gather_geo_bw_kernel(int n, const int64_t
 }
 
 // Batched views: colour of every (view, visible voxel) pair, direction from that view's camera.
-__global__ __launch_bounds__(256) void sh_views_forward_kernel(int P, const uint32_t* __restrict__ n_dup,
-                                                               const ViewDesc* __restrict__ views, int deg, int M,
+// One voxel per thread, looping over the views, so each voxel's coefficients are read once.
+template <int kDeg>
+__global__ __launch_bounds__(256) void sh_views_forward_kernel(int P, int V, const uint32_t* __restrict__ n_dup,
+                                                               const ViewDesc* __restrict__ views, int M,
                                                                const float* __restrict__ centers,
                                                                const float* __restrict__ sh0,
                                                                const float* __restrict__ shs,
                                                                float* __restrict__ rgbs_views) {
+  constexpr int NB = (kDeg + 1) * (kDeg + 1);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int view = blockIdx.y;
   if (i >= P) return;
-  const long long o = (long long)view * P + i;
-  if (n_dup[o] == 0) return;
-  const float* c2w = views[view].c2w;
-  const float cam[3] = {c2w[3], c2w[7], c2w[11]};
-  float x, y, z;
-  sh_dir(centers, cam, i, x, y, z);
-  float b[16];
-  const int nb = sh_basis(deg, x, y, z, b);
-  float r0 = b[0] * sh0[3 * i], r1 = b[0] * sh0[3 * i + 1], r2 = b[0] * sh0[3 * i + 2];
-  const float* c = shs + (long long)i * (M - 1) * 3;
-  for (int k = 1; k < nb; ++k) {
-    r0 += b[k] * c[3 * (k - 1)];
-    r1 += b[k] * c[3 * (k - 1) + 1];
-    r2 += b[k] * c[3 * (k - 1) + 2];
+  float c[NB][3];
+  c[0][0] = sh0[3 * i]; c[0][1] = sh0[3 * i + 1]; c[0][2] = sh0[3 * i + 2];
+  const float* cs = shs + (long long)i * (M - 1) * 3;
+#pragma unroll
+  for (int k = 1; k < NB; ++k) {
+    c[k][0] = cs[3 * (k - 1)]; c[k][1] = cs[3 * (k - 1) + 1]; c[k][2] = cs[3 * (k - 1) + 2];
   }
-  rgbs_views[3 * o] = fmaxf(r0 + 0.5f, 0.f);
-  rgbs_views[3 * o + 1] = fmaxf(r1 + 0.5f, 0.f);
-  rgbs_views[3 * o + 2] = fmaxf(r2 + 0.5f, 0.f);
+  for (int v = 0; v < V; ++v) {
+    const long long o = (long long)v * P + i;
+    if (n_dup[o] == 0) continue;
+    const float* c2w = views[v].c2w;
+    const float cam[3] = {c2w[3], c2w[7], c2w[11]};
+    float x, y, z;
+    sh_dir(centers, cam, i, x, y, z);
+    float b[16];
+    sh_basis(kDeg, x, y, z, b);
+    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      r0 += b[k] * c[k][0]; r1 += b[k] * c[k][1]; r2 += b[k] * c[k][2];
+    }
+    rgbs_views[3 * o] = fmaxf(r0 + 0.5f, 0.f);
+    rgbs_views[3 * o + 1] = fmaxf(r1 + 0.5f, 0.f);
+    rgbs_views[3 * o + 2] = fmaxf(r2 + 0.5f, 0.f);
+  }
 }
 
 // Transpose summed over the views, one voxel per thread: no atomics, and every output row is
@@ -275,8 +284,13 @@ hipError_t launch_sh_views_forward(int P, int V, const uint32_t* n_dup, const Vi
                                    const float* centers, const float* sh0, const float* shs, float* rgbs_views,
                                    hipStream_t st) {
   if (P <= 0 || V <= 0) return hipSuccess;
-  dim3 grid(blocks_for(P), V);
-  sh_views_forward_kernel<<<grid, 256, 0, st>>>(P, n_dup, views, deg, M, centers, sh0, shs, rgbs_views);
+  const unsigned nb = blocks_for(P);
+  switch (deg) {
+    case 0: sh_views_forward_kernel<0><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, sh0, shs, rgbs_views); break;
+    case 1: sh_views_forward_kernel<1><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, sh0, shs, rgbs_views); break;
+    case 2: sh_views_forward_kernel<2><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, sh0, shs, rgbs_views); break;
+    default: sh_views_forward_kernel<3><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, sh0, shs, rgbs_views); break;
+  }
   return hipGetLastError();
 }
 
