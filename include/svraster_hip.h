@@ -179,6 +179,15 @@ int svr_batch_setup(void* ws, int n_views, int P, long long n_pixels, long long 
 int svr_batch_preprocess(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                          const int64_t* octree_paths, const float* vox_centers, const float* vox_lengths,
                          const uint8_t* is_leaf, void* stream);
+/* out[i] = is_leaf[i] and voxel i can still produce a pair alpha >= 1e-5 (MIN_ALPHA,
+ * config.h:19) for the corner SDFs `geos` [P,8] and s: a conservative bound, so a voxel with
+ * out[i] = 0 contributes to no pixel (forward.cu:303-305 skips every such pair).  Passing `out`
+ * as svr_batch_preprocess's is_leaf leaves those voxels out of binning with unchanged images and
+ * gradients.  cam_scale >= max |camera origin coordinate| over the views rendered.  No reference
+ * counterpart (the reference bins every visible voxel). */
+int svr_voxel_activity(int P, const uint8_t* is_leaf, const float* geos, const float* s_val,
+                       const float* vox_centers, const float* vox_lengths, float cam_scale, uint8_t* out,
+                       void* stream);
 /* SH colour of every (view, visible voxel) pair into rgbs_views. */
 int svr_batch_sh_forward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                          const float* vox_centers, const float* sh0, const float* shs, int degree, int M,

@@ -53,6 +53,7 @@ _SIGS = {
     "svr_batch_workspace_bytes": (SZ, [I, I, LL, LL, LL]),
     "svr_batch_setup": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, P, P, P, P, P]),
     "svr_batch_preprocess": (I, [P, I, I, LL, LL, LL, P, P, P, P, P]),
+    "svr_voxel_activity": (I, [I, P, P, P, P, P, F, P, P]),
     "svr_batch_sh_forward": (I, [P, I, I, LL, LL, LL, P, P, P, I, I, P, P]),
     "svr_batch_rasterize": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, I, I, I, P, P, P, P, P]),
     "svr_batch_backward": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, P, P, P, P, I, I, P, P, P, P]),

@@ -35,7 +35,7 @@ class BatchedRasterizer:
 
     def __init__(self, octpath, octlevel, vox_center, vox_size, is_leaf, vox_key, cameras: List,
                  background: torch.Tensor, sh_degree: int, n_sh_coeffs: int = 16,
-                 key_capacity: Optional[int] = None, need_depth=False, need_normal=False):
+                 key_capacity: Optional[int] = None, need_depth=False, need_normal=False, cull: bool = True):
         self.dev = vox_center.device
         self.lib = _lib.load()
         self.V = len(cameras)
@@ -66,6 +66,11 @@ class BatchedRasterizer:
         self.w2c = torch.stack([c.w2c for c in cameras]).float().contiguous().to(self.dev)
         self.c2w = torch.stack([c.c2w for c in cameras]).float().contiguous().to(self.dev)
         self.bg = background.float().contiguous().to(self.dev)
+        # Voxels whose corner SDFs cannot reach the minimum alpha are left out of binning
+        # (svr_voxel_activity); images and gradients are unchanged, the key count drops.
+        self.cull = bool(cull)
+        self.cam_scale = float(self.c2w[:, :3, 3].abs().max().item()) if self.V else 0.0
+        self.bin_leaf = self.is_leaf.clone()
         if key_capacity is None:
             self.cap = 1 << 20
             self._alloc()
@@ -112,9 +117,10 @@ class BatchedRasterizer:
         return buf[b:b + channels * self.pix[v]].view(channels, self.Hs[v], self.Ws[v])
 
     # -- stages ----------------------------------------------------------------------------
-    def preprocess(self):
+    def preprocess(self, leaf: Optional[torch.Tensor] = None):
+        leaf = self.is_leaf if leaf is None else leaf
         _lib.check(self.lib.svr_batch_preprocess(_p(self.ws), *self._dims(), _p(self.octpath), _p(self.vox_center),
-                                                 _p(self.vox_size), _p(self.is_leaf), self._stream()),
+                                                 _p(self.vox_size), _p(leaf), self._stream()),
                    "svr_batch_preprocess")
 
     def key_count(self) -> int:
@@ -126,9 +132,16 @@ class BatchedRasterizer:
     def forward(self, grid_pts: torch.Tensor, sh0: torch.Tensor, shs: torch.Tensor, s_val: torch.Tensor):
         """Render every view.  Returns the concatenated colour buffer (views [3,H,W] back to back)."""
         st = self._stream()
-        self.preprocess()
         _lib.check(self.lib.svr_gather_triinterp_geo_params(_p(self.vox_key), self.P, None, self.P,
                                                             _p(grid_pts), _p(self.geos), st), "gather")
+        if self.cull:
+            _lib.check(self.lib.svr_voxel_activity(self.P, _p(self.is_leaf), _p(self.geos), _p(s_val),
+                                                   _p(self.vox_center), _p(self.vox_size),
+                                                   ctypes.c_float(self.cam_scale), _p(self.bin_leaf), st),
+                       "svr_voxel_activity")
+            self.preprocess(self.bin_leaf)
+        else:
+            self.preprocess()
         _lib.check(self.lib.svr_batch_sh_forward(_p(self.ws), *self._dims(), _p(self.vox_center), _p(sh0), _p(shs),
                                                  self.deg, self.M, _p(self.rgbs_views), st), "sh_forward")
         self.color.zero_()

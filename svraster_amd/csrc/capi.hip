@@ -511,6 +511,17 @@ int svr_batch_preprocess(void* ws, int n_views, int P, long long n_pixels, long 
   return 0;
 }
 
+int svr_voxel_activity(int P, const uint8_t* is_leaf, const float* geos, const float* s_val,
+                       const float* vox_centers, const float* vox_lengths, float cam_scale, uint8_t* out,
+                       void* stream) {
+  g_err.clear();
+  if (P < 0) return fail(1, "bad voxel count%s %lld", "", P);
+  if (!(cam_scale >= 0.f)) return fail(1, "cam_scale must be >= 0%s %lld", "", 0);
+  SVR_TRY(launch_voxel_activity(P, is_leaf, geos, s_val, vox_centers, vox_lengths, cam_scale, out,
+                                as_stream(stream)));
+  return 0;
+}
+
 int svr_batch_sh_forward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                          const float* vox_centers, const float* sh0, const float* shs, int degree, int M,
                          float* rgbs_views, void* stream) {

@@ -152,6 +152,38 @@ __device__ inline void eval_segment(const Ray& r, float a, float b, float minx, 
   seg.alpha = fmaxf(raw, 0.f);
 }
 
+// Conservative per-voxel test: false only if no ray segment through the voxel can reach
+// alpha >= kMinAlpha, so skipping the voxel leaves every pixel unchanged (the compositors skip
+// such pairs anyway, forward.cu:303-305, backward.cu:428).
+//
+// The main alpha is (phi(v0) - phi(v3)) / (phi(v0) + 1e-6) with phi = sigmoid(s * v) and v0, v3
+// trilinear SDF values at the segment ends.  For v in [lo, hi] the fraction is at most
+// (phi(hi) - phi(lo)) / (phi(hi) + 1e-6) =: B.  [lo, hi] is the corner range widened for
+// (i) trilinear extrapolation when rounding puts a sample slightly outside the voxel (local
+// coordinate error eta <= 2e-6 (|pos| + l) / l, giving at most 8 eta times the range), plus a
+// fixed quarter of the range, and (ii) 1e-6 of the largest |corner| for the rounding of the
+// lerps.  The float evaluation of phi (s * v product, exp, rcp) has relative error below
+// 2.7e-6, which can add at most 5.4e-6 to the computed alpha; B is evaluated in double, so
+// B < 3e-6 guarantees a computed alpha below 8.5e-6 < kMinAlpha.  NaN corners or s keep the
+// voxel.
+__device__ inline bool voxel_may_contribute(const float g[8], float s, float pos_scale, float l) {
+  float gmin = g[0], gmax = g[0], sum = g[0];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) {
+    gmin = fminf(gmin, g[k]);
+    gmax = fmaxf(gmax, g[k]);
+    sum += g[k];
+  }
+  if (__builtin_isnan(sum)) return true;
+  const double R = (double)gmax - (double)gmin;
+  const double eta = 2e-6 * ((double)pos_scale + (double)l) / (double)l;
+  const double sl = R * (0.25 + 8.0 * eta) + 1e-6 * fmax(fabs((double)gmin), fabs((double)gmax));
+  const double ds = (double)s;
+  const double hi = 1.0 / (1.0 + exp(-ds * ((double)gmax + sl)));
+  const double lo = 1.0 / (1.0 + exp(-ds * ((double)gmin - sl)));
+  return !((hi - lo) < 3e-6 * (hi + 1e-6));
+}
+
 // Depth of one segment from its three sub-interval alphas (forward.cu:349-357).
 __device__ inline void sub_alphas(const Segment& seg, float sub[3]) {
 #pragma clang fp contract(off)

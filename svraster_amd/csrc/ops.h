@@ -18,6 +18,9 @@ hipError_t launch_sh_views_forward(int P, int V, const uint32_t* n_dup, const Vi
 hipError_t launch_sh_views_backward(int P, int V, const uint32_t* n_dup, const ViewDesc* views, int deg, int M,
                                     const float* centers, const float* drgb_views, float* dsh0, float* dshs,
                                     hipStream_t st);
+hipError_t launch_voxel_activity(int P, const uint8_t* is_leaf, const float* geos, const float* s_val,
+                                 const float* centers, const float* lengths, float cam_scale, uint8_t* out,
+                                 hipStream_t st);
 hipError_t launch_gather_geo(int n, const int64_t* vox_key, const int64_t* care, const float* grid_pts,
                              float* geos, hipStream_t st);
 hipError_t launch_gather_geo_bw(int n, const int64_t* vox_key, const int64_t* care, const float* dgeos,

@@ -29,10 +29,11 @@ __device__ inline int find_view_b(const ViewDesc* views, int n_views, int g) {
 
 __device__ inline void stage_entry_b(const BwdArgs& a, uint32_t e, int vi, bool need_normal, float4* s_ch,
                                      float4* s_ml, float4* s_g0, float4* s_g1, float4* s_rgb, float4* s_nrm,
-                                     uint2* s_bb, int slot) {
+                                     uint2* s_bb, int slot, bool cull = false, float s = 0.f,
+                                     float cam_scale = 0.f) {
   const uint32_t val = a.vals[e];
   const uint32_t vox = val & kVoxMask;
-  s_bb[slot] = a.bboxes[(long long)vi * a.P + vox];
+  uint2 bb = a.bboxes[(long long)vi * a.P + vox];
   const float cx = a.centers[3 * vox], cy = a.centers[3 * vox + 1], cz = a.centers[3 * vox + 2];
   const float l = a.lengths[vox];
   s_ch[slot] = make_float4(cx, cy, cz, 0.5f * l);
@@ -41,6 +42,13 @@ __device__ inline void stage_entry_b(const BwdArgs& a, uint32_t e, int vi, bool 
   const float4 g1 = reinterpret_cast<const float4*>(a.geos)[2 * vox + 1];
   s_g0[slot] = g0;
   s_g1[slot] = g1;
+  if (cull) {
+    // An entry that cannot reach the minimum alpha gets an empty bbox (no pixel masks).
+    const float g[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+    const float ps = cam_scale + fmaxf(fabsf(cx), fmaxf(fabsf(cy), fabsf(cz)));
+    if (!voxel_may_contribute(g, s, ps, l)) bb = make_uint2(0xFFFFFFFFu, 0u);
+  }
+  s_bb[slot] = bb;
   const float* rgbv = a.rgbs + (long long)vi * a.rgb_view_stride;
   s_rgb[slot] = make_float4(rgbv[3 * vox], rgbv[3 * vox + 1], rgbv[3 * vox + 2], __uint_as_float(val));
   if (need_normal) {
@@ -49,23 +57,6 @@ __device__ inline void stage_entry_b(const BwdArgs& a, uint32_t e, int vi, bool 
     lin_normal(gg, nx, ny, nz, rn);
     s_nrm[slot] = make_float4(nx * rn, ny * rn, nz * rn, rn);
   }
-}
-
-__device__ inline int select_set_bit_b(uint64_t m, int k) {
-  int pos = 0;
-  const int c32 = __popc((uint32_t)m);
-  if (k >= c32) { k -= c32; m >>= 32; pos += 32; }
-  uint32_t x = (uint32_t)m;
-  int c = __popc(x & 0xFFFFu);
-  if (k >= c) { k -= c; x >>= 16; pos += 16; }
-  c = __popc(x & 0xFFu);
-  if (k >= c) { k -= c; x >>= 8; pos += 8; }
-  c = __popc(x & 0xFu);
-  if (k >= c) { k -= c; x >>= 4; pos += 4; }
-  c = __popc(x & 0x3u);
-  if (k >= c) { k -= c; x >>= 2; pos += 2; }
-  if (k >= (int)(x & 1u)) pos += 1;
-  return pos;
 }
 
 __device__ inline uint64_t bbox_block_mask_b(uint32_t b0, uint32_t b1, int qx, int qy) {
@@ -333,19 +324,54 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Pair-compacted variant of the backward.  Per staged batch (entries in back-to-front order):
-//   1. per wave, pixel masks of every entry (bbox, octant, pixels whose contributor bound
-//      reaches the batch) and the prefix of their popcounts;
-//   2. alpha (and depth value) of every (entry, pixel) pair, one pair per lane;
-//   3. the sequential reverse walk per pixel over the stored pair alphas: T <- T / (1 - a),
-//      dL/dalpha, the running dL/dT -- the only sequential part, a few FMAs per pair;
-//   4. per pair again, one pair per lane: the SDF-segment derivatives, accumulated per entry in
-//      LDS by all four waves; after the batch, one global atomic per (entry, gradient slot).
+// Pixel-major pair backward (default).  Per staged batch (entries in back-to-front order), each
+// wave:
+//   1. builds the pixel masks of the batch's entries (bbox, octant; empty for entries that
+//      cannot reach the minimum alpha), transposes them to per-pixel entry sets and drops the
+//      entries at or past each pixel's last contributor; pixel l owns the run
+//      [off_l, off_l + cnt_l) of a pixel-major pair list;
+//   2. evaluates alpha (and the depth value) of every pair, 64 per instruction;
+//   3. walks each pixel's run in back-to-front order: T <- T / (1 - a), dL/dalpha and the
+//      running dL/dT -- the only sequential part, a few FMAs per pair;
+//   4. compacts the contributing pairs and computes their SDF-segment derivatives, 64 per
+//      instruction, accumulated per entry in LDS by all four waves; after the batch, one global
+//      atomic per (entry, gradient slot).
 // The per-voxel global atomics fall from 12 per (pixel, entry) in the reference to 12 per
 // (tile, entry).
 // ------------------------------------------------------------------------------------------
 constexpr int kBwdStage = 128;
-constexpr int kBwdCap = 128;
+constexpr int kBwdWin = 256;
+
+__device__ inline uint64_t transpose64_b(uint64_t x, int lane) {
+  int sh = 32;
+#pragma unroll
+  for (int k = 0; k < 6; ++k, sh >>= 1) {
+    const uint32_t plo = (uint32_t)__shfl_xor((int)(uint32_t)x, sh, 64);
+    const uint32_t phi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), sh, 64);
+    const uint64_t p = ((uint64_t)phi << 32) | plo;
+    const uint64_t m = ~0ull / ((1ull << sh) + 1ull);   // bit positions whose bit `sh` is clear
+    x = (lane & sh) ? ((x & ~m) | ((p & ~m) >> sh)) : ((x & m) | ((p & m) << sh));
+  }
+  return x;
+}
+
+__device__ inline int pop_entry_b(uint64_t& lo, uint64_t& hi) {
+  if (lo) {
+    const int j = __builtin_ctzll(lo);
+    lo &= lo - 1ull;
+    return j;
+  }
+  const int j = 64 + __builtin_ctzll(hi);
+  hi &= hi - 1ull;
+  return j;
+}
+
+__device__ inline uint64_t entry_block_mask_b(const uint2* s_bb, const float4* s_rgb, int j, int qx, int qy,
+                                              const uint64_t* s_om) {
+  const uint2 bb = s_bb[j];
+  const uint32_t q = __float_as_uint(s_rgb[j].w) >> 29;
+  return bbox_block_mask_b(bb.x, bb.y, qx, qy) & s_om[q];
+}
 
 template <bool kDepth, bool kNormal, bool kConcen>
 __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
@@ -354,21 +380,22 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
   __shared__ uint2 s_bb[kBwdStage];
   __shared__ float s_acc[kBwdStage][12];
   __shared__ uint32_t s_used[kBwdStage];
-  __shared__ uint64_t s_mask[4][kBwdStage];
-  __shared__ uint32_t s_pref[4][kBwdStage + 1];
+  __shared__ uint64_t s_om[4][8];          // pixels of the block whose ray has octant q
   __shared__ float4 s_rd[4][64];           // ray direction xyz, dL/ddepth
-  __shared__ float4 s_dc[4][64];           // dL/dcolor xyz, contributor bound (bits)
+  __shared__ float4 s_ri[4][64];           // 1/direction xyz
+  __shared__ float4 s_dc[4][64];           // dL/dcolor xyz
   __shared__ float4 s_dn[4][kNormal ? 64 : 1];
   __shared__ float4 s_gt[4][kConcen ? 64 : 1];
-  __shared__ float s_pa[4][kBwdCap];       // pair alpha (0 if skipped)
-  __shared__ float s_pd[4][kDepth ? kBwdCap : 1];
-  __shared__ float s_pl[4][kBwdCap];       // dL/dalpha
-  __shared__ float s_pt[4][kBwdCap];       // T before the entry (negative: pair does not contribute)
+  __shared__ uint16_t s_desc[4][kBwdWin];  // entry slot | pixel << 7
+  __shared__ uint16_t s_cidx[4][kBwdWin];  // contributing pairs of the window
+  __shared__ float s_pa[4][kBwdWin];       // pair alpha (0 if the ray misses the box)
+  __shared__ float s_pd[4][kDepth ? kBwdWin : 1];
+  __shared__ float s_pl[4][kBwdWin];       // dL/dalpha
+  __shared__ float s_pt[4][kBwdWin];       // T before the entry (negative: pair does not contribute)
   __shared__ float s_red[4];
   __shared__ int s_tile;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const float s = *a.s_val;
-  const uint64_t lt_mask = (1ull << lane) - 1ull;
   float ds_acc = 0.f;
 
   for (int iter = 0;; ++iter) {
@@ -392,6 +419,7 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
     const int px = qx + (lane & 7), py = qy + (lane >> 3);
     const bool inside = px < W && py < H;
     const Ray r = pixel_ray(V, px, py);
+    const float cam_scale = fmaxf(fabsf(r.ox), fmaxf(fabsf(r.oy), fabsf(r.oz)));
     const uint32_t start = a.ranges[g].x;
     const uint32_t end = a.tile_last[g];
     const int n = (int)(end - start);
@@ -423,112 +451,104 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
     }
     const float wR = kConcen ? a.lambda_R_concen / (float)(W * H) : 0.f;
     s_rd[w][lane] = make_float4(r.dx, r.dy, r.dz, dD);
-    s_dc[w][lane] = make_float4(dC0, dC1, dC2, __uint_as_float(last));
+    s_ri[w][lane] = make_float4(r.ix, r.iy, r.iz, 0.f);
+    s_dc[w][lane] = make_float4(dC0, dC1, dC2, 0.f);
     if (kNormal) s_dn[w][lane] = make_float4(dN0, dN1, dN2, 0.f);
     if (kConcen) s_gt[w][lane] = make_float4(gt0, gt1, gt2, 0.f);
-    uint64_t oct_mask[8];
+    {
+      uint64_t om = 0ull;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) oct_mask[q] = __ballot(inside && r.octant == (uint32_t)q);
+      for (int q = 0; q < 8; ++q) {
+        const uint64_t b = __ballot(inside && r.octant == (uint32_t)q);
+        om = (lane == q) ? b : om;
+      }
+      if (lane < 8) s_om[w][lane] = om;
+    }
 
     for (int base = 0; base < n; base += kBwdStage) {
       __syncthreads();
       const int cnt = min(kBwdStage, n - base);
       if (t < cnt) {
         const uint32_t e = end - 1 - (uint32_t)(base + t);
-        stage_entry_b(a, e, vi, kNormal, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t);
+        stage_entry_b(a, e, vi, kNormal, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t, true, s, cam_scale);
         s_used[t] = 0u;
       }
       for (int k = t; k < kBwdStage * 12; k += 256) (&s_acc[0][0])[k] = 0.f;
       __syncthreads();
-      // Slot j holds list position rel_j = n - 1 - base - j (relative to the tile start).
-      const uint32_t rel_low = (uint32_t)(n - base - cnt);
-      const uint64_t alive = __ballot(inside && last > rel_low);
-
-      if (alive != 0ull) {
-        uint64_t m2[2];
-        uint32_t c2[2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int j = 2 * lane + u;
-          uint64_t m = 0ull;
-          if (j < cnt) {
-            const uint2 bb = s_bb[j];
-            const uint32_t q = __float_as_uint(s_rgb[j].w) >> 29;
-            uint64_t om = oct_mask[0];
-#pragma unroll
-            for (int qq = 1; qq < 8; ++qq) om = (q == (uint32_t)qq) ? oct_mask[qq] : om;
-            m = bbox_block_mask_b(bb.x, bb.y, qx, qy) & alive & om;
-          }
-          m2[u] = m;
-          c2[u] = (uint32_t)__popcll(m);
+      // Slot j holds list position rel_j = n - 1 - base - j (relative to the tile start); a pixel
+      // takes the slots with rel_j < last, i.e. j >= n - base - last.
+      const bool live = inside && last + (uint32_t)base + (uint32_t)cnt > (uint32_t)n;
+      if (__ballot(live) != 0ull) {
+        const uint64_t m_lo = lane < cnt ? entry_block_mask_b(s_bb, s_rgb, lane, qx, qy, s_om[w]) : 0ull;
+        const uint64_t m_hi = lane + 64 < cnt ? entry_block_mask_b(s_bb, s_rgb, lane + 64, qx, qy, s_om[w]) : 0ull;
+        uint64_t e_lo = transpose64_b(m_lo, lane);
+        uint64_t e_hi = transpose64_b(m_hi, lane);
+        if (!live) {
+          e_lo = 0ull;
+          e_hi = 0ull;
+        } else {
+          const int jmin = n - base - (int)last;   // first slot this pixel takes
+          if (jmin >= 128) { e_lo = 0ull; e_hi = 0ull; }
+          else if (jmin >= 64) { e_lo = 0ull; e_hi &= ~0ull << (jmin - 64); }
+          else if (jmin > 0) { e_lo &= ~0ull << jmin; }
         }
-        const uint32_t incl = wave_incl_scan_b(c2[0] + c2[1], lane);
-        const uint32_t excl = incl - c2[0] - c2[1];
-        s_mask[w][2 * lane] = m2[0];
-        s_mask[w][2 * lane + 1] = m2[1];
-        s_pref[w][2 * lane] = excl;
-        s_pref[w][2 * lane + 1] = excl + c2[0];
+        const uint32_t np_lane = (uint32_t)(__popcll(e_lo) + __popcll(e_hi));
+        const uint32_t incl = wave_incl_scan_b(np_lane, lane);
+        const uint32_t off = incl - np_lane, endp = incl;
         const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-        if (lane == 0) s_pref[w][kBwdStage] = total;
-        wave_lds_sync_b();
+        uint32_t cur = off;
 
-        int j0 = 0;
-        while (total != 0 && j0 < cnt) {
-          const uint32_t p0 = s_pref[w][j0];
-          int lo = j0 + 1, hi = cnt;
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_pref[w][mid] - p0 <= (uint32_t)kBwdCap) lo = mid; else hi = mid - 1;
+        for (uint32_t w0 = 0; w0 < total; w0 += kBwdWin) {
+          const uint32_t w1 = min(total, w0 + (uint32_t)kBwdWin);
+          const uint32_t stop = min(endp, w1);
+          while (cur < stop) {
+            const int j = pop_entry_b(e_lo, e_hi);
+            s_desc[w][cur - w0] = (uint16_t)(j | (lane << 7));
+            ++cur;
           }
-          const int j1 = lo;
-          const uint32_t np = s_pref[w][j1] - p0;
+          wave_lds_sync_b();
 
           // 2. pair alphas
+          const uint32_t np = w1 - w0;
           for (uint32_t pp = lane; pp < np; pp += 64) {
-            const uint32_t p = p0 + pp;
-            int el = j0, eh = j1 - 1;
-            while (el < eh) {
-              const int mid = (el + eh + 1) >> 1;
-              if (s_pref[w][mid] <= p) el = mid; else eh = mid - 1;
-            }
-            const int j = el;
-            const int bit = select_set_bit_b(s_mask[w][j], (int)(p - s_pref[w][j]));
-            const uint32_t rel = (uint32_t)(n - 1 - base - j);
+            const uint32_t d = s_desc[w][pp];
+            const int j = (int)(d & 127u), bit = (int)(d >> 7);
+            Ray pr = r;
+            const float4 dv = s_rd[w][bit];
+            const float4 iv = s_ri[w][bit];
+            pr.dx = dv.x; pr.dy = dv.y; pr.dz = dv.z;
+            pr.ix = iv.x; pr.iy = iv.y; pr.iz = iv.z;
+            const float4 ch = s_ch[j];
+            float ta, tb;
+            ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
             float alpha = 0.f, dval = 0.f;
-            if (rel < __float_as_uint(s_dc[w][bit].w)) {
-              const Ray pr = lane_ray(r, s_rd[w][bit]);
-              const float4 ch = s_ch[j];
-              float ta, tb;
-              ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
-              if (!(ta > tb)) {
-                const float4 ml = s_ml[j];
-                const float4 g0 = s_g0[j], g1 = s_g1[j];
-                const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
-                Segment seg;
-                eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
-                alpha = seg.alpha;
-                if (kDepth) {
-                  float sub[3];
-                  sub_alphas(seg, sub);
-                  dval = depth_value(sub, ta, tb);
-                }
+            if (!(ta > tb)) {
+              const float4 ml = s_ml[j];
+              const float4 g0 = s_g0[j], g1 = s_g1[j];
+              const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+              Segment seg;
+              eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
+              alpha = seg.alpha;
+              if (kDepth) {
+                float sub[3];
+                sub_alphas(seg, sub);
+                dval = depth_value(sub, ta, tb);
               }
             }
             s_pa[w][pp] = alpha;
             if (kDepth) s_pd[w][pp] = dval;
-            s_pt[w][pp] = -1.f;
           }
           wave_lds_sync_b();
 
-          // 3. sequential reverse walk per pixel
-          for (int j = j0; j < j1; ++j) {
-            const uint64_t m = s_mask[w][j];
-            if (!((m >> lane) & 1ull)) continue;
-            const uint32_t rel = (uint32_t)(n - 1 - base - j);
-            if (rel >= last) continue;
-            const uint32_t pp = s_pref[w][j] - p0 + (uint32_t)__popcll(m & lt_mask);
-            const float alpha = s_pa[w][pp];
-            if (alpha < kMinAlpha) continue;
+          // 3. sequential reverse walk per pixel over its run
+          const uint32_t r0 = max(off, w0), r1 = min(endp, w1);
+          for (uint32_t i = r0; i < r1; ++i) {
+            const float alpha = s_pa[w][i - w0];
+            if (alpha < kMinAlpha) {
+              s_pt[w][i - w0] = -1.f;
+              continue;
+            }
+            const int j = (int)(s_desc[w][i - w0] & 127u);
             T = T / (1.f - alpha);
             const float4 rgb = s_rgb[j];
             float dptw = dC0 * rgb.x + dC1 * rgb.y + dC2 * rgb.z;
@@ -538,26 +558,34 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
             }
             const float dLda = T * (dptw - last_dT);
             last_dT += alpha * (dptw - last_dT);
-            if (kDepth) last_dT += dD * s_pd[w][pp];
-            s_pl[w][pp] = dLda;
-            s_pt[w][pp] = T;
+            if (kDepth) last_dT += dD * s_pd[w][i - w0];
+            s_pl[w][i - w0] = dLda;
+            s_pt[w][i - w0] = T;
+          }
+          wave_lds_sync_b();
+
+          // 3b. compact the contributing pairs of the window
+          uint32_t nc = 0;
+          for (uint32_t c0 = 0; c0 < np; c0 += 64) {
+            const uint32_t pp = c0 + lane;
+            const bool f = pp < np && s_pt[w][pp] >= 0.f;
+            const uint64_t b = __ballot(f);
+            if (f) s_cidx[w][nc + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))] = (uint16_t)pp;
+            nc += (uint32_t)__popcll(b);
           }
           wave_lds_sync_b();
 
           // 4. per-pair derivatives, accumulated per entry in LDS
-          for (uint32_t pp = lane; pp < np; pp += 64) {
+          for (uint32_t k = lane; k < nc; k += 64) {
+            const uint32_t pp = s_cidx[w][k];
+            const uint32_t d = s_desc[w][pp];
+            const int j = (int)(d & 127u), bit = (int)(d >> 7);
             const float Tp = s_pt[w][pp];
-            if (Tp < 0.f) continue;
-            const uint32_t p = p0 + pp;
-            int el = j0, eh = j1 - 1;
-            while (el < eh) {
-              const int mid = (el + eh + 1) >> 1;
-              if (s_pref[w][mid] <= p) el = mid; else eh = mid - 1;
-            }
-            const int j = el;
-            const int bit = select_set_bit_b(s_mask[w][j], (int)(p - s_pref[w][j]));
             const float4 rdd = s_rd[w][bit];
-            const Ray pr = lane_ray(r, rdd);
+            const float4 iv = s_ri[w][bit];
+            Ray pr = r;
+            pr.dx = rdd.x; pr.dy = rdd.y; pr.dz = rdd.z;
+            pr.ix = iv.x; pr.iy = iv.y; pr.iz = iv.z;
             const float4 ch = s_ch[j];
             float ta, tb;
             ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
@@ -571,7 +599,7 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
             const float ptw = alpha * Tp;
             float gv[12];
 #pragma unroll
-            for (int k = 0; k < 12; ++k) gv[k] = 0.f;
+            for (int q = 0; q < 12; ++q) gv[q] = 0.f;
             const float p0v = seg.phi[0], p3v = seg.phi[3];
             const float rden = 1.f / (p0v + kPhiEps);
             const float q0 = p0v * (1.f - p0v), q3 = p3v * (1.f - p3v);
@@ -634,11 +662,10 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
             }
             gv[11] = fabsf(dLda * alpha);
 #pragma unroll
-            for (int k = 0; k < 12; ++k) atomicAdd(&s_acc[j][k], gv[k]);
+            for (int q = 0; q < 12; ++q) atomicAdd(&s_acc[j][q], gv[q]);
             s_used[j] = 1u;
           }
           wave_lds_sync_b();
-          j0 = j1;
         }
       }
       __syncthreads();

@@ -37,10 +37,11 @@ __device__ inline void stage_entry(const uint32_t* __restrict__ vals, uint32_t e
                                    const float* __restrict__ lengths, const float* __restrict__ geos,
                                    const float* __restrict__ rgbs, bool need_normal,
                                    float4* s_ch, float4* s_ml, float4* s_g0, float4* s_g1,
-                                   float4* s_rgb, float4* s_nrm, uint2* s_bb, int slot) {
+                                   float4* s_rgb, float4* s_nrm, uint2* s_bb, int slot,
+                                   bool cull = false, float s = 0.f, float cam_scale = 0.f) {
   const uint32_t val = vals[e];
   const uint32_t vox = val & kVoxMask;
-  s_bb[slot] = bboxes[(long long)view * P + vox];
+  uint2 bb = bboxes[(long long)view * P + vox];
   const float cx = centers[3 * vox + 0], cy = centers[3 * vox + 1], cz = centers[3 * vox + 2];
   const float l = lengths[vox];
   s_ch[slot] = make_float4(cx, cy, cz, 0.5f * l);
@@ -49,6 +50,13 @@ __device__ inline void stage_entry(const uint32_t* __restrict__ vals, uint32_t e
   const float4 g1 = reinterpret_cast<const float4*>(geos)[2 * vox + 1];
   s_g0[slot] = g0;
   s_g1[slot] = g1;
+  if (cull) {
+    // An entry that cannot reach the minimum alpha gets an empty bbox (no pixel masks).
+    const float g[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+    const float ps = cam_scale + fmaxf(fabsf(cx), fmaxf(fabsf(cy), fabsf(cz)));
+    if (!voxel_may_contribute(g, s, ps, l)) bb = make_uint2(0xFFFFFFFFu, 0u);
+  }
+  s_bb[slot] = bb;
   s_rgb[slot] = make_float4(rgbs[3 * vox + 0], rgbs[3 * vox + 1], rgbs[3 * vox + 2], __uint_as_float(val));
   if (need_normal) {
     const float g[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
@@ -206,35 +214,37 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Pair-compacted variant.  Sub-pixel voxels cover a handful of a wave's 64 pixels, so a loop in
-// which each lane tests every entry leaves most lanes idle during the expensive SDF segment.
-// Here each wave, per staged batch:
-//   1. builds for every entry the 64-bit mask of its pixels that lie in the entry's bbox, match
-//      its octant and are still unsaturated, and prefix-sums the popcounts;
-//   2. evaluates the (entry, pixel) pairs 64 at a time, one pair per lane, and stores alpha (and
-//      the depth value) per pair in LDS;
-//   3. walks the entries in order per pixel and composites from the stored pair results.
-// Compositing order and arithmetic per pixel are unchanged, so the output equals the
-// one-entry-at-a-time kernel bit for bit.
+// Pixel-major pair compositor (default).  Most voxels cover a handful of a tile's pixels and most
+// (pixel, voxel) candidates have alpha below the minimum, so the cost must follow the pairs that
+// exist, not the entries times the pixels.  Per staged batch of 128 entries each wave:
+//   1. builds the 64-bit mask of every entry's pixels in its 8x8 block (bbox, octant, pixels
+//      still unsaturated; empty for an entry whose corner SDFs cannot reach the minimum alpha),
+//      lane l holding entries l and l + 64;
+//   2. transposes the two 64x64 bit matrices, so lane l holds the 128-bit set of entries that
+//      cover its pixel, and prefix-sums the per-pixel counts: pixel l owns the run
+//      [off_l, off_l + cnt_l) of a pixel-major pair list;
+//   3. writes the (entry, pixel) descriptor of each pair, then evaluates alpha (and the depth
+//      value) of 64 pairs per instruction into LDS, a window of up to kPairWin pairs at a time;
+//   4. composites each pixel over its own contiguous run of results, in entry order.
+// Per-pixel arithmetic and order are those of the one-entry-at-a-time kernel, so the outputs are
+// identical bit for bit.
 // ------------------------------------------------------------------------------------------
-constexpr int kPairStage = 128;   // entries staged per workgroup round
-constexpr int kPairCap = 256;     // pair results held per wave at once
+constexpr int kPairStage = 128;   // entries staged per workgroup round (two 64-entry halves)
+constexpr int kPairWin = 512;     // pair results held per wave at once
 
-__device__ inline int select_set_bit(uint64_t m, int k) {
-  int pos = 0;
-  const int c32 = __popc((uint32_t)m);
-  if (k >= c32) { k -= c32; m >>= 32; pos += 32; }
-  uint32_t x = (uint32_t)m;
-  int c = __popc(x & 0xFFFFu);
-  if (k >= c) { k -= c; x >>= 16; pos += 16; }
-  c = __popc(x & 0xFFu);
-  if (k >= c) { k -= c; x >>= 8; pos += 8; }
-  c = __popc(x & 0xFu);
-  if (k >= c) { k -= c; x >>= 4; pos += 4; }
-  c = __popc(x & 0x3u);
-  if (k >= c) { k -= c; x >>= 2; pos += 2; }
-  if (k >= (int)(x & 1u)) pos += 1;
-  return pos;
+// Transposes a 64x64 bit matrix held one row per lane (lane i, bit b = A[i][b]): lane b returns
+// column b (bit i = A[i][b]).  Six butterfly stages swap the off-diagonal blocks of size 32..1.
+__device__ inline uint64_t transpose64(uint64_t x, int lane) {
+  int sh = 32;
+#pragma unroll
+  for (int k = 0; k < 6; ++k, sh >>= 1) {
+    const uint32_t plo = (uint32_t)__shfl_xor((int)(uint32_t)x, sh, 64);
+    const uint32_t phi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), sh, 64);
+    const uint64_t p = ((uint64_t)phi << 32) | plo;
+    const uint64_t m = ~0ull / ((1ull << sh) + 1ull);
+    x = (lane & sh) ? ((x & ~m) | ((p & ~m) >> sh)) : ((x & m) | ((p & m) << sh));
+  }
+  return x;
 }
 
 // Pixels of the wave's 8x8 block (bit = 8 * row + column) inside a packed bbox.
@@ -264,22 +274,41 @@ __device__ inline uint32_t wave_incl_scan_u32(uint32_t x, int lane) {
   return x;
 }
 
+// Mask of entry slot j for the wave's block: bbox, octant of the pixel rays, live pixels.
+__device__ inline uint64_t entry_block_mask(const uint2* s_bb, const float4* s_rgb, int j, int qx, int qy,
+                                            const uint64_t* s_om, uint64_t live) {
+  const uint2 bb = s_bb[j];
+  const uint32_t q = __float_as_uint(s_rgb[j].w) >> 29;
+  return bbox_block_mask(bb.x, bb.y, qx, qy) & live & s_om[q];
+}
+
+// Pops the lowest entry of a 128-bit entry set.
+__device__ inline int pop_entry(uint64_t& lo, uint64_t& hi) {
+  if (lo) {
+    const int j = __builtin_ctzll(lo);
+    lo &= lo - 1ull;
+    return j;
+  }
+  const int j = 64 + __builtin_ctzll(hi);
+  hi &= hi - 1ull;
+  return j;
+}
+
 template <bool kDepth, bool kNormal, bool kMaxW>
 __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
   __shared__ float4 s_ch[kPairStage], s_ml[kPairStage], s_g0[kPairStage], s_g1[kPairStage], s_rgb[kPairStage];
   __shared__ float4 s_nrm[kNormal ? kPairStage : 1];
   __shared__ uint2 s_bb[kPairStage];
-  __shared__ uint64_t s_mask[4][kPairStage];
-  __shared__ uint32_t s_pref[4][kPairStage + 1];
   __shared__ float4 s_rd[4][64];          // direction xyz
   __shared__ float4 s_ri[4][64];          // 1/direction xyz
-  __shared__ float s_pa[4][kPairCap];     // pair alpha (0 when the ray misses the box)
-  __shared__ float s_pd[4][kDepth ? kPairCap : 1];
+  __shared__ uint64_t s_om[4][8];            // pixels of the block whose ray has octant q
+  __shared__ uint16_t s_desc[4][kPairWin];   // entry slot | pixel << 7
+  __shared__ float s_pa[4][kPairWin];        // pair alpha (0 when the ray misses the box)
+  __shared__ float s_pd[4][kDepth ? kPairWin : 1];
   __shared__ uint32_t s_red[4];
   __shared__ int s_tile;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const float s = *a.s_val;
-  const uint64_t lt_mask = (1ull << lane) - 1ull;
 
   for (int iter = 0;; ++iter) {
     int g;
@@ -302,11 +331,18 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
     const int px = qx + (lane & 7), py = qy + (lane >> 3);
     const bool inside = px < W && py < H;
     const Ray r = pixel_ray(V, px, py);
+    const float cam_scale = fmaxf(fabsf(r.ox), fmaxf(fabsf(r.oy), fabsf(r.oz)));
     s_rd[w][lane] = make_float4(r.dx, r.dy, r.dz, 0.f);
     s_ri[w][lane] = make_float4(r.ix, r.iy, r.iz, 0.f);
-    uint64_t oct_mask[8];
+    {
+      uint64_t om = 0ull;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) oct_mask[q] = __ballot(inside && r.octant == (uint32_t)q);
+      for (int q = 0; q < 8; ++q) {
+        const uint64_t b = __ballot(inside && r.octant == (uint32_t)q);
+        om = (lane == q) ? b : om;
+      }
+      if (lane < 8) s_om[w][lane] = om;
+    }
     const uint2 rg = a.ranges[g];
     const int n = (int)(rg.y - rg.x);
 
@@ -321,65 +357,43 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
       if (t < kPairStage && base + t < n)
         stage_entry(a.vals, rg.x + base + t, vi, a.P, a.bboxes, a.centers, a.lengths, a.geos,
                     a.rgbs + (long long)vi * a.rgb_view_stride, kNormal, s_ch, s_ml, s_g0, s_g1, s_rgb,
-                    s_nrm, s_bb, t);
+                    s_nrm, s_bb, t, true, s, cam_scale);
       __syncthreads();
       const int cnt = min(kPairStage, n - base);
-      const uint64_t alive = __ballot(!done);
-      if (alive == 0) continue;
+      const uint64_t live = __ballot(!done);
+      if (live == 0ull) continue;
 
-      // 1. masks and prefix of pair counts (lane l owns entries 2l and 2l+1).
-      uint64_t m2[2];
-      uint32_t c2[2];
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int j = 2 * lane + u;
-        uint64_t m = 0ull;
-        if (j < cnt) {
-          const uint2 bb = s_bb[j];
-          const uint32_t q = __float_as_uint(s_rgb[j].w) >> 29;
-          uint64_t om = oct_mask[0];
-#pragma unroll
-          for (int qq = 1; qq < 8; ++qq) om = (q == (uint32_t)qq) ? oct_mask[qq] : om;
-          m = bbox_block_mask(bb.x, bb.y, qx, qy) & alive & om;
-        }
-        m2[u] = m;
-        c2[u] = (uint32_t)__popcll(m);
-      }
-      const uint32_t incl = wave_incl_scan_u32(c2[0] + c2[1], lane);
-      const uint32_t excl = incl - c2[0] - c2[1];
-      s_mask[w][2 * lane] = m2[0];
-      s_mask[w][2 * lane + 1] = m2[1];
-      s_pref[w][2 * lane] = excl;
-      s_pref[w][2 * lane + 1] = excl + c2[0];
+      // 1. + 2. entry masks, transposed to per-pixel entry sets; pixel-major run offsets.
+      const uint64_t m_lo = lane < cnt ? entry_block_mask(s_bb, s_rgb, lane, qx, qy, s_om[w], live) : 0ull;
+      const uint64_t m_hi = lane + 64 < cnt ? entry_block_mask(s_bb, s_rgb, lane + 64, qx, qy, s_om[w], live) : 0ull;
+      if (__ballot((m_lo | m_hi) != 0ull) == 0ull) continue;
+      uint64_t e_lo = transpose64(m_lo, lane);
+      uint64_t e_hi = transpose64(m_hi, lane);
+      const uint32_t np_lane = (uint32_t)(__popcll(e_lo) + __popcll(e_hi));
+      const uint32_t incl = wave_incl_scan_u32(np_lane, lane);
+      const uint32_t off = incl - np_lane, endp = incl;
       const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-      if (lane == 0) s_pref[w][kPairStage] = total;
-      wave_lds_sync();
-      if (total == 0) continue;
+      uint32_t cur = off;
 
-      // 2. + 3. over windows of whole entries holding at most kPairCap pairs.
-      int j0 = 0;
-      while (j0 < cnt) {
-        const uint32_t p0 = s_pref[w][j0];
-        int lo = j0 + 1, hi = cnt;           // largest j1 in (j0, cnt] with pref[j1] - p0 <= cap
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (s_pref[w][mid] - p0 <= (uint32_t)kPairCap) lo = mid; else hi = mid - 1;
+      for (uint32_t w0 = 0; w0 < total; w0 += kPairWin) {
+        const uint32_t w1 = min(total, w0 + (uint32_t)kPairWin);
+        // 3a. descriptors of this pixel's pairs that fall in [w0, w1)
+        const uint32_t stop = min(endp, w1);
+        while (cur < stop) {
+          const int j = pop_entry(e_lo, e_hi);
+          s_desc[w][cur - w0] = (uint16_t)(j | (lane << 7));
+          ++cur;
         }
-        const int j1 = lo;
-        const uint32_t np = s_pref[w][j1] - p0;
+        wave_lds_sync();
+        // 3b. alpha (and depth value) per pair
+        const uint32_t np = w1 - w0;
         for (uint32_t pp = lane; pp < np; pp += 64) {
-          const uint32_t p = p0 + pp;
-          int el = j0, eh = j1 - 1;              // last entry with pref <= p
-          while (el < eh) {
-            const int mid = (el + eh + 1) >> 1;
-            if (s_pref[w][mid] <= p) el = mid; else eh = mid - 1;
-          }
-          const int j = el;
-          const int bit = select_set_bit(s_mask[w][j], (int)(p - s_pref[w][j]));
+          const uint32_t d = s_desc[w][pp];
+          const int j = (int)(d & 127u), bit = (int)(d >> 7);
           Ray pr = r;
-          const float4 d = s_rd[w][bit];
+          const float4 dv = s_rd[w][bit];
           const float4 iv = s_ri[w][bit];
-          pr.dx = d.x; pr.dy = d.y; pr.dz = d.z;
+          pr.dx = dv.x; pr.dy = dv.y; pr.dz = dv.z;
           pr.ix = iv.x; pr.iy = iv.y; pr.iz = iv.z;
           const float4 ch = s_ch[j];
           float ta, tb;
@@ -402,17 +416,17 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
           if (kDepth) s_pd[w][pp] = dval;
         }
         wave_lds_sync();
-        for (int j = j0; j < j1; ++j) {
-          const uint64_t m = s_mask[w][j];
-          if (!((m >> lane) & 1ull) || done) continue;
-          const uint32_t pp = s_pref[w][j] - p0 + (uint32_t)__popcll(m & lt_mask);
-          const float alpha = s_pa[w][pp];
+        // 4. composite this pixel's run inside the window, in entry order
+        const uint32_t r0 = max(off, w0), r1 = min(endp, w1);
+        for (uint32_t i = r0; i < r1 && !done; ++i) {
+          const float alpha = s_pa[w][i - w0];
           if (alpha < kMinAlpha) continue;
+          const int j = (int)(s_desc[w][i - w0] & 127u);
           const float4 rgb = s_rgb[j];
           const float wgt = T * alpha;
           C0 += wgt * rgb.x; C1 += wgt * rgb.y; C2 += wgt * rgb.z;
           if (kDepth) {
-            D += T * s_pd[w][pp];
+            D += T * s_pd[w][i - w0];
             if (T > 0.5f) { dmed_vox = (int)(__float_as_uint(rgb.w) & kVoxMask); dmed_T = T; }
           }
           if (kNormal) {
@@ -426,7 +440,6 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
             atomicMax(reinterpret_cast<uint32_t*>(a.max_w) + (__float_as_uint(rgb.w) & kVoxMask), __float_as_uint(wgt));
         }
         wave_lds_sync();
-        j0 = j1;
       }
     }
 

@@ -308,6 +308,35 @@ hipError_t launch_sh_views_backward(int P, int V, const uint32_t* n_dup, const V
   return hipGetLastError();
 }
 
+// Leaf voxels that can still produce a pair alpha >= kMinAlpha under the current corner SDFs
+// and s (voxel_may_contribute); the rest can be left out of binning without changing any pixel.
+__global__ __launch_bounds__(256) void voxel_activity_kernel(int P, const uint8_t* __restrict__ is_leaf,
+                                                             const float* __restrict__ geos,
+                                                             const float* __restrict__ s_val,
+                                                             const float* __restrict__ centers,
+                                                             const float* __restrict__ lengths, float cam_scale,
+                                                             uint8_t* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  uint8_t keep = 0;
+  if (is_leaf[i]) {
+    const float4 g0 = reinterpret_cast<const float4*>(geos)[2 * i];
+    const float4 g1 = reinterpret_cast<const float4*>(geos)[2 * i + 1];
+    const float g[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+    const float c = fmaxf(fabsf(centers[3 * i]), fmaxf(fabsf(centers[3 * i + 1]), fabsf(centers[3 * i + 2])));
+    keep = voxel_may_contribute(g, *s_val, cam_scale + c, lengths[i]) ? 1 : 0;
+  }
+  out[i] = keep;
+}
+
+hipError_t launch_voxel_activity(int P, const uint8_t* is_leaf, const float* geos, const float* s_val,
+                                 const float* centers, const float* lengths, float cam_scale, uint8_t* out,
+                                 hipStream_t st) {
+  if (P <= 0) return hipSuccess;
+  voxel_activity_kernel<<<blocks_for(P), 256, 0, st>>>(P, is_leaf, geos, s_val, centers, lengths, cam_scale, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_gather_geo(int n, const int64_t* vox_key, const int64_t* care, const float* grid_pts,
                              float* geos, hipStream_t st) {
   if (n <= 0) return hipSuccess;

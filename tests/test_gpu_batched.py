@@ -82,3 +82,28 @@ def test_batched_graph_replays_match_eager():
         e_s = rel_l2(step.sh0_grad, ref_sh0)
         say(f"  replay {k}: colour {e_c:.2e} grid {e_g:.2e} sh0 {e_s:.2e} ({junk.item():.1f})")
         assert e_c == 0.0 and e_g <= 1e-5 and e_s <= 1e-5
+
+
+@pytest.mark.gpu
+def test_batched_voxel_skip_keeps_results():
+    say("[batched] leaving non-contributing voxels out of binning: same images, fewer keys")
+    w, sc, cams, targets = _setup()
+    m = VoxelModel(sc)
+    res = {}
+    for cull in (False, True):
+        r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key,
+                              cams, m.bg_color, sc.sh_degree, cull=cull)
+        step = BatchedTrainStep(r, m._geo_grid_pts.detach(), m._sh0.detach(), m._shs.detach(),
+                                m._log_s.detach(), targets)
+        step()
+        torch.cuda.synchronize()
+        res[cull] = (r.color.clone(), r.T.clone(), step.grid_grad.clone(), step.sh0_grad.clone(),
+                     step.shs_grad.clone(), step.s_grad.clone(), r.key_count())
+    a, b = res[False], res[True]
+    say(f"  keys {a[6]} -> {b[6]}")
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+    e = [rel_l2(x, y) for x, y in zip(b[2:5], a[2:5])]
+    e_s = abs(b[5].item() - a[5].item()) / max(abs(a[5].item()), 1e-12)
+    say(f"  rel-L2 grid {e[0]:.2e} sh0 {e[1]:.2e} shs {e[2]:.2e} s {e_s:.2e}")
+    assert max(e) <= 1e-5 and e_s <= 1e-4
+    assert b[6] < a[6]

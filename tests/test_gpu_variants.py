@@ -1,6 +1,8 @@
-"""The pair-compacted compositor (default) against the one-entry-at-a-time compositor: forward
-outputs must be bit-identical (same per-pixel arithmetic and order); backward gradients may
-differ only by the summation order of float atomics."""
+"""The pixel-major pair compositor (default; skips voxels whose corner SDFs cannot reach the
+minimum alpha) against the one-entry-at-a-time compositor (no skipping): forward outputs must be
+bit-identical (same per-pixel arithmetic and order, and a skipped voxel contributes to no pixel);
+backward gradients may differ only by the summation order of float atomics.  The SDF sharpness s
+is scaled to move many voxels across the skip bound."""
 import pytest
 import torch
 
@@ -9,14 +11,14 @@ from svraster_amd.synthetic import make_workload
 from tests._util import hip_raster, oracle_rgbs, rel_l2, say
 
 
-def _run(name, variant, window_grads=True):
+def _run(name, variant, s_scale):
     _lib.set_option("compositor_variant", variant)
     try:
         w = make_workload(name, device="cuda")
         sc, cam = w.scene, w.cameras[0]
         rgbs = oracle_rgbs(sc, cam).float().cuda()
         geos = sc.geo_grid_pts.reshape(-1)[sc.vox_key].float().contiguous()
-        s = torch.tensor([w.s_val], dtype=torch.float32, device="cuda")
+        s = torch.tensor([w.s_val * s_scale], dtype=torch.float32, device="cuda")
         bg = torch.ones(3, device="cuda")
         out = hip_raster(sc, cam, geos, rgbs, s, bg, track_max_w=True)
         H, W = cam.image_height, cam.image_width
@@ -39,11 +41,13 @@ def _run(name, variant, window_grads=True):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["small_128", "fd_256", "single_1237"])
-def test_pair_compositor_matches_uniform_loop(name):
-    say(f"[variants] {name}")
-    a, ga = _run(name, 0)
-    b, gb = _run(name, 1)
+@pytest.mark.parametrize("name,s_scale", [("small_128", 1.0), ("small_128", 0.25), ("small_128", 4.0),
+                                          ("fd_256", 1.0), ("fd_256", 0.1), ("fd_256", 10.0),
+                                          ("single_1237", 1.0)])
+def test_pair_compositor_matches_uniform_loop(name, s_scale):
+    say(f"[variants] {name} s x {s_scale}")
+    a, ga = _run(name, 0, s_scale)
+    b, gb = _run(name, 1, s_scale)
     for k in ("color", "T", "depth", "normal", "max_w"):
         assert torch.equal(a[k], b[k]), k
     _, _, na = _C.unpack_ImageState(a["color"].shape[2], a["color"].shape[1], a["img"])
