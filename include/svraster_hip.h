@@ -175,10 +175,16 @@ int svr_batch_setup(void* ws, int n_views, int P, long long n_pixels, long long 
                     const int* W, const int* H, const float* tanfovx, const float* tanfovy,
                     const float* cx, const float* cy, const float* near_plane, const float* w2c,
                     const float* c2w, const float* background, void* stream);
-/* Preprocess (bbox, octants, n_dup) of every (view, voxel) and the inclusive scan. */
+/* Rank order of every (voxel, octant) pair, computed once per voxel set: order_out[8P] holds
+ * voxel << 3 | octant sorted by the reference's in-tile rank octpath ^ repeat3(octant)
+ * (auxiliary.h:40-59, forward.cu:749-802).  tmp: svr_batch_pair_order_bytes(P) bytes. */
+size_t svr_batch_pair_order_bytes(int P);
+int svr_batch_pair_order(int P, const int64_t* octree_paths, void* tmp, uint32_t* order_out, void* stream);
+/* Preprocess (bbox, octants, n_dup; preprocess.cu:60-153) of every (view, voxel), voxel-major,
+ * and the key count of every (voxel, octant) pair over all views, scanned in rank order. */
 int svr_batch_preprocess(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
-                         const int64_t* octree_paths, const float* vox_centers, const float* vox_lengths,
-                         const uint8_t* is_leaf, void* stream);
+                         const float* vox_centers, const float* vox_lengths, const uint8_t* is_leaf,
+                         const uint32_t* pair_order, void* stream);
 /* out[i] = is_leaf[i] and voxel i can still produce a pair alpha >= 1e-5 (MIN_ALPHA,
  * config.h:19) for the corner SDFs `geos` [P,8] and s: a conservative bound, so a voxel with
  * out[i] = 0 contributes to no pixel (forward.cu:303-305 skips every such pair).  Passing `out`
@@ -192,13 +198,15 @@ int svr_voxel_activity(int P, const uint8_t* is_leaf, const float* geos, const f
 int svr_batch_sh_forward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                          const float* vox_centers, const float* sh0, const float* shs, int degree, int M,
                          float* rgbs_views, void* stream);
-/* Duplication, sort, tile ranges and one persistent compositor launch over all (view, tile)
- * pairs.  rank_shift: 3 * (16 - deepest octree level present).  Outputs zeroed by the caller. */
+/* Keys written pair by pair in rank order, a stable sort by global (view, tile) id only (each
+ * tile's entries then follow the reference's order, forward.cu:749-829), tile ranges, and one
+ * persistent compositor launch over all (view, tile) pairs.  Colour and T need no clearing;
+ * depth is zeroed by the caller. */
 int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
-                        const int64_t* octree_paths, const float* vox_centers, const float* vox_lengths,
-                        const float* geos, const float* rgbs_views, const float* s_val, int rank_shift,
-                        int need_depth, int need_normal, float* out_color, float* out_depth,
-                        float* out_normal, float* out_T, void* stream);
+                        const float* vox_centers, const float* vox_lengths, const float* geos,
+                        const float* rgbs_views, const float* s_val, const uint32_t* pair_order, int need_depth,
+                        int need_normal, float* out_color, float* out_depth, float* out_normal, float* out_T,
+                        void* stream);
 /* One persistent reverse compositor launch over all (view, tile) pairs.  dL_dvox [P,12]
  * (colour slots unused), dL_drgb_views [V,P,3] and dL_ds [1] zeroed by the caller. */
 int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
@@ -208,10 +216,17 @@ int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long lo
                        const float* dL_dT, int need_depth, int need_normal, float* dL_dvox,
                        float* dL_drgb_views, float* dL_ds, void* stream);
 /* SH transpose summed over the views, one voxel per thread; writes every row of dL_dsh0 [P,3]
- * and dL_dshs [P,M-1,3]. */
+ * and dL_dshs [P,M-1,3] (M <= 16).  The entries of dL_drgb_views it consumes are reset to zero,
+ * so a buffer that starts at zero is clear again for the next svr_batch_backward. */
 int svr_batch_sh_backward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
-                          const float* vox_centers, const float* dL_drgb_views, int degree, int M,
+                          const float* vox_centers, float* dL_drgb_views, int degree, int M,
                           float* dL_dsh0, float* dL_dshs, void* stream);
+/* Byte offsets inside the batch workspace of (for inspection, like unpack_ImageState /
+ * unpack_BinningState, raster_state.cu:120-230): out[0] tile ranges uint2[n_tiles], out[1]
+ * sorted values uint32[K], out[2] per-pixel contributor counts uint32[n_pixels], out[3]
+ * last-contributor bound per tile uint32[n_tiles]. */
+int svr_batch_state_offsets(int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                            long long* out);
 /* Total key count of the last svr_batch_preprocess (synchronises `stream`). */
 int svr_batch_key_count(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                         long long* K_out, void* stream);

@@ -52,12 +52,15 @@ _SIGS = {
     "svr_valid_gradient_table_fill": (I, [P, P, P, F, I, P, I, P, P, P, P, P]),
     "svr_batch_workspace_bytes": (SZ, [I, I, LL, LL, LL]),
     "svr_batch_setup": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, P, P, P, P, P]),
+    "svr_batch_pair_order_bytes": (SZ, [I]),
+    "svr_batch_pair_order": (I, [I, P, P, P, P]),
     "svr_batch_preprocess": (I, [P, I, I, LL, LL, LL, P, P, P, P, P]),
     "svr_voxel_activity": (I, [I, P, P, P, P, P, F, P, P]),
     "svr_batch_sh_forward": (I, [P, I, I, LL, LL, LL, P, P, P, I, I, P, P]),
-    "svr_batch_rasterize": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, I, I, I, P, P, P, P, P]),
+    "svr_batch_rasterize": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, I, I, P, P, P, P, P]),
     "svr_batch_backward": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, P, P, P, P, I, I, P, P, P, P]),
     "svr_batch_sh_backward": (I, [P, I, I, LL, LL, LL, P, P, I, I, P, P, P]),
+    "svr_batch_state_offsets": (I, [I, I, LL, LL, LL, P]),
     "svr_batch_key_count": (I, [P, I, I, LL, LL, LL, ctypes.POINTER(LL), P]),
 }
 

@@ -48,7 +48,12 @@ class BatchedRasterizer:
         self.deg = int(sh_degree)
         self.M = int(n_sh_coeffs)
         self.need_depth, self.need_normal = bool(need_depth), bool(need_normal)
-        self.rank_shift = 3 * (MAX_NUM_LEVELS - int(octlevel.max().item()))
+        # Rank order of every (voxel, octant) pair: fixed while the voxel set is.
+        tmp = torch.empty(int(self.lib.svr_batch_pair_order_bytes(self.P)), dtype=torch.uint8, device=self.dev)
+        self.pair_order = torch.empty(8 * self.P, dtype=torch.int32, device=self.dev)
+        _lib.check(self.lib.svr_batch_pair_order(self.P, _p(self.octpath), _p(tmp), _p(self.pair_order),
+                                                 self._stream()), "svr_batch_pair_order")
+        del tmp
         self.Ws = [c.image_width for c in cameras]
         self.Hs = [c.image_height for c in cameras]
         self.tiles = [((w + 15) // 16) * ((h + 15) // 16) for w, h in zip(self.Ws, self.Hs)]
@@ -119,9 +124,19 @@ class BatchedRasterizer:
     # -- stages ----------------------------------------------------------------------------
     def preprocess(self, leaf: Optional[torch.Tensor] = None):
         leaf = self.is_leaf if leaf is None else leaf
-        _lib.check(self.lib.svr_batch_preprocess(_p(self.ws), *self._dims(), _p(self.octpath), _p(self.vox_center),
-                                                 _p(self.vox_size), _p(leaf), self._stream()),
+        _lib.check(self.lib.svr_batch_preprocess(_p(self.ws), *self._dims(), _p(self.vox_center), _p(self.vox_size),
+                                                 _p(leaf), _p(self.pair_order), self._stream()),
                    "svr_batch_preprocess")
+
+    def state_views(self):
+        """(ranges int32 [n_tiles, 2], vals int32 [K], n_contrib int32 [n_pixels], tile_last int32
+        [n_tiles]) viewed inside the workspace (global tile ids, concatenated pixels)."""
+        off = (ctypes.c_longlong * 4)()
+        _lib.check(self.lib.svr_batch_state_offsets(*self._dims(), off), "svr_batch_state_offsets")
+        K = min(self.key_count(), self.cap)
+        i32 = lambda o, n: self.ws[o:o + 4 * n].view(torch.int32)
+        return (i32(off[0], 2 * self.n_tiles).view(self.n_tiles, 2), i32(off[1], K), i32(off[2], self.n_pix),
+                i32(off[3], self.n_tiles))
 
     def key_count(self) -> int:
         k = ctypes.c_longlong(0)
@@ -144,16 +159,15 @@ class BatchedRasterizer:
             self.preprocess()
         _lib.check(self.lib.svr_batch_sh_forward(_p(self.ws), *self._dims(), _p(self.vox_center), _p(sh0), _p(shs),
                                                  self.deg, self.M, _p(self.rgbs_views), st), "sh_forward")
-        self.color.zero_()
-        self.T.zero_()
+        # Every pixel of every tile is written by the compositor: colour and T need no clearing.
         if self.depth is not None:
             self.depth.zero_()
         if self.normal is not None:
             self.normal.zero_()
         self.s_val = s_val
-        _lib.check(self.lib.svr_batch_rasterize(_p(self.ws), *self._dims(), _p(self.octpath), _p(self.vox_center),
-                                                _p(self.vox_size), _p(self.geos), _p(self.rgbs_views), _p(s_val),
-                                                self.rank_shift, int(self.need_depth), int(self.need_normal),
+        _lib.check(self.lib.svr_batch_rasterize(_p(self.ws), *self._dims(), _p(self.vox_center), _p(self.vox_size),
+                                                _p(self.geos), _p(self.rgbs_views), _p(s_val), _p(self.pair_order),
+                                                int(self.need_depth), int(self.need_normal),
                                                 _p(self.color), _p(self.depth), _p(self.normal), _p(self.T), st),
                    "svr_batch_rasterize")
         return self.color
@@ -165,7 +179,7 @@ class BatchedRasterizer:
         if self.dgrid is None or self.dgrid.shape[0] != n_grid_pts:
             self.dgrid = torch.zeros(n_grid_pts, 1, dtype=torch.float32, device=self.dev)
         self.dvox.zero_()
-        self.drgb_views.zero_()
+        # drgb_views is zero here: it starts at zero and svr_batch_sh_backward clears what it reads.
         self.ds.zero_()
         self.dgrid.zero_()
         dT = self.zero_T if dL_dT is None else dL_dT

@@ -452,10 +452,12 @@ struct BatchPtrs {
   GeomPtrs g;
   BinPtrs b;
   ImagePtrs im;
+  PairPtrs pr;
   BatchPtrs(void* ws, int V, int P, long long n_pix, long long n_tiles, long long cap)
       : g(ws, V, P),
         b((char*)ws + GeomLayout(V, P).total, cap),
-        im((char*)ws + GeomLayout(V, P).total + BinLayout(cap).total, V, n_pix, n_tiles) {}
+        im((char*)ws + GeomLayout(V, P).total + BinLayout(cap).total, V, n_pix, n_tiles),
+        pr((char*)ws + GeomLayout(V, P).total + BinLayout(cap).total + ImageLayout(V, n_pix, n_tiles).total, P) {}
 };
 
 int check_batch(int V, int P, long long n_pix, long long n_tiles, long long cap) {
@@ -471,7 +473,8 @@ int check_batch(int V, int P, long long n_pix, long long n_tiles, long long cap)
 extern "C" {
 
 size_t svr_batch_workspace_bytes(int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap) {
-  return GeomLayout(n_views, P).total + BinLayout(K_cap).total + ImageLayout(n_views, n_pixels, n_tiles).total;
+  return GeomLayout(n_views, P).total + BinLayout(K_cap).total + ImageLayout(n_views, n_pixels, n_tiles).total +
+         PairLayout(P).total;
 }
 
 int svr_batch_setup(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
@@ -497,17 +500,37 @@ int svr_batch_setup(void* ws, int n_views, int P, long long n_pixels, long long 
   return 0;
 }
 
+size_t svr_batch_pair_order_bytes(int P) {
+  return P > 0 ? BinLayout(8LL * P).total : 0;
+}
+
+int svr_batch_pair_order(int P, const int64_t* octree_paths, void* tmp, uint32_t* order_out, void* stream) {
+  g_err.clear();
+  if (P <= 0 || P >= (1 << 29)) return fail(1, "voxel count must be 1..2^29%s (got %lld)", "", P);
+  hipStream_t st = as_stream(stream);
+  const long long n = 8LL * P;
+  BinPtrs t(tmp, n);
+  SVR_TRY(launch_pair_rank(n, octree_paths, t.keys, t.vals, st));
+  bool in_alt = false;
+  SVR_TRY(launch_radix_sort(t.keys, t.vals, t.keys_alt, t.vals_alt, nullptr, n, kRankBits, t.hist, t.partial,
+                            &in_alt, st));
+  SVR_TRY(hipMemcpyAsync(order_out, in_alt ? t.vals_alt : t.vals, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice,
+                         st));
+  return 0;
+}
+
 int svr_batch_preprocess(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
-                         const int64_t* octree_paths, const float* vox_centers, const float* vox_lengths,
-                         const uint8_t* is_leaf, void* stream) {
+                         const float* vox_centers, const float* vox_lengths, const uint8_t* is_leaf,
+                         const uint32_t* pair_order, void* stream) {
   g_err.clear();
   if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
   hipStream_t st = as_stream(stream);
   BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
-  const long long VP = (long long)n_views * P;
-  SVR_TRY(launch_preprocess(bp.g.views, n_views, P, vox_centers, vox_lengths, is_leaf, octree_paths, nullptr,
-                            bp.g.n_dup, bp.g.bbox, bp.g.quad, nullptr, st));
-  SVR_TRY(launch_scan(bp.g.n_dup, nullptr, VP, bp.g.partial, bp.g.scan, true, st));
+  const long long n = 8LL * P;
+  SVR_TRY(launch_preprocess_views(bp.g.views, n_views, P, vox_centers, vox_lengths, is_leaf, bp.g.n_dup, bp.g.bbox,
+                                  bp.g.quad, bp.pr.pair_cnt, st));
+  SVR_TRY(launch_pair_gather(n, pair_order, bp.pr.pair_cnt, bp.pr.cnt_ranked, st));
+  SVR_TRY(launch_scan(bp.pr.cnt_ranked, nullptr, n, bp.pr.partial, bp.pr.scan_ranked, true, st));
   return 0;
 }
 
@@ -528,6 +551,7 @@ int svr_batch_sh_forward(void* ws, int n_views, int P, long long n_pixels, long 
   g_err.clear();
   if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
   if (degree < 0 || degree > 3 || M < (degree + 1) * (degree + 1)) return fail(1, "bad SH degree%s %lld", "", degree);
+  if (M > 16) return fail(1, "batched SH supports at most 16 coefficients per channel%s (got %lld)", "", M);
   BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
   SVR_TRY(launch_sh_views_forward(P, n_views, bp.g.n_dup, bp.g.views, degree, M, vox_centers, sh0, shs, rgbs_views,
                                   as_stream(stream)));
@@ -535,31 +559,26 @@ int svr_batch_sh_forward(void* ws, int n_views, int P, long long n_pixels, long 
 }
 
 int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
-                        const int64_t* octree_paths, const float* vox_centers, const float* vox_lengths,
-                        const float* geos, const float* rgbs_views, const float* s_val, int rank_shift,
-                        int need_depth, int need_normal, float* out_color, float* out_depth, float* out_normal,
-                        float* out_T, void* stream) {
+                        const float* vox_centers, const float* vox_lengths, const float* geos,
+                        const float* rgbs_views, const float* s_val, const uint32_t* pair_order, int need_depth,
+                        int need_normal, float* out_color, float* out_depth, float* out_normal, float* out_T,
+                        void* stream) {
   g_err.clear();
   if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
-  if (rank_shift < 0 || rank_shift > kRankBits || rank_shift % 3) return fail(1, "bad rank_shift %s%lld", "", rank_shift);
-  const KeyLayout kl = key_layout(rank_shift, n_tiles);
-  if (kl.nbits > 64) return fail(1, "tile id + rank bits exceed 64%s (%lld)", "", kl.nbits);
   hipStream_t st = as_stream(stream);
   BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
-  const long long VP = (long long)n_views * P;
-  const uint32_t* n_dev = bp.g.scan + (VP - 1);
-  const int passes = (kl.nbits + 7) / 8;
-  const bool start_alt = (passes % 2) == 1;
-  uint64_t* k0 = start_alt ? bp.b.keys_alt : bp.b.keys;
-  uint32_t* v0 = start_alt ? bp.b.vals_alt : bp.b.vals;
-  uint64_t* k1 = start_alt ? bp.b.keys : bp.b.keys_alt;
-  uint32_t* v1 = start_alt ? bp.b.vals : bp.b.vals_alt;
-  SVR_TRY(launch_duplicate(bp.g.views, n_views, P, octree_paths, bp.g.n_dup, bp.g.scan, bp.g.bbox, bp.g.quad,
-                           kl.shift, kl.qb, kl.rank_bits, k0, v0, K_cap, st));
-  bool in_alt = false;
-  SVR_TRY(launch_radix_sort(k0, v0, k1, v1, n_dev, K_cap, kl.nbits, bp.b.hist, bp.b.partial, &in_alt, st));
+  const long long n = 8LL * P;
+  const uint32_t* n_dev = bp.pr.scan_ranked + (n - 1);
+  // Keys are written where the last tile-sort pass ends in (keys, vals).
+  uint32_t* tiles = reinterpret_cast<uint32_t*>(bp.b.keys);
+  uint32_t* tiles_alt = reinterpret_cast<uint32_t*>(bp.b.keys_alt);
+  const bool odd = tile_sort_passes(n_tiles) % 2 == 1;
+  SVR_TRY(launch_pair_emit(bp.g.views, n_views, P, n, pair_order, bp.pr.cnt_ranked, bp.pr.scan_ranked, bp.g.bbox,
+                           bp.g.quad, odd ? tiles_alt : tiles, odd ? bp.b.vals_alt : bp.b.vals, K_cap, st));
+  SVR_TRY(launch_tile_sort(tiles, bp.b.vals, tiles_alt, bp.b.vals_alt, n_dev, K_cap, n_tiles, bp.b.hist,
+                           bp.b.partial, st));
   SVR_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(bp.im.ranges), 2 * n_tiles, 0u, st));
-  SVR_TRY(launch_tile_ranges(bp.b.keys, n_dev, K_cap, kl.rank_bits, bp.im.ranges, st));
+  SVR_TRY(launch_tile_ranges_u32(tiles, n_dev, K_cap, bp.im.ranges, st));
   SVR_TRY(launch_fill_u32(bp.im.misc, 2, 0u, st));
   FwdArgs a;
   memset(&a, 0, sizeof(a));
@@ -616,14 +635,29 @@ int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long lo
 }
 
 int svr_batch_sh_backward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
-                          const float* vox_centers, const float* dL_drgb_views, int degree, int M,
+                          const float* vox_centers, float* dL_drgb_views, int degree, int M,
                           float* dL_dsh0, float* dL_dshs, void* stream) {
   g_err.clear();
   if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
   if (degree < 0 || degree > 3 || M < (degree + 1) * (degree + 1)) return fail(1, "bad SH degree%s %lld", "", degree);
+  if (M > 16) return fail(1, "batched SH supports at most 16 coefficients per channel%s (got %lld)", "", M);
   BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
   SVR_TRY(launch_sh_views_backward(P, n_views, bp.g.n_dup, bp.g.views, degree, M, vox_centers, dL_drgb_views,
                                    dL_dsh0, dL_dshs, as_stream(stream)));
+  return 0;
+}
+
+int svr_batch_state_offsets(int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                            long long* out) {
+  g_err.clear();
+  if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
+  const size_t g = GeomLayout(n_views, P).total;
+  const BinLayout bl(K_cap);
+  const ImageLayout il(n_views, n_pixels, n_tiles);
+  out[0] = (long long)(g + bl.total + il.ranges);
+  out[1] = (long long)(g + bl.vals);
+  out[2] = (long long)(g + bl.total + il.n_contrib);
+  out[3] = (long long)(g + bl.total + il.tile_last);
   return 0;
 }
 
@@ -635,8 +669,7 @@ int svr_batch_key_count(void* ws, int n_views, int P, long long n_pixels, long l
   hipStream_t st = as_stream(stream);
   BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
   uint32_t k = 0;
-  const long long VP = (long long)n_views * P;
-  SVR_TRY(hipMemcpyAsync(&k, bp.g.scan + (VP - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  SVR_TRY(hipMemcpyAsync(&k, bp.pr.scan_ranked + (8LL * P - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   SVR_TRY(hipStreamSynchronize(st));
   *K_out = k;
   return 0;

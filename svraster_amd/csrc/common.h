@@ -81,6 +81,15 @@ __device__ inline Ray pixel_ray(const ViewDesc& v, int px, int py) {
   return r;
 }
 
+// Ray of another pixel of the same view: shared origin, direction and its inverse from LDS
+// (both computed by pixel_ray).
+__device__ inline Ray pair_ray(const Ray& r, const float4 d, const float4 inv) {
+  Ray pr = r;
+  pr.dx = d.x; pr.dy = d.y; pr.dz = d.z;
+  pr.ix = inv.x; pr.iy = inv.y; pr.iz = inv.z;
+  return pr;
+}
+
 // Slab test of a ray against a cube of centre c and half size h (auxiliary.h:251-264).
 __device__ inline void ray_box(const Ray& r, float cx, float cy, float cz, float h,
                                float& a, float& b) {
@@ -150,6 +159,123 @@ __device__ inline void eval_segment(const Ray& r, float a, float b, float minx, 
   }
   const float raw = (seg.phi[0] - seg.phi[3]) * __builtin_amdgcn_rcpf(seg.phi[0] + kPhiEps);
   seg.alpha = fmaxf(raw, 0.f);
+}
+
+// Value of lane (lane ^ kMask) without going through LDS: DPP for masks 1..8 (quad_perm
+// [1,0,3,2] and [2,3,0,1]; half-row mirror then quad reverse for 4; row rotate by 8), and the
+// gfx950 row-swap instructions for 16 and 32.
+template <int kMask>
+__device__ inline uint32_t xor_lane(uint32_t x, int lane) {
+  if constexpr (kMask == 1) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, true);
+  } else if constexpr (kMask == 2) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, true);
+  } else if constexpr (kMask == 4) {
+    const int y = __builtin_amdgcn_mov_dpp((int)x, 0x141, 0xF, 0xF, true);
+    return (uint32_t)__builtin_amdgcn_mov_dpp(y, 0x1B, 0xF, 0xF, true);
+  } else if constexpr (kMask == 8) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x128, 0xF, 0xF, true);
+  } else if constexpr (kMask == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    return (lane & 16) ? r[0] : r[1];
+  } else {
+    static_assert(kMask == 32, "xor mask must be a power of two below 64");
+    const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return (lane & 32) ? r[0] : r[1];
+  }
+}
+
+template <int kSh>
+__device__ inline uint64_t transpose_stage(uint64_t x, int lane) {
+  const uint32_t plo = xor_lane<kSh>((uint32_t)x, lane);
+  const uint32_t phi = xor_lane<kSh>((uint32_t)(x >> 32), lane);
+  const uint64_t p = ((uint64_t)phi << 32) | plo;
+  constexpr uint64_t m = ~0ull / ((1ull << kSh) + 1ull);   // bit positions whose bit kSh is clear
+  return (lane & kSh) ? ((x & ~m) | ((p & ~m) >> kSh)) : ((x & m) | ((p & m) << kSh));
+}
+
+// Transposes a 64x64 bit matrix held one row per lane (lane i, bit b = A[i][b]): lane b returns
+// column b (bit i = A[i][b]).  Six butterfly stages swap the off-diagonal blocks of size 32..1.
+__device__ inline uint64_t transpose64(uint64_t x, int lane) {
+  x = transpose_stage<32>(x, lane);
+  x = transpose_stage<16>(x, lane);
+  x = transpose_stage<8>(x, lane);
+  x = transpose_stage<4>(x, lane);
+  x = transpose_stage<2>(x, lane);
+  return transpose_stage<1>(x, lane);
+}
+
+// Inclusive wave64 prefix sum with DPP: row shifts by 1, 2, 4, 8 (zero fill), then the last lane
+// of row 0 / 2 broadcast into rows 1 / 3 and the last lane of row 1 into rows 2 and 3.
+__device__ inline uint32_t wave_incl_scan_u32(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);
+  return x;
+}
+
+// Inclusive wave64 prefix maximum (same DPP pattern as wave_incl_scan_u32; 0 is the identity).
+__device__ inline uint32_t wave_incl_max_u32(uint32_t x) {
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false));
+  return x;
+}
+
+// Segmented inclusive sum over the wave of N floats per lane; lane l's segment starts at lane
+// seg_start <= l.  After the call the last lane of each segment holds the segment's sums.
+// DPP row shifts by 1, 2, 4, 8 inside 16-lane rows, then the last lane of the previous row
+// (rows 1 and 3) and lane 31 (rows 2 and 3) for segments that started in an earlier row.
+template <int N>
+__device__ inline void wave_segmented_sum(float (&v)[N], int lane, int seg_start) {
+  const int lr = lane & 15;
+  const bool c1 = lr >= 1 && lane - 1 >= seg_start;
+  const bool c2 = lr >= 2 && lane - 2 >= seg_start;
+  const bool c4 = lr >= 4 && lane - 4 >= seg_start;
+  const bool c8 = lr >= 8 && lane - 8 >= seg_start;
+  const bool cr = ((lane >> 4) & 1) && seg_start < (lane & ~15);
+  const bool cw = lane >= 32 && seg_start < 32;
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    float x = v[q];
+    float t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xF, 0xF, true));
+    x += c1 ? t : 0.f;
+    t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x112, 0xF, 0xF, true));
+    x += c2 ? t : 0.f;
+    t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xF, 0xF, true));
+    x += c4 ? t : 0.f;
+    t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x118, 0xF, 0xF, true));
+    x += c8 ? t : 0.f;
+    t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x142, 0xA, 0xF, false));
+    x += cr ? t : 0.f;
+    t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x143, 0xC, 0xF, false));
+    x += cw ? t : 0.f;
+    v[q] = x;
+  }
+}
+
+// Position of the k-th (0-based) set bit of m; k < popcount(m).
+__device__ inline int select_set_bit(uint64_t m, int k) {
+  int pos = 0;
+  const int c32 = __popc((uint32_t)m);
+  if (k >= c32) { k -= c32; m >>= 32; pos += 32; }
+  uint32_t x = (uint32_t)m;
+  int c = __popc(x & 0xFFFFu);
+  if (k >= c) { k -= c; x >>= 16; pos += 16; }
+  c = __popc(x & 0xFFu);
+  if (k >= c) { k -= c; x >>= 8; pos += 8; }
+  c = __popc(x & 0xFu);
+  if (k >= c) { k -= c; x >>= 4; pos += 4; }
+  c = __popc(x & 0x3u);
+  if (k >= c) { k -= c; x >>= 2; pos += 2; }
+  if (k >= (int)(x & 1u)) pos += 1;
+  return pos;
 }
 
 // Conservative per-voxel test: false only if no ray segment through the voxel can reach

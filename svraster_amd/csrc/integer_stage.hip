@@ -33,24 +33,16 @@ __global__ void init_view_kernel(ViewDesc* out, ViewDesc base, const float* w2c,
 // Preprocess.  The floating-point steps are evaluated without contraction so that the packed
 // bounding boxes are bit-identical to an IEEE FP32 restatement with the same operation order.
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void preprocess_kernel(
-    const ViewDesc* __restrict__ views, int P, const float* __restrict__ centers,
-    const float* __restrict__ lengths, const uint8_t* __restrict__ is_leaf,
-    const int64_t* __restrict__ octpaths, int32_t* __restrict__ out_n_dup,
-    uint32_t* __restrict__ n_dup, uint2* __restrict__ bboxes, uint8_t* __restrict__ quads,
-    uint32_t* __restrict__ tz_min) {
+// bbox, octant bits and key count of voxel (c, half size r) in view v (preprocess.cu:60-153);
+// shared by the per-view and the voxel-major batched kernels so both give identical bits.
+__device__ inline void preprocess_one(const ViewDesc& v, bool leaf, float vcx, float vcy, float vcz, float vlen,
+                                      uint32_t& nd, uint2& bb, uint32_t& qbits) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int view = blockIdx.y;
-  if (i >= P) return;
-  const ViewDesc& v = views[view];
-  const long long o = (long long)view * P + i;
-  uint32_t nd = 0;
-  uint2 bb = make_uint2(0u, 0u);
-  uint32_t qbits = 0;
-  if (is_leaf[i] != 0) {
-    const float vcx = centers[3 * i + 0], vcy = centers[3 * i + 1], vcz = centers[3 * i + 2];
-    const float r = 0.5f * lengths[i];
+  nd = 0;
+  bb = make_uint2(0u, 0u);
+  qbits = 0;
+  if (leaf) {
+    const float r = 0.5f * vlen;
     const float rox = v.c2w[3], roy = v.c2w[7], roz = v.c2w[11];
     const float rx = vcx - rox, ry = vcy - roy, rz = vcz - roz;
     const float d2 = rx * rx + ry * ry + rz * rz;
@@ -87,6 +79,25 @@ __global__ __launch_bounds__(256) void preprocess_kernel(
     }
   }
   if (nd == 0) { bb = make_uint2(0u, 0u); qbits = 0; }
+}
+
+__global__ __launch_bounds__(256) void preprocess_kernel(
+    const ViewDesc* __restrict__ views, int P, const float* __restrict__ centers,
+    const float* __restrict__ lengths, const uint8_t* __restrict__ is_leaf,
+    const int64_t* __restrict__ octpaths, int32_t* __restrict__ out_n_dup,
+    uint32_t* __restrict__ n_dup, uint2* __restrict__ bboxes, uint8_t* __restrict__ quads,
+    uint32_t* __restrict__ tz_min) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int view = blockIdx.y;
+  if (i >= P) return;
+  const ViewDesc& v = views[view];
+  const long long o = (long long)view * P + i;
+  uint32_t nd;
+  uint2 bb;
+  uint32_t qbits;
+  preprocess_one(v, is_leaf[i] != 0, centers[3 * i + 0], centers[3 * i + 1], centers[3 * i + 2], lengths[i], nd, bb,
+                 qbits);
   if (out_n_dup) out_n_dup[o] = (int32_t)nd;
   n_dup[o] = nd;
   bboxes[o] = bb;
@@ -240,40 +251,46 @@ __global__ __launch_bounds__(256) void duplicate_kernel(
 }
 
 // ------------------------------------------------------------------------------------------
-// Stable LSD radix sort, 8-bit digits.  Per pass: per-tile digit histograms, an exclusive scan
-// of the digit-major histogram table, then a stable scatter in which each wave64 ranks its 64
-// items with eight ballots (items with equal digits get consecutive slots in lane order).
+// Stable LSD radix sort over kBits-bit digits (8 for the 64-bit rank keys, 9 for 32-bit tile
+// ids).  Per pass: per-tile digit histograms, an exclusive scan of the digit-major histogram
+// table, then a stable scatter in which each wave64 ranks its 64 items with kBits ballots
+// (items with equal digits get consecutive slots in lane order).
 // ------------------------------------------------------------------------------------------
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kMaxDigitBits = 9;
 
+template <typename K, int kBits>
 __global__ __launch_bounds__(kSortThreads) void radix_histogram_kernel(
-    const uint64_t* __restrict__ keys, const uint32_t* __restrict__ n_dev, long long n_cap,
+    const K* __restrict__ keys, const uint32_t* __restrict__ n_dev, long long n_cap,
     int shift, uint32_t* __restrict__ hist, int nb) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+  constexpr int kD = 1 << kBits;
+  __shared__ uint32_t h[kD];
+  for (int d = threadIdx.x; d < kD; d += kSortThreads) h[d] = 0;
   __syncthreads();
   const long long n = n_dev ? min((long long)*n_dev, n_cap) : n_cap;
   const long long base = (long long)blockIdx.x * kSortTile;
 #pragma unroll 4
   for (int r = 0; r < kSortItems; ++r) {
     const long long i = base + r * kSortThreads + threadIdx.x;
-    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 0xFFu], 1u);
+    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & (uint32_t)(kD - 1)], 1u);
   }
   __syncthreads();
-  hist[(long long)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+  for (int d = threadIdx.x; d < kD; d += kSortThreads) hist[(long long)d * nb + blockIdx.x] = h[d];
 }
 
+template <typename K, int kBits>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
-    const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     const uint32_t* __restrict__ n_dev, long long n_cap, int shift,
-    const uint32_t* __restrict__ offsets, int nb, uint64_t* __restrict__ keys_out,
+    const uint32_t* __restrict__ offsets, int nb, K* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out) {
-  __shared__ uint32_t running[256];
-  __shared__ uint32_t wcnt[4][257];
+  constexpr int kD = 1 << kBits;
+  __shared__ uint32_t running[kD];
+  __shared__ uint32_t wcnt[4][kD + 1];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  running[t] = offsets[(long long)t * nb + blockIdx.x];
+  for (int d = t; d < kD; d += kSortThreads) running[d] = offsets[(long long)d * nb + blockIdx.x];
   const long long n = n_dev ? min((long long)*n_dev, n_cap) : n_cap;
   const long long base = (long long)blockIdx.x * kSortTile;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -281,21 +298,21 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     if (base + (long long)r * kSortThreads >= n) break;       // uniform across the block
     const long long i = base + r * kSortThreads + t;
     const bool valid = i < n;
-    uint64_t key = 0;
-    uint32_t val = 0, d = 256;
+    K key = 0;
+    uint32_t val = 0, d = kD;
     if (valid) {
       key = keys_in[i];
       val = vals_in[i];
-      d = (uint32_t)(key >> shift) & 0xFFu;
+      d = (uint32_t)(key >> shift) & (uint32_t)(kD - 1);
     }
     uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < kBits; ++b) {
       const uint64_t m = __ballot((d >> b) & 1u);
       peers &= ((d >> b) & 1u) ? m : ~m;
     }
     const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-    for (int k = lane; k < 257; k += 64) wcnt[w][k] = 0;
+    for (int k = lane; k < kD + 1; k += 64) wcnt[w][k] = 0;
     __syncthreads();
     if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
     __syncthreads();
@@ -305,7 +322,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
       for (int k = 0; k < w; ++k) dst += wcnt[k][d];
     }
     __syncthreads();
-    running[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+    for (int dd = t; dd < kD; dd += kSortThreads) running[dd] += wcnt[0][dd] + wcnt[1][dd] + wcnt[2][dd] + wcnt[3][dd];
     if (valid) {
       keys_out[dst] = key;
       vals_out[dst] = val;
@@ -331,6 +348,108 @@ __global__ __launch_bounds__(256) void tile_ranges_kernel(
   const uint32_t tile = (uint32_t)(keys[i] >> rank_bits);
   if (i == 0 || (uint32_t)(keys[i - 1] >> rank_bits) != tile) ranges[tile].x = (uint32_t)i;
   if (i == n - 1 || (uint32_t)(keys[i + 1] >> rank_bits) != tile) ranges[tile].y = (uint32_t)(i + 1);
+}
+
+// Tile ranges of a sorted list of 32-bit tile ids (ranges zero-filled first).
+__global__ __launch_bounds__(256) void tile_ranges_u32_kernel(
+    const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ n_dev, long long n_cap,
+    uint2* __restrict__ ranges) {
+  const long long n = n_dev ? min((long long)*n_dev, n_cap) : n_cap;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t tile = tiles[i];
+  if (i == 0 || tiles[i - 1] != tile) ranges[tile].x = (uint32_t)i;
+  if (i == n - 1 || tiles[i + 1] != tile) ranges[tile].y = (uint32_t)(i + 1);
+}
+
+// ------------------------------------------------------------------------------------------
+// Batched binning in rank order.  Within a tile the reference orders entries by the 48-bit
+// rank octpath ^ repeat3(octant) (auxiliary.h:40-59), a key that does not depend on the view.
+// Every (voxel, octant) pair is therefore sorted once by rank (pair_rank_kernel + the radix
+// sort; the order is fixed while the voxel set is), and each step writes its keys pair by pair
+// in that order and sorts them stably by global tile id only, which leaves each tile's entries
+// in the reference's order.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pair_rank_kernel(long long n_pairs, const int64_t* __restrict__ octpaths,
+                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_pairs) return;
+  const uint32_t q = (uint32_t)(i & 7);
+  keys[i] = ((uint64_t)octpaths[i >> 3] ^ rep3(q)) & ((1ull << kRankBits) - 1ull);
+  vals[i] = (uint32_t)i;                       // voxel << 3 | octant
+}
+
+// Voxel-major preprocess of all views: the voxel is read once, its bbox / octants / n_dup are
+// written per view, and per octant q the number of keys it emits over all views (tiles of the
+// views whose octant set holds q) goes to pair_cnt[8 * voxel + q].
+__global__ __launch_bounds__(256) void preprocess_views_kernel(
+    const ViewDesc* __restrict__ views, int V, int P, const float* __restrict__ centers,
+    const float* __restrict__ lengths, const uint8_t* __restrict__ is_leaf, uint32_t* __restrict__ n_dup,
+    uint2* __restrict__ bboxes, uint8_t* __restrict__ quads, uint32_t* __restrict__ pair_cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const bool leaf = is_leaf[i] != 0;
+  const float cx = centers[3 * i + 0], cy = centers[3 * i + 1], cz = centers[3 * i + 2];
+  const float len = lengths[i];
+  uint32_t c8[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  for (int v = 0; v < V; ++v) {
+    uint32_t nd, qbits;
+    uint2 bb;
+    preprocess_one(views[v], leaf, cx, cy, cz, len, nd, bb, qbits);
+    const long long o = (long long)v * P + i;
+    n_dup[o] = nd;
+    bboxes[o] = bb;
+    quads[o] = (uint8_t)qbits;
+    if (nd) {
+      const uint32_t nt = nd / (uint32_t)__popc(qbits);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) c8[q] += ((qbits >> q) & 1u) ? nt : 0u;
+    }
+  }
+  uint4* out = reinterpret_cast<uint4*>(pair_cnt + 8LL * i);
+  out[0] = make_uint4(c8[0], c8[1], c8[2], c8[3]);
+  out[1] = make_uint4(c8[4], c8[5], c8[6], c8[7]);
+}
+
+// Key counts in rank order.
+__global__ __launch_bounds__(256) void pair_gather_kernel(long long n_pairs, const uint32_t* __restrict__ order,
+                                                          const uint32_t* __restrict__ pair_cnt,
+                                                          uint32_t* __restrict__ cnt_ranked) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_pairs) cnt_ranked[i] = pair_cnt[order[i]];
+}
+
+// Keys of each (voxel, octant) pair at its rank-order offset: global tile id and value
+// octant << 29 | voxel, view by view and tile by tile.
+__global__ __launch_bounds__(256) void pair_emit_kernel(
+    const ViewDesc* __restrict__ views, int V, int P, long long n_pairs, const uint32_t* __restrict__ order,
+    const uint32_t* __restrict__ cnt_ranked, const uint32_t* __restrict__ scan_ranked,
+    const uint2* __restrict__ bboxes, const uint8_t* __restrict__ quads, uint32_t* __restrict__ tiles,
+    uint32_t* __restrict__ vals, long long cap) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_pairs) return;
+  const uint32_t c = cnt_ranked[i];
+  if (c == 0) return;
+  const uint32_t pv = order[i];
+  const uint32_t vox = pv >> 3, q = pv & 7u;
+  const uint32_t val = (q << 29) | vox;
+  uint32_t off = scan_ranked[i] - c;
+  for (int v = 0; v < V; ++v) {
+    const long long o = (long long)v * P + vox;
+    if (!((quads[o] >> q) & 1u)) continue;
+    const ViewDesc& vd = views[v];
+    const uint2 bb = bboxes[o];
+    int tx0, ty0, tx1, ty1;
+    bbox_tiles(bb.x, bb.y, vd.tiles_x, vd.tiles_y, tx0, ty0, tx1, ty1);
+    for (int y = ty0; y <= ty1; ++y)
+      for (int x = tx0; x <= tx1; ++x) {
+        if ((long long)off < cap) {
+          tiles[off] = (uint32_t)(vd.tile_base + y * vd.tiles_x + x);
+          vals[off] = val;
+        }
+        ++off;
+      }
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -390,14 +509,85 @@ hipError_t launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
   uint64_t* ki = keys; uint32_t* vi = vals;
   uint64_t* ko = keys_alt; uint32_t* vo = vals_alt;
   for (int shift = 0; shift < nbits; shift += 8) {
-    radix_histogram_kernel<<<nb, kSortThreads, 0, st>>>(ki, n_dev, n_cap, shift, hist, nb);
+    radix_histogram_kernel<uint64_t, 8><<<nb, kSortThreads, 0, st>>>(ki, n_dev, n_cap, shift, hist, nb);
     hipError_t e = launch_scan(hist, nullptr, hn, scan_partial, hist, false, st);
     if (e != hipSuccess) return e;
-    radix_scatter_kernel<<<nb, kSortThreads, 0, st>>>(ki, vi, n_dev, n_cap, shift, hist, nb, ko, vo);
+    radix_scatter_kernel<uint64_t, 8><<<nb, kSortThreads, 0, st>>>(ki, vi, n_dev, n_cap, shift, hist, nb, ko, vo);
     std::swap(ki, ko);
     std::swap(vi, vo);
     *result_in_alt = !*result_in_alt;
   }
+  return hipGetLastError();
+}
+
+int tile_sort_passes(long long n_tiles) {
+  int bits = 1;
+  while ((1LL << bits) < n_tiles) ++bits;
+  return (bits + kMaxDigitBits - 1) / kMaxDigitBits;
+}
+
+// Stable sort of (32-bit tile id, value) pairs by tile id, 9-bit digits.  The data starts in
+// the buffers that make the last pass end in (tiles, vals) (see tile_sort_passes).
+hipError_t launch_tile_sort(uint32_t* tiles, uint32_t* vals, uint32_t* tiles_alt, uint32_t* vals_alt,
+                            const uint32_t* n_dev, long long n_cap, long long n_tiles, uint32_t* hist,
+                            uint32_t* scan_partial, hipStream_t st) {
+  if (n_cap <= 0) return hipSuccess;
+  const int passes = tile_sort_passes(n_tiles);
+  const int nb = (int)sort_tile_count(n_cap);
+  const long long hn = (1LL << kMaxDigitBits) * nb;
+  uint32_t* ki = (passes % 2) ? tiles_alt : tiles;
+  uint32_t* vi = (passes % 2) ? vals_alt : vals;
+  uint32_t* ko = (passes % 2) ? tiles : tiles_alt;
+  uint32_t* vo = (passes % 2) ? vals : vals_alt;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * kMaxDigitBits;
+    radix_histogram_kernel<uint32_t, kMaxDigitBits><<<nb, kSortThreads, 0, st>>>(ki, n_dev, n_cap, shift, hist, nb);
+    hipError_t e = launch_scan(hist, nullptr, hn, scan_partial, hist, false, st);
+    if (e != hipSuccess) return e;
+    radix_scatter_kernel<uint32_t, kMaxDigitBits><<<nb, kSortThreads, 0, st>>>(ki, vi, n_dev, n_cap, shift, hist, nb,
+                                                                               ko, vo);
+    std::swap(ki, ko);
+    std::swap(vi, vo);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_rank(long long n_pairs, const int64_t* octpaths, uint64_t* keys, uint32_t* vals,
+                            hipStream_t st) {
+  if (n_pairs <= 0) return hipSuccess;
+  pair_rank_kernel<<<(unsigned)((n_pairs + 255) / 256), 256, 0, st>>>(n_pairs, octpaths, keys, vals);
+  return hipGetLastError();
+}
+
+hipError_t launch_preprocess_views(const ViewDesc* views, int V, int P, const float* centers, const float* lengths,
+                                   const uint8_t* is_leaf, uint32_t* n_dup, uint2* bboxes, uint8_t* quads,
+                                   uint32_t* pair_cnt, hipStream_t st) {
+  if (P <= 0 || V <= 0) return hipSuccess;
+  preprocess_views_kernel<<<ceil_div(P, 256), 256, 0, st>>>(views, V, P, centers, lengths, is_leaf, n_dup, bboxes,
+                                                            quads, pair_cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_gather(long long n_pairs, const uint32_t* order, const uint32_t* pair_cnt,
+                              uint32_t* cnt_ranked, hipStream_t st) {
+  if (n_pairs <= 0) return hipSuccess;
+  pair_gather_kernel<<<(unsigned)((n_pairs + 255) / 256), 256, 0, st>>>(n_pairs, order, pair_cnt, cnt_ranked);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_emit(const ViewDesc* views, int V, int P, long long n_pairs, const uint32_t* order,
+                            const uint32_t* cnt_ranked, const uint32_t* scan_ranked, const uint2* bboxes,
+                            const uint8_t* quads, uint32_t* tiles, uint32_t* vals, long long cap, hipStream_t st) {
+  if (n_pairs <= 0) return hipSuccess;
+  pair_emit_kernel<<<(unsigned)((n_pairs + 255) / 256), 256, 0, st>>>(views, V, P, n_pairs, order, cnt_ranked,
+                                                                      scan_ranked, bboxes, quads, tiles, vals, cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_ranges_u32(const uint32_t* tiles, const uint32_t* n_dev, long long n_cap, uint2* ranges,
+                                  hipStream_t st) {
+  if (n_cap <= 0) return hipSuccess;
+  tile_ranges_u32_kernel<<<(unsigned)((n_cap + 255) / 256), 256, 0, st>>>(tiles, n_dev, n_cap, ranges);
   return hipGetLastError();
 }
 

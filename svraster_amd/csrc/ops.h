@@ -16,7 +16,7 @@ hipError_t launch_sh_views_forward(int P, int V, const uint32_t* n_dup, const Vi
                                    const float* centers, const float* sh0, const float* shs, float* rgbs_views,
                                    hipStream_t st);
 hipError_t launch_sh_views_backward(int P, int V, const uint32_t* n_dup, const ViewDesc* views, int deg, int M,
-                                    const float* centers, const float* drgb_views, float* dsh0, float* dshs,
+                                    const float* centers, float* drgb_views, float* dsh0, float* dshs,
                                     hipStream_t st);
 hipError_t launch_voxel_activity(int P, const uint8_t* is_leaf, const float* geos, const float* s_val,
                                  const float* centers, const float* lengths, float cam_scale, uint8_t* out,

@@ -74,15 +74,6 @@ __device__ inline void wave_lds_sync_b() {
   __builtin_amdgcn_wave_barrier();
 }
 
-__device__ inline uint32_t wave_incl_scan_b(uint32_t x, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t t = (uint32_t)__shfl_up((int)x, o, 64);
-    if (lane >= o) x += t;
-  }
-  return x;
-}
-
 // Ray of another pixel of the wave: same origin, direction from LDS, inverse recomputed
 // exactly as pixel_ray computes it.
 __device__ inline Ray lane_ray(const Ray& r, const float4 d) {
@@ -340,20 +331,7 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
 // (tile, entry).
 // ------------------------------------------------------------------------------------------
 constexpr int kBwdStage = 128;
-constexpr int kBwdWin = 256;
-
-__device__ inline uint64_t transpose64_b(uint64_t x, int lane) {
-  int sh = 32;
-#pragma unroll
-  for (int k = 0; k < 6; ++k, sh >>= 1) {
-    const uint32_t plo = (uint32_t)__shfl_xor((int)(uint32_t)x, sh, 64);
-    const uint32_t phi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), sh, 64);
-    const uint64_t p = ((uint64_t)phi << 32) | plo;
-    const uint64_t m = ~0ull / ((1ull << sh) + 1ull);   // bit positions whose bit `sh` is clear
-    x = (lane & sh) ? ((x & ~m) | ((p & ~m) >> sh)) : ((x & m) | ((p & m) << sh));
-  }
-  return x;
-}
+constexpr int kBwdWin = 160;
 
 __device__ inline int pop_entry_b(uint64_t& lo, uint64_t& hi) {
   if (lo) {
@@ -387,15 +365,22 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
   __shared__ float4 s_dn[4][kNormal ? 64 : 1];
   __shared__ float4 s_gt[4][kConcen ? 64 : 1];
   __shared__ uint16_t s_desc[4][kBwdWin];  // entry slot | pixel << 7
-  __shared__ uint16_t s_cidx[4][kBwdWin];  // contributing pairs of the window
+  __shared__ uint16_t s_cmap[4][kBwdWin];  // window pairs that hit their box, then contributing pairs
+  __shared__ uint16_t s_cpos[4][kBwdWin + 1];  // window pair -> compact index (exclusive count)
+  __shared__ uint8_t s_own[4][kBwdWin];    // 1 + entry at the first position of its run, else 0
+  __shared__ uint64_t s_cm[4][kBwdStage];  // per entry: pixels of the block whose pair contributes
+  __shared__ uint16_t s_epref[4][kBwdStage];  // per entry: first entry-major position
+  __shared__ uint64_t s_csl[4][64], s_csh[4][64];  // per pixel: contributing entries (window)
+  __shared__ uint16_t s_coff[4][64];       // per pixel: first compact index (window)
   __shared__ float s_pa[4][kBwdWin];       // pair alpha (0 if the ray misses the box)
   __shared__ float s_pd[4][kDepth ? kBwdWin : 1];
-  __shared__ float s_pl[4][kBwdWin];       // dL/dalpha
-  __shared__ float s_pt[4][kBwdWin];       // T before the entry (negative: pair does not contribute)
+  __shared__ float s_pl[4][kBwdWin];       // dL/dalpha per contributing pair (compact index)
+  __shared__ float s_pt[4][kBwdWin];       // T before the entry per contributing pair
   __shared__ float s_red[4];
   __shared__ int s_tile;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const float s = *a.s_val;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
   float ds_acc = 0.f;
 
   for (int iter = 0;; ++iter) {
@@ -481,8 +466,8 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
       if (__ballot(live) != 0ull) {
         const uint64_t m_lo = lane < cnt ? entry_block_mask_b(s_bb, s_rgb, lane, qx, qy, s_om[w]) : 0ull;
         const uint64_t m_hi = lane + 64 < cnt ? entry_block_mask_b(s_bb, s_rgb, lane + 64, qx, qy, s_om[w]) : 0ull;
-        uint64_t e_lo = transpose64_b(m_lo, lane);
-        uint64_t e_hi = transpose64_b(m_hi, lane);
+        uint64_t e_lo = transpose64(m_lo, lane);
+        uint64_t e_hi = transpose64(m_hi, lane);
         if (!live) {
           e_lo = 0ull;
           e_hi = 0ull;
@@ -493,7 +478,7 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
           else if (jmin > 0) { e_lo &= ~0ull << jmin; }
         }
         const uint32_t np_lane = (uint32_t)(__popcll(e_lo) + __popcll(e_hi));
-        const uint32_t incl = wave_incl_scan_b(np_lane, lane);
+        const uint32_t incl = wave_incl_scan_u32(np_lane);
         const uint32_t off = incl - np_lane, endp = incl;
         const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         uint32_t cur = off;
@@ -508,47 +493,77 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
           }
           wave_lds_sync_b();
 
-          // 2. pair alphas
+          // 2. ray/box test per pair (misses get alpha 0, hits listed), then alpha (and depth
+          // value) per hit, 64 per instruction.
           const uint32_t np = w1 - w0;
-          for (uint32_t pp = lane; pp < np; pp += 64) {
+          uint32_t nh = 0;
+          for (uint32_t c0 = 0; c0 < np; c0 += 64) {
+            const uint32_t pp = c0 + lane;
+            bool hit = false;
+            if (pp < np) {
+              const uint32_t d = s_desc[w][pp];
+              const int j = (int)(d & 127u), bit = (int)(d >> 7);
+              const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
+              const float4 ch = s_ch[j];
+              float ta, tb;
+              ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
+              hit = !(ta > tb);
+              s_pa[w][pp] = 0.f;
+            }
+            const uint64_t bal = __ballot(hit);
+            if (hit) s_cmap[w][nh + (uint32_t)__popcll(bal & lt_mask)] = (uint16_t)pp;
+            nh += (uint32_t)__popcll(bal);
+          }
+          wave_lds_sync_b();
+          for (uint32_t k = lane; k < nh; k += 64) {
+            const uint32_t pp = s_cmap[w][k];
             const uint32_t d = s_desc[w][pp];
             const int j = (int)(d & 127u), bit = (int)(d >> 7);
-            Ray pr = r;
-            const float4 dv = s_rd[w][bit];
-            const float4 iv = s_ri[w][bit];
-            pr.dx = dv.x; pr.dy = dv.y; pr.dz = dv.z;
-            pr.ix = iv.x; pr.iy = iv.y; pr.iz = iv.z;
+            const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
             const float4 ch = s_ch[j];
             float ta, tb;
             ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
-            float alpha = 0.f, dval = 0.f;
-            if (!(ta > tb)) {
-              const float4 ml = s_ml[j];
-              const float4 g0 = s_g0[j], g1 = s_g1[j];
-              const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
-              Segment seg;
-              eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
-              alpha = seg.alpha;
-              if (kDepth) {
-                float sub[3];
-                sub_alphas(seg, sub);
-                dval = depth_value(sub, ta, tb);
-              }
+            const float4 ml = s_ml[j];
+            const float4 g0 = s_g0[j], g1 = s_g1[j];
+            const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+            Segment seg;
+            eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
+            s_pa[w][pp] = seg.alpha;
+            if (kDepth) {
+              float sub[3];
+              sub_alphas(seg, sub);
+              s_pd[w][pp] = depth_value(sub, ta, tb);
             }
-            s_pa[w][pp] = alpha;
-            if (kDepth) s_pd[w][pp] = dval;
           }
           wave_lds_sync_b();
 
-          // 3. sequential reverse walk per pixel over its run
+          // 2b. contributing pairs (alpha >= kMinAlpha) compacted in pixel-major order: s_cpos[pp]
+          // is the compact index of window pair pp, s_cmap the inverse.
+          uint32_t nc = 0;
+          for (uint32_t c0 = 0; c0 < np; c0 += 64) {
+            const uint32_t pp = c0 + lane;
+            const bool f = pp < np && s_pa[w][pp] >= kMinAlpha;
+            const uint64_t bal = __ballot(f);
+            const uint32_t rank = nc + (uint32_t)__popcll(bal & lt_mask);
+            if (pp < np) s_cpos[w][pp] = (uint16_t)rank;
+            if (f) s_cmap[w][rank] = (uint16_t)pp;
+            nc += (uint32_t)__popcll(bal);
+          }
+          if (lane == 0) s_cpos[w][np] = (uint16_t)nc;
+          wave_lds_sync_b();
+
+          // 3. sequential reverse walk per pixel over its contributing pairs
           const uint32_t r0 = max(off, w0), r1 = min(endp, w1);
-          for (uint32_t i = r0; i < r1; ++i) {
-            const float alpha = s_pa[w][i - w0];
-            if (alpha < kMinAlpha) {
-              s_pt[w][i - w0] = -1.f;
-              continue;
-            }
-            const int j = (int)(s_desc[w][i - w0] & 127u);
+          uint32_t cb = 0, ce = 0;
+          if (r0 < r1) {
+            cb = s_cpos[w][r0 - w0];
+            ce = s_cpos[w][r1 - w0];
+          }
+          uint64_t cs_lo = 0ull, cs_hi = 0ull;   // entries whose pair with this pixel contributes
+          for (uint32_t c = cb; c < ce; ++c) {
+            const uint32_t pp = s_cmap[w][c];
+            const float alpha = s_pa[w][pp];
+            const int j = (int)(s_desc[w][pp] & 127u);
             T = T / (1.f - alpha);
             const float4 rgb = s_rgb[j];
             float dptw = dC0 * rgb.x + dC1 * rgb.y + dC2 * rgb.z;
@@ -558,112 +573,142 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
             }
             const float dLda = T * (dptw - last_dT);
             last_dT += alpha * (dptw - last_dT);
-            if (kDepth) last_dT += dD * s_pd[w][i - w0];
-            s_pl[w][i - w0] = dLda;
-            s_pt[w][i - w0] = T;
+            if (kDepth) last_dT += dD * s_pd[w][pp];
+            s_pl[w][c] = dLda;
+            s_pt[w][c] = T;
+            if (j < 64) cs_lo |= 1ull << j; else cs_hi |= 1ull << (j - 64);
           }
+
+          // 4a. entry-major order of the contributing pairs: per entry the mask of its pixels
+          // (transpose of the per-pixel sets), their prefix, and the entry of each position.
+          s_coff[w][lane] = (uint16_t)cb;
+          s_csl[w][lane] = cs_lo;
+          s_csh[w][lane] = cs_hi;
+          const uint64_t cm_lo = transpose64(cs_lo, lane);   // pixels of entry `lane`
+          const uint64_t cm_hi = transpose64(cs_hi, lane);   // pixels of entry `lane + 64`
+          const uint32_t n_lo = (uint32_t)__popcll(cm_lo), n_hi = (uint32_t)__popcll(cm_hi);
+          const uint32_t i_lo = wave_incl_scan_u32(n_lo);
+          const uint32_t i_hi = wave_incl_scan_u32(n_hi);
+          const uint32_t t_lo = (uint32_t)__builtin_amdgcn_readlane((int)i_lo, 63);
+          const uint32_t p_lo = i_lo - n_lo, p_hi = t_lo + i_hi - n_hi;
+          s_cm[w][lane] = cm_lo;
+          s_cm[w][lane + 64] = cm_hi;
+          s_epref[w][lane] = (uint16_t)p_lo;
+          s_epref[w][lane + 64] = (uint16_t)p_hi;
+          for (uint32_t c = lane; c < nc; c += 64) s_own[w][c] = 0;
+          wave_lds_sync_b();
+          if (n_lo) s_own[w][p_lo] = (uint8_t)(lane + 1);
+          if (n_hi) s_own[w][p_hi] = (uint8_t)(lane + 65);
           wave_lds_sync_b();
 
-          // 3b. compact the contributing pairs of the window
-          uint32_t nc = 0;
-          for (uint32_t c0 = 0; c0 < np; c0 += 64) {
-            const uint32_t pp = c0 + lane;
-            const bool f = pp < np && s_pt[w][pp] >= 0.f;
-            const uint64_t b = __ballot(f);
-            if (f) s_cidx[w][nc + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))] = (uint16_t)pp;
-            nc += (uint32_t)__popcll(b);
-          }
-          wave_lds_sync_b();
-
-          // 4. per-pair derivatives, accumulated per entry in LDS
-          for (uint32_t k = lane; k < nc; k += 64) {
-            const uint32_t pp = s_cidx[w][k];
-            const uint32_t d = s_desc[w][pp];
-            const int j = (int)(d & 127u), bit = (int)(d >> 7);
-            const float Tp = s_pt[w][pp];
-            const float4 rdd = s_rd[w][bit];
-            const float4 iv = s_ri[w][bit];
-            Ray pr = r;
-            pr.dx = rdd.x; pr.dy = rdd.y; pr.dz = rdd.z;
-            pr.ix = iv.x; pr.iy = iv.y; pr.iz = iv.z;
-            const float4 ch = s_ch[j];
-            float ta, tb;
-            ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
-            const float4 ml = s_ml[j];
-            const float4 g0 = s_g0[j], g1 = s_g1[j];
-            const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
-            Segment seg;
-            eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
-            const float alpha = seg.alpha;
-            const float dLda = s_pl[w][pp];
-            const float ptw = alpha * Tp;
+          // 4b. per-pair derivatives, 64 pairs per instruction in entry-major order; each entry's
+          // run is summed across lanes (segmented DPP sum) and its last lane adds the 12 sums to
+          // the workgroup's LDS accumulators.
+          uint32_t carry = 0;
+          for (uint32_t c0 = 0; c0 < nc; c0 += 64) {
+            const uint32_t p = c0 + lane;
+            const bool valid = p < nc;
+            uint32_t h = valid ? (uint32_t)s_own[w][p] : 0u;
+            h = max(wave_incl_max_u32(h), carry);
+            carry = (uint32_t)__builtin_amdgcn_readlane((int)h, 63);
             float gv[12];
 #pragma unroll
             for (int q = 0; q < 12; ++q) gv[q] = 0.f;
-            const float p0v = seg.phi[0], p3v = seg.phi[3];
-            const float rden = 1.f / (p0v + kPhiEps);
-            const float q0 = p0v * (1.f - p0v), q3 = p3v * (1.f - p3v);
-            float c[4];
-            c[0] = dLda * s * q0 * (1.f - alpha) * rden;
-            c[1] = 0.f;
-            c[2] = 0.f;
-            c[3] = -dLda * s * q3 * rden;
-            ds_acc += dLda * (q0 * seg.sdf[0] * (1.f - alpha) - q3 * seg.sdf[3]) * rden;
-            if (kDepth) {
-              float sub[3];
-              sub_alphas(seg, sub);
-              const float step = kThird * (tb - ta);
-              const float t0 = ta + 0.5f * step, t1 = ta + 1.5f * step, t2 = ta + 2.5f * step;
-              const float a0 = sub[0], a1 = sub[1], a2 = sub[2];
-              const float kk = rdd.w * Tp;
-              const float gk[3] = {kk * (t0 + a1 * a2 * t2 - a1 * t1 - a2 * t2),
-                                   kk * (t1 + a0 * a2 * t2 - a0 * t1 - a2 * t2),
-                                   kk * (t2 + a0 * a1 * t2 - a0 * t2 - a1 * t2)};
+            int j = 0, seg_start = lane;
+            bool tail = false;
+            if (valid) {
+              j = (int)h - 1;
+              const uint32_t k = p - (uint32_t)s_epref[w][j];
+              const uint64_t cm = s_cm[w][j];
+              const int bit = select_set_bit(cm, (int)k);
+              tail = k + 1 == (uint32_t)__popcll(cm) || lane == 63 || p + 1 == nc;
+              seg_start = lane - (int)min(k, (uint32_t)lane);
+              const uint64_t bl = s_csl[w][bit], bh = s_csh[w][bit];
+              const uint32_t below = j < 64 ? (uint32_t)__popcll(bl & ((1ull << j) - 1ull))
+                                            : (uint32_t)(__popcll(bl) + __popcll(bh & ((1ull << (j - 64)) - 1ull)));
+              const uint32_t ci = (uint32_t)s_coff[w][bit] + below;
+              const float Tp = s_pt[w][ci];
+              const float4 rdd = s_rd[w][bit];
+              const Ray pr = pair_ray(r, rdd, s_ri[w][bit]);
+              const float4 ch = s_ch[j];
+              float ta, tb;
+              ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
+              const float4 ml = s_ml[j];
+              const float4 g0 = s_g0[j], g1 = s_g1[j];
+              const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+              Segment seg;
+              eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
+              const float alpha = seg.alpha;
+              const float dLda = s_pl[w][ci];
+              const float ptw = alpha * Tp;
+              const float p0v = seg.phi[0], p3v = seg.phi[3];
+              const float rden = 1.f / (p0v + kPhiEps);
+              const float q0 = p0v * (1.f - p0v), q3 = p3v * (1.f - p3v);
+              float c[4];
+              c[0] = dLda * s * q0 * (1.f - alpha) * rden;
+              c[1] = 0.f;
+              c[2] = 0.f;
+              c[3] = -dLda * s * q3 * rden;
+              ds_acc += dLda * (q0 * seg.sdf[0] * (1.f - alpha) - q3 * seg.sdf[3]) * rden;
+              if (kDepth) {
+                float sub[3];
+                sub_alphas(seg, sub);
+                const float step = kThird * (tb - ta);
+                const float t0 = ta + 0.5f * step, t1 = ta + 1.5f * step, t2 = ta + 2.5f * step;
+                const float a0 = sub[0], a1 = sub[1], a2 = sub[2];
+                const float kk = rdd.w * Tp;
+                const float gk[3] = {kk * (t0 + a1 * a2 * t2 - a1 * t1 - a2 * t2),
+                                     kk * (t1 + a0 * a2 * t2 - a0 * t1 - a2 * t2),
+                                     kk * (t2 + a0 * a1 * t2 - a0 * t2 - a1 * t2)};
 #pragma unroll
-              for (int mm = 0; mm < 3; ++mm) {
-                const float pk = seg.phi[mm], pk1 = seg.phi[mm + 1];
-                const float dk = 1.f / (pk + kPhiEps);
-                if ((pk - pk1) * dk > 0.f) {
-                  c[mm] += gk[mm] * pk1 * dk * dk * s * pk * (1.f - pk);
-                  c[mm + 1] += -gk[mm] * dk * s * pk1 * (1.f - pk1);
+                for (int mm = 0; mm < 3; ++mm) {
+                  const float pk = seg.phi[mm], pk1 = seg.phi[mm + 1];
+                  const float dk = 1.f / (pk + kPhiEps);
+                  if ((pk - pk1) * dk > 0.f) {
+                    c[mm] += gk[mm] * pk1 * dk * dk * s * pk * (1.f - pk);
+                    c[mm + 1] += -gk[mm] * dk * s * pk1 * (1.f - pk1);
+                  }
                 }
               }
-            }
 #pragma unroll
-            for (int mm = 0; mm < 4; ++mm) {
-              const float fm = (float)mm;
-              float wts[8];
-              trilinear_weights(seg.qx + seg.sx * fm, seg.qy + seg.sy * fm, seg.qz + seg.sz * fm, wts);
+              for (int mm = 0; mm < 4; ++mm) {
+                const float fm = (float)mm;
+                float wts[8];
+                trilinear_weights(seg.qx + seg.sx * fm, seg.qy + seg.sy * fm, seg.qz + seg.sz * fm, wts);
 #pragma unroll
-              for (int i = 0; i < 8; ++i) gv[i] += c[mm] * wts[i];
-            }
-            const float4 dcv = s_dc[w][bit];
-            if (kNormal) {
-              const float4 nn = s_nrm[j];
-              const float4 dnv = s_dn[w][bit];
-              const float sx = ptw * dnv.x, sy = ptw * dnv.y, sz = ptw * dnv.z;
-              const float pr2 = sx * nn.x + sy * nn.y + sz * nn.z;
-              const float lx = nn.w * (sx - pr2 * nn.x);
-              const float ly = nn.w * (sy - pr2 * nn.y);
-              const float lz = nn.w * (sz - pr2 * nn.z);
+                for (int i = 0; i < 8; ++i) gv[i] += c[mm] * wts[i];
+              }
+              const float4 dcv = s_dc[w][bit];
+              if (kNormal) {
+                const float4 nn = s_nrm[j];
+                const float4 dnv = s_dn[w][bit];
+                const float sx = ptw * dnv.x, sy = ptw * dnv.y, sz = ptw * dnv.z;
+                const float pr2 = sx * nn.x + sy * nn.y + sz * nn.z;
+                const float lx = nn.w * (sx - pr2 * nn.x);
+                const float ly = nn.w * (sy - pr2 * nn.y);
+                const float lz = nn.w * (sz - pr2 * nn.z);
 #pragma unroll
-              for (int i = 0; i < 8; ++i)
-                gv[i] += ((i & 4) ? lx : -lx) + ((i & 2) ? ly : -ly) + ((i & 1) ? lz : -lz);
+                for (int i = 0; i < 8; ++i)
+                  gv[i] += ((i & 4) ? lx : -lx) + ((i & 2) ? ly : -ly) + ((i & 1) ? lz : -lz);
+              }
+              gv[8] = ptw * dcv.x;
+              gv[9] = ptw * dcv.y;
+              gv[10] = ptw * dcv.z;
+              if (kConcen) {
+                const float4 gtv = s_gt[w][bit];
+                const float4 rgb = s_rgb[j];
+                gv[8] += wR * ptw * 2.f * (rgb.x - gtv.x);
+                gv[9] += wR * ptw * 2.f * (rgb.y - gtv.y);
+                gv[10] += wR * ptw * 2.f * (rgb.z - gtv.z);
+              }
+              gv[11] = fabsf(dLda * alpha);
             }
-            gv[8] = ptw * dcv.x;
-            gv[9] = ptw * dcv.y;
-            gv[10] = ptw * dcv.z;
-            if (kConcen) {
-              const float4 gtv = s_gt[w][bit];
-              const float4 rgb = s_rgb[j];
-              gv[8] += wR * ptw * 2.f * (rgb.x - gtv.x);
-              gv[9] += wR * ptw * 2.f * (rgb.y - gtv.y);
-              gv[10] += wR * ptw * 2.f * (rgb.z - gtv.z);
-            }
-            gv[11] = fabsf(dLda * alpha);
+            wave_segmented_sum<12>(gv, lane, seg_start);
+            if (valid && tail) {
 #pragma unroll
-            for (int q = 0; q < 12; ++q) atomicAdd(&s_acc[j][q], gv[q]);
-            s_used[j] = 1u;
+              for (int q = 0; q < 12; ++q) atomicAdd(&s_acc[j][q], gv[q]);
+              s_used[j] = 1u;
+            }
           }
           wave_lds_sync_b();
         }

@@ -126,6 +126,7 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
   __shared__ int s_tile;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const float s = *a.s_val;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
 
   for (int iter = 0;; ++iter) {
     int g;
@@ -230,22 +231,7 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
 // identical bit for bit.
 // ------------------------------------------------------------------------------------------
 constexpr int kPairStage = 128;   // entries staged per workgroup round (two 64-entry halves)
-constexpr int kPairWin = 512;     // pair results held per wave at once
-
-// Transposes a 64x64 bit matrix held one row per lane (lane i, bit b = A[i][b]): lane b returns
-// column b (bit i = A[i][b]).  Six butterfly stages swap the off-diagonal blocks of size 32..1.
-__device__ inline uint64_t transpose64(uint64_t x, int lane) {
-  int sh = 32;
-#pragma unroll
-  for (int k = 0; k < 6; ++k, sh >>= 1) {
-    const uint32_t plo = (uint32_t)__shfl_xor((int)(uint32_t)x, sh, 64);
-    const uint32_t phi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), sh, 64);
-    const uint64_t p = ((uint64_t)phi << 32) | plo;
-    const uint64_t m = ~0ull / ((1ull << sh) + 1ull);
-    x = (lane & sh) ? ((x & ~m) | ((p & ~m) >> sh)) : ((x & m) | ((p & m) << sh));
-  }
-  return x;
-}
+constexpr int kPairWin = 256;     // pair results held per wave at once
 
 // Pixels of the wave's 8x8 block (bit = 8 * row + column) inside a packed bbox.
 __device__ inline uint64_t bbox_block_mask(uint32_t b0, uint32_t b1, int qx, int qy) {
@@ -264,15 +250,6 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// Inclusive wave64 scan of u32.
-__device__ inline uint32_t wave_incl_scan_u32(uint32_t x, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t t = (uint32_t)__shfl_up((int)x, o, 64);
-    if (lane >= o) x += t;
-  }
-  return x;
-}
 
 // Mask of entry slot j for the wave's block: bbox, octant of the pixel rays, live pixels.
 __device__ inline uint64_t entry_block_mask(const uint2* s_bb, const float4* s_rgb, int j, int qx, int qy,
@@ -303,12 +280,15 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
   __shared__ float4 s_ri[4][64];          // 1/direction xyz
   __shared__ uint64_t s_om[4][8];            // pixels of the block whose ray has octant q
   __shared__ uint16_t s_desc[4][kPairWin];   // entry slot | pixel << 7
+  __shared__ uint16_t s_list[4][kPairWin];   // window pairs that hit their box, then contributing pairs
+  __shared__ uint16_t s_cpos[4][kPairWin + 1];  // window pair -> contributing index (exclusive count)
   __shared__ float s_pa[4][kPairWin];        // pair alpha (0 when the ray misses the box)
   __shared__ float s_pd[4][kDepth ? kPairWin : 1];
   __shared__ uint32_t s_red[4];
   __shared__ int s_tile;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const float s = *a.s_val;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
 
   for (int iter = 0;; ++iter) {
     int g;
@@ -370,7 +350,7 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
       uint64_t e_lo = transpose64(m_lo, lane);
       uint64_t e_hi = transpose64(m_hi, lane);
       const uint32_t np_lane = (uint32_t)(__popcll(e_lo) + __popcll(e_hi));
-      const uint32_t incl = wave_incl_scan_u32(np_lane, lane);
+      const uint32_t incl = wave_incl_scan_u32(np_lane);
       const uint32_t off = incl - np_lane, endp = incl;
       const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
       uint32_t cur = off;
@@ -385,48 +365,79 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
           ++cur;
         }
         wave_lds_sync();
-        // 3b. alpha (and depth value) per pair
+        // 3b. ray/box test per pair; misses get alpha 0, hits are listed for the SDF segment.
         const uint32_t np = w1 - w0;
-        for (uint32_t pp = lane; pp < np; pp += 64) {
+        uint32_t nh = 0;
+        for (uint32_t c0 = 0; c0 < np; c0 += 64) {
+          const uint32_t pp = c0 + lane;
+          bool hit = false;
+          if (pp < np) {
+            const uint32_t d = s_desc[w][pp];
+            const int j = (int)(d & 127u), bit = (int)(d >> 7);
+            const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
+            const float4 ch = s_ch[j];
+            float ta, tb;
+            ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
+            hit = !(ta > tb);
+            s_pa[w][pp] = 0.f;
+          }
+          const uint64_t bal = __ballot(hit);
+          if (hit) s_list[w][nh + (uint32_t)__popcll(bal & lt_mask)] = (uint16_t)pp;
+          nh += (uint32_t)__popcll(bal);
+        }
+        wave_lds_sync();
+        // 3c. alpha (and depth value) per hit, 64 per instruction
+        for (uint32_t k = lane; k < nh; k += 64) {
+          const uint32_t pp = s_list[w][k];
           const uint32_t d = s_desc[w][pp];
           const int j = (int)(d & 127u), bit = (int)(d >> 7);
-          Ray pr = r;
-          const float4 dv = s_rd[w][bit];
-          const float4 iv = s_ri[w][bit];
-          pr.dx = dv.x; pr.dy = dv.y; pr.dz = dv.z;
-          pr.ix = iv.x; pr.iy = iv.y; pr.iz = iv.z;
+          const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
           const float4 ch = s_ch[j];
           float ta, tb;
           ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
-          float alpha = 0.f, dval = 0.f;
-          if (!(ta > tb)) {
-            const float4 ml = s_ml[j];
-            const float4 g0 = s_g0[j], g1 = s_g1[j];
-            const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
-            Segment seg;
-            eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
-            alpha = seg.alpha;
-            if (kDepth) {
-              float sub[3];
-              sub_alphas(seg, sub);
-              dval = depth_value(sub, ta, tb);
-            }
+          const float4 ml = s_ml[j];
+          const float4 g0 = s_g0[j], g1 = s_g1[j];
+          const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+          Segment seg;
+          eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
+          s_pa[w][pp] = seg.alpha;
+          if (kDepth) {
+            float sub[3];
+            sub_alphas(seg, sub);
+            s_pd[w][pp] = depth_value(sub, ta, tb);
           }
-          s_pa[w][pp] = alpha;
-          if (kDepth) s_pd[w][pp] = dval;
         }
         wave_lds_sync();
-        // 4. composite this pixel's run inside the window, in entry order
+        // 3d. contributing pairs (alpha >= kMinAlpha) compacted in pixel-major order: s_cpos[pp]
+        // is the compact index of window pair pp, s_list the inverse.
+        uint32_t nc = 0;
+        for (uint32_t c0 = 0; c0 < np; c0 += 64) {
+          const uint32_t pp = c0 + lane;
+          const bool f = pp < np && s_pa[w][pp] >= kMinAlpha;
+          const uint64_t bal = __ballot(f);
+          const uint32_t rank = nc + (uint32_t)__popcll(bal & lt_mask);
+          if (pp < np) s_cpos[w][pp] = (uint16_t)rank;
+          if (f) s_list[w][rank] = (uint16_t)pp;
+          nc += (uint32_t)__popcll(bal);
+        }
+        if (lane == 0) s_cpos[w][np] = (uint16_t)nc;
+        wave_lds_sync();
+        // 4. composite this pixel's contributing pairs inside the window, in entry order
         const uint32_t r0 = max(off, w0), r1 = min(endp, w1);
-        for (uint32_t i = r0; i < r1 && !done; ++i) {
-          const float alpha = s_pa[w][i - w0];
-          if (alpha < kMinAlpha) continue;
-          const int j = (int)(s_desc[w][i - w0] & 127u);
+        uint32_t cb = 0, ce = 0;
+        if (r0 < r1) {
+          cb = s_cpos[w][r0 - w0];
+          ce = s_cpos[w][r1 - w0];
+        }
+        for (uint32_t c = cb; c < ce && !done; ++c) {
+          const uint32_t pp = s_list[w][c];
+          const float alpha = s_pa[w][pp];
+          const int j = (int)(s_desc[w][pp] & 127u);
           const float4 rgb = s_rgb[j];
           const float wgt = T * alpha;
           C0 += wgt * rgb.x; C1 += wgt * rgb.y; C2 += wgt * rgb.z;
           if (kDepth) {
-            D += T * s_pd[w][i - w0];
+            D += T * s_pd[w][pp];
             if (T > 0.5f) { dmed_vox = (int)(__float_as_uint(rgb.w) & kVoxMask); dmed_T = T; }
           }
           if (kNormal) {

@@ -76,7 +76,7 @@ struct BinLayout {
     vals = c.take<uint32_t>(K);
     keys_alt = c.take<uint64_t>(K);
     vals_alt = c.take<uint32_t>(K);
-    const long long hn = 256LL * (long long)sort_tile_count(K);
+    const long long hn = 512LL * (long long)sort_tile_count(K);   // up to 9-bit digits
     hist = c.take<uint32_t>(hn);
     partial = c.take<uint32_t>((long long)scan_partial_count(hn) + 1);
     total = c.off;
@@ -133,6 +133,36 @@ struct ImagePtrs {
   }
 };
 
+// Batched rank-order binning state (V views, P voxels): per (voxel, octant) pair its key count
+// over all views, the counts in rank order and their inclusive scan.
+struct PairLayout {
+  size_t pair_cnt, cnt_ranked, scan_ranked, partial, total;
+  explicit PairLayout(long long P) {
+    Carver c;
+    const long long n = 8 * P;
+    pair_cnt = c.take<uint32_t>(n);
+    cnt_ranked = c.take<uint32_t>(n);
+    scan_ranked = c.take<uint32_t>(n);
+    partial = c.take<uint32_t>((long long)scan_partial_count(n) + 1);
+    total = c.off;
+  }
+};
+
+struct PairPtrs {
+  uint32_t* pair_cnt;
+  uint32_t* cnt_ranked;
+  uint32_t* scan_ranked;
+  uint32_t* partial;
+  PairPtrs(void* base, long long P) {
+    PairLayout L(P);
+    char* b = (char*)base;
+    pair_cnt = (uint32_t*)(b + L.pair_cnt);
+    cnt_ranked = (uint32_t*)(b + L.cnt_ranked);
+    scan_ranked = (uint32_t*)(b + L.scan_ranked);
+    partial = (uint32_t*)(b + L.partial);
+  }
+};
+
 // Launch helpers implemented in integer_stage.hip.
 hipError_t launch_init_view(ViewDesc* out, const ViewDesc& base, const float* w2c, const float* c2w,
                             const float* bg, hipStream_t st);
@@ -152,6 +182,22 @@ hipError_t launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
 hipError_t launch_fill_u32(uint32_t* p, long long n, uint32_t v, hipStream_t st);
 hipError_t launch_tile_ranges(const uint64_t* keys, const uint32_t* n_dev, long long n_cap, int rank_bits,
                               uint2* ranges, hipStream_t st);
+int tile_sort_passes(long long n_tiles);
+hipError_t launch_tile_sort(uint32_t* tiles, uint32_t* vals, uint32_t* tiles_alt, uint32_t* vals_alt,
+                            const uint32_t* n_dev, long long n_cap, long long n_tiles, uint32_t* hist,
+                            uint32_t* scan_partial, hipStream_t st);
+hipError_t launch_pair_rank(long long n_pairs, const int64_t* octpaths, uint64_t* keys, uint32_t* vals,
+                            hipStream_t st);
+hipError_t launch_preprocess_views(const ViewDesc* views, int V, int P, const float* centers, const float* lengths,
+                                   const uint8_t* is_leaf, uint32_t* n_dup, uint2* bboxes, uint8_t* quads,
+                                   uint32_t* pair_cnt, hipStream_t st);
+hipError_t launch_pair_gather(long long n_pairs, const uint32_t* order, const uint32_t* pair_cnt,
+                              uint32_t* cnt_ranked, hipStream_t st);
+hipError_t launch_pair_emit(const ViewDesc* views, int V, int P, long long n_pairs, const uint32_t* order,
+                            const uint32_t* cnt_ranked, const uint32_t* scan_ranked, const uint2* bboxes,
+                            const uint8_t* quads, uint32_t* tiles, uint32_t* vals, long long cap, hipStream_t st);
+hipError_t launch_tile_ranges_u32(const uint32_t* tiles, const uint32_t* n_dev, long long n_cap, uint2* ranges,
+                                  hipStream_t st);
 
 // Render launch helpers implemented in render_fwd.hip / render_bwd.hip.
 struct FwdArgs {

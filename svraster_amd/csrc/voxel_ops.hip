@@ -134,6 +134,20 @@ __global__ __launch_bounds__(256) void gather_geo_bw_kernel(int n, const int64_t
 
 // Batched views: colour of every (view, visible voxel) pair, direction from that view's camera.
 // One voxel per thread, looping over the views, so each voxel's coefficients are read once.
+// The SH gradient rows of a block of 256 voxels are contiguous ([P, M-1, 3]); the backward
+// writes them through LDS so that the global stores are coalesced.  kShRowMax = 45 floats holds a
+// degree-3 row (M <= 16); the odd stride keeps the per-thread LDS writes free of bank conflicts.
+constexpr int kShRowMax = 45;
+
+// Bit k set when view v0 + k sees voxel i (n_dup > 0); the loads are independent, so they are
+// all in flight at once instead of one dependent load per view.
+__device__ inline uint32_t visible_views(const uint32_t* __restrict__ n_dup, int P, int i, int v0, int nv) {
+  uint32_t vis = 0u;
+#pragma unroll 8
+  for (int k = 0; k < nv; ++k) vis |= (n_dup[(long long)(v0 + k) * P + i] != 0u ? 1u : 0u) << k;
+  return vis;
+}
+
 template <int kDeg>
 __global__ __launch_bounds__(256) void sh_views_forward_kernel(int P, int V, const uint32_t* __restrict__ n_dup,
                                                                const ViewDesc* __restrict__ views, int M,
@@ -151,64 +165,83 @@ __global__ __launch_bounds__(256) void sh_views_forward_kernel(int P, int V, con
   for (int k = 1; k < NB; ++k) {
     c[k][0] = cs[3 * (k - 1)]; c[k][1] = cs[3 * (k - 1) + 1]; c[k][2] = cs[3 * (k - 1) + 2];
   }
-  for (int v = 0; v < V; ++v) {
-    const long long o = (long long)v * P + i;
-    if (n_dup[o] == 0) continue;
-    const float* c2w = views[v].c2w;
-    const float cam[3] = {c2w[3], c2w[7], c2w[11]};
-    float x, y, z;
-    sh_dir(centers, cam, i, x, y, z);
-    float b[16];
-    sh_basis(kDeg, x, y, z, b);
-    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+  for (int v0 = 0; v0 < V; v0 += 32) {
+    const uint32_t vis = visible_views(n_dup, P, i, v0, min(32, V - v0));
+    for (uint32_t m = vis; m; m &= m - 1u) {
+      const int v = v0 + __builtin_ctz(m);
+      const long long o = (long long)v * P + i;
+      const float* c2w = views[v].c2w;
+      const float cam[3] = {c2w[3], c2w[7], c2w[11]};
+      float x, y, z;
+      sh_dir(centers, cam, i, x, y, z);
+      float b[16];
+      sh_basis(kDeg, x, y, z, b);
+      float r0 = 0.f, r1 = 0.f, r2 = 0.f;
 #pragma unroll
-    for (int k = 0; k < NB; ++k) {
-      r0 += b[k] * c[k][0]; r1 += b[k] * c[k][1]; r2 += b[k] * c[k][2];
+      for (int k = 0; k < NB; ++k) {
+        r0 += b[k] * c[k][0]; r1 += b[k] * c[k][1]; r2 += b[k] * c[k][2];
+      }
+      rgbs_views[3 * o] = fmaxf(r0 + 0.5f, 0.f);
+      rgbs_views[3 * o + 1] = fmaxf(r1 + 0.5f, 0.f);
+      rgbs_views[3 * o + 2] = fmaxf(r2 + 0.5f, 0.f);
     }
-    rgbs_views[3 * o] = fmaxf(r0 + 0.5f, 0.f);
-    rgbs_views[3 * o + 1] = fmaxf(r1 + 0.5f, 0.f);
-    rgbs_views[3 * o + 2] = fmaxf(r2 + 0.5f, 0.f);
   }
 }
 
 // Transpose summed over the views, one voxel per thread: no atomics, and every output row is
-// written (zeros for voxels no view saw), so the outputs need no separate clearing.
+// written (zeros for voxels no view saw), so the outputs need no separate clearing.  The
+// per-view colour gradients it consumes are reset to zero, so the backward compositor's
+// accumulation buffer is clear again for the next step without a separate fill.
 template <int kDeg>
 __global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, const uint32_t* __restrict__ n_dup,
                                                                 const ViewDesc* __restrict__ views, int M,
                                                                 const float* __restrict__ centers,
-                                                                const float* __restrict__ drgb_views,
+                                                                float* __restrict__ drgb_views,
                                                                 float* __restrict__ dsh0,
                                                                 float* __restrict__ dshs) {
   constexpr int NB = (kDeg + 1) * (kDeg + 1);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P) return;
-  float acc[NB][3];
+  __shared__ float s_rows[256 * kShRowMax];
+  const int t = threadIdx.x;
+  const long long i0 = (long long)blockIdx.x * 256;
+  const int nv = (int)min(256LL, (long long)P - i0);
+  const int row = (M - 1) * 3;
+  if (t < nv) {
+    const int i = (int)i0 + t;
+    float acc[NB][3];
 #pragma unroll
-  for (int k = 0; k < NB; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.f;
-  for (int v = 0; v < V; ++v) {
-    const long long o = (long long)v * P + i;
-    if (n_dup[o] == 0) continue;
-    const float g0 = drgb_views[3 * o], g1 = drgb_views[3 * o + 1], g2 = drgb_views[3 * o + 2];
-    const float* c2w = views[v].c2w;
-    const float cam[3] = {c2w[3], c2w[7], c2w[11]};
-    float x, y, z;
-    sh_dir(centers, cam, i, x, y, z);
-    float b[16];
-    sh_basis(kDeg, x, y, z, b);
+    for (int k = 0; k < NB; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.f;
+    for (int v0 = 0; v0 < V; v0 += 32) {
+      const uint32_t vis = visible_views(n_dup, P, i, v0, min(32, V - v0));
+      for (uint32_t m = vis; m; m &= m - 1u) {
+        const int v = v0 + __builtin_ctz(m);
+        const long long o = (long long)v * P + i;
+        const float g0 = drgb_views[3 * o], g1 = drgb_views[3 * o + 1], g2 = drgb_views[3 * o + 2];
+        drgb_views[3 * o] = 0.f;
+        drgb_views[3 * o + 1] = 0.f;
+        drgb_views[3 * o + 2] = 0.f;
+        const float* c2w = views[v].c2w;
+        const float cam[3] = {c2w[3], c2w[7], c2w[11]};
+        float x, y, z;
+        sh_dir(centers, cam, i, x, y, z);
+        float b[16];
+        sh_basis(kDeg, x, y, z, b);
 #pragma unroll
-    for (int k = 0; k < NB; ++k) {
-      acc[k][0] += b[k] * g0; acc[k][1] += b[k] * g1; acc[k][2] += b[k] * g2;
+        for (int k = 0; k < NB; ++k) {
+          acc[k][0] += b[k] * g0; acc[k][1] += b[k] * g1; acc[k][2] += b[k] * g2;
+        }
+      }
     }
+    dsh0[3 * i] = acc[0][0]; dsh0[3 * i + 1] = acc[0][1]; dsh0[3 * i + 2] = acc[0][2];
+    float* c = s_rows + t * kShRowMax;
+#pragma unroll
+    for (int k = 1; k < NB; ++k) {
+      c[3 * (k - 1)] = acc[k][0]; c[3 * (k - 1) + 1] = acc[k][1]; c[3 * (k - 1) + 2] = acc[k][2];
+    }
+    for (int k = 3 * (NB - 1); k < row; ++k) c[k] = 0.f;
   }
-  dsh0[3 * i] = acc[0][0]; dsh0[3 * i + 1] = acc[0][1]; dsh0[3 * i + 2] = acc[0][2];
-  float* c = dshs + (long long)i * (M - 1) * 3;
-  for (int k = 1; k < M; ++k) {
-    const bool on = k < NB;
-    c[3 * (k - 1)] = on ? acc[k < NB ? k : 0][0] : 0.f;
-    c[3 * (k - 1) + 1] = on ? acc[k < NB ? k : 0][1] : 0.f;
-    c[3 * (k - 1) + 2] = on ? acc[k < NB ? k : 0][2] : 0.f;
-  }
+  __syncthreads();
+  float* dst = dshs + i0 * row;
+  for (int k = t; k < nv * row; k += 256) dst[k] = s_rows[(k / row) * kShRowMax + (k % row)];
 }
 
 // Adam (adam_step.cu:21-102).  lerp(v0, v1, t) = fma(t, v1, fma(-t, v0, v0)).
@@ -295,7 +328,7 @@ hipError_t launch_sh_views_forward(int P, int V, const uint32_t* n_dup, const Vi
 }
 
 hipError_t launch_sh_views_backward(int P, int V, const uint32_t* n_dup, const ViewDesc* views, int deg, int M,
-                                    const float* centers, const float* drgb_views, float* dsh0, float* dshs,
+                                    const float* centers, float* drgb_views, float* dsh0, float* dshs,
                                     hipStream_t st) {
   if (P <= 0) return hipSuccess;
   const unsigned nb = blocks_for(P);

@@ -107,3 +107,46 @@ def test_batched_voxel_skip_keeps_results():
     say(f"  rel-L2 grid {e[0]:.2e} sh0 {e[1]:.2e} shs {e[2]:.2e} s {e_s:.2e}")
     assert max(e) <= 1e-5 and e_s <= 1e-4
     assert b[6] < a[6]
+
+
+@pytest.mark.gpu
+def test_batched_tile_lists_match_per_view():
+    """Rank-order binning of all views at once gives every tile the per-view list (same values in
+    the same order) and the same per-pixel contributor counts."""
+    from svraster_amd import _C
+    say("[batched] per-tile lists and contributor counts vs the per-view binning")
+    w, sc, cams, targets = _setup()
+    m = VoxelModel(sc)
+    r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key, cams,
+                          m.bg_color, sc.sh_degree, cull=False)
+    s_val = torch.exp(m._log_s.detach().reshape(1) * 10.0)
+    r.forward(m._geo_grid_pts.detach(), m._sh0.detach(), m._shs.detach(), s_val)
+    torch.cuda.synchronize()
+    ranges_b, vals_b, ncon_b, _ = r.state_views()
+    tile_base = 0
+    for v, cam in enumerate(cams):
+        W, H = cam.image_width, cam.image_height
+        n_dup, geom = _C.rasterize_preprocess(W, H, cam.tanfovx, cam.tanfovy, cam.cx, cam.cy, cam.w2c, cam.c2w, 0,
+                                              cam.near, sc.octpath, sc.vox_center, sc.vox_size, sc.is_leaf)
+        rgbs = r.rgbs_views[v].contiguous()
+        K, binning, img, *_ = _C.rasterize_voxels(
+            3, 1, W, H, cam.tanfovx, cam.tanfovy, cam.cx, cam.cy, cam.w2c, cam.c2w, m.bg_color, 0, False, False,
+            False, False, sc.octpath, sc.vox_center, sc.vox_size, r.geos, rgbs,
+            torch.empty(sc.num_voxels, 0, device="cuda"), geom, False, s_val)
+        _, vals_v = _C.unpack_BinningState(K, binning)
+        rng_v, _, ncon_v = _C.unpack_ImageState(W, H, img)
+        nt = rng_v.shape[0] * rng_v.shape[1]
+        rb = ranges_b[tile_base:tile_base + nt].long()
+        rv = rng_v.reshape(nt, 2).long()
+        assert torch.equal(rb[:, 1] - rb[:, 0], rv[:, 1] - rv[:, 0]), f"view {v}: tile list lengths"
+        # Same values in the same order in every tile.
+        lens = (rv[:, 1] - rv[:, 0])
+        idx_b = torch.repeat_interleave(rb[:, 0], lens) + (torch.arange(int(lens.sum()), device="cuda")
+                                                          - torch.repeat_interleave(lens.cumsum(0) - lens, lens))
+        idx_v = torch.repeat_interleave(rv[:, 0], lens) + (torch.arange(int(lens.sum()), device="cuda")
+                                                          - torch.repeat_interleave(lens.cumsum(0) - lens, lens))
+        assert torch.equal(vals_b[idx_b], vals_v[idx_v]), f"view {v}: tile list contents/order"
+        nb = ncon_b[int(r.pix_base[v]):int(r.pix_base[v]) + W * H].view(H, W)
+        assert torch.equal(nb, ncon_v), f"view {v}: contributor counts"
+        say(f"  view {v} {W}x{H}: {int(K)} keys, lists and contributor counts identical")
+        tile_base += nt
