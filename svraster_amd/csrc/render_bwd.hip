@@ -331,7 +331,7 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
 // (tile, entry).
 // ------------------------------------------------------------------------------------------
 constexpr int kBwdStage = 128;
-constexpr int kBwdWin = 160;
+constexpr int kBwdWin = 128;
 
 __device__ inline int pop_entry_b(uint64_t& lo, uint64_t& hi) {
   if (lo) {
@@ -364,18 +364,18 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
   __shared__ float4 s_dc[4][64];           // dL/dcolor xyz
   __shared__ float4 s_dn[4][kNormal ? 64 : 1];
   __shared__ float4 s_gt[4][kConcen ? 64 : 1];
-  __shared__ uint16_t s_desc[4][kBwdWin];  // entry slot | pixel << 7
-  __shared__ uint16_t s_cmap[4][kBwdWin];  // window pairs that hit their box, then contributing pairs
-  __shared__ uint16_t s_cpos[4][kBwdWin + 1];  // window pair -> compact index (exclusive count)
-  __shared__ uint8_t s_own[4][kBwdWin];    // 1 + entry at the first position of its run, else 0
-  __shared__ uint64_t s_cm[4][kBwdStage];  // per entry: pixels of the block whose pair contributes
-  __shared__ uint16_t s_epref[4][kBwdStage];  // per entry: first entry-major position
-  __shared__ uint64_t s_csl[4][64], s_csh[4][64];  // per pixel: contributing entries (window)
-  __shared__ uint16_t s_coff[4][64];       // per pixel: first compact index (window)
-  __shared__ float s_pa[4][kBwdWin];       // pair alpha (0 if the ray misses the box)
+  __shared__ uint64_t s_m[4][kBwdStage];   // per entry: its pixels in the block
+  __shared__ uint16_t s_pref[4][kBwdStage];  // per entry: first position of its run
+  __shared__ int32_t s_jmin[4][64];        // per pixel: first slot before its last contributor
+  __shared__ unsigned long long s_cs[4][2][64];  // per pixel: contributing entries (lo, hi halves)
+  __shared__ uint16_t s_own[4][kBwdWin];   // 1 + entry whose run starts at the position; later the
+                                           // contributing hits in entry-major order
+  __shared__ uint16_t s_hit[4][kBwdWin];   // positions whose ray hits the box
+  __shared__ uint16_t s_hd[4][kBwdWin];    // their entry | pixel << 7
+  __shared__ float s_pa[4][kBwdWin];       // alpha per position (0 on a miss)
   __shared__ float s_pd[4][kDepth ? kBwdWin : 1];
-  __shared__ float s_pl[4][kBwdWin];       // dL/dalpha per contributing pair (compact index)
-  __shared__ float s_pt[4][kBwdWin];       // T before the entry per contributing pair
+  __shared__ float s_pl[4][kBwdWin];       // dL/dalpha per contributing position
+  __shared__ float s_pt[4][kBwdWin];       // T before the entry per contributing position
   __shared__ float s_red[4];
   __shared__ int s_tile;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -463,61 +463,71 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
       // Slot j holds list position rel_j = n - 1 - base - j (relative to the tile start); a pixel
       // takes the slots with rel_j < last, i.e. j >= n - base - last.
       const bool live = inside && last + (uint32_t)base + (uint32_t)cnt > (uint32_t)n;
-      if (__ballot(live) != 0ull) {
-        const uint64_t m_lo = lane < cnt ? entry_block_mask_b(s_bb, s_rgb, lane, qx, qy, s_om[w]) : 0ull;
-        const uint64_t m_hi = lane + 64 < cnt ? entry_block_mask_b(s_bb, s_rgb, lane + 64, qx, qy, s_om[w]) : 0ull;
-        uint64_t e_lo = transpose64(m_lo, lane);
-        uint64_t e_hi = transpose64(m_hi, lane);
-        if (!live) {
-          e_lo = 0ull;
-          e_hi = 0ull;
-        } else {
-          const int jmin = n - base - (int)last;   // first slot this pixel takes
-          if (jmin >= 128) { e_lo = 0ull; e_hi = 0ull; }
-          else if (jmin >= 64) { e_lo = 0ull; e_hi &= ~0ull << (jmin - 64); }
-          else if (jmin > 0) { e_lo &= ~0ull << jmin; }
-        }
-        const uint32_t np_lane = (uint32_t)(__popcll(e_lo) + __popcll(e_hi));
-        const uint32_t incl = wave_incl_scan_u32(np_lane);
-        const uint32_t off = incl - np_lane, endp = incl;
-        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-        uint32_t cur = off;
+      const uint64_t live_m = __ballot(live);
+      if (live_m != 0ull) {
+        s_jmin[w][lane] = live ? max(n - base - (int)last, 0) : kBwdStage;
+        const uint64_t m_lo = lane < cnt ? entry_block_mask_b(s_bb, s_rgb, lane, qx, qy, s_om[w]) & live_m : 0ull;
+        const uint64_t m_hi = lane + 64 < cnt ? entry_block_mask_b(s_bb, s_rgb, lane + 64, qx, qy, s_om[w]) & live_m : 0ull;
+        const uint32_t c_lo = (uint32_t)__popcll(m_lo), c_hi = (uint32_t)__popcll(m_hi);
+        const uint32_t i_lo = wave_incl_scan_u32(c_lo);
+        const uint32_t i_hi = wave_incl_scan_u32(c_hi);
+        const uint32_t t_lo = (uint32_t)__builtin_amdgcn_readlane((int)i_lo, 63);
+        const uint32_t total = t_lo + (uint32_t)__builtin_amdgcn_readlane((int)i_hi, 63);
+        const uint32_t p_lo = i_lo - c_lo, p_hi = t_lo + i_hi - c_hi;
+        s_m[w][lane] = m_lo;
+        s_m[w][lane + 64] = m_hi;
+        s_pref[w][lane] = (uint16_t)p_lo;
+        s_pref[w][lane + 64] = (uint16_t)p_hi;
+        uint32_t carry = 0;   // 1 + entry owning the last position handled so far
 
         for (uint32_t w0 = 0; w0 < total; w0 += kBwdWin) {
           const uint32_t w1 = min(total, w0 + (uint32_t)kBwdWin);
-          const uint32_t stop = min(endp, w1);
-          while (cur < stop) {
-            const int j = pop_entry_b(e_lo, e_hi);
-            s_desc[w][cur - w0] = (uint16_t)(j | (lane << 7));
-            ++cur;
-          }
+          const uint32_t np = w1 - w0;
+          for (uint32_t k = lane; k < np; k += 64) s_own[w][k] = 0;
+          s_cs[w][0][lane] = 0ull;
+          s_cs[w][1][lane] = 0ull;
+          wave_lds_sync_b();
+          if (c_lo && p_lo >= w0 && p_lo < w1) s_own[w][p_lo - w0] = (uint16_t)(lane + 1);
+          if (c_hi && p_hi >= w0 && p_hi < w1) s_own[w][p_hi - w0] = (uint16_t)(lane + 65);
           wave_lds_sync_b();
 
-          // 2. ray/box test per pair (misses get alpha 0, hits listed), then alpha (and depth
-          // value) per hit, 64 per instruction.
-          const uint32_t np = w1 - w0;
+          // 2a. ray/box test per position (pairs at or past the pixel's last contributor count as
+          // misses); hits listed with their (entry, pixel).
           uint32_t nh = 0;
           for (uint32_t c0 = 0; c0 < np; c0 += 64) {
-            const uint32_t pp = c0 + lane;
+            const uint32_t k = c0 + lane;
+            const bool valid = k < np;
+            uint32_t h = valid ? (uint32_t)s_own[w][k] : 0u;
+            h = max(wave_incl_max_u32(h), carry);
+            carry = (uint32_t)__builtin_amdgcn_readlane((int)h, 63);
             bool hit = false;
-            if (pp < np) {
-              const uint32_t d = s_desc[w][pp];
-              const int j = (int)(d & 127u), bit = (int)(d >> 7);
-              const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
-              const float4 ch = s_ch[j];
-              float ta, tb;
-              ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
-              hit = !(ta > tb);
-              s_pa[w][pp] = 0.f;
+            uint32_t desc = 0;
+            if (valid) {
+              const int j = (int)h - 1;
+              const int bit = select_set_bit(s_m[w][j], (int)(w0 + k - (uint32_t)s_pref[w][j]));
+              if (j >= s_jmin[w][bit]) {
+                const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
+                const float4 ch = s_ch[j];
+                float ta, tb;
+                ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
+                hit = !(ta > tb);
+              }
+              desc = (uint32_t)j | ((uint32_t)bit << 7);
+              s_pa[w][k] = 0.f;
             }
             const uint64_t bal = __ballot(hit);
-            if (hit) s_cmap[w][nh + (uint32_t)__popcll(bal & lt_mask)] = (uint16_t)pp;
+            if (hit) {
+              const uint32_t idx = nh + (uint32_t)__popcll(bal & lt_mask);
+              s_hit[w][idx] = (uint16_t)k;
+              s_hd[w][idx] = (uint16_t)desc;
+            }
             nh += (uint32_t)__popcll(bal);
           }
           wave_lds_sync_b();
-          for (uint32_t k = lane; k < nh; k += 64) {
-            const uint32_t pp = s_cmap[w][k];
-            const uint32_t d = s_desc[w][pp];
+          // 2b. alpha (and depth value) per hit; contributing pairs into the pixel's entry set.
+          for (uint32_t h = lane; h < nh; h += 64) {
+            const uint32_t k = s_hit[w][h];
+            const uint32_t d = s_hd[w][h];
             const int j = (int)(d & 127u), bit = (int)(d >> 7);
             const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
             const float4 ch = s_ch[j];
@@ -528,106 +538,78 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
             const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
             Segment seg;
             eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
-            s_pa[w][pp] = seg.alpha;
+            s_pa[w][k] = seg.alpha;
             if (kDepth) {
               float sub[3];
               sub_alphas(seg, sub);
-              s_pd[w][pp] = depth_value(sub, ta, tb);
+              s_pd[w][k] = depth_value(sub, ta, tb);
             }
+            if (seg.alpha >= kMinAlpha) atomicOr(&s_cs[w][j >> 6][bit], 1ull << (j & 63));
           }
           wave_lds_sync_b();
 
-          // 2b. contributing pairs (alpha >= kMinAlpha) compacted in pixel-major order: s_cpos[pp]
-          // is the compact index of window pair pp, s_cmap the inverse.
+          // 3. sequential reverse walk per pixel over its contributing entries (ascending slot =
+          // back to front along the list).
+          {
+            uint64_t cs_lo = s_cs[w][0][lane], cs_hi = s_cs[w][1][lane];
+            while ((cs_lo | cs_hi) != 0ull) {
+              const int j = pop_entry_b(cs_lo, cs_hi);
+              const uint32_t k = (uint32_t)s_pref[w][j] + (uint32_t)__popcll(s_m[w][j] & lt_mask) - w0;
+              const float alpha = s_pa[w][k];
+              T = T / (1.f - alpha);
+              const float4 rgb = s_rgb[j];
+              float dptw = dC0 * rgb.x + dC1 * rgb.y + dC2 * rgb.z;
+              if (kNormal) {
+                const float4 nn = s_nrm[j];
+                dptw += dN0 * nn.x + dN1 * nn.y + dN2 * nn.z;
+              }
+              const float dLda = T * (dptw - last_dT);
+              last_dT += alpha * (dptw - last_dT);
+              if (kDepth) last_dT += dD * s_pd[w][k];
+              s_pl[w][k] = dLda;
+              s_pt[w][k] = T;
+            }
+          }
+          // 4a. contributing hits in entry-major order (a subsequence of the hit list).
           uint32_t nc = 0;
-          for (uint32_t c0 = 0; c0 < np; c0 += 64) {
-            const uint32_t pp = c0 + lane;
-            const bool f = pp < np && s_pa[w][pp] >= kMinAlpha;
+          for (uint32_t c0 = 0; c0 < nh; c0 += 64) {
+            const uint32_t h = c0 + lane;
+            const bool f = h < nh && s_pa[w][s_hit[w][h]] >= kMinAlpha;
             const uint64_t bal = __ballot(f);
-            const uint32_t rank = nc + (uint32_t)__popcll(bal & lt_mask);
-            if (pp < np) s_cpos[w][pp] = (uint16_t)rank;
-            if (f) s_cmap[w][rank] = (uint16_t)pp;
+            if (f) s_own[w][nc + (uint32_t)__popcll(bal & lt_mask)] = (uint16_t)h;
             nc += (uint32_t)__popcll(bal);
           }
-          if (lane == 0) s_cpos[w][np] = (uint16_t)nc;
           wave_lds_sync_b();
 
-          // 3. sequential reverse walk per pixel over its contributing pairs
-          const uint32_t r0 = max(off, w0), r1 = min(endp, w1);
-          uint32_t cb = 0, ce = 0;
-          if (r0 < r1) {
-            cb = s_cpos[w][r0 - w0];
-            ce = s_cpos[w][r1 - w0];
-          }
-          uint64_t cs_lo = 0ull, cs_hi = 0ull;   // entries whose pair with this pixel contributes
-          for (uint32_t c = cb; c < ce; ++c) {
-            const uint32_t pp = s_cmap[w][c];
-            const float alpha = s_pa[w][pp];
-            const int j = (int)(s_desc[w][pp] & 127u);
-            T = T / (1.f - alpha);
-            const float4 rgb = s_rgb[j];
-            float dptw = dC0 * rgb.x + dC1 * rgb.y + dC2 * rgb.z;
-            if (kNormal) {
-              const float4 nn = s_nrm[j];
-              dptw += dN0 * nn.x + dN1 * nn.y + dN2 * nn.z;
-            }
-            const float dLda = T * (dptw - last_dT);
-            last_dT += alpha * (dptw - last_dT);
-            if (kDepth) last_dT += dD * s_pd[w][pp];
-            s_pl[w][c] = dLda;
-            s_pt[w][c] = T;
-            if (j < 64) cs_lo |= 1ull << j; else cs_hi |= 1ull << (j - 64);
-          }
-
-          // 4a. entry-major order of the contributing pairs: per entry the mask of its pixels
-          // (transpose of the per-pixel sets), their prefix, and the entry of each position.
-          s_coff[w][lane] = (uint16_t)cb;
-          s_csl[w][lane] = cs_lo;
-          s_csh[w][lane] = cs_hi;
-          const uint64_t cm_lo = transpose64(cs_lo, lane);   // pixels of entry `lane`
-          const uint64_t cm_hi = transpose64(cs_hi, lane);   // pixels of entry `lane + 64`
-          const uint32_t n_lo = (uint32_t)__popcll(cm_lo), n_hi = (uint32_t)__popcll(cm_hi);
-          const uint32_t i_lo = wave_incl_scan_u32(n_lo);
-          const uint32_t i_hi = wave_incl_scan_u32(n_hi);
-          const uint32_t t_lo = (uint32_t)__builtin_amdgcn_readlane((int)i_lo, 63);
-          const uint32_t p_lo = i_lo - n_lo, p_hi = t_lo + i_hi - n_hi;
-          s_cm[w][lane] = cm_lo;
-          s_cm[w][lane + 64] = cm_hi;
-          s_epref[w][lane] = (uint16_t)p_lo;
-          s_epref[w][lane + 64] = (uint16_t)p_hi;
-          for (uint32_t c = lane; c < nc; c += 64) s_own[w][c] = 0;
-          wave_lds_sync_b();
-          if (n_lo) s_own[w][p_lo] = (uint8_t)(lane + 1);
-          if (n_hi) s_own[w][p_hi] = (uint8_t)(lane + 65);
-          wave_lds_sync_b();
-
-          // 4b. per-pair derivatives, 64 pairs per instruction in entry-major order; each entry's
-          // run is summed across lanes (segmented DPP sum) and its last lane adds the 12 sums to
-          // the workgroup's LDS accumulators.
-          uint32_t carry = 0;
+          // 4b. per-pair derivatives, 64 pairs per instruction; each entry's run is summed across
+          // lanes (segmented DPP sum) and its last lane adds the 12 sums to the workgroup's LDS
+          // accumulators.
+          uint32_t jprev = 0xFFFFFFFFu;   // entry of the previous chunk's last pair
           for (uint32_t c0 = 0; c0 < nc; c0 += 64) {
-            const uint32_t p = c0 + lane;
-            const bool valid = p < nc;
-            uint32_t h = valid ? (uint32_t)s_own[w][p] : 0u;
-            h = max(wave_incl_max_u32(h), carry);
-            carry = (uint32_t)__builtin_amdgcn_readlane((int)h, 63);
+            const uint32_t i = c0 + lane;
+            const bool valid = i < nc;
+            uint32_t hh = 0, d = 0;
+            if (valid) {
+              hh = s_own[w][i];
+              d = s_hd[w][hh];
+            }
+            const uint32_t jj = valid ? (d & 127u) : 0xFFFFFFFEu;
+            uint32_t jp = (uint32_t)__shfl_up((int)jj, 1, 64);
+            if (lane == 0) jp = jprev;
+            const uint32_t jn = (uint32_t)__shfl_down((int)jj, 1, 64);
+            jprev = (uint32_t)__builtin_amdgcn_readlane((int)jj, 63);
+            const bool head = jj != jp;
+            const int seg_start = (int)wave_incl_max_u32(head ? (uint32_t)lane : 0u);
+            const bool tail = valid && (lane == 63 || i + 1 == nc || jn != jj);
             float gv[12];
 #pragma unroll
             for (int q = 0; q < 12; ++q) gv[q] = 0.f;
-            int j = 0, seg_start = lane;
-            bool tail = false;
+            const int j = (int)(d & 127u);
             if (valid) {
-              j = (int)h - 1;
-              const uint32_t k = p - (uint32_t)s_epref[w][j];
-              const uint64_t cm = s_cm[w][j];
-              const int bit = select_set_bit(cm, (int)k);
-              tail = k + 1 == (uint32_t)__popcll(cm) || lane == 63 || p + 1 == nc;
-              seg_start = lane - (int)min(k, (uint32_t)lane);
-              const uint64_t bl = s_csl[w][bit], bh = s_csh[w][bit];
-              const uint32_t below = j < 64 ? (uint32_t)__popcll(bl & ((1ull << j) - 1ull))
-                                            : (uint32_t)(__popcll(bl) + __popcll(bh & ((1ull << (j - 64)) - 1ull)));
-              const uint32_t ci = (uint32_t)s_coff[w][bit] + below;
-              const float Tp = s_pt[w][ci];
+              const int bit = (int)(d >> 7);
+              const uint32_t k = s_hit[w][hh];
+              const float Tp = s_pt[w][k];
+              const float dLda = s_pl[w][k];
               const float4 rdd = s_rd[w][bit];
               const Ray pr = pair_ray(r, rdd, s_ri[w][bit]);
               const float4 ch = s_ch[j];
@@ -639,7 +621,6 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
               Segment seg;
               eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
               const float alpha = seg.alpha;
-              const float dLda = s_pl[w][ci];
               const float ptw = alpha * Tp;
               const float p0v = seg.phi[0], p3v = seg.phi[3];
               const float rden = 1.f / (p0v + kPhiEps);
@@ -704,7 +685,7 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
               gv[11] = fabsf(dLda * alpha);
             }
             wave_segmented_sum<12>(gv, lane, seg_start);
-            if (valid && tail) {
+            if (tail) {
 #pragma unroll
               for (int q = 0; q < 12; ++q) atomicAdd(&s_acc[j][q], gv[q]);
               s_used[j] = 1u;

@@ -215,23 +215,23 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Pixel-major pair compositor (default).  Most voxels cover a handful of a tile's pixels and most
-// (pixel, voxel) candidates have alpha below the minimum, so the cost must follow the pairs that
-// exist, not the entries times the pixels.  Per staged batch of 128 entries each wave:
+// Pair compositor (default).  Most voxels cover a handful of a tile's pixels and most (pixel,
+// voxel) candidates have alpha below the minimum, so the cost must follow the pairs that exist,
+// not the entries times the pixels.  Per staged batch of 128 entries each wave:
 //   1. builds the 64-bit mask of every entry's pixels in its 8x8 block (bbox, octant, pixels
 //      still unsaturated; empty for an entry whose corner SDFs cannot reach the minimum alpha),
-//      lane l holding entries l and l + 64;
-//   2. transposes the two 64x64 bit matrices, so lane l holds the 128-bit set of entries that
-//      cover its pixel, and prefix-sums the per-pixel counts: pixel l owns the run
-//      [off_l, off_l + cnt_l) of a pixel-major pair list;
-//   3. writes the (entry, pixel) descriptor of each pair, then evaluates alpha (and the depth
-//      value) of 64 pairs per instruction into LDS, a window of up to kPairWin pairs at a time;
-//   4. composites each pixel over its own contiguous run of results, in entry order.
+//      lane l holding entries l and l + 64, and prefix-sums the counts: entry j owns the run
+//      [pref_j, pref_j + |m_j|) of an entry-major pair list;
+//   2. walks that list 64 pairs per instruction (the entry of a position from a prefix maximum
+//      over run heads, the pixel as the k-th set bit of the entry's mask): ray/box test, then the
+//      SDF segment for the hits, alpha (and depth value) stored per position;
+//   3. records the contributing pairs (alpha >= kMinAlpha) in a 128-bit entry set per pixel;
+//   4. composites each pixel over its set in entry order.
 // Per-pixel arithmetic and order are those of the one-entry-at-a-time kernel, so the outputs are
 // identical bit for bit.
 // ------------------------------------------------------------------------------------------
 constexpr int kPairStage = 128;   // entries staged per workgroup round (two 64-entry halves)
-constexpr int kPairWin = 256;     // pair results held per wave at once
+constexpr int kPairWin = 256;     // pair positions held per wave at once
 
 // Pixels of the wave's 8x8 block (bit = 8 * row + column) inside a packed bbox.
 __device__ inline uint64_t bbox_block_mask(uint32_t b0, uint32_t b1, int qx, int qy) {
@@ -276,13 +276,16 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
   __shared__ float4 s_ch[kPairStage], s_ml[kPairStage], s_g0[kPairStage], s_g1[kPairStage], s_rgb[kPairStage];
   __shared__ float4 s_nrm[kNormal ? kPairStage : 1];
   __shared__ uint2 s_bb[kPairStage];
-  __shared__ float4 s_rd[4][64];          // direction xyz
-  __shared__ float4 s_ri[4][64];          // 1/direction xyz
-  __shared__ uint64_t s_om[4][8];            // pixels of the block whose ray has octant q
-  __shared__ uint16_t s_desc[4][kPairWin];   // entry slot | pixel << 7
-  __shared__ uint16_t s_list[4][kPairWin];   // window pairs that hit their box, then contributing pairs
-  __shared__ uint16_t s_cpos[4][kPairWin + 1];  // window pair -> contributing index (exclusive count)
-  __shared__ float s_pa[4][kPairWin];        // pair alpha (0 when the ray misses the box)
+  __shared__ float4 s_rd[4][64];              // direction xyz
+  __shared__ float4 s_ri[4][64];              // 1/direction xyz
+  __shared__ uint64_t s_om[4][8];             // pixels of the block whose ray has octant q
+  __shared__ uint64_t s_m[4][kPairStage];     // per entry: its pixels in the block
+  __shared__ uint16_t s_pref[4][kPairStage];  // per entry: first position of its run
+  __shared__ unsigned long long s_cs[4][2][64];   // per pixel: contributing entries (lo, hi halves)
+  __shared__ uint8_t s_own[4][kPairWin];      // 1 + entry whose run starts at the position, else 0
+  __shared__ uint16_t s_hit[4][kPairWin];     // positions whose ray hits the box
+  __shared__ uint16_t s_hd[4][kPairWin];      // their entry | pixel << 7
+  __shared__ float s_pa[4][kPairWin];         // alpha per position (0 when the ray misses the box)
   __shared__ float s_pd[4][kDepth ? kPairWin : 1];
   __shared__ uint32_t s_red[4];
   __shared__ int s_tile;
@@ -343,53 +346,67 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
       const uint64_t live = __ballot(!done);
       if (live == 0ull) continue;
 
-      // 1. + 2. entry masks, transposed to per-pixel entry sets; pixel-major run offsets.
+      // 1. entry masks and the entry-major run offsets.
       const uint64_t m_lo = lane < cnt ? entry_block_mask(s_bb, s_rgb, lane, qx, qy, s_om[w], live) : 0ull;
       const uint64_t m_hi = lane + 64 < cnt ? entry_block_mask(s_bb, s_rgb, lane + 64, qx, qy, s_om[w], live) : 0ull;
-      if (__ballot((m_lo | m_hi) != 0ull) == 0ull) continue;
-      uint64_t e_lo = transpose64(m_lo, lane);
-      uint64_t e_hi = transpose64(m_hi, lane);
-      const uint32_t np_lane = (uint32_t)(__popcll(e_lo) + __popcll(e_hi));
-      const uint32_t incl = wave_incl_scan_u32(np_lane);
-      const uint32_t off = incl - np_lane, endp = incl;
-      const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-      uint32_t cur = off;
+      const uint32_t c_lo = (uint32_t)__popcll(m_lo), c_hi = (uint32_t)__popcll(m_hi);
+      if (__ballot((c_lo | c_hi) != 0u) == 0ull) continue;
+      const uint32_t i_lo = wave_incl_scan_u32(c_lo);
+      const uint32_t i_hi = wave_incl_scan_u32(c_hi);
+      const uint32_t t_lo = (uint32_t)__builtin_amdgcn_readlane((int)i_lo, 63);
+      const uint32_t total = t_lo + (uint32_t)__builtin_amdgcn_readlane((int)i_hi, 63);
+      const uint32_t p_lo = i_lo - c_lo, p_hi = t_lo + i_hi - c_hi;
+      s_m[w][lane] = m_lo;
+      s_m[w][lane + 64] = m_hi;
+      s_pref[w][lane] = (uint16_t)p_lo;
+      s_pref[w][lane + 64] = (uint16_t)p_hi;
+      uint32_t carry = 0;   // 1 + entry owning the last position handled so far
 
       for (uint32_t w0 = 0; w0 < total; w0 += kPairWin) {
         const uint32_t w1 = min(total, w0 + (uint32_t)kPairWin);
-        // 3a. descriptors of this pixel's pairs that fall in [w0, w1)
-        const uint32_t stop = min(endp, w1);
-        while (cur < stop) {
-          const int j = pop_entry(e_lo, e_hi);
-          s_desc[w][cur - w0] = (uint16_t)(j | (lane << 7));
-          ++cur;
-        }
-        wave_lds_sync();
-        // 3b. ray/box test per pair; misses get alpha 0, hits are listed for the SDF segment.
         const uint32_t np = w1 - w0;
+        for (uint32_t k = lane; k < np; k += 64) s_own[w][k] = 0;
+        s_cs[w][0][lane] = 0ull;
+        s_cs[w][1][lane] = 0ull;
+        wave_lds_sync();
+        if (c_lo && p_lo >= w0 && p_lo < w1) s_own[w][p_lo - w0] = (uint8_t)(lane + 1);
+        if (c_hi && p_hi >= w0 && p_hi < w1) s_own[w][p_hi - w0] = (uint8_t)(lane + 65);
+        wave_lds_sync();
+
+        // 2a. ray/box test per position; hits listed with their (entry, pixel).
         uint32_t nh = 0;
         for (uint32_t c0 = 0; c0 < np; c0 += 64) {
-          const uint32_t pp = c0 + lane;
+          const uint32_t k = c0 + lane;
+          const bool valid = k < np;
+          uint32_t h = valid ? (uint32_t)s_own[w][k] : 0u;
+          h = max(wave_incl_max_u32(h), carry);
+          carry = (uint32_t)__builtin_amdgcn_readlane((int)h, 63);
           bool hit = false;
-          if (pp < np) {
-            const uint32_t d = s_desc[w][pp];
-            const int j = (int)(d & 127u), bit = (int)(d >> 7);
+          uint32_t desc = 0;
+          if (valid) {
+            const int j = (int)h - 1;
+            const int bit = select_set_bit(s_m[w][j], (int)(w0 + k - (uint32_t)s_pref[w][j]));
             const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
             const float4 ch = s_ch[j];
             float ta, tb;
             ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
             hit = !(ta > tb);
-            s_pa[w][pp] = 0.f;
+            desc = (uint32_t)j | ((uint32_t)bit << 7);
+            s_pa[w][k] = 0.f;
           }
           const uint64_t bal = __ballot(hit);
-          if (hit) s_list[w][nh + (uint32_t)__popcll(bal & lt_mask)] = (uint16_t)pp;
+          if (hit) {
+            const uint32_t idx = nh + (uint32_t)__popcll(bal & lt_mask);
+            s_hit[w][idx] = (uint16_t)k;
+            s_hd[w][idx] = (uint16_t)desc;
+          }
           nh += (uint32_t)__popcll(bal);
         }
         wave_lds_sync();
-        // 3c. alpha (and depth value) per hit, 64 per instruction
-        for (uint32_t k = lane; k < nh; k += 64) {
-          const uint32_t pp = s_list[w][k];
-          const uint32_t d = s_desc[w][pp];
+        // 2b. + 3. alpha (and depth value) per hit; contributing pairs into the pixel's set.
+        for (uint32_t h = lane; h < nh; h += 64) {
+          const uint32_t k = s_hit[w][h];
+          const uint32_t d = s_hd[w][h];
           const int j = (int)(d & 127u), bit = (int)(d >> 7);
           const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
           const float4 ch = s_ch[j];
@@ -400,44 +417,26 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
           const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
           Segment seg;
           eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
-          s_pa[w][pp] = seg.alpha;
+          s_pa[w][k] = seg.alpha;
           if (kDepth) {
             float sub[3];
             sub_alphas(seg, sub);
-            s_pd[w][pp] = depth_value(sub, ta, tb);
+            s_pd[w][k] = depth_value(sub, ta, tb);
           }
+          if (seg.alpha >= kMinAlpha) atomicOr(&s_cs[w][j >> 6][bit], 1ull << (j & 63));
         }
         wave_lds_sync();
-        // 3d. contributing pairs (alpha >= kMinAlpha) compacted in pixel-major order: s_cpos[pp]
-        // is the compact index of window pair pp, s_list the inverse.
-        uint32_t nc = 0;
-        for (uint32_t c0 = 0; c0 < np; c0 += 64) {
-          const uint32_t pp = c0 + lane;
-          const bool f = pp < np && s_pa[w][pp] >= kMinAlpha;
-          const uint64_t bal = __ballot(f);
-          const uint32_t rank = nc + (uint32_t)__popcll(bal & lt_mask);
-          if (pp < np) s_cpos[w][pp] = (uint16_t)rank;
-          if (f) s_list[w][rank] = (uint16_t)pp;
-          nc += (uint32_t)__popcll(bal);
-        }
-        if (lane == 0) s_cpos[w][np] = (uint16_t)nc;
-        wave_lds_sync();
-        // 4. composite this pixel's contributing pairs inside the window, in entry order
-        const uint32_t r0 = max(off, w0), r1 = min(endp, w1);
-        uint32_t cb = 0, ce = 0;
-        if (r0 < r1) {
-          cb = s_cpos[w][r0 - w0];
-          ce = s_cpos[w][r1 - w0];
-        }
-        for (uint32_t c = cb; c < ce && !done; ++c) {
-          const uint32_t pp = s_list[w][c];
-          const float alpha = s_pa[w][pp];
-          const int j = (int)(s_desc[w][pp] & 127u);
+        // 4. composite this pixel's contributing pairs of the window, in entry order.
+        uint64_t cs_lo = s_cs[w][0][lane], cs_hi = s_cs[w][1][lane];
+        while ((cs_lo | cs_hi) != 0ull && !done) {
+          const int j = pop_entry(cs_lo, cs_hi);
+          const uint32_t k = (uint32_t)s_pref[w][j] + (uint32_t)__popcll(s_m[w][j] & lt_mask) - w0;
+          const float alpha = s_pa[w][k];
           const float4 rgb = s_rgb[j];
           const float wgt = T * alpha;
           C0 += wgt * rgb.x; C1 += wgt * rgb.y; C2 += wgt * rgb.z;
           if (kDepth) {
-            D += T * s_pd[w][pp];
+            D += T * s_pd[w][k];
             if (T > 0.5f) { dmed_vox = (int)(__float_as_uint(rgb.w) & kVoxMask); dmed_T = T; }
           }
           if (kNormal) {
