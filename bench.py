@@ -21,6 +21,7 @@ import time
 
 METRIC = "fwd+bwd Mrays/s on 1 MI355X, batched vs per-view, at ≤1e-4 img err; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+BWD_KERNEL = "render_bwd_pairs_kernel<false, false, false, true>"
 
 
 def log(*a):
@@ -56,45 +57,31 @@ def time_steps(fn, steps, warmup, sync):
     return (time.perf_counter() - t0) * 1000.0 / steps
 
 
-def backward_traffic(model, cams, targets):
-    """Algorithmic bytes of one backward-compositor launch, averaged over the views.
+def backward_traffic(rast, dvox):
+    """Algorithmic bytes of the multi-view backward-compositor launch (one launch, all views).
 
-    Per launch: 72 B per streamed (tile, entry) pair [value 4, bbox 8, centre 12, size 4,
-    8 corner SDFs 32, rgb 12], 12 B per tile [range, last-contributor bound], 24 B per pixel
-    [T, contributor count, dL/dcolor 12, dL/dT], 48 B per voxel that receives a gradient
-    [12 floats]."""
-    import torch
-    from svraster_amd import _C
-    tot, launches = 0.0, 0
-    for cam, tgt in zip(cams, targets):
-        rs = model.raster_settings(cam)
-        n_dup, geom = _C.rasterize_preprocess(cam.image_width, cam.image_height, cam.tanfovx, cam.tanfovy,
-                                              cam.cx, cam.cy, cam.w2c, cam.c2w, 0, cam.near, model.octpath,
-                                              model.vox_center, model.vox_size, model.is_leaf)
-        with torch.no_grad():
-            idx = torch.where(n_dup > 0)[0]
-            vp = model.vox_fn(idx, cam.c2w[:3, 3])
-            s = torch.exp(model._log_s * 10.0).reshape(1)
-            K, binning, img, color, depth, normal, T, mw, sdf0, feat = _C.rasterize_voxels(
-                3, 1, cam.image_width, cam.image_height, cam.tanfovx, cam.tanfovy, cam.cx, cam.cy,
-                cam.w2c, cam.c2w, rs.background, 0, False, False, False, False, model.octpath,
-                model.vox_center, model.vox_size, vp["geos"], vp["rgbs"],
-                torch.empty(model.num_voxels, 0, device="cuda"), geom, False, s)
-            ranges, tile_last, _ = _C.unpack_ImageState(cam.image_width, cam.image_height, img)
-            E = int((tile_last.long() - ranges[..., 0].long()).sum().item())
-            dC = 2.0 * (color - tgt) / color.numel()
-            dgeo, drgb, dsub, ds = _C.rasterize_voxels_backward(
-                K, 3, 1, cam.image_width, cam.image_height, cam.tanfovx, cam.tanfovy, cam.cx, cam.cy,
-                cam.w2c, cam.c2w, rs.background, model.octpath, model.vox_center, model.vox_size,
-                vp["geos"], vp["rgbs"], geom, binning, img, T, dC, torch.empty(0), torch.empty(0),
-                torch.zeros_like(T), 0.0, torch.empty(0), 0.0, 0.0, False, False, torch.empty(0),
-                torch.empty(0), False, s, sdf0)
-            touched = int(((dgeo != 0).any(1) | (drgb != 0).any(1)).sum().item())
-        R = cam.image_width * cam.image_height
-        Nt = ranges.shape[0] * ranges.shape[1]
-        tot += 72.0 * E + 12.0 * Nt + 24.0 * R + 48.0 * touched
-        launches += 1
-    return tot / launches
+    72 B per streamed (tile, entry) pair [value 4, bbox 8, centre 12, size 4, 8 corner SDFs 32,
+    rgb 12], 12 B per tile [range 8, last-contributor bound 4], 24 B per pixel [T 4, contributor
+    count 4, dL/dcolor 12, dL/dT 4], 48 B per voxel that receives a gradient [12 floats].  Streamed
+    entries E = sum over tiles of (last-contributor bound - range start), read from the launch's
+    own state."""
+    ranges, _, _, tile_last = rast.state_views()
+    E = int((tile_last.long() - ranges[:, 0].long()).clamp_min(0).sum().item())
+    touched = int((dvox != 0).any(1).sum().item())
+    total = 72.0 * E + 12.0 * rast.n_tiles + 24.0 * rast.n_pix + 48.0 * touched
+    return total, {"streamed_entries": E, "tiles": rast.n_tiles, "rays": rast.n_pix, "touched_voxels": touched}
+
+
+def traffic_from_profiles(kernel_prefix):
+    """HBM bytes per launch of the given kernel from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration); None if absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_multi32.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d["kernels"][kernel_prefix]["hbm_bytes_per_launch"], d
+    except (OSError, KeyError, ValueError):
+        return None, None
 
 
 def cpu_baseline(workload_name, gpu_color_view0=None):
@@ -161,7 +148,18 @@ def main():
     ap.add_argument("--workload", default="multi_32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(256 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)        # elementwise kernel: reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); the multi-GPU path is "
               f"design-only in this build.", file=sys.stderr)
@@ -221,17 +219,20 @@ def main():
     log(f"[bench] batched vs per-view: image max-abs {img_err:.2e}, grad rel-L2 {grad_err:.2e}, "
         f"keys {K_used} / capacity {rast.cap}")
     kt_b = kernel_times(lib, bstep)
-    bstep.capture()
+    if not args.profile:          # profiler runs time the same step eagerly (no graph replay)
+        bstep.capture()
     ms_b = time_steps(bstep, args.steps, args.warmup, sync)
     log(f"[bench] batched graph path: {ms_b:.2f} ms/step; compositor fwd {kt_b['render_fwd'][0]:.2f} ms "
         f"bwd {kt_b['render_bwd'][0]:.2f} ms")
 
-    # -- roofline of the dominant kernel (batched backward compositor: one launch, all views) --
-    bytes_per_view = backward_traffic(model, cams, targets)
-    bytes_launch = bytes_per_view * len(cams)
+    # -- roofline of the dominant kernel (multi-view backward compositor: one launch, all views) --
+    bstep()
+    sync()
+    bytes_launch, counts = backward_traffic(rast, rast.dvox)
     bwd_ms, bwd_n = kt_b["render_bwd"]
     avg_launch_s = bwd_ms / max(bwd_n, 1) / 1000.0
     achieved = bytes_launch / avg_launch_s / 1e9
+    traffic, _ = traffic_from_profiles(BWD_KERNEL)
 
     value = rays / (ms_b / 1000.0) / 1e6
     value_pv = rays / (ms_pv / 1000.0) / 1e6
@@ -252,7 +253,7 @@ def main():
                                f"{len(cams)} views (8x640x480, 8x1237x822, 8x1600x1064, 8x1920x1080), "
                                f"{rays} rays/step, fwd+bwd of MSE to random targets",
                    "path": "batched: one launch per stage over all views, persistent (view, tile) queue, "
-                           "HIP graph replay"},
+                           + ("eager (profiler run)" if args.profile else "HIP graph replay")},
         "paths": {"batched_mrays_s": round(value, 3), "batched_ms_per_step": round(ms_b, 3),
                   "per_view_mrays_s": round(value_pv, 3), "per_view_ms_per_step": round(ms_pv, 3),
                   "speedup_batched_vs_per_view": round(ms_pv / ms_b, 3),
@@ -260,9 +261,10 @@ def main():
                   "batched_vs_per_view_grad_rel_l2": grad_err,
                   "keys_per_step": K_used},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "render_bwd_kernel (batched, all 32 views in one launch)",
-                     "bytes_per_launch": int(bytes_launch),
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if traffic is None else int(traffic),
+                     "kernel": BWD_KERNEL + " (multi-view work queue: all views in one launch)",
+                     "bytes_per_launch": int(bytes_launch), "counts": counts,
                      "avg_launch_ms": round(avg_launch_s * 1000.0, 4)},
         "kernels_ms_per_step": {"batched_render_fwd": round(kt_b["render_fwd"][0], 3),
                                 "batched_render_bwd": round(kt_b["render_bwd"][0], 3),

@@ -351,7 +351,7 @@ __device__ inline uint64_t entry_block_mask_b(const uint2* s_bb, const float4* s
   return bbox_block_mask_b(bb.x, bb.y, qx, qy) & s_om[q];
 }
 
-template <bool kDepth, bool kNormal, bool kConcen>
+template <bool kDepth, bool kNormal, bool kConcen, bool kQueue>
 __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
   __shared__ float4 s_ch[kBwdStage], s_ml[kBwdStage], s_g0[kBwdStage], s_g1[kBwdStage], s_rgb[kBwdStage];
   __shared__ float4 s_nrm[kNormal ? kBwdStage : 1];
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
 
   for (int iter = 0;; ++iter) {
     int g;
-    if (a.work_counter) {
+    if (kQueue) {
       if (t == 0) s_tile = (int)atomicAdd(a.work_counter, 1u);
       __syncthreads();
       g = s_tile;
@@ -721,8 +721,14 @@ static hipError_t launch_bwd_c(const BwdArgs& a, hipStream_t st, unsigned grid) 
     if (concen) render_bwd_kernel<D, N, true><<<grid, 256, 0, st>>>(a);
     else render_bwd_kernel<D, N, false><<<grid, 256, 0, st>>>(a);
   } else {
-    if (concen) render_bwd_pairs_kernel<D, N, true><<<grid, 256, 0, st>>>(a);
-    else render_bwd_pairs_kernel<D, N, false><<<grid, 256, 0, st>>>(a);
+    // The work-queue (multi-view) launch is its own instantiation, so profiles tell it apart.
+    if (a.work_counter) {
+      if (concen) render_bwd_pairs_kernel<D, N, true, true><<<grid, 256, 0, st>>>(a);
+      else render_bwd_pairs_kernel<D, N, false, true><<<grid, 256, 0, st>>>(a);
+    } else {
+      if (concen) render_bwd_pairs_kernel<D, N, true, false><<<grid, 256, 0, st>>>(a);
+      else render_bwd_pairs_kernel<D, N, false, false><<<grid, 256, 0, st>>>(a);
+    }
   }
   return hipGetLastError();
 }

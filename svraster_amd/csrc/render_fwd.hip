@@ -271,7 +271,7 @@ __device__ inline int pop_entry(uint64_t& lo, uint64_t& hi) {
   return j;
 }
 
-template <bool kDepth, bool kNormal, bool kMaxW>
+template <bool kDepth, bool kNormal, bool kMaxW, bool kQueue>
 __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
   __shared__ float4 s_ch[kPairStage], s_ml[kPairStage], s_g0[kPairStage], s_g1[kPairStage], s_rgb[kPairStage];
   __shared__ float4 s_nrm[kNormal ? kPairStage : 1];
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
 
   for (int iter = 0;; ++iter) {
     int g;
-    if (a.work_counter) {
+    if (kQueue) {
       if (t == 0) s_tile = (int)atomicAdd(a.work_counter, 1u);
       __syncthreads();
       g = s_tile;
@@ -470,8 +470,14 @@ static hipError_t launch_fwd_mw(const FwdArgs& a, hipStream_t st, unsigned grid)
     if (a.max_w) render_fwd_kernel<D, N, true><<<grid, 256, 0, st>>>(a);
     else render_fwd_kernel<D, N, false><<<grid, 256, 0, st>>>(a);
   } else {
-    if (a.max_w) render_fwd_pairs_kernel<D, N, true><<<grid, 256, 0, st>>>(a);
-    else render_fwd_pairs_kernel<D, N, false><<<grid, 256, 0, st>>>(a);
+    // The work-queue (multi-view) launch is its own instantiation, so profiles tell it apart.
+    if (a.work_counter) {
+      if (a.max_w) render_fwd_pairs_kernel<D, N, true, true><<<grid, 256, 0, st>>>(a);
+      else render_fwd_pairs_kernel<D, N, false, true><<<grid, 256, 0, st>>>(a);
+    } else {
+      if (a.max_w) render_fwd_pairs_kernel<D, N, true, false><<<grid, 256, 0, st>>>(a);
+      else render_fwd_pairs_kernel<D, N, false, false><<<grid, 256, 0, st>>>(a);
+    }
   }
   return hipGetLastError();
 }
