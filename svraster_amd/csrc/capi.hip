@@ -272,6 +272,7 @@ int svr_rasterize_forward(int vox_geo_mode, int density_mode, int W, int H, floa
   a.tile_last = ip.tile_last; a.n_contrib = ip.n_contrib;
   a.color = out_color; a.depth = out_depth; a.normal = out_normal; a.T = out_T; a.max_w = max_w;
   a.work_counter = nullptr;
+  a.cbits = bp.cbits; a.k_cap = K;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;
   {
@@ -321,6 +322,9 @@ int svr_rasterize_backward(long long K, int vox_geo_mode, int density_mode, int 
   a.work_counter = nullptr;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;
+  SVR_TRY(launch_contrib_compact(bp.cbits, K, bp.cscan, bp.cpart, bp.cpos, ip.ranges, ip.tile_last, n_tiles,
+                                 ip.cranges, st));
+  a.cpos = bp.cpos; a.cranges = ip.cranges;
   {
     LaunchTimer timer(1, st);
     SVR_TRY(launch_render_backward(a, st));
@@ -457,7 +461,7 @@ struct BatchPtrs {
       : g(ws, V, P),
         b((char*)ws + GeomLayout(V, P).total, cap),
         im((char*)ws + GeomLayout(V, P).total + BinLayout(cap).total, V, n_pix, n_tiles),
-        pr((char*)ws + GeomLayout(V, P).total + BinLayout(cap).total + ImageLayout(V, n_pix, n_tiles).total, P) {}
+        pr((char*)ws + GeomLayout(V, P).total + BinLayout(cap).total + ImageLayout(V, n_pix, n_tiles).total, P, V) {}
 };
 
 int check_batch(int V, int P, long long n_pix, long long n_tiles, long long cap) {
@@ -474,7 +478,7 @@ extern "C" {
 
 size_t svr_batch_workspace_bytes(int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap) {
   return GeomLayout(n_views, P).total + BinLayout(K_cap).total + ImageLayout(n_views, n_pixels, n_tiles).total +
-         PairLayout(P).total;
+         PairLayout(P, n_views).total;
 }
 
 int svr_batch_setup(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
@@ -528,7 +532,7 @@ int svr_batch_preprocess(void* ws, int n_views, int P, long long n_pixels, long 
   BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
   const long long n = 8LL * P;
   SVR_TRY(launch_preprocess_views(bp.g.views, n_views, P, vox_centers, vox_lengths, is_leaf, bp.g.n_dup, bp.g.bbox,
-                                  bp.g.quad, bp.pr.pair_cnt, st));
+                                  bp.g.quad, bp.pr.pair_cnt, bp.pr.vmask, st));
   SVR_TRY(launch_pair_gather(n, pair_order, bp.pr.pair_cnt, bp.pr.cnt_ranked, st));
   SVR_TRY(launch_scan(bp.pr.cnt_ranked, nullptr, n, bp.pr.partial, bp.pr.scan_ranked, true, st));
   return 0;
@@ -574,7 +578,7 @@ int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long l
   uint32_t* tiles_alt = reinterpret_cast<uint32_t*>(bp.b.keys_alt);
   const bool odd = tile_sort_passes(n_tiles) % 2 == 1;
   SVR_TRY(launch_pair_emit(bp.g.views, n_views, P, n, pair_order, bp.pr.cnt_ranked, bp.pr.scan_ranked, bp.g.bbox,
-                           bp.g.quad, odd ? tiles_alt : tiles, odd ? bp.b.vals_alt : bp.b.vals, K_cap, st));
+                           bp.pr.vmask, odd ? tiles_alt : tiles, odd ? bp.b.vals_alt : bp.b.vals, K_cap, st));
   SVR_TRY(launch_tile_sort(tiles, bp.b.vals, tiles_alt, bp.b.vals_alt, n_dev, K_cap, n_tiles, bp.b.hist,
                            bp.b.partial, st));
   SVR_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(bp.im.ranges), 2 * n_tiles, 0u, st));
@@ -591,6 +595,7 @@ int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long l
   a.tile_last = bp.im.tile_last; a.n_contrib = bp.im.n_contrib;
   a.color = out_color; a.depth = out_depth; a.normal = out_normal; a.T = out_T; a.max_w = nullptr;
   a.work_counter = bp.im.misc;
+  a.cbits = bp.b.cbits; a.k_cap = K_cap;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;
   {
@@ -627,6 +632,9 @@ int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long lo
   a.work_counter = bp.im.misc + 1;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;
+  SVR_TRY(launch_contrib_compact(bp.b.cbits, K_cap, bp.b.cscan, bp.b.cpart, bp.b.cpos, bp.im.ranges, bp.im.tile_last,
+                                 n_tiles, bp.im.cranges, st));
+  a.cpos = bp.b.cpos; a.cranges = bp.im.cranges;
   {
     LaunchTimer timer(1, st);
     SVR_TRY(launch_render_backward(a, st));

@@ -385,25 +385,38 @@ __global__ __launch_bounds__(256) void pair_rank_kernel(long long n_pairs, const
 __global__ __launch_bounds__(256) void preprocess_views_kernel(
     const ViewDesc* __restrict__ views, int V, int P, const float* __restrict__ centers,
     const float* __restrict__ lengths, const uint8_t* __restrict__ is_leaf, uint32_t* __restrict__ n_dup,
-    uint2* __restrict__ bboxes, uint8_t* __restrict__ quads, uint32_t* __restrict__ pair_cnt) {
+    uint2* __restrict__ bboxes, uint8_t* __restrict__ quads, uint32_t* __restrict__ pair_cnt,
+    uint32_t* __restrict__ pair_vmask) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P) return;
   const bool leaf = is_leaf[i] != 0;
   const float cx = centers[3 * i + 0], cy = centers[3 * i + 1], cz = centers[3 * i + 2];
   const float len = lengths[i];
+  const int nw = (V + 31) >> 5;
   uint32_t c8[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  uint32_t m8[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};   // views (of the current 32) that see octant q
   for (int v = 0; v < V; ++v) {
     uint32_t nd, qbits;
     uint2 bb;
     preprocess_one(views[v], leaf, cx, cy, cz, len, nd, bb, qbits);
     const long long o = (long long)v * P + i;
     n_dup[o] = nd;
-    bboxes[o] = bb;
     quads[o] = (uint8_t)qbits;
     if (nd) {
+      bboxes[o] = bb;        // read only where n_dup > 0
       const uint32_t nt = nd / (uint32_t)__popc(qbits);
 #pragma unroll
-      for (int q = 0; q < 8; ++q) c8[q] += ((qbits >> q) & 1u) ? nt : 0u;
+      for (int q = 0; q < 8; ++q) {
+        c8[q] += ((qbits >> q) & 1u) ? nt : 0u;
+        m8[q] |= ((qbits >> q) & 1u) << (v & 31);
+      }
+    }
+    if ((v & 31) == 31 || v == V - 1) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        pair_vmask[(8LL * i + q) * nw + (v >> 5)] = m8[q];
+        m8[q] = 0u;
+      }
     }
   }
   uint4* out = reinterpret_cast<uint4*>(pair_cnt + 8LL * i);
@@ -420,36 +433,113 @@ __global__ __launch_bounds__(256) void pair_gather_kernel(long long n_pairs, con
 }
 
 // Keys of each (voxel, octant) pair at its rank-order offset: global tile id and value
-// octant << 29 | voxel, view by view and tile by tile.
-__global__ __launch_bounds__(256) void pair_emit_kernel(
-    const ViewDesc* __restrict__ views, int V, int P, long long n_pairs, const uint32_t* __restrict__ order,
-    const uint32_t* __restrict__ cnt_ranked, const uint32_t* __restrict__ scan_ranked,
-    const uint2* __restrict__ bboxes, const uint8_t* __restrict__ quads, uint32_t* __restrict__ tiles,
-    uint32_t* __restrict__ vals, long long cap) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_pairs) return;
-  const uint32_t c = cnt_ranked[i];
-  if (c == 0) return;
-  const uint32_t pv = order[i];
-  const uint32_t vox = pv >> 3, q = pv & 7u;
+// octant << 29 | voxel, view by view and tile by tile.  A workgroup handles 256 consecutive pairs,
+// whose keys form one contiguous output range; when that range fits kEmitBuf keys they are
+// assembled in LDS and written with coalesced stores (per-thread runs are short and would
+// otherwise leave partly written cache lines), else each thread writes its run directly.
+constexpr int kEmitBuf = 4096;
+
+__device__ inline void emit_pair_keys(const ViewDesc* __restrict__ views, int V, int P, uint32_t vox, uint32_t q,
+                                      const uint2* __restrict__ bboxes, const uint32_t* __restrict__ vmask,
+                                      uint32_t* tiles, uint32_t* vals, long long off, long long lim) {
   const uint32_t val = (q << 29) | vox;
-  uint32_t off = scan_ranked[i] - c;
-  for (int v = 0; v < V; ++v) {
+  const int nw = (V + 31) >> 5;
+  for (int wi = 0; wi < nw; ++wi)
+  for (uint32_t m = vmask[(8LL * vox + q) * nw + wi]; m; m &= m - 1u) {
+    const int v = (wi << 5) + __builtin_ctz(m);
     const long long o = (long long)v * P + vox;
-    if (!((quads[o] >> q) & 1u)) continue;
     const ViewDesc& vd = views[v];
     const uint2 bb = bboxes[o];
     int tx0, ty0, tx1, ty1;
     bbox_tiles(bb.x, bb.y, vd.tiles_x, vd.tiles_y, tx0, ty0, tx1, ty1);
     for (int y = ty0; y <= ty1; ++y)
       for (int x = tx0; x <= tx1; ++x) {
-        if ((long long)off < cap) {
+        if (off < lim) {
           tiles[off] = (uint32_t)(vd.tile_base + y * vd.tiles_x + x);
           vals[off] = val;
         }
         ++off;
       }
   }
+}
+
+__global__ __launch_bounds__(256) void pair_emit_kernel(
+    const ViewDesc* __restrict__ views, int V, int P, long long n_pairs, const uint32_t* __restrict__ order,
+    const uint32_t* __restrict__ cnt_ranked, const uint32_t* __restrict__ scan_ranked,
+    const uint2* __restrict__ bboxes, const uint32_t* __restrict__ vmask, uint32_t* __restrict__ tiles,
+    uint32_t* __restrict__ vals, long long cap) {
+  __shared__ uint32_t s_tiles[kEmitBuf], s_vals[kEmitBuf];
+  const long long i0 = (long long)blockIdx.x * 256;
+  const long long i = i0 + threadIdx.x;
+  const long long i_last = min(n_pairs, i0 + 256) - 1;
+  // Output range of the block: [first, end).
+  const long long first = (long long)scan_ranked[i0] - cnt_ranked[i0];
+  const long long end = scan_ranked[i_last];
+  const long long total = end - first;
+  uint32_t c = 0, pv = 0;
+  long long off = 0;
+  if (i < n_pairs) {
+    c = cnt_ranked[i];
+    if (c) {
+      pv = order[i];
+      off = (long long)scan_ranked[i] - c;
+    }
+  }
+  if (total <= kEmitBuf) {
+    if (c) emit_pair_keys(views, V, P, pv >> 3, pv & 7u, bboxes, vmask, s_tiles, s_vals, off - first, kEmitBuf);
+    __syncthreads();
+    for (long long k = threadIdx.x; k < total; k += 256) {
+      const long long dst = first + k;
+      if (dst < cap) {
+        tiles[dst] = s_tiles[k];
+        vals[dst] = s_vals[k];
+      }
+    }
+  } else if (c) {
+    emit_pair_keys(views, V, P, pv >> 3, pv & 7u, bboxes, vmask, tiles, vals, off, cap);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Compaction of the contributing entries marked by the forward (one bit per list position):
+// positions in list order and, per tile, its range in that compacted list.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void popc_words_kernel(const uint32_t* __restrict__ bits, long long nw,
+                                                         uint32_t* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nw) out[i] = (uint32_t)__popc(bits[i]);
+}
+
+__global__ __launch_bounds__(256) void contrib_positions_kernel(const uint32_t* __restrict__ bits,
+                                                                const uint32_t* __restrict__ cscan, long long nw,
+                                                                long long k_cap, uint32_t* __restrict__ cpos) {
+  const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw) return;
+  uint32_t b = bits[w];
+  uint32_t o = cscan[w];
+  while (b) {
+    const long long p = 32 * w + __builtin_ctz(b);
+    if (p < k_cap) cpos[o] = (uint32_t)p;
+    ++o;
+    b &= b - 1u;
+  }
+}
+
+__device__ inline uint32_t contrib_prefix(const uint32_t* __restrict__ bits, const uint32_t* __restrict__ cscan,
+                                          uint32_t p) {
+  const uint32_t w = p >> 5;
+  return cscan[w] + (uint32_t)__popc(bits[w] & ((1u << (p & 31)) - 1u));
+}
+
+__global__ __launch_bounds__(256) void contrib_ranges_kernel(const uint2* __restrict__ ranges,
+                                                             const uint32_t* __restrict__ tile_last, long long n_tiles,
+                                                             const uint32_t* __restrict__ bits,
+                                                             const uint32_t* __restrict__ cscan,
+                                                             uint2* __restrict__ cranges) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tiles) return;
+  const uint32_t b = ranges[t].x, e = tile_last[t];
+  cranges[t] = e > b ? make_uint2(contrib_prefix(bits, cscan, b), contrib_prefix(bits, cscan, e)) : make_uint2(0u, 0u);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -561,10 +651,10 @@ hipError_t launch_pair_rank(long long n_pairs, const int64_t* octpaths, uint64_t
 
 hipError_t launch_preprocess_views(const ViewDesc* views, int V, int P, const float* centers, const float* lengths,
                                    const uint8_t* is_leaf, uint32_t* n_dup, uint2* bboxes, uint8_t* quads,
-                                   uint32_t* pair_cnt, hipStream_t st) {
+                                   uint32_t* pair_cnt, uint32_t* pair_vmask, hipStream_t st) {
   if (P <= 0 || V <= 0) return hipSuccess;
   preprocess_views_kernel<<<ceil_div(P, 256), 256, 0, st>>>(views, V, P, centers, lengths, is_leaf, n_dup, bboxes,
-                                                            quads, pair_cnt);
+                                                            quads, pair_cnt, pair_vmask);
   return hipGetLastError();
 }
 
@@ -577,10 +667,25 @@ hipError_t launch_pair_gather(long long n_pairs, const uint32_t* order, const ui
 
 hipError_t launch_pair_emit(const ViewDesc* views, int V, int P, long long n_pairs, const uint32_t* order,
                             const uint32_t* cnt_ranked, const uint32_t* scan_ranked, const uint2* bboxes,
-                            const uint8_t* quads, uint32_t* tiles, uint32_t* vals, long long cap, hipStream_t st) {
+                            const uint32_t* vmask, uint32_t* tiles, uint32_t* vals, long long cap, hipStream_t st) {
   if (n_pairs <= 0) return hipSuccess;
   pair_emit_kernel<<<(unsigned)((n_pairs + 255) / 256), 256, 0, st>>>(views, V, P, n_pairs, order, cnt_ranked,
-                                                                      scan_ranked, bboxes, quads, tiles, vals, cap);
+                                                                      scan_ranked, bboxes, vmask, tiles, vals, cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_contrib_compact(const uint32_t* cbits, long long k_cap, uint32_t* cscan, uint32_t* cpart,
+                                  uint32_t* cpos, const uint2* ranges, const uint32_t* tile_last, long long n_tiles,
+                                  uint2* cranges, hipStream_t st) {
+  if (k_cap <= 0 || n_tiles <= 0) return hipSuccess;
+  const long long nw = contrib_words(k_cap);
+  const unsigned gw = (unsigned)((nw + 255) / 256);
+  popc_words_kernel<<<gw, 256, 0, st>>>(cbits, nw, cscan);
+  hipError_t e = launch_scan(cscan, nullptr, nw, cpart, cscan, false, st);
+  if (e != hipSuccess) return e;
+  contrib_positions_kernel<<<gw, 256, 0, st>>>(cbits, cscan, nw, k_cap, cpos);
+  contrib_ranges_kernel<<<(unsigned)((n_tiles + 255) / 256), 256, 0, st>>>(ranges, tile_last, n_tiles, cbits, cscan,
+                                                                           cranges);
   return hipGetLastError();
 }
 

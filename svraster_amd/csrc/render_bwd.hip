@@ -366,7 +366,8 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
   __shared__ float4 s_gt[4][kConcen ? 64 : 1];
   __shared__ uint64_t s_m[4][kBwdStage];   // per entry: its pixels in the block
   __shared__ uint16_t s_pref[4][kBwdStage];  // per entry: first position of its run
-  __shared__ int32_t s_jmin[4][64];        // per pixel: first slot before its last contributor
+  __shared__ uint32_t s_rel[kBwdStage];    // per slot: list position relative to the tile start
+  __shared__ uint32_t s_lastpix[4][64];    // per pixel: contributor count (list positions it takes: < it)
   __shared__ unsigned long long s_cs[4][2][64];  // per pixel: contributing entries (lo, hi halves)
   __shared__ uint16_t s_own[4][kBwdWin];   // 1 + entry whose run starts at the position; later the
                                            // contributing hits in entry-major order
@@ -406,8 +407,10 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
     const Ray r = pixel_ray(V, px, py);
     const float cam_scale = fmaxf(fabsf(r.ox), fmaxf(fabsf(r.oy), fabsf(r.oz)));
     const uint32_t start = a.ranges[g].x;
-    const uint32_t end = a.tile_last[g];
-    const int n = (int)(end - start);
+    // The tile's entries that contributed to any of its pixels in the forward (the others get no
+    // gradient), as positions in the compacted list, walked back to front.
+    const uint2 cr = a.cranges[g];
+    const int n = (int)(cr.y - cr.x);
 
     const long long HW = (long long)W * H;
     const long long pid = (long long)py * W + px;
@@ -454,18 +457,19 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
       __syncthreads();
       const int cnt = min(kBwdStage, n - base);
       if (t < cnt) {
-        const uint32_t e = end - 1 - (uint32_t)(base + t);
+        const uint32_t e = a.cpos[cr.y - 1 - (uint32_t)(base + t)];
         stage_entry_b(a, e, vi, kNormal, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t, true, s, cam_scale);
+        s_rel[t] = e - start;
         s_used[t] = 0u;
       }
       for (int k = t; k < kBwdStage * 12; k += 256) (&s_acc[0][0])[k] = 0.f;
       __syncthreads();
-      // Slot j holds list position rel_j = n - 1 - base - j (relative to the tile start); a pixel
-      // takes the slots with rel_j < last, i.e. j >= n - base - last.
-      const bool live = inside && last + (uint32_t)base + (uint32_t)cnt > (uint32_t)n;
+      // Slot j holds list position s_rel[j] (decreasing in j); a pixel takes the slots with
+      // s_rel[j] < last, its contributor count.
+      const bool live = inside && last > s_rel[cnt - 1];
       const uint64_t live_m = __ballot(live);
       if (live_m != 0ull) {
-        s_jmin[w][lane] = live ? max(n - base - (int)last, 0) : kBwdStage;
+        s_lastpix[w][lane] = last;
         const uint64_t m_lo = lane < cnt ? entry_block_mask_b(s_bb, s_rgb, lane, qx, qy, s_om[w]) & live_m : 0ull;
         const uint64_t m_hi = lane + 64 < cnt ? entry_block_mask_b(s_bb, s_rgb, lane + 64, qx, qy, s_om[w]) & live_m : 0ull;
         const uint32_t c_lo = (uint32_t)__popcll(m_lo), c_hi = (uint32_t)__popcll(m_hi);
@@ -505,7 +509,7 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
             if (valid) {
               const int j = (int)h - 1;
               const int bit = select_set_bit(s_m[w][j], (int)(w0 + k - (uint32_t)s_pref[w][j]));
-              if (j >= s_jmin[w][bit]) {
+              if (s_rel[j] < s_lastpix[w][bit]) {
                 const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
                 const float4 ch = s_ch[j];
                 float ta, tb;
@@ -735,6 +739,7 @@ static hipError_t launch_bwd_c(const BwdArgs& a, hipStream_t st, unsigned grid) 
 
 hipError_t launch_render_backward(const BwdArgs& a, hipStream_t st) {
   if (a.total_tiles <= 0) return hipSuccess;
+
   unsigned grid = (unsigned)a.total_tiles;
   if (a.work_counter) grid = (unsigned)min(a.total_tiles, 256 * 8);
   if (a.need_depth && a.need_normal) return launch_bwd_c<true, true>(a, st, grid);

@@ -271,6 +271,31 @@ __device__ inline int pop_entry(uint64_t& lo, uint64_t& hi) {
   return j;
 }
 
+// OR of a 128-bit entry set over the wave, added to the bit array at list position p0.
+__device__ inline void mark_contributing(uint32_t* bits, long long p0, uint64_t lo, uint64_t hi, int lane) {
+  uint32_t wv[4] = {(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    uint32_t x = wv[c];
+    x |= xor_lane<32>(x, lane);
+    x |= xor_lane<16>(x, lane);
+    x |= xor_lane<8>(x, lane);
+    x |= xor_lane<4>(x, lane);
+    x |= xor_lane<2>(x, lane);
+    x |= xor_lane<1>(x, lane);
+    wv[c] = x;
+  }
+  if (lane != 0) return;
+  const long long w0 = p0 >> 5;
+  const uint32_t sh = (uint32_t)(p0 & 31);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if (!wv[c]) continue;
+    atomicOr(bits + w0 + c, wv[c] << sh);
+    if (sh) atomicOr(bits + w0 + c + 1, wv[c] >> (32u - sh));
+  }
+}
+
 template <bool kDepth, bool kNormal, bool kMaxW, bool kQueue>
 __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
   __shared__ float4 s_ch[kPairStage], s_ml[kPairStage], s_g0[kPairStage], s_g1[kPairStage], s_rgb[kPairStage];
@@ -361,6 +386,7 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
       s_pref[w][lane] = (uint16_t)p_lo;
       s_pref[w][lane + 64] = (uint16_t)p_hi;
       uint32_t carry = 0;   // 1 + entry owning the last position handled so far
+      uint64_t pr_lo = 0ull, pr_hi = 0ull;   // entries this pixel composited in this batch
 
       for (uint32_t w0 = 0; w0 < total; w0 += kPairWin) {
         const uint32_t w1 = min(total, w0 + (uint32_t)kPairWin);
@@ -427,7 +453,8 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
         }
         wave_lds_sync();
         // 4. composite this pixel's contributing pairs of the window, in entry order.
-        uint64_t cs_lo = s_cs[w][0][lane], cs_hi = s_cs[w][1][lane];
+        const uint64_t cs0_lo = s_cs[w][0][lane], cs0_hi = s_cs[w][1][lane];
+        uint64_t cs_lo = cs0_lo, cs_hi = cs0_hi;
         while ((cs_lo | cs_hi) != 0ull && !done) {
           const int j = pop_entry(cs_lo, cs_hi);
           const uint32_t k = (uint32_t)s_pref[w][j] + (uint32_t)__popcll(s_m[w][j] & lt_mask) - w0;
@@ -449,8 +476,12 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
           if (kMaxW)
             atomicMax(reinterpret_cast<uint32_t*>(a.max_w) + (__float_as_uint(rgb.w) & kVoxMask), __float_as_uint(wgt));
         }
+        pr_lo |= cs0_lo & ~cs_lo;     // composited: the set minus what was left when T saturated
+        pr_hi |= cs0_hi & ~cs_hi;
         wave_lds_sync();
       }
+      // Entries of this batch that contributed to any pixel of the wave: one bit per list position.
+      if (__ballot((pr_lo | pr_hi) != 0ull) != 0ull) mark_contributing(a.cbits, (long long)rg.x + base, pr_lo, pr_hi, lane);
     }
 
     float Dmed = 0.f;
@@ -484,6 +515,10 @@ static hipError_t launch_fwd_mw(const FwdArgs& a, hipStream_t st, unsigned grid)
 
 hipError_t launch_render_forward(const FwdArgs& a, hipStream_t st) {
   if (a.total_tiles <= 0) return hipSuccess;
+  // Contributing-entry bits for the backward; the one-entry-at-a-time kernel marks nothing, so its
+  // bits are all set (the backward then takes every entry).
+  hipError_t e = launch_fill_u32(a.cbits, contrib_words(a.k_cap), a.variant == 1 ? 0xFFFFFFFFu : 0u, st);
+  if (e != hipSuccess) return e;
   unsigned grid = (unsigned)a.total_tiles;
   if (a.work_counter) grid = (unsigned)min(a.total_tiles, 256 * 8);
   if (a.need_depth && a.need_normal) return launch_fwd_mw<true, true>(a, st, grid);

@@ -68,8 +68,10 @@ struct GeomPtrs {
 };
 
 // Binning state.
+inline long long contrib_words(long long K) { return (K + 31) / 32 + 2; }
+
 struct BinLayout {
-  size_t keys, vals, keys_alt, vals_alt, hist, partial, total;
+  size_t keys, vals, keys_alt, vals_alt, hist, partial, cbits, cscan, cpart, cpos, total;
   explicit BinLayout(long long K) {
     Carver c;
     keys = c.take<uint64_t>(K);
@@ -79,6 +81,11 @@ struct BinLayout {
     const long long hn = 512LL * (long long)sort_tile_count(K);   // up to 9-bit digits
     hist = c.take<uint32_t>(hn);
     partial = c.take<uint32_t>((long long)scan_partial_count(hn) + 1);
+    const long long nw = contrib_words(K);
+    cbits = c.take<uint32_t>(nw);
+    cscan = c.take<uint32_t>(nw);
+    cpart = c.take<uint32_t>((long long)scan_partial_count(nw) + 1);
+    cpos = c.take<uint32_t>(K);
     total = c.off;
   }
 };
@@ -90,6 +97,10 @@ struct BinPtrs {
   uint32_t* vals_alt;
   uint32_t* hist;
   uint32_t* partial;
+  uint32_t* cbits;     // bit per list position: the entry contributed to a pixel of its tile
+  uint32_t* cscan;     // exclusive count of set bits before each word
+  uint32_t* cpart;
+  uint32_t* cpos;      // list positions of the contributing entries, in list order
   BinPtrs(void* base, long long K) {
     BinLayout L(K);
     char* b = (char*)base;
@@ -99,12 +110,16 @@ struct BinPtrs {
     vals_alt = (uint32_t*)(b + L.vals_alt);
     hist = (uint32_t*)(b + L.hist);
     partial = (uint32_t*)(b + L.partial);
+    cbits = (uint32_t*)(b + L.cbits);
+    cscan = (uint32_t*)(b + L.cscan);
+    cpart = (uint32_t*)(b + L.cpart);
+    cpos = (uint32_t*)(b + L.cpos);
   }
 };
 
 // Image state.
 struct ImageLayout {
-  size_t views, misc, ranges, tile_last, n_contrib, total;
+  size_t views, misc, ranges, tile_last, n_contrib, cranges, total;
   ImageLayout(long long n_views, long long n_pix, long long n_tiles) {
     Carver c;
     views = c.take<ViewDesc>(n_views);
@@ -112,6 +127,7 @@ struct ImageLayout {
     ranges = c.take<uint2>(n_tiles);
     tile_last = c.take<uint32_t>(n_tiles);
     n_contrib = c.take<uint32_t>(n_pix);
+    cranges = c.take<uint2>(n_tiles);
     total = c.off;
   }
 };
@@ -122,6 +138,7 @@ struct ImagePtrs {
   uint2* ranges;
   uint32_t* tile_last;
   uint32_t* n_contrib;
+  uint2* cranges;      // per tile: its contributing entries in the compacted position list
   ImagePtrs(void* base, long long n_views, long long n_pix, long long n_tiles) {
     ImageLayout L(n_views, n_pix, n_tiles);
     char* b = (char*)base;
@@ -130,20 +147,22 @@ struct ImagePtrs {
     ranges = (uint2*)(b + L.ranges);
     tile_last = (uint32_t*)(b + L.tile_last);
     n_contrib = (uint32_t*)(b + L.n_contrib);
+    cranges = (uint2*)(b + L.cranges);
   }
 };
 
 // Batched rank-order binning state (V views, P voxels): per (voxel, octant) pair its key count
-// over all views, the counts in rank order and their inclusive scan.
+// over all views, the counts in rank order and their inclusive scan, and its view mask.
 struct PairLayout {
-  size_t pair_cnt, cnt_ranked, scan_ranked, partial, total;
-  explicit PairLayout(long long P) {
+  size_t pair_cnt, cnt_ranked, scan_ranked, partial, vmask, total;
+  PairLayout(long long P, long long V) {
     Carver c;
     const long long n = 8 * P;
     pair_cnt = c.take<uint32_t>(n);
     cnt_ranked = c.take<uint32_t>(n);
     scan_ranked = c.take<uint32_t>(n);
     partial = c.take<uint32_t>((long long)scan_partial_count(n) + 1);
+    vmask = c.take<uint32_t>(n * ((V + 31) / 32));
     total = c.off;
   }
 };
@@ -153,13 +172,15 @@ struct PairPtrs {
   uint32_t* cnt_ranked;
   uint32_t* scan_ranked;
   uint32_t* partial;
-  PairPtrs(void* base, long long P) {
-    PairLayout L(P);
+  uint32_t* vmask;      // per pair: the views (32 per word) whose octant set holds its octant
+  PairPtrs(void* base, long long P, long long V) {
+    PairLayout L(P, V);
     char* b = (char*)base;
     pair_cnt = (uint32_t*)(b + L.pair_cnt);
     cnt_ranked = (uint32_t*)(b + L.cnt_ranked);
     scan_ranked = (uint32_t*)(b + L.scan_ranked);
     partial = (uint32_t*)(b + L.partial);
+    vmask = (uint32_t*)(b + L.vmask);
   }
 };
 
@@ -190,12 +211,15 @@ hipError_t launch_pair_rank(long long n_pairs, const int64_t* octpaths, uint64_t
                             hipStream_t st);
 hipError_t launch_preprocess_views(const ViewDesc* views, int V, int P, const float* centers, const float* lengths,
                                    const uint8_t* is_leaf, uint32_t* n_dup, uint2* bboxes, uint8_t* quads,
-                                   uint32_t* pair_cnt, hipStream_t st);
+                                   uint32_t* pair_cnt, uint32_t* pair_vmask, hipStream_t st);
 hipError_t launch_pair_gather(long long n_pairs, const uint32_t* order, const uint32_t* pair_cnt,
                               uint32_t* cnt_ranked, hipStream_t st);
 hipError_t launch_pair_emit(const ViewDesc* views, int V, int P, long long n_pairs, const uint32_t* order,
                             const uint32_t* cnt_ranked, const uint32_t* scan_ranked, const uint2* bboxes,
-                            const uint8_t* quads, uint32_t* tiles, uint32_t* vals, long long cap, hipStream_t st);
+                            const uint32_t* vmask, uint32_t* tiles, uint32_t* vals, long long cap, hipStream_t st);
+hipError_t launch_contrib_compact(const uint32_t* cbits, long long k_cap, uint32_t* cscan, uint32_t* cpart,
+                                  uint32_t* cpos, const uint2* ranges, const uint32_t* tile_last, long long n_tiles,
+                                  uint2* cranges, hipStream_t st);
 hipError_t launch_tile_ranges_u32(const uint32_t* tiles, const uint32_t* n_dev, long long n_cap, uint2* ranges,
                                   hipStream_t st);
 
@@ -210,6 +234,8 @@ struct FwdArgs {
   uint32_t* tile_last; uint32_t* n_contrib;
   float* color; float* depth; float* normal; float* T; float* max_w;
   uint32_t* work_counter;          // null: one workgroup per tile
+  uint32_t* cbits;                 // contributing-entry bits per list position (cleared by the launcher)
+  long long k_cap;                 // list positions covered by cbits
   int need_depth, need_normal;
   int variant;                     // 0: pair-compacted (default), 1: one entry at a time
 };
@@ -229,6 +255,8 @@ struct BwdArgs {
   float* dL_drgb_views;            // [V,P,3] per-view colour gradients (null: into dL_dvox)
   float* dL_ds;                    // [1]
   uint32_t* work_counter;
+  // Contributing entries per tile (launch_contrib_compact over the forward's bits).
+  const uint32_t* cpos; const uint2* cranges;
   int need_depth, need_normal;
   int variant;                     // 0: pair-compacted (default), 1: one entry at a time
 };
