@@ -59,6 +59,47 @@ __device__ inline void stage_entry_b(const BwdArgs& a, uint32_t e, int vi, bool 
   }
 }
 
+// One staged entry held in registers between its global loads and its LDS store, so the loads
+// of the next batch are in flight while the current batch is processed.
+struct EntryRegsB {
+  uint32_t pos, val;
+  uint2 bb;
+  float cx, cy, cz, l;
+  float4 g0, g1;
+  float r, g, b;
+};
+
+__device__ inline void fetch_entry_b(EntryRegsB& e, const BwdArgs& a, uint32_t pos, uint32_t val, int vi,
+                                     const float* __restrict__ rgbs) {
+  const uint32_t vox = val & kVoxMask;
+  e.pos = pos;
+  e.val = val;
+  e.bb = a.bboxes[(long long)vi * a.P + vox];
+  e.cx = a.centers[3 * vox]; e.cy = a.centers[3 * vox + 1]; e.cz = a.centers[3 * vox + 2];
+  e.l = a.lengths[vox];
+  e.g0 = reinterpret_cast<const float4*>(a.geos)[2 * vox];
+  e.g1 = reinterpret_cast<const float4*>(a.geos)[2 * vox + 1];
+  e.r = rgbs[3 * vox]; e.g = rgbs[3 * vox + 1]; e.b = rgbs[3 * vox + 2];
+}
+
+__device__ inline void store_entry_b(const EntryRegsB& e, bool need_normal, float s, float cam_scale, float4* s_ch,
+                                     float4* s_ml, float4* s_g0, float4* s_g1, float4* s_rgb, float4* s_nrm,
+                                     uint2* s_bb, int slot) {
+  s_ch[slot] = make_float4(e.cx, e.cy, e.cz, 0.5f * e.l);
+  s_ml[slot] = make_float4(e.cx - 0.5f * e.l, e.cy - 0.5f * e.l, e.cz - 0.5f * e.l, 1.f / e.l);
+  s_g0[slot] = e.g0;
+  s_g1[slot] = e.g1;
+  const float g[8] = {e.g0.x, e.g0.y, e.g0.z, e.g0.w, e.g1.x, e.g1.y, e.g1.z, e.g1.w};
+  const float ps = cam_scale + fmaxf(fabsf(e.cx), fmaxf(fabsf(e.cy), fabsf(e.cz)));
+  s_bb[slot] = voxel_may_contribute(g, s, ps, e.l) ? e.bb : make_uint2(0xFFFFFFFFu, 0u);
+  s_rgb[slot] = make_float4(e.r, e.g, e.b, __uint_as_float(e.val));
+  if (need_normal) {
+    float nx, ny, nz, rn;
+    lin_normal(g, nx, ny, nz, rn);
+    s_nrm[slot] = make_float4(nx * rn, ny * rn, nz * rn, rn);
+  }
+}
+
 __device__ inline uint64_t bbox_block_mask_b(uint32_t b0, uint32_t b1, int qx, int qy) {
   const int x0 = max((int)(b0 >> 16) - qx, 0), x1 = min((int)(b1 >> 16) - qx, 7);
   const int y0 = max((int)(b0 & 0xFFFFu) - qy, 0), y1 = min((int)(b1 & 0xFFFFu) - qy, 7);
@@ -453,17 +494,41 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
       if (lane < 8) s_om[w][lane] = om;
     }
 
+    // Pipelined staging: entry data one batch ahead in registers, its value two batches ahead of
+    // use, its list position three batches ahead.
+    const float* rgbs_v = a.rgbs + (long long)vi * a.rgb_view_stride;
+    EntryRegsB er;
+    uint32_t nval = 0, npos = 0;
+    if (t < kBwdStage) {
+      if (t < n) {
+        const uint32_t p0 = a.cpos[cr.y - 1 - (uint32_t)t];
+        fetch_entry_b(er, a, p0, a.vals[p0], vi, rgbs_v);
+      }
+      if (kBwdStage + t < n) {
+        npos = a.cpos[cr.y - 1 - (uint32_t)(kBwdStage + t)];
+        nval = a.vals[npos];
+      }
+    }
+    uint32_t npos2 = (t < kBwdStage && 2 * kBwdStage + t < n) ? a.cpos[cr.y - 1 - (uint32_t)(2 * kBwdStage + t)] : 0u;
+
     for (int base = 0; base < n; base += kBwdStage) {
       __syncthreads();
       const int cnt = min(kBwdStage, n - base);
       if (t < cnt) {
-        const uint32_t e = a.cpos[cr.y - 1 - (uint32_t)(base + t)];
-        stage_entry_b(a, e, vi, kNormal, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t, true, s, cam_scale);
-        s_rel[t] = e - start;
+        store_entry_b(er, kNormal, s, cam_scale, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t);
+        s_rel[t] = er.pos - start;
         s_used[t] = 0u;
       }
       for (int k = t; k < kBwdStage * 12; k += 256) (&s_acc[0][0])[k] = 0.f;
       __syncthreads();
+      if (t < kBwdStage) {
+        if (base + kBwdStage + t < n) fetch_entry_b(er, a, npos, nval, vi, rgbs_v);
+        if (base + 2 * kBwdStage + t < n) {
+          npos = npos2;
+          nval = a.vals[npos];
+        }
+        if (base + 3 * kBwdStage + t < n) npos2 = a.cpos[cr.y - 1 - (uint32_t)(base + 3 * kBwdStage + t)];
+      }
       // Slot j holds list position s_rel[j] (decreasing in j); a pixel takes the slots with
       // s_rel[j] < last, its contributor count.
       const bool live = inside && last > s_rel[cnt - 1];

@@ -66,6 +66,48 @@ __device__ inline void stage_entry(const uint32_t* __restrict__ vals, uint32_t e
   }
 }
 
+// One staged entry held in registers between its global loads and its LDS store, so the loads
+// of the next batch are in flight while the current batch is composited.
+struct EntryRegs {
+  uint32_t val;
+  uint2 bb;
+  float cx, cy, cz, l;
+  float4 g0, g1;
+  float r, g, b;
+};
+
+__device__ inline void fetch_entry(EntryRegs& e, uint32_t val, int view, long long P, const uint2* __restrict__ bboxes,
+                                   const float* __restrict__ centers, const float* __restrict__ lengths,
+                                   const float* __restrict__ geos, const float* __restrict__ rgbs) {
+  const uint32_t vox = val & kVoxMask;
+  e.val = val;
+  e.bb = bboxes[(long long)view * P + vox];
+  e.cx = centers[3 * vox + 0]; e.cy = centers[3 * vox + 1]; e.cz = centers[3 * vox + 2];
+  e.l = lengths[vox];
+  e.g0 = reinterpret_cast<const float4*>(geos)[2 * vox];
+  e.g1 = reinterpret_cast<const float4*>(geos)[2 * vox + 1];
+  e.r = rgbs[3 * vox + 0]; e.g = rgbs[3 * vox + 1]; e.b = rgbs[3 * vox + 2];
+}
+
+// LDS copy of a fetched entry; an entry that cannot reach the minimum alpha gets an empty bbox.
+__device__ inline void store_entry(const EntryRegs& e, bool need_normal, float s, float cam_scale, float4* s_ch,
+                                   float4* s_ml, float4* s_g0, float4* s_g1, float4* s_rgb, float4* s_nrm, uint2* s_bb,
+                                   int slot) {
+  s_ch[slot] = make_float4(e.cx, e.cy, e.cz, 0.5f * e.l);
+  s_ml[slot] = make_float4(e.cx - 0.5f * e.l, e.cy - 0.5f * e.l, e.cz - 0.5f * e.l, 1.f / e.l);
+  s_g0[slot] = e.g0;
+  s_g1[slot] = e.g1;
+  const float g[8] = {e.g0.x, e.g0.y, e.g0.z, e.g0.w, e.g1.x, e.g1.y, e.g1.z, e.g1.w};
+  const float ps = cam_scale + fmaxf(fabsf(e.cx), fmaxf(fabsf(e.cy), fabsf(e.cz)));
+  s_bb[slot] = voxel_may_contribute(g, s, ps, e.l) ? e.bb : make_uint2(0xFFFFFFFFu, 0u);
+  s_rgb[slot] = make_float4(e.r, e.g, e.b, __uint_as_float(e.val));
+  if (need_normal) {
+    float nx, ny, nz, rn;
+    lin_normal(g, nx, ny, nz, rn);
+    s_nrm[slot] = make_float4(nx * rn, ny * rn, nz * rn, rn);
+  }
+}
+
 // Median depth refined with 16 sub-steps inside the median voxel (forward.cu:450-521).
 __device__ inline float median_depth(const FwdArgs& a, const Ray& r, int v, float dmed_T, float s) {
   const float cx = a.centers[3 * v], cy = a.centers[3 * v + 1], cz = a.centers[3 * v + 2];
@@ -360,13 +402,24 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
     float dmed_T = 0.f;
     bool done = !inside;
 
+    // Entry data one batch ahead in registers, and its value one more batch ahead.
+    const float* rgbs_v = a.rgbs + (long long)vi * a.rgb_view_stride;
+    EntryRegs er;
+    uint32_t nval = 0;
+    if (t < kPairStage) {
+      if (t < n) fetch_entry(er, a.vals[rg.x + t], vi, a.P, a.bboxes, a.centers, a.lengths, a.geos, rgbs_v);
+      if (kPairStage + t < n) nval = a.vals[rg.x + kPairStage + t];
+    }
+
     for (int base = 0; base < n; base += kPairStage) {
       if (__syncthreads_count(done) == 256) break;
       if (t < kPairStage && base + t < n)
-        stage_entry(a.vals, rg.x + base + t, vi, a.P, a.bboxes, a.centers, a.lengths, a.geos,
-                    a.rgbs + (long long)vi * a.rgb_view_stride, kNormal, s_ch, s_ml, s_g0, s_g1, s_rgb,
-                    s_nrm, s_bb, t, true, s, cam_scale);
+        store_entry(er, kNormal, s, cam_scale, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t);
       __syncthreads();
+      if (t < kPairStage) {
+        if (base + kPairStage + t < n) fetch_entry(er, nval, vi, a.P, a.bboxes, a.centers, a.lengths, a.geos, rgbs_v);
+        if (base + 2 * kPairStage + t < n) nval = a.vals[rg.x + base + 2 * kPairStage + t];
+      }
       const int cnt = min(kPairStage, n - base);
       const uint64_t live = __ballot(!done);
       if (live == 0ull) continue;
