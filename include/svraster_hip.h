@@ -180,7 +180,7 @@ int svr_batch_setup(void* ws, int n_views, int P, long long n_pixels, long long 
  * (auxiliary.h:40-59, forward.cu:749-802).  tmp: svr_batch_pair_order_bytes(P) bytes. */
 size_t svr_batch_pair_order_bytes(int P);
 int svr_batch_pair_order(int P, const int64_t* octree_paths, void* tmp, uint32_t* order_out, void* stream);
-/* Preprocess (bbox, octants, n_dup; preprocess.cu:60-153) of every (view, voxel), voxel-major,
+/* Preprocess (bbox and octant set; preprocess.cu:60-153) of every (view, voxel), voxel-major,
  * and the key count of every (voxel, octant) pair over all views, scanned in rank order. */
 int svr_batch_preprocess(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                          const float* vox_centers, const float* vox_lengths, const uint8_t* is_leaf,

@@ -531,7 +531,7 @@ int svr_batch_preprocess(void* ws, int n_views, int P, long long n_pixels, long 
   hipStream_t st = as_stream(stream);
   BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
   const long long n = 8LL * P;
-  SVR_TRY(launch_preprocess_views(bp.g.views, n_views, P, vox_centers, vox_lengths, is_leaf, bp.g.n_dup, bp.g.bbox,
+  SVR_TRY(launch_preprocess_views(bp.g.views, n_views, P, vox_centers, vox_lengths, is_leaf, nullptr, bp.g.bbox,
                                   bp.g.quad, bp.pr.pair_cnt, bp.pr.vmask, st));
   SVR_TRY(launch_pair_gather(n, pair_order, bp.pr.pair_cnt, bp.pr.cnt_ranked, st));
   SVR_TRY(launch_scan(bp.pr.cnt_ranked, nullptr, n, bp.pr.partial, bp.pr.scan_ranked, true, st));
@@ -557,7 +557,7 @@ int svr_batch_sh_forward(void* ws, int n_views, int P, long long n_pixels, long 
   if (degree < 0 || degree > 3 || M < (degree + 1) * (degree + 1)) return fail(1, "bad SH degree%s %lld", "", degree);
   if (M > 16) return fail(1, "batched SH supports at most 16 coefficients per channel%s (got %lld)", "", M);
   BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
-  SVR_TRY(launch_sh_views_forward(P, n_views, bp.g.n_dup, bp.g.views, degree, M, vox_centers, sh0, shs, rgbs_views,
+  SVR_TRY(launch_sh_views_forward(P, n_views, bp.g.quad, bp.g.views, degree, M, vox_centers, sh0, shs, rgbs_views,
                                   as_stream(stream)));
   return 0;
 }
@@ -650,7 +650,7 @@ int svr_batch_sh_backward(void* ws, int n_views, int P, long long n_pixels, long
   if (degree < 0 || degree > 3 || M < (degree + 1) * (degree + 1)) return fail(1, "bad SH degree%s %lld", "", degree);
   if (M > 16) return fail(1, "batched SH supports at most 16 coefficients per channel%s (got %lld)", "", M);
   BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
-  SVR_TRY(launch_sh_views_backward(P, n_views, bp.g.n_dup, bp.g.views, degree, M, vox_centers, dL_drgb_views,
+  SVR_TRY(launch_sh_views_backward(P, n_views, bp.g.quad, bp.g.views, degree, M, vox_centers, dL_drgb_views,
                                    dL_dsh0, dL_dshs, as_stream(stream)));
   return 0;
 }

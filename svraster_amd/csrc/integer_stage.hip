@@ -400,7 +400,7 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(
     uint2 bb;
     preprocess_one(views[v], leaf, cx, cy, cz, len, nd, bb, qbits);
     const long long o = (long long)v * P + i;
-    n_dup[o] = nd;
+    if (n_dup) n_dup[o] = nd;
     quads[o] = (uint8_t)qbits;
     if (nd) {
       bboxes[o] = bb;        // read only where n_dup > 0

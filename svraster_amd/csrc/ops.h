@@ -12,10 +12,10 @@ hipError_t launch_sh_backward(int n, const int64_t* idx, int deg, int M, const f
                               const float* cam, const float* dL_drgb, float* dsh0, float* dshs,
                               hipStream_t st);
 struct ViewDesc;
-hipError_t launch_sh_views_forward(int P, int V, const uint32_t* n_dup, const ViewDesc* views, int deg, int M,
+hipError_t launch_sh_views_forward(int P, int V, const uint8_t* quads, const ViewDesc* views, int deg, int M,
                                    const float* centers, const float* sh0, const float* shs, float* rgbs_views,
                                    hipStream_t st);
-hipError_t launch_sh_views_backward(int P, int V, const uint32_t* n_dup, const ViewDesc* views, int deg, int M,
+hipError_t launch_sh_views_backward(int P, int V, const uint8_t* quads, const ViewDesc* views, int deg, int M,
                                     const float* centers, float* drgb_views, float* dsh0, float* dshs,
                                     hipStream_t st);
 hipError_t launch_voxel_activity(int P, const uint8_t* is_leaf, const float* geos, const float* s_val,

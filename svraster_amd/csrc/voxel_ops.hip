@@ -139,17 +139,17 @@ __global__ __launch_bounds__(256) void gather_geo_bw_kernel(int n, const int64_t
 // degree-3 row (M <= 16); the odd stride keeps the per-thread LDS writes free of bank conflicts.
 constexpr int kShRowMax = 45;
 
-// Bit k set when view v0 + k sees voxel i (n_dup > 0); the loads are independent, so they are
-// all in flight at once instead of one dependent load per view.
-__device__ inline uint32_t visible_views(const uint32_t* __restrict__ n_dup, int P, int i, int v0, int nv) {
+// Bit k set when view v0 + k sees voxel i (non-empty octant set, i.e. n_dup > 0); the loads are
+// independent, so they are all in flight at once instead of one dependent load per view.
+__device__ inline uint32_t visible_views(const uint8_t* __restrict__ quads, int P, int i, int v0, int nv) {
   uint32_t vis = 0u;
 #pragma unroll 8
-  for (int k = 0; k < nv; ++k) vis |= (n_dup[(long long)(v0 + k) * P + i] != 0u ? 1u : 0u) << k;
+  for (int k = 0; k < nv; ++k) vis |= (quads[(long long)(v0 + k) * P + i] != 0 ? 1u : 0u) << k;
   return vis;
 }
 
 template <int kDeg>
-__global__ __launch_bounds__(256) void sh_views_forward_kernel(int P, int V, const uint32_t* __restrict__ n_dup,
+__global__ __launch_bounds__(256) void sh_views_forward_kernel(int P, int V, const uint8_t* __restrict__ quads,
                                                                const ViewDesc* __restrict__ views, int M,
                                                                const float* __restrict__ centers,
                                                                const float* __restrict__ sh0,
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void sh_views_forward_kernel(int P, int V, con
     c[k][0] = cs[3 * (k - 1)]; c[k][1] = cs[3 * (k - 1) + 1]; c[k][2] = cs[3 * (k - 1) + 2];
   }
   for (int v0 = 0; v0 < V; v0 += 32) {
-    const uint32_t vis = visible_views(n_dup, P, i, v0, min(32, V - v0));
+    const uint32_t vis = visible_views(quads, P, i, v0, min(32, V - v0));
     for (uint32_t m = vis; m; m &= m - 1u) {
       const int v = v0 + __builtin_ctz(m);
       const long long o = (long long)v * P + i;
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256) void sh_views_forward_kernel(int P, int V, con
 // per-view colour gradients it consumes are reset to zero, so the backward compositor's
 // accumulation buffer is clear again for the next step without a separate fill.
 template <int kDeg>
-__global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, const uint32_t* __restrict__ n_dup,
+__global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, const uint8_t* __restrict__ quads,
                                                                 const ViewDesc* __restrict__ views, int M,
                                                                 const float* __restrict__ centers,
                                                                 float* __restrict__ drgb_views,
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, co
 #pragma unroll
     for (int k = 0; k < NB; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.f;
     for (int v0 = 0; v0 < V; v0 += 32) {
-      const uint32_t vis = visible_views(n_dup, P, i, v0, min(32, V - v0));
+      const uint32_t vis = visible_views(quads, P, i, v0, min(32, V - v0));
       for (uint32_t m = vis; m; m &= m - 1u) {
         const int v = v0 + __builtin_ctz(m);
         const long long o = (long long)v * P + i;
@@ -313,30 +313,30 @@ hipError_t launch_sh_backward(int n, const int64_t* idx, int deg, int M, const f
   return hipGetLastError();
 }
 
-hipError_t launch_sh_views_forward(int P, int V, const uint32_t* n_dup, const ViewDesc* views, int deg, int M,
+hipError_t launch_sh_views_forward(int P, int V, const uint8_t* quads, const ViewDesc* views, int deg, int M,
                                    const float* centers, const float* sh0, const float* shs, float* rgbs_views,
                                    hipStream_t st) {
   if (P <= 0 || V <= 0) return hipSuccess;
   const unsigned nb = blocks_for(P);
   switch (deg) {
-    case 0: sh_views_forward_kernel<0><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, sh0, shs, rgbs_views); break;
-    case 1: sh_views_forward_kernel<1><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, sh0, shs, rgbs_views); break;
-    case 2: sh_views_forward_kernel<2><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, sh0, shs, rgbs_views); break;
-    default: sh_views_forward_kernel<3><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, sh0, shs, rgbs_views); break;
+    case 0: sh_views_forward_kernel<0><<<nb, 256, 0, st>>>(P, V, quads, views, M, centers, sh0, shs, rgbs_views); break;
+    case 1: sh_views_forward_kernel<1><<<nb, 256, 0, st>>>(P, V, quads, views, M, centers, sh0, shs, rgbs_views); break;
+    case 2: sh_views_forward_kernel<2><<<nb, 256, 0, st>>>(P, V, quads, views, M, centers, sh0, shs, rgbs_views); break;
+    default: sh_views_forward_kernel<3><<<nb, 256, 0, st>>>(P, V, quads, views, M, centers, sh0, shs, rgbs_views); break;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_sh_views_backward(int P, int V, const uint32_t* n_dup, const ViewDesc* views, int deg, int M,
+hipError_t launch_sh_views_backward(int P, int V, const uint8_t* quads, const ViewDesc* views, int deg, int M,
                                     const float* centers, float* drgb_views, float* dsh0, float* dshs,
                                     hipStream_t st) {
   if (P <= 0) return hipSuccess;
   const unsigned nb = blocks_for(P);
   switch (deg) {
-    case 0: sh_views_backward_kernel<0><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, drgb_views, dsh0, dshs); break;
-    case 1: sh_views_backward_kernel<1><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, drgb_views, dsh0, dshs); break;
-    case 2: sh_views_backward_kernel<2><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, drgb_views, dsh0, dshs); break;
-    default: sh_views_backward_kernel<3><<<nb, 256, 0, st>>>(P, V, n_dup, views, M, centers, drgb_views, dsh0, dshs); break;
+    case 0: sh_views_backward_kernel<0><<<nb, 256, 0, st>>>(P, V, quads, views, M, centers, drgb_views, dsh0, dshs); break;
+    case 1: sh_views_backward_kernel<1><<<nb, 256, 0, st>>>(P, V, quads, views, M, centers, drgb_views, dsh0, dshs); break;
+    case 2: sh_views_backward_kernel<2><<<nb, 256, 0, st>>>(P, V, quads, views, M, centers, drgb_views, dsh0, dshs); break;
+    default: sh_views_backward_kernel<3><<<nb, 256, 0, st>>>(P, V, quads, views, M, centers, drgb_views, dsh0, dshs); break;
   }
   return hipGetLastError();
 }
