@@ -284,14 +284,18 @@ __device__ inline int select_set_bit(uint64_t m, int k) {
 //
 // The main alpha is (phi(v0) - phi(v3)) / (phi(v0) + 1e-6) with phi = sigmoid(s * v) and v0, v3
 // trilinear SDF values at the segment ends.  For v in [lo, hi] the fraction is at most
-// (phi(hi) - phi(lo)) / (phi(hi) + 1e-6) =: B.  [lo, hi] is the corner range widened for
-// (i) trilinear extrapolation when rounding puts a sample slightly outside the voxel (local
-// coordinate error eta <= 2e-6 (|pos| + l) / l, giving at most 8 eta times the range), plus a
-// fixed quarter of the range, and (ii) 1e-6 of the largest |corner| for the rounding of the
-// lerps.  The float evaluation of phi (s * v product, exp, rcp) has relative error below
-// 2.7e-6, which can add at most 5.4e-6 to the computed alpha; B is evaluated in double, so
-// B < 3e-6 guarantees a computed alpha below 8.5e-6 < kMinAlpha.  NaN corners or s keep the
-// voxel.
+// B = (sigmoid(a) - sigmoid(b)) / (sigmoid(a) + 1e-6), a = s * hi, b = s * lo.  [lo, hi] is the
+// corner range widened for (i) trilinear extrapolation when rounding puts a sample slightly
+// outside the voxel (local coordinate error eta <= 2e-6 (|pos| + l) / l, giving at most 8 eta
+// times the range), plus a fixed quarter of the range, and (ii) 1e-6 of the largest |corner| for
+// the rounding of the lerps.  The float evaluation of phi (s * v product, exp, rcp) has relative
+// error below 2.7e-6, which can add at most 5.4e-6 to the computed alpha, so B < 3e-6 guarantees
+// a computed alpha below 8.5e-6 < kMinAlpha.  B is evaluated without cancellation as
+//   sigmoid(a) - sigmoid(b) = (1 - e^-(a - b)) sigmoid(a) sigmoid(-b),  a - b = s (hi - lo),
+// with expm1 for the first factor, so its relative error stays near 1e-6.  NaN corners or s keep
+// the voxel.
+__device__ inline float sigmoid_acc(float x) { return 1.f / (1.f + expf(-x)); }
+
 __device__ inline bool voxel_may_contribute(const float g[8], float s, float pos_scale, float l) {
   float gmin = g[0], gmax = g[0], sum = g[0];
 #pragma unroll
@@ -301,13 +305,14 @@ __device__ inline bool voxel_may_contribute(const float g[8], float s, float pos
     sum += g[k];
   }
   if (__builtin_isnan(sum)) return true;
-  const double R = (double)gmax - (double)gmin;
-  const double eta = 2e-6 * ((double)pos_scale + (double)l) / (double)l;
-  const double sl = R * (0.25 + 8.0 * eta) + 1e-6 * fmax(fabs((double)gmin), fabs((double)gmax));
-  const double ds = (double)s;
-  const double hi = 1.0 / (1.0 + exp(-ds * ((double)gmax + sl)));
-  const double lo = 1.0 / (1.0 + exp(-ds * ((double)gmin - sl)));
-  return !((hi - lo) < 3e-6 * (hi + 1e-6));
+  const float R = gmax - gmin;
+  const float eta = 2e-6f * (pos_scale + l) / l;
+  const float sl = R * (0.25f + 8.f * eta) + 1e-6f * fmaxf(fabsf(gmin), fabsf(gmax));
+  const float a = s * (gmax + sl), b = s * (gmin - sl);
+  const float d = s * (R + 2.f * sl);
+  const float sa = sigmoid_acc(a);
+  const float B = (-expm1f(-d)) * sa * sigmoid_acc(-b) / (sa + 1e-6f);
+  return !(B < 3e-6f);
 }
 
 // Depth of one segment from its three sub-interval alphas (forward.cu:349-357).

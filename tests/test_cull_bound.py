@@ -7,14 +7,32 @@ import numpy as np
 
 
 def may_contribute(g, s, pos_scale, l):
-    """Restatement of voxel_may_contribute (double precision), vectorised over voxels."""
+    """Restatement of voxel_may_contribute in float32, vectorised over voxels."""
+    f = np.float32
+    g = g.astype(np.float32)
+    gmin, gmax = g.min(1), g.max(1)
+    R = gmax - gmin
+    eta = f(2e-6) * (f(pos_scale) + f(l)) / f(l)
+    sl = R * (f(0.25) + f(8) * eta) + f(1e-6) * np.maximum(np.abs(gmin), np.abs(gmax))
+    s = f(s)
+    a, b = s * (gmax + sl), s * (gmin - sl)
+    d = s * (R + f(2) * sl)
+    with np.errstate(over="ignore"):
+        sig = lambda x: f(1) / (f(1) + np.exp(-x))
+        sa = sig(a)
+        B = (-np.expm1(-d)) * sa * sig(-b) / (sa + f(1e-6))
+    return ~(B < f(3e-6))
+
+
+def bound_reference(g, s, pos_scale, l):
+    """The same bound in float64 with the direct difference of sigmoids."""
     gmin, gmax = g.min(1).astype(np.float64), g.max(1).astype(np.float64)
     R = gmax - gmin
     eta = 2e-6 * (pos_scale + l) / l
     sl = R * (0.25 + 8.0 * eta) + 1e-6 * np.maximum(np.abs(gmin), np.abs(gmax))
     hi = 1.0 / (1.0 + np.exp(-s * (gmax + sl)))
     lo = 1.0 / (1.0 + np.exp(-s * (gmin - sl)))
-    return ~((hi - lo) < 3e-6 * (hi + 1e-6))
+    return (hi - lo) / (hi + 1e-6)
 
 
 def trilerp32(g, q):
@@ -50,6 +68,10 @@ def test_skipped_voxels_never_reach_min_alpha():
     g = (off + (corners @ nrm.T).T * l + rng.normal(scale=0.05, size=(n, 8)) * l).astype(np.float32)
     keep = may_contribute(g, s, 3.0, float(l))
     assert 0.05 < keep.mean() < 0.95          # the test spans both sides of the bound
+    # The float32 factorised form agrees with the float64 difference of sigmoids away from the
+    # threshold (it only decides near 3e-6).
+    B = bound_reference(g, s, 3.0, float(l))
+    assert not np.any(~keep & (B > 3.3e-6)) and not np.any(keep & (B < 2.7e-6))
     skip = ~keep
     worst = 0.0
     for _ in range(8):
