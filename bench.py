@@ -140,6 +140,148 @@ def kernel_times(lib, fn):
     return out
 
 
+def train_iteration_bytes(it, smooth):
+    """Algorithmic HBM bytes of the non-render stages of one training iteration.
+    loss: colour, target and scale read, dL/dcolour written (4 x 12 B per ray).
+    smooth: every grid key once (key 4, active index 4, voxel id 4), the mask and dense index of
+    the res^3 grid once (1 + 4 B per cell), the voxel data of the table voxels once (corner ids
+    64, grid coordinate 12, size 4), corner SDFs read once and the gradient read + written
+    (12 B per corner).
+    adam: corner SDFs, their moments and gradient (p, m, v read + written, g read: 28 B each)."""
+    R = it.r.n_pix
+    M = it.grid_pts.numel()
+    out = {"loss": 48.0 * R, "adam": 28.0 * M}
+    if smooth is not None:
+        n_tab_vox = int(smooth.g2v.unique().numel())
+        out["smooth"] = 12.0 * smooth.G + 5.0 * smooth.res ** 3 + 80.0 * n_tab_vox + 12.0 * M
+    return out
+
+
+def bench_train(args):
+    """Full training iteration at scale (train_16): 16 views at 1920x1080 over the 16M-leaf scene;
+    render + MSE + backward + Laplacian smoothness (weight 1e-3) + sparse Adam (lr 1e-2, betas
+    0.9/0.99).  The plain per-view iteration runs the drop-in API the way the reference trainer
+    calls it (one view per call, randperm active list, SparseAdam over the three groups); the
+    batched iteration is one captured graph."""
+    import torch
+    import svraster_amd as svraster_cuda
+    from svraster_amd import _lib
+    from svraster_amd.batched import BatchedRasterizer, BatchedTrainIteration, GridSmoothness
+    from svraster_amd.model import VoxelModel
+    from svraster_amd.sparse_adam import SparseAdam
+    from svraster_amd.synthetic import make_workload
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    log(f"[bench] building scene {args.workload} ...")
+    w = make_workload(args.workload, device=dev)
+    sc, cams = w.scene, w.cameras
+    n_leaf = int(sc.is_leaf.sum().item())
+    log(f"[bench] scene {args.workload}: {sc.num_voxels} voxels ({n_leaf} leaves), "
+        f"{sc.geo_grid_pts.shape[0]} corners, {len(cams)} views, {w.rays} rays/step "
+        f"(built in {time.perf_counter() - t0:.1f} s)")
+    model = VoxelModel(sc)
+    targets = build_targets(cams, dev)
+    sync = torch.cuda.synchronize
+    lib = _lib.load()
+    rays = w.rays
+    weight, lr, betas, eps = 1e-3, 1e-2, (0.9, 0.99), 1e-15
+
+    t0 = time.perf_counter()
+    smooth = GridSmoothness(sc.vox_center, sc.vox_size, sc.octlevel, sc.is_leaf, sc.vox_key, sc.scene_center,
+                            sc.scene_extent, weight=weight, seed=0)
+    sync()
+    log(f"[bench] grid table: res {smooth.res}, {smooth.G} keys ({time.perf_counter() - t0:.1f} s)")
+
+    # -- plain per-view iteration (drop-in API) ---------------------------------------------------
+    opt = SparseAdam([{"params": [model._geo_grid_pts]}, {"params": [model._sh0]}, {"params": [model._shs]}],
+                     lr=lr, betas=betas, eps=eps, sparse=True)
+
+    def pv_render():
+        zero_grads(model)
+        per_view_step(model, cams, targets)
+
+    def pv_smooth():
+        active = torch.randperm(smooth.G, device=dev).to(torch.int32)
+        svraster_cuda.grid_loss_bw.laplacian_smoothness(
+            grid_pts=model._geo_grid_pts.detach(), vox_key=sc.vox_key, grid_voxel_coord=smooth.coord,
+            grid_voxel_size=smooth.size, grid_res=smooth.res, grid_mask=smooth.mask, grid_keys=smooth.keys,
+            grid2voxel=smooth.g2v, active_list=active, weight=weight, vox_size_inv=smooth.vox_size_inv,
+            no_tv_s=True, tv_sparse=False, grid_pts_grad=model._geo_grid_pts.grad)
+
+    def pv_adam():
+        opt.step()
+
+    pv_stages = (("render", pv_render), ("smooth", pv_smooth), ("adam", pv_adam))
+
+    def step_pv():
+        for _, fn in pv_stages:
+            fn()
+
+    def stage_times(stages):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
+        ev[0].record()
+        for i, (_, fn) in enumerate(stages):
+            fn()
+            ev[i + 1].record()
+        sync()
+        return {name: round(ev[i].elapsed_time(ev[i + 1]), 3) for i, (name, _) in enumerate(stages)}
+
+    ms_pv = time_steps(step_pv, args.steps, args.warmup, sync)
+    st_pv = stage_times(pv_stages)
+    log(f"[bench] per-view iteration: {ms_pv:.2f} ms; stages {st_pv}")
+    del opt
+    zero_grads(model)
+    torch.cuda.empty_cache()
+
+    # -- batched iteration, one captured graph ----------------------------------------------------
+    t0 = time.perf_counter()
+    rast = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key,
+                             cams, model.bg_color, sc.sh_degree)
+    it = BatchedTrainIteration(rast, model._geo_grid_pts.detach(), model._sh0.detach(), model._shs.detach(),
+                               model._log_s.detach(), targets, smooth, lr=lr, betas=betas, eps=eps)
+    it()
+    sync()
+    log(f"[bench] batched set-up {time.perf_counter() - t0:.1f} s, keys {rast.key_count()} / capacity {rast.cap}")
+    st_b = {k: round(v, 3) for k, v in it.stage_times().items()}
+    kt_b = kernel_times(lib, it)
+    it.capture()
+    ms_b = time_steps(it, args.steps, args.warmup, sync)
+    log(f"[bench] batched graph iteration: {ms_b:.2f} ms; stages (eager) {st_b}")
+
+    nbytes = train_iteration_bytes(it, smooth)
+    stage_gbs = {k: round(nbytes[k] / (st_b[k] / 1000.0) / 1e9, 1) for k in nbytes if st_b.get(k, 0) > 0}
+    bytes_launch, counts = backward_traffic(rast, rast.dvox)
+    bwd_ms, bwd_n = kt_b["render_bwd"]
+    avg_launch_s = bwd_ms / max(bwd_n, 1) / 1000.0
+    achieved = bytes_launch / avg_launch_s / 1e9
+    value = rays / (ms_b / 1000.0) / 1e6
+    result = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_b, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded 48-sphere + 8-box SDF scene, random SH, random targets)",
+        "config": {"workload": f"{args.workload}: {sc.num_voxels} voxels ({n_leaf} leaves, levels 7-13, parents "
+                               f"kept from level 9), SH deg 3, {len(cams)} views 1920x1080, {rays} rays/step; "
+                               f"training iteration: fwd + MSE + bwd + Laplacian smoothness (weight {weight}) "
+                               f"+ sparse Adam (lr {lr}, betas {betas})",
+                   "path": "batched: one captured HIP graph per iteration"},
+        "paths": {"batched_ms_per_step": round(ms_b, 3), "per_view_ms_per_step": round(ms_pv, 3),
+                  "speedup_batched_vs_per_view": round(ms_pv / ms_b, 3),
+                  "batched_mrays_s": round(value, 3), "per_view_mrays_s": round(rays / (ms_pv / 1000.0) / 1e6, 3),
+                  "keys_per_step": rast.key_count(), "grid_keys": smooth.G, "grid_res": smooth.res},
+        "stages_ms": {"batched_eager": st_b, "per_view": st_pv},
+        "stages_algorithmic_bytes": {k: int(v) for k, v in nbytes.items()},
+        "stages_gbs_batched": stage_gbs,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": BWD_KERNEL + " (multi-view work queue: all views in one launch)",
+                     "bytes_per_launch": int(bytes_launch), "counts": counts,
+                     "avg_launch_ms": round(avg_launch_s * 1000.0, 4)},
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +306,10 @@ def main():
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); the multi-GPU path is "
               f"design-only in this build.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "train_16":
+        bench_train(args)
+        return
 
     import torch
     from svraster_amd import _lib

@@ -118,6 +118,19 @@ int svr_unbiased_adam_step(int sparse, float* param, const float* grad, float* e
                            float* exp_avg_sq, long long numel, double step, double lr,
                            double beta1, double beta2, float eps, void* stream);
 
+/* Host-side table of the unbiased step's bias corrections for steps 1..n, out[2(t-1)] =
+ * (float)(lr / (1 - beta1^t)) and out[2(t-1)+1] = (float)(1 / sqrt(1 - beta2^t)), from the same
+ * double expressions as svr_unbiased_adam_step (src/adam_step.cu:117-122).  Copied to the device
+ * once, it lets a captured training step advance its step count on the device. */
+int svr_adam_bias_table(double lr, double beta1, double beta2, int n, float* out);
+
+/* svr_unbiased_adam_step with the step count read from device memory (*step, 1-based; steps
+ * beyond n_table use the last entry of the device copy of svr_adam_bias_table's table), so
+ * the call can sit inside a captured graph whose replays each advance the step. */
+int svr_sparse_adam_step_dev(int sparse, float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
+                             long long numel, const int32_t* step, const float* table, int n_table, float beta1,
+                             float beta2, float eps, void* stream);
+
 /* Replaces ADAM_STEP::biased_adam_step (src/adam_step.cu:137-158).  In place. */
 int svr_biased_adam_step(int sparse, float* param, const float* grad, float* exp_avg,
                          float* exp_avg_sq, long long numel, float lr, float beta1, float beta2,
@@ -141,6 +154,30 @@ int svr_laplacian_smoothness_bw(const float* grid_pts, long long n_pts, const in
                                 const int32_t* grid2voxel, int G, const int32_t* active_list,
                                 int A, float weight, float vox_size_inv, unsigned int seed,
                                 const float* randv, float* grid_pts_grad, void* stream);
+
+/* Cell -> grid_keys index for all res^3 cells (int32 out[res^3]): the entry the reference's
+ * binary search over the sorted keys returns (src/auxiliary.h:377-386), -1 for absent cells.  Built
+ * once per grid table; svr_grid_smoothness_bw_step then does one load per sample instead of a
+ * search. */
+int svr_grid_dense_index(const int32_t* grid_keys, int G, int grid_res, int32_t* out, void* stream);
+
+/* svr_grid_eikonal_bw (laplacian = 0) / svr_laplacian_smoothness_bw (laplacian = 1) for a
+ * captured training step: cells are looked up through dense_index (svr_grid_dense_index) and the
+ * random sample offsets are drawn from seed and the device step counter *step, so every replay
+ * samples new offsets as the reference's clock seed does (ls_compute.cu:43).
+ * active_list == NULL means every table entry, in table order (the reference trainer's list with
+ * ls_drop_ratio 0, train.py:640-642, up to order); the Laplacian then runs in two passes:
+ * per-key coefficients into coef_ws (float[3G]), then a walk over the table in voxel order
+ * (vm_keys / vm_vox: the cells and voxels of svr_valid_gradient_table_fill's output, before the
+ * key sort) that sums each voxel's contributions before adding them to grid_pts_grad.  That form
+ * needs every cell to appear at most once in grid_keys. */
+int svr_grid_smoothness_bw_step(int laplacian, const float* grid_pts, long long n_pts, const int64_t* vox_key, int N,
+                                const float* grid_voxel_coord, const float* grid_voxel_size, int grid_res,
+                                const uint8_t* grid_mask, const int32_t* grid_keys, const int32_t* grid2voxel, int G,
+                                const int32_t* dense_index, const int32_t* active_list, int A, float weight,
+                                float vox_size_inv, unsigned int seed, const int32_t* step,
+                                const int32_t* vm_keys, const int32_t* vm_vox, float* coef_ws,
+                                float* grid_pts_grad, void* stream);
 
 /* Replace UTILS::ijk_2_octpath / octpath_2_ijk (src/utils.cu:129-189, 329-361). */
 int svr_ijk_2_octpath(const int64_t* ijk, const int8_t* octlevel, int n, int64_t* octpath,
@@ -221,6 +258,17 @@ int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long lo
 int svr_batch_sh_backward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                           const float* vox_centers, float* dL_drgb_views, int degree, int M,
                           float* dL_dsh0, float* dL_dshs, void* stream);
+/* svr_batch_sh_backward fused with the sparse unbiased Adam step of SparseAdam.step
+ * (src/sparse_voxel_gears/optimizer.py:24-27, cuda/svraster_cuda/sparse_adam.py:33-72,
+ * src/adam_step.cu:21-135) on sh0 [P,3] and shs [P,M-1,3]: the summed gradient of each element
+ * updates (param, m, v) in place unless it is exactly 0, and is never written to memory.  The
+ * bias corrections come from svr_adam_bias_table tables (one per parameter group, as the
+ * groups' learning rates may differ) indexed by the device step counter *step. */
+int svr_batch_sh_backward_adam(void* ws, int n_views, int P, long long n_pixels, long long n_tiles,
+                               long long K_cap, const float* vox_centers, float* dL_drgb_views, int degree, int M,
+                               float* sh0, float* sh0_m, float* sh0_v, const float* table_sh0, float* shs,
+                               float* shs_m, float* shs_v, const float* table_shs, int n_table,
+                               const int32_t* step, float beta1, float beta2, float eps, void* stream);
 /* Byte offsets inside the batch workspace of (for inspection, like unpack_ImageState /
  * unpack_BinningState, raster_state.cu:120-230): out[0] tile ranges uint2[n_tiles], out[1]
  * sorted values uint32[K], out[2] per-pixel contributor counts uint32[n_pixels], out[3]

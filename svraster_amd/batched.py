@@ -173,8 +173,11 @@ class BatchedRasterizer:
         return self.color
 
     def backward(self, dL_dcolor: torch.Tensor, n_grid_pts: int, dL_dT: Optional[torch.Tensor] = None,
-                 dL_ddepth: Optional[torch.Tensor] = None, dL_dnormal: Optional[torch.Tensor] = None):
-        """Gradients of all views at once: (dL/dgrid_pts [M,1], dL/dsh0, dL/dshs, dL/ds, subdiv [P])."""
+                 dL_ddepth: Optional[torch.Tensor] = None, dL_dnormal: Optional[torch.Tensor] = None,
+                 sh_adam: Optional["ShAdamState"] = None):
+        """Gradients of all views at once: (dL/dgrid_pts [M,1], dL/dsh0, dL/dshs, dL/ds, subdiv [P]).
+        With `sh_adam`, the SH gradients are not returned (None): the SH transpose applies the
+        sparse Adam step to sh0/shs in place instead (svr_batch_sh_backward_adam)."""
         st = self._stream()
         if self.dgrid is None or self.dgrid.shape[0] != n_grid_pts:
             self.dgrid = torch.zeros(n_grid_pts, 1, dtype=torch.float32, device=self.dev)
@@ -188,13 +191,23 @@ class BatchedRasterizer:
                                                _p(dL_dcolor), _p(dL_ddepth), _p(dL_dnormal), _p(dT),
                                                int(self.need_depth), int(self.need_normal), _p(self.dvox),
                                                _p(self.drgb_views), _p(self.ds), st), "svr_batch_backward")
-        _lib.check(self.lib.svr_batch_sh_backward(_p(self.ws), *self._dims(), _p(self.vox_center),
-                                                  _p(self.drgb_views), self.deg, self.M, _p(self.dsh0),
-                                                  _p(self.dshs), st), "sh_backward")
+        if sh_adam is None:
+            _lib.check(self.lib.svr_batch_sh_backward(_p(self.ws), *self._dims(), _p(self.vox_center),
+                                                      _p(self.drgb_views), self.deg, self.M, _p(self.dsh0),
+                                                      _p(self.dshs), st), "sh_backward")
+            dsh0, dshs = self.dsh0, self.dshs
+        else:
+            a = sh_adam
+            _lib.check(self.lib.svr_batch_sh_backward_adam(
+                _p(self.ws), *self._dims(), _p(self.vox_center), _p(self.drgb_views), self.deg, self.M,
+                _p(a.sh0), _p(a.m0), _p(a.v0), _p(a.table0), _p(a.shs), _p(a.ms), _p(a.vs), _p(a.tables),
+                a.n_table, _p(a.step), ctypes.c_float(a.b1), ctypes.c_float(a.b2), ctypes.c_float(a.eps), st),
+                "sh_backward_adam")
+            dsh0 = dshs = None
         dgeo = self.dvox[:, :8].contiguous()
         _lib.check(self.lib.svr_gather_triinterp_geo_params_bw(_p(self.vox_key), self.P, None, self.P, _p(dgeo),
                                                                _p(self.dgrid), st), "gather_bw")
-        return self.dgrid, self.dsh0, self.dshs, self.ds, self.dvox[:, 11]
+        return self.dgrid, dsh0, dshs, self.ds, self.dvox[:, 11]
 
 
 class BatchedTrainStep:
@@ -225,6 +238,194 @@ class BatchedTrainStep:
             for _ in range(warmup):
                 self._run()
         torch.cuda.current_stream(self.r.dev).wait_stream(s)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._run()
+
+    def __call__(self):
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self._run()
+
+
+# ----------------------------------------------------------------------------------------------
+# Full training iteration: render + MSE + backward + grid Laplacian smoothness + sparse Adam
+# ----------------------------------------------------------------------------------------------
+def adam_bias_table(lr: float, betas, n: int, device) -> torch.Tensor:
+    """Device copy of svr_adam_bias_table: float32 [n, 2] = {lr / (1 - b1^t), 1 / sqrt(1 - b2^t)}
+    for t = 1..n, rounded from the same double expressions as the host-driven unbiased step."""
+    lib = _lib.load()
+    host = (ctypes.c_float * (2 * n))()
+    _lib.check(lib.svr_adam_bias_table(ctypes.c_double(lr), ctypes.c_double(betas[0]), ctypes.c_double(betas[1]),
+                                       n, host), "svr_adam_bias_table")
+    return torch.tensor(list(host), dtype=torch.float32).view(n, 2).to(device)
+
+
+def adam_table_length(betas) -> int:
+    """Steps after which both bias corrections are exactly 1 in double (b^t below 2^-54), so the
+    last table entry is exact for every later step; capped at 65536."""
+    import math
+    n = 1
+    for b in betas:
+        if 0.0 < b < 1.0:
+            n = max(n, int(math.ceil(54.0 * math.log(2.0) / -math.log(b))) + 1)
+    return min(n, 1 << 16)
+
+
+class ShAdamState:
+    """Adam moments and bias tables of sh0 / shs for the fused SH backward + Adam launch."""
+
+    def __init__(self, sh0, shs, step: torch.Tensor, lr_sh0: float, lr_shs: float, betas, eps: float):
+        self.sh0, self.shs, self.step = sh0, shs, step
+        self.m0, self.v0 = torch.zeros_like(sh0), torch.zeros_like(sh0)
+        self.ms, self.vs = torch.zeros_like(shs), torch.zeros_like(shs)
+        self.n_table = adam_table_length(betas)
+        self.table0 = adam_bias_table(lr_sh0, betas, self.n_table, sh0.device)
+        self.tables = adam_bias_table(lr_shs, betas, self.n_table, sh0.device)
+        self.b1, self.b2, self.eps = float(betas[0]), float(betas[1]), float(eps)
+
+
+class GridSmoothness:
+    """Laplacian smoothness backward on the valid-gradient grid table, as the reference trainer
+    calls it after topology changes (train.py:959-972, 636-654): grid resolution
+    2^min(max level, 9), voxel coordinates and sizes in grid cells rounded to integers, keys
+    sorted (octree_utils.py:293-315), weight * G / A, every key active.
+
+    The reference draws the active list as randperm(G)[:A] with A = G (ls_drop_ratio 0); the
+    identity order used here samples the same keys (only the order of the float sums differs).
+    Cells are looked up through a dense cell -> key index map built once (svr_grid_dense_index,
+    4 B per grid cell) instead of a binary search per sample, and the per-key random offsets come
+    from a hash of (seed, device step counter), so every replay of a captured step draws new
+    offsets.  When every cell appears once in the table (always, for tables built from a valid
+    octree), the loss runs in the two-pass form that sums each voxel's contributions before
+    touching the gradient (`pull`); otherwise, or with pull=False, one thread per key scatters
+    its 56 weighted terms with atomics as the reference kernel does."""
+
+    def __init__(self, vox_center, vox_size, octlevel, is_leaf, vox_key, scene_center, inside_extent: float,
+                 weight: float, seed: int = 0, pull: bool = True):
+        from . import _C
+        self.lib = _lib.load()
+        self.dev = vox_center.device
+        self.max_level = min(int(octlevel.max().item()), 9)
+        self.res = 1 << self.max_level
+        mask, keys, g2v = _C.valid_gradient_table(vox_center, vox_size, scene_center, inside_extent,
+                                                  self.max_level, is_leaf.reshape(-1).bool())
+        self.keys, order = keys.sort(stable=True)
+        self.g2v = g2v[order].contiguous()
+        del order
+        self.vm_keys, self.vm_vox = keys, g2v          # table order: voxel by voxel
+        self.mask = mask
+        self.G = int(self.keys.numel())
+        sc = scene_center.to(self.dev).reshape(1, 3).float()
+        self.coord = torch.round(((vox_center - vox_size.reshape(-1, 1) * 0.5) - (sc - inside_extent * 0.5))
+                                 / inside_extent * self.res).float().contiguous()
+        self.size = torch.round(vox_size.reshape(-1) / inside_extent * self.res).float().contiguous()
+        self.vox_key = vox_key.contiguous()
+        self.N = int(vox_center.shape[0])
+        self.dense = torch.empty(self.res ** 3, dtype=torch.int32, device=self.dev)
+        _lib.check(self.lib.svr_grid_dense_index(_p(self.keys), self.G, self.res, _p(self.dense), self._stream()),
+                   "svr_grid_dense_index")
+        self.active = torch.arange(self.G, dtype=torch.int32, device=self.dev)
+        unique = int((self.dense >= 0).sum().item()) == self.G
+        self.pull = bool(pull) and unique
+        self.coef = torch.empty(3 * self.G if self.pull else 0, dtype=torch.float32, device=self.dev)
+        self.weight = float(weight)          # * G / A with A = G
+        self.vox_size_inv = float(self.res / inside_extent)
+        self.seed = int(seed) & 0xFFFFFFFF
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+
+    def __call__(self, grid_pts: torch.Tensor, grid_grad: torch.Tensor, step: torch.Tensor):
+        pull = self.pull
+        _lib.check(self.lib.svr_grid_smoothness_bw_step(
+            1, _p(grid_pts), grid_pts.numel(), _p(self.vox_key), self.N, _p(self.coord), _p(self.size), self.res,
+            _p(self.mask), _p(self.keys), _p(self.g2v), self.G, _p(self.dense), None if pull else _p(self.active),
+            self.G, ctypes.c_float(self.weight), ctypes.c_float(self.vox_size_inv), ctypes.c_uint(self.seed),
+            _p(step), _p(self.vm_keys) if pull else None, _p(self.vm_vox) if pull else None,
+            _p(self.coef) if pull else None, _p(grid_grad), self._stream()), "svr_grid_smoothness_bw_step")
+
+
+class BatchedTrainIteration:
+    """One training iteration over a batch of views in the reference trainer's order
+    (train.py:324-740): render every view -> MSE to the targets -> backward -> grid Laplacian
+    smoothness into the corner-SDF gradient -> sparse unbiased Adam on the corner SDFs and SH
+    (elements with an exactly-zero gradient untouched).  The step counter lives on the device, so
+    `capture()` records the whole iteration once and each replay is the next iteration.
+
+    log_s is not updated (its learning rate is 0 in the reference's optimizer groups,
+    optimizer.py:16-22); the SH Adam step runs inside the SH transpose launch."""
+
+    def __init__(self, rast: BatchedRasterizer, grid_pts, sh0, shs, log_s, targets: List[torch.Tensor],
+                 smooth: Optional[GridSmoothness], lr: float = 1e-2, betas=(0.9, 0.99), eps: float = 1e-15,
+                 lr_sh0: Optional[float] = None, lr_shs: Optional[float] = None):
+        self.r = rast
+        self.dev = rast.dev
+        self.grid_pts, self.sh0, self.shs, self.log_s = grid_pts, sh0, shs, log_s
+        self.target = torch.cat([t.reshape(-1) for t in targets]).contiguous()
+        self.scale = torch.cat([torch.full((3 * p,), 2.0 / (3 * p), device=self.dev) for p in rast.pix])
+        self.s_val = torch.empty(1, dtype=torch.float32, device=self.dev)
+        self.dC = torch.empty_like(self.target)
+        self.step = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        self.smooth = smooth
+        self.betas, self.eps = (float(betas[0]), float(betas[1])), float(eps)
+        self.sh_adam = ShAdamState(sh0, shs, self.step, lr if lr_sh0 is None else lr_sh0,
+                                   lr if lr_shs is None else lr_shs, self.betas, self.eps)
+        self.grid_m, self.grid_v = torch.zeros_like(grid_pts), torch.zeros_like(grid_pts)
+        self.n_table = self.sh_adam.n_table
+        self.grid_table = adam_bias_table(lr, self.betas, self.n_table, self.dev)
+        self.graph = None
+        self.lib = _lib.load()
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+
+    def stage_render(self):
+        self.step.add_(1)
+        torch.exp(self.log_s.reshape(1) * 10.0, out=self.s_val)
+        self.color = self.r.forward(self.grid_pts, self.sh0, self.shs, self.s_val)
+
+    def stage_loss(self):
+        torch.mul(self.color - self.target, self.scale, out=self.dC)
+
+    def stage_backward(self):
+        self.grid_grad, _, _, self.s_grad, _ = self.r.backward(self.dC, self.grid_pts.shape[0],
+                                                                sh_adam=self.sh_adam)
+
+    def stage_smooth(self):
+        if self.smooth is not None:
+            self.smooth(self.grid_pts, self.grid_grad, self.step)
+
+    def stage_adam(self):
+        _lib.check(self.lib.svr_sparse_adam_step_dev(
+            1, _p(self.grid_pts), _p(self.grid_grad), _p(self.grid_m), _p(self.grid_v), self.grid_pts.numel(),
+            _p(self.step), _p(self.grid_table), self.n_table, ctypes.c_float(self.betas[0]),
+            ctypes.c_float(self.betas[1]), ctypes.c_float(self.eps), self._stream()), "svr_sparse_adam_step_dev")
+
+    STAGES = ("render", "loss", "backward", "smooth", "adam")
+
+    def _run(self):
+        for name in self.STAGES:
+            getattr(self, "stage_" + name)()
+
+    def stage_times(self):
+        """Milliseconds per stage of one eager iteration, from HIP events on the current stream."""
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(self.STAGES) + 1)]
+        ev[0].record()
+        for i, name in enumerate(self.STAGES):
+            getattr(self, "stage_" + name)()
+            ev[i + 1].record()
+        torch.cuda.synchronize(self.dev)
+        return {name: ev[i].elapsed_time(ev[i + 1]) for i, name in enumerate(self.STAGES)}
+
+    def capture(self, warmup: int = 1):
+        s = torch.cuda.Stream(device=self.dev)
+        s.wait_stream(torch.cuda.current_stream(self.dev))
+        with torch.cuda.stream(s):
+            for _ in range(warmup):
+                self._run()
+        torch.cuda.current_stream(self.dev).wait_stream(s)
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             self._run()

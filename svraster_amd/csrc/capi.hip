@@ -385,6 +385,35 @@ int svr_unbiased_adam_step(int sparse, float* param, const float* grad, float* e
   return 0;
 }
 
+int svr_adam_bias_table(double lr, double beta1, double beta2, int n, float* out) {
+  g_err.clear();
+  if (n <= 0 || !out) return fail(1, "table length%s %lld must be positive", "", n);
+  for (int t = 1; t <= n; ++t) {
+    // Same double expressions as svr_unbiased_adam_step (adam_step.cu:117-122).
+    const double bc1 = 1.0 - pow(beta1, (double)t);
+    const double bc2 = 1.0 - pow(beta2, (double)t);
+    out[2 * (t - 1)] = (float)(lr / bc1);
+    out[2 * (t - 1) + 1] = (float)(1.0 / sqrt(bc2));
+  }
+  return 0;
+}
+
+static AdamTable adam_table(const int32_t* step, const float* table, int n, float b1, float b2, float eps) {
+  AdamTable T;
+  T.step = step; T.tab = reinterpret_cast<const float2*>(table); T.n = n; T.b1 = b1; T.b2 = b2; T.eps = eps;
+  return T;
+}
+
+int svr_sparse_adam_step_dev(int sparse, float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
+                             long long numel, const int32_t* step, const float* table, int n_table, float beta1,
+                             float beta2, float eps, void* stream) {
+  g_err.clear();
+  if (!step || !table || n_table <= 0) return fail(1, "step counter and bias table%s are required", "", 0);
+  SVR_TRY(launch_adam_table(sparse != 0, numel, param, grad, exp_avg, exp_avg_sq,
+                            adam_table(step, table, n_table, beta1, beta2, eps), as_stream(stream)));
+  return 0;
+}
+
 int svr_biased_adam_step(int sparse, float* param, const float* grad, float* exp_avg,
                          float* exp_avg_sq, long long numel, float lr, float beta1, float beta2,
                          float eps, void* stream) {
@@ -398,14 +427,16 @@ static int grid_loss_common(bool lap, const float* grid_pts, long long n_pts, co
                             const float* coord, const float* size, int res, const uint8_t* mask,
                             const int32_t* keys, const int32_t* key2vox, int G, const int32_t* active, int A,
                             float weight, float inv, unsigned seed, const float* randv, float* grad,
-                            void* stream) {
+                            void* stream, const int32_t* dense = nullptr, const int32_t* step = nullptr) {
   g_err.clear();
   if (res <= 2) return fail(1, "grid_res must exceed 2%s (got %lld)", "", res);
+  if ((long long)res * res * res >= (1LL << 31)) return fail(1, "grid_res%s %lld too large (res^3 must fit int32)", "", res);
   GridTable T;
   T.grid_pts = grid_pts; T.vox_key = vox_key; T.keys = keys; T.key2vox = key2vox;
   T.vox_coord = coord; T.vox_size = size; T.n_keys = G; T.res = res; T.n_vox = N; T.n_pts = n_pts;
-  SVR_TRY(launch_grid_loss(lap, T, reinterpret_cast<const bool*>(mask), active, A, weight, inv, seed, randv, grad,
-                           as_stream(stream)));
+  T.dense = dense;
+  SVR_TRY(launch_grid_loss(lap, T, reinterpret_cast<const bool*>(mask), active, A, weight, inv, seed, step, randv,
+                           grad, as_stream(stream)));
   return 0;
 }
 
@@ -430,6 +461,42 @@ int svr_laplacian_smoothness_bw(const float* grid_pts, long long n_pts, const in
   return grid_loss_common(true, grid_pts, n_pts, vox_key, N, grid_voxel_coord, grid_voxel_size, grid_res,
                           grid_mask, grid_keys, grid2voxel, G, active_list, A, weight, vox_size_inv, seed, randv,
                           grid_pts_grad, stream);
+}
+
+int svr_grid_dense_index(const int32_t* grid_keys, int G, int grid_res, int32_t* out, void* stream) {
+  g_err.clear();
+  if (grid_res <= 2 || (long long)grid_res * grid_res * grid_res >= (1LL << 31))
+    return fail(1, "grid_res%s %lld out of range", "", grid_res);
+  SVR_TRY(launch_grid_dense_index((long long)grid_res * grid_res * grid_res, grid_keys, G, out, as_stream(stream)));
+  return 0;
+}
+
+int svr_grid_smoothness_bw_step(int laplacian, const float* grid_pts, long long n_pts, const int64_t* vox_key, int N,
+                                const float* grid_voxel_coord, const float* grid_voxel_size, int grid_res,
+                                const uint8_t* grid_mask, const int32_t* grid_keys, const int32_t* grid2voxel, int G,
+                                const int32_t* dense_index, const int32_t* active_list, int A, float weight,
+                                float vox_size_inv, unsigned int seed, const int32_t* step,
+                                const int32_t* vm_keys, const int32_t* vm_vox, float* coef_ws,
+                                float* grid_pts_grad, void* stream) {
+  g_err.clear();
+  if (!step) return fail(1, "step counter%s is required", "", 0);
+  if (!active_list) {
+    // Every entry active, in table order: the two-pass form (Laplacian only).
+    if (!laplacian || !dense_index || !vm_keys || !vm_vox || !coef_ws)
+      return fail(1, "the all-keys form%s needs laplacian, dense_index, vm_keys, vm_vox and coef_ws", "", 0);
+    if (grid_res <= 2 || (long long)grid_res * grid_res * grid_res >= (1LL << 31))
+      return fail(1, "grid_res%s %lld out of range", "", grid_res);
+    GridTable T;
+    T.grid_pts = grid_pts; T.vox_key = vox_key; T.keys = grid_keys; T.key2vox = grid2voxel;
+    T.vox_coord = grid_voxel_coord; T.vox_size = grid_voxel_size; T.n_keys = G; T.res = grid_res; T.n_vox = N;
+    T.n_pts = n_pts; T.dense = dense_index;
+    SVR_TRY(launch_grid_laplacian_pull(T, reinterpret_cast<const bool*>(grid_mask), vm_keys, vm_vox, weight,
+                                       vox_size_inv, seed, step, coef_ws, grid_pts_grad, as_stream(stream)));
+    return 0;
+  }
+  return grid_loss_common(laplacian != 0, grid_pts, n_pts, vox_key, N, grid_voxel_coord, grid_voxel_size, grid_res,
+                          grid_mask, grid_keys, grid2voxel, G, active_list, A, weight, vox_size_inv, seed, nullptr,
+                          grid_pts_grad, stream, dense_index, step);
 }
 
 int svr_ijk_2_octpath(const int64_t* ijk, const int8_t* octlevel, int n, int64_t* octpath, void* stream) {
@@ -651,7 +718,28 @@ int svr_batch_sh_backward(void* ws, int n_views, int P, long long n_pixels, long
   if (M > 16) return fail(1, "batched SH supports at most 16 coefficients per channel%s (got %lld)", "", M);
   BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
   SVR_TRY(launch_sh_views_backward(P, n_views, bp.g.quad, bp.g.views, degree, M, vox_centers, dL_drgb_views,
-                                   dL_dsh0, dL_dshs, as_stream(stream)));
+                                   dL_dsh0, dL_dshs, nullptr, as_stream(stream)));
+  return 0;
+}
+
+int svr_batch_sh_backward_adam(void* ws, int n_views, int P, long long n_pixels, long long n_tiles,
+                               long long K_cap, const float* vox_centers, float* dL_drgb_views, int degree, int M,
+                               float* sh0, float* sh0_m, float* sh0_v, const float* table_sh0, float* shs,
+                               float* shs_m, float* shs_v, const float* table_shs, int n_table,
+                               const int32_t* step, float beta1, float beta2, float eps, void* stream) {
+  g_err.clear();
+  if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
+  if (degree < 0 || degree > 3 || M < (degree + 1) * (degree + 1)) return fail(1, "bad SH degree%s %lld", "", degree);
+  if (M > 16) return fail(1, "batched SH supports at most 16 coefficients per channel%s (got %lld)", "", M);
+  if (!step || !table_sh0 || !table_shs || n_table <= 0) return fail(1, "step counter and bias tables%s are required", "", 0);
+  BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
+  ShAdam A;
+  A.sh0 = sh0; A.m0 = sh0_m; A.v0 = sh0_v;
+  A.shs = shs; A.ms = shs_m; A.vs = shs_v;
+  A.t0 = adam_table(step, table_sh0, n_table, beta1, beta2, eps);
+  A.ts = adam_table(step, table_shs, n_table, beta1, beta2, eps);
+  SVR_TRY(launch_sh_views_backward(P, n_views, bp.g.quad, bp.g.views, degree, M, vox_centers, dL_drgb_views,
+                                   nullptr, nullptr, &A, as_stream(stream)));
   return 0;
 }
 

@@ -61,7 +61,7 @@ __device__ inline int key_search(const int32_t* __restrict__ keys, int size, int
 __device__ inline float grid_sample(const GridTable& G, int gx, int gy, int gz, float rx, float ry, float rz,
                                     float w[8], int& vox) {
   const int key = gx + gy * G.res + gz * G.res * G.res;
-  const int idx = key_search(G.keys, G.n_keys, key);
+  const int idx = G.dense ? G.dense[key] : key_search(G.keys, G.n_keys, key);
   vox = -1;
   if (idx < 0) return 0.f;
   const int v = G.key2vox[idx];
@@ -90,6 +90,7 @@ __device__ inline void grid_scatter(const GridTable& G, int vox, const float w[8
 __global__ __launch_bounds__(256) void grid_eikonal_kernel(GridTable G, const bool* __restrict__ mask,
                                                            const int32_t* __restrict__ active, int A,
                                                            float weight, float inv, uint32_t seed,
+                                                           const int32_t* __restrict__ step,
                                                            const float* __restrict__ randv,
                                                            float* __restrict__ grad) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(256) void grid_eikonal_kernel(GridTable G, const bo
   const int x = gk % G.res, y = (gk / G.res) % G.res, z = gk / (G.res * G.res);
   if (!mask[gk] || !neighbours_valid(x, y, z, G.res, mask)) return;
   float rx, ry, rz;
-  random_offset(randv, seed, tid, rx, ry, rz);
+  random_offset(randv, step ? seed + 0x9E3779B9u * (uint32_t)*step : seed, tid, rx, ry, rz);
   const int dx[6] = {1, -1, 0, 0, 0, 0}, dy[6] = {0, 0, 1, -1, 0, 0}, dz[6] = {0, 0, 0, 0, 1, -1};
   float w[6][8], sd[6];
   int vox[6];
@@ -119,21 +120,20 @@ __global__ __launch_bounds__(256) void grid_eikonal_kernel(GridTable G, const bo
   grid_scatter(G, vox[5], w[5], -lz, grad);
 }
 
-__global__ __launch_bounds__(256) void grid_laplacian_kernel(GridTable G, const bool* __restrict__ mask,
-                                                             const int32_t* __restrict__ active, int A,
-                                                             float weight, float inv, uint32_t seed,
-                                                             const float* __restrict__ randv,
-                                                             float* __restrict__ grad) {
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= A) return;
-  const int gk = G.keys[active[tid]];
+// One grid key of the Laplacian loss (ls_compute.cu:14-109): samples at the cell and its six
+// face neighbours, all shifted by the key's random offset; returns false when the key or a
+// neighbour is outside the valid mask.  lx, ly, lz are the per-axis loss coefficients: sample
+// +-x receives lx, +-y ly, +-z lz and the centre -2 (lx + ly + lz), each times its trilinear
+// weights.
+__device__ inline bool laplacian_key(const GridTable& G, const bool* __restrict__ mask, int gk, int tid,
+                                     uint32_t seed, const float* __restrict__ randv, float weight, float inv,
+                                     float (&w)[7][8], int (&vox)[7], float& lx, float& ly, float& lz) {
   const int x = gk % G.res, y = (gk / G.res) % G.res, z = gk / (G.res * G.res);
-  if (!mask[gk] || !neighbours_valid(x, y, z, G.res, mask)) return;
+  if (!mask[gk] || !neighbours_valid(x, y, z, G.res, mask)) return false;
   float rx, ry, rz;
   random_offset(randv, seed, tid, rx, ry, rz);
   const int dx[7] = {0, 1, -1, 0, 0, 0, 0}, dy[7] = {0, 0, 0, 1, -1, 0, 0}, dz[7] = {0, 0, 0, 0, 0, 1, -1};
-  float w[7][8], sd[7];
-  int vox[7];
+  float sd[7];
 #pragma unroll
   for (int i = 0; i < 7; ++i) sd[i] = grid_sample(G, x + dx[i], y + dy[i], z + dz[i], rx, ry, rz, w[i], vox[i]);
   const float r0 = sd[1] + sd[2] - 2.f * sd[0];
@@ -141,7 +141,29 @@ __global__ __launch_bounds__(256) void grid_laplacian_kernel(GridTable G, const 
   const float r2 = sd[5] + sd[6] - 2.f * sd[0];
   const float lr = sqrtf(r0 * r0 + r1 * r1 + r2 * r2 + 1e-10f);
   const float sz = weight * 0.5f * inv * inv;
-  const float lx = r0 / lr * sz, ly = r1 / lr * sz, lz = r2 / lr * sz;
+  lx = r0 / lr * sz;
+  ly = r1 / lr * sz;
+  lz = r2 / lr * sz;
+  return true;
+}
+
+__device__ inline uint32_t step_seed(uint32_t seed, const int32_t* __restrict__ step) {
+  return step ? seed + 0x9E3779B9u * (uint32_t)*step : seed;
+}
+
+__global__ __launch_bounds__(256) void grid_laplacian_kernel(GridTable G, const bool* __restrict__ mask,
+                                                             const int32_t* __restrict__ active, int A,
+                                                             float weight, float inv, uint32_t seed,
+                                                             const int32_t* __restrict__ step,
+                                                             const float* __restrict__ randv,
+                                                             float* __restrict__ grad) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= A) return;
+  float w[7][8], lx, ly, lz;
+  int vox[7];
+  if (!laplacian_key(G, mask, G.keys[active[tid]], tid, step_seed(seed, step), randv, weight, inv, w, vox, lx, ly,
+                     lz))
+    return;
   const float lc = -2.f * (lx + ly + lz);
   grid_scatter(G, vox[0], w[0], lc, grad);
   grid_scatter(G, vox[1], w[1], lx, grad);
@@ -150,6 +172,93 @@ __global__ __launch_bounds__(256) void grid_laplacian_kernel(GridTable G, const 
   grid_scatter(G, vox[4], w[4], ly, grad);
   grid_scatter(G, vox[5], w[5], lz, grad);
   grid_scatter(G, vox[6], w[6], lz, grad);
+}
+
+// Two-pass ("pull") form of the same loss for an active list that is every table entry in
+// order, over a table whose cells are unique.  Pass 1 stores each key's coefficients (zero for
+// keys that fail the mask test).  Pass 2 walks the table voxel by voxel: the samples that land
+// in cell c come from key c itself (centre sample) and from its six face neighbours (their
+// sample toward c), all inside c's voxel, so a lane accumulates its cell's seven contributions
+// into that voxel's 8 corners; lanes of the same voxel are summed across the wavefront and one
+// lane per voxel run adds the 8 sums to the gradient.  Same terms as the per-key scatter (the
+// same products of coefficient and weight), summed in another order.
+__global__ __launch_bounds__(256) void grid_laplacian_coef_kernel(GridTable G, const bool* __restrict__ mask,
+                                                                  float weight, float inv, uint32_t seed,
+                                                                  const int32_t* __restrict__ step,
+                                                                  float* __restrict__ coef) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= G.n_keys) return;
+  float w[7][8], lx = 0.f, ly = 0.f, lz = 0.f;
+  int vox[7];
+  if (!laplacian_key(G, mask, G.keys[tid], tid, step_seed(seed, step), nullptr, weight, inv, w, vox, lx, ly, lz)) {
+    lx = ly = lz = 0.f;
+  }
+  coef[tid] = lx;
+  coef[(long long)G.n_keys + tid] = ly;
+  coef[2LL * G.n_keys + tid] = lz;
+}
+
+__global__ __launch_bounds__(256) void grid_laplacian_pull_kernel(GridTable G, const int32_t* __restrict__ vm_keys,
+                                                                  const int32_t* __restrict__ vm_vox,
+                                                                  uint32_t seed, const int32_t* __restrict__ step,
+                                                                  const float* __restrict__ coef,
+                                                                  float* __restrict__ grad) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int n = G.n_keys, res = G.res;
+  const long long n3 = (long long)n;
+  int v = -1;
+  float acc[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) acc[i] = 0.f;
+  if (e < n) {
+    const int c = vm_keys[e];
+    const int vv = vm_vox[e];
+    const int j = G.dense[c];
+    if (j >= 0 && vv >= 0 && vv < G.n_vox && G.key2vox[j] == vv) v = vv;
+    if (v >= 0) {
+      const uint32_t sd = step_seed(seed, step);
+      const int x = c % res, y = (c / res) % res, z = c / (res * res);
+      const float cx = G.vox_coord[3 * v], cy = G.vox_coord[3 * v + 1], cz = G.vox_coord[3 * v + 2];
+      const float size = G.vox_size[v];
+      // key offsets of the seven contributors: the cell, then the keys whose +x, -x, +y, -y, +z, -z
+      // sample lands in it (at -x, +x, -y, +y, -z, +z of the cell)
+      const int kx[7] = {0, -1, 1, 0, 0, 0, 0}, ky[7] = {0, 0, 0, -1, 1, 0, 0}, kz[7] = {0, 0, 0, 0, 0, -1, 1};
+#pragma unroll
+      for (int i = 0; i < 7; ++i) {
+        const int ax = x + kx[i], ay = y + ky[i], az = z + kz[i];
+        if (ax < 0 || ax >= res || ay < 0 || ay >= res || az < 0 || az >= res) continue;
+        const int jk = G.dense[ax + ay * res + az * res * res];
+        if (jk < 0) continue;
+        float l;
+        if (i == 0) l = -2.f * (coef[jk] + coef[n3 + jk] + coef[2 * n3 + jk]);
+        else if (i <= 2) l = coef[jk];
+        else if (i <= 4) l = coef[n3 + jk];
+        else l = coef[2 * n3 + jk];
+        if (l == 0.f) continue;
+        float rx, ry, rz;
+        random_offset(nullptr, sd, jk, rx, ry, rz);
+        float w[8];
+        trilinear_weights(((float)x + rx - cx) / size, ((float)y + ry - cy) / size, ((float)z + rz - cz) / size, w);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const float wq = fminf(fmaxf(w[q], 0.f), 1.f);
+          acc[q] += l * wq;
+        }
+      }
+    }
+  }
+  const int prev = __shfl_up(v, 1);
+  const int next = __shfl_down(v, 1);
+  const bool head = lane == 0 || prev != v;
+  const int seg_start = (int)wave_incl_max_u32(head ? (uint32_t)lane : 0u);
+  wave_segmented_sum<8>(acc, lane, seg_start);
+  const bool tail = lane == 63 || next != v;
+  if (tail && v >= 0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (acc[q] != 0.f) atomicAdd(grad + G.vox_key[(long long)v * 8 + q], acc[q]);
+  }
 }
 
 // Valid-gradient table: voxels (leaves with >= 0.9 cells, non-leaves with 0.9..1.5 cells) whose
@@ -218,12 +327,38 @@ __global__ __launch_bounds__(256) void table_write_kernel(int N, const float* __
 
 // ------------------------------------------------------------------------------------------
 hipError_t launch_grid_loss(bool laplacian, const GridTable& G, const bool* mask, const int32_t* active, int A,
-                            float weight, float inv, uint32_t seed, const float* randv, float* grad,
-                            hipStream_t st) {
+                            float weight, float inv, uint32_t seed, const int32_t* step, const float* randv,
+                            float* grad, hipStream_t st) {
   if (A <= 0) return hipSuccess;
   const unsigned nb = (unsigned)((A + 255) / 256);
-  if (laplacian) grid_laplacian_kernel<<<nb, 256, 0, st>>>(G, mask, active, A, weight, inv, seed, randv, grad);
-  else grid_eikonal_kernel<<<nb, 256, 0, st>>>(G, mask, active, A, weight, inv, seed, randv, grad);
+  if (laplacian) grid_laplacian_kernel<<<nb, 256, 0, st>>>(G, mask, active, A, weight, inv, seed, step, randv, grad);
+  else grid_eikonal_kernel<<<nb, 256, 0, st>>>(G, mask, active, A, weight, inv, seed, step, randv, grad);
+  return hipGetLastError();
+}
+
+hipError_t launch_grid_laplacian_pull(const GridTable& G, const bool* mask, const int32_t* vm_keys,
+                                      const int32_t* vm_vox, float weight, float inv, uint32_t seed,
+                                      const int32_t* step, float* coef, float* grad, hipStream_t st) {
+  if (G.n_keys <= 0) return hipSuccess;
+  const unsigned nb = (unsigned)((G.n_keys + 255) / 256);
+  grid_laplacian_coef_kernel<<<nb, 256, 0, st>>>(G, mask, weight, inv, seed, step, coef);
+  grid_laplacian_pull_kernel<<<nb, 256, 0, st>>>(G, vm_keys, vm_vox, seed, step, coef, grad);
+  return hipGetLastError();
+}
+
+// Cell -> table index for every cell of the res^3 grid: the entry key_search returns (so
+// duplicate keys resolve exactly as the search does), -1 where the cell has no key.  Turns the
+// six or seven binary searches per active key into one load each.
+__global__ __launch_bounds__(256) void grid_dense_index_kernel(long long n_cells, const int32_t* __restrict__ keys,
+                                                               int G, int32_t* __restrict__ out) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  out[c] = key_search(keys, G, (int)c);
+}
+
+hipError_t launch_grid_dense_index(long long n_cells, const int32_t* keys, int G, int32_t* out, hipStream_t st) {
+  if (n_cells <= 0) return hipSuccess;
+  grid_dense_index_kernel<<<(unsigned)((n_cells + 255) / 256), 256, 0, st>>>(n_cells, keys, G, out);
   return hipGetLastError();
 }
 

@@ -15,9 +15,22 @@ struct ViewDesc;
 hipError_t launch_sh_views_forward(int P, int V, const uint8_t* quads, const ViewDesc* views, int deg, int M,
                                    const float* centers, const float* sh0, const float* shs, float* rgbs_views,
                                    hipStream_t st);
+// Unbiased Adam with bias corrections looked up by a device-resident step counter.
+struct AdamTable {
+  const int32_t* step;   // 1-based step count
+  const float2* tab;     // n entries {lr / (1 - b1^t), 1 / sqrt(1 - b2^t)}
+  int n;
+  float b1, b2, eps;
+};
+// Adam state of the SH coefficients, for the SH backward that updates them in place.
+struct ShAdam {
+  float* sh0; float* m0; float* v0;
+  float* shs; float* ms; float* vs;
+  AdamTable t0, ts;
+};
 hipError_t launch_sh_views_backward(int P, int V, const uint8_t* quads, const ViewDesc* views, int deg, int M,
                                     const float* centers, float* drgb_views, float* dsh0, float* dshs,
-                                    hipStream_t st);
+                                    const ShAdam* adam, hipStream_t st);
 hipError_t launch_voxel_activity(int P, const uint8_t* is_leaf, const float* geos, const float* s_val,
                                  const float* centers, const float* lengths, float cam_scale, uint8_t* out,
                                  hipStream_t st);
@@ -27,16 +40,23 @@ hipError_t launch_gather_geo_bw(int n, const int64_t* vox_key, const int64_t* ca
                                 float* dgrid, hipStream_t st);
 hipError_t launch_adam(bool sparse, bool biased, long long n, float* p, const float* g, float* m, float* v,
                        float step_size, float b1, float b2, float rsqrt_bc2, float eps, hipStream_t st);
+hipError_t launch_adam_table(bool sparse, long long n, float* p, const float* g, float* m, float* v,
+                             const AdamTable& T, hipStream_t st);
 hipError_t launch_ijk_to_octpath(int n, const int64_t* ijk, const int8_t* lv, int64_t* out, hipStream_t st);
 hipError_t launch_octpath_to_ijk(int n, const int64_t* path, const int8_t* lv, int64_t* ijk, hipStream_t st);
 
 struct GridTable {
   const float* grid_pts; const int64_t* vox_key; const int32_t* keys; const int32_t* key2vox;
   const float* vox_coord; const float* vox_size; int n_keys; int res; int n_vox; long long n_pts;
+  const int32_t* dense = nullptr;   // optional cell -> key index map (launch_grid_dense_index)
 };
 hipError_t launch_grid_loss(bool laplacian, const GridTable& G, const bool* mask, const int32_t* active, int A,
-                            float weight, float inv, uint32_t seed, const float* randv, float* grad,
-                            hipStream_t st);
+                            float weight, float inv, uint32_t seed, const int32_t* step, const float* randv,
+                            float* grad, hipStream_t st);
+hipError_t launch_grid_laplacian_pull(const GridTable& G, const bool* mask, const int32_t* vm_keys,
+                                      const int32_t* vm_vox, float weight, float inv, uint32_t seed,
+                                      const int32_t* step, float* coef, float* grad, hipStream_t st);
+hipError_t launch_grid_dense_index(long long n_cells, const int32_t* keys, int G, int32_t* out, hipStream_t st);
 hipError_t launch_table_count(int N, const float* center, const float* size, const float* scene_center,
                               float extent, int res, const bool* is_leaf, uint32_t* count, hipStream_t st);
 hipError_t launch_table_write(int N, const float* center, const float* size, const float* scene_center,

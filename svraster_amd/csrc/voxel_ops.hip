@@ -192,19 +192,52 @@ __global__ __launch_bounds__(256) void sh_views_forward_kernel(int P, int V, con
 // written (zeros for voxels no view saw), so the outputs need no separate clearing.  The
 // per-view colour gradients it consumes are reset to zero, so the backward compositor's
 // accumulation buffer is clear again for the next step without a separate fill.
-template <int kDeg>
+// Adam (adam_step.cu:21-102).  lerp(v0, v1, t) = fma(t, v1, fma(-t, v0, v0)).
+__device__ inline float adam_lerp(float v0, float v1, float t) { return fmaf(t, v1, fmaf(-t, v0, v0)); }
+
+// One element: m = lerp(g, m, b1), v = lerp(g^2, v, b2), p -= step_size * m / denom.
+template <bool kBiased>
+__device__ inline void adam_update(float& p, float& m, float& v, float g, float step_size, float b1, float b2,
+                                   float rsqrt_bc2, float eps) {
+#pragma clang fp contract(off)
+  const float mv = adam_lerp(g, m, b1);
+  const float vv = adam_lerp(g * g, v, b2);
+  const float denom = kBiased ? sqrtf(vv) + eps : fmaf(sqrtf(vv), rsqrt_bc2, eps);
+  p -= step_size * (mv / denom);
+  m = mv;
+  v = vv;
+}
+
+// Unbiased step whose bias corrections come from a table indexed by a step counter held in
+// device memory: entry t-1 = {lr / (1 - b1^t), 1 / sqrt(1 - b2^t)}, rounded to float from the
+// same double expressions as svr_unbiased_adam_step; steps past the table use its last entry.
+__device__ inline float2 adam_table_entry(const AdamTable& T) {
+  const int t = min(max(*T.step, 1), T.n);
+  return T.tab[t - 1];
+}
+
+// kAdam: instead of writing dL/dsh0 and dL/dshs, apply the sparse unbiased Adam step of `A` to
+// sh0/shs right here (elements whose gradient is exactly 0 are left alone, as the sparse step
+// does), so the SH gradients never reach HBM.  Same gradient values and Adam arithmetic as
+// the unfused pair of calls.
+template <int kDeg, bool kAdam>
 __global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, const uint8_t* __restrict__ quads,
                                                                 const ViewDesc* __restrict__ views, int M,
                                                                 const float* __restrict__ centers,
                                                                 float* __restrict__ drgb_views,
                                                                 float* __restrict__ dsh0,
-                                                                float* __restrict__ dshs) {
+                                                                float* __restrict__ dshs, ShAdam A) {
   constexpr int NB = (kDeg + 1) * (kDeg + 1);
   __shared__ float s_rows[256 * kShRowMax];
   const int t = threadIdx.x;
   const long long i0 = (long long)blockIdx.x * 256;
   const int nv = (int)min(256LL, (long long)P - i0);
   const int row = (M - 1) * 3;
+  float2 h0 = make_float2(0.f, 0.f), hs = h0;
+  if (kAdam) {
+    h0 = adam_table_entry(A.t0);
+    hs = adam_table_entry(A.ts);
+  }
   if (t < nv) {
     const int i = (int)i0 + t;
     float acc[NB][3];
@@ -231,7 +264,20 @@ __global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, co
         }
       }
     }
-    dsh0[3 * i] = acc[0][0]; dsh0[3 * i + 1] = acc[0][1]; dsh0[3 * i + 2] = acc[0][2];
+    if (kAdam) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float g = acc[0][c];
+        if (g != 0.0f) {
+          const long long e = 3LL * i + c;
+          float pv = A.sh0[e], mv = A.m0[e], vv = A.v0[e];
+          adam_update<false>(pv, mv, vv, g, h0.x, A.t0.b1, A.t0.b2, h0.y, A.t0.eps);
+          A.sh0[e] = pv; A.m0[e] = mv; A.v0[e] = vv;
+        }
+      }
+    } else {
+      dsh0[3 * i] = acc[0][0]; dsh0[3 * i + 1] = acc[0][1]; dsh0[3 * i + 2] = acc[0][2];
+    }
     float* c = s_rows + t * kShRowMax;
 #pragma unroll
     for (int k = 1; k < NB; ++k) {
@@ -240,27 +286,49 @@ __global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, co
     for (int k = 3 * (NB - 1); k < row; ++k) c[k] = 0.f;
   }
   __syncthreads();
-  float* dst = dshs + i0 * row;
-  for (int k = t; k < nv * row; k += 256) dst[k] = s_rows[(k / row) * kShRowMax + (k % row)];
+  if (kAdam) {
+    for (int k = t; k < nv * row; k += 256) {
+      const float g = s_rows[(k / row) * kShRowMax + (k % row)];
+      if (g == 0.0f) continue;
+      const long long e = i0 * row + k;
+      float pv = A.shs[e], mv = A.ms[e], vv = A.vs[e];
+      adam_update<false>(pv, mv, vv, g, hs.x, A.ts.b1, A.ts.b2, hs.y, A.ts.eps);
+      A.shs[e] = pv; A.ms[e] = mv; A.vs[e] = vv;
+    }
+  } else {
+    float* dst = dshs + i0 * row;
+    for (int k = t; k < nv * row; k += 256) dst[k] = s_rows[(k / row) * kShRowMax + (k % row)];
+  }
 }
-
-// Adam (adam_step.cu:21-102).  lerp(v0, v1, t) = fma(t, v1, fma(-t, v0, v0)).
-__device__ inline float adam_lerp(float v0, float v1, float t) { return fmaf(t, v1, fmaf(-t, v0, v0)); }
 
 template <bool kSparse, bool kBiased>
 __global__ __launch_bounds__(256) void adam_kernel(long long n, float* __restrict__ p,
                                                    const float* __restrict__ g, float* __restrict__ m,
                                                    float* __restrict__ v, float step_size, float b1,
                                                    float b2, float rsqrt_bc2, float eps) {
-#pragma clang fp contract(off)
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float gv = g[i];
   if (kSparse && gv == 0.0f) return;
-  const float mv = adam_lerp(gv, m[i], b1);
-  const float vv = adam_lerp(gv * gv, v[i], b2);
-  const float denom = kBiased ? sqrtf(vv) + eps : fmaf(sqrtf(vv), rsqrt_bc2, eps);
-  p[i] -= step_size * (mv / denom);
+  float pv = p[i], mv = m[i], vv = v[i];
+  adam_update<kBiased>(pv, mv, vv, gv, step_size, b1, b2, rsqrt_bc2, eps);
+  p[i] = pv;
+  m[i] = mv;
+  v[i] = vv;
+}
+
+template <bool kSparse>
+__global__ __launch_bounds__(256) void adam_table_kernel(long long n, float* __restrict__ p,
+                                                         const float* __restrict__ g, float* __restrict__ m,
+                                                         float* __restrict__ v, AdamTable T) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gv = g[i];
+  if (kSparse && gv == 0.0f) return;
+  const float2 h = adam_table_entry(T);
+  float pv = p[i], mv = m[i], vv = v[i];
+  adam_update<false>(pv, mv, vv, gv, h.x, T.b1, T.b2, h.y, T.eps);
+  p[i] = pv;
   m[i] = mv;
   v[i] = vv;
 }
@@ -329,15 +397,22 @@ hipError_t launch_sh_views_forward(int P, int V, const uint8_t* quads, const Vie
 
 hipError_t launch_sh_views_backward(int P, int V, const uint8_t* quads, const ViewDesc* views, int deg, int M,
                                     const float* centers, float* drgb_views, float* dsh0, float* dshs,
-                                    hipStream_t st) {
+                                    const ShAdam* adam, hipStream_t st) {
   if (P <= 0) return hipSuccess;
   const unsigned nb = blocks_for(P);
+  const ShAdam A = adam ? *adam : ShAdam{};
+#define SVR_SHB(D)                                                                                           \
+  if (adam) sh_views_backward_kernel<D, true><<<nb, 256, 0, st>>>(P, V, quads, views, M, centers, drgb_views,  \
+                                                                  dsh0, dshs, A);                           \
+  else sh_views_backward_kernel<D, false><<<nb, 256, 0, st>>>(P, V, quads, views, M, centers, drgb_views, dsh0, \
+                                                              dshs, A);
   switch (deg) {
-    case 0: sh_views_backward_kernel<0><<<nb, 256, 0, st>>>(P, V, quads, views, M, centers, drgb_views, dsh0, dshs); break;
-    case 1: sh_views_backward_kernel<1><<<nb, 256, 0, st>>>(P, V, quads, views, M, centers, drgb_views, dsh0, dshs); break;
-    case 2: sh_views_backward_kernel<2><<<nb, 256, 0, st>>>(P, V, quads, views, M, centers, drgb_views, dsh0, dshs); break;
-    default: sh_views_backward_kernel<3><<<nb, 256, 0, st>>>(P, V, quads, views, M, centers, drgb_views, dsh0, dshs); break;
+    case 0: SVR_SHB(0) break;
+    case 1: SVR_SHB(1) break;
+    case 2: SVR_SHB(2) break;
+    default: SVR_SHB(3) break;
   }
+#undef SVR_SHB
   return hipGetLastError();
 }
 
@@ -392,6 +467,15 @@ hipError_t launch_adam(bool sparse, bool biased, long long n, float* p, const fl
   else if (sparse) adam_kernel<true, false><<<nb, 256, 0, st>>>(n, p, g, m, v, step_size, b1, b2, rsqrt_bc2, eps);
   else if (biased) adam_kernel<false, true><<<nb, 256, 0, st>>>(n, p, g, m, v, step_size, b1, b2, rsqrt_bc2, eps);
   else adam_kernel<false, false><<<nb, 256, 0, st>>>(n, p, g, m, v, step_size, b1, b2, rsqrt_bc2, eps);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam_table(bool sparse, long long n, float* p, const float* g, float* m, float* v,
+                             const AdamTable& T, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const unsigned nb = blocks_for(n);
+  if (sparse) adam_table_kernel<true><<<nb, 256, 0, st>>>(n, p, g, m, v, T);
+  else adam_table_kernel<false><<<nb, 256, 0, st>>>(n, p, g, m, v, T);
   return hipGetLastError();
 }
 

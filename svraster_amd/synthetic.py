@@ -20,6 +20,8 @@ analytic SDF plus N(0, 0.01 * size of the finest voxel touching that corner).
 from __future__ import annotations
 
 import math
+import os
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import List, Optional
 
@@ -32,14 +34,22 @@ MAX_NUM_LEVELS = 16
 # ----------------------------------------------------------------------------------------------
 # Octree helpers (pure integer arithmetic; identical formulas to the reference's utils.cu)
 # ----------------------------------------------------------------------------------------------
+def _spread3(v: np.ndarray) -> np.ndarray:
+    """Move bit b of a 16-bit value to bit 3*b (the classic Morton "part1by2" masks)."""
+    v = v & 0xFFFF
+    v = (v | (v << 16)) & 0x0000FF0000FF
+    v = (v | (v << 8)) & 0x00F00F00F00F
+    v = (v | (v << 4)) & 0x0C30C30C30C3
+    v = (v | (v << 2)) & 0x249249249249
+    return v
+
+
 def ijk_to_octpath_np(ijk: np.ndarray, level: np.ndarray) -> np.ndarray:
-    """Morton-encode ijk at `level` into a left-aligned 48-bit octpath (utils.cu:129-158)."""
+    """Morton-encode ijk at `level` into a left-aligned 48-bit octpath (utils.cu:129-158):
+    bit 3l+2 / 3l+1 / 3l of the path are bit l of i / j / k."""
     ijk = ijk.astype(np.int64)
     level = level.astype(np.int64).reshape(-1)
-    path = np.zeros(ijk.shape[0], dtype=np.int64)
-    for l in range(MAX_NUM_LEVELS):
-        bits = (((ijk[:, 0] >> l) & 1) << 2) | (((ijk[:, 1] >> l) & 1) << 1) | ((ijk[:, 2] >> l) & 1)
-        path |= bits << (3 * l)
+    path = (_spread3(ijk[:, 0]) << 2) | (_spread3(ijk[:, 1]) << 1) | _spread3(ijk[:, 2])
     return path << (3 * (MAX_NUM_LEVELS - level))
 
 
@@ -67,9 +77,13 @@ class SDFShapes:
     box_c: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
     box_h: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
 
-    def eval(self, x: np.ndarray, chunk: int = 1 << 20) -> np.ndarray:
+    def eval(self, x: np.ndarray, chunk: int = 1 << 15) -> np.ndarray:
+        """Union SDF at points x [n,3] (float64).  Chunks run on a thread pool (numpy releases
+        the GIL in its loops); every chunk does the same operations, so the result does not
+        depend on the thread count."""
         out = np.empty(x.shape[0], dtype=np.float64)
-        for s in range(0, x.shape[0], chunk):
+
+        def work(s):
             xs = x[s:s + chunk]
             d = np.full(xs.shape[0], np.inf)
             for c, r in zip(self.sphere_c, self.sphere_r):
@@ -80,6 +94,14 @@ class SDFShapes:
                 inside = np.minimum(q.max(-1), 0.0)
                 d = np.minimum(d, outside + inside)
             out[s:s + chunk] = d
+
+        starts = range(0, x.shape[0], chunk)
+        if x.shape[0] <= 4 * chunk:
+            for s in starts:
+                work(s)
+        else:
+            with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+                list(ex.map(work, starts))
         return out
 
 
@@ -136,26 +158,33 @@ def s_init_rule(h_min: float) -> float:
 
 def build_octree(shapes: SDFShapes, min_level: int, max_level: int,
                  extent: float = 2.0, center=(0.0, 0.0, 0.0),
-                 target_voxels: Optional[int] = None):
-    """Return (ijk [N,3] int64, level [N] int64) of leaves, ordered by octpath.
+                 target_voxels: Optional[int] = None,
+                 keep_parents_from: Optional[int] = None):
+    """Return (ijk [N,3] int64, level [N] int64, is_leaf [N] bool) ordered by (octpath, level).
 
     A cell at level L < max_level is a split candidate when |SDF(centre)| < its diagonal.
     With `target_voxels` set, candidates are split closest-to-surface first (smallest
     |SDF|/size, ties by position), spending half of the remaining leaf budget at each level
     and all of it at the last split, so every level up to max_level is populated and the
-    leaf count lands near the target."""
+    leaf count lands near the target.
+
+    With `keep_parents_from` set, a split cell at that level or deeper stays in the list as a
+    non-leaf voxel, as the reference's subdivision keeps parents at depth >= 9
+    (adaptive.py:60-65, 186-188); a parent sorts before its first child, which shares its
+    octpath."""
     cmin = np.asarray(center, dtype=np.float64) - 0.5 * extent
     n0 = 1 << min_level
     g = np.arange(n0, dtype=np.int64)
     ijk = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
     sub = np.array([[(c >> 2) & 1, (c >> 1) & 1, c & 1] for c in range(8)], dtype=np.int64)
-    leaves_ijk, leaves_lv = [], []
+    leaves_ijk, leaves_lv, leaves_flag = [], [], []
     n_leaves = 0
     for lv in range(min_level, max_level + 1):
         h = extent / (1 << lv)
         if lv == max_level:
             leaves_ijk.append(ijk)
             leaves_lv.append(np.full(ijk.shape[0], lv, dtype=np.int64))
+            leaves_flag.append(np.ones(ijk.shape[0], dtype=bool))
             break
         ctr = cmin + (ijk.astype(np.float64) + 0.5) * h
         d = np.abs(shapes.eval(ctr)) / h
@@ -169,27 +198,33 @@ def build_octree(shapes: SDFShapes, min_level: int, max_level: int,
                 cand.sort()
         split = np.zeros(ijk.shape[0], dtype=bool)
         split[cand] = True
-        leaves_ijk.append(ijk[~split])
-        leaves_lv.append(np.full(int((~split).sum()), lv, dtype=np.int64))
+        keep = ~split if keep_parents_from is None or lv < keep_parents_from else \
+            np.ones(ijk.shape[0], dtype=bool)
+        leaves_ijk.append(ijk[keep])
+        leaves_lv.append(np.full(int(keep.sum()), lv, dtype=np.int64))
+        leaves_flag.append(~split[keep])
         n_leaves += int((~split).sum())
         par = ijk[split]
         ijk = ((par[:, None, :] << 1) + sub[None]).reshape(-1, 3)
     ijk = np.concatenate(leaves_ijk, 0)
     lv = np.concatenate(leaves_lv, 0)
+    flag = np.concatenate(leaves_flag, 0)
     path = ijk_to_octpath_np(ijk, lv)
-    order = np.argsort(path, kind="stable")
-    return ijk[order], lv[order]
+    order = np.lexsort((lv, path))
+    return ijk[order], lv[order], flag[order]
 
 
 def build_scene(n_spheres: int, min_level: int, max_level: int, sh_degree: int,
                 seed: int = 0, n_boxes: int = 0, sh_init: str = "normal",
                 device="cpu", extent: float = 2.0,
-                target_voxels: Optional[int] = None) -> Scene:
+                target_voxels: Optional[int] = None,
+                keep_parents_from: Optional[int] = None) -> Scene:
     """Build a synthetic scene.  `sh_init` = "uniform" draws sh0 ~ U[0,1] (degree 0 configs),
     "normal" draws all SH coefficients ~ N(0, 0.1)."""
     rng = np.random.default_rng(seed)
     shapes = random_shapes(rng, n_spheres, n_boxes)
-    ijk, lv = build_octree(shapes, min_level, max_level, extent, target_voxels=target_voxels)
+    ijk, lv, leaf = build_octree(shapes, min_level, max_level, extent, target_voxels=target_voxels,
+                                 keep_parents_from=keep_parents_from)
     N = ijk.shape[0]
     center = np.zeros(3)
     cmin = center - 0.5 * extent
@@ -235,7 +270,7 @@ def build_scene(n_spheres: int, min_level: int, max_level: int, sh_degree: int,
         sh0=torch.from_numpy(sh0).to(f32).to(device),
         shs=torch.from_numpy(shs).to(f32).to(device),
         sh_degree=sh_degree,
-        is_leaf=torch.ones(N, dtype=torch.uint8, device=device),
+        is_leaf=torch.from_numpy(leaf.astype(np.uint8)).to(device),
         log_s=s_init_rule(h_min),
         scene_center=torch.zeros(3, dtype=f32, device=device),
         scene_extent=extent,
@@ -328,7 +363,8 @@ def make_workload(name: str, device="cpu", scale: Optional[float] = None) -> Wor
     fd_256       : 4 spheres, levels 4-7, SH deg 1, one 256x256 view, fx=204.8
     multi_32     : 24 spheres, levels 6-12, SH deg 3, 32 views (8 each of 640x480, 1237x822,
                    1600x1064, 1920x1080), fx=0.8*W, radii U[2,3]
-    train_16     : 48 spheres + 8 boxes, levels 7-13, SH deg 3, 16 views 1920x1080
+    train_16     : 48 spheres + 8 boxes, levels 7-13, SH deg 3, 16 views 1920x1080; split cells
+                   at level >= 9 stay as non-leaf parents (the smoothness-loss grid table)
     """
     if name == "small_128":
         sc = build_scene(3, 3, 6, 0, seed=0, sh_init="uniform", device=device,
@@ -354,7 +390,7 @@ def make_workload(name: str, device="cpu", scale: Optional[float] = None) -> Wor
                 v += 1
     elif name == "train_16":
         sc = build_scene(48, 7, 13, 3, seed=0, n_boxes=8, device=device,
-                         target_voxels=16_000_000)
+                         target_voxels=16_000_000, keep_parents_from=9)
         cams = [sphere_camera(2000 + v, 2.5, 1920, 1080, 1536.0, 1536.0, device=device)
                 for v in range(16)]
     else:
