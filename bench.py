@@ -227,9 +227,12 @@ def bench_train(args):
         sync()
         return {name: round(ev[i].elapsed_time(ev[i + 1]), 3) for i, (name, _) in enumerate(stages)}
 
-    ms_pv = time_steps(step_pv, args.steps, args.warmup, sync)
-    st_pv = stage_times(pv_stages)
-    log(f"[bench] per-view iteration: {ms_pv:.2f} ms; stages {st_pv}")
+    if args.profile:              # profiler runs time the batched iteration only, eagerly
+        ms_pv, st_pv = float("nan"), {}
+    else:
+        ms_pv = time_steps(step_pv, args.steps, args.warmup, sync)
+        st_pv = stage_times(pv_stages)
+        log(f"[bench] per-view iteration: {ms_pv:.2f} ms; stages {st_pv}")
     del opt
     zero_grads(model)
     torch.cuda.empty_cache()
@@ -245,7 +248,8 @@ def bench_train(args):
     log(f"[bench] batched set-up {time.perf_counter() - t0:.1f} s, keys {rast.key_count()} / capacity {rast.cap}")
     st_b = {k: round(v, 3) for k, v in it.stage_times().items()}
     kt_b = kernel_times(lib, it)
-    it.capture()
+    if not args.profile:
+        it.capture()
     ms_b = time_steps(it, args.steps, args.warmup, sync)
     log(f"[bench] batched graph iteration: {ms_b:.2f} ms; stages (eager) {st_b}")
 
@@ -265,7 +269,8 @@ def bench_train(args):
                                f"kept from level 9), SH deg 3, {len(cams)} views 1920x1080, {rays} rays/step; "
                                f"training iteration: fwd + MSE + bwd + Laplacian smoothness (weight {weight}) "
                                f"+ sparse Adam (lr {lr}, betas {betas})",
-                   "path": "batched: one captured HIP graph per iteration"},
+                   "path": "batched: " + ("eager (profiler run)" if args.profile
+                                          else "one captured HIP graph per iteration")},
         "paths": {"batched_ms_per_step": round(ms_b, 3), "per_view_ms_per_step": round(ms_pv, 3),
                   "speedup_batched_vs_per_view": round(ms_pv / ms_b, 3),
                   "batched_mrays_s": round(value, 3), "per_view_mrays_s": round(rays / (ms_pv / 1000.0) / 1e6, 3),

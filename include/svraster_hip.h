@@ -164,20 +164,35 @@ int svr_grid_dense_index(const int32_t* grid_keys, int G, int grid_res, int32_t*
 /* svr_grid_eikonal_bw (laplacian = 0) / svr_laplacian_smoothness_bw (laplacian = 1) for a
  * captured training step: cells are looked up through dense_index (svr_grid_dense_index) and the
  * random sample offsets are drawn from seed and the device step counter *step, so every replay
- * samples new offsets as the reference's clock seed does (ls_compute.cu:43).
- * active_list == NULL means every table entry, in table order (the reference trainer's list with
- * ls_drop_ratio 0, train.py:640-642, up to order); the Laplacian then runs in two passes:
- * per-key coefficients into coef_ws (float[3G]), then a walk over the table in voxel order
- * (vm_keys / vm_vox: the cells and voxels of svr_valid_gradient_table_fill's output, before the
- * key sort) that sums each voxel's contributions before adding them to grid_pts_grad.  That form
- * needs every cell to appear at most once in grid_keys. */
+ * samples new offsets as the reference's clock seed does (ls_compute.cu:43). */
 int svr_grid_smoothness_bw_step(int laplacian, const float* grid_pts, long long n_pts, const int64_t* vox_key, int N,
                                 const float* grid_voxel_coord, const float* grid_voxel_size, int grid_res,
                                 const uint8_t* grid_mask, const int32_t* grid_keys, const int32_t* grid2voxel, int G,
                                 const int32_t* dense_index, const int32_t* active_list, int A, float weight,
-                                float vox_size_inv, unsigned int seed, const int32_t* step,
-                                const int32_t* vm_keys, const int32_t* vm_vox, float* coef_ws,
-                                float* grid_pts_grad, void* stream);
+                                float vox_size_inv, unsigned int seed, const int32_t* step, float* grid_pts_grad,
+                                void* stream);
+
+/* LS_COMPUTE::laplacian_smoothness_bw (src/ls_compute.cu:112-155) with every table entry active
+ * once (the reference trainer's randperm list with ls_drop_ratio 0, train.py:640-642, taken in
+ * table order instead), for a captured training step.  The table: svr_valid_gradient_table_fill's
+ * cells before any key sort, voxel by voxel, with each voxel's cell box reordered x fastest (z, y,
+ * x loops).  This form requires exactly that layout with unique cells, grid_voxel_coord /
+ * grid_voxel_size equal to each table voxel's integer box (power-of-two sizes, as octree voxels
+ * have) and grid_mask the table's own mask.  The table voxels are numbered 0..T-1 (tvox: voxel
+ * id of each); table_keys / table_tvox: cell and table voxel of entry e (int32[G]); cell_entry /
+ * cell_tvox: entry and table voxel of each of the res^3 cells, -1 where absent.  The random
+ * offset of entry e's key is drawn from (seed, *step, e).  Three launches: a 12-float record
+ * per table voxel into rec_ws (float[12T]: box and corner SDFs), per-key coefficients into
+ * coef_ws (float[3G]), then one lane per entry sums the contributions landing in its cell, the
+ * lanes of a voxel are reduced across the wavefront and each voxel run adds its 8 corner sums
+ * to grid_pts_grad. */
+int svr_grid_laplacian_bw_table_order(const float* grid_pts, long long n_pts, const int64_t* vox_key, int N,
+                                      const float* grid_voxel_coord, const float* grid_voxel_size, int grid_res,
+                                      const uint8_t* grid_mask, const int32_t* table_keys,
+                                      const int32_t* table_tvox, int G, const int32_t* cell_entry,
+                                      const int32_t* cell_tvox, const int32_t* tvox, int T, float weight,
+                                      float vox_size_inv, unsigned int seed, const int32_t* step, float* rec_ws,
+                                      float* coef_ws, float* grid_pts_grad, void* stream);
 
 /* Replace UTILS::ijk_2_octpath / octpath_2_ijk (src/utils.cu:129-189, 329-361). */
 int svr_ijk_2_octpath(const int64_t* ijk, const int8_t* octlevel, int n, int64_t* octpath,

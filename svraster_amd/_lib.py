@@ -64,7 +64,8 @@ _SIGS = {
     "svr_adam_bias_table": (I, [D, D, D, I, P]),
     "svr_sparse_adam_step_dev": (I, [I, P, P, P, P, LL, P, P, I, F, F, F, P]),
     "svr_grid_dense_index": (I, [P, I, I, P, P]),
-    "svr_grid_smoothness_bw_step": (I, [I, P, LL, P, I, P, P, I, P, P, P, I, P, P, I, F, F, U, P, P, P, P, P, P]),
+    "svr_grid_smoothness_bw_step": (I, [I, P, LL, P, I, P, P, I, P, P, P, I, P, P, I, F, F, U, P, P, P]),
+    "svr_grid_laplacian_bw_table_order": (I, [P, LL, P, I, P, P, I, P, P, P, I, P, P, P, I, F, F, U, P, P, P, P, P]),
     "svr_batch_state_offsets": (I, [I, I, LL, LL, LL, P]),
     "svr_batch_key_count": (I, [P, I, I, LL, LL, LL, ctypes.POINTER(LL), P]),
 }

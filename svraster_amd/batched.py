@@ -292,15 +292,16 @@ class GridSmoothness:
     2^min(max level, 9), voxel coordinates and sizes in grid cells rounded to integers, keys
     sorted (octree_utils.py:293-315), weight * G / A, every key active.
 
-    The reference draws the active list as randperm(G)[:A] with A = G (ls_drop_ratio 0); the
-    identity order used here samples the same keys (only the order of the float sums differs).
-    Cells are looked up through a dense cell -> key index map built once (svr_grid_dense_index,
-    4 B per grid cell) instead of a binary search per sample, and the per-key random offsets come
-    from a hash of (seed, device step counter), so every replay of a captured step draws new
-    offsets.  When every cell appears once in the table (always, for tables built from a valid
-    octree), the loss runs in the two-pass form that sums each voxel's contributions before
-    touching the gradient (`pull`); otherwise, or with pull=False, one thread per key scatters
-    its 56 weighted terms with atomics as the reference kernel does."""
+    The reference draws the active list as randperm(G)[:A] with A = G (ls_drop_ratio 0): every
+    key once, in random order.  Here the keys are taken in table order (voxel by voxel), which
+    only changes which random sample offset goes with which key and the order of the float sums;
+    the offsets come from a hash of (seed, device step counter, position), so every replay of a
+    captured step draws new ones.  When the table has its built layout (each voxel's entries its
+    cell box, cells unique) the loss runs in the two-pass table-order form
+    (svr_grid_laplacian_bw_table_order) that sums each voxel's contributions before touching the
+    gradient; otherwise, or with pull=False, one thread per key scatters its 56 weighted terms
+    with atomics as the reference kernel does, with cells found through a dense cell -> key map
+    (svr_grid_dense_index) instead of a binary search per sample."""
 
     def __init__(self, vox_center, vox_size, octlevel, is_leaf, vox_key, scene_center, inside_extent: float,
                  weight: float, seed: int = 0, pull: bool = True):
@@ -314,7 +315,8 @@ class GridSmoothness:
         self.keys, order = keys.sort(stable=True)
         self.g2v = g2v[order].contiguous()
         del order
-        self.vm_keys, self.vm_vox = keys, g2v          # table order: voxel by voxel
+        self.table_keys, self.table_vox = keys, g2v          # table order: voxel by voxel
+        del keys, g2v
         self.mask = mask
         self.G = int(self.keys.numel())
         sc = scene_center.to(self.dev).reshape(1, 3).float()
@@ -323,28 +325,88 @@ class GridSmoothness:
         self.size = torch.round(vox_size.reshape(-1) / inside_extent * self.res).float().contiguous()
         self.vox_key = vox_key.contiguous()
         self.N = int(vox_center.shape[0])
-        self.dense = torch.empty(self.res ** 3, dtype=torch.int32, device=self.dev)
+        n3 = self.res ** 3
+        self.dense = torch.empty(n3, dtype=torch.int32, device=self.dev)
         _lib.check(self.lib.svr_grid_dense_index(_p(self.keys), self.G, self.res, _p(self.dense), self._stream()),
                    "svr_grid_dense_index")
-        self.active = torch.arange(self.G, dtype=torch.int32, device=self.dev)
-        unique = int((self.dense >= 0).sum().item()) == self.G
-        self.pull = bool(pull) and unique
-        self.coef = torch.empty(3 * self.G if self.pull else 0, dtype=torch.float32, device=self.dev)
+        layout_ok = self._table_layout_ok()
+        # Active list = every key in table order, as indices into the sorted keys.
+        self.active = self.dense[self.table_keys.long()].contiguous()
+        self.pull = bool(pull) and layout_ok
+        if self.pull:
+            tk = self.table_keys.long()
+            self.cell_entry = torch.full((n3,), -1, dtype=torch.int32, device=self.dev)
+            self.cell_entry[tk] = torch.arange(self.G, dtype=torch.int32, device=self.dev)
+            # compact numbering of the table's voxels, in table order
+            self.tvox, self.table_tv = torch.unique_consecutive(self.table_vox, return_inverse=True)
+            self.tvox = self.tvox.to(torch.int32).contiguous()
+            self.table_tv = self.table_tv.to(torch.int32).contiguous()
+            self.T = int(self.tvox.numel())
+            self.cell_tv = torch.full((n3,), -1, dtype=torch.int32, device=self.dev)
+            self.cell_tv[tk] = self.table_tv
+            del tk
+            self.rec = torch.empty(12 * self.T, dtype=torch.float32, device=self.dev)
+            self.coef = torch.empty(3 * self.G, dtype=torch.float32, device=self.dev)
         self.weight = float(weight)          # * G / A with A = G
         self.vox_size_inv = float(self.res / inside_extent)
         self.seed = int(seed) & 0xFFFFFFFF
+
+    def _table_layout_ok(self) -> bool:
+        """Cells unique and each voxel's entries exactly its integer cell box, of power-of-two
+        size (the table build writes a box x-major, z fastest); on success the entries of every
+        voxel are reordered in place to x fastest, the layout svr_grid_laplacian_bw_table_order
+        reads."""
+        if self.G == 0:
+            return False
+        if int((self.dense >= 0).sum().item()) != self.G:
+            return False
+        k = self.table_keys.long()
+        v = self.table_vox.long()
+        res = self.res
+        x, y, z = k % res, (k // res) % res, k // (res * res)
+        lo = self.coord.long()[v]
+        s = self.size.long()[v]
+        if not bool(((s > 0) & ((s & (s - 1)) == 0)).all()):
+            return False                               # sizes must be powers of two
+        dx, dy, dz = x - lo[:, 0], y - lo[:, 1], z - lo[:, 2]
+        inside = (dx >= 0) & (dx < s) & (dy >= 0) & (dy < s) & (dz >= 0) & (dz < s)
+        if not bool(inside.all()):
+            return False
+        e = torch.arange(self.G, device=self.dev)
+        local = (dx * s + dy) * s + dz
+        first = e - local
+        same = v[1:] == v[:-1]
+        # entries of a voxel are consecutive, start at local 0 and run through s^3 - 1
+        ok = bool((first[1:][same] == first[:-1][same]).all()) and bool((local[1:][~same] == 0).all())
+        ok = ok and int(local[0]) == 0
+        last = torch.cat([~same, torch.ones(1, dtype=torch.bool, device=self.dev)])
+        ok = ok and bool((local[last] == s[last] ** 3 - 1).all())
+        if not ok:
+            return False
+        # x fastest inside each voxel: order by (first entry of the voxel, local index z, y, x)
+        local_x = (dz * s + dy) * s + dx
+        order = torch.argsort(first * (1 << 30) + local_x)
+        self.table_keys = self.table_keys[order].contiguous()
+        self.table_vox = self.table_vox[order].contiguous()
+        return True
 
     def _stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
 
     def __call__(self, grid_pts: torch.Tensor, grid_grad: torch.Tensor, step: torch.Tensor):
-        pull = self.pull
-        _lib.check(self.lib.svr_grid_smoothness_bw_step(
-            1, _p(grid_pts), grid_pts.numel(), _p(self.vox_key), self.N, _p(self.coord), _p(self.size), self.res,
-            _p(self.mask), _p(self.keys), _p(self.g2v), self.G, _p(self.dense), None if pull else _p(self.active),
-            self.G, ctypes.c_float(self.weight), ctypes.c_float(self.vox_size_inv), ctypes.c_uint(self.seed),
-            _p(step), _p(self.vm_keys) if pull else None, _p(self.vm_vox) if pull else None,
-            _p(self.coef) if pull else None, _p(grid_grad), self._stream()), "svr_grid_smoothness_bw_step")
+        if self.pull:
+            _lib.check(self.lib.svr_grid_laplacian_bw_table_order(
+                _p(grid_pts), grid_pts.numel(), _p(self.vox_key), self.N, _p(self.coord), _p(self.size), self.res,
+                _p(self.mask), _p(self.table_keys), _p(self.table_tv), self.G, _p(self.cell_entry),
+                _p(self.cell_tv), _p(self.tvox), self.T, ctypes.c_float(self.weight),
+                ctypes.c_float(self.vox_size_inv), ctypes.c_uint(self.seed), _p(step), _p(self.rec), _p(self.coef),
+                _p(grid_grad), self._stream()), "svr_grid_laplacian_bw_table_order")
+        else:
+            _lib.check(self.lib.svr_grid_smoothness_bw_step(
+                1, _p(grid_pts), grid_pts.numel(), _p(self.vox_key), self.N, _p(self.coord), _p(self.size), self.res,
+                _p(self.mask), _p(self.keys), _p(self.g2v), self.G, _p(self.dense), _p(self.active), self.G,
+                ctypes.c_float(self.weight), ctypes.c_float(self.vox_size_inv), ctypes.c_uint(self.seed), _p(step),
+                _p(grid_grad), self._stream()), "svr_grid_smoothness_bw_step")
 
 
 class BatchedTrainIteration:

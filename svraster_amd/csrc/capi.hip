@@ -475,28 +475,37 @@ int svr_grid_smoothness_bw_step(int laplacian, const float* grid_pts, long long 
                                 const float* grid_voxel_coord, const float* grid_voxel_size, int grid_res,
                                 const uint8_t* grid_mask, const int32_t* grid_keys, const int32_t* grid2voxel, int G,
                                 const int32_t* dense_index, const int32_t* active_list, int A, float weight,
-                                float vox_size_inv, unsigned int seed, const int32_t* step,
-                                const int32_t* vm_keys, const int32_t* vm_vox, float* coef_ws,
-                                float* grid_pts_grad, void* stream) {
+                                float vox_size_inv, unsigned int seed, const int32_t* step, float* grid_pts_grad,
+                                void* stream) {
   g_err.clear();
-  if (!step) return fail(1, "step counter%s is required", "", 0);
-  if (!active_list) {
-    // Every entry active, in table order: the two-pass form (Laplacian only).
-    if (!laplacian || !dense_index || !vm_keys || !vm_vox || !coef_ws)
-      return fail(1, "the all-keys form%s needs laplacian, dense_index, vm_keys, vm_vox and coef_ws", "", 0);
-    if (grid_res <= 2 || (long long)grid_res * grid_res * grid_res >= (1LL << 31))
-      return fail(1, "grid_res%s %lld out of range", "", grid_res);
-    GridTable T;
-    T.grid_pts = grid_pts; T.vox_key = vox_key; T.keys = grid_keys; T.key2vox = grid2voxel;
-    T.vox_coord = grid_voxel_coord; T.vox_size = grid_voxel_size; T.n_keys = G; T.res = grid_res; T.n_vox = N;
-    T.n_pts = n_pts; T.dense = dense_index;
-    SVR_TRY(launch_grid_laplacian_pull(T, reinterpret_cast<const bool*>(grid_mask), vm_keys, vm_vox, weight,
-                                       vox_size_inv, seed, step, coef_ws, grid_pts_grad, as_stream(stream)));
-    return 0;
-  }
+  if (!step || !active_list) return fail(1, "step counter and active list%s are required", "", 0);
   return grid_loss_common(laplacian != 0, grid_pts, n_pts, vox_key, N, grid_voxel_coord, grid_voxel_size, grid_res,
                           grid_mask, grid_keys, grid2voxel, G, active_list, A, weight, vox_size_inv, seed, nullptr,
                           grid_pts_grad, stream, dense_index, step);
+}
+
+int svr_grid_laplacian_bw_table_order(const float* grid_pts, long long n_pts, const int64_t* vox_key, int N,
+                                      const float* grid_voxel_coord, const float* grid_voxel_size, int grid_res,
+                                      const uint8_t* grid_mask, const int32_t* table_keys,
+                                      const int32_t* table_tvox, int G, const int32_t* cell_entry,
+                                      const int32_t* cell_tvox, const int32_t* tvox, int T, float weight,
+                                      float vox_size_inv, unsigned int seed, const int32_t* step, float* rec_ws,
+                                      float* coef_ws, float* grid_pts_grad, void* stream) {
+  g_err.clear();
+  if (!step || !table_keys || !table_tvox || !cell_entry || !cell_tvox || !tvox || !rec_ws || !coef_ws)
+    return fail(1, "missing table-order argument%s (all pointers are required)", "", 0);
+  if (grid_res <= 2 || (long long)grid_res * grid_res * grid_res >= (1LL << 31))
+    return fail(1, "grid_res%s %lld out of range", "", grid_res);
+  if (T < 0 || T > N) return fail(1, "table voxel count%s %lld out of range", "", T);
+  GridOrderArgs A;
+  A.grid_pts = grid_pts; A.vox_key = vox_key; A.coord = grid_voxel_coord; A.size = grid_voxel_size;
+  A.mask = reinterpret_cast<const bool*>(grid_mask);
+  A.keys = table_keys; A.tv = table_tvox; A.cell_entry = cell_entry; A.cell_tv = cell_tvox; A.tvox = tvox;
+  A.rec = rec_ws; A.coef = coef_ws; A.grad = grid_pts_grad;
+  A.G = G; A.T = T; A.res = grid_res; A.weight = weight; A.inv = vox_size_inv; A.seed = seed; A.step = step;
+  (void)n_pts;
+  SVR_TRY(launch_grid_laplacian_pull(A, as_stream(stream)));
+  return 0;
 }
 
 int svr_ijk_2_octpath(const int64_t* ijk, const int8_t* octlevel, int n, int64_t* octpath, void* stream) {

@@ -174,80 +174,232 @@ __global__ __launch_bounds__(256) void grid_laplacian_kernel(GridTable G, const 
   grid_scatter(G, vox[6], w[6], lz, grad);
 }
 
-// Two-pass ("pull") form of the same loss for an active list that is every table entry in
-// order, over a table whose cells are unique.  Pass 1 stores each key's coefficients (zero for
-// keys that fail the mask test).  Pass 2 walks the table voxel by voxel: the samples that land
-// in cell c come from key c itself (centre sample) and from its six face neighbours (their
-// sample toward c), all inside c's voxel, so a lane accumulates its cell's seven contributions
-// into that voxel's 8 corners; lanes of the same voxel are summed across the wavefront and one
-// lane per voxel run adds the 8 sums to the gradient.  Same terms as the per-key scatter (the
-// same products of coefficient and weight), summed in another order.
-__global__ __launch_bounds__(256) void grid_laplacian_coef_kernel(GridTable G, const bool* __restrict__ mask,
+// Two-pass ("pull") form of the same loss with every table entry active once, taken in table
+// order (voxel by voxel; entry e of a voxel with cell box lo..lo+s is cell
+// lo + (i % s, i / s % s, i / s^2), i = e - first entry of the voxel, x fastest).  It needs a
+// table whose voxels' entries are exactly their cell boxes in that order with power-of-two
+// sizes, whose cells are unique, and whose mask is its own (true exactly on its cells).  The
+// random offset of the key at entry e is drawn with index e.
+//
+// The table's voxels are numbered compactly (t = 0..T-1, tvox[t] = voxel id); per step, a record
+// per table voxel holds its box origin and size and its 8 corner SDFs (12 floats), so a sample
+// costs one cell -> voxel lookup and three 16-byte loads.  Both passes compute every lookup
+// address first and issue the loads together (no load waits on a branch), since the time is
+// in load latency, not arithmetic.
+//
+// Pass 1, one lane per entry: the key's coefficients (zero when it fails the mask test).
+// Samples inside the lane's own voxel box use its own record, and such neighbours are in the
+// mask by construction; samples across a face read the mask and their voxel's record.
+// Pass 2, one lane per entry: the samples that land in cell c come from key c (centre sample)
+// and from its six face neighbours (their sample toward c), all inside c's voxel, so the lane
+// sums the seven weighted contributions into that voxel's 8 corners; lanes of one voxel are
+// summed across the wavefront and one lane per voxel run adds the 8 sums to the gradient.  Same
+// terms as the per-key scatter (the same products of coefficient and weight), summed in another
+// order.
+struct VoxelBox {
+  int lx, ly, lz, s;
+  float cx, cy, cz, inv;
+};
+
+// Power-of-two sizes: 1 / size is exact and t / size == t * (1 / size).
+__device__ inline VoxelBox record_box(float4 r) {
+  VoxelBox b;
+  b.cx = r.x; b.cy = r.y; b.cz = r.z;
+  b.lx = (int)r.x; b.ly = (int)r.y; b.lz = (int)r.z; b.s = (int)r.w;
+  b.inv = __builtin_amdgcn_rcpf(r.w);
+  return b;
+}
+
+__device__ inline bool in_box(const VoxelBox& b, int x, int y, int z) {
+  return x >= b.lx && x < b.lx + b.s && y >= b.ly && y < b.ly + b.s && z >= b.lz && z < b.lz + b.s;
+}
+
+// Local coordinates of cell (x, y, z) + r in the voxel box (the reference divides by the size).
+__device__ inline void box_local(const VoxelBox& b, int x, int y, int z, float rx, float ry, float rz, float& qx,
+                                 float& qy, float& qz) {
+  qx = ((float)x + rx - b.cx) * b.inv;
+  qy = ((float)y + ry - b.cy) * b.inv;
+  qz = ((float)z + rz - b.cz) * b.inv;
+}
+
+// SDF of the voxel (box b, corner SDFs g0, g1) at cell (x, y, z) + r: clamped trilinear weights
+// and the weighted corner sum, as grid_sample computes them.
+__device__ inline float record_sample(const VoxelBox& b, float4 g0, float4 g1, int x, int y, int z, float rx,
+                                      float ry, float rz) {
+  float qx, qy, qz, w[8];
+  box_local(b, x, y, z, rx, ry, rz, qx, qy, qz);
+  trilinear_weights(qx, qy, qz, w);
+  const float g[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+  float sdf = 0.f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) sdf += g[i] * fminf(fmaxf(w[i], 0.f), 1.f);
+  return sdf;
+}
+
+__global__ __launch_bounds__(256) void grid_voxel_records_kernel(int T, const int32_t* __restrict__ tvox,
+                                                                 const float* __restrict__ coord,
+                                                                 const float* __restrict__ size,
+                                                                 const int64_t* __restrict__ vox_key,
+                                                                 const float* __restrict__ grid_pts,
+                                                                 float4* __restrict__ rec) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  const long long v = tvox[t];
+  float g[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) g[i] = grid_pts[vox_key[v * 8 + i]];
+  rec[3LL * t] = make_float4(coord[3 * v], coord[3 * v + 1], coord[3 * v + 2], size[v]);
+  rec[3LL * t + 1] = make_float4(g[0], g[1], g[2], g[3]);
+  rec[3LL * t + 2] = make_float4(g[4], g[5], g[6], g[7]);
+}
+
+struct TableOrder {
+  const int32_t* keys;       // [G] cell of each entry
+  const int32_t* tv;         // [G] compact table voxel of each entry
+  const int32_t* cell_entry; // [res^3] entry of each cell, -1 if none
+  const int32_t* cell_tv;    // [res^3] compact table voxel of each cell, -1 if none
+  const int32_t* tvox;       // [T] voxel id of each compact table voxel
+  const float4* rec;         // [T][3] box + corner SDF records
+  int G, T, res;
+};
+
+__global__ __launch_bounds__(256) void grid_laplacian_coef_kernel(TableOrder O, const bool* __restrict__ mask,
                                                                   float weight, float inv, uint32_t seed,
                                                                   const int32_t* __restrict__ step,
                                                                   float* __restrict__ coef) {
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= G.n_keys) return;
-  float w[7][8], lx = 0.f, ly = 0.f, lz = 0.f;
-  int vox[7];
-  if (!laplacian_key(G, mask, G.keys[tid], tid, step_seed(seed, step), nullptr, weight, inv, w, vox, lx, ly, lz)) {
-    lx = ly = lz = 0.f;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = O.G, res = O.res;
+  if (e >= n) return;
+  const int c = O.keys[e];
+  const int t = O.tv[e];
+  const int x = c % res, y = (c / res) % res, z = c / (res * res);
+  bool valid = t >= 0 && t < O.T && x > 0 && x < res - 1 && y > 0 && y < res - 1 && z > 0 && z < res - 1;
+  const int ts = valid ? t : 0;
+  const float4 r0 = O.rec[3LL * ts], g0 = O.rec[3LL * ts + 1], g1 = O.rec[3LL * ts + 2];
+  const VoxelBox b = record_box(r0);
+  const int dx[7] = {0, 1, -1, 0, 0, 0, 0}, dy[7] = {0, 0, 0, 1, -1, 0, 0}, dz[7] = {0, 0, 0, 0, 0, 1, -1};
+  // neighbour cells (the lane's own cell when the key is invalid, so every address is in range)
+  int cell[7], tq[7];
+  bool ob[7];
+  cell[0] = c; ob[0] = false;
+#pragma unroll
+  for (int i = 1; i < 7; ++i) {
+    const int qx = x + dx[i], qy = y + dy[i], qz = z + dz[i];
+    ob[i] = valid && !in_box(b, qx, qy, qz);
+    cell[i] = valid ? qx + qy * res + qz * res * res : c;
   }
-  coef[tid] = lx;
-  coef[(long long)G.n_keys + tid] = ly;
-  coef[2LL * G.n_keys + tid] = lz;
+  bool m[7];
+#pragma unroll
+  for (int i = 1; i < 7; ++i) {
+    m[i] = mask[cell[i]];
+    tq[i] = O.cell_tv[cell[i]];
+  }
+#pragma unroll
+  for (int i = 1; i < 7; ++i) {
+    valid = valid && (!ob[i] || m[i]);
+    tq[i] = ob[i] ? tq[i] : ts;
+  }
+  float4 qr[7], qg0[7], qg1[7];
+#pragma unroll
+  for (int i = 1; i < 7; ++i) {
+    const int tt = tq[i] >= 0 && tq[i] < O.T ? tq[i] : ts;
+    qr[i] = O.rec[3LL * tt];
+    qg0[i] = O.rec[3LL * tt + 1];
+    qg1[i] = O.rec[3LL * tt + 2];
+  }
+  float lx = 0.f, ly = 0.f, lz = 0.f;
+  if (valid) {
+    float rx, ry, rz;
+    random_offset(nullptr, step_seed(seed, step), e, rx, ry, rz);
+    float sd[7];
+    sd[0] = record_sample(b, g0, g1, x, y, z, rx, ry, rz);
+#pragma unroll
+    for (int i = 1; i < 7; ++i) {
+      const int qx = x + dx[i], qy = y + dy[i], qz = z + dz[i];
+      if (tq[i] < 0 || tq[i] >= O.T) sd[i] = 0.f;
+      else if (!ob[i]) sd[i] = record_sample(b, g0, g1, qx, qy, qz, rx, ry, rz);
+      else sd[i] = record_sample(record_box(qr[i]), qg0[i], qg1[i], qx, qy, qz, rx, ry, rz);
+    }
+    const float r0s = sd[1] + sd[2] - 2.f * sd[0];
+    const float r1s = sd[3] + sd[4] - 2.f * sd[0];
+    const float r2s = sd[5] + sd[6] - 2.f * sd[0];
+    const float lr = sqrtf(r0s * r0s + r1s * r1s + r2s * r2s + 1e-10f);
+    const float sz = weight * 0.5f * inv * inv;
+    lx = r0s / lr * sz;
+    ly = r1s / lr * sz;
+    lz = r2s / lr * sz;
+  }
+  coef[e] = lx;
+  coef[(long long)n + e] = ly;
+  coef[2LL * n + e] = lz;
 }
 
-__global__ __launch_bounds__(256) void grid_laplacian_pull_kernel(GridTable G, const int32_t* __restrict__ vm_keys,
-                                                                  const int32_t* __restrict__ vm_vox,
+__global__ __launch_bounds__(256) void grid_laplacian_pull_kernel(TableOrder O, const int64_t* __restrict__ vox_key,
                                                                   uint32_t seed, const int32_t* __restrict__ step,
                                                                   const float* __restrict__ coef,
                                                                   float* __restrict__ grad) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const int n = G.n_keys, res = G.res;
+  const int n = O.G, res = O.res;
   const long long n3 = (long long)n;
-  int v = -1;
+  const bool live = e < n;
+  const int es = live ? e : 0;
+  const int c = O.keys[es];
+  const int t0 = O.tv[es];
+  const int t = live && t0 >= 0 && t0 < O.T ? t0 : -1;
+  const float4 r0 = O.rec[3LL * (t >= 0 ? t : 0)];
+  const VoxelBox b = record_box(r0);
+  const int x = c % res, y = (c / res) % res, z = c / (res * res);
+  // the cell, then the keys whose +x, -x, +y, -y, +z, -z sample lands in it
+  const int kx[7] = {0, -1, 1, 0, 0, 0, 0}, ky[7] = {0, 0, 0, -1, 1, 0, 0}, kz[7] = {0, 0, 0, 0, 0, -1, 1};
+  int ek[7];
+  bool use[7];
+  int cell[7];
+#pragma unroll
+  for (int i = 0; i < 7; ++i) {
+    const int ax = x + kx[i], ay = y + ky[i], az = z + kz[i];
+    const bool ingrid = ax >= 0 && ax < res && ay >= 0 && ay < res && az >= 0 && az < res;
+    const bool inb = in_box(b, ax, ay, az);
+    use[i] = t >= 0 && ingrid;
+    ek[i] = inb ? es + (kz[i] * b.s + ky[i]) * b.s + kx[i] : -1;
+    cell[i] = (!inb && ingrid) ? ax + ay * res + az * res * res : c;
+  }
+  int looked[7];
+#pragma unroll
+  for (int i = 0; i < 7; ++i) looked[i] = O.cell_entry[cell[i]];
+#pragma unroll
+  for (int i = 0; i < 7; ++i) {
+    if (ek[i] < 0) ek[i] = looked[i];
+    use[i] = use[i] && ek[i] >= 0;
+    if (!use[i]) ek[i] = es;
+  }
+  float l[7];
+  {
+    const float cx0 = coef[ek[0]], cy0 = coef[n3 + ek[0]], cz0 = coef[2 * n3 + ek[0]];
+    l[0] = -2.f * (cx0 + cy0 + cz0);
+    l[1] = coef[ek[1]];
+    l[2] = coef[ek[2]];
+    l[3] = coef[n3 + ek[3]];
+    l[4] = coef[n3 + ek[4]];
+    l[5] = coef[2 * n3 + ek[5]];
+    l[6] = coef[2 * n3 + ek[6]];
+  }
   float acc[8];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) acc[i] = 0.f;
-  if (e < n) {
-    const int c = vm_keys[e];
-    const int vv = vm_vox[e];
-    const int j = G.dense[c];
-    if (j >= 0 && vv >= 0 && vv < G.n_vox && G.key2vox[j] == vv) v = vv;
-    if (v >= 0) {
-      const uint32_t sd = step_seed(seed, step);
-      const int x = c % res, y = (c / res) % res, z = c / (res * res);
-      const float cx = G.vox_coord[3 * v], cy = G.vox_coord[3 * v + 1], cz = G.vox_coord[3 * v + 2];
-      const float size = G.vox_size[v];
-      // key offsets of the seven contributors: the cell, then the keys whose +x, -x, +y, -y, +z, -z
-      // sample lands in it (at -x, +x, -y, +y, -z, +z of the cell)
-      const int kx[7] = {0, -1, 1, 0, 0, 0, 0}, ky[7] = {0, 0, 0, -1, 1, 0, 0}, kz[7] = {0, 0, 0, 0, 0, -1, 1};
+  for (int q = 0; q < 8; ++q) acc[q] = 0.f;
+  const uint32_t sd = step_seed(seed, step);
 #pragma unroll
-      for (int i = 0; i < 7; ++i) {
-        const int ax = x + kx[i], ay = y + ky[i], az = z + kz[i];
-        if (ax < 0 || ax >= res || ay < 0 || ay >= res || az < 0 || az >= res) continue;
-        const int jk = G.dense[ax + ay * res + az * res * res];
-        if (jk < 0) continue;
-        float l;
-        if (i == 0) l = -2.f * (coef[jk] + coef[n3 + jk] + coef[2 * n3 + jk]);
-        else if (i <= 2) l = coef[jk];
-        else if (i <= 4) l = coef[n3 + jk];
-        else l = coef[2 * n3 + jk];
-        if (l == 0.f) continue;
-        float rx, ry, rz;
-        random_offset(nullptr, sd, jk, rx, ry, rz);
-        float w[8];
-        trilinear_weights(((float)x + rx - cx) / size, ((float)y + ry - cy) / size, ((float)z + rz - cz) / size, w);
+  for (int i = 0; i < 7; ++i) {
+    if (!use[i] || l[i] == 0.f) continue;
+    float rx, ry, rz;
+    random_offset(nullptr, sd, ek[i], rx, ry, rz);
+    float w[8], qx, qy, qz;
+    box_local(b, x, y, z, rx, ry, rz, qx, qy, qz);
+    trilinear_weights(qx, qy, qz, w);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const float wq = fminf(fmaxf(w[q], 0.f), 1.f);
-          acc[q] += l * wq;
-        }
-      }
-    }
+    for (int q = 0; q < 8; ++q) acc[q] += l[i] * fminf(fmaxf(w[q], 0.f), 1.f);
   }
+  const int v = t >= 0 ? t : -1;
   const int prev = __shfl_up(v, 1);
   const int next = __shfl_down(v, 1);
   const bool head = lane == 0 || prev != v;
@@ -255,9 +407,10 @@ __global__ __launch_bounds__(256) void grid_laplacian_pull_kernel(GridTable G, c
   wave_segmented_sum<8>(acc, lane, seg_start);
   const bool tail = lane == 63 || next != v;
   if (tail && v >= 0) {
+    const long long vid = O.tvox[v];
 #pragma unroll
     for (int q = 0; q < 8; ++q)
-      if (acc[q] != 0.f) atomicAdd(grad + G.vox_key[(long long)v * 8 + q], acc[q]);
+      if (acc[q] != 0.f) atomicAdd(grad + vox_key[vid * 8 + q], acc[q]);
   }
 }
 
@@ -336,13 +489,17 @@ hipError_t launch_grid_loss(bool laplacian, const GridTable& G, const bool* mask
   return hipGetLastError();
 }
 
-hipError_t launch_grid_laplacian_pull(const GridTable& G, const bool* mask, const int32_t* vm_keys,
-                                      const int32_t* vm_vox, float weight, float inv, uint32_t seed,
-                                      const int32_t* step, float* coef, float* grad, hipStream_t st) {
-  if (G.n_keys <= 0) return hipSuccess;
-  const unsigned nb = (unsigned)((G.n_keys + 255) / 256);
-  grid_laplacian_coef_kernel<<<nb, 256, 0, st>>>(G, mask, weight, inv, seed, step, coef);
-  grid_laplacian_pull_kernel<<<nb, 256, 0, st>>>(G, vm_keys, vm_vox, seed, step, coef, grad);
+hipError_t launch_grid_laplacian_pull(const GridOrderArgs& A, hipStream_t st) {
+  if (A.G <= 0) return hipSuccess;
+  if (A.T > 0)
+    grid_voxel_records_kernel<<<(unsigned)((A.T + 255) / 256), 256, 0, st>>>(A.T, A.tvox, A.coord, A.size, A.vox_key,
+                                                                             A.grid_pts, reinterpret_cast<float4*>(A.rec));
+  TableOrder O;
+  O.keys = A.keys; O.tv = A.tv; O.cell_entry = A.cell_entry; O.cell_tv = A.cell_tv; O.tvox = A.tvox;
+  O.rec = reinterpret_cast<const float4*>(A.rec); O.G = A.G; O.T = A.T; O.res = A.res;
+  const unsigned nb = (unsigned)((A.G + 255) / 256);
+  grid_laplacian_coef_kernel<<<nb, 256, 0, st>>>(O, A.mask, A.weight, A.inv, A.seed, A.step, A.coef);
+  grid_laplacian_pull_kernel<<<nb, 256, 0, st>>>(O, A.vox_key, A.seed, A.step, A.coef, A.grad);
   return hipGetLastError();
 }
 

@@ -53,9 +53,24 @@ struct GridTable {
 hipError_t launch_grid_loss(bool laplacian, const GridTable& G, const bool* mask, const int32_t* active, int A,
                             float weight, float inv, uint32_t seed, const int32_t* step, const float* randv,
                             float* grad, hipStream_t st);
-hipError_t launch_grid_laplacian_pull(const GridTable& G, const bool* mask, const int32_t* vm_keys,
-                                      const int32_t* vm_vox, float weight, float inv, uint32_t seed,
-                                      const int32_t* step, float* coef, float* grad, hipStream_t st);
+// Table-order Laplacian (grid_loss.hip): every table entry active once, voxel by voxel.
+struct GridOrderArgs {
+  const float* grid_pts; const int64_t* vox_key; const float* coord; const float* size;
+  const bool* mask;
+  const int32_t* keys;        // [G] cell of each entry (table order, x fastest inside a voxel)
+  const int32_t* tv;          // [G] compact table voxel of each entry
+  const int32_t* cell_entry;  // [res^3] entry of each cell, -1 if none
+  const int32_t* cell_tv;     // [res^3] compact table voxel of each cell, -1 if none
+  const int32_t* tvox;        // [T] voxel id of each compact table voxel
+  float* rec;                 // [T * 12] per-step records (scratch)
+  float* coef;                // [3 * G] per-step coefficients (scratch)
+  float* grad;
+  int G, T, res;
+  float weight, inv;
+  uint32_t seed;
+  const int32_t* step;
+};
+hipError_t launch_grid_laplacian_pull(const GridOrderArgs& A, hipStream_t st);
 hipError_t launch_grid_dense_index(long long n_cells, const int32_t* keys, int G, int32_t* out, hipStream_t st);
 hipError_t launch_table_count(int N, const float* center, const float* size, const float* scene_center,
                               float extent, int res, const bool* is_leaf, uint32_t* count, hipStream_t st);
