@@ -134,10 +134,12 @@ __global__ __launch_bounds__(256) void gather_geo_bw_kernel(int n, const int64_t
 
 // Batched views: colour of every (view, visible voxel) pair, direction from that view's camera.
 // One voxel per thread, looping over the views, so each voxel's coefficients are read once.
-// The SH gradient rows of a block of 256 voxels are contiguous ([P, M-1, 3]); the backward
-// writes them through LDS so that the global stores are coalesced.  kShRowMax = 45 floats holds a
-// degree-3 row (M <= 16); the odd stride keeps the per-thread LDS writes free of bank conflicts.
+// The SH gradient rows of a block of 256 voxels are contiguous ([P, M-1, 3], at most 45 floats
+// for M <= 16); the backward passes them through LDS in three chunks of kShChunk = 15 floats per
+// row so that the global accesses are coalesced while the block holds only 15 KB of LDS (the odd
+// stride keeps the per-thread LDS writes free of bank conflicts).
 constexpr int kShRowMax = 45;
+constexpr int kShChunk = 15;
 
 // Bit k set when view v0 + k sees voxel i (non-empty octant set, i.e. n_dup > 0); the loads are
 // independent, so they are all in flight at once instead of one dependent load per view.
@@ -159,16 +161,25 @@ __global__ __launch_bounds__(256) void sh_views_forward_kernel(int P, int V, con
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P) return;
   float c[NB][3];
-  c[0][0] = sh0[3 * i]; c[0][1] = sh0[3 * i + 1]; c[0][2] = sh0[3 * i + 2];
-  const float* cs = shs + (long long)i * (M - 1) * 3;
-#pragma unroll
-  for (int k = 1; k < NB; ++k) {
-    c[k][0] = cs[3 * (k - 1)]; c[k][1] = cs[3 * (k - 1) + 1]; c[k][2] = cs[3 * (k - 1) + 2];
-  }
+  bool loaded = false;
   for (int v0 = 0; v0 < V; v0 += 32) {
-    const uint32_t vis = visible_views(quads, P, i, v0, min(32, V - v0));
-    for (uint32_t m = vis; m; m &= m - 1u) {
-      const int v = v0 + __builtin_ctz(m);
+    const int nvw = min(32, V - v0);
+    const uint32_t vis = visible_views(quads, P, i, v0, nvw);
+    if (vis && !loaded) {          // coefficients only for voxels some view sees
+      c[0][0] = sh0[3 * i]; c[0][1] = sh0[3 * i + 1]; c[0][2] = sh0[3 * i + 2];
+      const float* cs = shs + (long long)i * (M - 1) * 3;
+#pragma unroll
+      for (int k = 1; k < NB; ++k) {
+        c[k][0] = cs[3 * (k - 1)]; c[k][1] = cs[3 * (k - 1) + 1]; c[k][2] = cs[3 * (k - 1) + 2];
+      }
+      loaded = true;
+    }
+    // all lanes on the same view at once: coalesced writes into the [V, P, 3] colours
+    for (int k = 0; k < nvw; ++k) {
+      const bool on = (vis >> k) & 1u;
+      if (!__any(on)) continue;
+      if (!on) continue;
+      const int v = v0 + k;
       const long long o = (long long)v * P + i;
       const float* c2w = views[v].c2w;
       const float cam[3] = {c2w[3], c2w[7], c2w[11]};
@@ -178,8 +189,8 @@ __global__ __launch_bounds__(256) void sh_views_forward_kernel(int P, int V, con
       sh_basis(kDeg, x, y, z, b);
       float r0 = 0.f, r1 = 0.f, r2 = 0.f;
 #pragma unroll
-      for (int k = 0; k < NB; ++k) {
-        r0 += b[k] * c[k][0]; r1 += b[k] * c[k][1]; r2 += b[k] * c[k][2];
+      for (int kk = 0; kk < NB; ++kk) {
+        r0 += b[kk] * c[kk][0]; r1 += b[kk] * c[kk][1]; r2 += b[kk] * c[kk][2];
       }
       rgbs_views[3 * o] = fmaxf(r0 + 0.5f, 0.f);
       rgbs_views[3 * o + 1] = fmaxf(r1 + 0.5f, 0.f);
@@ -228,7 +239,7 @@ __global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, co
                                                                 float* __restrict__ dsh0,
                                                                 float* __restrict__ dshs, ShAdam A) {
   constexpr int NB = (kDeg + 1) * (kDeg + 1);
-  __shared__ float s_rows[256 * kShRowMax];
+  __shared__ float s_rows[256 * kShChunk];
   const int t = threadIdx.x;
   const long long i0 = (long long)blockIdx.x * 256;
   const int nv = (int)min(256LL, (long long)P - i0);
@@ -238,17 +249,24 @@ __global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, co
     h0 = adam_table_entry(A.t0);
     hs = adam_table_entry(A.ts);
   }
+  float acc[NB][3];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.f;
   if (t < nv) {
     const int i = (int)i0 + t;
-    float acc[NB][3];
-#pragma unroll
-    for (int k = 0; k < NB; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.f;
     for (int v0 = 0; v0 < V; v0 += 32) {
-      const uint32_t vis = visible_views(quads, P, i, v0, min(32, V - v0));
-      for (uint32_t m = vis; m; m &= m - 1u) {
-        const int v = v0 + __builtin_ctz(m);
+      const int nvw = min(32, V - v0);
+      const uint32_t vis = visible_views(quads, P, i, v0, nvw);
+      // all lanes on the same view at once, so the per-view gradient rows are read and cleared
+      // with coalesced accesses ([V, P, 3] layout)
+      for (int k = 0; k < nvw; ++k) {
+        const bool on = (vis >> k) & 1u;
+        if (!__any(on)) continue;
+        if (!on) continue;
+        const int v = v0 + k;
         const long long o = (long long)v * P + i;
         const float g0 = drgb_views[3 * o], g1 = drgb_views[3 * o + 1], g2 = drgb_views[3 * o + 2];
+        if (g0 == 0.f && g1 == 0.f && g2 == 0.f) continue;     // no contribution in this view
         drgb_views[3 * o] = 0.f;
         drgb_views[3 * o + 1] = 0.f;
         drgb_views[3 * o + 2] = 0.f;
@@ -259,8 +277,8 @@ __global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, co
         float b[16];
         sh_basis(kDeg, x, y, z, b);
 #pragma unroll
-        for (int k = 0; k < NB; ++k) {
-          acc[k][0] += b[k] * g0; acc[k][1] += b[k] * g1; acc[k][2] += b[k] * g2;
+        for (int kk = 0; kk < NB; ++kk) {
+          acc[kk][0] += b[kk] * g0; acc[kk][1] += b[kk] * g1; acc[kk][2] += b[kk] * g2;
         }
       }
     }
@@ -278,26 +296,39 @@ __global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, co
     } else {
       dsh0[3 * i] = acc[0][0]; dsh0[3 * i + 1] = acc[0][1]; dsh0[3 * i + 2] = acc[0][2];
     }
-    float* c = s_rows + t * kShRowMax;
-#pragma unroll
-    for (int k = 1; k < NB; ++k) {
-      c[3 * (k - 1)] = acc[k][0]; c[3 * (k - 1) + 1] = acc[k][1]; c[3 * (k - 1) + 2] = acc[k][2];
-    }
-    for (int k = 3 * (NB - 1); k < row; ++k) c[k] = 0.f;
   }
-  __syncthreads();
-  if (kAdam) {
-    for (int k = t; k < nv * row; k += 256) {
-      const float g = s_rows[(k / row) * kShRowMax + (k % row)];
-      if (g == 0.0f) continue;
-      const long long e = i0 * row + k;
-      float pv = A.shs[e], mv = A.ms[e], vv = A.vs[e];
-      adam_update<false>(pv, mv, vv, g, hs.x, A.ts.b1, A.ts.b2, hs.y, A.ts.eps);
-      A.shs[e] = pv; A.ms[e] = mv; A.vs[e] = vv;
+  // Rows through LDS, 15 floats of each row at a time (element q of a row is coefficient
+  // 1 + q / 3, channel q % 3; elements past the active degree are zero).
+#pragma unroll
+  for (int q0 = 0; q0 < kShRowMax; q0 += kShChunk) {
+    if (q0 >= row) break;
+    const int qn = min(kShChunk, row - q0);
+    if (t < nv) {
+      float* c = s_rows + t * kShChunk;
+#pragma unroll
+      for (int j = 0; j < kShChunk; ++j) {
+        const int q = q0 + j;
+        c[j] = (1 + q / 3 < NB) ? acc[1 + q / 3 < NB ? 1 + q / 3 : 0][q % 3] : 0.f;
+      }
     }
-  } else {
-    float* dst = dshs + i0 * row;
-    for (int k = t; k < nv * row; k += 256) dst[k] = s_rows[(k / row) * kShRowMax + (k % row)];
+    __syncthreads();
+    if (kAdam) {
+      for (int k = t; k < nv * qn; k += 256) {
+        const int r = k / qn, col = k - r * qn;
+        const float g = s_rows[r * kShChunk + col];
+        if (g == 0.0f) continue;
+        const long long e = (i0 + r) * row + q0 + col;
+        float pv = A.shs[e], mv = A.ms[e], vv = A.vs[e];
+        adam_update<false>(pv, mv, vv, g, hs.x, A.ts.b1, A.ts.b2, hs.y, A.ts.eps);
+        A.shs[e] = pv; A.ms[e] = mv; A.vs[e] = vv;
+      }
+    } else {
+      for (int k = t; k < nv * qn; k += 256) {
+        const int r = k / qn, col = k - r * qn;
+        dshs[(i0 + r) * row + q0 + col] = s_rows[r * kShChunk + col];
+      }
+    }
+    __syncthreads();
   }
 }
 
