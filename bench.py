@@ -21,7 +21,27 @@ import time
 
 METRIC = "fwd+bwd Mrays/s on 1 MI355X, batched vs per-view, at ≤1e-4 img err; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-BWD_KERNEL = "render_bwd_pairs_kernel<false, false, false, true>"
+
+
+def bwd_kernel_name():
+    """Name of the multi-view backward compositor instantiation the current options launch."""
+    from svraster_amd import _lib
+    halves = _lib.get_option("batch_depth") // 64
+    return f"render_bwd_pairs_kernel<false, false, false, true, {halves}>"
+
+
+def tune(make_step, sig, enabled):
+    """Autotuned launch configuration of the batched path (svraster_amd/autotune.py)."""
+    from svraster_amd import autotune
+    if not enabled:
+        autotune.apply_options(autotune.DEFAULT)
+        return dict(autotune.DEFAULT), {"enabled": False}
+    best, table = autotune.autotune(make_step, sig, steps=3, warmup=1, use_cache=False, log=log)
+    rows = [{"config": {k: (list(v) if isinstance(v, tuple) else v) for k, v in c.items()}, "ms": round(ms, 3)}
+            for c, ms in table]
+    chosen = {k: (list(v) if isinstance(v, tuple) else v) for k, v in best.items()}
+    return best, {"enabled": True, "chosen": chosen, "search": "coordinate descent, eager step median of 3",
+                  "table": rows}
 
 
 def log(*a):
@@ -238,9 +258,21 @@ def bench_train(args):
     torch.cuda.empty_cache()
 
     # -- batched iteration, one captured graph ----------------------------------------------------
+    from svraster_amd import autotune
+    from svraster_amd.batched import BatchedTrainStep
+
+    def make_step(shape):
+        r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key,
+                              cams, model.bg_color, sc.sh_degree, tile_shape=shape)
+        return BatchedTrainStep(r, model._geo_grid_pts.detach(), model._sh0.detach(), model._shs.detach(),
+                                model._log_s.detach(), targets)
+
+    best, tune_info = tune(make_step, autotune.workload_signature(sc.num_voxels, cams, sc.sh_degree),
+                           not args.no_autotune)
+    torch.cuda.empty_cache()
     t0 = time.perf_counter()
     rast = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key,
-                             cams, model.bg_color, sc.sh_degree)
+                             cams, model.bg_color, sc.sh_degree, tile_shape=best["tile_shape"])
     it = BatchedTrainIteration(rast, model._geo_grid_pts.detach(), model._sh0.detach(), model._shs.detach(),
                                model._log_s.detach(), targets, smooth, lr=lr, betas=betas, eps=eps)
     it()
@@ -278,9 +310,10 @@ def bench_train(args):
         "stages_ms": {"batched_eager": st_b, "per_view": st_pv},
         "stages_algorithmic_bytes": {k: int(v) for k, v in nbytes.items()},
         "stages_gbs_batched": stage_gbs,
+        "autotune": tune_info,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": BWD_KERNEL + " (multi-view work queue: all views in one launch)",
+                     "kernel": bwd_kernel_name() + " (multi-view work queue: all views in one launch)",
                      "bytes_per_launch": int(bytes_launch), "counts": counts,
                      "avg_launch_ms": round(avg_launch_s * 1000.0, 4)},
     }
@@ -295,6 +328,7 @@ def main():
     ap.add_argument("--workload", default="multi_32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch configuration, no tuning")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -353,12 +387,21 @@ def main():
     ref_grid = model._geo_grid_pts.grad.clone()
     ref_sh0 = model._sh0.grad.clone()
 
-    # -- batched persistent path, captured as a graph ------------------------------------------
+    # -- batched persistent path, autotuned, captured as a graph --------------------------------
     sc = w.scene
-    rast = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key,
-                             cams, model.bg_color, sc.sh_degree)
-    bstep = BatchedTrainStep(rast, model._geo_grid_pts.detach(), model._sh0.detach(), model._shs.detach(),
-                             model._log_s.detach(), targets)
+    from svraster_amd import autotune
+
+    def make_step(shape):
+        r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key,
+                              cams, model.bg_color, sc.sh_degree, tile_shape=shape)
+        return BatchedTrainStep(r, model._geo_grid_pts.detach(), model._sh0.detach(), model._shs.detach(),
+                                model._log_s.detach(), targets)
+
+    best, tune_info = tune(make_step, autotune.workload_signature(sc.num_voxels, cams, sc.sh_degree),
+                           not args.no_autotune)
+    torch.cuda.empty_cache()
+    bstep = make_step(best["tile_shape"])
+    rast = bstep.r
     bstep()
     sync()
     K_used = rast.key_count()
@@ -383,6 +426,7 @@ def main():
     bwd_ms, bwd_n = kt_b["render_bwd"]
     avg_launch_s = bwd_ms / max(bwd_n, 1) / 1000.0
     achieved = bytes_launch / avg_launch_s / 1e9
+    BWD_KERNEL = bwd_kernel_name()
     traffic, _ = traffic_from_profiles(BWD_KERNEL)
 
     value = rays / (ms_b / 1000.0) / 1e6
@@ -417,6 +461,7 @@ def main():
                      "kernel": BWD_KERNEL + " (multi-view work queue: all views in one launch)",
                      "bytes_per_launch": int(bytes_launch), "counts": counts,
                      "avg_launch_ms": round(avg_launch_s * 1000.0, 4)},
+        "autotune": tune_info,
         "kernels_ms_per_step": {"batched_render_fwd": round(kt_b["render_fwd"][0], 3),
                                 "batched_render_bwd": round(kt_b["render_bwd"][0], 3),
                                 "per_view_render_fwd": round(kt_pv["render_fwd"][0], 3),

@@ -33,9 +33,12 @@ int svr_abi_version(void);
 void svr_timing_enable(int on);
 int svr_timing_read(int kind, int reset, double* total_ms, long long* launches);
 
-/* Runtime options (no counterpart in the reference).  "compositor_variant": 0 = pair-compacted
- * compositor kernels (default), 1 = one-entry-at-a-time kernels; both give the same images.
- * svr_get_option returns -1 for an unknown name. */
+/* Runtime options (no counterpart in the reference), the knobs the autotuner
+ * (svraster_amd/autotune.py) sets.  "compositor_variant": 0 = pair-compacted compositor kernels
+ * (default), 1 = one-entry-at-a-time kernels.  "batch_depth": entries staged in LDS per round by
+ * the multi-view pair-compacted compositors, 64 or 128 (default).  "persistent_blocks_per_cu":
+ * workgroups per CU of the multi-view persistent launches, 1..32 (default 8).  None of them
+ * changes an image or a gradient.  svr_get_option returns -1 for an unknown name. */
 int svr_set_option(const char* name, int value);
 int svr_get_option(const char* name);
 
@@ -221,12 +224,15 @@ int svr_valid_gradient_table_fill(const float* vox_center, const float* vox_size
  * Per-view colours: rgbs_views [V, P, 3].  No call below synchronises except
  * svr_batch_key_count; keys beyond K_cap are dropped (check with svr_batch_key_count).
  * svr_batch_setup: host arrays W, H, tanfovx, tanfovy, cx, cy, near_plane [V]; device
- * w2c, c2w [V,16] and background [3]. */
+ * w2c, c2w [V,16] and background [3]; tile shape tile_w x tile_h = 16x16 (the reference's, and
+ * the only shape whose per-tile lists match the per-view path's), 32x8 or 8x32 (same pixels
+ * per tile, four 8x8 quadrants per tile in a row or a column); n_tiles counts tiles of that
+ * shape.  Images and gradients do not depend on the shape. */
 size_t svr_batch_workspace_bytes(int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap);
 int svr_batch_setup(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                     const int* W, const int* H, const float* tanfovx, const float* tanfovy,
                     const float* cx, const float* cy, const float* near_plane, const float* w2c,
-                    const float* c2w, const float* background, void* stream);
+                    const float* c2w, const float* background, int tile_w, int tile_h, void* stream);
 /* Rank order of every (voxel, octant) pair, computed once per voxel set: order_out[8P] holds
  * voxel << 3 | octant sorted by the reference's in-tile rank octpath ^ repeat3(octant)
  * (auxiliary.h:40-59, forward.cu:749-802).  tmp: svr_batch_pair_order_bytes(P) bytes. */

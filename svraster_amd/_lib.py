@@ -51,7 +51,7 @@ _SIGS = {
     "svr_valid_gradient_table_count": (I, [P, P, P, F, I, P, I, P, ctypes.POINTER(LL), P]),
     "svr_valid_gradient_table_fill": (I, [P, P, P, F, I, P, I, P, P, P, P, P]),
     "svr_batch_workspace_bytes": (SZ, [I, I, LL, LL, LL]),
-    "svr_batch_setup": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, P, P, P, P, P]),
+    "svr_batch_setup": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, P, P, P, P, I, I, P]),
     "svr_batch_pair_order_bytes": (SZ, [I]),
     "svr_batch_pair_order": (I, [I, P, P, P, P]),
     "svr_batch_preprocess": (I, [P, I, I, LL, LL, LL, P, P, P, P, P]),

@@ -35,7 +35,8 @@ class BatchedRasterizer:
 
     def __init__(self, octpath, octlevel, vox_center, vox_size, is_leaf, vox_key, cameras: List,
                  background: torch.Tensor, sh_degree: int, n_sh_coeffs: int = 16,
-                 key_capacity: Optional[int] = None, need_depth=False, need_normal=False, cull: bool = True):
+                 key_capacity: Optional[int] = None, need_depth=False, need_normal=False, cull: bool = True,
+                 tile_shape=(16, 16)):
         self.dev = vox_center.device
         self.lib = _lib.load()
         self.V = len(cameras)
@@ -56,7 +57,9 @@ class BatchedRasterizer:
         del tmp
         self.Ws = [c.image_width for c in cameras]
         self.Hs = [c.image_height for c in cameras]
-        self.tiles = [((w + 15) // 16) * ((h + 15) // 16) for w, h in zip(self.Ws, self.Hs)]
+        self.tile_w, self.tile_h = int(tile_shape[0]), int(tile_shape[1])
+        tw, th = self.tile_w, self.tile_h
+        self.tiles = [((w + tw - 1) // tw) * ((h + th - 1) // th) for w, h in zip(self.Ws, self.Hs)]
         self.pix = [w * h for w, h in zip(self.Ws, self.Hs)]
         self.n_tiles = int(sum(self.tiles))
         self.n_pix = int(sum(self.pix))
@@ -96,7 +99,8 @@ class BatchedRasterizer:
         st = self._stream()
         h = self._host
         _lib.check(self.lib.svr_batch_setup(_p(self.ws), *self._dims(), h["W"], h["H"], h["tx"], h["ty"], h["cx"],
-                                            h["cy"], h["near"], _p(self.w2c), _p(self.c2w), _p(self.bg), st),
+                                            h["cy"], h["near"], _p(self.w2c), _p(self.c2w), _p(self.bg),
+                                            self.tile_w, self.tile_h, st),
                    "svr_batch_setup")
         f = dict(dtype=torch.float32, device=dev)
         self.geos = torch.empty(P, 8, **f)

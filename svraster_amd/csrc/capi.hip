@@ -72,8 +72,12 @@ struct LaunchTimer {
   }
 };
 
-// Runtime options (svr_set_option): compositor variant, 0 = pair-compacted, 1 = one entry at a time.
+// Runtime options (svr_set_option): compositor variant, 0 = pair-compacted, 1 = one entry at a time;
+// batch_depth = entries staged per round by the multi-view pair-compacted compositors (64 or 128);
+// persistent_blocks_per_cu = workgroups per CU of the multi-view persistent launches.
 int g_compositor_variant = 0;
+int g_batch_depth = 128;
+int g_blocks_per_cu = 8;
 
 int tile_bits_for(long long n_tiles) {
   int b = 0;
@@ -81,12 +85,14 @@ int tile_bits_for(long long n_tiles) {
   return b;
 }
 
-ViewDesc make_view(int W, int H, float tanfovx, float tanfovy, float cx, float cy, float near_plane) {
+ViewDesc make_view(int W, int H, float tanfovx, float tanfovy, float cx, float cy, float near_plane,
+                   int tile_w, int tile_h) {
   ViewDesc v;
   memset(&v, 0, sizeof(v));
   v.W = W; v.H = H;
-  v.tiles_x = ceil_div(W, kTile);
-  v.tiles_y = ceil_div(H, kTile);
+  v.tile_w = tile_w; v.tile_h = tile_h;
+  v.tiles_x = ceil_div(W, tile_w);
+  v.tiles_y = ceil_div(H, tile_h);
   v.tanfovx = tanfovx; v.tanfovy = tanfovy; v.cx = cx; v.cy = cy;
   // Focal lengths in FP32 with the reference's operation order (preprocess.cu:197-198).
   v.focal_x = (0.5f * (float)W) / tanfovx;
@@ -170,11 +176,23 @@ int svr_set_option(const char* name, int value) {
     g_compositor_variant = value;
     return 0;
   }
+  if (name && strcmp(name, "batch_depth") == 0) {
+    if (value != 64 && value != 128) return fail(1, "batch_depth must be 64 or 128%s (got %lld)", "", value);
+    g_batch_depth = value;
+    return 0;
+  }
+  if (name && strcmp(name, "persistent_blocks_per_cu") == 0) {
+    if (value < 1 || value > 32) return fail(1, "persistent_blocks_per_cu must be 1..32%s (got %lld)", "", value);
+    g_blocks_per_cu = value;
+    return 0;
+  }
   return fail(1, "unknown option %s%lld", name ? name : "(null)", 0);
 }
 
 int svr_get_option(const char* name) {
   if (name && strcmp(name, "compositor_variant") == 0) return g_compositor_variant;
+  if (name && strcmp(name, "batch_depth") == 0) return g_batch_depth;
+  if (name && strcmp(name, "persistent_blocks_per_cu") == 0) return g_blocks_per_cu;
   return -1;
 }
 
@@ -195,7 +213,7 @@ int svr_rasterize_preprocess(int W, int H, float tanfovx, float tanfovy, float c
   if (P == 0) return 0;
   hipStream_t st = as_stream(stream);
   GeomPtrs gp(geom_buffer, 1, P);
-  const ViewDesc v = make_view(W, H, tanfovx, tanfovy, cx, cy, near_plane);
+  const ViewDesc v = make_view(W, H, tanfovx, tanfovy, cx, cy, near_plane, kTile, kTile);
   SVR_TRY(launch_init_view(gp.views, v, w2c, c2w, nullptr, st));
   SVR_TRY(launch_fill_u32(gp.misc, 1, (uint32_t)kRankBits, st));
   SVR_TRY(launch_preprocess(gp.views, 1, P, vox_centers, vox_lengths, is_leaf, octree_paths, out_n_dup,
@@ -241,7 +259,7 @@ int svr_rasterize_forward(int vox_geo_mode, int density_mode, int W, int H, floa
   GeomPtrs gp(geom_buffer, 1, P);
   BinPtrs bp(binning_buffer, K);
   ImagePtrs ip(image_buffer, 1, (long long)W * H, n_tiles);
-  const ViewDesc v = make_view(W, H, tanfovx, tanfovy, cx, cy, 0.f);
+  const ViewDesc v = make_view(W, H, tanfovx, tanfovy, cx, cy, 0.f, kTile, kTile);
   SVR_TRY(launch_init_view(ip.views, v, w2c, c2w, background, st));
   const uint32_t* n_dev = gp.scan + (P - 1);
 
@@ -275,6 +293,8 @@ int svr_rasterize_forward(int vox_geo_mode, int density_mode, int W, int H, floa
   a.cbits = bp.cbits; a.k_cap = K;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;
+  a.halves = g_batch_depth / 64;
+  a.blocks_per_cu = g_blocks_per_cu;
   {
     LaunchTimer timer(0, st);
     SVR_TRY(launch_render_forward(a, st));
@@ -306,7 +326,7 @@ int svr_rasterize_backward(long long K, int vox_geo_mode, int density_mode, int 
   GeomPtrs gp(geom_buffer, 1, P);
   BinPtrs bp(binning_buffer, K);
   ImagePtrs ip(image_buffer, 1, (long long)W * H, n_tiles);
-  const ViewDesc v = make_view(W, H, tanfovx, tanfovy, cx, cy, 0.f);
+  const ViewDesc v = make_view(W, H, tanfovx, tanfovy, cx, cy, 0.f, kTile, kTile);
   SVR_TRY(launch_init_view(ip.views, v, w2c, c2w, background, st));
   BwdArgs a;
   memset(&a, 0, sizeof(a));
@@ -322,6 +342,8 @@ int svr_rasterize_backward(long long K, int vox_geo_mode, int density_mode, int 
   a.work_counter = nullptr;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;
+  a.halves = g_batch_depth / 64;
+  a.blocks_per_cu = g_blocks_per_cu;
   SVR_TRY(launch_contrib_compact(bp.cbits, K, bp.cscan, bp.cpart, bp.cpos, ip.ranges, ip.tile_last, n_tiles,
                                  ip.cranges, st));
   a.cpos = bp.cpos; a.cranges = ip.cranges;
@@ -560,15 +582,17 @@ size_t svr_batch_workspace_bytes(int n_views, int P, long long n_pixels, long lo
 int svr_batch_setup(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                     const int* W, const int* H, const float* tanfovx, const float* tanfovy,
                     const float* cx, const float* cy, const float* near_plane, const float* w2c,
-                    const float* c2w, const float* background, void* stream) {
+                    const float* c2w, const float* background, int tile_w, int tile_h, void* stream) {
   g_err.clear();
   if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
+  if (!((tile_w == 16 && tile_h == 16) || (tile_w == 32 && tile_h == 8) || (tile_w == 8 && tile_h == 32)))
+    return fail(1, "tile shape must be 16x16, 32x8 or 8x32%s (got width %lld)", "", tile_w);
   hipStream_t st = as_stream(stream);
   BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
   long long tile_base = 0, pix_base = 0;
   for (int v = 0; v < n_views; ++v) {
     if (int e = check_image(W[v], H[v], P)) return e;
-    ViewDesc d = make_view(W[v], H[v], tanfovx[v], tanfovy[v], cx[v], cy[v], near_plane[v]);
+    ViewDesc d = make_view(W[v], H[v], tanfovx[v], tanfovy[v], cx[v], cy[v], near_plane[v], tile_w, tile_h);
     d.tile_base = (int)tile_base;
     d.pix_base = pix_base;
     SVR_TRY(launch_init_view(bp.g.views + v, d, w2c + 16 * v, c2w + 16 * v, background, st));
@@ -674,6 +698,8 @@ int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long l
   a.cbits = bp.b.cbits; a.k_cap = K_cap;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;
+  a.halves = g_batch_depth / 64;
+  a.blocks_per_cu = g_blocks_per_cu;
   {
     LaunchTimer timer(0, st);
     SVR_TRY(launch_render_forward(a, st));
@@ -708,6 +734,8 @@ int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long lo
   a.work_counter = bp.im.misc + 1;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;
+  a.halves = g_batch_depth / 64;
+  a.blocks_per_cu = g_blocks_per_cu;
   SVR_TRY(launch_contrib_compact(bp.b.cbits, K_cap, bp.b.cscan, bp.b.cpart, bp.b.cpos, bp.im.ranges, bp.im.tile_last,
                                  n_tiles, bp.im.cranges, st));
   a.cpos = bp.b.cpos; a.cranges = bp.im.cranges;

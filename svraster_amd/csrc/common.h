@@ -37,8 +37,16 @@ struct alignas(16) ViewDesc {
   float bg[3];
   int tile_base;          // first global tile index of this view (batched launches)
   long long pix_base;     // first pixel of this view in concatenated per-pixel buffers
-  long long pad1;
+  int tile_w, tile_h;     // tile shape: 16x16 (the reference's), or 32x8 / 8x32 (batched path)
 };
+
+// The four 8x8 pixel quadrants of a 256-pixel tile, one per wavefront: quadrant origin of
+// wavefront w of tile (tx, ty).  16x16 tiles are 2x2 quadrants, 32x8 tiles 4x1, 8x32 tiles 1x4.
+__host__ __device__ inline void tile_quadrant(const ViewDesc& v, int tx, int ty, int w, int& qx, int& qy) {
+  const int across = v.tile_w >> 3;
+  qx = tx * v.tile_w + (w % across) * 8;
+  qy = ty * v.tile_h + (w / across) * 8;
+}
 
 __host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
@@ -339,13 +347,14 @@ __device__ inline void lin_normal(const float g[8], float& nx, float& ny, float&
   rn = rsqrtf(nx * nx + ny * ny + nz * nz + 1e-15f);
 }
 
-// Tile rectangle of a packed pixel bbox, clamped to the view (auxiliary.h:97-111).
-__device__ inline void bbox_tiles(uint32_t b0, uint32_t b1, int tiles_x, int tiles_y,
-                                  int& tx0, int& ty0, int& tx1, int& ty1) {
-  tx0 = max(0, min(tiles_x - 1, int(b0 >> 16) / kTile));
-  ty0 = max(0, min(tiles_y - 1, int(b0 & 0xFFFFu) / kTile));
-  tx1 = max(0, min(tiles_x - 1, int(b1 >> 16) / kTile));
-  ty1 = max(0, min(tiles_y - 1, int(b1 & 0xFFFFu) / kTile));
+// Tile rectangle of a packed pixel bbox, clamped to the view (auxiliary.h:97-111, there with
+// 16x16 tiles).
+__device__ inline void bbox_tiles(uint32_t b0, uint32_t b1, const ViewDesc& v, int& tx0, int& ty0, int& tx1,
+                                  int& ty1) {
+  tx0 = max(0, min(v.tiles_x - 1, int(b0 >> 16) / v.tile_w));
+  ty0 = max(0, min(v.tiles_y - 1, int(b0 & 0xFFFFu) / v.tile_h));
+  tx1 = max(0, min(v.tiles_x - 1, int(b1 >> 16) / v.tile_w));
+  ty1 = max(0, min(v.tiles_y - 1, int(b1 & 0xFFFFu) / v.tile_h));
 }
 
 __device__ inline uint64_t rep3(uint32_t q) {

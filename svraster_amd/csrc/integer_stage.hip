@@ -73,7 +73,7 @@ __device__ inline void preprocess_one(const ViewDesc& v, bool leaf, float vcx, f
         bb.x = ((uint32_t)__builtin_rintf(bx0) << 16) | (uint32_t)__builtin_rintf(by0);
         bb.y = ((uint32_t)__builtin_rintf(bx1) << 16) | (uint32_t)__builtin_rintf(by1);
         int tx0, ty0, tx1, ty1;
-        bbox_tiles(bb.x, bb.y, v.tiles_x, v.tiles_y, tx0, ty0, tx1, ty1);
+        bbox_tiles(bb.x, bb.y, v, tx0, ty0, tx1, ty1);
         nd = (uint32_t)((1 + ty1 - ty0) * (1 + tx1 - tx0)) * (uint32_t)__popc(qbits);
       }
     }
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(256) void duplicate_kernel(
   uint32_t off = scan[o] - nd;
   const uint2 bb = bboxes[o];
   int tx0, ty0, tx1, ty1;
-  bbox_tiles(bb.x, bb.y, v.tiles_x, v.tiles_y, tx0, ty0, tx1, ty1);
+  bbox_tiles(bb.x, bb.y, v, tx0, ty0, tx1, ty1);
   const uint64_t path = (uint64_t)octpaths[i];
   const uint32_t qbits = quads[o];
   for (uint32_t q = 0; q < 8; ++q) {
@@ -451,7 +451,7 @@ __device__ inline void emit_pair_keys(const ViewDesc* __restrict__ views, int V,
     const ViewDesc& vd = views[v];
     const uint2 bb = bboxes[o];
     int tx0, ty0, tx1, ty1;
-    bbox_tiles(bb.x, bb.y, vd.tiles_x, vd.tiles_y, tx0, ty0, tx1, ty1);
+    bbox_tiles(bb.x, bb.y, vd, tx0, ty0, tx1, ty1);
     for (int y = ty0; y <= ty1; ++y)
       for (int x = tx0; x <= tx1; ++x) {
         if (off < lim) {

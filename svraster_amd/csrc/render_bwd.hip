@@ -152,8 +152,10 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
     const int W = V.W, H = V.H;
     const int lt = g - V.tile_base;
     const int tx = lt % V.tiles_x, ty = lt / V.tiles_x;
-    const int px = tx * kTile + (w & 1) * 8 + (lane & 7);
-    const int py = ty * kTile + (w >> 1) * 8 + (lane >> 3);
+    int px, py;
+    tile_quadrant(V, tx, ty, w, px, py);
+    px += lane & 7;
+    py += lane >> 3;
     const bool inside = px < W && py < H;
     const Ray r = pixel_ray(V, px, py);
     const uint32_t start = a.ranges[g].x;
@@ -392,22 +394,23 @@ __device__ inline uint64_t entry_block_mask_b(const uint2* s_bb, const float4* s
   return bbox_block_mask_b(bb.x, bb.y, qx, qy) & s_om[q];
 }
 
-template <bool kDepth, bool kNormal, bool kConcen, bool kQueue>
+template <bool kDepth, bool kNormal, bool kConcen, bool kQueue, int kHalves>
 __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
-  __shared__ float4 s_ch[kBwdStage], s_ml[kBwdStage], s_g0[kBwdStage], s_g1[kBwdStage], s_rgb[kBwdStage];
-  __shared__ float4 s_nrm[kNormal ? kBwdStage : 1];
-  __shared__ uint2 s_bb[kBwdStage];
-  __shared__ float s_acc[kBwdStage][12];
-  __shared__ uint32_t s_used[kBwdStage];
+  constexpr int kStage = 64 * kHalves;   // entries staged per round (kStage or half of it)
+  __shared__ float4 s_ch[kStage], s_ml[kStage], s_g0[kStage], s_g1[kStage], s_rgb[kStage];
+  __shared__ float4 s_nrm[kNormal ? kStage : 1];
+  __shared__ uint2 s_bb[kStage];
+  __shared__ float s_acc[kStage][12];
+  __shared__ uint32_t s_used[kStage];
   __shared__ uint64_t s_om[4][8];          // pixels of the block whose ray has octant q
   __shared__ float4 s_rd[4][64];           // ray direction xyz, dL/ddepth
   __shared__ float4 s_ri[4][64];           // 1/direction xyz
   __shared__ float4 s_dc[4][64];           // dL/dcolor xyz
   __shared__ float4 s_dn[4][kNormal ? 64 : 1];
   __shared__ float4 s_gt[4][kConcen ? 64 : 1];
-  __shared__ uint64_t s_m[4][kBwdStage];   // per entry: its pixels in the block
-  __shared__ uint16_t s_pref[4][kBwdStage];  // per entry: first position of its run
-  __shared__ uint32_t s_rel[kBwdStage];    // per slot: list position relative to the tile start
+  __shared__ uint64_t s_m[4][kStage];   // per entry: its pixels in the block
+  __shared__ uint16_t s_pref[4][kStage];  // per entry: first position of its run
+  __shared__ uint32_t s_rel[kStage];    // per slot: list position relative to the tile start
   __shared__ uint32_t s_lastpix[4][64];    // per pixel: contributor count (list positions it takes: < it)
   __shared__ unsigned long long s_cs[4][2][64];  // per pixel: contributing entries (lo, hi halves)
   __shared__ uint16_t s_own[4][kBwdWin];   // 1 + entry whose run starts at the position; later the
@@ -442,7 +445,8 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
     const int W = V.W, H = V.H;
     const int lt = g - V.tile_base;
     const int tx = lt % V.tiles_x, ty = lt / V.tiles_x;
-    const int qx = tx * kTile + (w & 1) * 8, qy = ty * kTile + (w >> 1) * 8;
+    int qx, qy;
+    tile_quadrant(V, tx, ty, w, qx, qy);
     const int px = qx + (lane & 7), py = qy + (lane >> 3);
     const bool inside = px < W && py < H;
     const Ray r = pixel_ray(V, px, py);
@@ -499,35 +503,35 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
     const float* rgbs_v = a.rgbs + (long long)vi * a.rgb_view_stride;
     EntryRegsB er;
     uint32_t nval = 0, npos = 0;
-    if (t < kBwdStage) {
+    if (t < kStage) {
       if (t < n) {
         const uint32_t p0 = a.cpos[cr.y - 1 - (uint32_t)t];
         fetch_entry_b(er, a, p0, a.vals[p0], vi, rgbs_v);
       }
-      if (kBwdStage + t < n) {
-        npos = a.cpos[cr.y - 1 - (uint32_t)(kBwdStage + t)];
+      if (kStage + t < n) {
+        npos = a.cpos[cr.y - 1 - (uint32_t)(kStage + t)];
         nval = a.vals[npos];
       }
     }
-    uint32_t npos2 = (t < kBwdStage && 2 * kBwdStage + t < n) ? a.cpos[cr.y - 1 - (uint32_t)(2 * kBwdStage + t)] : 0u;
+    uint32_t npos2 = (t < kStage && 2 * kStage + t < n) ? a.cpos[cr.y - 1 - (uint32_t)(2 * kStage + t)] : 0u;
 
-    for (int base = 0; base < n; base += kBwdStage) {
+    for (int base = 0; base < n; base += kStage) {
       __syncthreads();
-      const int cnt = min(kBwdStage, n - base);
+      const int cnt = min(kStage, n - base);
       if (t < cnt) {
         store_entry_b(er, kNormal, s, cam_scale, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t);
         s_rel[t] = er.pos - start;
         s_used[t] = 0u;
       }
-      for (int k = t; k < kBwdStage * 12; k += 256) (&s_acc[0][0])[k] = 0.f;
+      for (int k = t; k < kStage * 12; k += 256) (&s_acc[0][0])[k] = 0.f;
       __syncthreads();
-      if (t < kBwdStage) {
-        if (base + kBwdStage + t < n) fetch_entry_b(er, a, npos, nval, vi, rgbs_v);
-        if (base + 2 * kBwdStage + t < n) {
+      if (t < kStage) {
+        if (base + kStage + t < n) fetch_entry_b(er, a, npos, nval, vi, rgbs_v);
+        if (base + 2 * kStage + t < n) {
           npos = npos2;
           nval = a.vals[npos];
         }
-        if (base + 3 * kBwdStage + t < n) npos2 = a.cpos[cr.y - 1 - (uint32_t)(base + 3 * kBwdStage + t)];
+        if (base + 3 * kStage + t < n) npos2 = a.cpos[cr.y - 1 - (uint32_t)(base + 3 * kStage + t)];
       }
       // Slot j holds list position s_rel[j] (decreasing in j); a pixel takes the slots with
       // s_rel[j] < last, its contributor count.
@@ -536,17 +540,20 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
       if (live_m != 0ull) {
         s_lastpix[w][lane] = last;
         const uint64_t m_lo = lane < cnt ? entry_block_mask_b(s_bb, s_rgb, lane, qx, qy, s_om[w]) & live_m : 0ull;
-        const uint64_t m_hi = lane + 64 < cnt ? entry_block_mask_b(s_bb, s_rgb, lane + 64, qx, qy, s_om[w]) & live_m : 0ull;
+        const uint64_t m_hi = (kHalves > 1 && lane + 64 < cnt)
+                                  ? entry_block_mask_b(s_bb, s_rgb, lane + 64, qx, qy, s_om[w]) & live_m : 0ull;
         const uint32_t c_lo = (uint32_t)__popcll(m_lo), c_hi = (uint32_t)__popcll(m_hi);
         const uint32_t i_lo = wave_incl_scan_u32(c_lo);
-        const uint32_t i_hi = wave_incl_scan_u32(c_hi);
+        const uint32_t i_hi = kHalves > 1 ? wave_incl_scan_u32(c_hi) : 0u;
         const uint32_t t_lo = (uint32_t)__builtin_amdgcn_readlane((int)i_lo, 63);
         const uint32_t total = t_lo + (uint32_t)__builtin_amdgcn_readlane((int)i_hi, 63);
         const uint32_t p_lo = i_lo - c_lo, p_hi = t_lo + i_hi - c_hi;
         s_m[w][lane] = m_lo;
-        s_m[w][lane + 64] = m_hi;
         s_pref[w][lane] = (uint16_t)p_lo;
-        s_pref[w][lane + 64] = (uint16_t)p_hi;
+        if (kHalves > 1) {
+          s_m[w][lane + 64] = m_hi;
+          s_pref[w][lane + 64] = (uint16_t)p_hi;
+        }
         uint32_t carry = 0;   // 1 + entry owning the last position handled so far
 
         for (uint32_t w0 = 0; w0 < total; w0 += kBwdWin) {
@@ -792,11 +799,12 @@ static hipError_t launch_bwd_c(const BwdArgs& a, hipStream_t st, unsigned grid) 
   } else {
     // The work-queue (multi-view) launch is its own instantiation, so profiles tell it apart.
     if (a.work_counter) {
-      if (concen) render_bwd_pairs_kernel<D, N, true, true><<<grid, 256, 0, st>>>(a);
-      else render_bwd_pairs_kernel<D, N, false, true><<<grid, 256, 0, st>>>(a);
+      if (concen) render_bwd_pairs_kernel<D, N, true, true, 2><<<grid, 256, 0, st>>>(a);
+      else if (a.halves == 1) render_bwd_pairs_kernel<D, N, false, true, 1><<<grid, 256, 0, st>>>(a);
+      else render_bwd_pairs_kernel<D, N, false, true, 2><<<grid, 256, 0, st>>>(a);
     } else {
-      if (concen) render_bwd_pairs_kernel<D, N, true, false><<<grid, 256, 0, st>>>(a);
-      else render_bwd_pairs_kernel<D, N, false, false><<<grid, 256, 0, st>>>(a);
+      if (concen) render_bwd_pairs_kernel<D, N, true, false, 2><<<grid, 256, 0, st>>>(a);
+      else render_bwd_pairs_kernel<D, N, false, false, 2><<<grid, 256, 0, st>>>(a);
     }
   }
   return hipGetLastError();
@@ -806,7 +814,7 @@ hipError_t launch_render_backward(const BwdArgs& a, hipStream_t st) {
   if (a.total_tiles <= 0) return hipSuccess;
 
   unsigned grid = (unsigned)a.total_tiles;
-  if (a.work_counter) grid = (unsigned)min(a.total_tiles, 256 * 8);
+  if (a.work_counter) grid = (unsigned)min(a.total_tiles, 256 * max(1, a.blocks_per_cu));
   if (a.need_depth && a.need_normal) return launch_bwd_c<true, true>(a, st, grid);
   if (a.need_depth) return launch_bwd_c<true, false>(a, st, grid);
   if (a.need_normal) return launch_bwd_c<false, true>(a, st, grid);

@@ -187,8 +187,10 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
     const int W = V.W, H = V.H;
     const int lt = g - V.tile_base;
     const int tx = lt % V.tiles_x, ty = lt / V.tiles_x;
-    const int px = tx * kTile + (w & 1) * 8 + (lane & 7);
-    const int py = ty * kTile + (w >> 1) * 8 + (lane >> 3);
+    int px, py;
+    tile_quadrant(V, tx, ty, w, px, py);
+    px += lane & 7;
+    py += lane >> 3;
     const bool inside = px < W && py < H;
     const Ray r = pixel_ray(V, px, py);
     const uint2 rg = a.ranges[g];
@@ -338,16 +340,17 @@ __device__ inline void mark_contributing(uint32_t* bits, long long p0, uint64_t 
   }
 }
 
-template <bool kDepth, bool kNormal, bool kMaxW, bool kQueue>
+template <bool kDepth, bool kNormal, bool kMaxW, bool kQueue, int kHalves>
 __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
-  __shared__ float4 s_ch[kPairStage], s_ml[kPairStage], s_g0[kPairStage], s_g1[kPairStage], s_rgb[kPairStage];
-  __shared__ float4 s_nrm[kNormal ? kPairStage : 1];
-  __shared__ uint2 s_bb[kPairStage];
+  constexpr int kStage = 64 * kHalves;   // entries staged per round (kStage or half of it)
+  __shared__ float4 s_ch[kStage], s_ml[kStage], s_g0[kStage], s_g1[kStage], s_rgb[kStage];
+  __shared__ float4 s_nrm[kNormal ? kStage : 1];
+  __shared__ uint2 s_bb[kStage];
   __shared__ float4 s_rd[4][64];              // direction xyz
   __shared__ float4 s_ri[4][64];              // 1/direction xyz
   __shared__ uint64_t s_om[4][8];             // pixels of the block whose ray has octant q
-  __shared__ uint64_t s_m[4][kPairStage];     // per entry: its pixels in the block
-  __shared__ uint16_t s_pref[4][kPairStage];  // per entry: first position of its run
+  __shared__ uint64_t s_m[4][kStage];     // per entry: its pixels in the block
+  __shared__ uint16_t s_pref[4][kStage];  // per entry: first position of its run
   __shared__ unsigned long long s_cs[4][2][64];   // per pixel: contributing entries (lo, hi halves)
   __shared__ uint8_t s_own[4][kPairWin];      // 1 + entry whose run starts at the position, else 0
   __shared__ uint16_t s_hit[4][kPairWin];     // positions whose ray hits the box
@@ -377,7 +380,8 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
     const int W = V.W, H = V.H;
     const int lt = g - V.tile_base;
     const int tx = lt % V.tiles_x, ty = lt / V.tiles_x;
-    const int qx = tx * kTile + (w & 1) * 8, qy = ty * kTile + (w >> 1) * 8;
+    int qx, qy;
+    tile_quadrant(V, tx, ty, w, qx, qy);
     const int px = qx + (lane & 7), py = qy + (lane >> 3);
     const bool inside = px < W && py < H;
     const Ray r = pixel_ray(V, px, py);
@@ -406,38 +410,41 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
     const float* rgbs_v = a.rgbs + (long long)vi * a.rgb_view_stride;
     EntryRegs er;
     uint32_t nval = 0;
-    if (t < kPairStage) {
+    if (t < kStage) {
       if (t < n) fetch_entry(er, a.vals[rg.x + t], vi, a.P, a.bboxes, a.centers, a.lengths, a.geos, rgbs_v);
-      if (kPairStage + t < n) nval = a.vals[rg.x + kPairStage + t];
+      if (kStage + t < n) nval = a.vals[rg.x + kStage + t];
     }
 
-    for (int base = 0; base < n; base += kPairStage) {
+    for (int base = 0; base < n; base += kStage) {
       if (__syncthreads_count(done) == 256) break;
-      if (t < kPairStage && base + t < n)
+      if (t < kStage && base + t < n)
         store_entry(er, kNormal, s, cam_scale, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t);
       __syncthreads();
-      if (t < kPairStage) {
-        if (base + kPairStage + t < n) fetch_entry(er, nval, vi, a.P, a.bboxes, a.centers, a.lengths, a.geos, rgbs_v);
-        if (base + 2 * kPairStage + t < n) nval = a.vals[rg.x + base + 2 * kPairStage + t];
+      if (t < kStage) {
+        if (base + kStage + t < n) fetch_entry(er, nval, vi, a.P, a.bboxes, a.centers, a.lengths, a.geos, rgbs_v);
+        if (base + 2 * kStage + t < n) nval = a.vals[rg.x + base + 2 * kStage + t];
       }
-      const int cnt = min(kPairStage, n - base);
+      const int cnt = min(kStage, n - base);
       const uint64_t live = __ballot(!done);
       if (live == 0ull) continue;
 
       // 1. entry masks and the entry-major run offsets.
       const uint64_t m_lo = lane < cnt ? entry_block_mask(s_bb, s_rgb, lane, qx, qy, s_om[w], live) : 0ull;
-      const uint64_t m_hi = lane + 64 < cnt ? entry_block_mask(s_bb, s_rgb, lane + 64, qx, qy, s_om[w], live) : 0ull;
+      const uint64_t m_hi = (kHalves > 1 && lane + 64 < cnt)
+                                ? entry_block_mask(s_bb, s_rgb, lane + 64, qx, qy, s_om[w], live) : 0ull;
       const uint32_t c_lo = (uint32_t)__popcll(m_lo), c_hi = (uint32_t)__popcll(m_hi);
       if (__ballot((c_lo | c_hi) != 0u) == 0ull) continue;
       const uint32_t i_lo = wave_incl_scan_u32(c_lo);
-      const uint32_t i_hi = wave_incl_scan_u32(c_hi);
+      const uint32_t i_hi = kHalves > 1 ? wave_incl_scan_u32(c_hi) : 0u;
       const uint32_t t_lo = (uint32_t)__builtin_amdgcn_readlane((int)i_lo, 63);
       const uint32_t total = t_lo + (uint32_t)__builtin_amdgcn_readlane((int)i_hi, 63);
       const uint32_t p_lo = i_lo - c_lo, p_hi = t_lo + i_hi - c_hi;
       s_m[w][lane] = m_lo;
-      s_m[w][lane + 64] = m_hi;
       s_pref[w][lane] = (uint16_t)p_lo;
-      s_pref[w][lane + 64] = (uint16_t)p_hi;
+      if (kHalves > 1) {
+        s_m[w][lane + 64] = m_hi;
+        s_pref[w][lane + 64] = (uint16_t)p_hi;
+      }
       uint32_t carry = 0;   // 1 + entry owning the last position handled so far
       uint64_t pr_lo = 0ull, pr_hi = 0ull;   // entries this pixel composited in this batch
 
@@ -554,13 +561,15 @@ static hipError_t launch_fwd_mw(const FwdArgs& a, hipStream_t st, unsigned grid)
     if (a.max_w) render_fwd_kernel<D, N, true><<<grid, 256, 0, st>>>(a);
     else render_fwd_kernel<D, N, false><<<grid, 256, 0, st>>>(a);
   } else {
-    // The work-queue (multi-view) launch is its own instantiation, so profiles tell it apart.
+    // The work-queue (multi-view) launch is its own instantiation, so profiles tell it apart; it
+    // also comes with 64-entry rounds (halves = 1) for the tuner.
     if (a.work_counter) {
-      if (a.max_w) render_fwd_pairs_kernel<D, N, true, true><<<grid, 256, 0, st>>>(a);
-      else render_fwd_pairs_kernel<D, N, false, true><<<grid, 256, 0, st>>>(a);
+      if (a.max_w) render_fwd_pairs_kernel<D, N, true, true, 2><<<grid, 256, 0, st>>>(a);
+      else if (a.halves == 1) render_fwd_pairs_kernel<D, N, false, true, 1><<<grid, 256, 0, st>>>(a);
+      else render_fwd_pairs_kernel<D, N, false, true, 2><<<grid, 256, 0, st>>>(a);
     } else {
-      if (a.max_w) render_fwd_pairs_kernel<D, N, true, false><<<grid, 256, 0, st>>>(a);
-      else render_fwd_pairs_kernel<D, N, false, false><<<grid, 256, 0, st>>>(a);
+      if (a.max_w) render_fwd_pairs_kernel<D, N, true, false, 2><<<grid, 256, 0, st>>>(a);
+      else render_fwd_pairs_kernel<D, N, false, false, 2><<<grid, 256, 0, st>>>(a);
     }
   }
   return hipGetLastError();
@@ -573,7 +582,7 @@ hipError_t launch_render_forward(const FwdArgs& a, hipStream_t st) {
   hipError_t e = launch_fill_u32(a.cbits, contrib_words(a.k_cap), a.variant == 1 ? 0xFFFFFFFFu : 0u, st);
   if (e != hipSuccess) return e;
   unsigned grid = (unsigned)a.total_tiles;
-  if (a.work_counter) grid = (unsigned)min(a.total_tiles, 256 * 8);
+  if (a.work_counter) grid = (unsigned)min(a.total_tiles, 256 * max(1, a.blocks_per_cu));
   if (a.need_depth && a.need_normal) return launch_fwd_mw<true, true>(a, st, grid);
   if (a.need_depth) return launch_fwd_mw<true, false>(a, st, grid);
   if (a.need_normal) return launch_fwd_mw<false, true>(a, st, grid);

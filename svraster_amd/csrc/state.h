@@ -238,6 +238,8 @@ struct FwdArgs {
   long long k_cap;                 // list positions covered by cbits
   int need_depth, need_normal;
   int variant;                     // 0: pair-compacted (default), 1: one entry at a time
+  int halves = 2;                  // entries staged per round: 64 * halves (pair-compacted kernel)
+  int blocks_per_cu = 8;           // persistent grid size with a work counter
 };
 hipError_t launch_render_forward(const FwdArgs& a, hipStream_t st);
 
@@ -259,6 +261,8 @@ struct BwdArgs {
   const uint32_t* cpos; const uint2* cranges;
   int need_depth, need_normal;
   int variant;                     // 0: pair-compacted (default), 1: one entry at a time
+  int halves = 2;                  // entries staged per round: 64 * halves (pair-compacted kernel)
+  int blocks_per_cu = 8;           // persistent grid size with a work counter
 };
 hipError_t launch_render_backward(const BwdArgs& a, hipStream_t st);
 
