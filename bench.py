@@ -272,7 +272,8 @@ def bench_train(args):
     torch.cuda.empty_cache()
     t0 = time.perf_counter()
     rast = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key,
-                             cams, model.bg_color, sc.sh_degree, tile_shape=best["tile_shape"])
+                             cams, model.bg_color, sc.sh_degree, tile_shape=best["tile_shape"],
+                             fuse_gather_bw=best["fuse_gather_bw"])
     it = BatchedTrainIteration(rast, model._geo_grid_pts.detach(), model._sh0.detach(), model._shs.detach(),
                                model._log_s.detach(), targets, smooth, lr=lr, betas=betas, eps=eps)
     it()
@@ -401,6 +402,7 @@ def main():
                            not args.no_autotune)
     torch.cuda.empty_cache()
     bstep = make_step(best["tile_shape"])
+    autotune.apply_options(best, bstep)
     rast = bstep.r
     bstep()
     sync()

@@ -266,13 +266,18 @@ int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long l
                         int need_normal, float* out_color, float* out_depth, float* out_normal, float* out_T,
                         void* stream);
 /* One persistent reverse compositor launch over all (view, tile) pairs.  dL_dvox [P,12]
- * (colour slots unused), dL_drgb_views [V,P,3] and dL_ds [1] zeroed by the caller. */
+ * (colour slots unused), dL_drgb_views [V,P,3] and dL_ds [1] zeroed by the caller.  With
+ * vox_key [P,8] and dL_dgrid_pts [M] (zeroed by the caller) the corner-SDF gradients are added
+ * straight to the grid points as they are flushed -- the transpose of GatherGeoParams
+ * (geo_params_gather.cu:22-72) fused into the compositor -- and dL_dvox slots 0-7 stay
+ * untouched. */
 int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                        const float* vox_centers, const float* vox_lengths, const float* geos,
                        const float* rgbs_views, const float* s_val, const float* out_T,
                        const float* dL_dcolor, const float* dL_ddepth, const float* dL_dnormal,
                        const float* dL_dT, int need_depth, int need_normal, float* dL_dvox,
-                       float* dL_drgb_views, float* dL_ds, void* stream);
+                       float* dL_drgb_views, float* dL_ds, const int64_t* vox_key, float* dL_dgrid_pts,
+                       void* stream);
 /* SH transpose summed over the views, one voxel per thread; writes every row of dL_dsh0 [P,3]
  * and dL_dshs [P,M-1,3] (M <= 16).  The entries of dL_drgb_views it consumes are reset to zero,
  * so a buffer that starts at zero is clear again for the next svr_batch_backward. */

@@ -8,14 +8,16 @@ path by timing its own fwd+bwd step on the GPU:
   overlaps (the key count) and how well a tile's entries are shared by its four wavefronts;
 * LDS staging depth of the pair-compacted compositors: 64 or 128 entries per round;
 * persistent grid size of the work-queue launches: workgroups per CU;
-* compositor variant: pair-compacted (entry-major) or one entry at a time.
+* compositor variant: pair-compacted (entry-major) or one entry at a time;
+* corner-SDF gradients: flushed by the backward compositor straight into the grid points, or
+  into a per-voxel buffer scattered by the gather transpose afterwards.
 
 None of these changes an image or a gradient (tests/test_gpu_variants.py and
 tests/test_gpu_autotune.py check that), so the tuner is free to pick the fastest.  The search
-is coordinate descent from the defaults (tile shape, then depth, then grid size, then variant),
-7 timed configurations instead of the 36 of the full product; each is timed as the median of a
-few eager steps after a warm-up step.  Results are cached per workload signature in a JSON
-file (SVRASTER_TUNE_CACHE, default ~/.cache/svraster_amd/tune.json).
+is coordinate descent from the defaults (tile shape, then depth, then grid size, then variant,
+then the gradient route), 8 timed configurations instead of the 72 of the full product; each is
+timed as the median of a few eager steps after a warm-up step.  Results are cached per workload
+signature in a JSON file (SVRASTER_TUNE_CACHE, default ~/.cache/svraster_amd/tune.json).
 
 There is no counterpart in the reference, whose kernels have fixed 16x16 tiles and 256-entry
 batches (cuda/src/config.h:14-39, forward.cu:42-564).
@@ -35,15 +37,21 @@ TILE_SHAPES: Tuple[Tuple[int, int], ...] = ((16, 16), (32, 8), (8, 32))
 BATCH_DEPTHS: Tuple[int, ...] = (128, 64)
 BLOCKS_PER_CU: Tuple[int, ...] = (8, 4, 16)
 VARIANTS: Tuple[int, ...] = (0, 1)
+FUSE_GATHER_BW: Tuple[bool, ...] = (False, True)
 
-DEFAULT = {"tile_shape": (16, 16), "batch_depth": 128, "persistent_blocks_per_cu": 8, "compositor_variant": 0}
+DEFAULT = {"tile_shape": (16, 16), "batch_depth": 128, "persistent_blocks_per_cu": 8, "compositor_variant": 0,
+           "fuse_gather_bw": False}
 
 
-def apply_options(cfg: Dict) -> None:
-    """Set the library options of a configuration (the tile shape belongs to the rasterizer)."""
+def apply_options(cfg: Dict, step=None) -> None:
+    """Set the library options of a configuration (the tile shape belongs to the rasterizer,
+    the gradient route to the step's rasterizer when one is given)."""
     _lib.set_option("batch_depth", int(cfg["batch_depth"]))
     _lib.set_option("persistent_blocks_per_cu", int(cfg["persistent_blocks_per_cu"]))
     _lib.set_option("compositor_variant", int(cfg["compositor_variant"]))
+    rast = getattr(step, "r", None)
+    if rast is not None and hasattr(rast, "fuse_gather_bw"):
+        rast.fuse_gather_bw = bool(cfg.get("fuse_gather_bw", False))
 
 
 def workload_signature(n_voxels: int, cameras: List, sh_degree: int, need_depth: bool = False,
@@ -122,7 +130,7 @@ def autotune(make_step: Callable[[Tuple[int, int]], Callable[[], None]], signatu
     best = dict(DEFAULT)
 
     def run(cfg, step):
-        apply_options(cfg)
+        apply_options(cfg, step)
         ms = time_step(step, steps, warmup)
         table.append((dict(cfg), ms))
         if log:
@@ -140,9 +148,9 @@ def autotune(make_step: Callable[[Tuple[int, int]], Callable[[], None]], signatu
         if ms < best_ms:
             best_ms, best = ms, cfg
     step = make_step(best["tile_shape"])
-    # 2.-4. the launch options, one at a time
+    # 2.-5. the launch options and the gradient route, one at a time
     for key, values in (("batch_depth", BATCH_DEPTHS), ("persistent_blocks_per_cu", BLOCKS_PER_CU),
-                        ("compositor_variant", VARIANTS)):
+                        ("compositor_variant", VARIANTS), ("fuse_gather_bw", FUSE_GATHER_BW)):
         for val in values:
             if val == best[key]:
                 continue
@@ -150,7 +158,7 @@ def autotune(make_step: Callable[[Tuple[int, int]], Callable[[], None]], signatu
             ms = run(cfg, step)
             if ms < best_ms:
                 best_ms, best = ms, cfg
-    apply_options(best)
+    apply_options(best, step)
     if signature is not None:
         store_cached(signature, best)
     return best, table

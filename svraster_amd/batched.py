@@ -36,7 +36,7 @@ class BatchedRasterizer:
     def __init__(self, octpath, octlevel, vox_center, vox_size, is_leaf, vox_key, cameras: List,
                  background: torch.Tensor, sh_degree: int, n_sh_coeffs: int = 16,
                  key_capacity: Optional[int] = None, need_depth=False, need_normal=False, cull: bool = True,
-                 tile_shape=(16, 16)):
+                 tile_shape=(16, 16), fuse_gather_bw: bool = False):
         self.dev = vox_center.device
         self.lib = _lib.load()
         self.V = len(cameras)
@@ -49,6 +49,9 @@ class BatchedRasterizer:
         self.deg = int(sh_degree)
         self.M = int(n_sh_coeffs)
         self.need_depth, self.need_normal = bool(need_depth), bool(need_normal)
+        # Corner-SDF gradients added to the grid points by the backward compositor's flush
+        # instead of a [P,8] buffer scattered by a separate gather transpose.
+        self.fuse_gather_bw = bool(fuse_gather_bw)
         # Rank order of every (voxel, octant) pair: fixed while the voxel set is.
         tmp = torch.empty(int(self.lib.svr_batch_pair_order_bytes(self.P)), dtype=torch.uint8, device=self.dev)
         self.pair_order = torch.empty(8 * self.P, dtype=torch.int32, device=self.dev)
@@ -190,11 +193,14 @@ class BatchedRasterizer:
         self.ds.zero_()
         self.dgrid.zero_()
         dT = self.zero_T if dL_dT is None else dL_dT
+        fuse = self.fuse_gather_bw
         _lib.check(self.lib.svr_batch_backward(_p(self.ws), *self._dims(), _p(self.vox_center), _p(self.vox_size),
                                                _p(self.geos), _p(self.rgbs_views), _p(self.s_val), _p(self.T),
                                                _p(dL_dcolor), _p(dL_ddepth), _p(dL_dnormal), _p(dT),
                                                int(self.need_depth), int(self.need_normal), _p(self.dvox),
-                                               _p(self.drgb_views), _p(self.ds), st), "svr_batch_backward")
+                                               _p(self.drgb_views), _p(self.ds),
+                                               _p(self.vox_key) if fuse else None, _p(self.dgrid) if fuse else None,
+                                               st), "svr_batch_backward")
         if sh_adam is None:
             _lib.check(self.lib.svr_batch_sh_backward(_p(self.ws), *self._dims(), _p(self.vox_center),
                                                       _p(self.drgb_views), self.deg, self.M, _p(self.dsh0),
@@ -208,9 +214,10 @@ class BatchedRasterizer:
                 a.n_table, _p(a.step), ctypes.c_float(a.b1), ctypes.c_float(a.b2), ctypes.c_float(a.eps), st),
                 "sh_backward_adam")
             dsh0 = dshs = None
-        dgeo = self.dvox[:, :8].contiguous()
-        _lib.check(self.lib.svr_gather_triinterp_geo_params_bw(_p(self.vox_key), self.P, None, self.P, _p(dgeo),
-                                                               _p(self.dgrid), st), "gather_bw")
+        if not fuse:
+            dgeo = self.dvox[:, :8].contiguous()
+            _lib.check(self.lib.svr_gather_triinterp_geo_params_bw(_p(self.vox_key), self.P, None, self.P,
+                                                                   _p(dgeo), _p(self.dgrid), st), "gather_bw")
         return self.dgrid, dsh0, dshs, self.ds, self.dvox[:, 11]
 
 

@@ -711,9 +711,12 @@ int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long lo
                        const float* vox_centers, const float* vox_lengths, const float* geos,
                        const float* rgbs_views, const float* s_val, const float* out_T, const float* dL_dcolor,
                        const float* dL_ddepth, const float* dL_dnormal, const float* dL_dT, int need_depth,
-                       int need_normal, float* dL_dvox, float* dL_drgb_views, float* dL_ds, void* stream) {
+                       int need_normal, float* dL_dvox, float* dL_drgb_views, float* dL_ds,
+                       const int64_t* vox_key, float* dL_dgrid_pts, void* stream) {
   g_err.clear();
   if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
+  if ((vox_key == nullptr) != (dL_dgrid_pts == nullptr))
+    return fail(1, "vox_key and dL_dgrid_pts go together%s%lld", "", 0);
   if (need_depth && !dL_ddepth) return fail(1, "need_depth requires dL_ddepth%s%lld", "", 0);
   if (need_normal && !dL_dnormal) return fail(1, "need_normal requires dL_dnormal%s%lld", "", 0);
   hipStream_t st = as_stream(stream);
@@ -731,6 +734,7 @@ int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long lo
   a.dL_dcolor = dL_dcolor; a.dL_ddepth = dL_ddepth; a.dL_dnormal = dL_dnormal; a.dL_dT = dL_dT;
   a.gt_color = nullptr; a.lambda_R_concen = 0.f;
   a.dL_dvox = dL_dvox; a.dL_drgb_views = dL_drgb_views; a.dL_ds = dL_ds;
+  a.vox_key = vox_key; a.dL_dgrid = dL_dgrid_pts;
   a.work_counter = bp.im.misc + 1;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;

@@ -345,6 +345,7 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
           float* dst = a.dL_dvox + (long long)vox * 12 + lane;
           if (a.dL_drgb_views && lane >= 8 && lane < 11)
             dst = a.dL_drgb_views + ((long long)vi * a.P + vox) * 3 + (lane - 8);
+          if (a.dL_dgrid && lane < 8) dst = a.dL_dgrid + a.vox_key[(long long)vox * 8 + lane];
           atomicAdd(dst, mine);
         }
       }
@@ -780,6 +781,7 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
         float* dst = a.dL_dvox + (long long)vox * 12 + slot;
         if (a.dL_drgb_views && slot >= 8 && slot < 11)
           dst = a.dL_drgb_views + ((long long)vi * a.P + vox) * 3 + (slot - 8);
+        if (a.dL_dgrid && slot < 8) dst = a.dL_dgrid + a.vox_key[(long long)vox * 8 + slot];
         atomicAdd(dst, v);
       }
     }

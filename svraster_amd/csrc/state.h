@@ -256,6 +256,9 @@ struct BwdArgs {
   float* dL_dvox;                  // [P,12]
   float* dL_drgb_views;            // [V,P,3] per-view colour gradients (null: into dL_dvox)
   float* dL_ds;                    // [1]
+  // Corner-SDF gradients straight into the grid points (the gather's transpose fused into the
+  // flush; null: into dL_dvox slots 0-7).
+  const int64_t* vox_key; float* dL_dgrid;
   uint32_t* work_counter;
   // Contributing entries per tile (launch_contrib_compact over the forward's bits).
   const uint32_t* cpos; const uint2* cranges;

@@ -71,7 +71,8 @@ def test_autotune_picks_and_caches(tmp_path, default_options):
         assert best["batch_depth"] in autotune.BATCH_DEPTHS
         assert best["persistent_blocks_per_cu"] in autotune.BLOCKS_PER_CU
         assert best["compositor_variant"] in autotune.VARIANTS
-        assert len(table) == 7
+        assert best["fuse_gather_bw"] in autotune.FUSE_GATHER_BW
+        assert len(table) == 8
         assert _lib.get_option("batch_depth") == best["batch_depth"]
         cached, rows = autotune.autotune(make_step, sig)
         assert cached == best and rows[0][0] == "cached"

@@ -150,3 +150,23 @@ def test_batched_tile_lists_match_per_view():
         assert torch.equal(nb, ncon_v), f"view {v}: contributor counts"
         say(f"  view {v} {W}x{H}: {int(K)} keys, lists and contributor counts identical")
         tile_base += nt
+
+
+@pytest.mark.gpu
+def test_fused_gather_transpose_matches_separate_pass():
+    say("[batched] corner-SDF gradients flushed straight to the grid points vs dL/dgeos + gather transpose")
+    w, sc, cams, targets = _setup()
+    out = []
+    for fuse in (False, True):
+        m = VoxelModel(sc)
+        r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key, cams,
+                              m.bg_color, sc.sh_degree, fuse_gather_bw=fuse)
+        step = BatchedTrainStep(r, m._geo_grid_pts.detach(), m._sh0.detach(), m._shs.detach(),
+                                m._log_s.detach(), targets)
+        step()
+        torch.cuda.synchronize()
+        out.append((step.grid_grad.clone(), r.dvox[:, :8].abs().sum().item()))
+    e = rel_l2(out[1][0], out[0][0])
+    say(f"  grid gradient rel-L2 {e:.2e}; |dL/dgeos| left in the voxel buffer: separate {out[0][1]:.3e}, "
+        f"fused {out[1][1]:.3e}")
+    assert e <= 1e-6 and out[0][1] > 0 and out[1][1] == 0.0
