@@ -158,6 +158,32 @@ int svr_laplacian_smoothness_bw(const float* grid_pts, long long n_pts, const in
                                 int A, float weight, float vox_size_inv, unsigned int seed,
                                 const float* randv, float* grid_pts_grad, void* stream);
 
+/* Replaces TV_COMPUTE::total_variation_bw (src/tv_compute.cu:23-189).  grid_pts / grid_pts_grad
+ * [M,C], vox_key [N,8], vox_size_inv [N] (read unless no_tv_s).  Corner i of each voxel and
+ * channel receives w * (sum over its three edges of +1 where it exceeds the neighbour, else -1),
+ * w = weight (* 0.01 * vox_size_inv unless no_tv_s).  With tv_sparse a voxel takes part only if a
+ * corner's gradient is already non-zero; the test reads grid_pts_grad as it was before the call
+ * (sparse_ws: N*C bytes of scratch), where the reference reads it while other threads write. */
+int svr_total_variation_bw(const float* grid_pts, long long M, int C, const int64_t* vox_key, long long N,
+                           float weight, const float* vox_size_inv, int no_tv_s, int tv_sparse,
+                           uint8_t* sparse_ws, float* grid_pts_grad, void* stream);
+
+/* Replaces VG_COMPUTE::voxel_gradient_bw (src/vg_compute.cu:23-103).  For each voxel in
+ * active_list [A] (int32): g = sum_i s_i S_i / (4h) over its corner SDFs (s_i = +-1 per axis),
+ * corner i receives 4 (|g|^2 - 2) weight (g . s_i) / 4 (the reference's constants). */
+int svr_voxel_gradient_bw(const float* grid_pts, long long M, const int64_t* vox_key, long long N,
+                          const float* vox_size_inv, const int32_t* active_list, int A, float weight,
+                          float* grid_pts_grad, void* stream);
+
+/* Replaces PL_COMPUTE::points_loss_bw (src/pl_compute.cu:14-104).  points_in_grid [Mp,3] in grid
+ * cells; each point in the grid whose cell is in grid_mask takes the voxel of its cell (binary
+ * search over the sorted grid_keys [G], grid2voxel [G]), and corner i of that voxel receives
+ * 2 weight sdf w_i (unclamped trilinear weights w_i at the point). */
+int svr_points_loss_bw(const float* points_in_grid, int Mp, const float* grid_pts, long long M,
+                       const int64_t* vox_key, int N, const float* grid_voxel_coord, const float* grid_voxel_size,
+                       int grid_res, const uint8_t* grid_mask, const int32_t* grid_keys, const int32_t* grid2voxel,
+                       int G, float weight, float* grid_pts_grad, void* stream);
+
 /* Cell -> grid_keys index for all res^3 cells (int32 out[res^3]): the entry the reference's
  * binary search over the sorted keys returns (src/auxiliary.h:377-386), -1 for absent cells.  Built
  * once per grid table; svr_grid_smoothness_bw_step then does one load per sample instead of a

@@ -13,6 +13,11 @@ sampling helpers cuda/src/auxiliary.h:363-455 (neighbour validity, midpoint bina
 clamped trilinear weights, corner scatter).  The per-key random offset is an explicit input.
 
 Pinning (tests/test_oracle_misc.py):
+  * total variation: corner values 0..7 by hand, ties count as "not greater", and central
+    finite differences of w * sum |edge differences| away from ties; tv_sparse by construction;
+  * voxel gradient: exact on a linear field (zero where |g|^2 = 2) and h times the finite
+    difference of w (|g|^2 - 2)^2;
+  * points loss: central finite differences of w * sum sdf^2, skipped points by hand;
   * eikonal on an exactly linear SDF f = a.x (corner values sampled from it, so trilinear
     interpolation is exact): the central difference is exact, |grad| = |a|, and the result
     must equal the closed form 2 (g - 1) w scattered with +-(a_i/|a|) 0.5 inv to the neighbour
@@ -159,3 +164,136 @@ def grid_loss_value(kind, grid_pts, vox_key, coord, vsize, res, mask, keys, owne
             r = np.array([sd[1] + sd[2] - 2 * sd[0], sd[3] + sd[4] - 2 * sd[0], sd[5] + sd[6] - 2 * sd[0]])
             total += weight * 0.5 * inv * inv * np.sqrt((r * r).sum() + 1e-10)
     return total
+
+
+# ----------------------------------------------------------------------------------------------
+# Per-voxel regularisers (tv_compute.cu:23-159, vg_compute.cu:23-75, pl_compute.cu:14-66)
+# ----------------------------------------------------------------------------------------------
+def tv_grad(grid_pts, vox_key, weight, vox_size_inv, no_tv_s, tv_sparse, grad_in):
+    """Total variation: for voxel n, channel c, corner i gets w * sum over the three corners
+    i^1, i^2, i^4 of (+1 if v_i > v_j else -1), w = weight (* 0.01 vox_size_inv[n] unless
+    no_tv_s).  tv_sparse: voxels whose 8 corners all have a zero gradient in grad_in (the
+    gradient before the call) are skipped.  Returns grad_in + the contributions (float64)."""
+    g = np.asarray(grid_pts, dtype=np.float64)
+    if g.ndim == 1:
+        g = g[:, None]
+    out = np.array(grad_in, dtype=np.float64).reshape(g.shape)
+    before = out.copy()
+    keys = np.asarray(vox_key, dtype=np.int64)
+    for n in range(keys.shape[0]):
+        k = keys[n]
+        for c in range(g.shape[1]):
+            if tv_sparse and not np.any(before[k, c] != 0.0):
+                continue
+            v = g[k, c]
+            w = float(weight) if no_tv_s else float(weight) * 0.01 * float(vox_size_inv[n])
+            for i in range(8):
+                s = sum(1 if v[i] > v[i ^ b] else -1 for b in (1, 2, 4))
+                out[k[i], c] += w * s
+    return out
+
+
+def tv_value(grid_pts, vox_key, weight, vox_size_inv, no_tv_s):
+    """The scalar the TV gradient differentiates where no two corners tie: sum over voxels of
+    w * (sum over the 12 edges of |v_a - v_b|)."""
+    g = np.asarray(grid_pts, dtype=np.float64).reshape(-1)
+    keys = np.asarray(vox_key, dtype=np.int64)
+    total = 0.0
+    for n in range(keys.shape[0]):
+        v = g[keys[n]]
+        w = float(weight) if no_tv_s else float(weight) * 0.01 * float(vox_size_inv[n])
+        for i in range(8):
+            for b in (1, 2, 4):
+                if i & b == 0:
+                    total += w * abs(v[i] - v[i | b])
+    return total
+
+
+def _signs(i):
+    return ((1.0 if i & 4 else -1.0), (1.0 if i & 2 else -1.0), (1.0 if i & 1 else -1.0))
+
+
+def vg_grad(grid_pts, vox_key, vox_size_inv, active, weight):
+    """Voxel gradient: g = sum_i s_i S_i / (4h) (s_i = +-1 per axis, corner bits x=4, y=2, z=1);
+    corner i gets 4 (|g|^2 - 2) weight (g . s_i) / 4."""
+    gp = np.asarray(grid_pts, dtype=np.float64).reshape(-1)
+    out = np.zeros_like(gp)
+    keys = np.asarray(vox_key, dtype=np.int64)
+    for v in np.asarray(active, dtype=np.int64):
+        S = gp[keys[v]]
+        inv4h = 0.25 * float(vox_size_inv[v])
+        gvec = np.zeros(3)
+        for i in range(8):
+            gvec += np.array(_signs(i)) * S[i]
+        gvec *= inv4h
+        scale = 4.0 * (gvec @ gvec - 2.0) * float(weight)
+        for i in range(8):
+            out[keys[v][i]] += scale * (gvec @ np.array(_signs(i))) * 0.25
+    return out
+
+
+def vg_value(grid_pts, vox_key, vox_size_inv, active, weight):
+    """weight * (|g|^2 - 2)^2 summed over the active voxels; vg_grad equals h times its gradient
+    (the reference chains with 1/4 where the derivative has 1/(4h))."""
+    gp = np.asarray(grid_pts, dtype=np.float64).reshape(-1)
+    keys = np.asarray(vox_key, dtype=np.int64)
+    total = 0.0
+    for v in np.asarray(active, dtype=np.int64):
+        S = gp[keys[v]]
+        gvec = np.zeros(3)
+        for i in range(8):
+            gvec += np.array(_signs(i)) * S[i]
+        gvec *= 0.25 * float(vox_size_inv[v])
+        total += float(weight) * (gvec @ gvec - 2.0) ** 2
+    return total
+
+
+def _point_voxel(p, res, mask, keys, owners):
+    ix, iy, iz = (int(np.trunc(p[0])), int(np.trunc(p[1])), int(np.trunc(p[2])))
+    if not (0 <= ix < res and 0 <= iy < res and 0 <= iz < res):
+        return -1
+    cell = ix + iy * res + iz * res * res
+    if not mask[cell]:
+        return -1
+    idx = key_search(keys, cell)
+    return -1 if idx < 0 else int(owners[idx])
+
+
+def pl_grad(points, grid_pts, vox_key, coord, vsize, res, mask, keys, owners, weight):
+    """Points loss: each point (grid cells) on a valid cell takes its cell's voxel; with the
+    unclamped trilinear weights w at the point, corner i gets 2 weight sdf w_i."""
+    gp = np.asarray(grid_pts, dtype=np.float64).reshape(-1)
+    out = np.zeros_like(gp)
+    keys_v = np.asarray(vox_key, dtype=np.int64)
+    for p in np.asarray(points, dtype=np.float64):
+        v = _point_voxel(p, res, mask, keys, owners)
+        if v < 0:
+            continue
+        q = (p - np.asarray(coord[v], dtype=np.float64)) / float(vsize[v])
+        w = _trilinear(q)
+        sdf = float(w @ gp[keys_v[v]])
+        for i in range(8):
+            out[keys_v[v][i]] += 2.0 * sdf * float(weight) * w[i]
+    return out
+
+
+def pl_value(points, grid_pts, vox_key, coord, vsize, res, mask, keys, owners, weight):
+    """weight * sum over the points of sdf^2, the scalar pl_grad differentiates."""
+    gp = np.asarray(grid_pts, dtype=np.float64).reshape(-1)
+    keys_v = np.asarray(vox_key, dtype=np.int64)
+    total = 0.0
+    for p in np.asarray(points, dtype=np.float64):
+        v = _point_voxel(p, res, mask, keys, owners)
+        if v < 0:
+            continue
+        q = (p - np.asarray(coord[v], dtype=np.float64)) / float(vsize[v])
+        total += float(weight) * float(_trilinear(q) @ gp[keys_v[v]]) ** 2
+    return total
+
+
+def _trilinear(q):
+    x, y, z = q
+    w = np.empty(8)
+    for i in range(8):
+        w[i] = (x if i & 4 else 1.0 - x) * (y if i & 2 else 1.0 - y) * (z if i & 1 else 1.0 - z)
+    return w

@@ -377,16 +377,43 @@ def laplacian_smoothness_bw(grid_pts, vox_key, grid_voxel_coord, grid_voxel_size
                grid_pts_grad, seed, randv)
 
 
-def total_variation_bw(*args, **kwargs):
-    raise NotImplementedError("total_variation_bw is not on the SDF training path (weight 0)")
+def total_variation_bw(grid_pts, vox_key, weight, vox_size_inv, no_tv_s, tv_sparse, grid_pts_grad):
+    """tv_compute.cu:162-189 (accumulates into grid_pts_grad [M,C])."""
+    _inplace_ok(grid_pts_grad, "grid_pts_grad")
+    dev = _dev(grid_pts)
+    g = _f32(grid_pts, "grid_pts")
+    C = g.shape[1] if g.dim() > 1 else 1
+    N = vox_key.shape[0]
+    vsi = None if no_tv_s else _f32(vox_size_inv.reshape(-1), "vox_size_inv")
+    ws = torch.empty(N * C if tv_sparse else 0, dtype=torch.uint8, device=dev)
+    _lib_call("svr_total_variation_bw", _ptr(g), g.shape[0], C, _ptr(vox_key.contiguous()), N, float(weight),
+              _ptr(vsi), int(bool(no_tv_s)), int(bool(tv_sparse)), _ptr(ws), _ptr(grid_pts_grad), _stream(dev))
 
 
-def voxel_gradient_bw(*args, **kwargs):
-    raise NotImplementedError("voxel_gradient_bw is not provided by this build")
+def voxel_gradient_bw(grid_pts, vox_key, vox_size_inv, active_list, weight, no_tv_s, tv_sparse, grid_pts_grad):
+    """vg_compute.cu:78-103 (accumulates into grid_pts_grad; no_tv_s and tv_sparse are unused
+    there as well)."""
+    _inplace_ok(grid_pts_grad, "grid_pts_grad")
+    dev = _dev(grid_pts)
+    active = active_list.to(device=dev, dtype=torch.int32).contiguous()
+    _lib_call("svr_voxel_gradient_bw", _ptr(_f32(grid_pts, "grid_pts")), grid_pts.shape[0],
+              _ptr(vox_key.contiguous()), vox_key.shape[0], _ptr(_f32(vox_size_inv.reshape(-1), "vox_size_inv")),
+              _ptr(active), active.numel(), float(weight), _ptr(grid_pts_grad), _stream(dev))
 
 
-def points_loss_bw(*args, **kwargs):
-    raise NotImplementedError("points_loss_bw is not provided by this build")
+def points_loss_bw(points_in_grid, grid_pts, vox_key, grid_voxel_coord, grid_voxel_size, grid_res, grid_mask,
+                   grid_keys, grid2voxel, weight, vox_size_inv, no_tv_s, tv_sparse, grid_pts_grad):
+    """pl_compute.cu:69-104 (accumulates into grid_pts_grad; vox_size_inv, no_tv_s and
+    tv_sparse are unused there as well)."""
+    _inplace_ok(grid_pts_grad, "grid_pts_grad")
+    dev = _dev(grid_pts)
+    pts = _f32(points_in_grid.to(dev), "points_in_grid")
+    mask = grid_mask.to(torch.uint8).contiguous() if grid_mask.dtype != torch.uint8 else grid_mask.contiguous()
+    _lib_call("svr_points_loss_bw", _ptr(pts), pts.shape[0], _ptr(_f32(grid_pts, "grid_pts")), grid_pts.shape[0],
+              _ptr(vox_key.contiguous()), vox_key.shape[0], _ptr(_f32(grid_voxel_coord, "grid_voxel_coord")),
+              _ptr(_f32(grid_voxel_size.reshape(-1), "grid_voxel_size")), int(grid_res), _ptr(mask),
+              _ptr(grid_keys.to(torch.int32).contiguous()), _ptr(grid2voxel.to(torch.int32).contiguous()),
+              grid_keys.numel(), float(weight), _ptr(grid_pts_grad), _stream(dev))
 
 
 # ----------------------------------------------------------------------------------------------

@@ -485,6 +485,46 @@ int svr_laplacian_smoothness_bw(const float* grid_pts, long long n_pts, const in
                           grid_pts_grad, stream);
 }
 
+int svr_total_variation_bw(const float* grid_pts, long long M, int C, const int64_t* vox_key, long long N,
+                           float weight, const float* vox_size_inv, int no_tv_s, int tv_sparse,
+                           uint8_t* sparse_ws, float* grid_pts_grad, void* stream) {
+  g_err.clear();
+  if (C <= 0) return fail(1, "channel count%s %lld must be positive", "", C);
+  if (N <= 0) return 0;
+  if (tv_sparse && !sparse_ws) return fail(1, "tv_sparse needs a workspace of N*C bytes%s%lld", "", 0);
+  if (!no_tv_s && !vox_size_inv) return fail(1, "vox_size_inv is required unless no_tv_s%s%lld", "", 0);
+  (void)M;
+  SVR_TRY(launch_total_variation(N, C, grid_pts, vox_key, weight, vox_size_inv, no_tv_s != 0,
+                                 tv_sparse ? sparse_ws : nullptr, grid_pts_grad, as_stream(stream)));
+  return 0;
+}
+
+int svr_voxel_gradient_bw(const float* grid_pts, long long M, const int64_t* vox_key, long long N,
+                          const float* vox_size_inv, const int32_t* active_list, int A, float weight,
+                          float* grid_pts_grad, void* stream) {
+  g_err.clear();
+  (void)M; (void)N;
+  SVR_TRY(launch_voxel_gradient(A, active_list, grid_pts, vox_key, vox_size_inv, weight, grid_pts_grad,
+                                as_stream(stream)));
+  return 0;
+}
+
+int svr_points_loss_bw(const float* points_in_grid, int Mp, const float* grid_pts, long long M,
+                       const int64_t* vox_key, int N, const float* grid_voxel_coord, const float* grid_voxel_size,
+                       int grid_res, const uint8_t* grid_mask, const int32_t* grid_keys, const int32_t* grid2voxel,
+                       int G, float weight, float* grid_pts_grad, void* stream) {
+  g_err.clear();
+  if (grid_res <= 0 || (long long)grid_res * grid_res * grid_res >= (1LL << 31))
+    return fail(1, "grid_res%s %lld out of range", "", grid_res);
+  GridTable T;
+  T.grid_pts = grid_pts; T.vox_key = vox_key; T.keys = grid_keys; T.key2vox = grid2voxel;
+  T.vox_coord = grid_voxel_coord; T.vox_size = grid_voxel_size; T.n_keys = G; T.res = grid_res; T.n_vox = N;
+  T.n_pts = M;
+  SVR_TRY(launch_points_loss(Mp, points_in_grid, T, reinterpret_cast<const bool*>(grid_mask), weight,
+                             grid_pts_grad, as_stream(stream)));
+  return 0;
+}
+
 int svr_grid_dense_index(const int32_t* grid_keys, int G, int grid_res, int32_t* out, void* stream) {
   g_err.clear();
   if (grid_res <= 2 || (long long)grid_res * grid_res * grid_res >= (1LL << 31))

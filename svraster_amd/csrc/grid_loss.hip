@@ -503,6 +503,156 @@ hipError_t launch_grid_laplacian_pull(const GridOrderArgs& A, hipStream_t st) {
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------------------
+// Per-voxel regularisers of the corner SDFs.
+//
+// Total variation (tv_compute.cu:23-159): per voxel and channel, corner i receives
+// w * sum over its three edges of (+1 if its value exceeds the neighbour's, else -1), w = weight
+// (times 0.01 * vox_size_inv[voxel] unless no_tv_s).  With tv_sparse a voxel takes part only
+// when one of its corners already has a non-zero gradient; the reference reads the gradient
+// while other threads add to it, so here the test is made for every voxel first, on the
+// gradient as it was before the launch (one pass writes a flag per voxel and channel).
+__global__ __launch_bounds__(256) void tv_sparse_flags_kernel(long long NC, int C, const int64_t* __restrict__ vox_key,
+                                                              const float* __restrict__ grad,
+                                                              uint8_t* __restrict__ flags) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= NC) return;
+  const long long n = idx / C;
+  const int c = (int)(idx - n * C);
+  bool any = false;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) any |= grad[vox_key[n * 8 + i] * C + c] != 0.f;
+  flags[idx] = any ? 1 : 0;
+}
+
+template <bool kNoTvS>
+__global__ __launch_bounds__(256) void total_variation_kernel(long long NC, int C, const float* __restrict__ grid_pts,
+                                                              const int64_t* __restrict__ vox_key, float weight,
+                                                              const float* __restrict__ vox_size_inv,
+                                                              const uint8_t* __restrict__ flags,
+                                                              float* __restrict__ grad) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= NC) return;
+  if (flags && !flags[idx]) return;
+  const long long n = idx / C;
+  const int c = (int)(idx - n * C);
+  long long key[8];
+  float v[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    key[i] = vox_key[n * 8 + i];
+    v[i] = grid_pts[key[i] * C + c];
+  }
+  float w = weight;
+  if (!kNoTvS) w *= 0.01f * vox_size_inv[n];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int g = (v[i] > v[i ^ 1] ? 1 : -1) + (v[i] > v[i ^ 2] ? 1 : -1) + (v[i] > v[i ^ 4] ? 1 : -1);
+    atomicAdd(grad + key[i] * C + c, w * (float)g);
+  }
+}
+
+// Voxel gradient (vg_compute.cu:23-75): per active voxel, the SDF gradient at the centre from
+// the four edge differences per axis, g = (1 / 4h) sum_i s_i S_i; corner i receives
+// 4 (|g|^2 - 2) weight (g . s_i) / 4 -- the reference's own constants (2, and 1/4 in place of
+// 1/(4h) in the chain rule), kept as they are.
+__global__ __launch_bounds__(256) void voxel_gradient_kernel(int A, const int32_t* __restrict__ active,
+                                                             const float* __restrict__ grid_pts,
+                                                             const int64_t* __restrict__ vox_key,
+                                                             const float* __restrict__ vox_size_inv, float weight,
+                                                             float* __restrict__ grad) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= A) return;
+  const long long v = active[tid];
+  const float inv4h = 0.25f * vox_size_inv[v];
+  long long key[8];
+  float S[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    key[i] = vox_key[v * 8 + i];
+    S[i] = grid_pts[key[i]];
+  }
+  // corner i = (bit 2: x, bit 1: y, bit 0: z); s = 2 * bit - 1
+  float gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    gx += ((i & 4) ? S[i] : -S[i]);
+    gy += ((i & 2) ? S[i] : -S[i]);
+    gz += ((i & 1) ? S[i] : -S[i]);
+  }
+  gx *= inv4h; gy *= inv4h; gz *= inv4h;
+  const float g2 = gx * gx + gy * gy + gz * gz;
+  const float scale = 4.f * (g2 - 2.f) * weight;
+  const float dx = scale * gx, dy = scale * gy, dz = scale * gz;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const float d = ((i & 4) ? dx : -dx) + ((i & 2) ? dy : -dy) + ((i & 1) ? dz : -dz);
+    atomicAdd(grad + key[i], d * 0.25f);
+  }
+}
+
+// Points loss (pl_compute.cu:14-66): each point (in grid cells) inside the grid and on a valid
+// cell takes the voxel the table assigns to its cell, interpolates the SDF there with
+// unclamped trilinear weights (auxiliary.h:319-334) and adds 2 weight sdf w_i to corner i
+// (auxiliary.h:443-455), i.e. the gradient of weight * sdf^2.
+__global__ __launch_bounds__(256) void points_loss_kernel(int Mp, const float* __restrict__ pts, GridTable G,
+                                                          const bool* __restrict__ mask, float weight,
+                                                          float* __restrict__ grad) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= Mp) return;
+  const float px = pts[3 * tid], py = pts[3 * tid + 1], pz = pts[3 * tid + 2];
+  const int ix = (int)px, iy = (int)py, iz = (int)pz;
+  const int res = G.res;
+  if (ix < 0 || ix >= res || iy < 0 || iy >= res || iz < 0 || iz >= res) return;
+  const int cell = ix + iy * res + iz * res * res;
+  if (!mask[cell]) return;
+  const int idx = G.dense ? G.dense[cell] : key_search(G.keys, G.n_keys, cell);
+  if (idx < 0) return;
+  const int v = G.key2vox[idx];
+  if (v < 0 || v >= G.n_vox) return;
+  const float size = G.vox_size[v];
+  const float qx = (px - G.vox_coord[3 * v]) / size;
+  const float qy = (py - G.vox_coord[3 * v + 1]) / size;
+  const float qz = (pz - G.vox_coord[3 * v + 2]) / size;
+  float w[8];
+  trilinear_weights(qx, qy, qz, w);
+  float sdf = 0.f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) sdf += w[i] * G.grid_pts[G.vox_key[(long long)v * 8 + i]];
+  const float d = 2.f * sdf * weight;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) atomicAdd(grad + G.vox_key[(long long)v * 8 + i], d * w[i]);
+}
+
+hipError_t launch_total_variation(long long N, int C, const float* grid_pts, const int64_t* vox_key, float weight,
+                                  const float* vox_size_inv, bool no_tv_s, uint8_t* sparse_flags, float* grad,
+                                  hipStream_t st) {
+  const long long NC = N * C;
+  if (NC <= 0) return hipSuccess;
+  const unsigned nb = (unsigned)((NC + 255) / 256);
+  if (sparse_flags) tv_sparse_flags_kernel<<<nb, 256, 0, st>>>(NC, C, vox_key, grad, sparse_flags);
+  if (no_tv_s) total_variation_kernel<true><<<nb, 256, 0, st>>>(NC, C, grid_pts, vox_key, weight, vox_size_inv,
+                                                                 sparse_flags, grad);
+  else total_variation_kernel<false><<<nb, 256, 0, st>>>(NC, C, grid_pts, vox_key, weight, vox_size_inv,
+                                                         sparse_flags, grad);
+  return hipGetLastError();
+}
+
+hipError_t launch_voxel_gradient(int A, const int32_t* active, const float* grid_pts, const int64_t* vox_key,
+                                 const float* vox_size_inv, float weight, float* grad, hipStream_t st) {
+  if (A <= 0) return hipSuccess;
+  voxel_gradient_kernel<<<(unsigned)((A + 255) / 256), 256, 0, st>>>(A, active, grid_pts, vox_key, vox_size_inv,
+                                                                     weight, grad);
+  return hipGetLastError();
+}
+
+hipError_t launch_points_loss(int Mp, const float* pts, const GridTable& G, const bool* mask, float weight,
+                              float* grad, hipStream_t st) {
+  if (Mp <= 0) return hipSuccess;
+  points_loss_kernel<<<(unsigned)((Mp + 255) / 256), 256, 0, st>>>(Mp, pts, G, mask, weight, grad);
+  return hipGetLastError();
+}
+
 // Cell -> table index for every cell of the res^3 grid: the entry key_search returns (so
 // duplicate keys resolve exactly as the search does), -1 where the cell has no key.  Turns the
 // six or seven binary searches per active key into one load each.

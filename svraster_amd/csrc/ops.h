@@ -71,6 +71,13 @@ struct GridOrderArgs {
   const int32_t* step;
 };
 hipError_t launch_grid_laplacian_pull(const GridOrderArgs& A, hipStream_t st);
+hipError_t launch_total_variation(long long N, int C, const float* grid_pts, const int64_t* vox_key, float weight,
+                                  const float* vox_size_inv, bool no_tv_s, uint8_t* sparse_flags, float* grad,
+                                  hipStream_t st);
+hipError_t launch_voxel_gradient(int A, const int32_t* active, const float* grid_pts, const int64_t* vox_key,
+                                 const float* vox_size_inv, float weight, float* grad, hipStream_t st);
+hipError_t launch_points_loss(int Mp, const float* pts, const GridTable& G, const bool* mask, float weight,
+                              float* grad, hipStream_t st);
 hipError_t launch_grid_dense_index(long long n_cells, const int32_t* keys, int G, int32_t* out, hipStream_t st);
 hipError_t launch_table_count(int N, const float* center, const float* size, const float* scene_center,
                               float extent, int res, const bool* is_leaf, uint32_t* count, hipStream_t st);

@@ -132,3 +132,59 @@ def test_grid_losses(kind):
     e = rel_l2(grad.reshape(-1), torch.from_numpy(ref))
     say(f"  rel-L2 {e:.2e}, |ref| {np.linalg.norm(ref):.3e}")
     assert np.linalg.norm(ref) > 0 and e <= 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("no_tv_s,tv_sparse", [(True, False), (False, False), (True, True)])
+def test_total_variation(no_tv_s, tv_sparse):
+    say(f"[grid loss] total variation no_tv_s={no_tv_s} tv_sparse={tv_sparse}")
+    w = make_workload("fd_256", device="cuda")
+    sc = w.scene
+    M = sc.geo_grid_pts.shape[0]
+    rng = np.random.default_rng(11)
+    gin = np.zeros(M, dtype=np.float32)
+    if tv_sparse:
+        hit = rng.choice(M, M // 50, replace=False)
+        gin[hit] = rng.normal(size=hit.size).astype(np.float32)
+    grad = torch.from_numpy(gin).cuda().reshape(M, 1).contiguous()
+    inv = (1.0 / sc.vox_size.reshape(-1)).contiguous()
+    svraster_cuda.grid_loss_bw.total_variation(sc.geo_grid_pts, sc.vox_key, 1e-3, inv, no_tv_s, tv_sparse, grad)
+    ref = ogl.tv_grad(sc.geo_grid_pts.cpu().numpy(), sc.vox_key.cpu().numpy(), 1e-3, inv.cpu().numpy(), no_tv_s,
+                      tv_sparse, gin)
+    e = rel_l2(grad.reshape(-1), torch.from_numpy(ref.reshape(-1)))
+    say(f"  rel-L2 {e:.2e} over {M} grid points")
+    assert e <= 1e-6
+
+
+@pytest.mark.gpu
+def test_voxel_gradient():
+    say("[grid loss] voxel gradient")
+    w = make_workload("fd_256", device="cuda")
+    sc = w.scene
+    N = sc.num_voxels
+    rng = np.random.default_rng(12)
+    active = torch.from_numpy(rng.integers(0, N, size=N // 2).astype(np.int32)).cuda()   # repeats included
+    inv = (1.0 / sc.vox_size.reshape(-1)).contiguous()
+    grad = torch.zeros_like(sc.geo_grid_pts)
+    svraster_cuda.grid_loss_bw.voxel_gradient(sc.geo_grid_pts, sc.vox_key, inv, active, 1e-4, True, False, grad)
+    ref = ogl.vg_grad(sc.geo_grid_pts.cpu().numpy(), sc.vox_key.cpu().numpy(), inv.cpu().numpy(),
+                      active.cpu().numpy(), 1e-4)
+    e = rel_l2(grad.reshape(-1), torch.from_numpy(ref))
+    say(f"  rel-L2 {e:.2e}, |ref| {np.linalg.norm(ref):.3e}")
+    assert np.linalg.norm(ref) > 0 and e <= 1e-5
+
+
+@pytest.mark.gpu
+def test_points_loss():
+    say("[grid loss] points loss")
+    sc, mask, keys, sk, g2v, coord, size, res = _grid_setup()
+    rng = np.random.default_rng(13)
+    pts = rng.uniform(-2.0, res + 2.0, size=(20000, 3)).astype(np.float32)   # some outside the grid
+    grad = torch.zeros_like(sc.geo_grid_pts)
+    svraster_cuda.grid_loss_bw.points_loss(torch.from_numpy(pts).cuda(), sc.geo_grid_pts, sc.vox_key, coord, size,
+                                           res, mask, sk, g2v, 1e-2, float(res / 2.0), True, False, grad)
+    ref = ogl.pl_grad(pts, sc.geo_grid_pts.cpu().numpy(), sc.vox_key.cpu().numpy(), coord.cpu().numpy(),
+                      size.cpu().numpy(), res, mask.cpu().numpy(), sk.cpu().numpy(), g2v.cpu().numpy(), 1e-2)
+    e = rel_l2(grad.reshape(-1), torch.from_numpy(ref))
+    say(f"  rel-L2 {e:.2e}, |ref| {np.linalg.norm(ref):.3e}")
+    assert np.linalg.norm(ref) > 0 and e <= 1e-5

@@ -145,3 +145,83 @@ def test_valid_gradient_table_rules():
     assert sorted(set(owners.tolist())) == [0, 1]
     assert (owners == 0).sum() == 8 and (owners == 1).sum() == 1
     assert mask.sum() == 9
+
+
+def _fd(f, x, idx, h=1e-6):
+    xp, xm = x.copy(), x.copy()
+    xp[idx] += h
+    xm[idx] -= h
+    return (f(xp) - f(xm)) / (2 * h)
+
+
+def test_total_variation_closed_form_ties_and_finite_differences():
+    # one voxel, corner values 0..7: corner i compares with i^1, i^2, i^4
+    keys = np.arange(8)[None, :]
+    g = ogl.tv_grad(np.arange(8.0), keys, 0.5, [1.0], True, False, np.zeros(8)).reshape(-1)
+    assert np.array_equal(g, 0.5 * np.array([-3, -1, -1, 1, -1, 1, 1, 3], dtype=float))
+    # ties count as "not greater": all equal -> -3 everywhere
+    g = ogl.tv_grad(np.ones(8), keys, 1.0, [1.0], True, False, np.zeros(8)).reshape(-1)
+    assert np.array_equal(g, -3.0 * np.ones(8))
+    # no_tv_s False scales by 0.01 * vox_size_inv
+    g2 = ogl.tv_grad(np.arange(8.0), keys, 0.5, [4.0], False, False, np.zeros(8)).reshape(-1)
+    assert np.allclose(g2, 0.5 * 0.01 * 4.0 * np.array([-3, -1, -1, 1, -1, 1, 1, 3]))
+    # random voxels sharing corners: the gradient of w * sum |edge differences|
+    rng = np.random.default_rng(0)
+    vk = np.array([[0, 1, 2, 3, 4, 5, 6, 7], [4, 5, 6, 7, 8, 9, 10, 11]])
+    x = rng.normal(size=12)
+    inv = np.array([2.0, 3.0])
+    got = ogl.tv_grad(x, vk, 0.3, inv, False, False, np.zeros(12)).reshape(-1)
+    fd = np.array([_fd(lambda y: ogl.tv_value(y, vk, 0.3, inv, False), x, i) for i in range(12)])
+    assert np.allclose(got, fd, atol=1e-6)
+    # tv_sparse: only voxels with a corner whose gradient is already non-zero
+    gin = np.zeros(12)
+    gin[0] = 1.0            # corner of voxel 0 only
+    got = ogl.tv_grad(x, vk, 0.3, inv, True, True, gin).reshape(-1)
+    ref = ogl.tv_grad(x, vk[:1], 0.3, inv[:1], True, False, gin).reshape(-1)
+    assert np.array_equal(got, ref)
+
+
+def test_voxel_gradient_closed_form_and_finite_differences():
+    # corners of a voxel of size h sampled from a linear field a.x: g = a exactly
+    h = 0.5
+    a = np.array([1.0, 0.5, -0.25])
+    corners = np.array([[(hx - 0.5) * h, (hy - 0.5) * h, (hz - 0.5) * h]
+                        for hx in (0, 1) for hy in (0, 1) for hz in (0, 1)])   # bit order x=4, y=2, z=1
+    S = corners @ a
+    keys = np.arange(8)[None, :]
+    g = ogl.vg_grad(S, keys, [1.0 / h], [0], 0.7)
+    # |g|^2 = 1.3125: corner i gets 4 (|a|^2 - 2) w (a . s_i) / 4
+    s = np.array([[1.0 if i & 4 else -1.0, 1.0 if i & 2 else -1.0, 1.0 if i & 1 else -1.0] for i in range(8)])
+    assert np.allclose(g, (a @ a - 2.0) * 0.7 * (s @ a))
+    a2 = a / np.linalg.norm(a) * np.sqrt(2.0)           # |g|^2 = 2: zero gradient
+    assert np.allclose(ogl.vg_grad(corners @ a2, keys, [1.0 / h], [0], 0.7), 0.0, atol=1e-12)
+    # h times the gradient of w (|g|^2 - 2)^2, repeated active entries counted twice
+    rng = np.random.default_rng(1)
+    vk = np.array([[0, 1, 2, 3, 4, 5, 6, 7], [4, 5, 6, 7, 8, 9, 10, 11]])
+    x = rng.normal(size=12)
+    inv = np.array([2.0, 2.0])
+    act = [0, 1, 1]
+    got = ogl.vg_grad(x, vk, inv, act, 0.3)
+    fd = np.array([_fd(lambda y: ogl.vg_value(y, vk, inv, act, 0.3), x, i) for i in range(12)])
+    assert np.allclose(got, 0.5 * fd, atol=1e-6)
+
+
+def test_points_loss_finite_differences_and_skips():
+    res = 4
+    mask = np.zeros(res ** 3, dtype=bool)
+    keys = np.array([0, 1, 4, 5, 16, 17, 20, 21])      # cells of a 2x2x2 voxel at the origin
+    mask[keys] = True
+    owners = np.zeros(8, dtype=np.int64)
+    vk = np.arange(8)[None, :]
+    coord = np.zeros((1, 3))
+    vsize = np.array([2.0])
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=8)
+    pts = np.array([[0.5, 0.7, 1.2], [1.9, 0.1, 0.3], [-0.5, 0.2, 0.2],      # last: trunc(-0.5) = 0, kept
+                    [3.5, 3.5, 3.5], [4.5, 0.0, 0.0], [-1.5, 0.0, 0.0]])      # masked, outside, outside
+    got = ogl.pl_grad(pts, x, vk, coord, vsize, res, mask, keys, owners, 0.4)
+    fd = np.array([_fd(lambda y: ogl.pl_value(pts, y, vk, coord, vsize, res, mask, keys, owners, 0.4), x, i)
+                   for i in range(8)])
+    assert np.allclose(got, fd, atol=1e-6)
+    only3 = ogl.pl_grad(pts[:3], x, vk, coord, vsize, res, mask, keys, owners, 0.4)
+    assert np.array_equal(got, only3)
