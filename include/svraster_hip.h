@@ -223,6 +223,21 @@ int svr_grid_laplacian_bw_table_order(const float* grid_pts, long long n_pts, co
                                       float vox_size_inv, unsigned int seed, const int32_t* step, float* rec_ws,
                                       float* coef_ws, float* grid_pts_grad, void* stream);
 
+/* The voxel <-> grid-point link of octree_utils.build_grid_pts_link (src/utils/octree_utils.py:
+ * 129-144, 203-222): the 8 corners of each voxel as integer coordinates at level 16, deduplicated
+ * in lexicographic (x, y, z) order.  Outputs vox_key int64 [N,8] and grid_pts_key int64 [M,3]
+ * (room for 8N rows); *n_grid_pts = M.  Synchronises the stream (M goes to the host).
+ * ws: svr_grid_pts_link_workspace_bytes(N) bytes. */
+size_t svr_grid_pts_link_workspace_bytes(long long N);
+int svr_build_grid_pts_link(const int64_t* octpath, const int8_t* octlevel, long long N, void* ws,
+                            int64_t* vox_key, int64_t* grid_pts_key, long long* n_grid_pts, void* stream);
+
+/* agg_voxel_into_grid_pts (src/sparse_voxel_gears/adaptive.py:336-360): out [M,C] = mean over the
+ * voxels naming each grid point of their per-corner values vox_val [N,8,C]; points no voxel
+ * names get 0.  count_ws: M uint32 of scratch. */
+int svr_agg_voxel_into_grid_pts(const int64_t* vox_key, long long N, const float* vox_val, int C, long long M,
+                                uint32_t* count_ws, float* out, void* stream);
+
 /* Replace UTILS::ijk_2_octpath / octpath_2_ijk (src/utils.cu:129-189, 329-361). */
 int svr_ijk_2_octpath(const int64_t* ijk, const int8_t* octlevel, int n, int64_t* octpath,
                       void* stream);

@@ -64,6 +64,9 @@ _SIGS = {
     "svr_adam_bias_table": (I, [D, D, D, I, P]),
     "svr_sparse_adam_step_dev": (I, [I, P, P, P, P, LL, P, P, I, F, F, F, P]),
     "svr_grid_dense_index": (I, [P, I, I, P, P]),
+    "svr_grid_pts_link_workspace_bytes": (SZ, [LL]),
+    "svr_build_grid_pts_link": (I, [P, P, LL, P, P, P, P, P]),
+    "svr_agg_voxel_into_grid_pts": (I, [P, LL, P, I, LL, P, P, P]),
     "svr_total_variation_bw": (I, [P, LL, I, P, LL, F, P, I, I, P, P, P]),
     "svr_voxel_gradient_bw": (I, [P, LL, P, LL, P, P, I, F, P, P]),
     "svr_points_loss_bw": (I, [P, I, P, LL, P, I, P, P, I, P, P, P, I, F, P, P]),

@@ -570,6 +570,55 @@ int svr_grid_laplacian_bw_table_order(const float* grid_pts, long long n_pts, co
   return 0;
 }
 
+// Workspace of svr_build_grid_pts_link for N voxels: a binning layout for the 8N corner codes,
+// then flags and ranks (u32 each) and a scan partial large enough for 8N.
+struct LinkLayout {
+  size_t bin, flags, rank, partial, nuniq, total;
+  explicit LinkLayout(long long N) {
+    const long long n = 8 * N;
+    Carver c;
+    bin = c.take<uint8_t>((long long)BinLayout(n).total);
+    flags = c.take<uint32_t>(n);
+    rank = c.take<uint32_t>(n);
+    partial = c.take<uint32_t>((long long)scan_partial_count(n) + 1);
+    nuniq = c.take<uint32_t>(1);
+    total = c.off;
+  }
+};
+
+size_t svr_grid_pts_link_workspace_bytes(long long N) { return N > 0 ? LinkLayout(N).total : 0; }
+
+int svr_build_grid_pts_link(const int64_t* octpath, const int8_t* octlevel, long long N, void* ws,
+                            int64_t* vox_key, int64_t* grid_pts_key, long long* n_grid_pts, void* stream) {
+  g_err.clear();
+  if (N < 0 || 8 * N >= (1LL << 32)) return fail(1, "voxel count%s %lld out of range", "", N);
+  if (n_grid_pts) *n_grid_pts = 0;
+  if (N == 0) return 0;
+  hipStream_t st = as_stream(stream);
+  const LinkLayout L(N);
+  uint8_t* base = static_cast<uint8_t*>(ws);
+  BinPtrs b(base + L.bin, 8 * N);
+  uint32_t* flags = reinterpret_cast<uint32_t*>(base + L.flags);
+  uint32_t* rank = reinterpret_cast<uint32_t*>(base + L.rank);
+  uint32_t* partial = reinterpret_cast<uint32_t*>(base + L.partial);
+  uint32_t* nuniq = reinterpret_cast<uint32_t*>(base + L.nuniq);
+  SVR_TRY(launch_grid_pts_link(N, octpath, octlevel, b.keys, b.vals, b.keys_alt, b.vals_alt, b.hist, b.partial,
+                               partial, flags, rank, vox_key, grid_pts_key, nuniq, st));
+  uint32_t m = 0;
+  SVR_TRY(hipMemcpyAsync(&m, nuniq, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  SVR_TRY(hipStreamSynchronize(st));
+  if (n_grid_pts) *n_grid_pts = (long long)m;
+  return 0;
+}
+
+int svr_agg_voxel_into_grid_pts(const int64_t* vox_key, long long N, const float* vox_val, int C, long long M,
+                                uint32_t* count_ws, float* out, void* stream) {
+  g_err.clear();
+  if (C <= 0 || M < 0 || N < 0) return fail(1, "bad sizes%s %lld", "", C);
+  SVR_TRY(launch_grid_pts_mean(N, C, vox_key, vox_val, M, count_ws, out, as_stream(stream)));
+  return 0;
+}
+
 int svr_ijk_2_octpath(const int64_t* ijk, const int8_t* octlevel, int n, int64_t* octpath, void* stream) {
   g_err.clear();
   SVR_TRY(launch_ijk_to_octpath(n, ijk, octlevel, octpath, as_stream(stream)));

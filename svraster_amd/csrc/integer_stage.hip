@@ -709,4 +709,82 @@ hipError_t launch_tile_ranges(const uint64_t* keys, const uint32_t* n_dev, long 
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------------------
+// Voxel <-> grid-point link (octree_utils.py:129-144, 203-222): the 8 corners of every voxel as
+// integer coordinates at the finest level, deduplicated.  The reference takes
+// unique(dim=0, return_inverse=True) over the [8N, 3] rows, i.e. grid points in lexicographic
+// (x, y, z) order; the same order is the order of the packed code x << 34 | y << 17 | z
+// (coordinates <= 2^16 take 17 bits), so a stable radix sort of the codes, a flag where the
+// code changes and a scan give the grid points in that order and every corner's index.
+__global__ __launch_bounds__(256) void corner_codes_kernel(long long n, const int64_t* __restrict__ octpath,
+                                                           const int8_t* __restrict__ octlevel,
+                                                           uint64_t* __restrict__ codes,
+                                                           uint32_t* __restrict__ idx) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const long long v = t >> 3;
+  const int c = (int)(t & 7);
+  const int lv = octlevel[v];
+  uint64_t path = (uint64_t)octpath[v] >> (3 * (kMaxLevels - lv));
+  uint64_t i = 0, j = 0, k = 0;
+  for (int l = 0; l < kMaxLevels; ++l) {
+    const uint64_t b = path & 7u;
+    i |= ((b >> 2) & 1u) << l;
+    j |= ((b >> 1) & 1u) << l;
+    k |= (b & 1u) << l;
+    path >>= 3;
+  }
+  const int sh = kMaxLevels - lv;
+  const uint64_t x = (i + ((c >> 2) & 1)) << sh, y = (j + ((c >> 1) & 1)) << sh, z = (k + (c & 1)) << sh;
+  codes[t] = (x << 34) | (y << 17) | z;
+  idx[t] = (uint32_t)t;
+}
+
+__global__ __launch_bounds__(256) void code_change_kernel(long long n, const uint64_t* __restrict__ codes,
+                                                          uint32_t* __restrict__ flags) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  flags[p] = (p == 0 || codes[p] != codes[p - 1]) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void link_write_kernel(long long n, const uint64_t* __restrict__ codes,
+                                                         const uint32_t* __restrict__ idx,
+                                                         const uint32_t* __restrict__ flags,
+                                                         const uint32_t* __restrict__ rank_incl,
+                                                         int64_t* __restrict__ vox_key,
+                                                         int64_t* __restrict__ grid_key) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const long long r = (long long)rank_incl[p] - 1;
+  vox_key[idx[p]] = r;
+  if (flags[p]) {
+    const uint64_t c = codes[p];
+    grid_key[3 * r] = (int64_t)(c >> 34);
+    grid_key[3 * r + 1] = (int64_t)((c >> 17) & 0x1FFFFu);
+    grid_key[3 * r + 2] = (int64_t)(c & 0x1FFFFu);
+  }
+}
+
+hipError_t launch_grid_pts_link(long long N, const int64_t* octpath, const int8_t* octlevel, uint64_t* keys,
+                                uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint32_t* hist,
+                                uint32_t* sort_partial, uint32_t* scan_partial, uint32_t* flags, uint32_t* rank,
+                                int64_t* vox_key, int64_t* grid_key, uint32_t* n_unique, hipStream_t st) {
+  const long long n = 8 * N;
+  if (n <= 0) return hipSuccess;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  corner_codes_kernel<<<nb, 256, 0, st>>>(n, octpath, octlevel, keys, vals);
+  bool in_alt = false;
+  hipError_t e = launch_radix_sort(keys, vals, keys_alt, vals_alt, nullptr, n, 51, hist, sort_partial, &in_alt, st);
+  if (e != hipSuccess) return e;
+  const uint64_t* ks = in_alt ? keys_alt : keys;
+  const uint32_t* vs = in_alt ? vals_alt : vals;
+  code_change_kernel<<<nb, 256, 0, st>>>(n, ks, flags);
+  e = launch_scan(flags, nullptr, n, scan_partial, rank, true, st);
+  if (e != hipSuccess) return e;
+  link_write_kernel<<<nb, 256, 0, st>>>(n, ks, vs, flags, rank, vox_key, grid_key);
+  e = hipMemcpyAsync(n_unique, rank + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
 }  // namespace svr

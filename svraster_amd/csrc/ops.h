@@ -42,6 +42,8 @@ hipError_t launch_adam(bool sparse, bool biased, long long n, float* p, const fl
                        float step_size, float b1, float b2, float rsqrt_bc2, float eps, hipStream_t st);
 hipError_t launch_adam_table(bool sparse, long long n, float* p, const float* g, float* m, float* v,
                              const AdamTable& T, hipStream_t st);
+hipError_t launch_grid_pts_mean(long long N, int C, const int64_t* vox_key, const float* vox_val, long long M,
+                                uint32_t* count, float* out, hipStream_t st);
 hipError_t launch_ijk_to_octpath(int n, const int64_t* ijk, const int8_t* lv, int64_t* out, hipStream_t st);
 hipError_t launch_octpath_to_ijk(int n, const int64_t* path, const int8_t* lv, int64_t* ijk, hipStream_t st);
 

@@ -201,6 +201,10 @@ hipError_t launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
                              const uint32_t* n_dev, long long n_cap, int nbits, uint32_t* hist,
                              uint32_t* scan_partial, bool* result_in_alt, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, long long n, uint32_t v, hipStream_t st);
+hipError_t launch_grid_pts_link(long long N, const int64_t* octpath, const int8_t* octlevel, uint64_t* keys,
+                                uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint32_t* hist,
+                                uint32_t* sort_partial, uint32_t* scan_partial, uint32_t* flags, uint32_t* rank,
+                                int64_t* vox_key, int64_t* grid_key, uint32_t* n_unique, hipStream_t st);
 hipError_t launch_tile_ranges(const uint64_t* keys, const uint32_t* n_dev, long long n_cap, int rank_bits,
                               uint2* ranges, hipStream_t st);
 int tile_sort_passes(long long n_tiles);

@@ -393,8 +393,53 @@ __global__ void octpath_to_ijk_kernel(int n, const int64_t* __restrict__ path_in
   ijk[3 * t] = i; ijk[3 * t + 1] = j; ijk[3 * t + 2] = k;
 }
 
+// Per-voxel corner values into the grid points, averaged over the voxels sharing each point
+// (agg_voxel_into_grid_pts, adaptive.py:336-360: index_reduce "mean", include_self False).
+// Sums and counts by atomics, then one division per point; points no voxel names are 0.
+__global__ __launch_bounds__(256) void grid_pts_accumulate_kernel(long long n, int C, const int64_t* __restrict__ vox_key,
+                                                                  const float* __restrict__ vox_val,
+                                                                  float* __restrict__ sum,
+                                                                  uint32_t* __restrict__ count) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const long long corner = t / C;
+  const int c = (int)(t - corner * C);
+  const long long key = vox_key[corner];
+  atomicAdd(sum + key * C + c, vox_val[t]);
+  if (c == 0) atomicAdd(count + key, 1u);
+}
+
+__global__ __launch_bounds__(256) void grid_pts_mean_kernel(long long n, int C, const uint32_t* __restrict__ count,
+                                                            float* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t k = count[t / C];
+  out[t] = k ? out[t] / (float)k : 0.f;
+}
+
+__global__ void fill_f32_kernel(long long n, float* __restrict__ p, float v) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+__global__ void fill_u32v_kernel(long long n, uint32_t* __restrict__ p, uint32_t v) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
 // ------------------------------------------------------------------------------------------
 static inline unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
+
+hipError_t launch_grid_pts_mean(long long N, int C, const int64_t* vox_key, const float* vox_val, long long M,
+                                uint32_t* count, float* out, hipStream_t st) {
+  if (M <= 0) return hipSuccess;
+  fill_f32_kernel<<<blocks_for(M * C), 256, 0, st>>>(M * C, out, 0.f);
+  fill_u32v_kernel<<<blocks_for(M), 256, 0, st>>>(M, count, 0u);
+  if (N > 0)
+    grid_pts_accumulate_kernel<<<blocks_for(8 * N * C), 256, 0, st>>>(8 * N * C, C, vox_key, vox_val, out, count);
+  grid_pts_mean_kernel<<<blocks_for(M * C), 256, 0, st>>>(M * C, C, count, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_sh_forward(int n, const int64_t* idx, int deg, int M, const float* centers,
                              const float* cam, const float* sh0, const float* shs, float* rgbs,

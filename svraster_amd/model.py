@@ -23,7 +23,11 @@ class VoxelModel:
         self.vox_size = scene.vox_size
         self.vox_size_inv = 1.0 / scene.vox_size
         self.vox_key = scene.vox_key
+        self.grid_pts_key = scene.grid_pts_key
         self.is_leaf = scene.is_leaf
+        self.scene_center = scene.scene_center
+        self.scene_extent = scene.scene_extent
+        self.outside_level = 0
         self.active_sh_degree = scene.sh_degree
         self._geo_grid_pts = scene.geo_grid_pts.clone().requires_grad_(True)
         self._sh0 = scene.sh0.clone().requires_grad_(True)

@@ -225,3 +225,30 @@ def test_points_loss_finite_differences_and_skips():
     assert np.allclose(got, fd, atol=1e-6)
     only3 = ogl.pl_grad(pts[:3], x, vk, coord, vsize, res, mask, keys, owners, 0.4)
     assert np.array_equal(got, only3)
+
+
+def test_octree_topology_oracle_pins():
+    rng = np.random.default_rng(4)
+    lv = rng.integers(1, 6, size=40)
+    ijk = np.stack([rng.integers(0, 1 << l, size=3) for l in lv])
+    path = np.array([oot.ijk_to_octpath(int(a), int(b), int(c), int(l)) for (a, b, c), l in zip(ijk, lv)])
+    # children tile the parent: their ijk at level + 1 are 2 * parent + {0,1}^3, octant order
+    cp, cl = oot.gen_children(path, lv)
+    cijk = oot.ijk_array(cp, cl).reshape(-1, 8, 3)
+    offs = np.array([[(c >> 2) & 1, (c >> 1) & 1, c & 1] for c in range(8)])
+    assert np.array_equal(cijk, 2 * ijk[:, None, :] + offs[None])
+    assert np.array_equal(cl, np.repeat(lv + 1, 8))
+    # the link reproduces every voxel's corner coordinates, keys unique and sorted
+    key, vk = oot.build_grid_pts_link(path, lv)
+    assert np.array_equal(key[vk], oot.corner_coords(path, lv))
+    assert np.all(np.diff(key[:, 0] * (1 << 40) + key[:, 1] * (1 << 20) + key[:, 2]) > 0)
+    # trilinear subdivision of a linear field's corner values = the field at the children's corners
+    a = rng.normal(size=3)
+    field = lambda x: x @ a + 0.7
+    par = oot.corner_coords(path, lv).astype(np.float64)
+    kids = oot.corner_coords(cp, cl).astype(np.float64)
+    sub = oot.subdivide_by_interp(field(par))
+    assert np.allclose(sub, field(kids), rtol=1e-12, atol=1e-9)
+    # the mean of a constant is the constant; unnamed points are 0
+    m = oot.agg_mean(key.shape[0] + 3, vk, np.full(vk.shape, 2.5))
+    assert np.all(m[:key.shape[0]] == 2.5) and np.all(m[key.shape[0]:] == 0.0)
