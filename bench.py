@@ -344,7 +344,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); the multi-GPU path is "
-              f"design-only in this build.", file=sys.stderr)
+              f"design-only here (one process per GPU is not built).", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "train_16":

@@ -237,7 +237,7 @@ def unpack_BinningState(K, binningBuffer):
 
 
 def filter_geomState(*args, **kwargs):
-    raise NotImplementedError("filter_geomState is a debug helper not provided by this build")
+    raise NotImplementedError("filter_geomState is a debug helper not provided by this library")
 
 
 # ----------------------------------------------------------------------------------------------
@@ -463,16 +463,16 @@ def valid_gradient_table(vox_center, vox_size, scene_center, inside_extent, grid
 
 
 def is_in_cone(*args, **kwargs):
-    raise NotImplementedError("is_in_cone is not provided by this build")
+    raise NotImplementedError("is_in_cone is not provided by this library")
 
 
 def compute_rd(*args, **kwargs):
-    raise NotImplementedError("compute_rd is not provided by this build")
+    raise NotImplementedError("compute_rd is not provided by this library")
 
 
 def depth2pts(*args, **kwargs):
-    raise NotImplementedError("depth2pts is not provided by this build")
+    raise NotImplementedError("depth2pts is not provided by this library")
 
 
 def voxel_order_rank(*args, **kwargs):
-    raise NotImplementedError("voxel_order_rank is not provided by this build")
+    raise NotImplementedError("voxel_order_rank is not provided by this library")

@@ -5,7 +5,7 @@ build; the names exist so that `import svraster_cuda` succeeds, and calling them
 
 def _unavailable(name):
     def fn(*args, **kwargs):
-        raise NotImplementedError(f"{name}: mesh extraction is not provided by this build")
+        raise NotImplementedError(f"{name}: mesh extraction is not provided by this library")
     fn.__name__ = name
     return fn
 
