@@ -197,7 +197,7 @@ int svr_get_option(const char* name) {
 }
 
 size_t svr_geom_state_bytes(int n_views, int P) { return GeomLayout(n_views, P).total; }
-size_t svr_binning_state_bytes(long long K) { return BinLayout(K).total; }
+size_t svr_binning_state_bytes(long long K) { return raster_bin_bytes(K); }
 size_t svr_image_state_bytes(int n_views, long long n_pixels, long long n_tiles) {
   return ImageLayout(n_views, n_pixels, n_tiles).total;
 }
@@ -290,7 +290,8 @@ int svr_rasterize_forward(int vox_geo_mode, int density_mode, int W, int H, floa
   a.tile_last = ip.tile_last; a.n_contrib = ip.n_contrib;
   a.color = out_color; a.depth = out_depth; a.normal = out_normal; a.T = out_T; a.max_w = max_w;
   a.work_counter = nullptr;
-  a.cbits = bp.cbits; a.k_cap = K;
+  const ContribPtrs cp((char*)binning_buffer + BinLayout(K).total, K);
+  a.cvals = cp.cvals; a.pmask = cp.pmask; a.cranges = ip.cranges;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;
   a.halves = g_batch_depth / 64;
@@ -344,9 +345,8 @@ int svr_rasterize_backward(long long K, int vox_geo_mode, int density_mode, int 
   a.variant = g_compositor_variant;
   a.halves = g_batch_depth / 64;
   a.blocks_per_cu = g_blocks_per_cu;
-  SVR_TRY(launch_contrib_compact(bp.cbits, K, bp.cscan, bp.cpart, bp.cpos, ip.ranges, ip.tile_last, n_tiles,
-                                 ip.cranges, st));
-  a.cpos = bp.cpos; a.cranges = ip.cranges;
+  const ContribPtrs cp((char*)binning_buffer + BinLayout(K).total, K);
+  a.cvals = cp.cvals; a.pmask = cp.pmask; a.cranges = ip.cranges;
   {
     LaunchTimer timer(1, st);
     SVR_TRY(launch_render_backward(a, st));
@@ -642,13 +642,15 @@ namespace {
 struct BatchPtrs {
   GeomPtrs g;
   BinPtrs b;
+  ContribPtrs c;
   ImagePtrs im;
   PairPtrs pr;
   BatchPtrs(void* ws, int V, int P, long long n_pix, long long n_tiles, long long cap)
       : g(ws, V, P),
         b((char*)ws + GeomLayout(V, P).total, cap),
-        im((char*)ws + GeomLayout(V, P).total + BinLayout(cap).total, V, n_pix, n_tiles),
-        pr((char*)ws + GeomLayout(V, P).total + BinLayout(cap).total + ImageLayout(V, n_pix, n_tiles).total, P, V) {}
+        c((char*)ws + GeomLayout(V, P).total + BinLayout(cap).total, cap),
+        im((char*)ws + GeomLayout(V, P).total + raster_bin_bytes(cap), V, n_pix, n_tiles),
+        pr((char*)ws + GeomLayout(V, P).total + raster_bin_bytes(cap) + ImageLayout(V, n_pix, n_tiles).total, P, V) {}
 };
 
 int check_batch(int V, int P, long long n_pix, long long n_tiles, long long cap) {
@@ -664,7 +666,7 @@ int check_batch(int V, int P, long long n_pix, long long n_tiles, long long cap)
 extern "C" {
 
 size_t svr_batch_workspace_bytes(int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap) {
-  return GeomLayout(n_views, P).total + BinLayout(K_cap).total + ImageLayout(n_views, n_pixels, n_tiles).total +
+  return GeomLayout(n_views, P).total + raster_bin_bytes(K_cap) + ImageLayout(n_views, n_pixels, n_tiles).total +
          PairLayout(P, n_views).total;
 }
 
@@ -784,7 +786,7 @@ int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long l
   a.tile_last = bp.im.tile_last; a.n_contrib = bp.im.n_contrib;
   a.color = out_color; a.depth = out_depth; a.normal = out_normal; a.T = out_T; a.max_w = nullptr;
   a.work_counter = bp.im.misc;
-  a.cbits = bp.b.cbits; a.k_cap = K_cap;
+  a.cvals = bp.c.cvals; a.pmask = bp.c.pmask; a.cranges = bp.im.cranges;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;
   a.halves = g_batch_depth / 64;
@@ -829,9 +831,7 @@ int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long lo
   a.variant = g_compositor_variant;
   a.halves = g_batch_depth / 64;
   a.blocks_per_cu = g_blocks_per_cu;
-  SVR_TRY(launch_contrib_compact(bp.b.cbits, K_cap, bp.b.cscan, bp.b.cpart, bp.b.cpos, bp.im.ranges, bp.im.tile_last,
-                                 n_tiles, bp.im.cranges, st));
-  a.cpos = bp.b.cpos; a.cranges = bp.im.cranges;
+  a.cvals = bp.c.cvals; a.pmask = bp.c.pmask; a.cranges = bp.im.cranges;
   {
     LaunchTimer timer(1, st);
     SVR_TRY(launch_render_backward(a, st));
@@ -879,11 +879,12 @@ int svr_batch_state_offsets(int n_views, int P, long long n_pixels, long long n_
   if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
   const size_t g = GeomLayout(n_views, P).total;
   const BinLayout bl(K_cap);
+  const size_t im = g + raster_bin_bytes(K_cap);
   const ImageLayout il(n_views, n_pixels, n_tiles);
-  out[0] = (long long)(g + bl.total + il.ranges);
+  out[0] = (long long)(im + il.ranges);
   out[1] = (long long)(g + bl.vals);
-  out[2] = (long long)(g + bl.total + il.n_contrib);
-  out[3] = (long long)(g + bl.total + il.tile_last);
+  out[2] = (long long)(im + il.n_contrib);
+  out[3] = (long long)(im + il.tile_last);
   return 0;
 }
 

@@ -501,48 +501,6 @@ __global__ __launch_bounds__(256) void pair_emit_kernel(
 }
 
 // ------------------------------------------------------------------------------------------
-// Compaction of the contributing entries marked by the forward (one bit per list position):
-// positions in list order and, per tile, its range in that compacted list.
-// ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void popc_words_kernel(const uint32_t* __restrict__ bits, long long nw,
-                                                         uint32_t* __restrict__ out) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nw) out[i] = (uint32_t)__popc(bits[i]);
-}
-
-__global__ __launch_bounds__(256) void contrib_positions_kernel(const uint32_t* __restrict__ bits,
-                                                                const uint32_t* __restrict__ cscan, long long nw,
-                                                                long long k_cap, uint32_t* __restrict__ cpos) {
-  const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= nw) return;
-  uint32_t b = bits[w];
-  uint32_t o = cscan[w];
-  while (b) {
-    const long long p = 32 * w + __builtin_ctz(b);
-    if (p < k_cap) cpos[o] = (uint32_t)p;
-    ++o;
-    b &= b - 1u;
-  }
-}
-
-__device__ inline uint32_t contrib_prefix(const uint32_t* __restrict__ bits, const uint32_t* __restrict__ cscan,
-                                          uint32_t p) {
-  const uint32_t w = p >> 5;
-  return cscan[w] + (uint32_t)__popc(bits[w] & ((1u << (p & 31)) - 1u));
-}
-
-__global__ __launch_bounds__(256) void contrib_ranges_kernel(const uint2* __restrict__ ranges,
-                                                             const uint32_t* __restrict__ tile_last, long long n_tiles,
-                                                             const uint32_t* __restrict__ bits,
-                                                             const uint32_t* __restrict__ cscan,
-                                                             uint2* __restrict__ cranges) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_tiles) return;
-  const uint32_t b = ranges[t].x, e = tile_last[t];
-  cranges[t] = e > b ? make_uint2(contrib_prefix(bits, cscan, b), contrib_prefix(bits, cscan, e)) : make_uint2(0u, 0u);
-}
-
-// ------------------------------------------------------------------------------------------
 // Host-side launch helpers (used by the C-ABI entry points).
 // ------------------------------------------------------------------------------------------
 hipError_t launch_init_view(ViewDesc* out, const ViewDesc& base, const float* w2c,
@@ -671,21 +629,6 @@ hipError_t launch_pair_emit(const ViewDesc* views, int V, int P, long long n_pai
   if (n_pairs <= 0) return hipSuccess;
   pair_emit_kernel<<<(unsigned)((n_pairs + 255) / 256), 256, 0, st>>>(views, V, P, n_pairs, order, cnt_ranked,
                                                                       scan_ranked, bboxes, vmask, tiles, vals, cap);
-  return hipGetLastError();
-}
-
-hipError_t launch_contrib_compact(const uint32_t* cbits, long long k_cap, uint32_t* cscan, uint32_t* cpart,
-                                  uint32_t* cpos, const uint2* ranges, const uint32_t* tile_last, long long n_tiles,
-                                  uint2* cranges, hipStream_t st) {
-  if (k_cap <= 0 || n_tiles <= 0) return hipSuccess;
-  const long long nw = contrib_words(k_cap);
-  const unsigned gw = (unsigned)((nw + 255) / 256);
-  popc_words_kernel<<<gw, 256, 0, st>>>(cbits, nw, cscan);
-  hipError_t e = launch_scan(cscan, nullptr, nw, cpart, cscan, false, st);
-  if (e != hipSuccess) return e;
-  contrib_positions_kernel<<<gw, 256, 0, st>>>(cbits, cscan, nw, k_cap, cpos);
-  contrib_ranges_kernel<<<(unsigned)((n_tiles + 255) / 256), 256, 0, st>>>(ranges, tile_last, n_tiles, cbits, cscan,
-                                                                           cranges);
   return hipGetLastError();
 }
 

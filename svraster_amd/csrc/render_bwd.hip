@@ -27,54 +27,18 @@ __device__ inline int find_view_b(const ViewDesc* views, int n_views, int g) {
   return lo;
 }
 
-__device__ inline void stage_entry_b(const BwdArgs& a, uint32_t e, int vi, bool need_normal, float4* s_ch,
-                                     float4* s_ml, float4* s_g0, float4* s_g1, float4* s_rgb, float4* s_nrm,
-                                     uint2* s_bb, int slot, bool cull = false, float s = 0.f,
-                                     float cam_scale = 0.f) {
-  const uint32_t val = a.vals[e];
-  const uint32_t vox = val & kVoxMask;
-  uint2 bb = a.bboxes[(long long)vi * a.P + vox];
-  const float cx = a.centers[3 * vox], cy = a.centers[3 * vox + 1], cz = a.centers[3 * vox + 2];
-  const float l = a.lengths[vox];
-  s_ch[slot] = make_float4(cx, cy, cz, 0.5f * l);
-  s_ml[slot] = make_float4(cx - 0.5f * l, cy - 0.5f * l, cz - 0.5f * l, 1.f / l);
-  const float4 g0 = reinterpret_cast<const float4*>(a.geos)[2 * vox];
-  const float4 g1 = reinterpret_cast<const float4*>(a.geos)[2 * vox + 1];
-  s_g0[slot] = g0;
-  s_g1[slot] = g1;
-  if (cull) {
-    // An entry that cannot reach the minimum alpha gets an empty bbox (no pixel masks).
-    const float g[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
-    const float ps = cam_scale + fmaxf(fabsf(cx), fmaxf(fabsf(cy), fabsf(cz)));
-    if (!voxel_may_contribute(g, s, ps, l)) bb = make_uint2(0xFFFFFFFFu, 0u);
-  }
-  s_bb[slot] = bb;
-  const float* rgbv = a.rgbs + (long long)vi * a.rgb_view_stride;
-  s_rgb[slot] = make_float4(rgbv[3 * vox], rgbv[3 * vox + 1], rgbv[3 * vox + 2], __uint_as_float(val));
-  if (need_normal) {
-    const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
-    float nx, ny, nz, rn;
-    lin_normal(gg, nx, ny, nz, rn);
-    s_nrm[slot] = make_float4(nx * rn, ny * rn, nz * rn, rn);
-  }
-}
-
 // One staged entry held in registers between its global loads and its LDS store, so the loads
 // of the next batch are in flight while the current batch is processed.
 struct EntryRegsB {
-  uint32_t pos, val;
-  uint2 bb;
+  uint32_t val;
   float cx, cy, cz, l;
   float4 g0, g1;
   float r, g, b;
 };
 
-__device__ inline void fetch_entry_b(EntryRegsB& e, const BwdArgs& a, uint32_t pos, uint32_t val, int vi,
-                                     const float* __restrict__ rgbs) {
+__device__ inline void fetch_entry_b(EntryRegsB& e, const BwdArgs& a, uint32_t val, const float* __restrict__ rgbs) {
   const uint32_t vox = val & kVoxMask;
-  e.pos = pos;
   e.val = val;
-  e.bb = a.bboxes[(long long)vi * a.P + vox];
   e.cx = a.centers[3 * vox]; e.cy = a.centers[3 * vox + 1]; e.cz = a.centers[3 * vox + 2];
   e.l = a.lengths[vox];
   e.g0 = reinterpret_cast<const float4*>(a.geos)[2 * vox];
@@ -82,46 +46,24 @@ __device__ inline void fetch_entry_b(EntryRegsB& e, const BwdArgs& a, uint32_t p
   e.r = rgbs[3 * vox]; e.g = rgbs[3 * vox + 1]; e.b = rgbs[3 * vox + 2];
 }
 
-__device__ inline void store_entry_b(const EntryRegsB& e, bool need_normal, float s, float cam_scale, float4* s_ch,
-                                     float4* s_ml, float4* s_g0, float4* s_g1, float4* s_rgb, float4* s_nrm,
-                                     uint2* s_bb, int slot) {
+__device__ inline void store_entry_b(const EntryRegsB& e, bool need_normal, float4* s_ch, float4* s_ml, float4* s_g0,
+                                     float4* s_g1, float4* s_rgb, float4* s_nrm, int slot) {
   s_ch[slot] = make_float4(e.cx, e.cy, e.cz, 0.5f * e.l);
   s_ml[slot] = make_float4(e.cx - 0.5f * e.l, e.cy - 0.5f * e.l, e.cz - 0.5f * e.l, 1.f / e.l);
   s_g0[slot] = e.g0;
   s_g1[slot] = e.g1;
-  const float g[8] = {e.g0.x, e.g0.y, e.g0.z, e.g0.w, e.g1.x, e.g1.y, e.g1.z, e.g1.w};
-  const float ps = cam_scale + fmaxf(fabsf(e.cx), fmaxf(fabsf(e.cy), fabsf(e.cz)));
-  s_bb[slot] = voxel_may_contribute(g, s, ps, e.l) ? e.bb : make_uint2(0xFFFFFFFFu, 0u);
   s_rgb[slot] = make_float4(e.r, e.g, e.b, __uint_as_float(e.val));
   if (need_normal) {
+    const float g[8] = {e.g0.x, e.g0.y, e.g0.z, e.g0.w, e.g1.x, e.g1.y, e.g1.z, e.g1.w};
     float nx, ny, nz, rn;
     lin_normal(g, nx, ny, nz, rn);
     s_nrm[slot] = make_float4(nx * rn, ny * rn, nz * rn, rn);
   }
 }
 
-__device__ inline uint64_t bbox_block_mask_b(uint32_t b0, uint32_t b1, int qx, int qy) {
-  const int x0 = max((int)(b0 >> 16) - qx, 0), x1 = min((int)(b1 >> 16) - qx, 7);
-  const int y0 = max((int)(b0 & 0xFFFFu) - qy, 0), y1 = min((int)(b1 & 0xFFFFu) - qy, 7);
-  if (x0 > x1 || y0 > y1) return 0ull;
-  const uint64_t cols = (uint64_t)(((1u << (x1 + 1)) - 1u) & ~((1u << x0) - 1u));
-  const uint64_t hi = (y1 == 7) ? ~0ull : ((1ull << (8 * (y1 + 1))) - 1ull);
-  const uint64_t lo = (1ull << (8 * y0)) - 1ull;
-  return cols * (0x0101010101010101ull & hi & ~lo);
-}
-
 __device__ inline void wave_lds_sync_b() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
-}
-
-// Ray of another pixel of the wave: same origin, direction from LDS, inverse recomputed
-// exactly as pixel_ray computes it.
-__device__ inline Ray lane_ray(const Ray& r, const float4 d) {
-  Ray pr = r;
-  pr.dx = d.x; pr.dy = d.y; pr.dz = d.z;
-  pr.ix = 1.f / d.x; pr.iy = 1.f / d.y; pr.iz = 1.f / d.z;
-  return pr;
 }
 
 template <bool kDepth, bool kNormal, bool kConcen>
@@ -359,69 +301,45 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Pixel-major pair backward (default).  Per staged batch (entries in back-to-front order), each
-// wave:
-//   1. builds the pixel masks of the batch's entries (bbox, octant; empty for entries that
-//      cannot reach the minimum alpha), transposes them to per-pixel entry sets and drops the
-//      entries at or past each pixel's last contributor; pixel l owns the run
-//      [off_l, off_l + cnt_l) of a pixel-major pair list;
+// Pair backward (default).  The forward hands over, per tile, its contributing entries (those
+// that composited into at least one pixel) and, per entry and 8x8 block, the pixels that
+// composited it.  Those (pixel, entry) pairs are exactly the ones the reverse walk visits: alpha
+// at least the minimum, list position before the pixel's last contributor.  Per staged batch
+// (entries in back-to-front order), each wave:
+//   1. reads its block's pixel masks of the batch's entries and prefix-sums their counts: entry j
+//      owns the run [pref_j, pref_j + |m_j|) of an entry-major pair list; the transposed masks
+//      are each pixel's entry set;
 //   2. evaluates alpha (and the depth value) of every pair, 64 per instruction;
-//   3. walks each pixel's run in back-to-front order: T <- T / (1 - a), dL/dalpha and the
-//      running dL/dT -- the only sequential part, a few FMAs per pair;
-//   4. compacts the contributing pairs and computes their SDF-segment derivatives, 64 per
-//      instruction, accumulated per entry in LDS by all four waves; after the batch, one global
-//      atomic per (entry, gradient slot).
+//   3. walks each pixel's set in back-to-front order: T <- T / (1 - a), dL/dalpha and the running
+//      dL/dT -- the only sequential part, a few FMAs per pair;
+//   4. computes the SDF-segment derivatives of every pair, 64 per instruction, summed over each
+//      entry's run (segmented DPP sum) into per-entry LDS accumulators shared by the four waves;
+//      after the batch, one global atomic per (entry, gradient slot).
 // The per-voxel global atomics fall from 12 per (pixel, entry) in the reference to 12 per
 // (tile, entry).
 // ------------------------------------------------------------------------------------------
-constexpr int kBwdStage = 128;
 constexpr int kBwdWin = 128;
-
-__device__ inline int pop_entry_b(uint64_t& lo, uint64_t& hi) {
-  if (lo) {
-    const int j = __builtin_ctzll(lo);
-    lo &= lo - 1ull;
-    return j;
-  }
-  const int j = 64 + __builtin_ctzll(hi);
-  hi &= hi - 1ull;
-  return j;
-}
-
-__device__ inline uint64_t entry_block_mask_b(const uint2* s_bb, const float4* s_rgb, int j, int qx, int qy,
-                                              const uint64_t* s_om) {
-  const uint2 bb = s_bb[j];
-  const uint32_t q = __float_as_uint(s_rgb[j].w) >> 29;
-  return bbox_block_mask_b(bb.x, bb.y, qx, qy) & s_om[q];
-}
 
 template <bool kDepth, bool kNormal, bool kConcen, bool kQueue, int kHalves>
 __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
-  constexpr int kStage = 64 * kHalves;   // entries staged per round (kStage or half of it)
+  constexpr int kStage = 64 * kHalves;   // entries staged per round (two 64-entry halves or one)
   __shared__ float4 s_ch[kStage], s_ml[kStage], s_g0[kStage], s_g1[kStage], s_rgb[kStage];
   __shared__ float4 s_nrm[kNormal ? kStage : 1];
-  __shared__ uint2 s_bb[kStage];
   __shared__ float s_acc[kStage][12];
   __shared__ uint32_t s_used[kStage];
-  __shared__ uint64_t s_om[4][8];          // pixels of the block whose ray has octant q
   __shared__ float4 s_rd[4][64];           // ray direction xyz, dL/ddepth
   __shared__ float4 s_ri[4][64];           // 1/direction xyz
   __shared__ float4 s_dc[4][64];           // dL/dcolor xyz
   __shared__ float4 s_dn[4][kNormal ? 64 : 1];
   __shared__ float4 s_gt[4][kConcen ? 64 : 1];
-  __shared__ uint64_t s_m[4][kStage];   // per entry: its pixels in the block
-  __shared__ uint16_t s_pref[4][kStage];  // per entry: first position of its run
-  __shared__ uint32_t s_rel[kStage];    // per slot: list position relative to the tile start
-  __shared__ uint32_t s_lastpix[4][64];    // per pixel: contributor count (list positions it takes: < it)
-  __shared__ unsigned long long s_cs[4][2][64];  // per pixel: contributing entries (lo, hi halves)
-  __shared__ uint16_t s_own[4][kBwdWin];   // 1 + entry whose run starts at the position; later the
-                                           // contributing hits in entry-major order
-  __shared__ uint16_t s_hit[4][kBwdWin];   // positions whose ray hits the box
-  __shared__ uint16_t s_hd[4][kBwdWin];    // their entry | pixel << 7
-  __shared__ float s_pa[4][kBwdWin];       // alpha per position (0 on a miss)
+  __shared__ uint64_t s_m[4][kStage];      // per entry: the block's pixels it contributed to
+  __shared__ uint16_t s_pref[4][kStage];   // per entry: first position of its run
+  __shared__ uint8_t s_own[4][kBwdWin];    // 1 + entry whose run starts at the position, else 0
+  __shared__ uint16_t s_hd[4][kBwdWin];    // entry | pixel << 7 per position
+  __shared__ float s_pa[4][kBwdWin];       // alpha per position
   __shared__ float s_pd[4][kDepth ? kBwdWin : 1];
-  __shared__ float s_pl[4][kBwdWin];       // dL/dalpha per contributing position
-  __shared__ float s_pt[4][kBwdWin];       // T before the entry per contributing position
+  __shared__ float s_pl[4][kBwdWin];       // dL/dalpha per position
+  __shared__ float s_pt[4][kBwdWin];       // T before the entry per position
   __shared__ float s_red[4];
   __shared__ int s_tile;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -451,22 +369,18 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
     const int px = qx + (lane & 7), py = qy + (lane >> 3);
     const bool inside = px < W && py < H;
     const Ray r = pixel_ray(V, px, py);
-    const float cam_scale = fmaxf(fabsf(r.ox), fmaxf(fabsf(r.oy), fabsf(r.oz)));
-    const uint32_t start = a.ranges[g].x;
-    // The tile's entries that contributed to any of its pixels in the forward (the others get no
-    // gradient), as positions in the compacted list, walked back to front.
+    // The tile's contributing entries: cvals / pmask [cr.x, cr.y), walked back to front.
     const uint2 cr = a.cranges[g];
     const int n = (int)(cr.y - cr.x);
+    const long long top = (long long)cr.y - 1;   // entry of batch slot 0 in the first batch
 
     const long long HW = (long long)W * H;
     const long long pid = (long long)py * W + px;
     const long long pb = V.pix_base;
     float T = 0.f, dC0 = 0.f, dC1 = 0.f, dC2 = 0.f, last_dT = 0.f, dD = 0.f;
     float dN0 = 0.f, dN1 = 0.f, dN2 = 0.f, gt0 = 0.f, gt1 = 0.f, gt2 = 0.f;
-    uint32_t last = 0;
     if (inside) {
       T = a.T[pb + pid];
-      last = a.n_contrib[pb + pid];
       dC0 = a.dL_dcolor[pb * 3 + 0 * HW + pid];
       dC1 = a.dL_dcolor[pb * 3 + 1 * HW + pid];
       dC2 = a.dL_dcolor[pb * 3 + 2 * HW + pid];
@@ -489,61 +403,40 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
     s_dc[w][lane] = make_float4(dC0, dC1, dC2, 0.f);
     if (kNormal) s_dn[w][lane] = make_float4(dN0, dN1, dN2, 0.f);
     if (kConcen) s_gt[w][lane] = make_float4(gt0, gt1, gt2, 0.f);
-    {
-      uint64_t om = 0ull;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const uint64_t b = __ballot(inside && r.octant == (uint32_t)q);
-        om = (lane == q) ? b : om;
-      }
-      if (lane < 8) s_om[w][lane] = om;
-    }
 
-    // Pipelined staging: entry data one batch ahead in registers, its value two batches ahead of
-    // use, its list position three batches ahead.
+    // Pipelined staging: entry data one batch ahead in registers, its value two batches ahead;
+    // each wave's block masks of the next batch (lane: slots lane and lane + 64).
     const float* rgbs_v = a.rgbs + (long long)vi * a.rgb_view_stride;
     EntryRegsB er;
-    uint32_t nval = 0, npos = 0;
+    uint32_t nval = 0;
     if (t < kStage) {
-      if (t < n) {
-        const uint32_t p0 = a.cpos[cr.y - 1 - (uint32_t)t];
-        fetch_entry_b(er, a, p0, a.vals[p0], vi, rgbs_v);
-      }
-      if (kStage + t < n) {
-        npos = a.cpos[cr.y - 1 - (uint32_t)(kStage + t)];
-        nval = a.vals[npos];
-      }
+      if (t < n) fetch_entry_b(er, a, a.cvals[top - t], rgbs_v);
+      if (kStage + t < n) nval = a.cvals[top - kStage - t];
     }
-    uint32_t npos2 = (t < kStage && 2 * kStage + t < n) ? a.cpos[cr.y - 1 - (uint32_t)(2 * kStage + t)] : 0u;
+    uint64_t nm_lo = lane < n ? a.pmask[4 * (top - lane) + w] : 0ull;
+    uint64_t nm_hi = (kHalves > 1 && 64 + lane < n) ? a.pmask[4 * (top - 64 - lane) + w] : 0ull;
 
     for (int base = 0; base < n; base += kStage) {
       __syncthreads();
       const int cnt = min(kStage, n - base);
       if (t < cnt) {
-        store_entry_b(er, kNormal, s, cam_scale, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t);
-        s_rel[t] = er.pos - start;
+        store_entry_b(er, kNormal, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, t);
         s_used[t] = 0u;
       }
       for (int k = t; k < kStage * 12; k += 256) (&s_acc[0][0])[k] = 0.f;
       __syncthreads();
       if (t < kStage) {
-        if (base + kStage + t < n) fetch_entry_b(er, a, npos, nval, vi, rgbs_v);
-        if (base + 2 * kStage + t < n) {
-          npos = npos2;
-          nval = a.vals[npos];
-        }
-        if (base + 3 * kStage + t < n) npos2 = a.cpos[cr.y - 1 - (uint32_t)(base + 3 * kStage + t)];
+        if (base + kStage + t < n) fetch_entry_b(er, a, nval, rgbs_v);
+        if (base + 2 * kStage + t < n) nval = a.cvals[top - (base + 2 * kStage + t)];
       }
-      // Slot j holds list position s_rel[j] (decreasing in j); a pixel takes the slots with
-      // s_rel[j] < last, its contributor count.
-      const bool live = inside && last > s_rel[cnt - 1];
-      const uint64_t live_m = __ballot(live);
-      if (live_m != 0ull) {
-        s_lastpix[w][lane] = last;
-        const uint64_t m_lo = lane < cnt ? entry_block_mask_b(s_bb, s_rgb, lane, qx, qy, s_om[w]) & live_m : 0ull;
-        const uint64_t m_hi = (kHalves > 1 && lane + 64 < cnt)
-                                  ? entry_block_mask_b(s_bb, s_rgb, lane + 64, qx, qy, s_om[w]) & live_m : 0ull;
-        const uint32_t c_lo = (uint32_t)__popcll(m_lo), c_hi = (uint32_t)__popcll(m_hi);
+      const uint64_t m_lo = nm_lo, m_hi = nm_hi;
+      {
+        const int nb = base + kStage;
+        nm_lo = nb + lane < n ? a.pmask[4 * (top - nb - lane) + w] : 0ull;
+        nm_hi = (kHalves > 1 && nb + 64 + lane < n) ? a.pmask[4 * (top - nb - 64 - lane) + w] : 0ull;
+      }
+      const uint32_t c_lo = (uint32_t)__popcll(m_lo), c_hi = (uint32_t)__popcll(m_hi);
+      if (__ballot((c_lo | c_hi) != 0u) != 0ull) {
         const uint32_t i_lo = wave_incl_scan_u32(c_lo);
         const uint32_t i_hi = kHalves > 1 ? wave_incl_scan_u32(c_hi) : 0u;
         const uint32_t t_lo = (uint32_t)__builtin_amdgcn_readlane((int)i_lo, 63);
@@ -555,121 +448,81 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
           s_m[w][lane + 64] = m_hi;
           s_pref[w][lane + 64] = (uint16_t)p_hi;
         }
+        // This pixel's entries of the batch (ascending slot = back to front along the list).
+        uint64_t cs_lo = transpose64(m_lo, lane);
+        uint64_t cs_hi = kHalves > 1 ? transpose64(m_hi, lane) : 0ull;
         uint32_t carry = 0;   // 1 + entry owning the last position handled so far
 
         for (uint32_t w0 = 0; w0 < total; w0 += kBwdWin) {
           const uint32_t w1 = min(total, w0 + (uint32_t)kBwdWin);
           const uint32_t np = w1 - w0;
           for (uint32_t k = lane; k < np; k += 64) s_own[w][k] = 0;
-          s_cs[w][0][lane] = 0ull;
-          s_cs[w][1][lane] = 0ull;
           wave_lds_sync_b();
-          if (c_lo && p_lo >= w0 && p_lo < w1) s_own[w][p_lo - w0] = (uint16_t)(lane + 1);
-          if (c_hi && p_hi >= w0 && p_hi < w1) s_own[w][p_hi - w0] = (uint16_t)(lane + 65);
+          if (c_lo && p_lo >= w0 && p_lo < w1) s_own[w][p_lo - w0] = (uint8_t)(lane + 1);
+          if (c_hi && p_hi >= w0 && p_hi < w1) s_own[w][p_hi - w0] = (uint8_t)(lane + 65);
           wave_lds_sync_b();
 
-          // 2a. ray/box test per position (pairs at or past the pixel's last contributor count as
-          // misses); hits listed with their (entry, pixel).
-          uint32_t nh = 0;
+          // 2. alpha (and depth value) per position.
           for (uint32_t c0 = 0; c0 < np; c0 += 64) {
             const uint32_t k = c0 + lane;
             const bool valid = k < np;
             uint32_t h = valid ? (uint32_t)s_own[w][k] : 0u;
             h = max(wave_incl_max_u32(h), carry);
             carry = (uint32_t)__builtin_amdgcn_readlane((int)h, 63);
-            bool hit = false;
-            uint32_t desc = 0;
             if (valid) {
               const int j = (int)h - 1;
               const int bit = select_set_bit(s_m[w][j], (int)(w0 + k - (uint32_t)s_pref[w][j]));
-              if (s_rel[j] < s_lastpix[w][bit]) {
-                const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
-                const float4 ch = s_ch[j];
-                float ta, tb;
-                ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
-                hit = !(ta > tb);
+              const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
+              const float4 ch = s_ch[j];
+              float ta, tb;
+              ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
+              const float4 ml = s_ml[j];
+              const float4 g0 = s_g0[j], g1 = s_g1[j];
+              const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+              Segment seg;
+              eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
+              s_pa[w][k] = seg.alpha;
+              if (kDepth) {
+                float sub[3];
+                sub_alphas(seg, sub);
+                s_pd[w][k] = depth_value(sub, ta, tb);
               }
-              desc = (uint32_t)j | ((uint32_t)bit << 7);
-              s_pa[w][k] = 0.f;
+              s_hd[w][k] = (uint16_t)((uint32_t)j | ((uint32_t)bit << 7));
             }
-            const uint64_t bal = __ballot(hit);
-            if (hit) {
-              const uint32_t idx = nh + (uint32_t)__popcll(bal & lt_mask);
-              s_hit[w][idx] = (uint16_t)k;
-              s_hd[w][idx] = (uint16_t)desc;
-            }
-            nh += (uint32_t)__popcll(bal);
-          }
-          wave_lds_sync_b();
-          // 2b. alpha (and depth value) per hit; contributing pairs into the pixel's entry set.
-          for (uint32_t h = lane; h < nh; h += 64) {
-            const uint32_t k = s_hit[w][h];
-            const uint32_t d = s_hd[w][h];
-            const int j = (int)(d & 127u), bit = (int)(d >> 7);
-            const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
-            const float4 ch = s_ch[j];
-            float ta, tb;
-            ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
-            const float4 ml = s_ml[j];
-            const float4 g0 = s_g0[j], g1 = s_g1[j];
-            const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
-            Segment seg;
-            eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
-            s_pa[w][k] = seg.alpha;
-            if (kDepth) {
-              float sub[3];
-              sub_alphas(seg, sub);
-              s_pd[w][k] = depth_value(sub, ta, tb);
-            }
-            if (seg.alpha >= kMinAlpha) atomicOr(&s_cs[w][j >> 6][bit], 1ull << (j & 63));
           }
           wave_lds_sync_b();
 
-          // 3. sequential reverse walk per pixel over its contributing entries (ascending slot =
-          // back to front along the list).
-          {
-            uint64_t cs_lo = s_cs[w][0][lane], cs_hi = s_cs[w][1][lane];
-            while ((cs_lo | cs_hi) != 0ull) {
-              const int j = pop_entry_b(cs_lo, cs_hi);
-              const uint32_t k = (uint32_t)s_pref[w][j] + (uint32_t)__popcll(s_m[w][j] & lt_mask) - w0;
-              const float alpha = s_pa[w][k];
-              T = T / (1.f - alpha);
-              const float4 rgb = s_rgb[j];
-              float dptw = dC0 * rgb.x + dC1 * rgb.y + dC2 * rgb.z;
-              if (kNormal) {
-                const float4 nn = s_nrm[j];
-                dptw += dN0 * nn.x + dN1 * nn.y + dN2 * nn.z;
-              }
-              const float dLda = T * (dptw - last_dT);
-              last_dT += alpha * (dptw - last_dT);
-              if (kDepth) last_dT += dD * s_pd[w][k];
-              s_pl[w][k] = dLda;
-              s_pt[w][k] = T;
+          // 3. sequential reverse walk per pixel over its entries whose pair lies in the window.
+          while ((cs_lo | cs_hi) != 0ull) {
+            const int j = cs_lo ? __builtin_ctzll(cs_lo) : 64 + __builtin_ctzll(cs_hi);
+            const uint32_t k = (uint32_t)s_pref[w][j] + (uint32_t)__popcll(s_m[w][j] & lt_mask);
+            if (k >= w1) break;
+            if (cs_lo) cs_lo &= cs_lo - 1ull; else cs_hi &= cs_hi - 1ull;
+            const uint32_t kk = k - w0;
+            const float alpha = s_pa[w][kk];
+            T = T / (1.f - alpha);
+            const float4 rgb = s_rgb[j];
+            float dptw = dC0 * rgb.x + dC1 * rgb.y + dC2 * rgb.z;
+            if (kNormal) {
+              const float4 nn = s_nrm[j];
+              dptw += dN0 * nn.x + dN1 * nn.y + dN2 * nn.z;
             }
-          }
-          // 4a. contributing hits in entry-major order (a subsequence of the hit list).
-          uint32_t nc = 0;
-          for (uint32_t c0 = 0; c0 < nh; c0 += 64) {
-            const uint32_t h = c0 + lane;
-            const bool f = h < nh && s_pa[w][s_hit[w][h]] >= kMinAlpha;
-            const uint64_t bal = __ballot(f);
-            if (f) s_own[w][nc + (uint32_t)__popcll(bal & lt_mask)] = (uint16_t)h;
-            nc += (uint32_t)__popcll(bal);
+            const float dLda = T * (dptw - last_dT);
+            last_dT += alpha * (dptw - last_dT);
+            if (kDepth) last_dT += dD * s_pd[w][kk];
+            s_pl[w][kk] = dLda;
+            s_pt[w][kk] = T;
           }
           wave_lds_sync_b();
 
-          // 4b. per-pair derivatives, 64 pairs per instruction; each entry's run is summed across
+          // 4. per-pair derivatives, 64 pairs per instruction; each entry's run is summed across
           // lanes (segmented DPP sum) and its last lane adds the 12 sums to the workgroup's LDS
           // accumulators.
           uint32_t jprev = 0xFFFFFFFFu;   // entry of the previous chunk's last pair
-          for (uint32_t c0 = 0; c0 < nc; c0 += 64) {
-            const uint32_t i = c0 + lane;
-            const bool valid = i < nc;
-            uint32_t hh = 0, d = 0;
-            if (valid) {
-              hh = s_own[w][i];
-              d = s_hd[w][hh];
-            }
+          for (uint32_t c0 = 0; c0 < np; c0 += 64) {
+            const uint32_t k = c0 + lane;
+            const bool valid = k < np;
+            const uint32_t d = valid ? (uint32_t)s_hd[w][k] : 0u;
             const uint32_t jj = valid ? (d & 127u) : 0xFFFFFFFEu;
             uint32_t jp = (uint32_t)__shfl_up((int)jj, 1, 64);
             if (lane == 0) jp = jprev;
@@ -677,14 +530,13 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
             jprev = (uint32_t)__builtin_amdgcn_readlane((int)jj, 63);
             const bool head = jj != jp;
             const int seg_start = (int)wave_incl_max_u32(head ? (uint32_t)lane : 0u);
-            const bool tail = valid && (lane == 63 || i + 1 == nc || jn != jj);
+            const bool tail = valid && (lane == 63 || k + 1 == np || jn != jj);
             float gv[12];
 #pragma unroll
             for (int q = 0; q < 12; ++q) gv[q] = 0.f;
             const int j = (int)(d & 127u);
             if (valid) {
               const int bit = (int)(d >> 7);
-              const uint32_t k = s_hit[w][hh];
               const float Tp = s_pt[w][k];
               const float dLda = s_pl[w][k];
               const float4 rdd = s_rd[w][bit];
@@ -714,10 +566,10 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
                 const float step = kThird * (tb - ta);
                 const float t0 = ta + 0.5f * step, t1 = ta + 1.5f * step, t2 = ta + 2.5f * step;
                 const float a0 = sub[0], a1 = sub[1], a2 = sub[2];
-                const float kk = rdd.w * Tp;
-                const float gk[3] = {kk * (t0 + a1 * a2 * t2 - a1 * t1 - a2 * t2),
-                                     kk * (t1 + a0 * a2 * t2 - a0 * t1 - a2 * t2),
-                                     kk * (t2 + a0 * a1 * t2 - a0 * t2 - a1 * t2)};
+                const float kd = rdd.w * Tp;
+                const float gk[3] = {kd * (t0 + a1 * a2 * t2 - a1 * t1 - a2 * t2),
+                                     kd * (t1 + a0 * a2 * t2 - a0 * t1 - a2 * t2),
+                                     kd * (t2 + a0 * a1 * t2 - a0 * t2 - a1 * t2)};
 #pragma unroll
                 for (int mm = 0; mm < 3; ++mm) {
                   const float pk = seg.phi[mm], pk1 = seg.phi[mm + 1];

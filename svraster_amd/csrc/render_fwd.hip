@@ -202,14 +202,19 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
     float dmed_T = 0.f;
     bool done = !inside;
 
+    // Every staged entry is handed to the backward with its pixel masks (empty where no pixel of
+    // the block composited it): the tile's contributing range is [start, tile_last).
     for (int base = 0; base < n; base += 256) {
       if (__syncthreads_count(done) == 256) break;
-      if (base + t < n)
+      if (base + t < n) {
         stage_entry(a.vals, rg.x + base + t, vi, a.P, a.bboxes, a.centers, a.lengths, a.geos,
                     a.rgbs + (long long)vi * a.rgb_view_stride, kNormal, s_ch, s_ml, s_g0, s_g1, s_rgb,
                     s_nrm, s_bb, t);
+        a.cvals[rg.x + base + t] = a.vals[rg.x + base + t];
+      }
       __syncthreads();
       const int cnt = min(256, n - base);
+      uint64_t cm0 = 0ull, cm1 = 0ull, cm2 = 0ull, cm3 = 0ull;   // entries this pixel composited
       for (int j = 0; j < cnt; ++j) {
         if (__all(done)) break;
         if (done) continue;
@@ -243,7 +248,20 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
         done = T < kEarlyStopT;
         last = (uint32_t)(base + j + 1);
         if (kMaxW) atomicMax(reinterpret_cast<uint32_t*>(a.max_w) + (val & kVoxMask), __float_as_uint(wgt));
+        const uint64_t bit = 1ull << (j & 63);
+        const int q = j >> 6;   // wave-uniform
+        if (q == 0) cm0 |= bit;
+        else if (q == 1) cm1 |= bit;
+        else if (q == 2) cm2 |= bit;
+        else cm3 |= bit;
       }
+      const long long o = (long long)rg.x + base;
+      const uint64_t e0 = transpose64(cm0, lane), e1 = transpose64(cm1, lane);
+      const uint64_t e2 = transpose64(cm2, lane), e3 = transpose64(cm3, lane);
+      if (lane < cnt) a.pmask[4 * (o + lane) + w] = e0;
+      if (64 + lane < cnt) a.pmask[4 * (o + 64 + lane) + w] = e1;
+      if (128 + lane < cnt) a.pmask[4 * (o + 128 + lane) + w] = e2;
+      if (192 + lane < cnt) a.pmask[4 * (o + 192 + lane) + w] = e3;
     }
 
     float Dmed = 0.f;
@@ -253,7 +271,11 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
     const uint32_t tl = wave_max_u32(rg.x + (inside ? last : 0u));
     if (lane == 0) s_red[w] = tl;
     __syncthreads();
-    if (t == 0) a.tile_last[g] = max(max(s_red[0], s_red[1]), max(s_red[2], s_red[3]));
+    if (t == 0) {
+      const uint32_t tlast = max(max(s_red[0], s_red[1]), max(s_red[2], s_red[3]));
+      a.tile_last[g] = tlast;
+      a.cranges[g] = make_uint2(rg.x, tlast);
+    }
     __syncthreads();
   }
 }
@@ -315,33 +337,8 @@ __device__ inline int pop_entry(uint64_t& lo, uint64_t& hi) {
   return j;
 }
 
-// OR of a 128-bit entry set over the wave, added to the bit array at list position p0.
-__device__ inline void mark_contributing(uint32_t* bits, long long p0, uint64_t lo, uint64_t hi, int lane) {
-  uint32_t wv[4] = {(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    uint32_t x = wv[c];
-    x |= xor_lane<32>(x, lane);
-    x |= xor_lane<16>(x, lane);
-    x |= xor_lane<8>(x, lane);
-    x |= xor_lane<4>(x, lane);
-    x |= xor_lane<2>(x, lane);
-    x |= xor_lane<1>(x, lane);
-    wv[c] = x;
-  }
-  if (lane != 0) return;
-  const long long w0 = p0 >> 5;
-  const uint32_t sh = (uint32_t)(p0 & 31);
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    if (!wv[c]) continue;
-    atomicOr(bits + w0 + c, wv[c] << sh);
-    if (sh) atomicOr(bits + w0 + c + 1, wv[c] >> (32u - sh));
-  }
-}
-
 template <bool kDepth, bool kNormal, bool kMaxW, bool kQueue, int kHalves>
-__global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
+__global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(FwdArgs a) {
   constexpr int kStage = 64 * kHalves;   // entries staged per round (kStage or half of it)
   __shared__ float4 s_ch[kStage], s_ml[kStage], s_g0[kStage], s_g1[kStage], s_rgb[kStage];
   __shared__ float4 s_nrm[kNormal ? kStage : 1];
@@ -357,6 +354,7 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
   __shared__ uint16_t s_hd[4][kPairWin];      // their entry | pixel << 7
   __shared__ float s_pa[4][kPairWin];         // alpha per position (0 when the ray misses the box)
   __shared__ float s_pd[4][kDepth ? kPairWin : 1];
+  __shared__ uint64_t s_u[4][2];              // per wave: entries of the last round it composited
   __shared__ uint32_t s_red[4];
   __shared__ int s_tile;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -415,11 +413,37 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
       if (kStage + t < n) nval = a.vals[rg.x + kStage + t];
     }
 
-    for (int base = 0; base < n; base += kStage) {
-      if (__syncthreads_count(done) == 256) break;
+    // A round's contributing entries are known once every wave has finished it, so they are
+    // written at the top of the next round: packed after the tile's earlier ones (cc so far), each
+    // wave its block's pixel masks, the staging lane the value.  At the end of a round each wave
+    // leaves in s_m its masks (the block's pixels that composited each entry) and in s_u the
+    // entries whose mask is not empty (cleared at the start of every round).
+    uint32_t cc = 0;
+    if (lane == 0) { s_u[w][0] = 0ull; s_u[w][1] = 0ull; }
+    for (int base = 0;; base += kStage) {
+      const int ndone = __syncthreads_count(done || base >= n);
+      {
+        const uint64_t mine0 = s_u[w][0], mine1 = s_u[w][1];
+        const uint64_t u0 = (s_u[0][0] | s_u[1][0]) | (s_u[2][0] | s_u[3][0]);
+        const uint64_t u1 = kHalves > 1 ? (s_u[0][1] | s_u[1][1]) | (s_u[2][1] | s_u[3][1]) : 0ull;
+        const uint32_t n0 = (uint32_t)__popcll(u0);
+        const long long o = (long long)rg.x + cc;
+        if ((u0 >> lane) & 1ull)
+          a.pmask[4 * (o + __popcll(u0 & lt_mask)) + w] = ((mine0 >> lane) & 1ull) ? s_m[w][lane] : 0ull;
+        if (kHalves > 1 && ((u1 >> lane) & 1ull))
+          a.pmask[4 * (o + n0 + __popcll(u1 & lt_mask)) + w] = ((mine1 >> lane) & 1ull) ? s_m[w][lane + 64] : 0ull;
+        if (t < kStage) {   // waves 0 and 1 staged entries 0-63 and 64-127
+          const uint64_t u = w == 0 ? u0 : u1;
+          if ((u >> lane) & 1ull)
+            a.cvals[o + (w == 0 ? 0u : n0) + __popcll(u & lt_mask)] = __float_as_uint(s_rgb[t].w);
+        }
+        cc += n0 + (uint32_t)__popcll(u1);
+      }
+      if (ndone == 256) break;
       if (t < kStage && base + t < n)
         store_entry(er, kNormal, s, cam_scale, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t);
       __syncthreads();
+      if (lane == 0) { s_u[w][0] = 0ull; s_u[w][1] = 0ull; }
       if (t < kStage) {
         if (base + kStage + t < n) fetch_entry(er, nval, vi, a.P, a.bboxes, a.centers, a.lengths, a.geos, rgbs_v);
         if (base + 2 * kStage + t < n) nval = a.vals[rg.x + base + 2 * kStage + t];
@@ -540,8 +564,14 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
         pr_hi |= cs0_hi & ~cs_hi;
         wave_lds_sync();
       }
-      // Entries of this batch that contributed to any pixel of the wave: one bit per list position.
-      if (__ballot((pr_lo | pr_hi) != 0ull) != 0ull) mark_contributing(a.cbits, (long long)rg.x + base, pr_lo, pr_hi, lane);
+      {
+        const uint64_t e_lo = transpose64(pr_lo, lane);
+        const uint64_t e_hi = kHalves > 1 ? transpose64(pr_hi, lane) : 0ull;
+        const uint64_t u_lo = __ballot(e_lo != 0ull), u_hi = __ballot(e_hi != 0ull);
+        s_m[w][lane] = e_lo;
+        if (kHalves > 1) s_m[w][lane + 64] = e_hi;
+        if (lane == 0) { s_u[w][0] = u_lo; s_u[w][1] = u_hi; }
+      }
     }
 
     float Dmed = 0.f;
@@ -550,7 +580,10 @@ __global__ __launch_bounds__(256) void render_fwd_pairs_kernel(FwdArgs a) {
     const uint32_t tl = wave_max_u32(rg.x + (inside ? last : 0u));
     if (lane == 0) s_red[w] = tl;
     __syncthreads();
-    if (t == 0) a.tile_last[g] = max(max(s_red[0], s_red[1]), max(s_red[2], s_red[3]));
+    if (t == 0) {
+      a.tile_last[g] = max(max(s_red[0], s_red[1]), max(s_red[2], s_red[3]));
+      a.cranges[g] = make_uint2(rg.x, rg.x + cc);
+    }
     __syncthreads();
   }
 }
@@ -577,10 +610,6 @@ static hipError_t launch_fwd_mw(const FwdArgs& a, hipStream_t st, unsigned grid)
 
 hipError_t launch_render_forward(const FwdArgs& a, hipStream_t st) {
   if (a.total_tiles <= 0) return hipSuccess;
-  // Contributing-entry bits for the backward; the one-entry-at-a-time kernel marks nothing, so its
-  // bits are all set (the backward then takes every entry).
-  hipError_t e = launch_fill_u32(a.cbits, contrib_words(a.k_cap), a.variant == 1 ? 0xFFFFFFFFu : 0u, st);
-  if (e != hipSuccess) return e;
   unsigned grid = (unsigned)a.total_tiles;
   if (a.work_counter) grid = (unsigned)min(a.total_tiles, 256 * max(1, a.blocks_per_cu));
   if (a.need_depth && a.need_normal) return launch_fwd_mw<true, true>(a, st, grid);

@@ -4,7 +4,8 @@
 //
 //   geometry state  (V views x P voxels): view table, counters, n_dup, inclusive scan,
 //                   packed pixel bboxes, octant bitsets, scan scratch
-//   binning state   (K keys): sorted keys/values, the ping-pong pair, radix-sort scratch
+//   binning state   (K keys): sorted keys/values, the ping-pong pair, radix-sort scratch, and
+//                   the contributing entries with their pixel masks (forward -> backward)
 //   image state     (pixels, tiles): tile ranges, last-contributor bound, per-pixel contributor
 //                   count, work-queue counter, view table copy
 #pragma once
@@ -68,10 +69,8 @@ struct GeomPtrs {
 };
 
 // Binning state.
-inline long long contrib_words(long long K) { return (K + 31) / 32 + 2; }
-
 struct BinLayout {
-  size_t keys, vals, keys_alt, vals_alt, hist, partial, cbits, cscan, cpart, cpos, total;
+  size_t keys, vals, keys_alt, vals_alt, hist, partial, total;
   explicit BinLayout(long long K) {
     Carver c;
     keys = c.take<uint64_t>(K);
@@ -81,11 +80,6 @@ struct BinLayout {
     const long long hn = 512LL * (long long)sort_tile_count(K);   // up to 9-bit digits
     hist = c.take<uint32_t>(hn);
     partial = c.take<uint32_t>((long long)scan_partial_count(hn) + 1);
-    const long long nw = contrib_words(K);
-    cbits = c.take<uint32_t>(nw);
-    cscan = c.take<uint32_t>(nw);
-    cpart = c.take<uint32_t>((long long)scan_partial_count(nw) + 1);
-    cpos = c.take<uint32_t>(K);
     total = c.off;
   }
 };
@@ -97,10 +91,6 @@ struct BinPtrs {
   uint32_t* vals_alt;
   uint32_t* hist;
   uint32_t* partial;
-  uint32_t* cbits;     // bit per list position: the entry contributed to a pixel of its tile
-  uint32_t* cscan;     // exclusive count of set bits before each word
-  uint32_t* cpart;
-  uint32_t* cpos;      // list positions of the contributing entries, in list order
   BinPtrs(void* base, long long K) {
     BinLayout L(K);
     char* b = (char*)base;
@@ -110,12 +100,36 @@ struct BinPtrs {
     vals_alt = (uint32_t*)(b + L.vals_alt);
     hist = (uint32_t*)(b + L.hist);
     partial = (uint32_t*)(b + L.partial);
-    cbits = (uint32_t*)(b + L.cbits);
-    cscan = (uint32_t*)(b + L.cscan);
-    cpart = (uint32_t*)(b + L.cpart);
-    cpos = (uint32_t*)(b + L.cpos);
   }
 };
+
+// Contributing entries, written by the forward compositor for the backward (K keys): each tile's
+// entries that composited into at least one of its pixels, packed in list order from the start
+// of the tile's list range, and per entry the pixels of each of the tile's four 8x8 blocks that
+// composited it (bit = 8 * row + column).
+struct ContribLayout {
+  size_t cvals, pmask, total;
+  explicit ContribLayout(long long K) {
+    Carver c;
+    cvals = c.take<uint32_t>(K);
+    pmask = c.take<uint64_t>(4 * K);
+    total = c.off;
+  }
+};
+
+struct ContribPtrs {
+  uint32_t* cvals;     // value (octant << 29 | voxel) of each contributing entry
+  uint64_t* pmask;     // [entry][block]: pixels of the block that composited the entry
+  ContribPtrs(void* base, long long K) {
+    ContribLayout L(K);
+    char* b = (char*)base;
+    cvals = (uint32_t*)(b + L.cvals);
+    pmask = (uint64_t*)(b + L.pmask);
+  }
+};
+
+// The per-view binning state: the sort buffers, then the contributing entries.
+inline size_t raster_bin_bytes(long long K) { return BinLayout(K).total + ContribLayout(K).total; }
 
 // Image state.
 struct ImageLayout {
@@ -138,7 +152,7 @@ struct ImagePtrs {
   uint2* ranges;
   uint32_t* tile_last;
   uint32_t* n_contrib;
-  uint2* cranges;      // per tile: its contributing entries in the compacted position list
+  uint2* cranges;      // per tile: its contributing entries [x, y) in the contributing-entry arrays
   ImagePtrs(void* base, long long n_views, long long n_pix, long long n_tiles) {
     ImageLayout L(n_views, n_pix, n_tiles);
     char* b = (char*)base;
@@ -221,9 +235,6 @@ hipError_t launch_pair_gather(long long n_pairs, const uint32_t* order, const ui
 hipError_t launch_pair_emit(const ViewDesc* views, int V, int P, long long n_pairs, const uint32_t* order,
                             const uint32_t* cnt_ranked, const uint32_t* scan_ranked, const uint2* bboxes,
                             const uint32_t* vmask, uint32_t* tiles, uint32_t* vals, long long cap, hipStream_t st);
-hipError_t launch_contrib_compact(const uint32_t* cbits, long long k_cap, uint32_t* cscan, uint32_t* cpart,
-                                  uint32_t* cpos, const uint2* ranges, const uint32_t* tile_last, long long n_tiles,
-                                  uint2* cranges, hipStream_t st);
 hipError_t launch_tile_ranges_u32(const uint32_t* tiles, const uint32_t* n_dev, long long n_cap, uint2* ranges,
                                   hipStream_t st);
 
@@ -238,8 +249,7 @@ struct FwdArgs {
   uint32_t* tile_last; uint32_t* n_contrib;
   float* color; float* depth; float* normal; float* T; float* max_w;
   uint32_t* work_counter;          // null: one workgroup per tile
-  uint32_t* cbits;                 // contributing-entry bits per list position (cleared by the launcher)
-  long long k_cap;                 // list positions covered by cbits
+  uint32_t* cvals; uint64_t* pmask; uint2* cranges;   // contributing entries (written)
   int need_depth, need_normal;
   int variant;                     // 0: pair-compacted (default), 1: one entry at a time
   int halves = 2;                  // entries staged per round: 64 * halves (pair-compacted kernel)
@@ -264,8 +274,8 @@ struct BwdArgs {
   // flush; null: into dL_dvox slots 0-7).
   const int64_t* vox_key; float* dL_dgrid;
   uint32_t* work_counter;
-  // Contributing entries per tile (launch_contrib_compact over the forward's bits).
-  const uint32_t* cpos; const uint2* cranges;
+  // Contributing entries per tile and their pixel masks, as the forward wrote them.
+  const uint32_t* cvals; const uint64_t* pmask; const uint2* cranges;
   int need_depth, need_normal;
   int variant;                     // 0: pair-compacted (default), 1: one entry at a time
   int halves = 2;                  // entries staged per round: 64 * halves (pair-compacted kernel)
