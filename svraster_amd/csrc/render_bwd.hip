@@ -336,10 +336,9 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
   __shared__ uint16_t s_pref[4][kStage];   // per entry: first position of its run
   __shared__ uint8_t s_own[4][kBwdWin];    // 1 + entry whose run starts at the position, else 0
   __shared__ uint16_t s_hd[4][kBwdWin];    // entry | pixel << 7 per position
-  __shared__ float s_pa[4][kBwdWin];       // alpha per position
+  __shared__ float s_pa[4][kBwdWin];       // alpha per position; after the walk, T before the entry
   __shared__ float s_pd[4][kDepth ? kBwdWin : 1];
   __shared__ float s_pl[4][kBwdWin];       // dL/dalpha per position
-  __shared__ float s_pt[4][kBwdWin];       // T before the entry per position
   __shared__ float s_red[4];
   __shared__ int s_tile;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -511,7 +510,7 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
             last_dT += alpha * (dptw - last_dT);
             if (kDepth) last_dT += dD * s_pd[w][kk];
             s_pl[w][kk] = dLda;
-            s_pt[w][kk] = T;
+            s_pa[w][kk] = T;
           }
           wave_lds_sync_b();
 
@@ -537,7 +536,7 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
             const int j = (int)(d & 127u);
             if (valid) {
               const int bit = (int)(d >> 7);
-              const float Tp = s_pt[w][k];
+              const float Tp = s_pa[w][k];
               const float dLda = s_pl[w][k];
               const float4 rdd = s_rd[w][bit];
               const Ray pr = pair_ray(r, rdd, s_ri[w][bit]);
@@ -582,6 +581,7 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
               }
 #pragma unroll
               for (int mm = 0; mm < 4; ++mm) {
+                if (!kDepth && (mm == 1 || mm == 2)) continue;   // c[1] = c[2] = 0 without depth
                 const float fm = (float)mm;
                 float wts[8];
                 trilinear_weights(seg.qx + seg.sx * fm, seg.qy + seg.sy * fm, seg.qz + seg.sz * fm, wts);
