@@ -48,6 +48,28 @@ __host__ __device__ inline void tile_quadrant(const ViewDesc& v, int tx, int ty,
   qy = ty * v.tile_h + (w / across) * 8;
 }
 
+// The pixels of wavefront w of tile (tx, ty): an 8x8 lattice with strides (2^kx, 2^ky),
+// 2^kx * 2^ky = 4, interleaved with the lattices of the tile's other three wavefronts, so the four
+// see alike shares of every entry's footprint (the work per staged entry is balanced across the
+// workgroup).  16x16 tiles: strides 2x2, 32x8: 4x1, 8x32: 1x4.  Lane l holds lattice point
+// (l & 7, l >> 3), the pixel (ox + (l & 7) * 2^kx, oy + (l >> 3) * 2^ky).
+struct WaveLattice {
+  int ox, oy, kx, ky;
+};
+
+__host__ __device__ inline WaveLattice wave_lattice(const ViewDesc& v, int tx, int ty, int w) {
+  const int kx = v.tile_w == 32 ? 2 : (v.tile_w == 16 ? 1 : 0);
+  WaveLattice L;
+  L.kx = kx;
+  L.ky = 2 - kx;
+  L.ox = tx * v.tile_w + (w & ((1 << kx) - 1));
+  L.oy = ty * v.tile_h + (w >> kx);
+  return L;
+}
+
+__host__ __device__ inline int lattice_px(const WaveLattice& L, int lane) { return L.ox + ((lane & 7) << L.kx); }
+__host__ __device__ inline int lattice_py(const WaveLattice& L, int lane) { return L.oy + ((lane >> 3) << L.ky); }
+
 __host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 // Octant of a direction / of a corner relative to the camera (auxiliary.h:76-84).

@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
     if (a.work_counter) {
       if (t == 0) s_tile = (int)atomicAdd(a.work_counter, 1u);
       __syncthreads();
-      g = s_tile;
+      g = __builtin_amdgcn_readfirstlane(s_tile);   // uniform: tile and view data in scalar registers
       __syncthreads();
       if (g >= a.total_tiles) break;
     } else {
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
     if (kQueue) {
       if (t == 0) s_tile = (int)atomicAdd(a.work_counter, 1u);
       __syncthreads();
-      g = s_tile;
+      g = __builtin_amdgcn_readfirstlane(s_tile);   // uniform: tile and view data in scalar registers
       __syncthreads();
       if (g >= a.total_tiles) break;
     } else {

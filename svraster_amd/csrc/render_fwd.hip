@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
     if (a.work_counter) {
       if (t == 0) s_tile = (int)atomicAdd(a.work_counter, 1u);
       __syncthreads();
-      g = s_tile;
+      g = __builtin_amdgcn_readfirstlane(s_tile);   // uniform: tile and view data in scalar registers
       __syncthreads();
       if (g >= a.total_tiles) return;
     } else {
@@ -297,12 +297,13 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
 // identical bit for bit.
 // ------------------------------------------------------------------------------------------
 constexpr int kPairStage = 128;   // entries staged per workgroup round (two 64-entry halves)
-constexpr int kPairWin = 256;     // pair positions held per wave at once
+constexpr int kPairWin = 192;     // pair positions held per wave at once
 
-// Pixels of the wave's 8x8 block (bit = 8 * row + column) inside a packed bbox.
-__device__ inline uint64_t bbox_block_mask(uint32_t b0, uint32_t b1, int qx, int qy) {
-  const int x0 = max((int)(b0 >> 16) - qx, 0), x1 = min((int)(b1 >> 16) - qx, 7);
-  const int y0 = max((int)(b0 & 0xFFFFu) - qy, 0), y1 = min((int)(b1 & 0xFFFFu) - qy, 7);
+// Pixels of the wave's 8x8 lattice (bit = 8 * row + column) inside a packed bbox.
+__device__ inline uint64_t bbox_block_mask(uint32_t b0, uint32_t b1, const WaveLattice& L) {
+  const int mx = (1 << L.kx) - 1, my = (1 << L.ky) - 1;
+  const int x0 = max(((int)(b0 >> 16) - L.ox + mx) >> L.kx, 0), x1 = min(((int)(b1 >> 16) - L.ox) >> L.kx, 7);
+  const int y0 = max(((int)(b0 & 0xFFFFu) - L.oy + my) >> L.ky, 0), y1 = min(((int)(b1 & 0xFFFFu) - L.oy) >> L.ky, 7);
   if (x0 > x1 || y0 > y1) return 0ull;
   const uint64_t cols = (uint64_t)(((1u << (x1 + 1)) - 1u) & ~((1u << x0) - 1u));
   const uint64_t hi = (y1 == 7) ? ~0ull : ((1ull << (8 * (y1 + 1))) - 1ull);
@@ -317,12 +318,24 @@ __device__ inline void wave_lds_sync() {
 }
 
 
-// Mask of entry slot j for the wave's block: bbox, octant of the pixel rays, live pixels.
-__device__ inline uint64_t entry_block_mask(const uint2* s_bb, const float4* s_rgb, int j, int qx, int qy,
+// Mask of entry slot j for the wave's pixels: bbox, octant of the pixel rays, live pixels.
+__device__ inline uint64_t entry_block_mask(const uint2* s_bb, const float4* s_rgb, int j, const WaveLattice& L,
                                             const uint64_t* s_om, uint64_t live) {
   const uint2 bb = s_bb[j];
   const uint32_t q = __float_as_uint(s_rgb[j].w) >> 29;
-  return bbox_block_mask(bb.x, bb.y, qx, qy) & live & s_om[q];
+  return bbox_block_mask(bb.x, bb.y, L) & live & s_om[q];
+}
+
+// OR of a 64-bit value over the wave.
+__device__ inline uint64_t wave_or_u64(uint64_t v, int lane) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  lo |= xor_lane<32>(lo, lane); hi |= xor_lane<32>(hi, lane);
+  lo |= xor_lane<16>(lo, lane); hi |= xor_lane<16>(hi, lane);
+  lo |= xor_lane<8>(lo, lane);  hi |= xor_lane<8>(hi, lane);
+  lo |= xor_lane<4>(lo, lane);  hi |= xor_lane<4>(hi, lane);
+  lo |= xor_lane<2>(lo, lane);  hi |= xor_lane<2>(hi, lane);
+  lo |= xor_lane<1>(lo, lane);  hi |= xor_lane<1>(hi, lane);
+  return ((uint64_t)hi << 32) | lo;
 }
 
 // Pops the lowest entry of a 128-bit entry set.
@@ -355,6 +368,7 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
   __shared__ float s_pa[4][kPairWin];         // alpha per position (0 when the ray misses the box)
   __shared__ float s_pd[4][kDepth ? kPairWin : 1];
   __shared__ uint64_t s_u[4][2];              // per wave: entries of the last round it composited
+  __shared__ uint64_t s_pix[4][kHalves][64];  // per pixel, in quadrant order: entries it composited
   __shared__ uint32_t s_red[4];
   __shared__ int s_tile;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -366,7 +380,7 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
     if (kQueue) {
       if (t == 0) s_tile = (int)atomicAdd(a.work_counter, 1u);
       __syncthreads();
-      g = s_tile;
+      g = __builtin_amdgcn_readfirstlane(s_tile);   // uniform: tile and view data in scalar registers
       __syncthreads();
       if (g >= a.total_tiles) return;
     } else {
@@ -378,9 +392,8 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
     const int W = V.W, H = V.H;
     const int lt = g - V.tile_base;
     const int tx = lt % V.tiles_x, ty = lt / V.tiles_x;
-    int qx, qy;
-    tile_quadrant(V, tx, ty, w, qx, qy);
-    const int px = qx + (lane & 7), py = qy + (lane >> 3);
+    const WaveLattice L = wave_lattice(V, tx, ty, w);
+    const int px = lattice_px(L, lane), py = lattice_py(L, lane);
     const bool inside = px < W && py < H;
     const Ray r = pixel_ray(V, px, py);
     const float cam_scale = fmaxf(fabsf(r.ox), fmaxf(fabsf(r.oy), fabsf(r.oz)));
@@ -415,161 +428,167 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
 
     // A round's contributing entries are known once every wave has finished it, so they are
     // written at the top of the next round: packed after the tile's earlier ones (cc so far), each
-    // wave its block's pixel masks, the staging lane the value.  At the end of a round each wave
-    // leaves in s_m its masks (the block's pixels that composited each entry) and in s_u the
-    // entries whose mask is not empty (cleared at the start of every round).
+    // wave one quadrant's pixel masks, the staging lane the value.  At the end of every round each
+    // wave leaves in s_pix the entry sets of its pixels, at their quadrant positions, and in s_u
+    // the union of its pixels' sets.
+    int qw, qb;   // quadrant and bit of this lane's pixel
+    {
+      const int xt = px - tx * V.tile_w, yt = py - ty * V.tile_h;
+      qw = (xt >> 3) + (V.tile_w >> 3) * (yt >> 3);
+      qb = (xt & 7) + 8 * (yt & 7);
+    }
     uint32_t cc = 0;
     if (lane == 0) { s_u[w][0] = 0ull; s_u[w][1] = 0ull; }
     for (int base = 0;; base += kStage) {
       const int ndone = __syncthreads_count(done || base >= n);
-      {
-        const uint64_t mine0 = s_u[w][0], mine1 = s_u[w][1];
-        const uint64_t u0 = (s_u[0][0] | s_u[1][0]) | (s_u[2][0] | s_u[3][0]);
-        const uint64_t u1 = kHalves > 1 ? (s_u[0][1] | s_u[1][1]) | (s_u[2][1] | s_u[3][1]) : 0ull;
-        const uint32_t n0 = (uint32_t)__popcll(u0);
-        const long long o = (long long)rg.x + cc;
-        if ((u0 >> lane) & 1ull)
-          a.pmask[4 * (o + __popcll(u0 & lt_mask)) + w] = ((mine0 >> lane) & 1ull) ? s_m[w][lane] : 0ull;
-        if (kHalves > 1 && ((u1 >> lane) & 1ull))
-          a.pmask[4 * (o + n0 + __popcll(u1 & lt_mask)) + w] = ((mine1 >> lane) & 1ull) ? s_m[w][lane + 64] : 0ull;
-        if (t < kStage) {   // waves 0 and 1 staged entries 0-63 and 64-127
-          const uint64_t u = w == 0 ? u0 : u1;
-          if ((u >> lane) & 1ull)
-            a.cvals[o + (w == 0 ? 0u : n0) + __popcll(u & lt_mask)] = __float_as_uint(s_rgb[t].w);
-        }
-        cc += n0 + (uint32_t)__popcll(u1);
+      const bool fin = ndone == 256;
+      const uint64_t u0 = (s_u[0][0] | s_u[1][0]) | (s_u[2][0] | s_u[3][0]);
+      const uint64_t u1 = kHalves > 1 ? (s_u[0][1] | s_u[1][1]) | (s_u[2][1] | s_u[3][1]) : 0ull;
+      const uint32_t n0 = (uint32_t)__popcll(u0);
+      const long long o = (long long)rg.x + cc;
+      if (t < kStage) {   // waves 0 and 1 staged entries 0-63 and 64-127
+        const uint64_t u = w == 0 ? u0 : u1;
+        if ((u >> lane) & 1ull)
+          a.cvals[o + (w == 0 ? 0u : n0) + __popcll(u & lt_mask)] = __float_as_uint(s_rgb[t].w);
       }
-      if (ndone == 256) break;
-      if (t < kStage && base + t < n)
+      cc += n0 + (uint32_t)__popcll(u1);
+      if (!fin && t < kStage && base + t < n)
         store_entry(er, kNormal, s, cam_scale, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t);
+      // The backward takes the pixels by quadrant: wave w transposes quadrant w's pixel sets into
+      // per-entry masks.
+      {
+        const uint64_t q_lo = transpose64(s_pix[w][0][lane], lane);
+        const uint64_t q_hi = kHalves > 1 ? transpose64(s_pix[w][kHalves - 1][lane], lane) : 0ull;
+        if ((u0 >> lane) & 1ull) a.pmask[4 * (o + __popcll(u0 & lt_mask)) + w] = q_lo;
+        if (kHalves > 1 && ((u1 >> lane) & 1ull)) a.pmask[4 * (o + n0 + __popcll(u1 & lt_mask)) + w] = q_hi;
+      }
       __syncthreads();
-      if (lane == 0) { s_u[w][0] = 0ull; s_u[w][1] = 0ull; }
+      if (fin) break;
       if (t < kStage) {
         if (base + kStage + t < n) fetch_entry(er, nval, vi, a.P, a.bboxes, a.centers, a.lengths, a.geos, rgbs_v);
         if (base + 2 * kStage + t < n) nval = a.vals[rg.x + base + 2 * kStage + t];
       }
       const int cnt = min(kStage, n - base);
       const uint64_t live = __ballot(!done);
-      if (live == 0ull) continue;
+      uint64_t pr_lo = 0ull, pr_hi = 0ull;   // entries this pixel composited in this round
 
-      // 1. entry masks and the entry-major run offsets.
-      const uint64_t m_lo = lane < cnt ? entry_block_mask(s_bb, s_rgb, lane, qx, qy, s_om[w], live) : 0ull;
+      // 1. entry masks (empty once every pixel is saturated) and the entry-major run offsets.
+      const uint64_t m_lo = lane < cnt ? entry_block_mask(s_bb, s_rgb, lane, L, s_om[w], live) : 0ull;
       const uint64_t m_hi = (kHalves > 1 && lane + 64 < cnt)
-                                ? entry_block_mask(s_bb, s_rgb, lane + 64, qx, qy, s_om[w], live) : 0ull;
+                                ? entry_block_mask(s_bb, s_rgb, lane + 64, L, s_om[w], live) : 0ull;
       const uint32_t c_lo = (uint32_t)__popcll(m_lo), c_hi = (uint32_t)__popcll(m_hi);
-      if (__ballot((c_lo | c_hi) != 0u) == 0ull) continue;
-      const uint32_t i_lo = wave_incl_scan_u32(c_lo);
-      const uint32_t i_hi = kHalves > 1 ? wave_incl_scan_u32(c_hi) : 0u;
-      const uint32_t t_lo = (uint32_t)__builtin_amdgcn_readlane((int)i_lo, 63);
-      const uint32_t total = t_lo + (uint32_t)__builtin_amdgcn_readlane((int)i_hi, 63);
-      const uint32_t p_lo = i_lo - c_lo, p_hi = t_lo + i_hi - c_hi;
-      s_m[w][lane] = m_lo;
-      s_pref[w][lane] = (uint16_t)p_lo;
-      if (kHalves > 1) {
-        s_m[w][lane + 64] = m_hi;
-        s_pref[w][lane + 64] = (uint16_t)p_hi;
-      }
-      uint32_t carry = 0;   // 1 + entry owning the last position handled so far
-      uint64_t pr_lo = 0ull, pr_hi = 0ull;   // entries this pixel composited in this batch
+      if (__ballot((c_lo | c_hi) != 0u) != 0ull) {
+        const uint32_t i_lo = wave_incl_scan_u32(c_lo);
+        const uint32_t i_hi = kHalves > 1 ? wave_incl_scan_u32(c_hi) : 0u;
+        const uint32_t t_lo = (uint32_t)__builtin_amdgcn_readlane((int)i_lo, 63);
+        const uint32_t total = t_lo + (uint32_t)__builtin_amdgcn_readlane((int)i_hi, 63);
+        const uint32_t p_lo = i_lo - c_lo, p_hi = t_lo + i_hi - c_hi;
+        s_m[w][lane] = m_lo;
+        s_pref[w][lane] = (uint16_t)p_lo;
+        if (kHalves > 1) {
+          s_m[w][lane + 64] = m_hi;
+          s_pref[w][lane + 64] = (uint16_t)p_hi;
+        }
+        uint32_t carry = 0;   // 1 + entry owning the last position handled so far
 
-      for (uint32_t w0 = 0; w0 < total; w0 += kPairWin) {
-        const uint32_t w1 = min(total, w0 + (uint32_t)kPairWin);
-        const uint32_t np = w1 - w0;
-        for (uint32_t k = lane; k < np; k += 64) s_own[w][k] = 0;
-        s_cs[w][0][lane] = 0ull;
-        s_cs[w][1][lane] = 0ull;
-        wave_lds_sync();
-        if (c_lo && p_lo >= w0 && p_lo < w1) s_own[w][p_lo - w0] = (uint8_t)(lane + 1);
-        if (c_hi && p_hi >= w0 && p_hi < w1) s_own[w][p_hi - w0] = (uint8_t)(lane + 65);
-        wave_lds_sync();
+        for (uint32_t w0 = 0; w0 < total; w0 += kPairWin) {
+          const uint32_t w1 = min(total, w0 + (uint32_t)kPairWin);
+          const uint32_t np = w1 - w0;
+          for (uint32_t k = lane; k < np; k += 64) s_own[w][k] = 0;
+          s_cs[w][0][lane] = 0ull;
+          s_cs[w][1][lane] = 0ull;
+          wave_lds_sync();
+          if (c_lo && p_lo >= w0 && p_lo < w1) s_own[w][p_lo - w0] = (uint8_t)(lane + 1);
+          if (c_hi && p_hi >= w0 && p_hi < w1) s_own[w][p_hi - w0] = (uint8_t)(lane + 65);
+          wave_lds_sync();
 
-        // 2a. ray/box test per position; hits listed with their (entry, pixel).
-        uint32_t nh = 0;
-        for (uint32_t c0 = 0; c0 < np; c0 += 64) {
-          const uint32_t k = c0 + lane;
-          const bool valid = k < np;
-          uint32_t h = valid ? (uint32_t)s_own[w][k] : 0u;
-          h = max(wave_incl_max_u32(h), carry);
-          carry = (uint32_t)__builtin_amdgcn_readlane((int)h, 63);
-          bool hit = false;
-          uint32_t desc = 0;
-          if (valid) {
-            const int j = (int)h - 1;
-            const int bit = select_set_bit(s_m[w][j], (int)(w0 + k - (uint32_t)s_pref[w][j]));
+          // 2a. ray/box test per position; hits listed with their (entry, pixel).
+          uint32_t nh = 0;
+          for (uint32_t c0 = 0; c0 < np; c0 += 64) {
+            const uint32_t k = c0 + lane;
+            const bool valid = k < np;
+            uint32_t h = valid ? (uint32_t)s_own[w][k] : 0u;
+            h = max(wave_incl_max_u32(h), carry);
+            carry = (uint32_t)__builtin_amdgcn_readlane((int)h, 63);
+            bool hit = false;
+            uint32_t desc = 0;
+            if (valid) {
+              const int j = (int)h - 1;
+              const int bit = select_set_bit(s_m[w][j], (int)(w0 + k - (uint32_t)s_pref[w][j]));
+              const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
+              const float4 ch = s_ch[j];
+              float ta, tb;
+              ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
+              hit = !(ta > tb);
+              desc = (uint32_t)j | ((uint32_t)bit << 7);
+              s_pa[w][k] = 0.f;
+            }
+            const uint64_t bal = __ballot(hit);
+            if (hit) {
+              const uint32_t idx = nh + (uint32_t)__popcll(bal & lt_mask);
+              s_hit[w][idx] = (uint16_t)k;
+              s_hd[w][idx] = (uint16_t)desc;
+            }
+            nh += (uint32_t)__popcll(bal);
+          }
+          wave_lds_sync();
+          // 2b. + 3. alpha (and depth value) per hit; contributing pairs into the pixel's set.
+          for (uint32_t h = lane; h < nh; h += 64) {
+            const uint32_t k = s_hit[w][h];
+            const uint32_t d = s_hd[w][h];
+            const int j = (int)(d & 127u), bit = (int)(d >> 7);
             const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
             const float4 ch = s_ch[j];
             float ta, tb;
             ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
-            hit = !(ta > tb);
-            desc = (uint32_t)j | ((uint32_t)bit << 7);
-            s_pa[w][k] = 0.f;
+            const float4 ml = s_ml[j];
+            const float4 g0 = s_g0[j], g1 = s_g1[j];
+            const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+            Segment seg;
+            eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
+            s_pa[w][k] = seg.alpha;
+            if (kDepth) {
+              float sub[3];
+              sub_alphas(seg, sub);
+              s_pd[w][k] = depth_value(sub, ta, tb);
+            }
+            if (seg.alpha >= kMinAlpha) atomicOr(&s_cs[w][j >> 6][bit], 1ull << (j & 63));
           }
-          const uint64_t bal = __ballot(hit);
-          if (hit) {
-            const uint32_t idx = nh + (uint32_t)__popcll(bal & lt_mask);
-            s_hit[w][idx] = (uint16_t)k;
-            s_hd[w][idx] = (uint16_t)desc;
+          wave_lds_sync();
+          // 4. composite this pixel's contributing pairs of the window, in entry order.
+          const uint64_t cs0_lo = s_cs[w][0][lane], cs0_hi = s_cs[w][1][lane];
+          uint64_t cs_lo = cs0_lo, cs_hi = cs0_hi;
+          while ((cs_lo | cs_hi) != 0ull && !done) {
+            const int j = pop_entry(cs_lo, cs_hi);
+            const uint32_t k = (uint32_t)s_pref[w][j] + (uint32_t)__popcll(s_m[w][j] & lt_mask) - w0;
+            const float alpha = s_pa[w][k];
+            const float4 rgb = s_rgb[j];
+            const float wgt = T * alpha;
+            C0 += wgt * rgb.x; C1 += wgt * rgb.y; C2 += wgt * rgb.z;
+            if (kDepth) {
+              D += T * s_pd[w][k];
+              if (T > 0.5f) { dmed_vox = (int)(__float_as_uint(rgb.w) & kVoxMask); dmed_T = T; }
+            }
+            if (kNormal) {
+              const float4 nn = s_nrm[j];
+              N0 += wgt * nn.x; N1 += wgt * nn.y; N2 += wgt * nn.z;
+            }
+            T *= (1.f - alpha);
+            done = T < kEarlyStopT;
+            last = (uint32_t)(base + j + 1);
+            if (kMaxW)
+              atomicMax(reinterpret_cast<uint32_t*>(a.max_w) + (__float_as_uint(rgb.w) & kVoxMask), __float_as_uint(wgt));
           }
-          nh += (uint32_t)__popcll(bal);
+          pr_lo |= cs0_lo & ~cs_lo;     // composited: the set minus what was left when T saturated
+          pr_hi |= cs0_hi & ~cs_hi;
+          wave_lds_sync();
         }
-        wave_lds_sync();
-        // 2b. + 3. alpha (and depth value) per hit; contributing pairs into the pixel's set.
-        for (uint32_t h = lane; h < nh; h += 64) {
-          const uint32_t k = s_hit[w][h];
-          const uint32_t d = s_hd[w][h];
-          const int j = (int)(d & 127u), bit = (int)(d >> 7);
-          const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
-          const float4 ch = s_ch[j];
-          float ta, tb;
-          ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
-          const float4 ml = s_ml[j];
-          const float4 g0 = s_g0[j], g1 = s_g1[j];
-          const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
-          Segment seg;
-          eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
-          s_pa[w][k] = seg.alpha;
-          if (kDepth) {
-            float sub[3];
-            sub_alphas(seg, sub);
-            s_pd[w][k] = depth_value(sub, ta, tb);
-          }
-          if (seg.alpha >= kMinAlpha) atomicOr(&s_cs[w][j >> 6][bit], 1ull << (j & 63));
-        }
-        wave_lds_sync();
-        // 4. composite this pixel's contributing pairs of the window, in entry order.
-        const uint64_t cs0_lo = s_cs[w][0][lane], cs0_hi = s_cs[w][1][lane];
-        uint64_t cs_lo = cs0_lo, cs_hi = cs0_hi;
-        while ((cs_lo | cs_hi) != 0ull && !done) {
-          const int j = pop_entry(cs_lo, cs_hi);
-          const uint32_t k = (uint32_t)s_pref[w][j] + (uint32_t)__popcll(s_m[w][j] & lt_mask) - w0;
-          const float alpha = s_pa[w][k];
-          const float4 rgb = s_rgb[j];
-          const float wgt = T * alpha;
-          C0 += wgt * rgb.x; C1 += wgt * rgb.y; C2 += wgt * rgb.z;
-          if (kDepth) {
-            D += T * s_pd[w][k];
-            if (T > 0.5f) { dmed_vox = (int)(__float_as_uint(rgb.w) & kVoxMask); dmed_T = T; }
-          }
-          if (kNormal) {
-            const float4 nn = s_nrm[j];
-            N0 += wgt * nn.x; N1 += wgt * nn.y; N2 += wgt * nn.z;
-          }
-          T *= (1.f - alpha);
-          done = T < kEarlyStopT;
-          last = (uint32_t)(base + j + 1);
-          if (kMaxW)
-            atomicMax(reinterpret_cast<uint32_t*>(a.max_w) + (__float_as_uint(rgb.w) & kVoxMask), __float_as_uint(wgt));
-        }
-        pr_lo |= cs0_lo & ~cs_lo;     // composited: the set minus what was left when T saturated
-        pr_hi |= cs0_hi & ~cs_hi;
-        wave_lds_sync();
       }
+      s_pix[qw][0][qb] = pr_lo;
+      if (kHalves > 1) s_pix[qw][kHalves - 1][qb] = pr_hi;
       {
-        const uint64_t e_lo = transpose64(pr_lo, lane);
-        const uint64_t e_hi = kHalves > 1 ? transpose64(pr_hi, lane) : 0ull;
-        const uint64_t u_lo = __ballot(e_lo != 0ull), u_hi = __ballot(e_hi != 0ull);
-        s_m[w][lane] = e_lo;
-        if (kHalves > 1) s_m[w][lane + 64] = e_hi;
+        const uint64_t u_lo = wave_or_u64(pr_lo, lane);
+        const uint64_t u_hi = kHalves > 1 ? wave_or_u64(pr_hi, lane) : 0ull;
         if (lane == 0) { s_u[w][0] = u_lo; s_u[w][1] = u_hi; }
       }
     }
