@@ -252,9 +252,10 @@ __global__ __launch_bounds__(256) void duplicate_kernel(
 
 // ------------------------------------------------------------------------------------------
 // Stable LSD radix sort over kBits-bit digits (8 for the 64-bit rank keys, 9 for 32-bit tile
-// ids).  Per pass: per-tile digit histograms, an exclusive scan of the digit-major histogram
-// table, then a stable scatter in which each wave64 ranks its 64 items with kBits ballots
-// (items with equal digits get consecutive slots in lane order).
+// ids).  Per pass: per-tile digit histograms written tile-major (each workgroup one contiguous
+// row), a column scan of that table (per digit over tiles, then over digits), then a stable
+// scatter in which each wave64 ranks its 64 items with kBits ballots (items with equal digits get
+// consecutive slots in lane order) and reads its row of output offsets.
 // ------------------------------------------------------------------------------------------
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
@@ -277,7 +278,41 @@ __global__ __launch_bounds__(kSortThreads) void radix_histogram_kernel(
     if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & (uint32_t)(kD - 1)], 1u);
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < kD; d += kSortThreads) hist[(long long)d * nb + blockIdx.x] = h[d];
+  for (int d = threadIdx.x; d < kD; d += kSortThreads) hist[(long long)blockIdx.x * kD + d] = h[d];
+}
+
+// Column scan of the tile-major histogram table hist[tile][digit] (nb rows of kD), in place:
+// entry (b, d) becomes the output position of tile b's first key with digit d -- the keys with
+// smaller digits plus those with digit d in earlier tiles.  Rows are grouped in chunks of
+// kColChunk: per chunk and digit a sum, the sums transposed digit-major and scanned, then each
+// chunk walks its rows from its prefix.  Every pass over the table reads whole rows.
+constexpr int kColChunk = 128;
+
+int col_chunks(int nb) { return (nb + kColChunk - 1) / kColChunk; }
+
+template <int kD>
+__global__ __launch_bounds__(kD) void hist_chunk_sum_kernel(const uint32_t* __restrict__ hist, int nb,
+                                                            uint32_t* __restrict__ csum_t, int nc) {
+  const int d = threadIdx.x, c = blockIdx.x;
+  const int b0 = c * kColChunk, b1 = min(nb, b0 + kColChunk);
+  uint32_t s = 0;
+#pragma unroll 8
+  for (int b = b0; b < b1; ++b) s += hist[(long long)b * kD + d];
+  csum_t[(long long)d * nc + c] = s;   // digit-major, for the scan
+}
+
+template <int kD>
+__global__ __launch_bounds__(kD) void hist_chunk_apply_kernel(uint32_t* __restrict__ hist, int nb,
+                                                              const uint32_t* __restrict__ cpre_t, int nc) {
+  const int d = threadIdx.x, c = blockIdx.x;
+  const int b0 = c * kColChunk, b1 = min(nb, b0 + kColChunk);
+  uint32_t run = cpre_t[(long long)d * nc + c];
+#pragma unroll 8
+  for (int b = b0; b < b1; ++b) {
+    const uint32_t v = hist[(long long)b * kD + d];
+    hist[(long long)b * kD + d] = run;
+    run += v;
+  }
 }
 
 template <typename K, int kBits>
@@ -290,7 +325,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   __shared__ uint32_t running[kD];
   __shared__ uint32_t wcnt[4][kD + 1];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  for (int d = t; d < kD; d += kSortThreads) running[d] = offsets[(long long)d * nb + blockIdx.x];
+  for (int d = t; d < kD; d += kSortThreads) running[d] = offsets[(long long)blockIdx.x * kD + d];
   const long long n = n_dev ? min((long long)*n_dev, n_cap) : n_cap;
   const long long base = (long long)blockIdx.x * kSortTile;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -545,6 +580,24 @@ hipError_t launch_duplicate(const ViewDesc* views, int n_views, int P, const int
 
 size_t sort_tile_count(long long n) { return (size_t)((n + kSortTile - 1) / kSortTile); }
 
+// Scratch of the histogram column scan for n keys (up to 9-bit digits): the digit-major chunk
+// sums, scanned in place, and that scan's partials.
+size_t sort_scratch_count(long long n) {
+  const long long m = (1LL << kMaxDigitBits) * (long long)col_chunks((int)sort_tile_count(n));
+  return (size_t)(m + (long long)scan_partial_count(m) + 1);
+}
+
+template <int kD>
+static hipError_t hist_column_scan(uint32_t* hist, int nb, uint32_t* scratch, hipStream_t st) {
+  const int nc = col_chunks(nb);
+  const long long m = (long long)kD * nc;
+  hist_chunk_sum_kernel<kD><<<nc, kD, 0, st>>>(hist, nb, scratch, nc);
+  hipError_t e = launch_scan(scratch, nullptr, m, scratch + m, scratch, false, st);
+  if (e != hipSuccess) return e;
+  hist_chunk_apply_kernel<kD><<<nc, kD, 0, st>>>(hist, nb, scratch, nc);
+  return hipGetLastError();
+}
+
 // Sorts (keys, vals) by bits [0, nbits).  Buffers ping-pong; returns in `*result_in_alt`
 // whether the sorted data ended in the alt buffers.
 hipError_t launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
@@ -553,12 +606,11 @@ hipError_t launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
   *result_in_alt = false;
   if (n_cap <= 0) return hipSuccess;
   const int nb = (int)sort_tile_count(n_cap);
-  const long long hn = 256LL * nb;
   uint64_t* ki = keys; uint32_t* vi = vals;
   uint64_t* ko = keys_alt; uint32_t* vo = vals_alt;
   for (int shift = 0; shift < nbits; shift += 8) {
     radix_histogram_kernel<uint64_t, 8><<<nb, kSortThreads, 0, st>>>(ki, n_dev, n_cap, shift, hist, nb);
-    hipError_t e = launch_scan(hist, nullptr, hn, scan_partial, hist, false, st);
+    hipError_t e = hist_column_scan<256>(hist, nb, scan_partial, st);
     if (e != hipSuccess) return e;
     radix_scatter_kernel<uint64_t, 8><<<nb, kSortThreads, 0, st>>>(ki, vi, n_dev, n_cap, shift, hist, nb, ko, vo);
     std::swap(ki, ko);
@@ -582,7 +634,6 @@ hipError_t launch_tile_sort(uint32_t* tiles, uint32_t* vals, uint32_t* tiles_alt
   if (n_cap <= 0) return hipSuccess;
   const int passes = tile_sort_passes(n_tiles);
   const int nb = (int)sort_tile_count(n_cap);
-  const long long hn = (1LL << kMaxDigitBits) * nb;
   uint32_t* ki = (passes % 2) ? tiles_alt : tiles;
   uint32_t* vi = (passes % 2) ? vals_alt : vals;
   uint32_t* ko = (passes % 2) ? tiles : tiles_alt;
@@ -590,7 +641,7 @@ hipError_t launch_tile_sort(uint32_t* tiles, uint32_t* vals, uint32_t* tiles_alt
   for (int p = 0; p < passes; ++p) {
     const int shift = p * kMaxDigitBits;
     radix_histogram_kernel<uint32_t, kMaxDigitBits><<<nb, kSortThreads, 0, st>>>(ki, n_dev, n_cap, shift, hist, nb);
-    hipError_t e = launch_scan(hist, nullptr, hn, scan_partial, hist, false, st);
+    hipError_t e = hist_column_scan<1 << kMaxDigitBits>(hist, nb, scan_partial, st);
     if (e != hipSuccess) return e;
     radix_scatter_kernel<uint32_t, kMaxDigitBits><<<nb, kSortThreads, 0, st>>>(ki, vi, n_dev, n_cap, shift, hist, nb,
                                                                                ko, vo);

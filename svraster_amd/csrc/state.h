@@ -17,6 +17,7 @@ namespace svr {
 
 size_t scan_partial_count(long long n);
 size_t sort_tile_count(long long n);
+size_t sort_scratch_count(long long n);
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -79,7 +80,7 @@ struct BinLayout {
     vals_alt = c.take<uint32_t>(K);
     const long long hn = 512LL * (long long)sort_tile_count(K);   // up to 9-bit digits
     hist = c.take<uint32_t>(hn);
-    partial = c.take<uint32_t>((long long)scan_partial_count(hn) + 1);
+    partial = c.take<uint32_t>((long long)sort_scratch_count(K));
     total = c.off;
   }
 };
