@@ -80,16 +80,21 @@ def time_steps(fn, steps, warmup, sync):
 def backward_traffic(rast, dvox):
     """Algorithmic bytes of the multi-view backward-compositor launch (one launch, all views).
 
-    72 B per streamed (tile, entry) pair [value 4, bbox 8, centre 12, size 4, 8 corner SDFs 32,
-    rgb 12], 12 B per tile [range 8, last-contributor bound 4], 24 B per pixel [T 4, contributor
-    count 4, dL/dcolor 12, dL/dT 4], 48 B per voxel that receives a gradient [12 floats].  Streamed
-    entries E = sum over tiles of (last-contributor bound - range start), read from the launch's
-    own state."""
+    The backward streams only the entries the forward handed over (those that composited into
+    a pixel of their tile): 96 B per streamed (tile, entry) pair [value 4, four 8x8-block pixel
+    masks 32, centre 12, size 4, 8 corner SDFs 32, rgb 12], 12 B per tile [contributing range 8,
+    list start 4], 24 B per pixel [T 4, contributor count 4, dL/dcolor 12, dL/dT 4], 48 B per voxel
+    that receives a gradient [12 floats].  Streamed entries E_b = sum over tiles of the
+    contributing range, read from the launch's own state; the entries the forward streamed
+    (up to each tile's last-contributor bound) are reported beside it."""
     ranges, _, _, tile_last = rast.state_views()
-    E = int((tile_last.long() - ranges[:, 0].long()).clamp_min(0).sum().item())
+    E_f = int((tile_last.long() - ranges[:, 0].long()).clamp_min(0).sum().item())
+    cr = rast.contrib_ranges()
+    E_b = int((cr[:, 1].long() - cr[:, 0].long()).clamp_min(0).sum().item())
     touched = int((dvox != 0).any(1).sum().item())
-    total = 72.0 * E + 12.0 * rast.n_tiles + 24.0 * rast.n_pix + 48.0 * touched
-    return total, {"streamed_entries": E, "tiles": rast.n_tiles, "rays": rast.n_pix, "touched_voxels": touched}
+    total = 96.0 * E_b + 12.0 * rast.n_tiles + 24.0 * rast.n_pix + 48.0 * touched
+    return total, {"streamed_entries": E_b, "forward_streamed_entries": E_f, "tiles": rast.n_tiles,
+                   "rays": rast.n_pix, "touched_voxels": touched}
 
 
 def traffic_from_profiles(kernel_prefix):
@@ -162,7 +167,8 @@ def kernel_times(lib, fn):
 
 def train_iteration_bytes(it, smooth):
     """Algorithmic HBM bytes of the non-render stages of one training iteration.
-    loss: colour, target and scale read, dL/dcolour written (4 x 12 B per ray).
+    loss: colour and the scaled target read, dL/dcolour written (3 x 12 B per ray; with views of
+    different sizes a per-element scale is read too, 4 x 12 B).
     smooth: every grid key once (key 4, active index 4, voxel id 4), the mask and dense index of
     the res^3 grid once (1 + 4 B per cell), the voxel data of the table voxels once (corner ids
     64, grid coordinate 12, size 4), corner SDFs read once and the gradient read + written
@@ -170,7 +176,7 @@ def train_iteration_bytes(it, smooth):
     adam: corner SDFs, their moments and gradient (p, m, v read + written, g read: 28 B each)."""
     R = it.r.n_pix
     M = it.grid_pts.numel()
-    out = {"loss": 48.0 * R, "adam": 28.0 * M}
+    out = {"loss": (36.0 if it.loss_grad.scale is None else 48.0) * R, "adam": 28.0 * M}
     if smooth is not None:
         n_tab_vox = int(smooth.g2v.unique().numel())
         out["smooth"] = 12.0 * smooth.G + 5.0 * smooth.res ** 3 + 80.0 * n_tab_vox + 12.0 * M

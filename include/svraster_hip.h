@@ -339,7 +339,8 @@ int svr_batch_sh_backward_adam(void* ws, int n_views, int P, long long n_pixels,
 /* Byte offsets inside the batch workspace of (for inspection, like unpack_ImageState /
  * unpack_BinningState, raster_state.cu:120-230): out[0] tile ranges uint2[n_tiles], out[1]
  * sorted values uint32[K], out[2] per-pixel contributor counts uint32[n_pixels], out[3]
- * last-contributor bound per tile uint32[n_tiles]. */
+ * last-contributor bound per tile uint32[n_tiles], out[4] per tile the range [x, y) of its
+ * contributing entries (those the backward walks) uint2[n_tiles].  `out` holds 5 values. */
 int svr_batch_state_offsets(int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                             long long* out);
 /* Total key count of the last svr_batch_preprocess (synchronises `stream`). */

@@ -138,12 +138,22 @@ class BatchedRasterizer:
     def state_views(self):
         """(ranges int32 [n_tiles, 2], vals int32 [K], n_contrib int32 [n_pixels], tile_last int32
         [n_tiles]) viewed inside the workspace (global tile ids, concatenated pixels)."""
-        off = (ctypes.c_longlong * 4)()
-        _lib.check(self.lib.svr_batch_state_offsets(*self._dims(), off), "svr_batch_state_offsets")
+        off = self._state_offsets()
         K = min(self.key_count(), self.cap)
         i32 = lambda o, n: self.ws[o:o + 4 * n].view(torch.int32)
         return (i32(off[0], 2 * self.n_tiles).view(self.n_tiles, 2), i32(off[1], K), i32(off[2], self.n_pix),
                 i32(off[3], self.n_tiles))
+
+    def _state_offsets(self):
+        off = (ctypes.c_longlong * 5)()
+        _lib.check(self.lib.svr_batch_state_offsets(*self._dims(), off), "svr_batch_state_offsets")
+        return off
+
+    def contrib_ranges(self):
+        """Per tile, the range [x, y) of its contributing entries in the list the forward wrote for
+        the backward (int32 [n_tiles, 2], inside the workspace)."""
+        o = self._state_offsets()[4]
+        return self.ws[o:o + 8 * self.n_tiles].view(torch.int32).view(self.n_tiles, 2)
 
     def key_count(self) -> int:
         k = ctypes.c_longlong(0)
@@ -221,6 +231,27 @@ class BatchedRasterizer:
         return self.dgrid, dsh0, dshs, self.ds, self.dvox[:, 11]
 
 
+class MseColorGrad:
+    """dL/dcolor of the mean squared error per view, summed over views:
+    (color - target) * 2 / (3 * pixels of the view).  When every view has the same size it is one
+    fused pass, color * a - target * a with target * a precomputed; else per-element scales."""
+
+    def __init__(self, targets: List[torch.Tensor], pix: List[int], dev):
+        self.target = torch.cat([t.reshape(-1) for t in targets]).contiguous()
+        if len(set(pix)) == 1:
+            self.alpha = 2.0 / (3 * pix[0])
+            self.neg_ts = self.target * -self.alpha
+            self.scale = None
+        else:
+            self.scale = torch.cat([torch.full((3 * p,), 2.0 / (3 * p), device=dev) for p in pix])
+
+    def __call__(self, color: torch.Tensor, out: torch.Tensor) -> None:
+        if self.scale is None:
+            torch.add(self.neg_ts, color, alpha=self.alpha, out=out)
+        else:
+            torch.mul(color - self.target, self.scale, out=out)
+
+
 class BatchedTrainStep:
     """fwd + MSE-to-target + bwd of all views as one step, optionally captured as a HIP graph.
     Gradients land in `grid_grad`, `sh0_grad`, `shs_grad`, `s_grad`."""
@@ -228,9 +259,8 @@ class BatchedTrainStep:
     def __init__(self, rast: BatchedRasterizer, grid_pts, sh0, shs, log_s, targets: List[torch.Tensor]):
         self.r = rast
         self.grid_pts, self.sh0, self.shs, self.log_s = grid_pts, sh0, shs, log_s
-        self.target = torch.cat([t.reshape(-1) for t in targets]).contiguous()
-        scale = [torch.full((3 * p,), 2.0 / (3 * p), device=rast.dev) for p in rast.pix]
-        self.scale = torch.cat(scale)
+        self.loss_grad = MseColorGrad(targets, rast.pix, rast.dev)
+        self.target = self.loss_grad.target
         self.s_val = torch.empty(1, dtype=torch.float32, device=rast.dev)
         self.dC = torch.empty_like(self.target)
         self.graph = None
@@ -238,7 +268,7 @@ class BatchedTrainStep:
     def _run(self):
         torch.exp(self.log_s.reshape(1) * 10.0, out=self.s_val)
         color = self.r.forward(self.grid_pts, self.sh0, self.shs, self.s_val)
-        torch.mul(color - self.target, self.scale, out=self.dC)
+        self.loss_grad(color, self.dC)
         self.grid_grad, self.sh0_grad, self.shs_grad, self.s_grad, _ = self.r.backward(
             self.dC, self.grid_pts.shape[0])
 
@@ -436,8 +466,8 @@ class BatchedTrainIteration:
         self.r = rast
         self.dev = rast.dev
         self.grid_pts, self.sh0, self.shs, self.log_s = grid_pts, sh0, shs, log_s
-        self.target = torch.cat([t.reshape(-1) for t in targets]).contiguous()
-        self.scale = torch.cat([torch.full((3 * p,), 2.0 / (3 * p), device=self.dev) for p in rast.pix])
+        self.loss_grad = MseColorGrad(targets, rast.pix, self.dev)
+        self.target = self.loss_grad.target
         self.s_val = torch.empty(1, dtype=torch.float32, device=self.dev)
         self.dC = torch.empty_like(self.target)
         self.step = torch.zeros(1, dtype=torch.int32, device=self.dev)
@@ -460,7 +490,7 @@ class BatchedTrainIteration:
         self.color = self.r.forward(self.grid_pts, self.sh0, self.shs, self.s_val)
 
     def stage_loss(self):
-        torch.mul(self.color - self.target, self.scale, out=self.dC)
+        self.loss_grad(self.color, self.dC)
 
     def stage_backward(self):
         self.grid_grad, _, _, self.s_grad, _ = self.r.backward(self.dC, self.grid_pts.shape[0],

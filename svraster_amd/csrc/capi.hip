@@ -885,6 +885,7 @@ int svr_batch_state_offsets(int n_views, int P, long long n_pixels, long long n_
   out[1] = (long long)(g + bl.vals);
   out[2] = (long long)(im + il.n_contrib);
   out[3] = (long long)(im + il.tile_last);
+  out[4] = (long long)(im + il.cranges);
   return 0;
 }
 
