@@ -469,29 +469,38 @@ __global__ __launch_bounds__(256) void pair_gather_kernel(long long n_pairs, con
 
 // Keys of each (voxel, octant) pair at its rank-order offset: global tile id and value
 // octant << 29 | voxel, view by view and tile by tile.  A workgroup handles 256 consecutive pairs,
-// whose keys form one contiguous output range; when that range fits kEmitBuf keys they are
-// assembled in LDS and written with coalesced stores (per-thread runs are short and would
-// otherwise leave partly written cache lines), else each thread writes its run directly.
+// whose keys form one contiguous output range; it assembles that range in LDS, kEmitBuf keys at a
+// time, and writes it with coalesced stores (per-thread runs are short and would otherwise leave
+// partly written cache lines).
 constexpr int kEmitBuf = 4096;
 
 __device__ inline void emit_pair_keys(const ViewDesc* __restrict__ views, int V, int P, uint32_t vox, uint32_t q,
                                       const uint2* __restrict__ bboxes, const uint32_t* __restrict__ vmask,
-                                      uint32_t* tiles, uint32_t* vals, long long off, long long lim) {
+                                      uint32_t* tiles, uint32_t* vals, long long off, long long lo, long long hi) {
+  // Writes the pair's keys whose output position (off, off + 1, ...) lies in [lo, hi), at
+  // position - lo.
   const uint32_t val = (q << 29) | vox;
   const int nw = (V + 31) >> 5;
   for (int wi = 0; wi < nw; ++wi)
   for (uint32_t m = vmask[(8LL * vox + q) * nw + wi]; m; m &= m - 1u) {
+    if (off >= hi) return;
     const int v = (wi << 5) + __builtin_ctz(m);
     const long long o = (long long)v * P + vox;
     const ViewDesc& vd = views[v];
     const uint2 bb = bboxes[o];
     int tx0, ty0, tx1, ty1;
     bbox_tiles(bb.x, bb.y, vd, tx0, ty0, tx1, ty1);
+    const int tw = tx1 - tx0 + 1;
+    const long long nv = (long long)tw * (ty1 - ty0 + 1);
+    if (off + nv <= lo) {   // the view's keys all precede the window
+      off += nv;
+      continue;
+    }
     for (int y = ty0; y <= ty1; ++y)
       for (int x = tx0; x <= tx1; ++x) {
-        if (off < lim) {
-          tiles[off] = (uint32_t)(vd.tile_base + y * vd.tiles_x + x);
-          vals[off] = val;
+        if (off >= lo && off < hi) {
+          tiles[off - lo] = (uint32_t)(vd.tile_base + y * vd.tiles_x + x);
+          vals[off - lo] = val;
         }
         ++off;
       }
@@ -507,10 +516,10 @@ __global__ __launch_bounds__(256) void pair_emit_kernel(
   const long long i0 = (long long)blockIdx.x * 256;
   const long long i = i0 + threadIdx.x;
   const long long i_last = min(n_pairs, i0 + 256) - 1;
-  // Output range of the block: [first, end).
+  // Output range of the block: [first, end), staged through LDS kEmitBuf keys at a time so the
+  // global writes are coalesced.
   const long long first = (long long)scan_ranked[i0] - cnt_ranked[i0];
-  const long long end = scan_ranked[i_last];
-  const long long total = end - first;
+  const long long end = min((long long)scan_ranked[i_last], cap);
   uint32_t c = 0, pv = 0;
   long long off = 0;
   if (i < n_pairs) {
@@ -520,18 +529,16 @@ __global__ __launch_bounds__(256) void pair_emit_kernel(
       off = (long long)scan_ranked[i] - c;
     }
   }
-  if (total <= kEmitBuf) {
-    if (c) emit_pair_keys(views, V, P, pv >> 3, pv & 7u, bboxes, vmask, s_tiles, s_vals, off - first, kEmitBuf);
+  for (long long lo = first; lo < end; lo += kEmitBuf) {
+    const long long hi = min(end, lo + kEmitBuf);
+    if (c && off < hi && off + c > lo)
+      emit_pair_keys(views, V, P, pv >> 3, pv & 7u, bboxes, vmask, s_tiles, s_vals, off, lo, hi);
     __syncthreads();
-    for (long long k = threadIdx.x; k < total; k += 256) {
-      const long long dst = first + k;
-      if (dst < cap) {
-        tiles[dst] = s_tiles[k];
-        vals[dst] = s_vals[k];
-      }
+    for (long long k = threadIdx.x; k < hi - lo; k += 256) {
+      tiles[lo + k] = s_tiles[k];
+      vals[lo + k] = s_vals[k];
     }
-  } else if (c) {
-    emit_pair_keys(views, V, P, pv >> 3, pv & 7u, bboxes, vmask, tiles, vals, off, cap);
+    __syncthreads();
   }
 }
 
