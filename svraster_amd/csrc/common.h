@@ -403,12 +403,6 @@ __device__ inline float wave_sum_uniform(float x) {
   return (r0 + r1) + (r2 + r3);
 }
 
-__device__ inline float wave_sum(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
 // Minimum over the currently active lanes of the wave (inactive lanes contribute nothing).
 __device__ inline uint32_t wave_min_active_u32(uint32_t v) {
   const uint64_t act = __ballot(1);

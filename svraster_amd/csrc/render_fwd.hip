@@ -2,14 +2,16 @@
 //
 // Reference: FORWARD::renderCUDA<..., n_samp = 3, SDF> (cuda/src/forward.cu:42-564).
 //
-// One 256-thread workgroup per 16x16 tile; wave w covers the 8x8 quadrant (w&1, w>>1) and
-// lane l the pixel (l&7, l>>3) inside it.  The tile's sorted entries are staged 256 at a time
-// into LDS with the per-entry constants the pixel loop needs (box centre / half size, box
-// minimum and 1/size, 8 corner SDFs, colour, packed bbox, normalised linear normal), so the
-// per-pixel loop reads only LDS broadcasts.  The loop over staged entries is wave-uniform: a
-// lane whose pixel misses the entry is simply masked, and a wave whose 64 pixels are all
-// saturated leaves the batch early.  Optionally the workgroups pull tiles from a global
-// (view, tile) queue so one launch can serve many views.
+// One 256-thread workgroup per 256-pixel tile (16x16, 32x8 or 8x32), four wavefronts of 64
+// pixels.  The tile's sorted entries are staged into LDS with the per-entry constants the pixel
+// work needs (box centre / half size, box minimum and 1/size, 8 corner SDFs, colour, packed bbox,
+// normalised linear normal).  Two kernels:
+//   render_fwd_kernel        one entry at a time, wave-uniform, wave w on the 8x8 quadrant w
+//                            (lane l: pixel (l&7, l>>3) of it); 256 entries per round;
+//   render_fwd_pairs_kernel  the default, pair-compacted (below), wave w on an interleaved 8x8
+//                            lattice of the tile.
+// Optionally the workgroups pull tiles from a global (view, tile) queue so one launch can serve
+// many views.
 #include "common.h"
 #include "state.h"
 
@@ -283,8 +285,10 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
 // ------------------------------------------------------------------------------------------
 // Pair compositor (default).  Most voxels cover a handful of a tile's pixels and most (pixel,
 // voxel) candidates have alpha below the minimum, so the cost must follow the pairs that exist,
-// not the entries times the pixels.  Per staged batch of 128 entries each wave:
-//   1. builds the 64-bit mask of every entry's pixels in its 8x8 block (bbox, octant, pixels
+// not the entries times the pixels.  Wave w takes the interleaved lattice of the tile's pixels
+// (wave_lattice: strides 2x2 on 16x16 tiles), so every entry's footprint falls about evenly on the
+// four waves and their work per round is balanced.  Per staged batch of 128 entries each wave:
+//   1. builds the 64-bit mask of every entry's pixels in its lattice (bbox, octant, pixels
 //      still unsaturated; empty for an entry whose corner SDFs cannot reach the minimum alpha),
 //      lane l holding entries l and l + 64, and prefix-sums the counts: entry j owns the run
 //      [pref_j, pref_j + |m_j|) of an entry-major pair list;
@@ -294,9 +298,9 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
 //   3. records the contributing pairs (alpha >= kMinAlpha) in a 128-bit entry set per pixel;
 //   4. composites each pixel over its set in entry order.
 // Per-pixel arithmetic and order are those of the one-entry-at-a-time kernel, so the outputs are
-// identical bit for bit.
+// identical bit for bit.  Each round's contributing entries (and, per contributing entry and 8x8
+// quadrant, the pixels that composited it) are written for the backward.
 // ------------------------------------------------------------------------------------------
-constexpr int kPairStage = 128;   // entries staged per workgroup round (two 64-entry halves)
 constexpr int kPairWin = 192;     // pair positions held per wave at once
 
 // Pixels of the wave's 8x8 lattice (bit = 8 * row + column) inside a packed bbox.
