@@ -491,16 +491,26 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
           }
           wave_lds_sync_b();
 
-          // 3. sequential reverse walk per pixel over its entries whose pair lies in the window.
-          while ((cs_lo | cs_hi) != 0ull) {
-            const int j = cs_lo ? __builtin_ctzll(cs_lo) : 64 + __builtin_ctzll(cs_hi);
-            const uint32_t k = (uint32_t)s_pref[w][j] + (uint32_t)__popcll(s_m[w][j] & lt_mask);
-            if (k >= w1) break;
+          // 3. sequential reverse walk per pixel over its entries whose pair lies in the window,
+          // software-pipelined: the next entry's position is looked up while this one is walked.
+          int jc = -1;
+          uint32_t kc = 0;
+          if ((cs_lo | cs_hi) != 0ull) {
+            jc = cs_lo ? __builtin_ctzll(cs_lo) : 64 + __builtin_ctzll(cs_hi);
+            kc = (uint32_t)s_pref[w][jc] + (uint32_t)__popcll(s_m[w][jc] & lt_mask);
+          }
+          while (jc >= 0 && kc < w1) {
+            const int j = jc;
+            const uint32_t kk = kc - w0;
             if (cs_lo) cs_lo &= cs_lo - 1ull; else cs_hi &= cs_hi - 1ull;
-            const uint32_t kk = k - w0;
             const float alpha = s_pa[w][kk];
-            T = T / (1.f - alpha);
             const float4 rgb = s_rgb[j];
+            jc = -1;
+            if ((cs_lo | cs_hi) != 0ull) {
+              jc = cs_lo ? __builtin_ctzll(cs_lo) : 64 + __builtin_ctzll(cs_hi);
+              kc = (uint32_t)s_pref[w][jc] + (uint32_t)__popcll(s_m[w][jc] & lt_mask);
+            }
+            T = T / (1.f - alpha);
             float dptw = dC0 * rgb.x + dC1 * rgb.y + dC2 * rgb.z;
             if (kNormal) {
               const float4 nn = s_nrm[j];
