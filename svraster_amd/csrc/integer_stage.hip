@@ -104,7 +104,8 @@ __global__ __launch_bounds__(256) void preprocess_kernel(
   quads[o] = (uint8_t)qbits;
   if (tz_min) {
     // Number of low rank bits (a multiple of 3) that are zero in this voxel's octpath; reduced
-    // across the wave so that one atomic per wave reaches memory.
+    // across the wave, and the wave's atomic is skipped when it cannot lower the current minimum
+    // (all waves hit one address, so issuing every atomic would serialise them).
     uint32_t tz = (uint32_t)kRankBits;
     if (nd) {
       const uint64_t p = (uint64_t)octpaths[i] & ((1ull << kRankBits) - 1ull);
@@ -112,7 +113,9 @@ __global__ __launch_bounds__(256) void preprocess_kernel(
       tz = min(tz - tz % 3u, (uint32_t)kRankBits);
     }
     tz = wave_min_active_u32(tz);
-    if (__lane_id() == __ffsll((long long)__ballot(1)) - 1 && tz < (uint32_t)kRankBits) atomicMin(tz_min, tz);
+    if (__lane_id() == __ffsll((long long)__ballot(1)) - 1 && tz < (uint32_t)kRankBits &&
+        tz < __hip_atomic_load(tz_min, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMin(tz_min, tz);
   }
 }
 

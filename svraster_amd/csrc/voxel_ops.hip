@@ -53,53 +53,115 @@ __device__ inline void sh_dir(const float* __restrict__ centers, const float* __
   x *= rn; y *= rn; z *= rn;
 }
 
-__global__ __launch_bounds__(256) void sh_forward_kernel(int n, const int64_t* __restrict__ idx, int deg,
-                                                         int M, const float* __restrict__ centers,
+// Per-voxel SH colour of one view (voxels t of the list, or of idx).  Each wavefront moves the
+// coefficient rows of its 64 voxels through LDS: consecutive lanes read consecutive floats of a
+// row, so a load instruction touches a few cache lines rather than one per lane.  kDeg is the
+// SH degree; a row holds 3 * ((kDeg + 1)^2 - 1) used floats of its 3 * (M - 1).
+template <int kDeg>
+struct ShRows {
+  static constexpr int kNb = (kDeg + 1) * (kDeg + 1);
+  static constexpr int kL = 3 * (kNb - 1);                  // floats used per row
+  static constexpr int kS = (kL % 2 == 1 || kL == 0) ? (kL > 0 ? kL : 1) : kL + 1;   // odd LDS stride
+};
+
+template <int kDeg>
+__global__ __launch_bounds__(256) void sh_forward_kernel(int n, const int64_t* __restrict__ idx, int M,
+                                                         const float* __restrict__ centers,
                                                          const float* __restrict__ cam,
                                                          const float* __restrict__ sh0,
                                                          const float* __restrict__ shs,
                                                          float* __restrict__ rgbs) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int v = idx ? (int)idx[t] : t;
-  float x, y, z;
-  sh_dir(centers, cam, v, x, y, z);
+  constexpr int kL = ShRows<kDeg>::kL, kS = ShRows<kDeg>::kS;
+  __shared__ float s_row[4][64 * kS];
+  __shared__ int s_vox[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int t0 = blockIdx.x * 256 + w * 64;
+  if (t0 >= n) return;                       // whole wavefront past the end
+  const int cnt = min(64, n - t0);
+  const bool valid = lane < cnt;
+  const int v = valid ? (idx ? (int)idx[t0 + lane] : t0 + lane) : 0;
+  float r0 = 0.f, r1 = 0.f, r2 = 0.f;
   float b[16];
-  const int nb = sh_basis(deg, x, y, z, b);
-  float r0 = b[0] * sh0[3 * v], r1 = b[0] * sh0[3 * v + 1], r2 = b[0] * sh0[3 * v + 2];
-  const float* c = shs + (long long)v * (M - 1) * 3;
-  for (int k = 1; k < nb; ++k) {
-    r0 += b[k] * c[3 * (k - 1)];
-    r1 += b[k] * c[3 * (k - 1) + 1];
-    r2 += b[k] * c[3 * (k - 1) + 2];
+  if (valid) {
+    float x, y, z;
+    sh_dir(centers, cam, v, x, y, z);
+    sh_basis(kDeg, x, y, z, b);
+    r0 = b[0] * sh0[3 * v]; r1 = b[0] * sh0[3 * v + 1]; r2 = b[0] * sh0[3 * v + 2];
   }
+  if (kL > 0) {
+    s_vox[w][lane] = v;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const long long stride = 3LL * (M - 1);
+#pragma unroll
+    for (int i = 0; i < kL; ++i) {
+      const int e = lane + 64 * i;
+      const int row = e / kL, f = e - row * kL;
+      if (row < cnt) s_row[w][row * kS + f] = shs[(long long)s_vox[w][row] * stride + f];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int k = 1; k < ShRows<kDeg>::kNb; ++k) {
+      const float* c = &s_row[w][lane * kS + 3 * (k - 1)];
+      r0 += b[k] * c[0];
+      r1 += b[k] * c[1];
+      r2 += b[k] * c[2];
+    }
+  }
+  if (!valid) return;
   r0 += 0.5f; r1 += 0.5f; r2 += 0.5f;
   rgbs[3 * v] = fmaxf(r0, 0.f);
   rgbs[3 * v + 1] = fmaxf(r1, 0.f);
   rgbs[3 * v + 2] = fmaxf(r2, 0.f);
 }
 
-// Transpose of the linear part; the forward clamp is not masked (sh_compute.cu:118-121).
-__global__ __launch_bounds__(256) void sh_backward_kernel(int n, const int64_t* __restrict__ idx, int deg,
-                                                          int M, const float* __restrict__ centers,
+// Transpose of the linear part; the forward clamp is not masked (sh_compute.cu:118-121).  Rows
+// of dL/dshs are assembled in LDS and written with consecutive lanes on consecutive floats.
+template <int kDeg>
+__global__ __launch_bounds__(256) void sh_backward_kernel(int n, const int64_t* __restrict__ idx, int M,
+                                                          const float* __restrict__ centers,
                                                           const float* __restrict__ cam,
                                                           const float* __restrict__ dL_drgb,
                                                           float* __restrict__ dsh0,
                                                           float* __restrict__ dshs) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int v = idx ? (int)idx[t] : t;
-  float x, y, z;
-  sh_dir(centers, cam, v, x, y, z);
+  constexpr int kL = ShRows<kDeg>::kL, kS = ShRows<kDeg>::kS;
+  __shared__ float s_row[4][64 * kS];
+  __shared__ int s_vox[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int t0 = blockIdx.x * 256 + w * 64;
+  if (t0 >= n) return;
+  const int cnt = min(64, n - t0);
+  const bool valid = lane < cnt;
+  const int v = valid ? (idx ? (int)idx[t0 + lane] : t0 + lane) : 0;
   float b[16];
-  const int nb = sh_basis(deg, x, y, z, b);
-  const float g0 = dL_drgb[3 * v], g1 = dL_drgb[3 * v + 1], g2 = dL_drgb[3 * v + 2];
-  dsh0[3 * v] = b[0] * g0; dsh0[3 * v + 1] = b[0] * g1; dsh0[3 * v + 2] = b[0] * g2;
-  float* c = dshs + (long long)v * (M - 1) * 3;
-  for (int k = 1; k < nb && k < M; ++k) {
-    c[3 * (k - 1)] = b[k] * g0;
-    c[3 * (k - 1) + 1] = b[k] * g1;
-    c[3 * (k - 1) + 2] = b[k] * g2;
+  float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+  if (valid) {
+    float x, y, z;
+    sh_dir(centers, cam, v, x, y, z);
+    sh_basis(kDeg, x, y, z, b);
+    g0 = dL_drgb[3 * v]; g1 = dL_drgb[3 * v + 1]; g2 = dL_drgb[3 * v + 2];
+    dsh0[3 * v] = b[0] * g0; dsh0[3 * v + 1] = b[0] * g1; dsh0[3 * v + 2] = b[0] * g2;
+  }
+  const int lw = min(kL, 3 * (M - 1));   // floats of the row that exist
+  if (kL > 0 && lw > 0) {
+    s_vox[w][lane] = v;
+    if (valid) {
+#pragma unroll
+      for (int k = 1; k < ShRows<kDeg>::kNb; ++k) {
+        float* c = &s_row[w][lane * kS + 3 * (k - 1)];
+        c[0] = b[k] * g0; c[1] = b[k] * g1; c[2] = b[k] * g2;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const long long stride = 3LL * (M - 1);
+#pragma unroll
+    for (int i = 0; i < kL; ++i) {
+      const int e = lane + 64 * i;
+      const int row = e / kL, f = e - row * kL;
+      if (row < cnt && f < lw) dshs[(long long)s_vox[w][row] * stride + f] = s_row[w][row * kS + f];
+    }
   }
 }
 
@@ -445,7 +507,11 @@ hipError_t launch_sh_forward(int n, const int64_t* idx, int deg, int M, const fl
                              const float* cam, const float* sh0, const float* shs, float* rgbs,
                              hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  sh_forward_kernel<<<blocks_for(n), 256, 0, st>>>(n, idx, deg, M, centers, cam, sh0, shs, rgbs);
+  const unsigned g = blocks_for(n);
+  if (deg == 0) sh_forward_kernel<0><<<g, 256, 0, st>>>(n, idx, M, centers, cam, sh0, shs, rgbs);
+  else if (deg == 1) sh_forward_kernel<1><<<g, 256, 0, st>>>(n, idx, M, centers, cam, sh0, shs, rgbs);
+  else if (deg == 2) sh_forward_kernel<2><<<g, 256, 0, st>>>(n, idx, M, centers, cam, sh0, shs, rgbs);
+  else sh_forward_kernel<3><<<g, 256, 0, st>>>(n, idx, M, centers, cam, sh0, shs, rgbs);
   return hipGetLastError();
 }
 
@@ -453,7 +519,11 @@ hipError_t launch_sh_backward(int n, const int64_t* idx, int deg, int M, const f
                               const float* cam, const float* dL_drgb, float* dsh0, float* dshs,
                               hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  sh_backward_kernel<<<blocks_for(n), 256, 0, st>>>(n, idx, deg, M, centers, cam, dL_drgb, dsh0, dshs);
+  const unsigned g = blocks_for(n);
+  if (deg == 0) sh_backward_kernel<0><<<g, 256, 0, st>>>(n, idx, M, centers, cam, dL_drgb, dsh0, dshs);
+  else if (deg == 1) sh_backward_kernel<1><<<g, 256, 0, st>>>(n, idx, M, centers, cam, dL_drgb, dsh0, dshs);
+  else if (deg == 2) sh_backward_kernel<2><<<g, 256, 0, st>>>(n, idx, M, centers, cam, dL_drgb, dsh0, dshs);
+  else sh_backward_kernel<3><<<g, 256, 0, st>>>(n, idx, M, centers, cam, dL_drgb, dsh0, dshs);
   return hipGetLastError();
 }
 

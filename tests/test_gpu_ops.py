@@ -15,23 +15,23 @@ from tests._util import rel_l2, say
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("deg", [0, 1, 2, 3])
-def test_sh_forward_backward(deg):
-    say(f"[sh] degree {deg}")
+@pytest.mark.parametrize("deg,use_idx", [(0, True), (1, True), (2, True), (3, True), (3, False), (2, False)])
+def test_sh_forward_backward(deg, use_idx):
+    say(f"[sh] degree {deg}, {'index subset' if use_idx else 'all voxels'}")
     rng = np.random.default_rng(deg)
-    P = 5000
+    P = 5000 if use_idx else 4999          # all voxels: a partial last wavefront
     centers = torch.from_numpy(rng.normal(size=(P, 3))).float()
     cam = torch.tensor([2.0, -1.0, 0.5])
     sh0 = torch.from_numpy(rng.normal(size=(P, 3))).float()
     shs = torch.from_numpy(rng.normal(scale=0.3, size=(P, 15, 3))).float()
-    idx = torch.from_numpy(rng.choice(P, 3000, replace=False)).long()
-    got = _C.sh_compute(deg, idx.cuda(), centers.cuda(), cam.cuda(), sh0.cuda(), shs.cuda())
+    idx = torch.from_numpy(rng.choice(P, 3000, replace=False)).long() if use_idx else None
+    got = _C.sh_compute(deg, idx.cuda() if use_idx else None, centers.cuda(), cam.cuda(), sh0.cuda(), shs.cuda())
     s0 = sh0.double().requires_grad_(True)
     ss = shs.double().requires_grad_(True)
     ref = osh.sh_rgb(deg, idx, centers.double(), cam.double(), s0, ss)
     assert (got.cpu().double() - ref.detach()).abs().max().item() <= 1e-5
     g = torch.from_numpy(rng.normal(size=(P, 3))).float()
-    d0, dss = _C.sh_compute_bw(deg, 16, idx.cuda(), centers.cuda(), cam.cuda(), got, g.cuda())
+    d0, dss = _C.sh_compute_bw(deg, 16, idx.cuda() if use_idx else None, centers.cuda(), cam.cuda(), got, g.cuda())
     r0, rss = torch.autograd.grad((ref * g.double()).sum(), [s0, ss])
     assert rel_l2(d0, r0) <= 1e-6 and rel_l2(dss, rss) <= 1e-6
 
