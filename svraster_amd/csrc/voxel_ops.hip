@@ -54,14 +54,17 @@ __device__ inline void sh_dir(const float* __restrict__ centers, const float* __
 }
 
 // Per-voxel SH colour of one view (voxels t of the list, or of idx).  Each wavefront moves the
-// coefficient rows of its 64 voxels through LDS: consecutive lanes read consecutive floats of a
-// row, so a load instruction touches a few cache lines rather than one per lane.  kDeg is the
-// SH degree; a row holds 3 * ((kDeg + 1)^2 - 1) used floats of its 3 * (M - 1).
+// coefficient rows of its 64 voxels through LDS, 15 floats (5 coefficients) of every row at a
+// time: consecutive lanes read consecutive floats of a row, so a load instruction touches a few
+// cache lines rather than one per lane.  kDeg is the SH degree; a row holds
+// 3 * ((kDeg + 1)^2 - 1) used floats of its 3 * (M - 1).
 template <int kDeg>
 struct ShRows {
   static constexpr int kNb = (kDeg + 1) * (kDeg + 1);
-  static constexpr int kL = 3 * (kNb - 1);                  // floats used per row
-  static constexpr int kS = (kL % 2 == 1 || kL == 0) ? (kL > 0 ? kL : 1) : kL + 1;   // odd LDS stride
+  static constexpr int kL = 3 * (kNb - 1);     // floats used per row
+  static constexpr int kC = 15;                // floats per chunk (odd: conflict-free LDS rows)
+  static constexpr int kChunks = (kL + kC - 1) / kC;
+  static constexpr int kRowStride = (kL % 2 == 1 || kL == 0) ? (kL > 0 ? kL : 1) : kL + 1;   // whole rows, odd
 };
 
 template <int kDeg>
@@ -71,8 +74,8 @@ __global__ __launch_bounds__(256) void sh_forward_kernel(int n, const int64_t* _
                                                          const float* __restrict__ sh0,
                                                          const float* __restrict__ shs,
                                                          float* __restrict__ rgbs) {
-  constexpr int kL = ShRows<kDeg>::kL, kS = ShRows<kDeg>::kS;
-  __shared__ float s_row[4][64 * kS];
+  using R = ShRows<kDeg>;
+  __shared__ float s_row[4][64 * R::kC];
   __shared__ int s_vox[4][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int t0 = blockIdx.x * 256 + w * 64;
@@ -88,25 +91,34 @@ __global__ __launch_bounds__(256) void sh_forward_kernel(int n, const int64_t* _
     sh_basis(kDeg, x, y, z, b);
     r0 = b[0] * sh0[3 * v]; r1 = b[0] * sh0[3 * v + 1]; r2 = b[0] * sh0[3 * v + 2];
   }
-  if (kL > 0) {
+  if (R::kL > 0) {
     s_vox[w][lane] = v;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const long long stride = 3LL * (M - 1);
 #pragma unroll
-    for (int i = 0; i < kL; ++i) {
-      const int e = lane + 64 * i;
-      const int row = e / kL, f = e - row * kL;
-      if (row < cnt) s_row[w][row * kS + f] = shs[(long long)s_vox[w][row] * stride + f];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    for (int c = 0; c < R::kChunks; ++c) {
+      const int lc = min(R::kC, R::kL - R::kC * c);   // floats of this chunk (constant once unrolled)
 #pragma unroll
-    for (int k = 1; k < ShRows<kDeg>::kNb; ++k) {
-      const float* c = &s_row[w][lane * kS + 3 * (k - 1)];
-      r0 += b[k] * c[0];
-      r1 += b[k] * c[1];
-      r2 += b[k] * c[2];
+      for (int i = 0; i < R::kC; ++i) {
+        if (i >= lc) break;
+        const int e = lane + 64 * i;
+        const int row = e / lc, f = e - row * lc;
+        if (row < cnt) s_row[w][row * lc + f] = shs[(long long)s_vox[w][row] * stride + R::kC * c + f];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int kk = 0; kk < R::kC / 3; ++kk) {
+        if (3 * kk >= lc) break;
+        const float* cf = &s_row[w][lane * lc + 3 * kk];
+        const float bk = b[1 + (R::kC / 3) * c + kk];
+        r0 += bk * cf[0];
+        r1 += bk * cf[1];
+        r2 += bk * cf[2];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
     }
   }
   if (!valid) return;
@@ -125,7 +137,7 @@ __global__ __launch_bounds__(256) void sh_backward_kernel(int n, const int64_t* 
                                                           const float* __restrict__ dL_drgb,
                                                           float* __restrict__ dsh0,
                                                           float* __restrict__ dshs) {
-  constexpr int kL = ShRows<kDeg>::kL, kS = ShRows<kDeg>::kS;
+  constexpr int kL = ShRows<kDeg>::kL, kS = ShRows<kDeg>::kRowStride;
   __shared__ float s_row[4][64 * kS];
   __shared__ int s_vox[4][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
