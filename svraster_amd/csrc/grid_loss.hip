@@ -210,8 +210,10 @@ __device__ inline VoxelBox record_box(float4 r) {
   return b;
 }
 
+// One unsigned compare per axis, combined without short-circuit branches.
 __device__ inline bool in_box(const VoxelBox& b, int x, int y, int z) {
-  return x >= b.lx && x < b.lx + b.s && y >= b.ly && y < b.ly + b.s && z >= b.lz && z < b.lz + b.s;
+  const unsigned s = (unsigned)b.s;
+  return ((unsigned)(x - b.lx) < s) & ((unsigned)(y - b.ly) < s) & ((unsigned)(z - b.lz) < s);
 }
 
 // Local coordinates of cell (x, y, z) + r in the voxel box (the reference divides by the size).
@@ -260,7 +262,7 @@ struct TableOrder {
   const int32_t* cell_tv;    // [res^3] compact table voxel of each cell, -1 if none
   const int32_t* tvox;       // [T] voxel id of each compact table voxel
   const float4* rec;         // [T][3] box + corner SDF records
-  int G, T, res;
+  int G, T, res, lg;         // lg = log2(res): the table's resolution is a power of two
 };
 
 __global__ __launch_bounds__(256) void grid_laplacian_coef_kernel(TableOrder O, const bool* __restrict__ mask,
@@ -272,8 +274,10 @@ __global__ __launch_bounds__(256) void grid_laplacian_coef_kernel(TableOrder O, 
   if (e >= n) return;
   const int c = O.keys[e];
   const int t = O.tv[e];
-  const int x = c % res, y = (c / res) % res, z = c / (res * res);
-  bool valid = t >= 0 && t < O.T && x > 0 && x < res - 1 && y > 0 && y < res - 1 && z > 0 && z < res - 1;
+  const int x = c & (res - 1), y = (c >> O.lg) & (res - 1), z = c >> (2 * O.lg);
+  const unsigned inner = (unsigned)(res - 2);   // x, y, z in 1 .. res-2
+  bool valid = ((unsigned)t < (unsigned)O.T) & ((unsigned)(x - 1) < inner) & ((unsigned)(y - 1) < inner) &
+               ((unsigned)(z - 1) < inner);
   const int ts = valid ? t : 0;
   const float4 r0 = O.rec[3LL * ts], g0 = O.rec[3LL * ts + 1], g1 = O.rec[3LL * ts + 2];
   const VoxelBox b = record_box(r0);
@@ -285,8 +289,8 @@ __global__ __launch_bounds__(256) void grid_laplacian_coef_kernel(TableOrder O, 
 #pragma unroll
   for (int i = 1; i < 7; ++i) {
     const int qx = x + dx[i], qy = y + dy[i], qz = z + dz[i];
-    ob[i] = valid && !in_box(b, qx, qy, qz);
-    cell[i] = valid ? qx + qy * res + qz * res * res : c;
+    ob[i] = valid & !in_box(b, qx, qy, qz);
+    cell[i] = valid ? qx + (qy << O.lg) + (qz << (2 * O.lg)) : c;
   }
   bool m[7];
 #pragma unroll
@@ -296,13 +300,13 @@ __global__ __launch_bounds__(256) void grid_laplacian_coef_kernel(TableOrder O, 
   }
 #pragma unroll
   for (int i = 1; i < 7; ++i) {
-    valid = valid && (!ob[i] || m[i]);
+    valid = valid & (!ob[i] | m[i]);
     tq[i] = ob[i] ? tq[i] : ts;
   }
   float4 qr[7], qg0[7], qg1[7];
 #pragma unroll
   for (int i = 1; i < 7; ++i) {
-    const int tt = tq[i] >= 0 && tq[i] < O.T ? tq[i] : ts;
+    const int tt = (unsigned)tq[i] < (unsigned)O.T ? tq[i] : ts;
     qr[i] = O.rec[3LL * tt];
     qg0[i] = O.rec[3LL * tt + 1];
     qg1[i] = O.rec[3LL * tt + 2];
@@ -315,10 +319,10 @@ __global__ __launch_bounds__(256) void grid_laplacian_coef_kernel(TableOrder O, 
     sd[0] = record_sample(b, g0, g1, x, y, z, rx, ry, rz);
 #pragma unroll
     for (int i = 1; i < 7; ++i) {
+      // an in-box neighbour has tq[i] == ts, so its record is the key's own
       const int qx = x + dx[i], qy = y + dy[i], qz = z + dz[i];
-      if (tq[i] < 0 || tq[i] >= O.T) sd[i] = 0.f;
-      else if (!ob[i]) sd[i] = record_sample(b, g0, g1, qx, qy, qz, rx, ry, rz);
-      else sd[i] = record_sample(record_box(qr[i]), qg0[i], qg1[i], qx, qy, qz, rx, ry, rz);
+      const float v = record_sample(record_box(qr[i]), qg0[i], qg1[i], qx, qy, qz, rx, ry, rz);
+      sd[i] = (unsigned)tq[i] < (unsigned)O.T ? v : 0.f;
     }
     const float r0s = sd[1] + sd[2] - 2.f * sd[0];
     const float r1s = sd[3] + sd[4] - 2.f * sd[0];
@@ -346,10 +350,10 @@ __global__ __launch_bounds__(256) void grid_laplacian_pull_kernel(TableOrder O, 
   const int es = live ? e : 0;
   const int c = O.keys[es];
   const int t0 = O.tv[es];
-  const int t = live && t0 >= 0 && t0 < O.T ? t0 : -1;
+  const int t = live & ((unsigned)t0 < (unsigned)O.T) ? t0 : -1;
   const float4 r0 = O.rec[3LL * (t >= 0 ? t : 0)];
   const VoxelBox b = record_box(r0);
-  const int x = c % res, y = (c / res) % res, z = c / (res * res);
+  const int x = c & (res - 1), y = (c >> O.lg) & (res - 1), z = c >> (2 * O.lg);
   // the cell, then the keys whose +x, -x, +y, -y, +z, -z sample lands in it
   const int kx[7] = {0, -1, 1, 0, 0, 0, 0}, ky[7] = {0, 0, 0, -1, 1, 0, 0}, kz[7] = {0, 0, 0, 0, 0, -1, 1};
   int ek[7];
@@ -358,11 +362,11 @@ __global__ __launch_bounds__(256) void grid_laplacian_pull_kernel(TableOrder O, 
 #pragma unroll
   for (int i = 0; i < 7; ++i) {
     const int ax = x + kx[i], ay = y + ky[i], az = z + kz[i];
-    const bool ingrid = ax >= 0 && ax < res && ay >= 0 && ay < res && az >= 0 && az < res;
+    const bool ingrid = ((ax | ay | az) & -res) == 0;   // res is a power of two
     const bool inb = in_box(b, ax, ay, az);
-    use[i] = t >= 0 && ingrid;
+    use[i] = (t >= 0) & ingrid;
     ek[i] = inb ? es + (kz[i] * b.s + ky[i]) * b.s + kx[i] : -1;
-    cell[i] = (!inb && ingrid) ? ax + ay * res + az * res * res : c;
+    cell[i] = (!inb & ingrid) ? ax + (ay << O.lg) + (az << (2 * O.lg)) : c;
   }
   int looked[7];
 #pragma unroll
@@ -370,7 +374,7 @@ __global__ __launch_bounds__(256) void grid_laplacian_pull_kernel(TableOrder O, 
 #pragma unroll
   for (int i = 0; i < 7; ++i) {
     if (ek[i] < 0) ek[i] = looked[i];
-    use[i] = use[i] && ek[i] >= 0;
+    use[i] = use[i] & (ek[i] >= 0);
     if (!use[i]) ek[i] = es;
   }
   float l[7];
@@ -393,11 +397,17 @@ __global__ __launch_bounds__(256) void grid_laplacian_pull_kernel(TableOrder O, 
     if (!use[i] || l[i] == 0.f) continue;
     float rx, ry, rz;
     random_offset(nullptr, sd, ek[i], rx, ry, rz);
-    float w[8], qx, qy, qz;
+    // The sample lies in this cell, inside the voxel's own box, so its local coordinates are in
+    // [0, 1] and the trilinear weights need no clamping; l is folded into the xy products.
+    float qx, qy, qz;
     box_local(b, x, y, z, rx, ry, rz, qx, qy, qz);
-    trilinear_weights(qx, qy, qz, w);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] += l[i] * fminf(fmaxf(w[q], 0.f), 1.f);
+    const float lx0 = l[i] * (1.f - qx), lx1 = l[i] * qx;
+    const float a00 = lx0 * (1.f - qy), a01 = lx0 * qy, a10 = lx1 * (1.f - qy), a11 = lx1 * qy;
+    const float z0 = 1.f - qz;
+    acc[0] = fmaf(a00, z0, acc[0]); acc[1] = fmaf(a00, qz, acc[1]);
+    acc[2] = fmaf(a01, z0, acc[2]); acc[3] = fmaf(a01, qz, acc[3]);
+    acc[4] = fmaf(a10, z0, acc[4]); acc[5] = fmaf(a10, qz, acc[5]);
+    acc[6] = fmaf(a11, z0, acc[6]); acc[7] = fmaf(a11, qz, acc[7]);
   }
   const int v = t >= 0 ? t : -1;
   const int prev = __shfl_up(v, 1);
@@ -491,12 +501,14 @@ hipError_t launch_grid_loss(bool laplacian, const GridTable& G, const bool* mask
 
 hipError_t launch_grid_laplacian_pull(const GridOrderArgs& A, hipStream_t st) {
   if (A.G <= 0) return hipSuccess;
+  if (A.res <= 0 || (A.res & (A.res - 1)) != 0) return hipErrorInvalidValue;
   if (A.T > 0)
     grid_voxel_records_kernel<<<(unsigned)((A.T + 255) / 256), 256, 0, st>>>(A.T, A.tvox, A.coord, A.size, A.vox_key,
                                                                              A.grid_pts, reinterpret_cast<float4*>(A.rec));
   TableOrder O;
   O.keys = A.keys; O.tv = A.tv; O.cell_entry = A.cell_entry; O.cell_tv = A.cell_tv; O.tvox = A.tvox;
   O.rec = reinterpret_cast<const float4*>(A.rec); O.G = A.G; O.T = A.T; O.res = A.res;
+  O.lg = __builtin_ctz((unsigned)A.res);
   const unsigned nb = (unsigned)((A.G + 255) / 256);
   grid_laplacian_coef_kernel<<<nb, 256, 0, st>>>(O, A.mask, A.weight, A.inv, A.seed, A.step, A.coef);
   grid_laplacian_pull_kernel<<<nb, 256, 0, st>>>(O, A.vox_key, A.seed, A.step, A.coef, A.grad);
