@@ -471,76 +471,105 @@ __global__ __launch_bounds__(256) void pair_gather_kernel(long long n_pairs, con
 }
 
 // Keys of each (voxel, octant) pair at its rank-order offset: global tile id and value
-// octant << 29 | voxel, view by view and tile by tile.  A workgroup handles 256 consecutive pairs,
-// whose keys form one contiguous output range; it assembles that range in LDS, kEmitBuf keys at a
-// time, and writes it with coalesced stores (per-thread runs are short and would otherwise leave
-// partly written cache lines).
-constexpr int kEmitBuf = 4096;
-
-__device__ inline void emit_pair_keys(const ViewDesc* __restrict__ views, int V, int P, uint32_t vox, uint32_t q,
-                                      const uint2* __restrict__ bboxes, const uint32_t* __restrict__ vmask,
-                                      uint32_t* tiles, uint32_t* vals, long long off, long long lo, long long hi) {
-  // Writes the pair's keys whose output position (off, off + 1, ...) lies in [lo, hi), at
-  // position - lo.
-  const uint32_t val = (q << 29) | vox;
-  const int nw = (V + 31) >> 5;
-  for (int wi = 0; wi < nw; ++wi)
-  for (uint32_t m = vmask[(8LL * vox + q) * nw + wi]; m; m &= m - 1u) {
-    if (off >= hi) return;
-    const int v = (wi << 5) + __builtin_ctz(m);
-    const long long o = (long long)v * P + vox;
-    const ViewDesc& vd = views[v];
-    const uint2 bb = bboxes[o];
-    int tx0, ty0, tx1, ty1;
-    bbox_tiles(bb.x, bb.y, vd, tx0, ty0, tx1, ty1);
-    const int tw = tx1 - tx0 + 1;
-    const long long nv = (long long)tw * (ty1 - ty0 + 1);
-    if (off + nv <= lo) {   // the view's keys all precede the window
-      off += nv;
-      continue;
-    }
-    for (int y = ty0; y <= ty1; ++y)
-      for (int x = tx0; x <= tx1; ++x) {
-        if (off >= lo && off < hi) {
-          tiles[off - lo] = (uint32_t)(vd.tile_base + y * vd.tiles_x + x);
-          vals[off - lo] = val;
-        }
-        ++off;
-      }
-  }
-}
+// octant << 29 | voxel, view by view and tile by tile.  A workgroup takes 256 consecutive pairs and
+// lists their (pair, view) tile rectangles in LDS: each pair's thread enters its rectangles' views,
+// then each thread completes kEmitRectsPerThread consecutive rectangles (view box, tile count) and a
+// workgroup scan over the tile counts gives every rectangle its output offset.  Every thread then
+// writes whole output positions, finding the rectangle that holds its position by a binary search
+// over the offsets.  So the loads run in parallel across the workgroup, the work per thread does not
+// depend on how the keys are spread over the pairs (a coarse voxel can own thousands of keys while
+// its neighbours in rank order own none), and the stores are coalesced; the small LDS footprint
+// keeps 8 wavefronts per SIMD resident to overlap the workgroups' chains of dependent loads.
+constexpr int kEmitRectsPerThread = 4;
+constexpr int kEmitRects = 256 * kEmitRectsPerThread;
 
 __global__ __launch_bounds__(256) void pair_emit_kernel(
     const ViewDesc* __restrict__ views, int V, int P, long long n_pairs, const uint32_t* __restrict__ order,
     const uint32_t* __restrict__ cnt_ranked, const uint32_t* __restrict__ scan_ranked,
     const uint2* __restrict__ bboxes, const uint32_t* __restrict__ vmask, uint32_t* __restrict__ tiles,
     uint32_t* __restrict__ vals, long long cap) {
-  __shared__ uint32_t s_tiles[kEmitBuf], s_vals[kEmitBuf];
+  __shared__ uint32_t s_off[kEmitRects];    // output offset of the rectangle, from the block's first key
+  __shared__ uint32_t s_tile0[kEmitRects];  // global tile id of its top-left tile
+  __shared__ uint32_t s_shape[kEmitRects];  // view, then width | row stride << 16 (tiles)
+  __shared__ uint32_t s_val[kEmitRects];    // octant << 29 | voxel
+  __shared__ uint32_t s_w[4];
   const long long i0 = (long long)blockIdx.x * 256;
   const long long i = i0 + threadIdx.x;
   const long long i_last = min(n_pairs, i0 + 256) - 1;
-  // Output range of the block: [first, end), staged through LDS kEmitBuf keys at a time so the
-  // global writes are coalesced.
+  // Output range of the block: [first, end), the same for every thread.
   const long long first = (long long)scan_ranked[i0] - cnt_ranked[i0];
   const long long end = min((long long)scan_ranked[i_last], cap);
-  uint32_t c = 0, pv = 0;
-  long long off = 0;
+  if (first >= end) return;
+  const int nw = (V + 31) >> 5;
+  uint32_t c = 0, pv = 0, nrect = 0;
   if (i < n_pairs) {
     c = cnt_ranked[i];
-    if (c) {
-      pv = order[i];
-      off = (long long)scan_ranked[i] - c;
-    }
+    pv = order[i];
   }
-  for (long long lo = first; lo < end; lo += kEmitBuf) {
-    const long long hi = min(end, lo + kEmitBuf);
-    if (c && off < hi && off + c > lo)
-      emit_pair_keys(views, V, P, pv >> 3, pv & 7u, bboxes, vmask, s_tiles, s_vals, off, lo, hi);
-    __syncthreads();
-    for (long long k = threadIdx.x; k < hi - lo; k += 256) {
-      tiles[lo + k] = s_tiles[k];
-      vals[lo + k] = s_vals[k];
+  const uint32_t* pm = vmask + (8LL * (pv >> 3) + (pv & 7u)) * nw;   // the pair's view mask words
+  if (c)
+    for (int wi = 0; wi < nw; ++wi) nrect += __popc(pm[wi]);
+  uint32_t total;
+  const uint32_t rbase = block_exclusive_scan(nrect, total, s_w);
+  uint32_t base = 0;   // output offset of the batch's first rectangle
+  for (uint32_t b0 = 0; b0 < total; b0 += kEmitRects) {
+    const uint32_t nb = min(total - b0, (uint32_t)kEmitRects);
+    // 1. views of the batch's rectangles, in pair order and view order within a pair
+    if (nrect && rbase < b0 + nb && rbase + nrect > b0) {
+      uint32_t k = rbase;
+      for (int wi = 0; wi < nw && k < b0 + nb; ++wi)
+        for (uint32_t m = pm[wi]; m && k < b0 + nb; m &= m - 1u, ++k)
+          if (k >= b0) {
+            s_shape[k - b0] = (uint32_t)((wi << 5) + __builtin_ctz(m));
+            s_val[k - b0] = pv;
+          }
     }
+    __syncthreads();
+    // 2. boxes and tile counts of kEmitRectsPerThread consecutive rectangles per thread
+    uint32_t n[kEmitRectsPerThread], cum = 0;
+#pragma unroll
+    for (int j = 0; j < kEmitRectsPerThread; ++j) {
+      const uint32_t r = threadIdx.x * kEmitRectsPerThread + j;
+      n[j] = 0;
+      if (r < nb) {
+        const int v = (int)s_shape[r];
+        const uint32_t pr = s_val[r];
+        const ViewDesc& vd = views[v];
+        const uint2 bb = bboxes[(long long)v * P + (pr >> 3)];
+        int tx0, ty0, tx1, ty1;
+        bbox_tiles(bb.x, bb.y, vd, tx0, ty0, tx1, ty1);
+        const uint32_t tw = (uint32_t)(tx1 - tx0 + 1);
+        n[j] = tw * (uint32_t)(ty1 - ty0 + 1);
+        s_tile0[r] = (uint32_t)(vd.tile_base + ty0 * vd.tiles_x + tx0);
+        s_shape[r] = tw | ((uint32_t)vd.tiles_x << 16);
+        s_val[r] = ((pr & 7u) << 29) | (pr >> 3);
+      }
+      cum += n[j];
+    }
+    uint32_t batch_keys;
+    uint32_t o = base + block_exclusive_scan(cum, batch_keys, s_w);
+#pragma unroll
+    for (int j = 0; j < kEmitRectsPerThread; ++j) {
+      const uint32_t r = threadIdx.x * kEmitRectsPerThread + j;
+      if (r < nb) s_off[r] = o;
+      o += n[j];
+    }
+    __syncthreads();
+    // 3. the batch's output positions, clipped to the block's range
+    const uint32_t p_end = (uint32_t)min((long long)(base + batch_keys), end - first);
+    for (uint32_t p = base + threadIdx.x; p < p_end; p += 256) {
+      uint32_t r = 0;   // last rectangle whose offset is <= p
+#pragma unroll
+      for (uint32_t step = kEmitRects / 2; step > 0; step >>= 1) {
+        const uint32_t t = r + step;
+        if (t < nb && s_off[t] <= p) r = t;
+      }
+      const uint32_t shape = s_shape[r], tw = shape & 0xFFFFu;
+      const uint32_t rel = p - s_off[r], ty = rel / tw;
+      tiles[first + p] = s_tile0[r] + ty * (shape >> 16) + (rel - ty * tw);
+      vals[first + p] = s_val[r];
+    }
+    base += batch_keys;
     __syncthreads();
   }
 }
