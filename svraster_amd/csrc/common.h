@@ -370,13 +370,14 @@ __device__ inline void lin_normal(const float g[8], float& nx, float& ny, float&
 }
 
 // Tile rectangle of a packed pixel bbox, clamped to the view (auxiliary.h:97-111, there with
-// 16x16 tiles).
+// 16x16 tiles).  Tile sizes are powers of two (8, 16 or 32), so the divisions are shifts.
 __device__ inline void bbox_tiles(uint32_t b0, uint32_t b1, const ViewDesc& v, int& tx0, int& ty0, int& tx1,
                                   int& ty1) {
-  tx0 = max(0, min(v.tiles_x - 1, int(b0 >> 16) / v.tile_w));
-  ty0 = max(0, min(v.tiles_y - 1, int(b0 & 0xFFFFu) / v.tile_h));
-  tx1 = max(0, min(v.tiles_x - 1, int(b1 >> 16) / v.tile_w));
-  ty1 = max(0, min(v.tiles_y - 1, int(b1 & 0xFFFFu) / v.tile_h));
+  const int sx = __builtin_ctz((unsigned)v.tile_w), sy = __builtin_ctz((unsigned)v.tile_h);
+  tx0 = max(0, min(v.tiles_x - 1, int(b0 >> 16) >> sx));
+  ty0 = max(0, min(v.tiles_y - 1, int(b0 & 0xFFFFu) >> sy));
+  tx1 = max(0, min(v.tiles_x - 1, int(b1 >> 16) >> sx));
+  ty1 = max(0, min(v.tiles_y - 1, int(b1 & 0xFFFFu) >> sy));
 }
 
 __device__ inline uint64_t rep3(uint32_t q) {

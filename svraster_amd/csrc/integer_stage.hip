@@ -36,9 +36,10 @@ __global__ void init_view_kernel(ViewDesc* out, ViewDesc base, const float* w2c,
 // bbox, octant bits and key count of voxel (c, half size r) in view v (preprocess.cu:60-153);
 // shared by the per-view and the voxel-major batched kernels so both give identical bits.
 __device__ inline void preprocess_one(const ViewDesc& v, bool leaf, float vcx, float vcy, float vcz, float vlen,
-                                      uint32_t& nd, uint2& bb, uint32_t& qbits) {
+                                      uint32_t& nd, uint2& bb, uint32_t& qbits, uint32_t& ntiles) {
 #pragma clang fp contract(off)
   nd = 0;
+  ntiles = 0;
   bb = make_uint2(0u, 0u);
   qbits = 0;
   if (leaf) {
@@ -74,11 +75,12 @@ __device__ inline void preprocess_one(const ViewDesc& v, bool leaf, float vcx, f
         bb.y = ((uint32_t)__builtin_rintf(bx1) << 16) | (uint32_t)__builtin_rintf(by1);
         int tx0, ty0, tx1, ty1;
         bbox_tiles(bb.x, bb.y, v, tx0, ty0, tx1, ty1);
-        nd = (uint32_t)((1 + ty1 - ty0) * (1 + tx1 - tx0)) * (uint32_t)__popc(qbits);
+        ntiles = (uint32_t)((1 + ty1 - ty0) * (1 + tx1 - tx0));
+        nd = ntiles * (uint32_t)__popc(qbits);
       }
     }
   }
-  if (nd == 0) { bb = make_uint2(0u, 0u); qbits = 0; }
+  if (nd == 0) { bb = make_uint2(0u, 0u); qbits = 0; ntiles = 0; }
 }
 
 __global__ __launch_bounds__(256) void preprocess_kernel(
@@ -93,11 +95,11 @@ __global__ __launch_bounds__(256) void preprocess_kernel(
   if (i >= P) return;
   const ViewDesc& v = views[view];
   const long long o = (long long)view * P + i;
-  uint32_t nd;
+  uint32_t nd, ntiles;
   uint2 bb;
   uint32_t qbits;
   preprocess_one(v, is_leaf[i] != 0, centers[3 * i + 0], centers[3 * i + 1], centers[3 * i + 2], lengths[i], nd, bb,
-                 qbits);
+                 qbits, ntiles);
   if (out_n_dup) out_n_dup[o] = (int32_t)nd;
   n_dup[o] = nd;
   bboxes[o] = bb;
@@ -434,15 +436,14 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(
   uint32_t c8[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
   uint32_t m8[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};   // views (of the current 32) that see octant q
   for (int v = 0; v < V; ++v) {
-    uint32_t nd, qbits;
+    uint32_t nd, qbits, nt;   // nt: tiles of the bbox, the keys per octant
     uint2 bb;
-    preprocess_one(views[v], leaf, cx, cy, cz, len, nd, bb, qbits);
+    preprocess_one(views[v], leaf, cx, cy, cz, len, nd, bb, qbits, nt);
     const long long o = (long long)v * P + i;
     if (n_dup) n_dup[o] = nd;
     quads[o] = (uint8_t)qbits;
     if (nd) {
       bboxes[o] = bb;        // read only where n_dup > 0
-      const uint32_t nt = nd / (uint32_t)__popc(qbits);
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         c8[q] += ((qbits >> q) & 1u) ? nt : 0u;
