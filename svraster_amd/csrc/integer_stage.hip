@@ -391,15 +391,30 @@ __global__ __launch_bounds__(256) void tile_ranges_kernel(
 }
 
 // Tile ranges of a sorted list of 32-bit tile ids (ranges zero-filled first).
+// Four sorted entries per thread (one 16-byte load; the tiles buffer is 256-byte aligned), compared
+// with their neighbours; ~0u stands for "no entry" (tile ids are far below it).
 __global__ __launch_bounds__(256) void tile_ranges_u32_kernel(
     const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ n_dev, long long n_cap,
     uint2* __restrict__ ranges) {
   const long long n = n_dev ? min((long long)*n_dev, n_cap) : n_cap;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t tile = tiles[i];
-  if (i == 0 || tiles[i - 1] != tile) ranges[tile].x = (uint32_t)i;
-  if (i == n - 1 || tiles[i + 1] != tile) ranges[tile].y = (uint32_t)(i + 1);
+  const long long i0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i0 >= n) return;
+  uint32_t t[6];
+  t[0] = i0 > 0 ? tiles[i0 - 1] : ~0u;
+  if (i0 + 4 <= n) {
+    const uint4 q = *reinterpret_cast<const uint4*>(tiles + i0);
+    t[1] = q.x; t[2] = q.y; t[3] = q.z; t[4] = q.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t[k + 1] = i0 + k < n ? tiles[i0 + k] : ~0u;
+  }
+  t[5] = i0 + 4 < n ? tiles[i0 + 4] : ~0u;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (i0 + k >= n) break;
+    if (t[k] != t[k + 1]) ranges[t[k + 1]].x = (uint32_t)(i0 + k);
+    if (t[k + 2] != t[k + 1]) ranges[t[k + 1]].y = (uint32_t)(i0 + k + 1);
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -726,7 +741,7 @@ hipError_t launch_pair_emit(const ViewDesc* views, int V, int P, long long n_pai
 hipError_t launch_tile_ranges_u32(const uint32_t* tiles, const uint32_t* n_dev, long long n_cap, uint2* ranges,
                                   hipStream_t st) {
   if (n_cap <= 0) return hipSuccess;
-  tile_ranges_u32_kernel<<<(unsigned)((n_cap + 255) / 256), 256, 0, st>>>(tiles, n_dev, n_cap, ranges);
+  tile_ranges_u32_kernel<<<(unsigned)((n_cap + 1023) / 1024), 256, 0, st>>>(tiles, n_dev, n_cap, ranges);
   return hipGetLastError();
 }
 
