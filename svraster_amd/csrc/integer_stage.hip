@@ -320,6 +320,11 @@ __global__ __launch_bounds__(kD) void hist_chunk_apply_kernel(uint32_t* __restri
   }
 }
 
+// Stable scatter of one tile of kSortTile keys.  Wavefront w ranks the contiguous quarter
+// [w * kSortTile / 4, (w + 1) * kSortTile / 4) of the tile, 64 keys at a time, against its own
+// per-digit running counts (kBits ballots per round, no workgroup barrier); the counts then give
+// every key its slot in the tile sorted by digit, the keys are reordered through LDS, and each
+// thread writes consecutive slots, so a wavefront's stores cover runs of consecutive addresses.
 template <typename K, int kBits>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
@@ -327,24 +332,39 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ offsets, int nb, K* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out) {
   constexpr int kD = 1 << kBits;
-  __shared__ uint32_t running[kD];
-  __shared__ uint32_t wcnt[4][kD + 1];
+  constexpr int kPerThread = kD / kSortThreads;          // digits per thread in the digit scan
+  constexpr int kQuarter = kSortTile / 4;
+  static_assert(kD % kSortThreads == 0 && kSortItems * 64 == kQuarter, "tile shape");
+  __shared__ uint32_t wrun[4][kD];   // per wavefront: keys of digit d so far, then its first slot
+  __shared__ uint32_t adj[kD];       // output position of slot 0 if it held digit d
+  __shared__ K s_key[kSortTile];
+  __shared__ uint32_t s_val[kSortTile];
+  __shared__ uint32_t s_w[4];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  for (int d = t; d < kD; d += kSortThreads) running[d] = offsets[(long long)blockIdx.x * kD + d];
   const long long n = n_dev ? min((long long)*n_dev, n_cap) : n_cap;
   const long long base = (long long)blockIdx.x * kSortTile;
+  if (base >= n) return;                                  // uniform across the block
+  const int n_tile = (int)min((long long)kSortTile, n - base);
+  for (int d = t; d < kD; d += kSortThreads) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wrun[k][d] = 0;
+  }
+  __syncthreads();
+  // 1. per wavefront, rank of each key among the earlier keys of its digit in the quarter
   const uint64_t lt_mask = (1ull << lane) - 1ull;
+  K key[kSortItems];
+  uint32_t val[kSortItems], pos[kSortItems];
+#pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    if (base + (long long)r * kSortThreads >= n) break;       // uniform across the block
-    const long long i = base + r * kSortThreads + t;
-    const bool valid = i < n;
-    K key = 0;
-    uint32_t val = 0, d = kD;
-    if (valid) {
-      key = keys_in[i];
-      val = vals_in[i];
-      d = (uint32_t)(key >> shift) & (uint32_t)(kD - 1);
-    }
+    const int li = w * kQuarter + r * 64 + lane;         // index within the tile
+    const bool valid = li < n_tile;
+    key[r] = valid ? keys_in[base + li] : (K)0;
+    val[r] = valid ? vals_in[base + li] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const bool valid = w * kQuarter + r * 64 + lane < n_tile;
+    const uint32_t d = valid ? (uint32_t)(key[r] >> shift) & (uint32_t)(kD - 1) : (uint32_t)kD;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < kBits; ++b) {
@@ -352,22 +372,54 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
       peers &= ((d >> b) & 1u) ? m : ~m;
     }
     const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-    for (int k = lane; k < kD + 1; k += 64) wcnt[w][k] = 0;
-    __syncthreads();
-    if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    uint32_t dst = 0;
-    if (valid) {
-      dst = running[d] + rank;
-      for (int k = 0; k < w; ++k) dst += wcnt[k][d];
+    uint32_t before = 0;
+    if (valid) before = wrun[w][d];
+    // LDS accesses of one wavefront complete in order: every lane has read the count above
+    // before the leader's update lands.
+    if (valid && rank == 0) wrun[w][d] = before + (uint32_t)__popcll(peers);
+    pos[r] = valid ? before + rank : 0u;
+  }
+  __syncthreads();
+  // 2. first slot of each digit in the sorted tile, and of each wavefront's keys within it
+  uint32_t tot[kPerThread], sum = 0;
+#pragma unroll
+  for (int j = 0; j < kPerThread; ++j) {
+    const int d = t * kPerThread + j;
+    tot[j] = wrun[0][d] + wrun[1][d] + wrun[2][d] + wrun[3][d];
+    sum += tot[j];
+  }
+  uint32_t all;
+  uint32_t start = block_exclusive_scan(sum, all, s_w);
+#pragma unroll
+  for (int j = 0; j < kPerThread; ++j) {
+    const int d = t * kPerThread + j;
+    adj[d] = offsets[(long long)blockIdx.x * kD + d] - start;
+    uint32_t s0 = start;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t c = wrun[k][d];
+      wrun[k][d] = s0;
+      s0 += c;
     }
-    __syncthreads();
-    for (int dd = t; dd < kD; dd += kSortThreads) running[dd] += wcnt[0][dd] + wcnt[1][dd] + wcnt[2][dd] + wcnt[3][dd];
-    if (valid) {
-      keys_out[dst] = key;
-      vals_out[dst] = val;
+    start += tot[j];
+  }
+  __syncthreads();
+  // 3. keys into their slots, then out in slot order
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    if (w * kQuarter + r * 64 + lane < n_tile) {
+      const uint32_t d = (uint32_t)(key[r] >> shift) & (uint32_t)(kD - 1);
+      const uint32_t slot = wrun[w][d] + pos[r];
+      s_key[slot] = key[r];
+      s_val[slot] = val[r];
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  for (int sl = t; sl < n_tile; sl += kSortThreads) {
+    const K k = s_key[sl];
+    const uint32_t dst = adj[(uint32_t)(k >> shift) & (uint32_t)(kD - 1)] + (uint32_t)sl;
+    keys_out[dst] = k;
+    vals_out[dst] = s_val[sl];
   }
 }
 
