@@ -10,10 +10,12 @@ from tests._util import rel_l2, say
 
 
 def _setup(n_views=4):
+    """fd_256 scene seen by n_views cameras of mixed sizes from random directions."""
     w = make_workload("fd_256", device="cuda")
     sc = w.scene
     rng = np.random.default_rng(7)
-    sizes = [(128, 128), (96, 64), (200, 150), (50, 70), (256, 256), (33, 17)][:n_views]
+    sizes = [(128, 128), (96, 64), (200, 150), (50, 70), (256, 256), (33, 17)]
+    sizes = [sizes[i % len(sizes)] for i in range(n_views)]
     cams = []
     for (W, H) in sizes:
         d = rng.normal(size=3)
@@ -110,12 +112,15 @@ def test_batched_voxel_skip_keeps_results():
 
 
 @pytest.mark.gpu
-def test_batched_tile_lists_match_per_view():
+@pytest.mark.parametrize("n_views", [4, 40])
+def test_batched_tile_lists_match_per_view(n_views):
     """Rank-order binning of all views at once gives every tile the per-view list (same values in
-    the same order) and the same per-pixel contributor counts."""
+    the same order) and the same per-pixel contributor counts.  With 40 views each (voxel, octant)
+    pair is seen by about 5 of them, so a workgroup's 256 pairs own more (pair, view) rectangles than
+    the key emission's LDS list holds at once (1024) and it runs in several batches."""
     from svraster_amd import _C
-    say("[batched] per-tile lists and contributor counts vs the per-view binning")
-    w, sc, cams, targets = _setup()
+    say(f"[batched] per-tile lists and contributor counts vs the per-view binning, {n_views} views")
+    w, sc, cams, targets = _setup(n_views)
     m = VoxelModel(sc)
     r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key, cams,
                           m.bg_color, sc.sh_degree, cull=False)
