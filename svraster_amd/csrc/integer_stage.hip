@@ -277,10 +277,20 @@ __global__ __launch_bounds__(kSortThreads) void radix_histogram_kernel(
   __syncthreads();
   const long long n = n_dev ? min((long long)*n_dev, n_cap) : n_cap;
   const long long base = (long long)blockIdx.x * kSortTile;
+  // Runs of equal digits in consecutive lanes (frequent in the pass over the high digit) are
+  // counted by their first lane alone, so they do not serialise on one LDS address.
+  const int lane = threadIdx.x & 63;
 #pragma unroll 4
   for (int r = 0; r < kSortItems; ++r) {
     const long long i = base + r * kSortThreads + threadIdx.x;
-    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & (uint32_t)(kD - 1)], 1u);
+    const bool valid = i < n;
+    const uint32_t d = valid ? (uint32_t)(keys[i] >> shift) & (uint32_t)(kD - 1) : (uint32_t)kD;
+    const uint32_t prev = (uint32_t)__shfl_up((int)d, 1);
+    const bool head = valid && (lane == 0 || prev != d);
+    // run end: the next head or invalid lane above this one (valid lanes form a prefix)
+    const uint64_t stops = __ballot(!valid || lane == 0 || prev != d) & ~((2ull << lane) - 1ull);
+    const int next = stops ? (int)__builtin_ctzll(stops) : 64;
+    if (head) atomicAdd(&h[d], (uint32_t)(next - lane));
   }
   __syncthreads();
   for (int d = threadIdx.x; d < kD; d += kSortThreads) hist[(long long)blockIdx.x * kD + d] = h[d];
