@@ -233,8 +233,8 @@ class BatchedRasterizer:
 
 class MseColorGrad:
     """dL/dcolor of the mean squared error per view, summed over views:
-    (color - target) * 2 / (3 * pixels of the view).  When every view has the same size it is one
-    fused pass, color * a - target * a with target * a precomputed; else per-element scales."""
+    (color - target) * 2 / (3 * pixels of the view), as one fused pass color * a - target * a with
+    -target * a precomputed: a scalar when every view has the same size, else per-element scales."""
 
     def __init__(self, targets: List[torch.Tensor], pix: List[int], dev):
         self.target = torch.cat([t.reshape(-1) for t in targets]).contiguous()
@@ -244,12 +244,13 @@ class MseColorGrad:
             self.scale = None
         else:
             self.scale = torch.cat([torch.full((3 * p,), 2.0 / (3 * p), device=dev) for p in pix])
+            self.neg_ts = -(self.target * self.scale)
 
     def __call__(self, color: torch.Tensor, out: torch.Tensor) -> None:
         if self.scale is None:
             torch.add(self.neg_ts, color, alpha=self.alpha, out=out)
         else:
-            torch.mul(color - self.target, self.scale, out=out)
+            torch.addcmul(self.neg_ts, color, self.scale, out=out)
 
 
 class BatchedTrainStep:
