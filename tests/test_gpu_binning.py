@@ -82,3 +82,41 @@ def test_binning_all_culled():
                          device="cuda")
     bn = _check_binning(w.scene, cam)
     assert bn.K == 0
+
+
+def _pair_order_gpu(octpaths: np.ndarray) -> np.ndarray:
+    """Rank order of all (voxel, octant) pairs from svr_batch_pair_order (the 64-bit-key radix sort)."""
+    from svraster_amd import _lib
+    lib = _lib.load()
+    P = octpaths.shape[0]
+    op = torch.from_numpy(octpaths.astype(np.int64)).cuda()
+    tmp = torch.empty(int(lib.svr_batch_pair_order_bytes(P)), dtype=torch.uint8, device="cuda")
+    order = torch.empty(8 * P, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    _lib.check(lib.svr_batch_pair_order(P, op.data_ptr(), tmp.data_ptr(), order.data_ptr(), stream),
+               "svr_batch_pair_order")
+    torch.cuda.synchronize()
+    return order.cpu().numpy().view(np.uint32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("P,kind", [(1, "random"), (511, "random"), (512, "random"), (513, "random"),
+                                    (6000, "random"), (3000, "equal"), (3000, "three")])
+def test_pair_rank_sort_stable(P, kind):
+    """The radix sort behind the rank order equals a stable argsort of the 48-bit rank keys
+    octpath ^ repeat3(octant): one partial sort tile, exactly one tile (4096 pairs), one key past
+    it, several tiles; all voxels on one octpath (every digit in runs, ties resolved by pair
+    index) and three distinct octpaths."""
+    say(f"[binning] pair rank sort, P={P} ({kind})")
+    rng = np.random.default_rng(P)
+    full = rng.integers(0, 1 << 48, size=P, dtype=np.int64)
+    if kind == "equal":
+        octpaths = np.full(P, full[0])
+    elif kind == "three":
+        octpaths = full[:3][rng.integers(0, 3, size=P)]
+    else:
+        octpaths = full
+    from oracle.binning import REP3
+    keys = (octpaths.astype(np.uint64)[:, None] ^ REP3[None, :]).reshape(-1) & np.uint64((1 << 48) - 1)
+    expect = np.argsort(keys, kind="stable").astype(np.uint32)
+    np.testing.assert_array_equal(_pair_order_gpu(octpaths), expect)
