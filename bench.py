@@ -185,10 +185,11 @@ def train_iteration_bytes(it, smooth):
 
 def bench_train(args):
     """Full training iteration at scale (train_16): 16 views at 1920x1080 over the 16M-leaf scene;
-    render + MSE + backward + Laplacian smoothness (weight 1e-3) + sparse Adam (lr 1e-2, betas
-    0.9/0.99).  The plain per-view iteration runs the drop-in API the way the reference trainer
-    calls it (one view per call, randperm active list, SparseAdam over the three groups); the
-    batched iteration is one captured graph."""
+    render + MSE + backward + Laplacian smoothness (weight 1e-3) + Adam (lr 1e-2, betas 0.9/0.99)
+    in the reference trainer's mode: SparseAdam with its defaults (sparse=False, biased=False),
+    i.e. every element of every group steps every iteration.  The plain per-view iteration runs
+    the drop-in API the way the reference trainer calls it (one view per call, randperm active
+    list, SparseAdam over the three groups); the batched iteration is one captured graph."""
     import torch
     import svraster_amd as svraster_cuda
     from svraster_amd import _lib
@@ -221,7 +222,7 @@ def bench_train(args):
 
     # -- plain per-view iteration (drop-in API) ---------------------------------------------------
     opt = SparseAdam([{"params": [model._geo_grid_pts]}, {"params": [model._sh0]}, {"params": [model._shs]}],
-                     lr=lr, betas=betas, eps=eps, sparse=True)
+                     lr=lr, betas=betas, eps=eps)
 
     def pv_render():
         zero_grads(model)
@@ -307,7 +308,7 @@ def bench_train(args):
         "config": {"workload": f"{args.workload}: {sc.num_voxels} voxels ({n_leaf} leaves, levels 7-13, parents "
                                f"kept from level 9), SH deg 3, {len(cams)} views 1920x1080, {rays} rays/step; "
                                f"training iteration: fwd + MSE + bwd + Laplacian smoothness (weight {weight}) "
-                               f"+ sparse Adam (lr {lr}, betas {betas})",
+                               f"+ dense Adam (lr {lr}, betas {betas}, SparseAdam defaults)",
                    "path": "batched: " + ("eager (profiler run)" if args.profile
                                           else "one captured HIP graph per iteration")},
         "paths": {"batched_ms_per_step": round(ms_b, 3), "per_view_ms_per_step": round(ms_pv, 3),

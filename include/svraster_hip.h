@@ -244,6 +244,25 @@ int svr_ijk_2_octpath(const int64_t* ijk, const int8_t* octlevel, int n, int64_t
 int svr_octpath_2_ijk(const int64_t* octpath, const int8_t* octlevel, int n, int64_t* ijk,
                       void* stream);
 
+/* Replaces UTILS::compute_rd (src/utils.cu:45-73, 276-292; bound as _C.compute_rd,
+ * binding.cpp, called from src/cameras.py:75-87).  out_rd: float [3, H, W], the world-space
+ * direction of each pixel centre, c2w rotation of ((x + 0.5 - cx) 2 tanfovx / W,
+ * (y + 0.5 - cy) 2 tanfovy / H, 1) (not normalised).  c2w: device float[16] row-major. */
+int svr_compute_rd(int W, int H, float cx, float cy, float tanfovx, float tanfovy, const float* c2w,
+                   float* out_rd, void* stream);
+/* Replaces UTILS::depth2pts (src/utils.cu:75-104, 294-310).  depth: float [H*W];
+ * out_pts: float [3, H, W] = camera position + depth * compute_rd direction. */
+int svr_depth2pts(int W, int H, float cx, float cy, float tanfovx, float tanfovy, const float* c2w,
+                  const float* depth, float* out_pts, void* stream);
+/* Replaces UTILS::is_in_cone (src/utils.cu:23-43, 254-274).  pts: float [N,3];
+ * out_is_in: bool/uint8 [N] = (z > near) and |x| <= tanfovx z and |y| <= tanfovy z in camera
+ * space (w2c: device float[16] row-major). */
+int svr_is_in_cone(float tanfovx, float tanfovy, float near_plane, const float* w2c, const float* pts,
+                   long long N, uint8_t* out_is_in, void* stream);
+/* Replaces UTILS::voxel_order_rank (src/utils.cu:107-127, 312-327).  out_ranks: int64 [8, P],
+ * row q = octree_path XOR (q repeated in every 3-bit level field). */
+int svr_voxel_order_rank(const int64_t* octree_paths, int P, int64_t* out_ranks, void* stream);
+
 /* Replaces UTILS::valid_gradient_table (src/utils.cu:190-252, 362-408) in two calls: count
  * (synchronises `stream`; workspace of svr_valid_gradient_table_workspace_bytes(N)), then
  * fill.  grid_mask uint8[(2^grid_res_pow2)^3] zeroed by the caller; keys / voxel ids int32[count]
@@ -325,17 +344,20 @@ int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long lo
 int svr_batch_sh_backward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                           const float* vox_centers, float* dL_drgb_views, int degree, int M,
                           float* dL_dsh0, float* dL_dshs, void* stream);
-/* svr_batch_sh_backward fused with the sparse unbiased Adam step of SparseAdam.step
+/* svr_batch_sh_backward fused with the unbiased Adam step of SparseAdam.step
  * (src/sparse_voxel_gears/optimizer.py:24-27, cuda/svraster_cuda/sparse_adam.py:33-72,
  * src/adam_step.cu:21-135) on sh0 [P,3] and shs [P,M-1,3]: the summed gradient of each element
- * updates (param, m, v) in place unless it is exactly 0, and is never written to memory.  The
- * bias corrections come from svr_adam_bias_table tables (one per parameter group, as the
- * groups' learning rates may differ) indexed by the device step counter *step. */
+ * updates (param, m, v) in place and is never written to memory.  sparse = 0 is the
+ * reference trainer's dense step (every element, zero gradients included); sparse != 0 leaves
+ * elements whose gradient is exactly 0 alone (adam_step.cu:44-45).  The bias corrections come
+ * from svr_adam_bias_table tables (one per parameter group, as the groups' learning rates may
+ * differ) indexed by the device step counter *step. */
 int svr_batch_sh_backward_adam(void* ws, int n_views, int P, long long n_pixels, long long n_tiles,
                                long long K_cap, const float* vox_centers, float* dL_drgb_views, int degree, int M,
                                float* sh0, float* sh0_m, float* sh0_v, const float* table_sh0, float* shs,
                                float* shs_m, float* shs_v, const float* table_shs, int n_table,
-                               const int32_t* step, float beta1, float beta2, float eps, void* stream);
+                               const int32_t* step, float beta1, float beta2, float eps, int sparse,
+                               void* stream);
 /* Byte offsets inside the batch workspace of (for inspection, like unpack_ImageState /
  * unpack_BinningState, raster_state.cu:120-230): out[0] tile ranges uint2[n_tiles], out[1]
  * sorted values uint32[K], out[2] per-pixel contributor counts uint32[n_pixels], out[3]
@@ -343,6 +365,13 @@ int svr_batch_sh_backward_adam(void* ws, int n_views, int P, long long n_pixels,
  * contributing entries (those the backward walks) uint2[n_tiles].  `out` holds 5 values. */
 int svr_batch_state_offsets(int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                             long long* out);
+/* Largest total key count any svr_batch_preprocess asked of this workspace since set-up or the
+ * last reset (synchronises `stream`; no counterpart in the reference, whose binning state is
+ * resized per call).  Keys past K_cap are not written, so a peak above K_cap means the images
+ * and gradients of that step are incomplete: callers check it after eager steps and after
+ * graph replays.  reset != 0 clears it (not inside a graph capture). */
+int svr_batch_key_peak(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                       int reset, long long* peak_out, void* stream);
 /* Total key count of the last svr_batch_preprocess (synchronises `stream`). */
 int svr_batch_key_count(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                         long long* K_out, void* stream);

@@ -28,8 +28,10 @@ gradient of the per-pair alpha tensor.
 
 Pinning (tests/test_oracle_raster.py): central finite differences in FP64 of the same loss,
 closed forms (constant SDF -> alpha = 0; single voxel with linear SDF along the ray; sum of
-weights + T = 1; sub-alpha telescoping), and the scalar per-pixel loop `render_scalar` which
-restates the reference's batch/j_lst/contributor control flow literally.
+weights + T = 1; sub-alpha telescoping; the median depth of one or two voxels sharing a linear
+SDF, where the refinement's transmittance telescopes and its stopping step follows from the
+logit of the crossing), and the scalar per-pixel loop `render_scalar` which restates the
+reference's batch/j_lst/contributor control flow literally.
 """
 from __future__ import annotations
 
@@ -161,6 +163,12 @@ class OracleOut:
     reg_loss: torch.Tensor    # scalar: colour-concentration surrogate (0 when off)
     pair_alpha: list          # per tile: (alpha [npix, n] with grad, pc mask, vox ids, dalpha/ds)
     window: tuple             # (x0, y0, x1, y1)
+    # [h,w]: how far the median depth (ch2) is from a different decision, in units of an FP32
+    # evaluation's error: the smallest of |T - 0.5| / 1e-4 over the median-voxel choice and the
+    # refinement steps, and |alpha - 1e-5| / 1e-6 over the hit entries entered with T > 0.5
+    # (whether they count as contributors).  Below 1, FP32 rounding may flip a decision and move
+    # ch2 by a refinement step or to another voxel.
+    dmed_margin: Optional[torch.Tensor] = None
 
 
 def render(cam: RasterCamera, bg, bin_vals: np.ndarray, bin_ranges: np.ndarray,
@@ -188,6 +196,7 @@ def render(cam: RasterCamera, bg, bin_vals: np.ndarray, bin_ranges: np.ndarray,
     normal = torch.zeros(3, h, w, dtype=dt)
     Tout = torch.ones(1, h, w, dtype=dt)
     n_contrib = torch.zeros(h, w, dtype=torch.long)
+    dmed_margin = torch.full((h, w), float("inf"), dtype=dt)
     max_w = torch.zeros(P, dtype=dt)
     reg = torch.zeros((), dtype=dt)
     pair_alpha = []
@@ -216,6 +225,7 @@ def render(cam: RasterCamera, bg, bin_vals: np.ndarray, bin_ranges: np.ndarray,
             Tfin = torch.ones(npix, dtype=dt)
             ncon = torch.zeros(npix, dtype=torch.long)
             dmed = torch.zeros(npix, dtype=dt)
+            margin = torch.full((npix,), float("inf"), dtype=dt)
             if n > 0:
                 vid = torch.from_numpy(vals & ((1 << 29) - 1))
                 vq = torch.from_numpy(vals >> 29)
@@ -264,8 +274,14 @@ def render(cam: RasterCamera, bg, bin_vals: np.ndarray, bin_ranges: np.ndarray,
                     diff = rgbs[vid][None] - gt[:, None, :]
                     reg = reg + lambda_R_concen / (W * H) * (wd * (diff * diff).sum(-1)).sum()
                 if need_depth:
-                    dmed = median_depth(T_excl.detach(), pc, vid, centers, lengths,
-                                        geos.detach(), float(s.detach()), ro, rd)
+                    dmed, margin = median_depth(T_excl.detach(), pc, vid, centers, lengths,
+                                                geos.detach(), float(s.detach()), ro, rd, return_margin=True)
+                    with torch.no_grad():
+                        Te = T_excl.detach()
+                        m_sel = torch.where(pc, (Te - 0.5).abs(), torch.full_like(Te, float("inf"))).min(1).values
+                        m_alpha = torch.where(hit & (Te > 0.5), (alpha.detach() - MIN_ALPHA).abs() / 1e-6,
+                                              torch.full_like(Te, float("inf"))).min(1).values
+                        margin = torch.minimum(torch.minimum(margin, m_sel) / 1e-4, m_alpha)
                 with torch.no_grad():
                     p0 = torch.sigmoid(s * sdf4[..., 0])
                     p3 = torch.sigmoid(s * sdf4[..., 3])
@@ -276,6 +292,7 @@ def render(cam: RasterCamera, bg, bin_vals: np.ndarray, bin_ranges: np.ndarray,
             col_parts.append((ly, lx, Cpix + Tfin[:, None] * bg[None]))
             T_parts.append((ly, lx, Tfin))
             n_contrib[ly, lx] = ncon
+            dmed_margin[ly, lx] = margin
             if need_depth:
                 dep_parts.append((ly, lx, Dpix / rd_norm, dmed / rd_norm))
             if need_normal:
@@ -289,7 +306,8 @@ def render(cam: RasterCamera, bg, bin_vals: np.ndarray, bin_ranges: np.ndarray,
                                   for ly, lx, d0, d2 in dep_parts])
     if need_normal:
         normal = _assemble(normal, [(ly, lx, v.T) for ly, lx, v in nrm_parts])
-    return OracleOut(color, depth, normal, Tout, n_contrib, max_w, reg, pair_alpha, window)
+    return OracleOut(color, depth, normal, Tout, n_contrib, max_w, reg, pair_alpha, window,
+                     dmed_margin if need_depth else None)
 
 
 def _assemble(base: torch.Tensor, parts):
@@ -308,13 +326,15 @@ def _assemble(base: torch.Tensor, parts):
 
 
 @torch.no_grad()
-def median_depth(T_excl, pc, vid, centers, lengths, geos, s, ro, rd):
-    """16-step refinement inside the last contributor with T > 0.5 (forward.cu:450-521)."""
+def median_depth(T_excl, pc, vid, centers, lengths, geos, s, ro, rd, return_margin=False):
+    """16-step refinement inside the last contributor with T > 0.5 (forward.cu:450-521).
+    With return_margin, also the smallest |T - 0.5| of the refinement's comparisons per pixel."""
     npix, n = pc.shape
     cand = pc & (T_excl > 0.5)
     idx = torch.arange(n)[None].expand(npix, n)
     last = torch.where(cand, idx, torch.full_like(idx, -1)).max(1).values
     out = torch.zeros(npix, dtype=torch.float64)
+    margin = torch.full((npix,), float("inf"), dtype=torch.float64)
     for p in range(npix):
         j = int(last[p])
         if j < 0:
@@ -332,6 +352,7 @@ def median_depth(T_excl, pc, vid, centers, lengths, geos, s, ro, rd):
         qstep = step * rdp / l
         phi_prev = 1.0 / (1.0 + np.exp(-s * float((trilinear_weights(q) * g).sum())))
         for k in range(16):
+            margin[p] = min(float(margin[p]), abs(Tm - 0.5))
             if not Tm > 0.5:
                 break
             dm += step
@@ -341,7 +362,7 @@ def median_depth(T_excl, pc, vid, centers, lengths, geos, s, ro, rd):
             Tm *= (1.0 - ak)
             phi_prev = phi
         out[p] = dm
-    return out
+    return (out, margin) if return_margin else out
 
 
 def backward(out: OracleOut, geos, rgbs, s, dL_dcolor, dL_ddepth=None, dL_dnormal=None,

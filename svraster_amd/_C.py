@@ -462,17 +462,59 @@ def valid_gradient_table(vox_center, vox_size, scene_center, inside_extent, grid
     return mask, keys, g2v
 
 
-def is_in_cone(*args, **kwargs):
-    raise NotImplementedError("is_in_cone is not provided by this library")
+def is_in_cone(tanfovx, tanfovy, near, w2c_matrix, pts):
+    """utils.cu:254-274 -> bool [N]."""
+    dev = _dev(pts)
+    if pts.ndim != 2 or pts.shape[1] != 3:
+        raise RuntimeError("pts must have dimensions (num_points, 3)")
+    N = pts.shape[0]
+    out = torch.zeros(N, dtype=torch.bool, device=dev)
+    if N == 0:
+        return out
+    _lib_call("svr_is_in_cone", float(tanfovx), float(tanfovy), float(near),
+              _ptr(_f32(w2c_matrix.to(dev), "w2c_matrix")), _ptr(_f32(pts, "pts")), N, _ptr(out), _stream(dev))
+    return out
 
 
-def compute_rd(*args, **kwargs):
-    raise NotImplementedError("compute_rd is not provided by this library")
+def _c2w_on(c2w_matrix):
+    if c2w_matrix.is_cuda:
+        return c2w_matrix
+    return c2w_matrix.to(torch.device("cuda", torch.cuda.current_device()))
 
 
-def depth2pts(*args, **kwargs):
-    raise NotImplementedError("depth2pts is not provided by this library")
+def compute_rd(width, height, cx, cy, tanfovx, tanfovy, c2w_matrix):
+    """utils.cu:276-292 -> float32 [3, height, width] world-space pixel ray directions."""
+    c2w = _f32(_c2w_on(c2w_matrix), "c2w_matrix")
+    dev = c2w.device
+    W, H = int(width), int(height)
+    rd = torch.empty(3, H, W, dtype=torch.float32, device=dev)
+    _lib_call("svr_compute_rd", W, H, float(cx), float(cy), float(tanfovx), float(tanfovy), _ptr(c2w), _ptr(rd),
+              _stream(dev))
+    return rd
 
 
-def voxel_order_rank(*args, **kwargs):
-    raise NotImplementedError("voxel_order_rank is not provided by this library")
+def depth2pts(width, height, cx, cy, tanfovx, tanfovy, c2w_matrix, depth):
+    """utils.cu:294-310 -> float32 [3, height, width] world points of a depth map."""
+    c2w = _f32(_c2w_on(c2w_matrix), "c2w_matrix")
+    dev = c2w.device
+    W, H = int(width), int(height)
+    d = _f32(depth.to(dev), "depth")
+    if d.numel() != W * H:
+        raise RuntimeError("depth must hold width * height values")
+    pts = torch.empty(3, H, W, dtype=torch.float32, device=dev)
+    _lib_call("svr_depth2pts", W, H, float(cx), float(cy), float(tanfovx), float(tanfovy), _ptr(c2w), _ptr(d),
+              _ptr(pts), _stream(dev))
+    return pts
+
+
+def voxel_order_rank(octree_paths):
+    """utils.cu:312-327 -> int64 [8, P]: octpath XOR the octant repeated in every level field."""
+    dev = _dev(octree_paths)
+    paths = octree_paths.contiguous()
+    if paths.dtype != torch.int64:
+        raise RuntimeError("octree_paths must be int64")
+    P = paths.shape[0]
+    out = torch.empty(8, P, dtype=torch.int64, device=dev)
+    if P > 0:
+        _lib_call("svr_voxel_order_rank", _ptr(paths), P, _ptr(out), _stream(dev))
+    return out

@@ -47,6 +47,10 @@ _SIGS = {
     "svr_laplacian_smoothness_bw": (I, [P, LL, P, I, P, P, I, P, P, P, I, P, I, F, F, U, P, P, P]),
     "svr_ijk_2_octpath": (I, [P, P, I, P, P]),
     "svr_octpath_2_ijk": (I, [P, P, I, P, P]),
+    "svr_compute_rd": (I, [I, I, F, F, F, F, P, P, P]),
+    "svr_depth2pts": (I, [I, I, F, F, F, F, P, P, P, P]),
+    "svr_is_in_cone": (I, [F, F, F, P, P, LL, P, P]),
+    "svr_voxel_order_rank": (I, [P, I, P, P]),
     "svr_valid_gradient_table_workspace_bytes": (SZ, [I]),
     "svr_valid_gradient_table_count": (I, [P, P, P, F, I, P, I, P, ctypes.POINTER(LL), P]),
     "svr_valid_gradient_table_fill": (I, [P, P, P, F, I, P, I, P, P, P, P, P]),
@@ -60,7 +64,7 @@ _SIGS = {
     "svr_batch_rasterize": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, I, I, P, P, P, P, P]),
     "svr_batch_backward": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, P, P, P, P, I, I, P, P, P, P, P, P]),
     "svr_batch_sh_backward": (I, [P, I, I, LL, LL, LL, P, P, I, I, P, P, P]),
-    "svr_batch_sh_backward_adam": (I, [P, I, I, LL, LL, LL, P, P, I, I, P, P, P, P, P, P, P, P, I, P, F, F, F, P]),
+    "svr_batch_sh_backward_adam": (I, [P, I, I, LL, LL, LL, P, P, I, I, P, P, P, P, P, P, P, P, I, P, F, F, F, I, P]),
     "svr_adam_bias_table": (I, [D, D, D, I, P]),
     "svr_sparse_adam_step_dev": (I, [I, P, P, P, P, LL, P, P, I, F, F, F, P]),
     "svr_grid_dense_index": (I, [P, I, I, P, P]),
@@ -74,6 +78,7 @@ _SIGS = {
     "svr_grid_laplacian_bw_table_order": (I, [P, LL, P, I, P, P, I, P, P, P, I, P, P, P, I, F, F, U, P, P, P, P, P]),
     "svr_batch_state_offsets": (I, [I, I, LL, LL, LL, P]),
     "svr_batch_key_count": (I, [P, I, I, LL, LL, LL, ctypes.POINTER(LL), P]),
+    "svr_batch_key_peak": (I, [P, I, I, LL, LL, LL, I, ctypes.POINTER(LL), P]),
 }
 
 _lib = None

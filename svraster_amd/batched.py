@@ -36,8 +36,11 @@ class BatchedRasterizer:
     def __init__(self, octpath, octlevel, vox_center, vox_size, is_leaf, vox_key, cameras: List,
                  background: torch.Tensor, sh_degree: int, n_sh_coeffs: int = 16,
                  key_capacity: Optional[int] = None, need_depth=False, need_normal=False, cull: bool = True,
-                 tile_shape=(16, 16), fuse_gather_bw: bool = False):
+                 tile_shape=(16, 16), fuse_gather_bw: bool = False, check_eager: bool = True):
         self.dev = vox_center.device
+        # Eager forwards read the key peak back (one sync) and raise on overflow; graph replays
+        # are checked by the caller with check_capacity().
+        self.check_eager = bool(check_eager)
         self.lib = _lib.load()
         self.V = len(cameras)
         self.P = int(vox_center.shape[0])
@@ -82,12 +85,17 @@ class BatchedRasterizer:
         self.cull = bool(cull)
         self.cam_scale = float(self.c2w[:, :3, 3].abs().max().item()) if self.V else 0.0
         self.bin_leaf = self.is_leaf.clone()
+        # Key count without the activity cull: a bound for every parameter state of this voxel
+        # set and camera set (the cull only removes keys).
+        self.cap = 1 << 20
+        self._alloc()
+        self.preprocess()
+        self.keys_unculled = self.key_count()
         if key_capacity is None:
-            self.cap = 1 << 20
-            self._alloc()
-            self.preprocess()
-            k = self.key_count()
-            key_capacity = int(k * 1.25) + 4096
+            key_capacity = int(self.keys_unculled * 1.25) + 4096
+        elif int(key_capacity) < self.keys_unculled:
+            raise ValueError(f"key_capacity {int(key_capacity)} is below the {self.keys_unculled} keys this voxel "
+                             f"set emits for these cameras")
         self.cap = int(key_capacity)
         self._alloc()
 
@@ -155,6 +163,19 @@ class BatchedRasterizer:
         o = self._state_offsets()[4]
         return self.ws[o:o + 8 * self.n_tiles].view(torch.int32).view(self.n_tiles, 2)
 
+    def check_capacity(self, reset: bool = False) -> int:
+        """Raise if any preprocess since set-up (or the last reset) needed more keys than the
+        capacity: keys past it are not binned, so that step's images and gradients would be
+        incomplete.  Returns the peak key count.  Synchronises the stream; call it after eager
+        steps and after graph replays, not inside a capture."""
+        k = ctypes.c_longlong(0)
+        _lib.check(self.lib.svr_batch_key_peak(_p(self.ws), *self._dims(), int(bool(reset)), ctypes.byref(k),
+                                               self._stream()), "svr_batch_key_peak")
+        if k.value > self.cap:
+            raise RuntimeError(f"batched binning needed {k.value} keys but its capacity is {self.cap}; "
+                               f"rebuild the BatchedRasterizer with a larger key_capacity")
+        return int(k.value)
+
     def key_count(self) -> int:
         k = ctypes.c_longlong(0)
         _lib.check(self.lib.svr_batch_key_count(_p(self.ws), *self._dims(), ctypes.byref(k), self._stream()),
@@ -174,6 +195,8 @@ class BatchedRasterizer:
             self.preprocess(self.bin_leaf)
         else:
             self.preprocess()
+        if self.check_eager and not torch.cuda.is_current_stream_capturing():
+            self.check_capacity()
         _lib.check(self.lib.svr_batch_sh_forward(_p(self.ws), *self._dims(), _p(self.vox_center), _p(sh0), _p(shs),
                                                  self.deg, self.M, _p(self.rgbs_views), st), "sh_forward")
         # Every pixel of every tile is written by the compositor: colour and T need no clearing.
@@ -194,7 +217,7 @@ class BatchedRasterizer:
                  sh_adam: Optional["ShAdamState"] = None):
         """Gradients of all views at once: (dL/dgrid_pts [M,1], dL/dsh0, dL/dshs, dL/ds, subdiv [P]).
         With `sh_adam`, the SH gradients are not returned (None): the SH transpose applies the
-        sparse Adam step to sh0/shs in place instead (svr_batch_sh_backward_adam)."""
+        Adam step to sh0/shs in place instead (svr_batch_sh_backward_adam)."""
         st = self._stream()
         if self.dgrid is None or self.dgrid.shape[0] != n_grid_pts:
             self.dgrid = torch.zeros(n_grid_pts, 1, dtype=torch.float32, device=self.dev)
@@ -221,8 +244,8 @@ class BatchedRasterizer:
             _lib.check(self.lib.svr_batch_sh_backward_adam(
                 _p(self.ws), *self._dims(), _p(self.vox_center), _p(self.drgb_views), self.deg, self.M,
                 _p(a.sh0), _p(a.m0), _p(a.v0), _p(a.table0), _p(a.shs), _p(a.ms), _p(a.vs), _p(a.tables),
-                a.n_table, _p(a.step), ctypes.c_float(a.b1), ctypes.c_float(a.b2), ctypes.c_float(a.eps), st),
-                "sh_backward_adam")
+                a.n_table, _p(a.step), ctypes.c_float(a.b1), ctypes.c_float(a.b2), ctypes.c_float(a.eps),
+                int(a.sparse), st), "sh_backward_adam")
             dsh0 = dshs = None
         if not fuse:
             dgeo = self.dvox[:, :8].contiguous()
@@ -290,6 +313,9 @@ class BatchedTrainStep:
         else:
             self._run()
 
+    def check_capacity(self) -> int:
+        return self.r.check_capacity()
+
 
 # ----------------------------------------------------------------------------------------------
 # Full training iteration: render + MSE + backward + grid Laplacian smoothness + sparse Adam
@@ -316,10 +342,13 @@ def adam_table_length(betas) -> int:
 
 
 class ShAdamState:
-    """Adam moments and bias tables of sh0 / shs for the fused SH backward + Adam launch."""
+    """Adam moments and bias tables of sh0 / shs for the fused SH backward + Adam launch.
+    sparse=False is the dense step (every element, zero gradients included)."""
 
-    def __init__(self, sh0, shs, step: torch.Tensor, lr_sh0: float, lr_shs: float, betas, eps: float):
+    def __init__(self, sh0, shs, step: torch.Tensor, lr_sh0: float, lr_shs: float, betas, eps: float,
+                 sparse: bool = False):
         self.sh0, self.shs, self.step = sh0, shs, step
+        self.sparse = bool(sparse)
         self.m0, self.v0 = torch.zeros_like(sh0), torch.zeros_like(sh0)
         self.ms, self.vs = torch.zeros_like(shs), torch.zeros_like(shs)
         self.n_table = adam_table_length(betas)
@@ -328,11 +357,31 @@ class ShAdamState:
         self.b1, self.b2, self.eps = float(betas[0]), float(betas[1]), float(eps)
 
 
+def grid_table_level(octlevel, outside_level: int = 0) -> int:
+    """Level of the valid-gradient grid table: min(max octree level - outside_level, 9)
+    (train.py:961, 597, 637); the table has 2^level cells per axis over the inside region."""
+    lv = min(int(octlevel.max().item()) - int(outside_level), 9)
+    if lv < 0:
+        raise ValueError(f"outside_level {outside_level} exceeds the deepest octree level")
+    return lv
+
+
+def grid_voxel_coords(vox_center, vox_size, scene_center, inside_extent: float, level: int):
+    """Voxel min corners and sizes in cells of the 2^level grid over the inside region, rounded
+    to integers (train.py:962-965): float32 [P,3] and [P]."""
+    res = 1 << int(level)
+    sc = torch.as_tensor(scene_center, device=vox_center.device).reshape(1, 3).float()
+    coord = torch.round(((vox_center - vox_size.reshape(-1, 1) * 0.5) - (sc - inside_extent * 0.5))
+                        / inside_extent * res).float().contiguous()
+    size = torch.round(vox_size.reshape(-1) / inside_extent * res).float().contiguous()
+    return coord, size
+
+
 class GridSmoothness:
     """Laplacian smoothness backward on the valid-gradient grid table, as the reference trainer
     calls it after topology changes (train.py:959-972, 636-654): grid resolution
-    2^min(max level, 9), voxel coordinates and sizes in grid cells rounded to integers, keys
-    sorted (octree_utils.py:293-315), weight * G / A, every key active.
+    2^min(max level - outside_level, 9), voxel coordinates and sizes in grid cells rounded to
+    integers, keys sorted (octree_utils.py:293-315), weight * G / A, every key active.
 
     The reference draws the active list as randperm(G)[:A] with A = G (ls_drop_ratio 0): every
     key once, in random order.  Here the keys are taken in table order (voxel by voxel), which
@@ -346,11 +395,12 @@ class GridSmoothness:
     (svr_grid_dense_index) instead of a binary search per sample."""
 
     def __init__(self, vox_center, vox_size, octlevel, is_leaf, vox_key, scene_center, inside_extent: float,
-                 weight: float, seed: int = 0, pull: bool = True):
+                 weight: float, seed: int = 0, pull: bool = True, outside_level: int = 0):
         from . import _C
         self.lib = _lib.load()
         self.dev = vox_center.device
-        self.max_level = min(int(octlevel.max().item()), 9)
+        self.outside_level = int(outside_level)
+        self.max_level = grid_table_level(octlevel, self.outside_level)
         self.res = 1 << self.max_level
         mask, keys, g2v = _C.valid_gradient_table(vox_center, vox_size, scene_center, inside_extent,
                                                   self.max_level, is_leaf.reshape(-1).bool())
@@ -361,10 +411,8 @@ class GridSmoothness:
         del keys, g2v
         self.mask = mask
         self.G = int(self.keys.numel())
-        sc = scene_center.to(self.dev).reshape(1, 3).float()
-        self.coord = torch.round(((vox_center - vox_size.reshape(-1, 1) * 0.5) - (sc - inside_extent * 0.5))
-                                 / inside_extent * self.res).float().contiguous()
-        self.size = torch.round(vox_size.reshape(-1) / inside_extent * self.res).float().contiguous()
+        self.coord, self.size = grid_voxel_coords(vox_center, vox_size, scene_center, inside_extent,
+                                                  self.max_level)
         self.vox_key = vox_key.contiguous()
         self.N = int(vox_center.shape[0])
         n3 = self.res ** 3
@@ -454,17 +502,21 @@ class GridSmoothness:
 class BatchedTrainIteration:
     """One training iteration over a batch of views in the reference trainer's order
     (train.py:324-740): render every view -> MSE to the targets -> backward -> grid Laplacian
-    smoothness into the corner-SDF gradient -> sparse unbiased Adam on the corner SDFs and SH
-    (elements with an exactly-zero gradient untouched).  The step counter lives on the device, so
-    `capture()` records the whole iteration once and each replay is the next iteration.
+    smoothness into the corner-SDF gradient -> unbiased Adam on the corner SDFs and SH.  The Adam
+    step is dense by default, as the reference trainer builds SparseAdam with sparse=False
+    (src/sparse_voxel_gears/optimizer.py:24): every element steps every iteration and the moments
+    of untouched elements keep decaying.  sparse=True leaves elements with an exactly-zero
+    gradient untouched.  The step counter lives on the device, so `capture()` records the whole
+    iteration once and each replay is the next iteration.
 
     log_s is not updated (its learning rate is 0 in the reference's optimizer groups,
     optimizer.py:16-22); the SH Adam step runs inside the SH transpose launch."""
 
     def __init__(self, rast: BatchedRasterizer, grid_pts, sh0, shs, log_s, targets: List[torch.Tensor],
                  smooth: Optional[GridSmoothness], lr: float = 1e-2, betas=(0.9, 0.99), eps: float = 1e-15,
-                 lr_sh0: Optional[float] = None, lr_shs: Optional[float] = None):
+                 lr_sh0: Optional[float] = None, lr_shs: Optional[float] = None, sparse: bool = False):
         self.r = rast
+        self.sparse = bool(sparse)
         self.dev = rast.dev
         self.grid_pts, self.sh0, self.shs, self.log_s = grid_pts, sh0, shs, log_s
         self.loss_grad = MseColorGrad(targets, rast.pix, self.dev)
@@ -475,7 +527,7 @@ class BatchedTrainIteration:
         self.smooth = smooth
         self.betas, self.eps = (float(betas[0]), float(betas[1])), float(eps)
         self.sh_adam = ShAdamState(sh0, shs, self.step, lr if lr_sh0 is None else lr_sh0,
-                                   lr if lr_shs is None else lr_shs, self.betas, self.eps)
+                                   lr if lr_shs is None else lr_shs, self.betas, self.eps, sparse=self.sparse)
         self.grid_m, self.grid_v = torch.zeros_like(grid_pts), torch.zeros_like(grid_pts)
         self.n_table = self.sh_adam.n_table
         self.grid_table = adam_bias_table(lr, self.betas, self.n_table, self.dev)
@@ -503,7 +555,7 @@ class BatchedTrainIteration:
 
     def stage_adam(self):
         _lib.check(self.lib.svr_sparse_adam_step_dev(
-            1, _p(self.grid_pts), _p(self.grid_grad), _p(self.grid_m), _p(self.grid_v), self.grid_pts.numel(),
+            int(self.sparse), _p(self.grid_pts), _p(self.grid_grad), _p(self.grid_m), _p(self.grid_v), self.grid_pts.numel(),
             _p(self.step), _p(self.grid_table), self.n_table, ctypes.c_float(self.betas[0]),
             ctypes.c_float(self.betas[1]), ctypes.c_float(self.eps), self._stream()), "svr_sparse_adam_step_dev")
 
@@ -539,3 +591,6 @@ class BatchedTrainIteration:
             self.graph.replay()
         else:
             self._run()
+
+    def check_capacity(self) -> int:
+        return self.r.check_capacity()

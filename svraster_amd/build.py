@@ -27,7 +27,7 @@ FLAGS = [
     "-Wno-unused-result",
 ]
 SOURCES = ["integer_stage.hip", "render_fwd.hip", "render_bwd.hip", "voxel_ops.hip",
-           "grid_loss.hip", "capi.hip"]
+           "grid_loss.hip", "camera.hip", "capi.hip"]
 
 
 def _compile(src: str) -> str:

@@ -167,7 +167,7 @@ int svr_timing_read(int kind, int reset, double* total_ms, long long* launches) 
   }
   return 0;
 }
-int svr_abi_version(void) { return 1; }
+int svr_abi_version(void) { return 2; }
 
 int svr_set_option(const char* name, int value) {
   g_err.clear();
@@ -339,7 +339,7 @@ int svr_rasterize_backward(long long K, int vox_geo_mode, int density_mode, int 
   a.T = out_T; a.n_contrib = ip.n_contrib;
   a.dL_dcolor = dL_dcolor; a.dL_ddepth = dL_ddepth; a.dL_dnormal = dL_dnormal; a.dL_dT = dL_dT;
   a.gt_color = gt_color; a.lambda_R_concen = lambda_R_concen;
-  a.dL_dvox = dL_dvox; a.dL_ds = dL_ds;
+  a.dL_dvox = dL_dvox; a.dL_ds = dL_ds; a.ds_part = ip.ds_part;
   a.work_counter = nullptr;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;
@@ -631,6 +631,42 @@ int svr_octpath_2_ijk(const int64_t* octpath, const int8_t* octlevel, int n, int
   return 0;
 }
 
+int svr_compute_rd(int W, int H, float cx, float cy, float tanfovx, float tanfovy, const float* c2w,
+                   float* out_rd, void* stream) {
+  g_err.clear();
+  if (W < 0 || H < 0) return fail(1, "image size must be non-negative%s (got %lld)", "", (long long)W * H);
+  if ((long long)W * H > 0 && (!c2w || !out_rd)) return fail(1, "c2w and out_rd are required%s", "", 0);
+  SVR_TRY(launch_compute_rd(W, H, cx, cy, tanfovx, tanfovy, c2w, out_rd, as_stream(stream)));
+  return 0;
+}
+
+int svr_depth2pts(int W, int H, float cx, float cy, float tanfovx, float tanfovy, const float* c2w,
+                  const float* depth, float* out_pts, void* stream) {
+  g_err.clear();
+  if (W < 0 || H < 0) return fail(1, "image size must be non-negative%s (got %lld)", "", (long long)W * H);
+  if ((long long)W * H > 0 && (!c2w || !depth || !out_pts))
+    return fail(1, "c2w, depth and out_pts are required%s", "", 0);
+  SVR_TRY(launch_depth2pts(W, H, cx, cy, tanfovx, tanfovy, c2w, depth, out_pts, as_stream(stream)));
+  return 0;
+}
+
+int svr_is_in_cone(float tanfovx, float tanfovy, float near_plane, const float* w2c, const float* pts,
+                   long long N, uint8_t* out_is_in, void* stream) {
+  g_err.clear();
+  if (N < 0) return fail(1, "point count must be non-negative%s (got %lld)", "", N);
+  if (N > 0 && (!w2c || !pts || !out_is_in)) return fail(1, "w2c, pts and out_is_in are required%s", "", 0);
+  SVR_TRY(launch_is_in_cone(N, tanfovx, tanfovy, near_plane, w2c, pts, out_is_in, as_stream(stream)));
+  return 0;
+}
+
+int svr_voxel_order_rank(const int64_t* octree_paths, int P, int64_t* out_ranks, void* stream) {
+  g_err.clear();
+  if (P < 0) return fail(1, "voxel count must be non-negative%s (got %lld)", "", P);
+  if (P > 0 && (!octree_paths || !out_ranks)) return fail(1, "octree_paths and out_ranks are required%s", "", 0);
+  SVR_TRY(launch_voxel_order_rank(P, octree_paths, out_ranks, as_stream(stream)));
+  return 0;
+}
+
 // ------------------------------------------------------------------------------------------
 // Batched views: one workspace holding the geometry state of V views x P voxels, a binning
 // state of fixed capacity and the image state of all views' tiles and pixels.  Every launch
@@ -692,6 +728,7 @@ int svr_batch_setup(void* ws, int n_views, int P, long long n_pixels, long long 
   }
   if (tile_base != n_tiles || pix_base != n_pixels)
     return fail(1, "tile/pixel totals do not match the views%s (tiles %lld)", "", tile_base);
+  SVR_TRY(hipMemsetAsync(bp.g.misc + 8, 0, sizeof(uint32_t), st));
   return 0;
 }
 
@@ -726,6 +763,7 @@ int svr_batch_preprocess(void* ws, int n_views, int P, long long n_pixels, long 
                                   bp.g.quad, bp.pr.pair_cnt, bp.pr.vmask, st));
   SVR_TRY(launch_pair_gather(n, pair_order, bp.pr.pair_cnt, bp.pr.cnt_ranked, st));
   SVR_TRY(launch_scan(bp.pr.cnt_ranked, nullptr, n, bp.pr.partial, bp.pr.scan_ranked, true, st));
+  SVR_TRY(launch_key_peak(bp.pr.scan_ranked + (n - 1), bp.g.misc + 8, st));
   return 0;
 }
 
@@ -824,7 +862,7 @@ int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long lo
   a.T = out_T; a.n_contrib = bp.im.n_contrib;
   a.dL_dcolor = dL_dcolor; a.dL_ddepth = dL_ddepth; a.dL_dnormal = dL_dnormal; a.dL_dT = dL_dT;
   a.gt_color = nullptr; a.lambda_R_concen = 0.f;
-  a.dL_dvox = dL_dvox; a.dL_drgb_views = dL_drgb_views; a.dL_ds = dL_ds;
+  a.dL_dvox = dL_dvox; a.dL_drgb_views = dL_drgb_views; a.dL_ds = dL_ds; a.ds_part = bp.im.ds_part;
   a.vox_key = vox_key; a.dL_dgrid = dL_dgrid_pts;
   a.work_counter = bp.im.misc + 1;
   a.need_depth = need_depth; a.need_normal = need_normal;
@@ -856,7 +894,8 @@ int svr_batch_sh_backward_adam(void* ws, int n_views, int P, long long n_pixels,
                                long long K_cap, const float* vox_centers, float* dL_drgb_views, int degree, int M,
                                float* sh0, float* sh0_m, float* sh0_v, const float* table_sh0, float* shs,
                                float* shs_m, float* shs_v, const float* table_shs, int n_table,
-                               const int32_t* step, float beta1, float beta2, float eps, void* stream) {
+                               const int32_t* step, float beta1, float beta2, float eps, int sparse,
+                               void* stream) {
   g_err.clear();
   if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
   if (degree < 0 || degree > 3 || M < (degree + 1) * (degree + 1)) return fail(1, "bad SH degree%s %lld", "", degree);
@@ -868,6 +907,7 @@ int svr_batch_sh_backward_adam(void* ws, int n_views, int P, long long n_pixels,
   A.shs = shs; A.ms = shs_m; A.vs = shs_v;
   A.t0 = adam_table(step, table_sh0, n_table, beta1, beta2, eps);
   A.ts = adam_table(step, table_shs, n_table, beta1, beta2, eps);
+  A.sparse = sparse ? 1 : 0;
   SVR_TRY(launch_sh_views_backward(P, n_views, bp.g.quad, bp.g.views, degree, M, vox_centers, dL_drgb_views,
                                    nullptr, nullptr, &A, as_stream(stream)));
   return 0;
@@ -886,6 +926,25 @@ int svr_batch_state_offsets(int n_views, int P, long long n_pixels, long long n_
   out[2] = (long long)(im + il.n_contrib);
   out[3] = (long long)(im + il.tile_last);
   out[4] = (long long)(im + il.cranges);
+  return 0;
+}
+
+int svr_batch_key_peak(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                       int reset, long long* peak_out, void* stream) {
+  g_err.clear();
+  *peak_out = 0;
+  if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
+  hipStream_t st = as_stream(stream);
+  BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
+  uint32_t k = 0;
+  SVR_TRY(hipMemcpyAsync(&k, bp.g.misc + 8, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  SVR_TRY(hipStreamSynchronize(st));
+  *peak_out = k;
+  if (reset) {
+    const uint32_t zero = 0;
+    SVR_TRY(hipMemcpyAsync(bp.g.misc + 8, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    SVR_TRY(hipStreamSynchronize(st));
+  }
   return 0;
 }
 

@@ -290,6 +290,41 @@ __device__ inline void wave_segmented_sum(float (&v)[N], int lane, int seg_start
   }
 }
 
+// Compensated FP32 summation.  two_sum is Knuth's error-free transformation: s = fl(a + b) and
+// e = (a + b) - s exactly.  A running (hi, lo) pair adds each term to hi and its rounding error to
+// lo (Neumaier), so hi + lo carries the sum of many terms of mixed sign to about one rounding of
+// the result instead of one rounding per term.
+__device__ inline void two_sum(float a, float b, float& s, float& e) {
+#pragma clang fp contract(off)
+  s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+__device__ inline void comp_add(float& hi, float& lo, float x) {
+#pragma clang fp contract(off)
+  float s, e;
+  two_sum(hi, x, s, e);
+  hi = s;
+  lo += e;
+}
+
+// Sum of the (hi, lo) pairs of the 64 lanes, returned in every lane (xor butterfly; every lane
+// forms the same sums, so the result is the same in all lanes).
+__device__ inline float2 wave_comp_sum(float hi, float lo) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const float ohi = __shfl_xor(hi, off, 64);
+    const float olo = __shfl_xor(lo, off, 64);
+    float s, e;
+    two_sum(hi, ohi, s, e);
+    hi = s;
+    lo = (lo + olo) + e;
+  }
+  return make_float2(hi, lo);
+}
+
 // Position of the k-th (0-based) set bit of m; k < popcount(m).
 __device__ inline int select_set_bit(uint64_t m, int k) {
   int pos = 0;

@@ -674,6 +674,15 @@ hipError_t launch_preprocess(const ViewDesc* views, int n_views, int P, const fl
 
 size_t scan_partial_count(long long n) { return (size_t)((n + kScanTile - 1) / kScanTile); }
 
+__global__ void key_peak_kernel(const uint32_t* __restrict__ total, uint32_t* __restrict__ peak) {
+  if (threadIdx.x == 0) atomicMax(peak, *total);
+}
+
+hipError_t launch_key_peak(const uint32_t* total, uint32_t* peak, hipStream_t st) {
+  key_peak_kernel<<<1, 64, 0, st>>>(total, peak);
+  return hipGetLastError();
+}
+
 hipError_t launch_scan(const uint32_t* in, const uint32_t* n_dev, long long n_cap, uint32_t* partial,
                        uint32_t* out, bool inclusive, hipStream_t st) {
   if (n_cap <= 0) return hipSuccess;

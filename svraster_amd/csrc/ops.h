@@ -27,6 +27,7 @@ struct ShAdam {
   float* sh0; float* m0; float* v0;
   float* shs; float* ms; float* vs;
   AdamTable t0, ts;
+  int sparse;            // 1: elements whose gradient is exactly 0 are left alone
 };
 hipError_t launch_sh_views_backward(int P, int V, const uint8_t* quads, const ViewDesc* views, int deg, int M,
                                     const float* centers, float* drgb_views, float* dsh0, float* dshs,
@@ -86,5 +87,13 @@ hipError_t launch_table_count(int N, const float* center, const float* size, con
 hipError_t launch_table_write(int N, const float* center, const float* size, const float* scene_center,
                               float extent, int res, const bool* is_leaf, const uint32_t* scan, bool* mask,
                               int32_t* keys, int32_t* key2vox, hipStream_t st);
+// Camera utilities (camera.hip).
+hipError_t launch_compute_rd(int W, int H, float cx, float cy, float tfx, float tfy, const float* c2w, float* rd,
+                             hipStream_t st);
+hipError_t launch_depth2pts(int W, int H, float cx, float cy, float tfx, float tfy, const float* c2w,
+                            const float* depth, float* pts, hipStream_t st);
+hipError_t launch_is_in_cone(long long N, float tfx, float tfy, float near_plane, const float* w2c,
+                             const float* pts, uint8_t* out, hipStream_t st);
+hipError_t launch_voxel_order_rank(int P, const int64_t* paths, int64_t* out, hipStream_t st);
 
 }  // namespace svr

@@ -11,8 +11,10 @@
 // Per-voxel gradients are reduced across the wave before touching global memory: for each
 // staged entry the 12 gradient values [8 corner SDFs | rgb | |dL/da * a|] of all lanes whose
 // pixel used the entry are summed with DPP row reductions (or read from the single contributing
-// lane), then lanes 0..11 issue one vector atomic add of 12 consecutive floats.  dL/ds is kept
-// per lane and reduced once per workgroup.
+// lane), then lanes 0..11 issue one vector atomic add of 12 consecutive floats.  dL/ds, a
+// single scalar summed over every (pixel, entry) pair with terms of both signs, is accumulated
+// per lane with compensated FP32 sums, reduced per tile into a (hi, lo) partial and summed over
+// the tiles in a fixed order by a one-workgroup kernel: deterministic, no float atomics.
 #include "common.h"
 #include "state.h"
 
@@ -66,16 +68,64 @@ __device__ inline void wave_lds_sync_b() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// dL/ds of one tile: the workgroup's compensated partials (per lane, then per wave, then the
+// four waves in order) written to part[g] as a (hi, lo) pair, and the lane accumulators reset.
+// Every tile slot of the launch is written once, by the workgroup that rendered the tile, so
+// the sum over tiles (ds_finalize_kernel) does not depend on which workgroup took which tile.
+__device__ inline void flush_tile_ds(float& hi, float& lo, float2* s_red, float2* part, int g, int t, int lane,
+                                     int w) {
+  const float2 ws = wave_comp_sum(hi, lo);
+  if (lane == 0) s_red[w] = ws;
+  __syncthreads();
+  if (t == 0) {
+    float h = 0.f, l = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      comp_add(h, l, s_red[k].x);
+      l += s_red[k].y;
+    }
+    part[g] = make_float2(h, l);
+  }
+  hi = 0.f;
+  lo = 0.f;
+}
+
+// Sum of the per-tile (hi, lo) partials in a fixed order, added to *dL_ds.  One workgroup.
+__global__ __launch_bounds__(256) void ds_finalize_kernel(const float2* __restrict__ part, int n,
+                                                          float* __restrict__ dL_ds) {
+  __shared__ float2 s_w[4];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  float hi = 0.f, lo = 0.f;
+  for (int i = t; i < n; i += 256) {
+    const float2 p = part[i];
+    comp_add(hi, lo, p.x);
+    lo += p.y;
+  }
+  const float2 ws = wave_comp_sum(hi, lo);
+  if (lane == 0) s_w[w] = ws;
+  __syncthreads();
+  if (t == 0) {
+    float h = 0.f, l = 0.f;
+    comp_add(h, l, *dL_ds);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      comp_add(h, l, s_w[k].x);
+      l += s_w[k].y;
+    }
+    *dL_ds = h + l;
+  }
+}
+
 template <bool kDepth, bool kNormal, bool kConcen>
 __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
   __shared__ float4 s_ch[256], s_ml[256], s_g0[256], s_g1[256], s_rgb[256];
   __shared__ float4 s_nrm[kNormal ? 256 : 1];
   __shared__ uint2 s_bb[256];
-  __shared__ float s_red[4];
+  __shared__ float2 s_red[4];
   __shared__ int s_tile;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const float s = *a.s_val;
-  float ds_acc = 0.f;
+  float ds_hi = 0.f, ds_lo = 0.f;
 
   for (int iter = 0;; ++iter) {
     int g;
@@ -204,7 +254,7 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
             c[1] = 0.f;
             c[2] = 0.f;
             c[3] = -dLda * s * q3 * rden;
-            ds_acc += dLda * (q0 * seg.sdf[0] * (1.f - alpha) - q3 * seg.sdf[3]) * rden;
+            comp_add(ds_hi, ds_lo, dLda * (q0 * seg.sdf[0] * (1.f - alpha) - q3 * seg.sdf[3]) * rden);
 
             if (kDepth) {
               // Depth through the three sub-interval alphas (backward.cu:338-388, 682-768).
@@ -292,12 +342,8 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
         }
       }
     }
+    flush_tile_ds(ds_hi, ds_lo, s_red, a.ds_part, g, t, lane, w);
   }
-  // One dL/ds atomic per workgroup.
-  const float wsum = wave_sum_uniform(ds_acc);
-  if (lane == 0) s_red[w] = wsum;
-  __syncthreads();
-  if (t == 0) atomicAdd(a.dL_ds, (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -339,12 +385,12 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
   __shared__ float s_pa[4][kBwdWin];       // alpha per position; after the walk, T before the entry
   __shared__ float s_pd[4][kDepth ? kBwdWin : 1];
   __shared__ float s_pl[4][kBwdWin];       // dL/dalpha per position
-  __shared__ float s_red[4];
+  __shared__ float2 s_red[4];
   __shared__ int s_tile;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const float s = *a.s_val;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
-  float ds_acc = 0.f;
+  float ds_hi = 0.f, ds_lo = 0.f;
 
   for (int iter = 0;; ++iter) {
     int g;
@@ -568,7 +614,7 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
               c[1] = 0.f;
               c[2] = 0.f;
               c[3] = -dLda * s * q3 * rden;
-              ds_acc += dLda * (q0 * seg.sdf[0] * (1.f - alpha) - q3 * seg.sdf[3]) * rden;
+              comp_add(ds_hi, ds_lo, dLda * (q0 * seg.sdf[0] * (1.f - alpha) - q3 * seg.sdf[3]) * rden);
               if (kDepth) {
                 float sub[3];
                 sub_alphas(seg, sub);
@@ -647,11 +693,8 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
         atomicAdd(dst, v);
       }
     }
+    flush_tile_ds(ds_hi, ds_lo, s_red, a.ds_part, g, t, lane, w);
   }
-  const float wsum = wave_sum_uniform(ds_acc);
-  if (lane == 0) s_red[w] = wsum;
-  __syncthreads();
-  if (t == 0) atomicAdd(a.dL_ds, (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]));
 }
 
 template <bool D, bool N>
@@ -679,10 +722,14 @@ hipError_t launch_render_backward(const BwdArgs& a, hipStream_t st) {
 
   unsigned grid = (unsigned)a.total_tiles;
   if (a.work_counter) grid = (unsigned)min(a.total_tiles, 256 * max(1, a.blocks_per_cu));
-  if (a.need_depth && a.need_normal) return launch_bwd_c<true, true>(a, st, grid);
-  if (a.need_depth) return launch_bwd_c<true, false>(a, st, grid);
-  if (a.need_normal) return launch_bwd_c<false, true>(a, st, grid);
-  return launch_bwd_c<false, false>(a, st, grid);
+  hipError_t e;
+  if (a.need_depth && a.need_normal) e = launch_bwd_c<true, true>(a, st, grid);
+  else if (a.need_depth) e = launch_bwd_c<true, false>(a, st, grid);
+  else if (a.need_normal) e = launch_bwd_c<false, true>(a, st, grid);
+  else e = launch_bwd_c<false, false>(a, st, grid);
+  if (e != hipSuccess) return e;
+  ds_finalize_kernel<<<1, 256, 0, st>>>(a.ds_part, a.total_tiles, a.dL_ds);
+  return hipGetLastError();
 }
 
 }  // namespace svr

@@ -50,7 +50,8 @@ struct GeomLayout {
 
 struct GeomPtrs {
   ViewDesc* views;
-  uint32_t* misc;      // [0] = low zero rank bits shared by all visible voxels
+  uint32_t* misc;      // [0] = low zero rank bits shared by all visible voxels; [8] = peak key
+                       // count of the batched preprocesses since the last svr_batch_key_peak reset
   uint32_t* n_dup;
   uint32_t* scan;
   uint2* bbox;
@@ -134,7 +135,7 @@ inline size_t raster_bin_bytes(long long K) { return BinLayout(K).total + Contri
 
 // Image state.
 struct ImageLayout {
-  size_t views, misc, ranges, tile_last, n_contrib, cranges, total;
+  size_t views, misc, ranges, tile_last, n_contrib, cranges, ds_part, total;
   ImageLayout(long long n_views, long long n_pix, long long n_tiles) {
     Carver c;
     views = c.take<ViewDesc>(n_views);
@@ -143,6 +144,7 @@ struct ImageLayout {
     tile_last = c.take<uint32_t>(n_tiles);
     n_contrib = c.take<uint32_t>(n_pix);
     cranges = c.take<uint2>(n_tiles);
+    ds_part = c.take<float2>(n_tiles);
     total = c.off;
   }
 };
@@ -154,6 +156,7 @@ struct ImagePtrs {
   uint32_t* tile_last;
   uint32_t* n_contrib;
   uint2* cranges;      // per tile: its contributing entries [x, y) in the contributing-entry arrays
+  float2* ds_part;     // per backward workgroup: its dL/ds partial sum as an unevaluated (hi, lo) pair
   ImagePtrs(void* base, long long n_views, long long n_pix, long long n_tiles) {
     ImageLayout L(n_views, n_pix, n_tiles);
     char* b = (char*)base;
@@ -163,6 +166,7 @@ struct ImagePtrs {
     tile_last = (uint32_t*)(b + L.tile_last);
     n_contrib = (uint32_t*)(b + L.n_contrib);
     cranges = (uint2*)(b + L.cranges);
+    ds_part = (float2*)(b + L.ds_part);
   }
 };
 
@@ -231,6 +235,8 @@ hipError_t launch_pair_rank(long long n_pairs, const int64_t* octpaths, uint64_t
 hipError_t launch_preprocess_views(const ViewDesc* views, int V, int P, const float* centers, const float* lengths,
                                    const uint8_t* is_leaf, uint32_t* n_dup, uint2* bboxes, uint8_t* quads,
                                    uint32_t* pair_cnt, uint32_t* pair_vmask, hipStream_t st);
+// *peak = max(*peak, *total): the largest key count asked of a fixed-capacity binning state.
+hipError_t launch_key_peak(const uint32_t* total, uint32_t* peak, hipStream_t st);
 hipError_t launch_pair_gather(long long n_pairs, const uint32_t* order, const uint32_t* pair_cnt,
                               uint32_t* cnt_ranked, hipStream_t st);
 hipError_t launch_pair_emit(const ViewDesc* views, int V, int P, long long n_pairs, const uint32_t* order,
@@ -270,7 +276,8 @@ struct BwdArgs {
   const float* gt_color; float lambda_R_concen;
   float* dL_dvox;                  // [P,12]
   float* dL_drgb_views;            // [V,P,3] per-view colour gradients (null: into dL_dvox)
-  float* dL_ds;                    // [1]
+  float* dL_ds;                    // [1], accumulated into
+  float2* ds_part;                 // [launch grid]: per-workgroup dL/ds partials (hi, lo)
   // Corner-SDF gradients straight into the grid points (the gather's transpose fused into the
   // flush; null: into dL_dvox slots 0-7).
   const int64_t* vox_key; float* dL_dgrid;

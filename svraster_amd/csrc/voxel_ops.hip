@@ -301,10 +301,11 @@ __device__ inline float2 adam_table_entry(const AdamTable& T) {
   return T.tab[t - 1];
 }
 
-// kAdam: instead of writing dL/dsh0 and dL/dshs, apply the sparse unbiased Adam step of `A` to
-// sh0/shs right here (elements whose gradient is exactly 0 are left alone, as the sparse step
-// does), so the SH gradients never reach HBM.  Same gradient values and Adam arithmetic as
-// the unfused pair of calls.
+// kAdam: instead of writing dL/dsh0 and dL/dshs, apply the unbiased Adam step of `A` to sh0/shs
+// right here, so the SH gradients never reach HBM.  Dense (A.sparse == 0, the reference
+// trainer's default): every element of every voxel steps, a zero gradient included.  Sparse:
+// elements whose gradient is exactly 0 are left alone.  Same gradient values and Adam
+// arithmetic as the unfused pair of calls.
 template <int kDeg, bool kAdam>
 __global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, const uint8_t* __restrict__ quads,
                                                                 const ViewDesc* __restrict__ views, int M,
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, co
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const float g = acc[0][c];
-        if (g != 0.0f) {
+        if (!A.sparse || g != 0.0f) {
           const long long e = 3LL * i + c;
           float pv = A.sh0[e], mv = A.m0[e], vv = A.v0[e];
           adam_update<false>(pv, mv, vv, g, h0.x, A.t0.b1, A.t0.b2, h0.y, A.t0.eps);
@@ -390,7 +391,7 @@ __global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, co
       for (int k = t; k < nv * qn; k += 256) {
         const int r = k / qn, col = k - r * qn;
         const float g = s_rows[r * kShChunk + col];
-        if (g == 0.0f) continue;
+        if (A.sparse && g == 0.0f) continue;
         const long long e = (i0 + r) * row + q0 + col;
         float pv = A.shs[e], mv = A.ms[e], vv = A.vs[e];
         adam_update<false>(pv, mv, vv, g, hs.x, A.ts.b1, A.ts.b2, hs.y, A.ts.eps);

@@ -62,7 +62,7 @@ class LoadedModel:
     """The fields a loaded checkpoint provides (io.py:49-86), with the VoxelModel render path."""
 
     def __init__(self, state: Dict, device="cuda", white_background: bool = True,
-                 vox_geo_mode: str = "triinterp3", density_mode: str = "sdf"):
+                 vox_geo_mode: str = "triinterp3", density_mode: str = "sdf", outside_level=None):
         from .model import VoxelModel
         dev = torch.device(device)
         self.active_sh_degree = int(state["active_sh_degree"])
@@ -84,7 +84,10 @@ class LoadedModel:
         self._subdiv_p = torch.full([N, 1], 1.0, dtype=torch.float32, device=dev).requires_grad_()
         self.subdiv_meta = torch.zeros([N, 1], dtype=torch.float32, device=dev)
         self.vox_size_inv = 1.0 / self.vox_size
-        self.outside_level = 0
+        # The reference takes outside_level from its config; the checkpoint fixes it through
+        # scene_extent = inside_extent * 2^outside_level (constructor.py:25).
+        self.outside_level = (outside_level_of(state["scene_extent"], state["inside_extent"])
+                              if outside_level is None else int(outside_level))
         bg = 1.0 if white_background else 0.0
         self.bg_color = torch.full((3,), bg, dtype=torch.float32, device=dev)
         self.vox_geo_mode, self.density_mode = vox_geo_mode, density_mode
@@ -103,6 +106,17 @@ class LoadedModel:
         return self.active_sh_degree
 
 
+def outside_level_of(scene_extent, inside_extent) -> int:
+    """log2(scene_extent / inside_extent), which the reference's constructor makes an integer
+    (constructor.py:25); raises when the ratio is not a power of two."""
+    import math
+    r = float(torch.as_tensor(scene_extent).reshape(-1)[0]) / float(torch.as_tensor(inside_extent).reshape(-1)[0])
+    lv = int(round(math.log2(r))) if r > 0 else -1
+    if lv < 0 or abs(r - 2.0 ** lv) > 1e-5 * r:
+        raise ValueError(f"scene_extent / inside_extent = {r} is not a power of two")
+    return lv
+
+
 def load(path: str, device="cuda", white_background: bool = True) -> LoadedModel:
     """io.py:49-86 (the reference loads with weights_only=False; the format needs no pickled
     objects, so this loader refuses them)."""
@@ -113,7 +127,9 @@ def load(path: str, device="cuda", white_background: bool = True) -> LoadedModel
 
 
 # ----------------------------------------------------------------------------------------------
-# 8-bit codebook quantization (io.py:106-170)
+# 8-bit codebook quantization.  On-disk layout of src/sparse_voxel_gears/io.py:110-114 and
+# 166-168: the corner SDFs as one {index uint8 (shape of the tensor), codebook float32[256]}
+# dict, sh0 and shs as lists of such dicts, one per channel column.
 # ----------------------------------------------------------------------------------------------
 def quantize_state_dict(state: Dict) -> None:
     state["_geo_grid_pts"] = quantization(state["_geo_grid_pts"])
@@ -127,38 +143,56 @@ def dequantize_state_dict(state: Dict) -> None:
     state["_shs"] = torch.cat([dequantization(v) for v in state["_shs"]], dim=1)
 
 
+N_CODES = 256
+
+
+def _run_means(sv: torch.Tensor, csum: torch.Tensor, bounds: torch.Tensor) -> torch.Tensor:
+    """Mean of each run sv[bounds[k]:bounds[k+1]] (float64).  An empty run takes the sorted
+    value at its start (the last value past the end), which lies between its neighbours' means,
+    so the codebook stays nondecreasing."""
+    n = sv.numel()
+    s, e = bounds[:-1], bounds[1:]
+    cnt = (e - s).to(torch.float64)
+    means = (csum[e] - csum[s]) / cnt.clamp_min(1.0)
+    fill = sv[s.clamp_max(n - 1)].to(torch.float64)
+    return torch.where(cnt > 0, means, fill)
+
+
 def quantization(src: torch.Tensor, max_iter: int = 10) -> Dict:
-    """256-entry codebook: start from the 256 upper quantiles (257 evenly spaced order
-    statistics, the first dropped), assign every value to the nearer of its two bracketing
-    entries, then up to max_iter rounds of (entries = mean of their values; move a value one
-    entry toward a strictly nearer neighbour); a final mean.  io.py:116-167."""
+    """Fit a 256-word scalar codebook to the values of `src` by one-dimensional k-means
+    (Lloyd's algorithm) and return {index uint8 [src.shape], codebook float32 [256]}.
+
+    In one dimension every k-means cell is an interval, so after one sort each codeword owns a
+    contiguous run of the sorted values and a Lloyd step is two vector operations: the run
+    boundaries are the positions of the midpoints between consecutive codewords (a value equal
+    to a midpoint joins the upper codeword), and each codeword becomes the mean of its run,
+    read from a float64 prefix sum.  The fit starts from 256 runs of equal count (sizes differ by
+    at most one) and stops when no boundary moves or after `max_iter` steps.  Every used
+    codeword is the mean of exactly the values indexed to it; the codebook is nondecreasing.
+    Runs on the host: this is checkpoint IO, not part of the rendering path."""
     shape = src.shape
-    vals = src.flatten().contiguous().float()
+    vals = src.detach().reshape(-1).to("cpu", torch.float32)
     n = vals.numel()
-    order = vals.argsort()
-    q = (torch.linspace(0, 1, 257, device=vals.device) * (n - 1)).long().clamp_(0, n - 1)
-    codebook = vals[order[q]].contiguous()
-    codebook[0] = -torch.inf
-    ind = torch.searchsorted(codebook, vals)
-    codebook = codebook[1:]
-    ind = (ind - 1).clamp_(0, 255)
-    diff_l = (vals - codebook[ind - 1]).abs()
-    diff_m = (vals - codebook[ind]).abs()
-    ind = ind - 1 + (diff_m < diff_l)
-    ind.clamp_(0, 255)
+    if n == 0:
+        return {"index": torch.zeros(shape, dtype=torch.uint8), "codebook": torch.zeros(N_CODES)}
+    sv, order = torch.sort(vals)
+    csum = torch.zeros(n + 1, dtype=torch.float64)
+    torch.cumsum(sv.to(torch.float64), 0, out=csum[1:])
+    k = torch.arange(N_CODES + 1, dtype=torch.int64)
+    bounds = (k * n) // N_CODES
+    code = _run_means(sv, csum, bounds)
     for _ in range(max_iter):
-        codebook = torch.zeros_like(codebook).index_reduce_(0, ind, vals, reduce="mean", include_self=False)
-        diff_l = (vals - codebook[ind - 1]).abs()
-        diff_r = (vals - codebook[(ind + 1).clamp_max(255)]).abs()
-        diff_m = (vals - codebook[ind]).abs()
-        upd = torch.minimum(diff_l, diff_r) < diff_m
-        if int(upd.sum()) == 0:
+        mid = 0.5 * (code[:-1] + code[1:])
+        inner = torch.searchsorted(sv.to(torch.float64), mid, side="left")
+        nb = torch.cat([bounds[:1], inner, bounds[-1:]])
+        if torch.equal(nb, bounds):
             break
-        shift = (diff_r < diff_l).long() * 2 - 1
-        ind[upd] += shift[upd]
-        ind.clamp_(0, 255)
-    codebook = torch.zeros_like(codebook).index_reduce_(0, ind, vals, reduce="mean", include_self=False)
-    return {"index": ind.reshape(shape).to(torch.uint8), "codebook": codebook}
+        bounds = nb
+        code = _run_means(sv, csum, bounds)
+    run_of_sorted = torch.searchsorted(bounds[1:-1], torch.arange(n, dtype=torch.int64), side="right")
+    index = torch.empty(n, dtype=torch.int64)
+    index[order] = run_of_sorted
+    return {"index": index.reshape(shape).to(torch.uint8), "codebook": code.to(torch.float32)}
 
 
 def dequantization(qd: Dict) -> torch.Tensor:

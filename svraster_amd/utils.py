@@ -6,18 +6,30 @@ from . import _C
 
 
 def voxel_order_rank(octree_paths):
+    """The eight per-octant rendering ranks of each voxel, int64 [8, P] (utils.py:13-16)."""
     return _C.voxel_order_rank(octree_paths)
 
 
 def is_in_cone(pts, cam):
+    """Points inside the camera's view cone and in front of its near plane (utils.py:19-29)."""
+    assert torch.is_tensor(pts)
+    assert pts.device == cam.w2c.device
+    assert len(pts.shape) == 2
+    assert pts.shape[1] == 3
     return _C.is_in_cone(cam.tanfovx, cam.tanfovy, cam.near, cam.w2c, pts)
 
 
 def compute_rd(width, height, cx, cy, tanfovx, tanfovy, c2w_matrix):
+    """World-space ray direction of every pixel centre, float32 [3, H, W] (utils.py:32-34)."""
+    assert torch.is_tensor(c2w_matrix)
     return _C.compute_rd(width, height, cx, cy, tanfovx, tanfovy, c2w_matrix)
 
 
 def depth2pts(width, height, cx, cy, tanfovx, tanfovy, c2w_matrix, depth):
+    """World point of every pixel of a depth map, float32 [3, H, W] (utils.py:37-41)."""
+    assert torch.is_tensor(c2w_matrix)
+    assert depth.device == c2w_matrix.device
+    assert depth.numel() == width * height
     return _C.depth2pts(width, height, cx, cy, tanfovx, tanfovy, c2w_matrix, depth)
 
 

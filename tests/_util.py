@@ -73,3 +73,51 @@ def full_keys_from_vals(vals: np.ndarray, octpaths: np.ndarray, ranges: np.ndarr
         tile[ranges[t, 0]:ranges[t, 1]] = t
     rank = octpaths.reshape(-1).astype(np.uint64)[v] ^ ob.REP3[q]
     return (tile << np.uint64(48)) | rank
+
+
+def centre_crop(cam, size=64):
+    """(x0, y0, x1, y1) of a size x size window at the image centre, aligned to 16-pixel tiles."""
+    cx, cy = cam.image_width // 2, cam.image_height // 2
+    h = size // 2
+    return (cx - h - cx % 16, cy - h - cy % 16, cx + h - cx % 16, cy + h - cy % 16)
+
+
+def oracle_crop_grads(scene, cam, bn, window, s64: float, dC_win: torch.Tensor, grid_pts, sh0, shs, bg=(1.0, 1.0, 1.0)):
+    """FP64 oracle forward + backward of one window of one view, restricted to the voxels listed
+    in the window's tiles (every other voxel cannot touch those pixels).  The corner SDFs are
+    gathered from the grid points and the colours evaluated from the SH coefficients of the view,
+    so the gradients come back in the parameters the batched path returns:
+    (out, voxel ids, grid point ids, dL/dgrid [G], dL/dsh0 [n,3], dL/dshs [n,M-1,3], dL/ds)."""
+    W, H = cam.image_width, cam.image_height
+    x0, y0, x1, y1 = window
+    tiles_x = (W + 15) // 16
+    mask29 = (1 << 29) - 1
+    segs = []
+    for ty in range(y0 // 16, (y1 + 15) // 16):
+        for tx in range(x0 // 16, (x1 + 15) // 16):
+            lo, hi = bn.ranges[ty * tiles_x + tx]
+            segs.append(bn.vals[lo:hi])
+    ent = np.concatenate(segs).astype(np.int64) if segs else np.zeros(0, np.int64)
+    vid = np.unique(ent & mask29)
+    P = scene.num_voxels
+    lut = np.zeros(P, dtype=np.int64)
+    lut[vid] = np.arange(vid.shape[0])
+    vals = bn.vals.astype(np.int64)
+    new_vals = ((vals >> 29) << 29) | lut[vals & mask29]
+    vt = torch.from_numpy(vid)
+    centers = scene.vox_center.cpu()[vt]
+    sizes = scene.vox_size.cpu().reshape(-1)[vt]
+    keys = scene.vox_key.cpu()[vt]
+    gid, inv = torch.unique(keys.reshape(-1), return_inverse=True)
+    grid = grid_pts.detach().reshape(-1).cpu()[gid].double().requires_grad_(True)
+    geos = grid[inv.reshape(-1, 8)]
+    s0 = sh0.detach().cpu()[vt].double().requires_grad_(True)
+    ss = shs.detach().cpu()[vt].double().requires_grad_(True)
+    rgbs = osh.sh_rgb(scene.sh_degree, None, centers, cam.c2w[:3, 3].cpu(), s0, ss)
+    s = torch.tensor(float(s64), dtype=torch.float64, requires_grad=True)
+    out = orc.render(orc.camera_from(cam), list(bg), new_vals, bn.ranges, bn.pre.bbox[vid], centers, sizes,
+                     geos, rgbs, s, need_depth=False, need_normal=False, window=window)
+    loss = (out.color * dC_win.double()).sum()
+    g_grid, g_s0, g_ss, g_s = torch.autograd.grad(loss, [grid, s0, ss, s], allow_unused=True)
+    z = lambda g, ref: torch.zeros_like(ref) if g is None else g
+    return out, vid, gid.numpy(), z(g_grid, grid), z(g_s0, s0), z(g_ss, ss), z(g_s, s)

@@ -175,3 +175,27 @@ def test_fused_gather_transpose_matches_separate_pass():
     say(f"  grid gradient rel-L2 {e:.2e}; |dL/dgeos| left in the voxel buffer: separate {out[0][1]:.3e}, "
         f"fused {out[1][1]:.3e}")
     assert e <= 1e-6 and out[0][1] > 0 and out[1][1] == 0.0
+
+
+@pytest.mark.gpu
+def test_batched_key_capacity_is_enforced():
+    say("[batched] key capacity: below the un-culled count is refused; an eager forward needing more raises")
+    w, sc, cams, targets = _setup()
+    m = VoxelModel(sc)
+    r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key, cams,
+                          m.bg_color, sc.sh_degree)
+    with pytest.raises(ValueError):
+        BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key, cams,
+                          m.bg_color, sc.sh_degree, key_capacity=r.keys_unculled - 1)
+    s_val = torch.exp(m._log_s.detach().reshape(1) * 10.0)
+    r.forward(m._geo_grid_pts.detach(), m._sh0.detach(), m._shs.detach(), s_val)
+    peak = r.check_capacity()
+    say(f"  un-culled {r.keys_unculled} keys, capacity {r.cap}, peak {peak}")
+    assert 0 < peak <= r.keys_unculled <= r.cap
+    # the same voxel set with every voxel twice as large needs more keys than this capacity:
+    # the eager forward stops right after the key count, before any key is emitted
+    big = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size.clone(), sc.is_leaf, sc.vox_key,
+                            cams, m.bg_color, sc.sh_degree, key_capacity=r.keys_unculled + 4096, cull=False)
+    big.vox_size.mul_(2.0)
+    with pytest.raises(RuntimeError, match="capacity"):
+        big.forward(m._geo_grid_pts.detach(), m._sh0.detach(), m._shs.detach(), s_val)

@@ -1,0 +1,167 @@
+"""The two headline workloads under the GPU tests, against the FP64 oracle itself.
+
+multi_32: 4M voxels (24 spheres, levels 6-12, SH degree 3), 32 views of mixed resolution, one
+batched step.  For views 0, 8, 16 and 24 (one per resolution) the 64x64 centre crop matches the
+oracle: colour and T max-abs <= 1e-4; with cotangents on those four crops only, the corner-SDF,
+sh0 and shs gradients match to relative L2 <= 1e-3 and dL/ds to relative 1e-3; the tile lists of
+the four views (binning without the activity cull) equal the oracle's sorted lists bit for bit.
+
+train_16: 16M leaves (+ non-leaf parents kept from level 9), 16 views at 1920x1080.  One captured
+training iteration (render, MSE, backward, Laplacian smoothness, dense Adam) is replayed once: its
+colour on a crop of view 0 matches the oracle rendering of the parameters it started from
+(<= 1e-4), and its corner-SDF update equals the host-driven Adam step on the replay's gradient.
+
+Each test builds its scene (tens of seconds of host time) and runs the oracle on crops; they are
+the slowest GPU tests."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import binning as ob
+from svraster_amd import _C
+from svraster_amd.batched import BatchedRasterizer, BatchedTrainIteration, GridSmoothness
+from svraster_amd.model import VoxelModel
+from svraster_amd.synthetic import make_workload
+from tests._util import centre_crop, oracle_crop_grads, rel_l2, say
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_multi_32_batched_step_vs_oracle():
+    say("[headline] multi_32: building the 4M-voxel scene ...")
+    w = make_workload("multi_32", device="cuda")
+    sc, cams = w.scene, w.cameras
+    m = VoxelModel(sc)
+    grid, sh0, shs = m._geo_grid_pts.detach(), m._sh0.detach(), m._shs.detach()
+    s_val = torch.exp(m._log_s.detach().reshape(1) * 10.0)
+    s64 = float(s_val.item())
+    r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key, cams,
+                          m.bg_color, sc.sh_degree)
+    r.forward(grid, sh0, shs, s_val)
+    torch.cuda.synchronize()
+    say(f"  {sc.num_voxels} voxels, {r.key_count()} keys (capacity {r.cap}), {r.n_pix} rays")
+    views = (0, 8, 16, 24)
+    g = torch.Generator().manual_seed(5)
+    dC = torch.zeros(3 * r.n_pix, device="cuda")
+    crops, cots = {}, {}
+    for v in views:
+        win = centre_crop(cams[v])
+        x0, y0, x1, y1 = win
+        cot = torch.rand(3, y1 - y0, x1 - x0, generator=g) - 0.5
+        r.view_slice(dC, 3, v)[:, y0:y1, x0:x1] = cot.cuda()
+        crops[v], cots[v] = win, cot
+    colors = {v: r.view_slice(r.color, 3, v)[:, crops[v][1]:crops[v][3], crops[v][0]:crops[v][2]].cpu().double()
+              for v in views}
+    Ts = {v: r.view_slice(r.T, 1, v)[:, crops[v][1]:crops[v][3], crops[v][0]:crops[v][2]].cpu().double()
+          for v in views}
+    dgrid, dsh0, dshs, ds, _ = r.backward(dC, grid.shape[0])
+    r.check_capacity()
+    torch.cuda.synchronize()
+
+    # oracle, view by view, on the crops; gradients summed over the views on the union of the
+    # voxels / grid points the crops touch
+    M, P = grid.shape[0], sc.num_voxels
+    parts, ods, bins = [], 0.0, {}
+    for v in views:
+        bn = ob.binning_for_camera(cams[v], sc)
+        bins[v] = bn
+        out, vid, gid, g_grid, g_s0, g_ss, g_s = oracle_crop_grads(sc, cams[v], bn, crops[v], s64, cots[v],
+                                                                   grid, sh0, shs)
+        e_c = (colors[v] - out.color.detach()).abs().max().item()
+        e_t = (Ts[v] - out.T.detach()).abs().max().item()
+        say(f"  view {v} {cams[v].image_width}x{cams[v].image_height}: crop colour {e_c:.2e} T {e_t:.2e} "
+            f"({vid.shape[0]} voxels in the crop's tiles)")
+        assert e_c <= 1e-4 and e_t <= 1e-4
+        parts.append((torch.from_numpy(vid), torch.from_numpy(gid), g_grid, g_s0, g_ss))
+        ods += float(g_s)
+    vk = torch.unique(torch.cat([p[0] for p in parts]))
+    gk = torch.unique(torch.cat([p[1] for p in parts]))
+    ref_grid = torch.zeros(gk.numel(), dtype=torch.float64)
+    ref_s0 = torch.zeros(vk.numel(), 3, dtype=torch.float64)
+    ref_ss = torch.zeros(vk.numel(), sc.shs.shape[1], 3, dtype=torch.float64)
+    for vid, gid, g_grid, g_s0, g_ss in parts:
+        ref_grid.index_add_(0, torch.searchsorted(gk, gid), g_grid)
+        iv = torch.searchsorted(vk, vid)
+        ref_s0.index_add_(0, iv, g_s0)
+        ref_ss.index_add_(0, iv, g_ss)
+    got_grid = dgrid.reshape(-1)[gk.cuda()].cpu().double()
+    got_s0 = dsh0[vk.cuda()].cpu().double()
+    got_ss = dshs[vk.cuda()].cpu().double()
+    # nothing outside the crops' voxels and grid points received a gradient
+    off_g = torch.ones(M, dtype=torch.bool, device="cuda")
+    off_g[gk.cuda()] = False
+    off_v = torch.ones(P, dtype=torch.bool, device="cuda")
+    off_v[vk.cuda()] = False
+    assert float(dgrid.reshape(-1)[off_g].abs().max()) == 0.0
+    assert float(dsh0[off_v].abs().max()) == 0.0 and float(dshs[off_v].abs().max()) == 0.0
+    e_g, e_0, e_s = rel_l2(got_grid, ref_grid), rel_l2(got_s0, ref_s0), rel_l2(got_ss, ref_ss)
+    e_ds = abs(ds.item() - ods) / max(abs(ods), 1e-30)
+    say(f"  gradients rel-L2: grid {e_g:.2e} sh0 {e_0:.2e} shs {e_s:.2e}; dL/ds {ds.item():.6e} vs {ods:.6e} "
+        f"({e_ds:.2e})")
+    assert e_g <= 1e-3 and e_0 <= 1e-3 and e_s <= 1e-3 and e_ds <= 1e-3
+
+    # tile lists bit-exact: the same batched binning without the activity cull
+    del r
+    torch.cuda.empty_cache()
+    r2 = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key, cams,
+                           m.bg_color, sc.sh_degree, cull=False)
+    r2.forward(grid, sh0, shs, s_val)
+    torch.cuda.synchronize()
+    ranges_b, vals_b, _, _ = r2.state_views()
+    tile_base = np.concatenate([[0], np.cumsum(r2.tiles)])
+    for v in views:
+        bn = bins[v]
+        nt = r2.tiles[v]
+        rb = ranges_b[int(tile_base[v]):int(tile_base[v]) + nt].cpu().numpy().astype(np.int64)
+        lb, lo_ = rb[:, 1] - rb[:, 0], bn.ranges[:, 1].astype(np.int64) - bn.ranges[:, 0].astype(np.int64)
+        assert np.array_equal(lb, lo_), f"view {v}: tile list lengths"
+        ne = np.nonzero(lb)[0]
+        start, end = int(rb[ne[0], 0]), int(rb[ne[-1], 1])
+        got = vals_b[start:end].cpu().numpy().astype(np.uint32)
+        assert got.shape[0] == bn.vals.shape[0]
+        assert np.array_equal(got, bn.vals.astype(np.uint32)), f"view {v}: tile list contents"
+        say(f"  view {v}: {got.shape[0]} keys, every tile list equal to the oracle's")
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_train_16_captured_iteration_vs_oracle():
+    say("[headline] train_16: building the 16M-voxel scene (about a minute of host time) ...")
+    w = make_workload("train_16", device="cuda")
+    sc, cams = w.scene, w.cameras
+    m = VoxelModel(sc)
+    g = torch.Generator().manual_seed(1)
+    targets = [torch.rand(3, c.image_height, c.image_width, generator=g).cuda() for c in cams]
+    smooth = GridSmoothness(sc.vox_center, sc.vox_size, sc.octlevel, sc.is_leaf, sc.vox_key, sc.scene_center,
+                            sc.scene_extent, weight=1e-3, seed=0)
+    r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key, cams,
+                          m.bg_color, sc.sh_degree)
+    grid = m._geo_grid_pts.detach().clone()
+    sh0, shs = m._sh0.detach().clone(), m._shs.detach().clone()
+    lr, betas, eps = 1e-2, (0.9, 0.99), 1e-15
+    it = BatchedTrainIteration(r, grid, sh0, shs, m._log_s.detach(), targets, smooth, lr=lr, betas=betas, eps=eps)
+    it.capture(warmup=1)
+    torch.cuda.synchronize()
+    t = int(it.step.item()) + 1
+    g0, gm, gv = grid.clone(), it.grid_m.clone(), it.grid_v.clone()
+    g_start, s0, ss0 = grid.clone(), sh0.clone(), shs.clone()
+    it()
+    torch.cuda.synchronize()
+    it.check_capacity()
+    assert int(it.step.item()) == t
+    say(f"  {sc.num_voxels} voxels, {r.key_count()} keys, grid table {smooth.G} keys; replayed step {t}")
+    # corner-SDF Adam: the host-driven dense step on the replay's gradient
+    _C.unbiased_adam_step(False, g0, it.grid_grad, gm, gv, t, lr, betas[0], betas[1], eps)
+    assert torch.equal(g0, grid) and torch.equal(gm, it.grid_m) and torch.equal(gv, it.grid_v)
+    # crop of view 0 vs the oracle rendering of the parameters the replay started from
+    cam = cams[0]
+    win = centre_crop(cam)
+    x0, y0, x1, y1 = win
+    bn = ob.binning_for_camera(cam, sc)
+    dC = torch.zeros(3, y1 - y0, x1 - x0)
+    s64 = float(torch.exp(m._log_s.detach().reshape(1) * 10.0).item())
+    out, vid, *_ = oracle_crop_grads(sc, cam, bn, win, s64, dC, g_start, s0, ss0)
+    got = r.view_slice(r.color, 3, 0)[:, y0:y1, x0:x1].cpu().double()
+    e = (got - out.color.detach()).abs().max().item()
+    say(f"  view 0 crop colour vs oracle {e:.2e} ({vid.shape[0]} voxels in the crop's tiles)")
+    assert e <= 1e-4

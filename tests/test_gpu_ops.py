@@ -135,6 +135,54 @@ def test_grid_losses(kind):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["table", "ge", "ls"])
+def test_grid_table_with_parents(kind):
+    """Scene of levels 4-7 with the split cells of level >= 6 kept as non-leaf parents, table at
+    2^6 cells per axis: the level-6 parents (size / cell = 1) enter the table, their level-7
+    children (size / cell = 0.5) do not (utils.cu:205-206), and parent and children share their
+    coincident corner grid points, so the grid losses couple them."""
+    from svraster_amd.synthetic import build_scene
+    say(f"[table] non-leaf parents in the valid-gradient table: {kind}")
+    sc = build_scene(4, 4, 7, 1, seed=0, device="cuda", target_voxels=20_000, keep_parents_from=6)
+    leaf = sc.is_leaf.bool()
+    n_par = int((~leaf).sum())
+    mask, keys, g2v = svraster_cuda.utils.valid_gradient_table(sc.vox_center, sc.vox_size, sc.scene_center, 2.0, 6,
+                                                               leaf)
+    omask, okeys, oown = ogl.valid_gradient_table(sc.vox_center.cpu().numpy(), sc.vox_size.cpu().numpy(),
+                                                  np.zeros(3), 2.0, 6, leaf.cpu().numpy())
+    own_leaf = leaf[g2v.long()]
+    say(f"  {sc.num_voxels} voxels ({n_par} parents), {keys.numel()} keys, {int((~own_leaf).sum())} owned by parents")
+    assert n_par > 0 and int((~own_leaf).sum()) > 0
+    assert np.array_equal(mask.cpu().numpy(), omask) and np.array_equal(keys.cpu().numpy(), okeys)
+    assert np.array_equal(g2v.cpu().numpy(), oown)
+    if kind == "table":
+        return
+    sk, order = keys.sort(stable=True)
+    g2v = g2v[order]
+    res = 64
+    coord = torch.round(((sc.vox_center - sc.vox_size * 0.5) - (sc.scene_center - 1.0)) / 2.0 * res)
+    size = torch.round(sc.vox_size / 2.0 * res).reshape(-1)
+    G = sk.numel()
+    rng = np.random.default_rng(9)
+    active = torch.from_numpy(rng.permutation(G).astype(np.int32)).cuda()
+    randv = torch.from_numpy(rng.uniform(size=(G, 3)).astype(np.float32)).cuda()
+    grad = torch.zeros_like(sc.geo_grid_pts)
+    fn = svraster_cuda.grid_loss_bw.grid_eikonal if kind == "ge" else svraster_cuda.grid_loss_bw.laplacian_smoothness
+    fn(grid_pts=sc.geo_grid_pts, vox_key=sc.vox_key, grid_voxel_coord=coord, grid_voxel_size=size, grid_res=res,
+       grid_mask=mask, grid_keys=sk, grid2voxel=g2v, active_list=active, weight=1e-3, vox_size_inv=float(res / 2.0),
+       no_tv_s=True, tv_sparse=False, grid_pts_grad=grad, randv=randv)
+    ref = ogl.grid_loss_grad(kind, sc.geo_grid_pts.cpu().numpy(), sc.vox_key.cpu().numpy(), coord.cpu().numpy(),
+                             size.cpu().numpy(), res, mask.cpu().numpy(), sk.cpu().numpy(), g2v.cpu().numpy(),
+                             active.cpu().numpy(), 1e-3, res / 2.0, randv.cpu().numpy())
+    e = rel_l2(grad.reshape(-1), torch.from_numpy(ref))
+    # grid points of parent voxels receive gradient (the coupling to their children's corners)
+    par_pts = torch.unique(sc.vox_key[~leaf].reshape(-1)).cpu().numpy()
+    say(f"  rel-L2 {e:.2e}, |ref| {np.linalg.norm(ref):.3e}, parent corners with gradient "
+        f"{int((ref[par_pts] != 0).sum())} of {par_pts.shape[0]}")
+    assert np.linalg.norm(ref) > 0 and e <= 1e-4 and int((ref[par_pts] != 0).sum()) > 0
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("no_tv_s,tv_sparse", [(True, False), (False, False), (True, True)])
 def test_total_variation(no_tv_s, tv_sparse):
     say(f"[grid loss] total variation no_tv_s={no_tv_s} tv_sparse={tv_sparse}")

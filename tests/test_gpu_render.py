@@ -2,7 +2,13 @@
 
 Tolerances (FP32 kernel vs FP64 oracle): image, T and depth channel 0 max-abs <= 1e-4; normal
 max-abs <= 1e-4; n_contrib equal on >= 99.9 % of pixels (a saturation or 1e-5 alpha cut landing
-within FP32 rounding of the threshold may move it); gradients relative L2 <= 1e-3."""
+within FP32 rounding of the threshold may move it); gradients relative L2 <= 1e-3.
+
+Median depth (channel 2) is a piecewise-constant function of T and alpha: it jumps by one
+refinement step (a sixteenth of the voxel's chord) or to another voxel when a comparison with
+T = 0.5 or alpha = 1e-5 flips.  It is gated at max-abs <= 1e-4 on every pixel whose decisions
+the oracle finds clear of their thresholds (|T - 0.5| >= 1e-4 and |alpha - 1e-5| >= 1e-6), and
+those pixels must be >= 99 % of the frame."""
 import numpy as np
 import pytest
 import torch
@@ -40,12 +46,16 @@ def _compare_forward(name, window=None, s_mult=1.0):
     err_n = (out["normal"][sl].cpu().double() - ref.normal.detach()).abs().max().item()
     _, _, ncon = _C.unpack_ImageState(cam.image_width, cam.image_height, out["img"])
     same = (ncon[y0:y1, x0:x1].cpu().long() == ref.n_contrib).double().mean().item()
-    dmed_ok = ((out["depth"][2, y0:y1, x0:x1].cpu().double() - ref.depth[2]).abs() <= 1e-3).double().mean().item()
+    clear = ref.dmed_margin >= 1.0
+    dmed_err = (out["depth"][2, y0:y1, x0:x1].cpu().double() - ref.depth[2]).abs()
+    err_m = dmed_err[clear].max().item() if bool(clear.any()) else 0.0
+    frac_clear = clear.double().mean().item()
     say(f"  color {err_c:.2e} T {err_t:.2e} depth {err_d:.2e} normal {err_n:.2e} "
-        f"n_contrib-match {same:.5f} median-depth-match {dmed_ok:.5f}")
+        f"n_contrib-match {same:.5f} median depth {err_m:.2e} on {frac_clear:.5f} of pixels (clear decisions), "
+        f"{dmed_err.max().item():.2e} overall")
     assert err_c <= 1e-4 and err_t <= 1e-4 and err_d <= 1e-4 and err_n <= 1e-4
     assert same >= 0.999
-    assert dmed_ok >= 0.999
+    assert err_m <= 1e-4 and frac_clear >= 0.99
     if window is None:
         mw = out["max_w"].reshape(-1).cpu().double()
         assert (mw - ref.max_w).abs().max().item() <= 1e-4
@@ -117,7 +127,8 @@ def _compare_backward(name, window=None, concen=0.0, s_mult=1.0):
     say(f"  rel-L2 geo {e_geo:.2e} rgb {e_rgb:.2e} subdiv {e_sub:.2e} ds {e_s:.2e} "
         f"(|ds err| / sum|terms| {e_s_scaled:.2e}, sum|terms|/|ds| {ds_abs / max(abs(gsd.item()), 1e-30):.1f})")
     assert e_geo <= 1e-3 and e_rgb <= 1e-3 and e_sub <= 1e-3
-    assert e_s <= 1e-3 or e_s_scaled <= 1e-5
+    assert e_s <= 1e-3
+    return dgeo, drgb, ds
 
 
 @pytest.mark.gpu
@@ -139,3 +150,40 @@ def test_backward_single_1237_crop():
     cx, cy = w.cameras[0].image_width // 2, w.cameras[0].image_height // 2
     _compare_backward("single_1237", window=(cx - 32 - cx % 16, cy - 32 - cy % 16,
                                              cx + 32 - cx % 16, cy + 32 - cy % 16))
+
+
+@pytest.mark.gpu
+def test_backward_small_128_sharp_s():
+    say("[backward] small_128, s x 33 (late-training sharpness)")
+    _compare_backward("small_128", s_mult=33.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_backward_fd_256_full_frame():
+    say("[backward] fd_256 full 256x256 frame vs the FP64 oracle (about a minute of oracle time)")
+    _compare_backward("fd_256")
+
+
+@pytest.mark.gpu
+def test_backward_is_deterministic():
+    say("[backward] two backward calls on the same inputs: identical dL/ds and gradients within atomics")
+    w, sc, cam, geos32, rgbs64, s = _setup("small_128")
+    bg = torch.ones(3, device="cuda")
+    s32 = torch.tensor([s], dtype=torch.float32, device="cuda")
+    rgbs32 = rgbs64.float().cuda()
+    out = hip_raster(sc, cam, geos32, rgbs32, s32, bg, need_depth=False, need_normal=False)
+    H, W = cam.image_height, cam.image_width
+    dC = (torch.rand(3, H, W, generator=torch.Generator().manual_seed(2)) - 0.5).cuda()
+    res = []
+    for _ in range(2):
+        r = _C.rasterize_voxels_backward(
+            out["K"], 3, 1, W, H, cam.tanfovx, cam.tanfovy, cam.cx, cam.cy, cam.w2c, cam.c2w, bg,
+            sc.octpath, sc.vox_center, sc.vox_size, geos32, rgbs32, out["geom"], out["binning"], out["img"],
+            out["T"], dC, torch.empty(0), torch.empty(0), torch.empty(0), 0.0, None, 0.0, 0.0, False, False,
+            torch.empty(0), torch.empty(0), False, s32, torch.empty(0))
+        res.append([x.clone() for x in r])
+    torch.cuda.synchronize()
+    say(f"  dL/ds {res[0][3].item():.9e} / {res[1][3].item():.9e}")
+    assert torch.equal(res[0][3], res[1][3])
+    assert rel_l2(res[0][0], res[1][0]) <= 1e-6

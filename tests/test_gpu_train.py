@@ -49,8 +49,9 @@ def test_device_step_adam_matches_host_step(sparse):
 
 
 @pytest.mark.gpu
-def test_sh_backward_adam_matches_unfused():
-    say("[sh] SH transpose + Adam in one launch vs svr_batch_sh_backward + unbiased_adam_step")
+@pytest.mark.parametrize("sparse", [False, True])
+def test_sh_backward_adam_matches_unfused(sparse):
+    say(f"[sh] SH transpose + Adam in one launch vs svr_batch_sh_backward + unbiased_adam_step, sparse={sparse}")
     w, sc, cams, targets = _setup()
     m = VoxelModel(sc)
     r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key, cams,
@@ -61,26 +62,30 @@ def test_sh_backward_adam_matches_unfused():
     lib = _lib.load()
     it_state = BatchedTrainIteration(r, m._geo_grid_pts.detach().clone(), m._sh0.detach().clone(),
                                      m._shs.detach().clone(), m._log_s.detach(), targets, smooth=None,
-                                     lr=1e-2, lr_sh0=2e-3, lr_shs=5e-3)
+                                     lr=1e-2, lr_sh0=2e-3, lr_shs=5e-3, sparse=sparse)
     A = it_state.sh_adam
     ref = [m._sh0.detach().clone(), torch.zeros_like(A.m0), torch.zeros_like(A.v0),
            m._shs.detach().clone(), torch.zeros_like(A.ms), torch.zeros_like(A.vs)]
     g = torch.Generator().manual_seed(5)
     for t in (1, 2):
         drgb = torch.randn(r.drgb_views.shape, generator=g).cuda()
+        if t == 2:
+            # half the voxels get no gradient on the second step: the dense step still moves
+            # them (decaying moments), the sparse step leaves them alone
+            drgb[:, ::2] = 0.0
         dsh0 = torch.empty_like(r.dsh0)
         dshs = torch.empty_like(r.dshs)
         d1 = drgb.clone()
         _lib.check(lib.svr_batch_sh_backward(_p(r.ws), *r._dims(), _p(r.vox_center), _p(d1), r.deg, r.M, _p(dsh0),
                                              _p(dshs), None))
-        _C.unbiased_adam_step(True, ref[0], dsh0, ref[1], ref[2], t, 2e-3, 0.9, 0.99, 1e-15)
-        _C.unbiased_adam_step(True, ref[3], dshs, ref[4], ref[5], t, 5e-3, 0.9, 0.99, 1e-15)
+        _C.unbiased_adam_step(sparse, ref[0], dsh0, ref[1], ref[2], t, 2e-3, 0.9, 0.99, 1e-15)
+        _C.unbiased_adam_step(sparse, ref[3], dshs, ref[4], ref[5], t, 5e-3, 0.9, 0.99, 1e-15)
         A.step.fill_(t)
         d2 = drgb.clone()
         _lib.check(lib.svr_batch_sh_backward_adam(
             _p(r.ws), *r._dims(), _p(r.vox_center), _p(d2), r.deg, r.M, _p(A.sh0), _p(A.m0), _p(A.v0), _p(A.table0),
             _p(A.shs), _p(A.ms), _p(A.vs), _p(A.tables), A.n_table, _p(A.step), ctypes.c_float(0.9),
-            ctypes.c_float(0.99), ctypes.c_float(1e-15), None))
+            ctypes.c_float(0.99), ctypes.c_float(1e-15), int(sparse), None))
         torch.cuda.synchronize()
         n_upd = int((dshs != 0).sum()) + int((dsh0 != 0).sum())
         say(f"  step {t}: {n_upd} SH elements updated")
@@ -142,8 +147,10 @@ def test_grid_smoothness_step_matches_api():
 
 
 @pytest.mark.gpu
-def test_train_iteration_graph_replays():
-    say("[train] captured iteration: each replay = drop-in render + smoothness + Adam on the same inputs")
+@pytest.mark.parametrize("sparse", [False, True])
+def test_train_iteration_graph_replays(sparse):
+    say(f"[train] captured iteration: each replay = drop-in render + smoothness + Adam on the same inputs, "
+        f"sparse={sparse}")
     w, sc, cams, targets, sm = _smooth_setup()
     m = VoxelModel(sc)
     r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key, cams,
@@ -151,7 +158,7 @@ def test_train_iteration_graph_replays():
     grid = m._geo_grid_pts.detach().clone()
     sh0, shs = m._sh0.detach().clone(), m._shs.detach().clone()
     it = BatchedTrainIteration(r, grid, sh0, shs, m._log_s.detach(), targets, smooth=sm, lr=1e-2,
-                               betas=(0.9, 0.99))
+                               betas=(0.9, 0.99), sparse=sparse)
     it.capture(warmup=1)
     torch.cuda.synchronize()
     for k in range(3):
@@ -170,8 +177,8 @@ def test_train_iteration_graph_replays():
             mv._shs.copy_(ss0)
             err = max((r.view_slice(r.color, 3, v) - mv.render(cam)["color"]).abs().max().item()
                       for v, cam in enumerate(cams))
-        # Adam on the corner SDFs: exactly the host-driven sparse step on the replay's gradient
-        _C.unbiased_adam_step(True, g0, it.grid_grad, gm, gv, t, 1e-2, 0.9, 0.99, 1e-15)
+        # Adam on the corner SDFs: exactly the host-driven step on the replay's gradient
+        _C.unbiased_adam_step(sparse, g0, it.grid_grad, gm, gv, t, 1e-2, 0.9, 0.99, 1e-15)
         n_upd = int((it.grid_grad != 0).sum())
         n_sh = int((sh0 != s0).sum()) + int((shs != ss0).sum())
         say(f"  replay {k}: step {t}, colour max-abs vs drop-in render {err:.2e}, {n_upd} corner SDFs and "

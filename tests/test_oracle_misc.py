@@ -48,6 +48,39 @@ def test_adam_matches_torch_optim():
     np.testing.assert_allclose(mine[0], p.detach().numpy(), rtol=1e-12, atol=1e-12)
 
 
+def test_biased_adam_by_hand_and_via_torch_optim():
+    """Biased form (adam_step.cu:137-158): hand values, and the identity that the biased step
+    with lr' = lr sqrt(1-b2^t) / (1-b1^t) and eps' = eps sqrt(1-b2^t) is the bias-corrected step,
+    i.e. torch.optim.Adam (a library routine)."""
+    # step 1 from zero moments, eps = 0: m = 0.1 g, v = 0.01 g^2, p -= 0.1 * 0.1 g / (0.1 |g|)
+    p, m, v = oad.adam_step([1.0, -2.0, 0.5], [4.0, -0.25, 0.0], [0.0] * 3, [0.0] * 3, 1, 0.1, 0.9, 0.99, 0.0,
+                            biased=True, sparse=True)
+    np.testing.assert_allclose(p, [0.9, -1.9, 0.5], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(m, [0.4, -0.025, 0.0], rtol=1e-15)
+    np.testing.assert_allclose(v, [0.16, 0.000625, 0.0], rtol=1e-13)
+    # step 2 with g = 2 on the first element: m = 0.9*0.4 + 0.1*2 = 0.56,
+    # v = 0.99*0.16 + 0.01*4 = 0.1984, p = 0.9 - 0.1 * 0.56 / sqrt(0.1984)
+    p2, m2, v2 = oad.adam_step(p[:1], [2.0], m[:1], v[:1], 2, 0.1, 0.9, 0.99, 0.0, biased=True)
+    np.testing.assert_allclose(m2, [0.56], rtol=1e-15)
+    np.testing.assert_allclose(v2, [0.1984], rtol=1e-14)
+    np.testing.assert_allclose(p2, [0.9 - 0.1 * 0.56 / np.sqrt(0.1984)], rtol=1e-15)
+    # biased with per-step rescaled lr / eps == torch.optim.Adam
+    rng = np.random.default_rng(12)
+    lr, b1, b2, eps = 1e-2, 0.9, 0.99, 1e-8
+    p0 = rng.normal(size=50)
+    pt = torch.tensor(p0, requires_grad=True)
+    opt = torch.optim.Adam([pt], lr=lr, betas=(b1, b2), eps=eps)
+    mine = (p0.copy(), np.zeros(50), np.zeros(50))
+    for t in range(1, 8):
+        g = rng.normal(size=50)
+        pt.grad = torch.tensor(g)
+        opt.step()
+        bc1, bc2 = 1 - b1 ** t, 1 - b2 ** t
+        mine = oad.adam_step(mine[0], g, mine[1], mine[2], t, lr * np.sqrt(bc2) / bc1, b1, b2, eps * np.sqrt(bc2),
+                             biased=True)
+    np.testing.assert_allclose(mine[0], pt.detach().numpy(), rtol=1e-12, atol=1e-12)
+
+
 def test_adam_sparse_skips_zero_gradients():
     p, m, v = oad.adam_step(np.ones(4), np.array([0.0, 1.0, 0.0, -2.0]), np.full(4, 0.5), np.full(4, 0.25),
                             3, 0.1, 0.9, 0.99, 1e-8, sparse=True)

@@ -56,6 +56,87 @@ def test_linear_sdf_alpha_closed_form():
     assert abs(out.color[0, 16, 16].item() - alpha) < 1e-12
 
 
+def _median_closed_form(s, f, a, b, phi_ref):
+    """Median depth distance along the ray for a linear SDF f(t) refined in [a, b] whose running
+    transmittance is T_m = (phi_m + eps) / (phi_ref + eps): the reference's 16-step loop
+    (forward.cu:473-519) telescopes, since 1 - alpha_k = (phi_k + eps) / (phi_{k-1} + eps).
+    The loop stops after the first step m with T_m <= 0.5, i.e. once t_m passes the root t* of
+    sigmoid(s f(t*)) = 0.5 (phi_ref + eps) - eps, solved here with the logit."""
+    step = (b - a) / 16.0
+    target = 0.5 * (phi_ref + orc.EPS_PHI) - orc.EPS_PHI
+    if target <= 0.0:
+        n = 16
+    else:
+        f_star = np.log(target / (1.0 - target)) / s
+        fa, fb = f(a), f(b)
+        t_star = a + (f_star - fa) / ((fb - fa) / (b - a))
+        n = int(min(16, max(1, np.ceil((t_star - a) / step))))
+    return a + (n - 0.5) * step, n
+
+
+@pytest.mark.parametrize("s", [20.0, 45.0, 2.0])
+def test_median_depth_closed_form_single_voxel(s):
+    """Depth channel 2 of one voxel with an SDF linear along the ray (trilinear interpolation is
+    exact): the refinement stops at the step computed from the telescoped transmittance.  s = 2
+    never reaches T <= 0.5 inside the voxel, so all 16 steps run."""
+    centers, sizes, _, geos = _one_voxel_scene(lambda p: -p[2] + 0.05)
+    cam, out = _render_single(geos, centers, sizes, s, W=32, H=32)
+    rc = orc.camera_from(cam)
+    sig = lambda x: 1.0 / (1.0 + np.exp(-x))
+    checked = 0
+    for (px, py) in [(16, 16), (15, 17), (17, 15)]:
+        ro, rd, nrm = orc.pixel_rays(rc, torch.tensor([px]), torch.tensor([py]))
+        a, b = orc.ray_aabb(centers.double(), sizes.double().reshape(-1), ro, (1.0 / rd[0])[None])
+        a, b = float(a), float(b)
+        f = lambda t: -(float(ro[2]) + t * float(rd[0, 2])) + 0.05
+        dmed, n = _median_closed_form(s, f, a, b, sig(s * f(a)))
+        got = out.depth[2, py, px].item()
+        assert abs(got - dmed / float(nrm[0])) < 1e-9, (px, py, got, dmed / float(nrm[0]), n)
+        checked += 1
+        if s == 2.0:
+            assert n == 16
+        else:
+            assert 1 < n < 16
+    assert checked == 3
+
+
+def test_median_depth_closed_form_two_voxels():
+    """Two voxels stacked along z sharing one linear SDF.  With the crossing in the back voxel the
+    median voxel is the back one (last contributor entered with T > 0.5) and its refinement starts
+    from the front voxel's transmittance; with the crossing in the front voxel the back voxel is
+    never the median voxel."""
+    sig = lambda x: 1.0 / (1.0 + np.exp(-x))
+    for z0, s, which in ((0.2, 12.0, 1), (-0.15, 12.0, 0)):
+        c = np.array([[0.0, 0.0, -0.25], [0.0, 0.0, 0.25]])
+        corners = np.array([[(i >> 2) & 1, (i >> 1) & 1, i & 1] for i in range(8)], dtype=np.float64)
+        geos = torch.tensor([[-(cc[2] - 0.25 + k[2] * 0.5) + z0 for k in corners] for cc in c], dtype=torch.float64)
+        centers = torch.tensor(c, dtype=torch.float32)
+        sizes = torch.full((2, 1), 0.5, dtype=torch.float32)
+        W = H = 32
+        cam = look_at_camera(np.array([0.0, 0.0, -2.0]), W, H, 0.8 * W, 0.8 * W)
+        paths = np.array([0, 1 << 45], dtype=np.int64)
+        bn = ob.binning(W, H, cam.tanfovx, cam.tanfovy, cam.cx, cam.cy, cam.w2c.numpy(), cam.c2w.numpy(), 0.02,
+                        paths, centers.numpy(), sizes.numpy(), np.ones(2))
+        out = orc.render(orc.camera_from(cam), [0.0] * 3, bn.vals, bn.ranges, bn.pre.bbox, centers, sizes, geos,
+                         torch.ones(2, 3, dtype=torch.float64), torch.tensor(s, dtype=torch.float64))
+        rc = orc.camera_from(cam)
+        px, py = 16, 15
+        ro, rd, nrm = orc.pixel_rays(rc, torch.tensor([px]), torch.tensor([py]))
+        f = lambda t: -(float(ro[2]) + t * float(rd[0, 2])) + z0
+        a0, b0 = (float(x) for x in orc.ray_aabb(centers[:1].double(), sizes[:1].double().reshape(-1), ro,
+                                                 (1.0 / rd[0])[None]))
+        a1, b1 = (float(x) for x in orc.ray_aabb(centers[1:].double(), sizes[1:].double().reshape(-1), ro,
+                                                 (1.0 / rd[0])[None]))
+        phi_entry = sig(s * f(a0))
+        T_front = (sig(s * f(b0)) + orc.EPS_PHI) / (phi_entry + orc.EPS_PHI)
+        assert (T_front > 0.5) == (which == 1)
+        a, b = (a1, b1) if which == 1 else (a0, b0)
+        dmed, n = _median_closed_form(s, f, a, b, phi_entry)
+        got = out.depth[2, py, px].item()
+        assert 1 < n < 16
+        assert abs(got - dmed / float(nrm[0])) < 1e-9, (which, got, dmed / float(nrm[0]))
+
+
 def test_weights_plus_T_sum_to_one_and_subalpha_telescoping():
     w = make_workload("small_128")
     sc, cam = w.scene, w.cameras[0]
