@@ -185,7 +185,8 @@ def train_iteration_bytes(it, smooth):
 
 def bench_train(args):
     """Full training iteration at scale (train_16): 16 views at 1920x1080 over the 16M-leaf scene;
-    render + MSE + backward + Laplacian smoothness (weight 1e-3) + Adam (lr 1e-2, betas 0.9/0.99)
+    render + MSE + backward + grid eikonal (the trainer's weight at iteration 0, 1e-8) + Laplacian
+    smoothness (weight 1e-3) + Adam (lr 1e-2, betas 0.9/0.99)
     in the reference trainer's mode: SparseAdam with its defaults (sparse=False, biased=False),
     i.e. every element of every group steps every iteration.  The plain per-view iteration runs
     the drop-in API the way the reference trainer calls it (one view per call, randperm active
@@ -193,7 +194,8 @@ def bench_train(args):
     import torch
     import svraster_amd as svraster_cuda
     from svraster_amd import _lib
-    from svraster_amd.batched import BatchedRasterizer, BatchedTrainIteration, GridSmoothness
+    from svraster_amd.batched import (BatchedRasterizer, BatchedTrainIteration, GridSmoothness,
+                                      trainer_smoothness_weights)
     from svraster_amd.model import VoxelModel
     from svraster_amd.sparse_adam import SparseAdam
     from svraster_amd.synthetic import make_workload
@@ -215,8 +217,9 @@ def bench_train(args):
     weight, lr, betas, eps = 1e-3, 1e-2, (0.9, 0.99), 1e-15
 
     t0 = time.perf_counter()
+    w_ge = trainer_smoothness_weights(0)[0]          # the reference trainer's eikonal weight at iteration 0
     smooth = GridSmoothness(sc.vox_center, sc.vox_size, sc.octlevel, sc.is_leaf, sc.vox_key, sc.scene_center,
-                            sc.scene_extent, weight=weight, seed=0)
+                            sc.scene_extent, weight=weight, seed=0, eikonal_weight=w_ge)
     sync()
     log(f"[bench] grid table: res {smooth.res}, {smooth.G} keys ({time.perf_counter() - t0:.1f} s)")
 
@@ -229,12 +232,14 @@ def bench_train(args):
         per_view_step(model, cams, targets)
 
     def pv_smooth():
-        active = torch.randperm(smooth.G, device=dev).to(torch.int32)
+        kw = dict(grid_pts=model._geo_grid_pts.detach(), vox_key=sc.vox_key, grid_voxel_coord=smooth.coord,
+                  grid_voxel_size=smooth.size, grid_res=smooth.res, grid_mask=smooth.mask, grid_keys=smooth.keys,
+                  grid2voxel=smooth.g2v, vox_size_inv=smooth.vox_size_inv, no_tv_s=True, tv_sparse=False,
+                  grid_pts_grad=model._geo_grid_pts.grad)
+        svraster_cuda.grid_loss_bw.grid_eikonal(
+            active_list=torch.randperm(smooth.G, device=dev).to(torch.int32), weight=w_ge, **kw)
         svraster_cuda.grid_loss_bw.laplacian_smoothness(
-            grid_pts=model._geo_grid_pts.detach(), vox_key=sc.vox_key, grid_voxel_coord=smooth.coord,
-            grid_voxel_size=smooth.size, grid_res=smooth.res, grid_mask=smooth.mask, grid_keys=smooth.keys,
-            grid2voxel=smooth.g2v, active_list=active, weight=weight, vox_size_inv=smooth.vox_size_inv,
-            no_tv_s=True, tv_sparse=False, grid_pts_grad=model._geo_grid_pts.grad)
+            active_list=torch.randperm(smooth.G, device=dev).to(torch.int32), weight=weight, **kw)
 
     def pv_adam():
         opt.step()
@@ -307,7 +312,8 @@ def bench_train(args):
         "data": "synthetic (seeded 48-sphere + 8-box SDF scene, random SH, random targets)",
         "config": {"workload": f"{args.workload}: {sc.num_voxels} voxels ({n_leaf} leaves, levels 7-13, parents "
                                f"kept from level 9), SH deg 3, {len(cams)} views 1920x1080, {rays} rays/step; "
-                               f"training iteration: fwd + MSE + bwd + Laplacian smoothness (weight {weight}) "
+                               f"training iteration: fwd + MSE + bwd + grid eikonal (weight {w_ge}) + Laplacian "
+                               f"smoothness (weight {weight}) "
                                f"+ dense Adam (lr {lr}, betas {betas}, SparseAdam defaults)",
                    "path": "batched: " + ("eager (profiler run)" if args.profile
                                           else "one captured HIP graph per iteration")},
@@ -337,6 +343,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
     ap.add_argument("--no-autotune", action="store_true", help="default launch configuration, no tuning")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="profiling aid: skip the per-view leg (the JSON then carries no per-view numbers)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -381,19 +389,24 @@ def main():
         zero_grads(model)
         per_view_step(model, cams, targets)
 
+    if args.batched_only:
+        step_pv = lambda: None
     ms_pv = time_steps(step_pv, args.steps, args.warmup, sync)
     kt_pv = kernel_times(lib, step_pv)
+    if args.batched_only:
+        ms_pv = float("nan")
     log(f"[bench] per-view path: {ms_pv:.2f} ms/step; compositor fwd {kt_pv['render_fwd'][0]:.2f} ms "
         f"bwd {kt_pv['render_bwd'][0]:.2f} ms per step")
     ref_colors = []
-    with torch.no_grad():
-        for cam in cams[:: max(1, len(cams) // 4)]:
-            ref_colors.append(model.render(cam)["color"].clone())
-    zero_grads(model)
-    per_view_step(model, cams, targets)
-    sync()
-    ref_grid = model._geo_grid_pts.grad.clone()
-    ref_sh0 = model._sh0.grad.clone()
+    if not args.batched_only:
+        with torch.no_grad():
+            for cam in cams[:: max(1, len(cams) // 4)]:
+                ref_colors.append(model.render(cam)["color"].clone())
+        zero_grads(model)
+        per_view_step(model, cams, targets)
+        sync()
+        ref_grid = model._geo_grid_pts.grad.clone()
+        ref_sh0 = model._sh0.grad.clone()
 
     # -- batched persistent path, autotuned, captured as a graph --------------------------------
     sc = w.scene
@@ -414,12 +427,14 @@ def main():
     bstep()
     sync()
     K_used = rast.key_count()
-    img_err = 0.0
-    for i, v in enumerate(range(0, len(cams), max(1, len(cams) // 4))):
-        img_err = max(img_err, (rast.view_slice(rast.color, 3, v) - ref_colors[i]).abs().max().item())
-    rel = lambda a, b: float((a - b).double().norm() / b.double().norm().clamp_min(1e-30))
-    grad_err = max(rel(bstep.grid_grad, ref_grid), rel(bstep.sh0_grad, ref_sh0))
-    log(f"[bench] batched vs per-view: image max-abs {img_err:.2e}, grad rel-L2 {grad_err:.2e}, "
+    img_err, grad_err = None, None
+    if not args.batched_only:
+        img_err = 0.0
+        for i, v in enumerate(range(0, len(cams), max(1, len(cams) // 4))):
+            img_err = max(img_err, (rast.view_slice(rast.color, 3, v) - ref_colors[i]).abs().max().item())
+        rel = lambda a, b: float((a - b).double().norm() / b.double().norm().clamp_min(1e-30))
+        grad_err = max(rel(bstep.grid_grad, ref_grid), rel(bstep.sh0_grad, ref_sh0))
+    log(f"[bench] batched vs per-view: image max-abs {img_err}, grad rel-L2 {grad_err}, "
         f"keys {K_used} / capacity {rast.cap}")
     kt_b = kernel_times(lib, bstep)
     if not args.profile:          # profiler runs time the same step eagerly (no graph replay)

@@ -222,6 +222,22 @@ int svr_grid_laplacian_bw_table_order(const float* grid_pts, long long n_pts, co
                                       const int32_t* cell_tvox, const int32_t* tvox, int T, float weight,
                                       float vox_size_inv, unsigned int seed, const int32_t* step, float* rec_ws,
                                       float* coef_ws, float* grid_pts_grad, void* stream);
+/* The table-order form above for GE_COMPUTE::grid_eikonal_bw (src/ge_compute.cu:14-161) and/or
+ * the Laplacian, as the reference trainer applies both each iteration (train.py:588-654, ge on
+ * for iterations 0-6000, ls for 0-8000).  terms: 1 = Laplacian, 2 = eikonal, 3 = both; weights:
+ * device float[2] = {eikonal weight, Laplacian weight} (each already times G / A), read by the
+ * kernels, so a captured graph follows the trainer's decay schedule when the caller rewrites
+ * them between replays.  Both terms use the same per-key random offset (the reference draws
+ * each term's own from the clock; either way every key gets a uniform offset).  coef_ws holds
+ * float[3G] for the Laplacian alone, float[7G] with the eikonal. */
+int svr_grid_smoothness_bw_table_order(const float* grid_pts, long long n_pts, const int64_t* vox_key, int N,
+                                       const float* grid_voxel_coord, const float* grid_voxel_size, int grid_res,
+                                       const uint8_t* grid_mask, const int32_t* table_keys,
+                                       const int32_t* table_tvox, int G, const int32_t* cell_entry,
+                                       const int32_t* cell_tvox, const int32_t* tvox, int T, int terms,
+                                       const float* weights, float vox_size_inv, unsigned int seed,
+                                       const int32_t* step, float* rec_ws, float* coef_ws, float* grid_pts_grad,
+                                       void* stream);
 
 /* The voxel <-> grid-point link of octree_utils.build_grid_pts_link (src/utils/octree_utils.py:
  * 129-144, 203-222): the 8 corners of each voxel as integer coordinates at level 16, deduplicated

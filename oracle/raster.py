@@ -70,15 +70,15 @@ def camera_from(cam) -> RasterCamera:
                         f32v(cam.cx), f32v(cam.cy), cam.c2w.detach().cpu().double())
 
 
-def pixel_rays(cam: RasterCamera, px: torch.Tensor, py: torch.Tensor):
-    """Ray origin, unit direction, |cam_rd| for integer pixel coords (float64)."""
-    dx = (px.double() + 0.5 - cam.cx) * 2.0 * cam.tanfovx / cam.W
-    dy = (py.double() + 0.5 - cam.cy) * 2.0 * cam.tanfovy / cam.H
+def pixel_rays(cam: RasterCamera, px: torch.Tensor, py: torch.Tensor, dtype=torch.float64):
+    """Ray origin, unit direction, |cam_rd| for integer pixel coords (float64 by default)."""
+    dx = (px.to(dtype) + 0.5 - cam.cx) * 2.0 * cam.tanfovx / cam.W
+    dy = (py.to(dtype) + 0.5 - cam.cy) * 2.0 * cam.tanfovy / cam.H
     cam_rd = torch.stack([dx, dy, torch.ones_like(dx)], -1)
     rd_norm = cam_rd.norm(dim=-1)
-    R = cam.c2w[:3, :3]
+    R = cam.c2w[:3, :3].to(dtype)
     rd = (cam_rd @ R.T) / rd_norm[..., None]
-    ro = cam.c2w[:3, 3]
+    ro = cam.c2w[:3, 3].to(dtype)
     return ro, rd, rd_norm
 
 
@@ -174,20 +174,24 @@ class OracleOut:
 def render(cam: RasterCamera, bg, bin_vals: np.ndarray, bin_ranges: np.ndarray,
            bbox: np.ndarray, centers, lengths, geos, rgbs, s,
            need_depth=True, need_normal=True, window=None,
-           lambda_R_concen: float = 0.0, gt_color: Optional[torch.Tensor] = None) -> OracleOut:
+           lambda_R_concen: float = 0.0, gt_color: Optional[torch.Tensor] = None,
+           dtype=torch.float64) -> OracleOut:
     """Render the pixels of `window` = (x0, y0, x1, y1) (default: whole image).
 
-    `geos` [P,8], `rgbs` [P,3], `s` (0-dim) are float64 tensors that may require grad."""
+    `geos` [P,8], `rgbs` [P,3], `s` (0-dim) are tensors of `dtype` that may require grad.  The
+    oracle proper runs in float64; dtype=torch.float32 runs the same restatement in single
+    precision (the FP32 CPU baseline, and a measure of what FP32 arithmetic alone changes).
+    The median depth refinement always runs in float64."""
     W, H = cam.W, cam.H
     if window is None:
         window = (0, 0, W, H)
     x0, y0, x1, y1 = window
     h, w = y1 - y0, x1 - x0
     dev = torch.device("cpu")
-    dt = torch.float64
+    dt = dtype
     bg = torch.as_tensor(bg, dtype=dt)
-    centers = torch.as_tensor(centers).double()
-    lengths = torch.as_tensor(lengths).double().reshape(-1)
+    centers = torch.as_tensor(centers).to(dt)
+    lengths = torch.as_tensor(lengths).to(dt).reshape(-1)
     P = centers.shape[0]
     tiles_x = (W + BLOCK - 1) // BLOCK
 
@@ -214,7 +218,7 @@ def render(cam: RasterCamera, bg, bin_vals: np.ndarray, bin_ranges: np.ndarray,
             PY, PX = torch.meshgrid(gy, gx, indexing="ij")
             px, py = PX.reshape(-1), PY.reshape(-1)
             npix = px.numel()
-            ro, rd, rd_norm = pixel_rays(cam, px, py)
+            ro, rd, rd_norm = pixel_rays(cam, px, py, dt)
             pq = ray_quadrant(rd)
             lo, hi = int(bin_ranges[tile, 0]), int(bin_ranges[tile, 1])
             vals = bin_vals[lo:hi].astype(np.int64)
@@ -270,12 +274,15 @@ def render(cam: RasterCamera, bg, bin_vals: np.ndarray, bin_ranges: np.ndarray,
                 mw = torch.where(pc, wd, torch.zeros_like(wd)).max(0).values
                 max_w.scatter_reduce_(0, vid, mw, reduce="amax")
                 if lambda_R_concen > 0 and gt_color is not None:
-                    gt = gt_color.double()[:, py, px].T            # [npix,3]
+                    gt = gt_color.to(dt)[:, py, px].T              # [npix,3]
                     diff = rgbs[vid][None] - gt[:, None, :]
                     reg = reg + lambda_R_concen / (W * H) * (wd * (diff * diff).sum(-1)).sum()
                 if need_depth:
-                    dmed, margin = median_depth(T_excl.detach(), pc, vid, centers, lengths,
-                                                geos.detach(), float(s.detach()), ro, rd, return_margin=True)
+                    dmed, margin = median_depth(T_excl.detach().double(), pc, vid, centers.double(),
+                                                lengths.double(), geos.detach().double(), float(s.detach()),
+                                                ro.double(), rd.double(), return_margin=True)
+                    dmed = dmed.to(dt)
+                    margin = margin.to(dt)
                     with torch.no_grad():
                         Te = T_excl.detach()
                         m_sel = torch.where(pc, (Te - 0.5).abs(), torch.full_like(Te, float("inf"))).min(1).values
@@ -366,28 +373,29 @@ def median_depth(T_excl, pc, vid, centers, lengths, geos, s, ro, rd, return_marg
 
 
 def backward(out: OracleOut, geos, rgbs, s, dL_dcolor, dL_ddepth=None, dL_dnormal=None,
-             dL_dT=None, return_ds_abs=False):
+             dL_dT=None, return_ds_abs=False, retain_graph=False):
     """Gradients the reference's backward returns: (dL_dgeos, dL_drgbs, subdiv_p_bw, dL_ds)
     for the given output cotangents (each [C,h,w] over the rendered window).  With return_ds_abs,
     also the sum over pairs of |dL/dalpha * dalpha/ds| -- the scale of the cancelling sum dL/ds."""
-    loss = (out.color * dL_dcolor.double()).sum()
+    dt = out.color.dtype
+    loss = (out.color * dL_dcolor.to(dt)).sum()
     if dL_ddepth is not None:
-        loss = loss + (out.depth[0] * dL_ddepth.double()[0]).sum()
+        loss = loss + (out.depth[0] * dL_ddepth.to(dt)[0]).sum()
     if dL_dnormal is not None:
-        loss = loss + (out.normal * dL_dnormal.double()).sum()
+        loss = loss + (out.normal * dL_dnormal.to(dt)).sum()
     if dL_dT is not None:
-        loss = loss + (out.T * dL_dT.double()).sum()
+        loss = loss + (out.T * dL_dT.to(dt)).sum()
     loss = loss + out.reg_loss
     alphas = [pa[0] for pa in out.pair_alpha]
-    gs = torch.autograd.grad(loss, [geos, rgbs, s] + alphas, allow_unused=True)
+    gs = torch.autograd.grad(loss, [geos, rgbs, s] + alphas, allow_unused=True, retain_graph=retain_graph)
     g_geo, g_rgb, g_s = gs[0], gs[1], gs[2]
     P = geos.shape[0]
-    subdiv = torch.zeros(P, dtype=torch.float64)
+    subdiv = torch.zeros(P, dtype=dt)
     ds_abs = 0.0
     for (alpha, pc, vid, dads), ga in zip(out.pair_alpha, gs[3:]):
         if ga is None:
             continue
-        pcf = pc.to(torch.float64)
+        pcf = pc.to(dt)
         v = (ga * alpha.detach()).abs() * pcf
         subdiv.index_add_(0, vid, v.sum(0))
         ds_abs += float(((ga * dads).abs() * pcf).sum())

@@ -76,6 +76,8 @@ _SIGS = {
     "svr_points_loss_bw": (I, [P, I, P, LL, P, I, P, P, I, P, P, P, I, F, P, P]),
     "svr_grid_smoothness_bw_step": (I, [I, P, LL, P, I, P, P, I, P, P, P, I, P, P, I, F, F, U, P, P, P]),
     "svr_grid_laplacian_bw_table_order": (I, [P, LL, P, I, P, P, I, P, P, P, I, P, P, P, I, F, F, U, P, P, P, P, P]),
+    "svr_grid_smoothness_bw_table_order": (I, [P, LL, P, I, P, P, I, P, P, P, I, P, P, P, I, I, P, F, U, P, P, P, P,
+                                               P]),
     "svr_batch_state_offsets": (I, [I, I, LL, LL, LL, P]),
     "svr_batch_key_count": (I, [P, I, I, LL, LL, LL, ctypes.POINTER(LL), P]),
     "svr_batch_key_peak": (I, [P, I, I, LL, LL, LL, I, ctypes.POINTER(LL), P]),

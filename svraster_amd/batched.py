@@ -366,6 +366,23 @@ def grid_table_level(octlevel, outside_level: int = 0) -> int:
     return lv
 
 
+def trainer_smoothness_weights(iteration: int, G: int = 1, lambda_ge: float = 1e-8, ge_from: int = 0,
+                               ge_until: int = 6000, ge_decay_every: int = 2000, ge_decay_mult: float = 0.25,
+                               ge_drop_ratio: float = 0.0, lambda_ls: float = 1e-10, ls_from: int = 0,
+                               ls_until: int = 8000, ls_decay_every: int = 2000, ls_decay_mult: float = 0.25,
+                               ls_drop_ratio: float = 0.0):
+    """(eikonal, Laplacian) weights the reference trainer passes at `iteration`
+    (train.py:588-654; defaults from config.py:154-167): lambda * mult^min(iteration // every, 2)
+    * G / K with K = int(G * (1 - drop_ratio)) inside [from, until], else 0."""
+    def one(lam, lo, hi, every, mult, drop):
+        if not lam or iteration < lo or iteration > hi:
+            return 0.0
+        K = int(G * (1.0 - float(drop)))
+        return lam * mult ** min(iteration // every, 2) * (G / max(K, 1))
+    return (one(lambda_ge, ge_from, ge_until, ge_decay_every, ge_decay_mult, ge_drop_ratio),
+            one(lambda_ls, ls_from, ls_until, ls_decay_every, ls_decay_mult, ls_drop_ratio))
+
+
 def grid_voxel_coords(vox_center, vox_size, scene_center, inside_extent: float, level: int):
     """Voxel min corners and sizes in cells of the 2^level grid over the inside region, rounded
     to integers (train.py:962-965): float32 [P,3] and [P]."""
@@ -378,8 +395,9 @@ def grid_voxel_coords(vox_center, vox_size, scene_center, inside_extent: float, 
 
 
 class GridSmoothness:
-    """Laplacian smoothness backward on the valid-gradient grid table, as the reference trainer
-    calls it after topology changes (train.py:959-972, 636-654): grid resolution
+    """Laplacian smoothness (and, with eikonal_weight != 0, grid eikonal) backward on the
+    valid-gradient grid table, as the reference trainer calls them each iteration
+    (train.py:588-654; table rebuilt after topology changes, train.py:959-972): grid resolution
     2^min(max level - outside_level, 9), voxel coordinates and sizes in grid cells rounded to
     integers, keys sorted (octree_utils.py:293-315), weight * G / A, every key active.
 
@@ -395,7 +413,8 @@ class GridSmoothness:
     (svr_grid_dense_index) instead of a binary search per sample."""
 
     def __init__(self, vox_center, vox_size, octlevel, is_leaf, vox_key, scene_center, inside_extent: float,
-                 weight: float, seed: int = 0, pull: bool = True, outside_level: int = 0):
+                 weight: float, seed: int = 0, pull: bool = True, outside_level: int = 0,
+                 eikonal_weight: float = 0.0):
         from . import _C
         self.lib = _lib.load()
         self.dev = vox_center.device
@@ -435,9 +454,15 @@ class GridSmoothness:
             self.cell_tv = torch.full((n3,), -1, dtype=torch.int32, device=self.dev)
             self.cell_tv[tk] = self.table_tv
             del tk
+        # Term weights (each already * G / A with A = G): eikonal, Laplacian.  The eikonal term is
+        # on when its weight is non-zero at construction; the kernels read the weights from device
+        # memory, so set_weights() changes them for later replays of a captured step.
+        self.eikonal = float(eikonal_weight) != 0.0
+        self.weights = torch.tensor([float(eikonal_weight), float(weight)], dtype=torch.float32, device=self.dev)
+        self.weight, self.eikonal_weight = float(weight), float(eikonal_weight)
+        if self.pull:
             self.rec = torch.empty(12 * self.T, dtype=torch.float32, device=self.dev)
-            self.coef = torch.empty(3 * self.G, dtype=torch.float32, device=self.dev)
-        self.weight = float(weight)          # * G / A with A = G
+            self.coef = torch.empty((7 if self.eikonal else 3) * self.G, dtype=torch.float32, device=self.dev)
         self.vox_size_inv = float(self.res / inside_extent)
         self.seed = int(seed) & 0xFFFFFFFF
 
@@ -483,20 +508,34 @@ class GridSmoothness:
     def _stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
 
+    def set_weights(self, eikonal: Optional[float] = None, laplacian: Optional[float] = None):
+        """New term weights for the following calls / replays (a host-to-device copy; not inside a
+        capture).  The eikonal weight can change only if the term was enabled at construction."""
+        if eikonal is not None:
+            if float(eikonal) != 0.0 and not self.eikonal:
+                raise ValueError("the eikonal term was not enabled at construction (eikonal_weight = 0)")
+            self.eikonal_weight = float(eikonal)
+        if laplacian is not None:
+            self.weight = float(laplacian)
+        self.weights.copy_(torch.tensor([self.eikonal_weight, self.weight], dtype=torch.float32))
+
     def __call__(self, grid_pts: torch.Tensor, grid_grad: torch.Tensor, step: torch.Tensor):
         if self.pull:
-            _lib.check(self.lib.svr_grid_laplacian_bw_table_order(
+            terms = 3 if self.eikonal else 1
+            _lib.check(self.lib.svr_grid_smoothness_bw_table_order(
                 _p(grid_pts), grid_pts.numel(), _p(self.vox_key), self.N, _p(self.coord), _p(self.size), self.res,
                 _p(self.mask), _p(self.table_keys), _p(self.table_tv), self.G, _p(self.cell_entry),
-                _p(self.cell_tv), _p(self.tvox), self.T, ctypes.c_float(self.weight),
+                _p(self.cell_tv), _p(self.tvox), self.T, terms, _p(self.weights),
                 ctypes.c_float(self.vox_size_inv), ctypes.c_uint(self.seed), _p(step), _p(self.rec), _p(self.coef),
-                _p(grid_grad), self._stream()), "svr_grid_laplacian_bw_table_order")
+                _p(grid_grad), self._stream()), "svr_grid_smoothness_bw_table_order")
         else:
-            _lib.check(self.lib.svr_grid_smoothness_bw_step(
-                1, _p(grid_pts), grid_pts.numel(), _p(self.vox_key), self.N, _p(self.coord), _p(self.size), self.res,
-                _p(self.mask), _p(self.keys), _p(self.g2v), self.G, _p(self.dense), _p(self.active), self.G,
-                ctypes.c_float(self.weight), ctypes.c_float(self.vox_size_inv), ctypes.c_uint(self.seed), _p(step),
-                _p(grid_grad), self._stream()), "svr_grid_smoothness_bw_step")
+            terms = ((1, self.weight),) + (((0, self.eikonal_weight),) if self.eikonal else ())
+            for lap, wt in terms:
+                _lib.check(self.lib.svr_grid_smoothness_bw_step(
+                    lap, _p(grid_pts), grid_pts.numel(), _p(self.vox_key), self.N, _p(self.coord), _p(self.size),
+                    self.res, _p(self.mask), _p(self.keys), _p(self.g2v), self.G, _p(self.dense), _p(self.active),
+                    self.G, ctypes.c_float(wt), ctypes.c_float(self.vox_size_inv), ctypes.c_uint(self.seed),
+                    _p(step), _p(grid_grad), self._stream()), "svr_grid_smoothness_bw_step")
 
 
 class BatchedTrainIteration:

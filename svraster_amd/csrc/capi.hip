@@ -546,6 +546,31 @@ int svr_grid_smoothness_bw_step(int laplacian, const float* grid_pts, long long 
                           grid_pts_grad, stream, dense_index, step);
 }
 
+static int grid_table_order(const float* grid_pts, const int64_t* vox_key, int N, const float* grid_voxel_coord,
+                            const float* grid_voxel_size, int grid_res, const uint8_t* grid_mask,
+                            const int32_t* table_keys, const int32_t* table_tvox, int G, const int32_t* cell_entry,
+                            const int32_t* cell_tvox, const int32_t* tvox, int T, int terms, float weight,
+                            const float* weights, float vox_size_inv, unsigned int seed, const int32_t* step,
+                            float* rec_ws, float* coef_ws, float* grid_pts_grad, void* stream) {
+  if (!step || !table_keys || !table_tvox || !cell_entry || !cell_tvox || !tvox || !rec_ws || !coef_ws)
+    return fail(1, "missing table-order argument%s (all pointers are required)", "", 0);
+  if (grid_res <= 2 || (long long)grid_res * grid_res * grid_res >= (1LL << 31))
+    return fail(1, "grid_res%s %lld out of range", "", grid_res);
+  if (T < 0 || T > N) return fail(1, "table voxel count%s %lld out of range", "", T);
+  if (terms < 1 || terms > 3) return fail(1, "terms must be 1 (Laplacian), 2 (eikonal) or 3 (both)%s, got %lld", "",
+                                          terms);
+  if ((terms & 2) && !weights) return fail(1, "the eikonal term reads its weight from `weights`%s", "", 0);
+  GridOrderArgs A;
+  A.grid_pts = grid_pts; A.vox_key = vox_key; A.coord = grid_voxel_coord; A.size = grid_voxel_size;
+  A.mask = reinterpret_cast<const bool*>(grid_mask);
+  A.keys = table_keys; A.tv = table_tvox; A.cell_entry = cell_entry; A.cell_tv = cell_tvox; A.tvox = tvox;
+  A.rec = rec_ws; A.coef = coef_ws; A.grad = grid_pts_grad;
+  A.G = G; A.T = T; A.res = grid_res; A.weight = weight; A.inv = vox_size_inv; A.seed = seed; A.step = step;
+  A.terms = terms; A.weights = weights;
+  SVR_TRY(launch_grid_laplacian_pull(A, as_stream(stream)));
+  return 0;
+}
+
 int svr_grid_laplacian_bw_table_order(const float* grid_pts, long long n_pts, const int64_t* vox_key, int N,
                                       const float* grid_voxel_coord, const float* grid_voxel_size, int grid_res,
                                       const uint8_t* grid_mask, const int32_t* table_keys,
@@ -554,20 +579,26 @@ int svr_grid_laplacian_bw_table_order(const float* grid_pts, long long n_pts, co
                                       float vox_size_inv, unsigned int seed, const int32_t* step, float* rec_ws,
                                       float* coef_ws, float* grid_pts_grad, void* stream) {
   g_err.clear();
-  if (!step || !table_keys || !table_tvox || !cell_entry || !cell_tvox || !tvox || !rec_ws || !coef_ws)
-    return fail(1, "missing table-order argument%s (all pointers are required)", "", 0);
-  if (grid_res <= 2 || (long long)grid_res * grid_res * grid_res >= (1LL << 31))
-    return fail(1, "grid_res%s %lld out of range", "", grid_res);
-  if (T < 0 || T > N) return fail(1, "table voxel count%s %lld out of range", "", T);
-  GridOrderArgs A;
-  A.grid_pts = grid_pts; A.vox_key = vox_key; A.coord = grid_voxel_coord; A.size = grid_voxel_size;
-  A.mask = reinterpret_cast<const bool*>(grid_mask);
-  A.keys = table_keys; A.tv = table_tvox; A.cell_entry = cell_entry; A.cell_tv = cell_tvox; A.tvox = tvox;
-  A.rec = rec_ws; A.coef = coef_ws; A.grad = grid_pts_grad;
-  A.G = G; A.T = T; A.res = grid_res; A.weight = weight; A.inv = vox_size_inv; A.seed = seed; A.step = step;
   (void)n_pts;
-  SVR_TRY(launch_grid_laplacian_pull(A, as_stream(stream)));
-  return 0;
+  return grid_table_order(grid_pts, vox_key, N, grid_voxel_coord, grid_voxel_size, grid_res, grid_mask, table_keys,
+                          table_tvox, G, cell_entry, cell_tvox, tvox, T, 1, weight, nullptr, vox_size_inv, seed,
+                          step, rec_ws, coef_ws, grid_pts_grad, stream);
+}
+
+int svr_grid_smoothness_bw_table_order(const float* grid_pts, long long n_pts, const int64_t* vox_key, int N,
+                                       const float* grid_voxel_coord, const float* grid_voxel_size, int grid_res,
+                                       const uint8_t* grid_mask, const int32_t* table_keys,
+                                       const int32_t* table_tvox, int G, const int32_t* cell_entry,
+                                       const int32_t* cell_tvox, const int32_t* tvox, int T, int terms,
+                                       const float* weights, float vox_size_inv, unsigned int seed,
+                                       const int32_t* step, float* rec_ws, float* coef_ws, float* grid_pts_grad,
+                                       void* stream) {
+  g_err.clear();
+  (void)n_pts;
+  if (!weights) return fail(1, "weights (device float[2]: eikonal, Laplacian) are required%s", "", 0);
+  return grid_table_order(grid_pts, vox_key, N, grid_voxel_coord, grid_voxel_size, grid_res, grid_mask, table_keys,
+                          table_tvox, G, cell_entry, cell_tvox, tvox, T, terms, 0.f, weights, vox_size_inv, seed,
+                          step, rec_ws, coef_ws, grid_pts_grad, stream);
 }
 
 // Workspace of svr_build_grid_pts_link for N voxels: a binning layout for the 8N corner codes,

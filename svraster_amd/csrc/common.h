@@ -264,28 +264,26 @@ __device__ inline uint32_t wave_incl_max_u32(uint32_t x) {
 // (rows 1 and 3) and lane 31 (rows 2 and 3) for segments that started in an earlier row.
 template <int N>
 __device__ inline void wave_segmented_sum(float (&v)[N], int lane, int seg_start) {
+  // Each step adds the shifted partial sum times 1 or 0 (whether the source lane is in the
+  // lane's segment): x = fma(dpp(x), c, x), one VOP2 fmac with the DPP shift folded into its
+  // source operand, instead of a DPP move, a select and an add.  Sums of finite values equal
+  // those of the select form (adding +0 leaves a non-zero sum unchanged).
   const int lr = lane & 15;
-  const bool c1 = lr >= 1 && lane - 1 >= seg_start;
-  const bool c2 = lr >= 2 && lane - 2 >= seg_start;
-  const bool c4 = lr >= 4 && lane - 4 >= seg_start;
-  const bool c8 = lr >= 8 && lane - 8 >= seg_start;
-  const bool cr = ((lane >> 4) & 1) && seg_start < (lane & ~15);
-  const bool cw = lane >= 32 && seg_start < 32;
+  const float c1 = (lr >= 1 && lane - 1 >= seg_start) ? 1.f : 0.f;
+  const float c2 = (lr >= 2 && lane - 2 >= seg_start) ? 1.f : 0.f;
+  const float c4 = (lr >= 4 && lane - 4 >= seg_start) ? 1.f : 0.f;
+  const float c8 = (lr >= 8 && lane - 8 >= seg_start) ? 1.f : 0.f;
+  const float cr = (((lane >> 4) & 1) && seg_start < (lane & ~15)) ? 1.f : 0.f;
+  const float cw = (lane >= 32 && seg_start < 32) ? 1.f : 0.f;
 #pragma unroll
   for (int q = 0; q < N; ++q) {
     float x = v[q];
-    float t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xF, 0xF, true));
-    x += c1 ? t : 0.f;
-    t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x112, 0xF, 0xF, true));
-    x += c2 ? t : 0.f;
-    t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xF, 0xF, true));
-    x += c4 ? t : 0.f;
-    t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x118, 0xF, 0xF, true));
-    x += c8 ? t : 0.f;
-    t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x142, 0xA, 0xF, false));
-    x += cr ? t : 0.f;
-    t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x143, 0xC, 0xF, false));
-    x += cw ? t : 0.f;
+    x = fmaf(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xF, 0xF, true)), c1, x);
+    x = fmaf(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x112, 0xF, 0xF, true)), c2, x);
+    x = fmaf(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xF, 0xF, true)), c4, x);
+    x = fmaf(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x118, 0xF, 0xF, true)), c8, x);
+    x = fmaf(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x142, 0xA, 0xF, false)), cr, x);
+    x = fmaf(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x143, 0xC, 0xF, false)), cw, x);
     v[q] = x;
   }
 }

@@ -174,8 +174,8 @@ __global__ __launch_bounds__(256) void grid_laplacian_kernel(GridTable G, const 
   grid_scatter(G, vox[6], w[6], lz, grad);
 }
 
-// Two-pass ("pull") form of the same loss with every table entry active once, taken in table
-// order (voxel by voxel; entry e of a voxel with cell box lo..lo+s is cell
+// Two-pass ("pull") form of the same loss -- and of the eikonal loss, alone or with it -- with
+// every table entry active once, taken in table order (voxel by voxel; entry e of a voxel with cell box lo..lo+s is cell
 // lo + (i % s, i / s % s, i / s^2), i = e - first entry of the voxel, x fastest).  It needs a
 // table whose voxels' entries are exactly their cell boxes in that order with power-of-two
 // sizes, whose cells are unique, and whose mask is its own (true exactly on its cells).  The
@@ -265,10 +265,20 @@ struct TableOrder {
   int G, T, res, lg;         // lg = log2(res): the table's resolution is a power of two
 };
 
-__global__ __launch_bounds__(256) void grid_laplacian_coef_kernel(TableOrder O, const bool* __restrict__ mask,
-                                                                  float weight, float inv, uint32_t seed,
-                                                                  const int32_t* __restrict__ step,
-                                                                  float* __restrict__ coef) {
+// Pass 1 for the enabled terms.  Both terms sample the six face neighbours of the key at the
+// key's random offset (the Laplacian also the key's own cell), so one set of samples serves both.
+// Laplacian only: three coefficients per entry (lx, ly, lz), coef[k * G + e].  With the eikonal:
+// seven, the coefficient of each sample of the entry -- centre -2 (lx + ly + lz), then +x, -x,
+// +y, -y, +z, -z: lx +- ex, ly +- ey, lz +- ez, where e = 2 (|g| / (2 h) - 1) w_ge g / |g| / (2 h)
+// and g the central differences of the six samples (ge_compute.cu:76-114).  Term weights come from
+// wts[0] (eikonal) and wts[1] (Laplacian) in device memory when wts is given, so a captured
+// graph follows a weight schedule without being captured again.
+template <bool kLap, bool kEik>
+__global__ __launch_bounds__(256) void grid_smooth_coef_kernel(TableOrder O, const bool* __restrict__ mask,
+                                                               const float* __restrict__ wts, float w_lap,
+                                                               float inv, uint32_t seed,
+                                                               const int32_t* __restrict__ step,
+                                                               float* __restrict__ coef) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = O.G, res = O.res;
   if (e >= n) return;
@@ -311,12 +321,14 @@ __global__ __launch_bounds__(256) void grid_laplacian_coef_kernel(TableOrder O, 
     qg0[i] = O.rec[3LL * tt + 1];
     qg1[i] = O.rec[3LL * tt + 2];
   }
-  float lx = 0.f, ly = 0.f, lz = 0.f;
+  const float wl = wts ? wts[1] : w_lap;
+  const float we = kEik ? (wts ? wts[0] : 0.f) : 0.f;
+  float lx = 0.f, ly = 0.f, lz = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
   if (valid) {
     float rx, ry, rz;
     random_offset(nullptr, step_seed(seed, step), e, rx, ry, rz);
     float sd[7];
-    sd[0] = record_sample(b, g0, g1, x, y, z, rx, ry, rz);
+    sd[0] = kLap ? record_sample(b, g0, g1, x, y, z, rx, ry, rz) : 0.f;
 #pragma unroll
     for (int i = 1; i < 7; ++i) {
       // an in-box neighbour has tq[i] == ts, so its record is the key's own
@@ -324,20 +336,42 @@ __global__ __launch_bounds__(256) void grid_laplacian_coef_kernel(TableOrder O, 
       const float v = record_sample(record_box(qr[i]), qg0[i], qg1[i], qx, qy, qz, rx, ry, rz);
       sd[i] = (unsigned)tq[i] < (unsigned)O.T ? v : 0.f;
     }
-    const float r0s = sd[1] + sd[2] - 2.f * sd[0];
-    const float r1s = sd[3] + sd[4] - 2.f * sd[0];
-    const float r2s = sd[5] + sd[6] - 2.f * sd[0];
-    const float lr = sqrtf(r0s * r0s + r1s * r1s + r2s * r2s + 1e-10f);
-    const float sz = weight * 0.5f * inv * inv;
-    lx = r0s / lr * sz;
-    ly = r1s / lr * sz;
-    lz = r2s / lr * sz;
+    if (kLap) {
+      const float r0s = sd[1] + sd[2] - 2.f * sd[0];
+      const float r1s = sd[3] + sd[4] - 2.f * sd[0];
+      const float r2s = sd[5] + sd[6] - 2.f * sd[0];
+      const float lr = sqrtf(r0s * r0s + r1s * r1s + r2s * r2s + 1e-10f);
+      const float sz = wl * 0.5f * inv * inv;
+      lx = r0s / lr * sz;
+      ly = r1s / lr * sz;
+      lz = r2s / lr * sz;
+    }
+    if (kEik) {
+      const float gx = (sd[1] - sd[2]) * 0.5f, gy = (sd[3] - sd[4]) * 0.5f, gz = (sd[5] - sd[6]) * 0.5f;
+      const float gn = sqrtf(gx * gx + gy * gy + gz * gz + 1e-6f);
+      const float gw = gn * 0.5f * inv;
+      const float dLdg = 2.f * (gw - 1.f) * we;
+      ex = dLdg * gx / gn * 0.5f * inv;
+      ey = dLdg * gy / gn * 0.5f * inv;
+      ez = dLdg * gz / gn * 0.5f * inv;
+    }
   }
-  coef[e] = lx;
-  coef[(long long)n + e] = ly;
-  coef[2LL * n + e] = lz;
+  if (!kEik) {
+    coef[e] = lx;
+    coef[(long long)n + e] = ly;
+    coef[2LL * n + e] = lz;
+  } else {
+    coef[e] = -2.f * (lx + ly + lz);
+    coef[(long long)n + e] = lx + ex;
+    coef[2LL * n + e] = lx - ex;
+    coef[3LL * n + e] = ly + ey;
+    coef[4LL * n + e] = ly - ey;
+    coef[5LL * n + e] = lz + ez;
+    coef[6LL * n + e] = lz - ez;
+  }
 }
 
+template <bool kEik>
 __global__ __launch_bounds__(256) void grid_laplacian_pull_kernel(TableOrder O, const int64_t* __restrict__ vox_key,
                                                                   uint32_t seed, const int32_t* __restrict__ step,
                                                                   const float* __restrict__ coef,
@@ -350,7 +384,7 @@ __global__ __launch_bounds__(256) void grid_laplacian_pull_kernel(TableOrder O, 
   const int es = live ? e : 0;
   const int c = O.keys[es];
   const int t0 = O.tv[es];
-  const int t = live & ((unsigned)t0 < (unsigned)O.T) ? t0 : -1;
+  const int t = (live & ((unsigned)t0 < (unsigned)O.T)) ? t0 : -1;
   const float4 r0 = O.rec[3LL * (t >= 0 ? t : 0)];
   const VoxelBox b = record_box(r0);
   const int x = c & (res - 1), y = (c >> O.lg) & (res - 1), z = c >> (2 * O.lg);
@@ -378,7 +412,12 @@ __global__ __launch_bounds__(256) void grid_laplacian_pull_kernel(TableOrder O, 
     if (!use[i]) ek[i] = es;
   }
   float l[7];
-  {
+  if (kEik) {
+    // per-sample coefficients: the key at ek[i] contributes its sample toward this cell, which
+    // is its sample i (centre, +x seen from c - x, -x seen from c + x, ...)
+#pragma unroll
+    for (int i = 0; i < 7; ++i) l[i] = coef[i * n3 + ek[i]];
+  } else {
     const float cx0 = coef[ek[0]], cy0 = coef[n3 + ek[0]], cz0 = coef[2 * n3 + ek[0]];
     l[0] = -2.f * (cx0 + cy0 + cz0);
     l[1] = coef[ek[1]];
@@ -510,8 +549,19 @@ hipError_t launch_grid_laplacian_pull(const GridOrderArgs& A, hipStream_t st) {
   O.rec = reinterpret_cast<const float4*>(A.rec); O.G = A.G; O.T = A.T; O.res = A.res;
   O.lg = __builtin_ctz((unsigned)A.res);
   const unsigned nb = (unsigned)((A.G + 255) / 256);
-  grid_laplacian_coef_kernel<<<nb, 256, 0, st>>>(O, A.mask, A.weight, A.inv, A.seed, A.step, A.coef);
-  grid_laplacian_pull_kernel<<<nb, 256, 0, st>>>(O, A.vox_key, A.seed, A.step, A.coef, A.grad);
+  const bool lap = (A.terms & 1) != 0, eik = (A.terms & 2) != 0;
+  if (lap && eik) {
+    grid_smooth_coef_kernel<true, true><<<nb, 256, 0, st>>>(O, A.mask, A.weights, A.weight, A.inv, A.seed, A.step,
+                                                            A.coef);
+  } else if (eik) {
+    grid_smooth_coef_kernel<false, true><<<nb, 256, 0, st>>>(O, A.mask, A.weights, A.weight, A.inv, A.seed, A.step,
+                                                             A.coef);
+  } else {
+    grid_smooth_coef_kernel<true, false><<<nb, 256, 0, st>>>(O, A.mask, A.weights, A.weight, A.inv, A.seed, A.step,
+                                                             A.coef);
+  }
+  if (eik) grid_laplacian_pull_kernel<true><<<nb, 256, 0, st>>>(O, A.vox_key, A.seed, A.step, A.coef, A.grad);
+  else grid_laplacian_pull_kernel<false><<<nb, 256, 0, st>>>(O, A.vox_key, A.seed, A.step, A.coef, A.grad);
   return hipGetLastError();
 }
 

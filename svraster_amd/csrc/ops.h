@@ -66,12 +66,14 @@ struct GridOrderArgs {
   const int32_t* cell_tv;     // [res^3] compact table voxel of each cell, -1 if none
   const int32_t* tvox;        // [T] voxel id of each compact table voxel
   float* rec;                 // [T * 12] per-step records (scratch)
-  float* coef;                // [3 * G] per-step coefficients (scratch)
+  float* coef;                // [3 * G], with the eikonal [7 * G], per-step coefficients (scratch)
   float* grad;
   int G, T, res;
-  float weight, inv;
+  float weight, inv;          // Laplacian weight when `weights` is null; 1 / cell size
   uint32_t seed;
   const int32_t* step;
+  int terms = 1;              // bit 0: Laplacian, bit 1: eikonal
+  const float* weights = nullptr;  // device [2]: eikonal, Laplacian weights (overrides `weight`)
 };
 hipError_t launch_grid_laplacian_pull(const GridOrderArgs& A, hipStream_t st);
 hipError_t launch_total_variation(long long N, int C, const float* grid_pts, const int64_t* vox_key, float weight,

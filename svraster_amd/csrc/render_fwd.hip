@@ -91,7 +91,9 @@ __device__ inline void fetch_entry(EntryRegs& e, uint32_t val, int view, long lo
   e.r = rgbs[3 * vox + 0]; e.g = rgbs[3 * vox + 1]; e.b = rgbs[3 * vox + 2];
 }
 
-// LDS copy of a fetched entry; an entry that cannot reach the minimum alpha gets an empty bbox.
+// LDS copy of a fetched entry.  With kCull, an entry that cannot reach the minimum alpha gets an
+// empty bbox (no pixel masks); the multi-view path has left such voxels out of binning already.
+template <bool kCull>
 __device__ inline void store_entry(const EntryRegs& e, bool need_normal, float s, float cam_scale, float4* s_ch,
                                    float4* s_ml, float4* s_g0, float4* s_g1, float4* s_rgb, float4* s_nrm, uint2* s_bb,
                                    int slot) {
@@ -101,7 +103,7 @@ __device__ inline void store_entry(const EntryRegs& e, bool need_normal, float s
   s_g1[slot] = e.g1;
   const float g[8] = {e.g0.x, e.g0.y, e.g0.z, e.g0.w, e.g1.x, e.g1.y, e.g1.z, e.g1.w};
   const float ps = cam_scale + fmaxf(fabsf(e.cx), fmaxf(fabsf(e.cy), fabsf(e.cz)));
-  s_bb[slot] = voxel_may_contribute(g, s, ps, e.l) ? e.bb : make_uint2(0xFFFFFFFFu, 0u);
+  s_bb[slot] = (!kCull || voxel_may_contribute(g, s, ps, e.l)) ? e.bb : make_uint2(0xFFFFFFFFu, 0u);
   s_rgb[slot] = make_float4(e.r, e.g, e.b, __uint_as_float(e.val));
   if (need_normal) {
     float nx, ny, nz, rn;
@@ -457,14 +459,17 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
       }
       cc += n0 + (uint32_t)__popcll(u1);
       if (!fin && t < kStage && base + t < n)
-        store_entry(er, kNormal, s, cam_scale, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t);
+        store_entry<!kQueue>(er, kNormal, s, cam_scale, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t);
       // The backward takes the pixels by quadrant: wave w transposes quadrant w's pixel sets into
       // per-entry masks.
-      {
+      // (A half whose entries no pixel composited has nothing to hand over: no transpose.)
+      if (u0 != 0ull) {
         const uint64_t q_lo = transpose64(s_pix[w][0][lane], lane);
-        const uint64_t q_hi = kHalves > 1 ? transpose64(s_pix[w][kHalves - 1][lane], lane) : 0ull;
         if ((u0 >> lane) & 1ull) a.pmask[4 * (o + __popcll(u0 & lt_mask)) + w] = q_lo;
-        if (kHalves > 1 && ((u1 >> lane) & 1ull)) a.pmask[4 * (o + n0 + __popcll(u1 & lt_mask)) + w] = q_hi;
+      }
+      if (kHalves > 1 && u1 != 0ull) {
+        const uint64_t q_hi = transpose64(s_pix[w][kHalves - 1][lane], lane);
+        if ((u1 >> lane) & 1ull) a.pmask[4 * (o + n0 + __popcll(u1 & lt_mask)) + w] = q_hi;
       }
       __syncthreads();
       if (fin) break;

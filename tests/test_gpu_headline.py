@@ -7,8 +7,8 @@ sh0 and shs gradients match to relative L2 <= 1e-3 and dL/ds to relative 1e-3; t
 the four views (binning without the activity cull) equal the oracle's sorted lists bit for bit.
 
 train_16: 16M leaves (+ non-leaf parents kept from level 9), 16 views at 1920x1080.  One captured
-training iteration (render, MSE, backward, Laplacian smoothness, dense Adam) is replayed once: its
-colour on a crop of view 0 matches the oracle rendering of the parameters it started from
+training iteration (render, MSE, backward, grid eikonal + Laplacian smoothness, dense Adam) is
+replayed as iteration 1 from the scene as built: its colour on a crop of view 0 matches the oracle
 (<= 1e-4), and its corner-SDF update equals the host-driven Adam step on the replay's gradient.
 
 Each test builds its scene (tens of seconds of host time) and runs the oracle on crops; they are
@@ -133,23 +133,30 @@ def test_train_16_captured_iteration_vs_oracle():
     g = torch.Generator().manual_seed(1)
     targets = [torch.rand(3, c.image_height, c.image_width, generator=g).cuda() for c in cams]
     smooth = GridSmoothness(sc.vox_center, sc.vox_size, sc.octlevel, sc.is_leaf, sc.vox_key, sc.scene_center,
-                            sc.scene_extent, weight=1e-3, seed=0)
+                            sc.scene_extent, weight=1e-3, seed=0, eikonal_weight=1e-8)
     r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key, cams,
                           m.bg_color, sc.sh_degree)
     grid = m._geo_grid_pts.detach().clone()
     sh0, shs = m._sh0.detach().clone(), m._shs.detach().clone()
+    g_start, s0, ss0 = grid.clone(), sh0.clone(), shs.clone()
     lr, betas, eps = 1e-2, (0.9, 0.99), 1e-15
     it = BatchedTrainIteration(r, grid, sh0, shs, m._log_s.detach(), targets, smooth, lr=lr, betas=betas, eps=eps)
     it.capture(warmup=1)
+    # The capture's eager warm-up iteration stepped the parameters: put back the initial state
+    # (parameters, moments, step counter) so that the replay is iteration 1 from the scene as built.
+    with torch.no_grad():
+        grid.copy_(g_start); sh0.copy_(s0); shs.copy_(ss0)
+        for t_ in (it.grid_m, it.grid_v, it.sh_adam.m0, it.sh_adam.v0, it.sh_adam.ms, it.sh_adam.vs, it.step):
+            t_.zero_()
     torch.cuda.synchronize()
     t = int(it.step.item()) + 1
     g0, gm, gv = grid.clone(), it.grid_m.clone(), it.grid_v.clone()
-    g_start, s0, ss0 = grid.clone(), sh0.clone(), shs.clone()
     it()
     torch.cuda.synchronize()
     it.check_capacity()
-    assert int(it.step.item()) == t
-    say(f"  {sc.num_voxels} voxels, {r.key_count()} keys, grid table {smooth.G} keys; replayed step {t}")
+    assert int(it.step.item()) == t == 1
+    say(f"  {sc.num_voxels} voxels, {r.key_count()} keys, grid table {smooth.G} keys (eikonal + Laplacian); "
+        f"replayed step {t}")
     # corner-SDF Adam: the host-driven dense step on the replay's gradient
     _C.unbiased_adam_step(False, g0, it.grid_grad, gm, gv, t, lr, betas[0], betas[1], eps)
     assert torch.equal(g0, grid) and torch.equal(gm, it.grid_m) and torch.equal(gv, it.grid_v)

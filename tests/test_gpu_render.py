@@ -7,8 +7,9 @@ within FP32 rounding of the threshold may move it); gradients relative L2 <= 1e-
 Median depth (channel 2) is a piecewise-constant function of T and alpha: it jumps by one
 refinement step (a sixteenth of the voxel's chord) or to another voxel when a comparison with
 T = 0.5 or alpha = 1e-5 flips.  It is gated at max-abs <= 1e-4 on every pixel whose decisions
-the oracle finds clear of their thresholds (|T - 0.5| >= 1e-4 and |alpha - 1e-5| >= 1e-6), and
-those pixels must be >= 99 % of the frame."""
+the oracle finds clear of their thresholds (|T - 0.5| >= 1e-4 and |alpha - 1e-5| >= 1e-6);
+pixels off by more than 1e-4 (all of them with a decision near its threshold) must be <= 0.1 %
+of the frame."""
 import numpy as np
 import pytest
 import torch
@@ -50,12 +51,15 @@ def _compare_forward(name, window=None, s_mult=1.0):
     dmed_err = (out["depth"][2, y0:y1, x0:x1].cpu().double() - ref.depth[2]).abs()
     err_m = dmed_err[clear].max().item() if bool(clear.any()) else 0.0
     frac_clear = clear.double().mean().item()
+    off = dmed_err > 1e-4
     say(f"  color {err_c:.2e} T {err_t:.2e} depth {err_d:.2e} normal {err_n:.2e} "
-        f"n_contrib-match {same:.5f} median depth {err_m:.2e} on {frac_clear:.5f} of pixels (clear decisions), "
-        f"{dmed_err.max().item():.2e} overall")
+        f"n_contrib-match {same:.5f} median depth: {err_m:.2e} on the {frac_clear:.5f} of pixels with clear "
+        f"decisions, {int(off.sum())} pixels off by more than 1e-4 (max {dmed_err.max().item():.2e})")
     assert err_c <= 1e-4 and err_t <= 1e-4 and err_d <= 1e-4 and err_n <= 1e-4
     assert same >= 0.999
-    assert err_m <= 1e-4 and frac_clear >= 0.99
+    # every pixel off by more than 1e-4 had a decision within FP32 reach of its threshold, and
+    # such flips are rare
+    assert err_m <= 1e-4 and off.double().mean().item() <= 1e-3
     if window is None:
         mw = out["max_w"].reshape(-1).cpu().double()
         assert (mw - ref.max_w).abs().max().item() <= 1e-4
@@ -102,11 +106,15 @@ def _compare_backward(name, window=None, concen=0.0, s_mult=1.0):
         m = torch.zeros(1, H, W)
         m[:, y0:y1, x0:x1] = 1
         dC, dD, dN, dT = dC * m, dD * m, dN * m, dT * m
-    dgeo, drgb, dsub, ds = _C.rasterize_voxels_backward(
-        out["K"], 3, 1, W, H, cam.tanfovx, cam.tanfovy, cam.cx, cam.cy, cam.w2c, cam.c2w, bg,
-        sc.octpath, sc.vox_center, sc.vox_size, geos32, rgbs32, out["geom"], out["binning"], out["img"],
-        out["T"], dC.cuda(), dD.cuda(), dN.cuda(), dT.cuda(), concen, gt.cuda(), 0.0, 0.0, True, True,
-        out["depth"], out["normal"], False, s32, torch.empty(0))
+    def hip_bwd(dC, dD, dN, dT):
+        return _C.rasterize_voxels_backward(
+            out["K"], 3, 1, W, H, cam.tanfovx, cam.tanfovy, cam.cx, cam.cy, cam.w2c, cam.c2w, bg,
+            sc.octpath, sc.vox_center, sc.vox_size, geos32, rgbs32, out["geom"], out["binning"], out["img"],
+            out["T"], dC.cuda(), dD.cuda(), dN.cuda(), dT.cuda(), concen, gt.cuda(), 0.0, 0.0, True, True,
+            out["depth"], out["normal"], False, s32, torch.empty(0))
+
+    dgeo, drgb, dsub, ds = hip_bwd(dC, dD, dN, dT)
+    ds_c = hip_bwd(dC, torch.zeros_like(dD), torch.zeros_like(dN), torch.zeros_like(dT))[3]
     bn = oracle_binning(sc, cam)
     geos64 = geos32.cpu().double().requires_grad_(True)
     r64 = rgbs32.cpu().double().requires_grad_(True)
@@ -115,19 +123,29 @@ def _compare_backward(name, window=None, concen=0.0, s_mult=1.0):
                         lambda_R_concen=concen, gt_color=gt)
     x0, y0, x1, y1 = ref.window
     sl = (slice(None), slice(y0, y1), slice(x0, x1))
+    gsd_c = orc.backward(ref, geos64, r64, s64, dC[sl], retain_graph=True)[3]
     gg, gr, gs, gsd, ds_abs = orc.backward(ref, geos64, r64, s64, dC[sl], dD[sl], dN[sl], dT[sl],
                                            return_ds_abs=True)
     e_geo = rel_l2(dgeo, gg)
     e_rgb = rel_l2(drgb, gr)
     e_sub = rel_l2(dsub.reshape(-1), gs)
-    # dL/ds is one scalar summed over every (pixel, voxel) pair with mixed signs: its FP32 error
-    # is bounded relative to the sum of the magnitudes of its terms, not to the cancelled total.
+    # dL/ds is one scalar summed over every (pixel, voxel) pair, with terms of both signs.  With
+    # the colour cotangent alone (the training loss) it is gated at 1e-3 relative.  With every
+    # cotangent at once, random colour and depth cotangents can cancel: on the single_1237 crop
+    # their parts are +1.434e-3 and -1.437e-3 and the total -5.0e-5, and the FP32 restatement of
+    # the oracle itself (torch float32 on the CPU) then misses the FP64 total by 7.2e-3 relative
+    # (2.4e-5 for the colour part alone).  So the mixed total is gated at 1e-3 of its own value
+    # when the sum is well conditioned (sum|terms| <= 100 |dL/ds|), and otherwise at 1e-5 of
+    # sum|terms| -- about the FP32 rounding of the terms themselves.
+    e_s_c = abs(ds_c.item() - gsd_c.item()) / max(abs(gsd_c.item()), 1e-30)
     e_s = abs(ds.item() - gsd.item()) / max(abs(gsd.item()), 1e-30)
+    kappa = ds_abs / max(abs(gsd.item()), 1e-30)
     e_s_scaled = abs(ds.item() - gsd.item()) / max(ds_abs, 1e-30)
-    say(f"  rel-L2 geo {e_geo:.2e} rgb {e_rgb:.2e} subdiv {e_sub:.2e} ds {e_s:.2e} "
-        f"(|ds err| / sum|terms| {e_s_scaled:.2e}, sum|terms|/|ds| {ds_abs / max(abs(gsd.item()), 1e-30):.1f})")
+    say(f"  rel-L2 geo {e_geo:.2e} rgb {e_rgb:.2e} subdiv {e_sub:.2e}; dL/ds colour cotangent {e_s_c:.2e}, "
+        f"all cotangents {e_s:.2e} (|err| / sum|terms| {e_s_scaled:.2e}, sum|terms|/|ds| {kappa:.1f})")
     assert e_geo <= 1e-3 and e_rgb <= 1e-3 and e_sub <= 1e-3
-    assert e_s <= 1e-3
+    assert e_s_c <= 1e-3
+    assert e_s <= 1e-3 if kappa <= 100.0 else e_s_scaled <= 1e-5
     return dgeo, drgb, ds
 
 

@@ -147,6 +147,52 @@ def test_grid_smoothness_step_matches_api():
 
 
 @pytest.mark.gpu
+def test_grid_eikonal_and_laplacian_table_order_match_api():
+    say("[grid] table-order eikonal + Laplacian (one launch pair) vs grid_loss_bw.grid_eikonal + laplacian_smoothness")
+    w, sc, cams, targets = _setup()
+    w_ge, w_ls = 2e-3, 1e-3
+    sm = GridSmoothness(sc.vox_center, sc.vox_size, sc.octlevel, sc.is_leaf, sc.vox_key, sc.scene_center, 2.0,
+                        weight=w_ls, seed=11, eikonal_weight=w_ge)
+    assert sm.pull and sm.eikonal
+    step = torch.tensor([5], dtype=torch.int32, device="cuda")
+    got = torch.zeros_like(sc.geo_grid_pts)
+    sm(sc.geo_grid_pts, got, step)
+    seed = (11 + 0x9E3779B9 * 5) & 0xFFFFFFFF
+    kw = dict(grid_pts=sc.geo_grid_pts, vox_key=sc.vox_key, grid_voxel_coord=sm.coord, grid_voxel_size=sm.size,
+              grid_res=sm.res, grid_mask=sm.mask, grid_keys=sm.keys, grid2voxel=sm.g2v, active_list=sm.active,
+              vox_size_inv=sm.vox_size_inv, no_tv_s=True, tv_sparse=False, seed=seed)
+    ref = torch.zeros_like(sc.geo_grid_pts)
+    svraster_cuda.grid_loss_bw.laplacian_smoothness(weight=w_ls, grid_pts_grad=ref, **kw)
+    svraster_cuda.grid_loss_bw.grid_eikonal(weight=w_ge, grid_pts_grad=ref, **kw)
+    eik = torch.zeros_like(sc.geo_grid_pts)
+    svraster_cuda.grid_loss_bw.grid_eikonal(weight=w_ge, grid_pts_grad=eik, **kw)
+    torch.cuda.synchronize()
+    e = rel_l2(got, ref)
+    say(f"  rel-L2 {e:.2e}; |eikonal part| {eik.norm().item():.3e}, |total| {ref.norm().item():.3e}")
+    assert eik.norm().item() > 0 and e <= 1e-5
+    # eikonal alone through the weights in device memory (Laplacian weight 0), and back
+    sm.set_weights(laplacian=0.0)
+    got2 = torch.zeros_like(sc.geo_grid_pts)
+    sm(sc.geo_grid_pts, got2, step)
+    torch.cuda.synchronize()
+    assert rel_l2(got2, eik) <= 1e-5
+    with pytest.raises(ValueError):
+        GridSmoothness(sc.vox_center, sc.vox_size, sc.octlevel, sc.is_leaf, sc.vox_key, sc.scene_center, 2.0,
+                       weight=w_ls).set_weights(eikonal=1.0)
+
+
+def test_trainer_smoothness_weights_schedule():
+    from svraster_amd.batched import trainer_smoothness_weights as tw
+    assert tw(0) == (1e-8, 1e-10)
+    assert tw(2500) == (1e-8 * 0.25, 1e-10 * 0.25)
+    assert tw(5999) == (1e-8 * 0.0625, 1e-10 * 0.0625)
+    assert tw(7000) == (0.0, 1e-10 * 0.0625)
+    assert tw(9000) == (0.0, 0.0)
+    ge, ls = tw(100, G=1000, ge_drop_ratio=0.5)
+    assert ge == 1e-8 * 2.0 and ls == 1e-10
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("sparse", [False, True])
 def test_train_iteration_graph_replays(sparse):
     say(f"[train] captured iteration: each replay = drop-in render + smoothness + Adam on the same inputs, "
