@@ -253,6 +253,14 @@ int svr_build_grid_pts_link(const int64_t* octpath, const int8_t* octlevel, long
  * names get 0.  count_ws: M uint32 of scratch. */
 int svr_agg_voxel_into_grid_pts(const int64_t* vox_key, long long N, const float* vox_val, int C, long long M,
                                 uint32_t* count_ws, float* out, void* stream);
+/* svr_agg_voxel_into_grid_pts without a workspace: the per-point counts live in a fixed device
+ * array of the library (16 MiB per GPU, held until the process ends), so the call allocates
+ * nothing.  No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  M must be
+ * at most 4194304 grid points (larger M fails with a message).  Calls that overlap on one GPU
+ * share the array and race; the counts are cleared inside the call. */
+int svr_agg_voxel_into_grid_pts_noalloc(const int64_t* vox_key, long long N, const float* vox_val, int C,
+                                        long long M, float* out, void* stream);
 
 /* Replace UTILS::ijk_2_octpath / octpath_2_ijk (src/utils.cu:129-189, 329-361). */
 int svr_ijk_2_octpath(const int64_t* ijk, const int8_t* octlevel, int n, int64_t* octpath,

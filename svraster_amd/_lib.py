@@ -71,6 +71,7 @@ _SIGS = {
     "svr_grid_pts_link_workspace_bytes": (SZ, [LL]),
     "svr_build_grid_pts_link": (I, [P, P, LL, P, P, P, P, P]),
     "svr_agg_voxel_into_grid_pts": (I, [P, LL, P, I, LL, P, P, P]),
+    "svr_agg_voxel_into_grid_pts_noalloc": (I, [P, LL, P, I, LL, P, P]),
     "svr_total_variation_bw": (I, [P, LL, I, P, LL, F, P, I, I, P, P, P]),
     "svr_voxel_gradient_bw": (I, [P, LL, P, LL, P, P, I, F, P, P]),
     "svr_points_loss_bw": (I, [P, I, P, LL, P, I, P, P, I, P, P, P, I, F, P, P]),

@@ -135,6 +135,38 @@ KeyLayout key_layout(int rank_shift, long long n_tiles) {
   return k;
 }
 
+// Fixed device scratch of the allocation-free entry points: per-grid-point voxel counts of
+// svr_agg_voxel_into_grid_pts_noalloc, at most kScratchPoints grid points (16 MiB).  Each GPU
+// the library runs on holds its own copy until the process ends.  Host code must not pass the
+// array's name to a kernel (in host code it is only a stand-in for the symbol): scratch_ptr()
+// looks up its device address once per device with hipGetSymbolAddress and keeps it.
+constexpr long long kScratchPoints = 1LL << 22;
+static __device__ __align__(256) uint32_t scratch[kScratchPoints];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (nullptr and a message on failure).
+uint32_t* scratch_ptr() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    fail(2, "no current HIP device%s (device %lld)", "", dev);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fail_hip(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return reinterpret_cast<uint32_t*>(g_scratch_ptr[dev]);
+}
+
 }  // namespace
 
 extern "C" {
@@ -647,6 +679,21 @@ int svr_agg_voxel_into_grid_pts(const int64_t* vox_key, long long N, const float
   g_err.clear();
   if (C <= 0 || M < 0 || N < 0) return fail(1, "bad sizes%s %lld", "", C);
   SVR_TRY(launch_grid_pts_mean(N, C, vox_key, vox_val, M, count_ws, out, as_stream(stream)));
+  return 0;
+}
+
+int svr_agg_voxel_into_grid_pts_noalloc(const int64_t* vox_key, long long N, const float* vox_val, int C,
+                                        long long M, float* out, void* stream) {
+  g_err.clear();
+  if (C <= 0 || M < 0 || N < 0) return fail(1, "bad sizes%s %lld", "", C);
+  if (M > kScratchPoints)
+    return fail(1, "svr_agg_voxel_into_grid_pts_noalloc takes at most 4194304 grid points%s (got %lld); "
+                   "use svr_agg_voxel_into_grid_pts with a workspace", "", M);
+  uint32_t* count = scratch_ptr();
+  if (!count) return 2;
+  // The counts are cleared by a fill kernel inside launch_grid_pts_mean: the scratch keeps the
+  // previous call's contents, and no memset node is recorded if the caller captures the call.
+  SVR_TRY(launch_grid_pts_mean(N, C, vox_key, vox_val, M, count, out, as_stream(stream)));
   return 0;
 }
 
