@@ -8,8 +8,10 @@ One step = forward + backward of all 32 views (corner gather, SH, compositor, lo
 compositor, SH and gather transposes), inputs resident in HBM.
 
 Prints one JSON line.  Extra fields: `paths` (per-view vs batched numbers), `roofline` of the
-dominant kernel (backward compositor, HIP events on its own stream), `cpu_baseline` (the FP64
-CPU oracle on a bounded crop).
+dominant kernel (the slower of the forward and backward compositor launches, HIP events on the
+stream they run on; both are listed under `roofline.kernels`, with the VALU instruction rate where
+the committed rocprofv3 summary has it), `cpu_baseline` (the CPU oracle on this host: FP64 and
+FP32 on a bounded crop of view 0, and whole frames of the CPU-sized scenes).
 """
 from __future__ import annotations
 
@@ -77,73 +79,160 @@ def time_steps(fn, steps, warmup, sync):
     return (time.perf_counter() - t0) * 1000.0 / steps
 
 
-def backward_traffic(rast, dvox):
-    """Algorithmic bytes of the multi-view backward-compositor launch (one launch, all views).
+def compositor_bytes(rast, dvox):
+    """Algorithmic HBM bytes of the two multi-view compositor launches (one launch each, all views),
+    per the byte model of the path's measurement plan (72 B per streamed (tile, entry) pair: value 4,
+    bbox 8, centre 12, size 4, 8 corner SDFs 32, rgb 12):
 
-    The backward streams only the entries the forward handed over (those that composited into
-    a pixel of their tile): 96 B per streamed (tile, entry) pair [value 4, four 8x8-block pixel
-    masks 32, centre 12, size 4, 8 corner SDFs 32, rgb 12], 12 B per tile [contributing range 8,
-    list start 4], 24 B per pixel [T 4, contributor count 4, dL/dcolor 12, dL/dT 4], 48 B per voxel
-    that receives a gradient [12 floats].  Streamed entries E_b = sum over tiles of the
-    contributing range, read from the launch's own state; the entries the forward streamed
-    (up to each tile's last-contributor bound) are reported beside it."""
+    forward  72 E_f + 8 Nt + 20 R    (tile range 8; per ray colour 12, T 4, contributor count 4
+                                       written; depth and normal are not requested by this step)
+    backward 72 E_b + 16 Nt + 20 R + 52 Pv
+                                      (per ray dL/dcolour 12, dL/dT 4, T 4 read; 13 floats of
+                                       gradient per touched voxel written once)
+
+    E_f: entries the forward streamed (up to each tile's last-contributor bound); E_b: entries the
+    backward streams (those the forward handed over as contributing).  The hand-over itself --
+    per contributing entry its four 8x8-block pixel masks, 32 B written by the forward and read by
+    the backward -- is this design's own traffic, not algorithmic bytes; it is reported beside."""
     ranges, _, _, tile_last = rast.state_views()
     E_f = int((tile_last.long() - ranges[:, 0].long()).clamp_min(0).sum().item())
     cr = rast.contrib_ranges()
     E_b = int((cr[:, 1].long() - cr[:, 0].long()).clamp_min(0).sum().item())
     touched = int((dvox != 0).any(1).sum().item())
-    total = 96.0 * E_b + 12.0 * rast.n_tiles + 24.0 * rast.n_pix + 48.0 * touched
-    return total, {"streamed_entries": E_b, "forward_streamed_entries": E_f, "tiles": rast.n_tiles,
-                   "rays": rast.n_pix, "touched_voxels": touched}
+    Nt, R = rast.n_tiles, rast.n_pix
+    fwd = 72.0 * E_f + 8.0 * Nt + 20.0 * R
+    bwd = 72.0 * E_b + 16.0 * Nt + 20.0 * R + 52.0 * touched
+    return fwd, bwd, {"forward_streamed_entries": E_f, "backward_streamed_entries": E_b, "tiles": Nt,
+                      "rays": R, "touched_voxels": touched, "handover_bytes": int(36 * E_b)}
 
 
-def traffic_from_profiles(kernel_prefix):
-    """HBM bytes per launch of the given kernel from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration); None if absent."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_multi32.json")
+def profile_counters(kernel_prefix, workload):
+    """Per-launch figures of the given kernel from the committed rocprofv3 --pmc summary of the
+    workload's step (profiles/pmc_compositors_<workload>.json): HBM bytes (FETCH_SIZE + WRITE_SIZE
+    scaled by the copy-kernel calibration) and VALU instructions.  None when absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"pmc_compositors_{workload.replace('_', '')}.json")
     try:
         with open(path) as f:
-            d = json.load(f)
-        return d["kernels"][kernel_prefix]["hbm_bytes_per_launch"], d
+            return json.load(f)["kernels"][kernel_prefix]
     except (OSError, KeyError, ValueError):
-        return None, None
+        return None
 
 
-def cpu_baseline(workload_name, gpu_color_view0=None):
-    """FP64 CPU oracle fwd+bwd on a 64x64 crop at the centre of view 0 (bounded sample).
-    Also returns the max-abs difference of the GPU colour on that crop vs the oracle."""
+def cpu_threads():
+    """Host threads the CPU leg uses: all cores, bounded by the job's thread allowance when one is set."""
+    n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_fwd_bwd(sc, cam, bn, s_val, dtype, window=None):
+    """Oracle forward + backward (colour cotangent) of one view; returns (seconds, rays, output)."""
+    import torch
+    from oracle import raster as orc
+    from oracle import sh as osh
+    t0 = time.perf_counter()
+    rgbs = osh.sh_rgb(sc.sh_degree, None, sc.vox_center, cam.c2w[:3, 3], sc.sh0.double(),
+                      sc.shs.double()).detach().to(dtype).requires_grad_(True)
+    geos = sc.geo_grid_pts.reshape(-1)[sc.vox_key].to(dtype).requires_grad_(True)
+    s = torch.tensor(s_val, dtype=dtype, requires_grad=True)
+    out = orc.render(orc.camera_from(cam), [1.0, 1.0, 1.0], bn.vals, bn.ranges, bn.pre.bbox,
+                     sc.vox_center, sc.vox_size, geos, rgbs, s, need_depth=False, need_normal=False,
+                     window=window, dtype=dtype)
+    g = torch.rand(out.color.shape, generator=torch.Generator().manual_seed(3), dtype=torch.float64)
+    orc.backward(out, geos, rgbs, s, g)
+    x0, y0, x1, y1 = out.window
+    return time.perf_counter() - t0, (x1 - x0) * (y1 - y0), out
+
+
+def cpu_baseline(workload_name, gpu_color_view0=None, full_frames=("small_128",)):
+    """The CPU oracle timed on this host: fwd+bwd (colour loss) of a 64x64 crop at the centre of
+    view 0 of the workload in FP64 (the headline CPU figure) and FP32, plus whole frames of the
+    small CPU-sized workloads.  Also returns the max-abs difference of the GPU colour on the crop
+    vs the FP64 oracle."""
     import torch
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import binning as ob
-    from oracle import raster as orc
-    from oracle import sh as osh
     from svraster_amd.synthetic import make_workload
-    threads = torch.get_num_threads()
+    threads = cpu_threads()
+    torch.set_num_threads(threads)
     w = make_workload(workload_name, device="cpu")
     sc, cam = w.scene, w.cameras[0]
     bn = ob.binning_for_camera(cam, sc)
     cx, cy = cam.image_width // 2, cam.image_height // 2
     x0, y0 = cx - 32 - cx % 16, cy - 32 - cy % 16
     win = (x0, y0, x0 + 64, y0 + 64)
-    t0 = time.perf_counter()
-    rgbs = osh.sh_rgb(sc.sh_degree, None, sc.vox_center, cam.c2w[:3, 3], sc.sh0.double(),
-                      sc.shs.double()).detach().requires_grad_(True)
-    geos = sc.geo_grid_pts.reshape(-1)[sc.vox_key].double().requires_grad_(True)
-    s = torch.tensor(w.s_val, dtype=torch.float64, requires_grad=True)
-    out = orc.render(orc.camera_from(cam), [1.0, 1.0, 1.0], bn.vals, bn.ranges, bn.pre.bbox,
-                     sc.vox_center, sc.vox_size, geos, rgbs, s, need_depth=False, need_normal=False,
-                     window=win)
-    g = torch.rand_like(out.color)
-    orc.backward(out, geos, rgbs, s, g)
-    dt = time.perf_counter() - t0
-    rays = 64 * 64
+    dt64, rays, out = _oracle_fwd_bwd(sc, cam, bn, w.s_val, torch.float64, win)
+    dt32, _, _ = _oracle_fwd_bwd(sc, cam, bn, w.s_val, torch.float32, win)
     err = None
     if gpu_color_view0 is not None:
         got = gpu_color_view0[:, win[1]:win[3], win[0]:win[2]].detach().cpu().double()
         err = float((got - out.color.detach()).abs().max())
-    return ({"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-             "sample": f"FP64 torch oracle fwd+bwd (color loss), 64x64 crop at the centre of view 0 of "
-                       f"{workload_name}; {dt:.1f} s"}, err)
+    frames = {}
+    for name in full_frames:
+        wf = make_workload(name, device="cpu")
+        bnf = ob.binning_for_camera(wf.cameras[0], wf.scene)
+        for dt, tag in ((torch.float64, "fp64"), (torch.float32, "fp32")):
+            sec, r, _ = _oracle_fwd_bwd(wf.scene, wf.cameras[0], bnf, wf.s_val, dt)
+            frames[f"{name} full frame {tag}"] = {"mrays_s": round(r / sec / 1e6, 6), "seconds": round(sec, 2),
+                                                   "rays": r}
+    return ({"value": round(rays / dt64 / 1e6, 6), "unit": "Mrays/s", "cores": threads, "kind": "port",
+             "sample": f"FP64 torch oracle fwd+bwd (colour loss), 64x64 crop at the centre of view 0 of "
+                       f"{workload_name}; {dt64:.1f} s",
+             "fp32_value": round(rays / dt32 / 1e6, 6), "cpu_model": cpu_model(), "full_frames": frames}, err)
+
+
+VALU_PEAK_PER_S = 256 * 2.4e9     # wave64 FP32 VALU instructions: 1 per cycle per CU (4 SIMD16), 2.4 GHz
+
+
+def fwd_kernel_name():
+    from svraster_amd import _lib
+    halves = _lib.get_option("batch_depth") // 64
+    return f"render_fwd_pairs_kernel<false, false, false, true, {halves}>"
+
+
+def roofline_block(rast, kt, workload):
+    """HBM roofline of both multi-view compositor launches (algorithmic bytes / the launch's
+    HIP-event time on its own stream); the dominant (slower) one fills the top-level fields.
+    Measured HBM traffic and VALU instruction counts per launch come from the committed rocprofv3
+    summary when it describes the same kernel."""
+    fb, bb, counts = compositor_bytes(rast, rast.dvox)
+    entries = []
+    for key, name, nbytes in (("render_fwd", fwd_kernel_name(), fb), ("render_bwd", bwd_kernel_name(), bb)):
+        ms, n = kt[key]
+        avg_s = ms / max(n, 1) / 1000.0
+        achieved = nbytes / avg_s / 1e9
+        prof = profile_counters(name, workload) or {}
+        e = {"kernel": name + " (multi-view work queue: all views in one launch)", "bound": "hbm",
+             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": prof.get("hbm_bytes_per_launch"),
+             "bytes_per_launch": int(nbytes), "avg_launch_ms": round(avg_s * 1000.0, 4)}
+        if prof.get("valu_insts_per_launch"):
+            rate = prof["valu_insts_per_launch"] / avg_s
+            e["valu"] = {"insts_per_launch": prof["valu_insts_per_launch"], "achieved_per_s": rate,
+                         "peak_per_s": VALU_PEAK_PER_S, "frac": round(rate / VALU_PEAK_PER_S, 4)}
+        entries.append(e)
+    dom = max(entries, key=lambda e: e["avg_launch_ms"])
+    out = {k: dom[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic", "kernel",
+                               "bytes_per_launch", "avg_launch_ms")}
+    if "valu" in dom:
+        out["valu"] = dom["valu"]
+    out["counts"] = counts
+    out["kernels"] = entries
+    return out
 
 
 def kernel_times(lib, fn):
@@ -300,10 +389,8 @@ def bench_train(args):
 
     nbytes = train_iteration_bytes(it, smooth)
     stage_gbs = {k: round(nbytes[k] / (st_b[k] / 1000.0) / 1e9, 1) for k in nbytes if st_b.get(k, 0) > 0}
-    bytes_launch, counts = backward_traffic(rast, rast.dvox)
-    bwd_ms, bwd_n = kt_b["render_bwd"]
-    avg_launch_s = bwd_ms / max(bwd_n, 1) / 1000.0
-    achieved = bytes_launch / avg_launch_s / 1e9
+    roof = roofline_block(rast, kt_b, args.workload)
+    it.check_capacity()
     value = rays / (ms_b / 1000.0) / 1e6
     result = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -325,11 +412,7 @@ def bench_train(args):
         "stages_algorithmic_bytes": {k: int(v) for k, v in nbytes.items()},
         "stages_gbs_batched": stage_gbs,
         "autotune": tune_info,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": bwd_kernel_name() + " (multi-view work queue: all views in one launch)",
-                     "bytes_per_launch": int(bytes_launch), "counts": counts,
-                     "avg_launch_ms": round(avg_launch_s * 1000.0, 4)},
+        "roofline": roof,
     }
     print(json.dumps(result), flush=True)
 
@@ -341,6 +424,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="multi_32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-full-frames", action="store_true",
+                    help="CPU leg also times the 256x256 fd_256 frame (about a minute of host time)")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
     ap.add_argument("--no-autotune", action="store_true", help="default launch configuration, no tuning")
     ap.add_argument("--batched-only", action="store_true",
@@ -443,15 +528,11 @@ def main():
     log(f"[bench] batched graph path: {ms_b:.2f} ms/step; compositor fwd {kt_b['render_fwd'][0]:.2f} ms "
         f"bwd {kt_b['render_bwd'][0]:.2f} ms")
 
-    # -- roofline of the dominant kernel (multi-view backward compositor: one launch, all views) --
+    # -- roofline of the dominant kernel (the slower of the two compositor launches) --------------
     bstep()
     sync()
-    bytes_launch, counts = backward_traffic(rast, rast.dvox)
-    bwd_ms, bwd_n = kt_b["render_bwd"]
-    avg_launch_s = bwd_ms / max(bwd_n, 1) / 1000.0
-    achieved = bytes_launch / avg_launch_s / 1e9
-    BWD_KERNEL = bwd_kernel_name()
-    traffic, _ = traffic_from_profiles(BWD_KERNEL)
+    roof = roofline_block(rast, kt_b, args.workload)
+    bstep.check_capacity()
 
     value = rays / (ms_b / 1000.0) / 1e6
     value_pv = rays / (ms_pv / 1000.0) / 1e6
@@ -479,12 +560,7 @@ def main():
                   "batched_vs_per_view_image_max_abs": img_err,
                   "batched_vs_per_view_grad_rel_l2": grad_err,
                   "keys_per_step": K_used},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None if traffic is None else int(traffic),
-                     "kernel": BWD_KERNEL + " (multi-view work queue: all views in one launch)",
-                     "bytes_per_launch": int(bytes_launch), "counts": counts,
-                     "avg_launch_ms": round(avg_launch_s * 1000.0, 4)},
+        "roofline": roof,
         "autotune": tune_info,
         "kernels_ms_per_step": {"batched_render_fwd": round(kt_b["render_fwd"][0], 3),
                                 "batched_render_bwd": round(kt_b["render_bwd"][0], 3),
@@ -493,7 +569,8 @@ def main():
     }
     if not args.no_cpu_baseline and not args.profile:
         log("[bench] CPU oracle baseline ...")
-        result["cpu_baseline"], crop_err = cpu_baseline(args.workload, rast.view_slice(rast.color, 3, 0))
+        frames = ("small_128", "fd_256") if args.cpu_full_frames else ("small_128",)
+        result["cpu_baseline"], crop_err = cpu_baseline(args.workload, rast.view_slice(rast.color, 3, 0), frames)
         result["paths"]["image_max_abs_vs_fp64_oracle_crop"] = crop_err
     print(json.dumps(result), flush=True)
 
