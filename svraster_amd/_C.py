@@ -337,6 +337,36 @@ def biased_adam_step(sparse, param, grad, exp_avg, exp_avg_sq, lr, beta1, beta2,
 # ----------------------------------------------------------------------------------------------
 # Grid losses
 # ----------------------------------------------------------------------------------------------
+# Table-order plan of the last grid table seen by the grid losses (one: a plan holds two res^3
+# index maps), keyed by the identity and version of the tensors that define the table.
+_plan_cache = {}
+
+
+def _table_key(*ts):
+    return tuple((t.data_ptr(), t._version, tuple(t.shape)) for t in ts)
+
+
+def _table_order_plan(grid_keys, grid2voxel, coord, size, res, vox_key):
+    from .batched import TableOrderPlan
+    key = _table_key(grid_keys, grid2voxel, coord, size, vox_key) + (int(res),)
+    hit = _plan_cache.get("plan")
+    if hit is not None and hit[0] == key:
+        return hit[1]
+    _plan_cache.clear()
+    keys = grid_keys.to(torch.int32).contiguous()
+    plan = None
+    if keys.numel() > 0 and bool((keys[1:] > keys[:-1]).all()):       # sorted and unique, as train.py passes
+        plan = TableOrderPlan(keys, grid2voxel.to(torch.int32).contiguous(), _f32(coord, "grid_voxel_coord"),
+                              _f32(size.reshape(-1), "grid_voxel_size"), int(res), vox_key.contiguous())
+        plan = plan if plan.ok else None
+        if plan is not None:
+            plan.step0 = torch.zeros(1, dtype=torch.int32, device=keys.device)
+            plan.weights = torch.zeros(2, dtype=torch.float32, device=keys.device)
+            plan.coef = None
+    _plan_cache["plan"] = (key, plan)
+    return plan
+
+
 def _grid_loss(name, grid_pts, vox_key, grid_voxel_coord, grid_voxel_size, grid_res, grid_mask,
                grid_keys, grid2voxel, active_list, weight, vox_size_inv, grid_pts_grad, seed, randv):
     _inplace_ok(grid_pts_grad, "grid_pts_grad")
@@ -344,6 +374,23 @@ def _grid_loss(name, grid_pts, vox_key, grid_voxel_coord, grid_voxel_size, grid_
     if seed is None:
         seed = next(_seed_counter)
     active = active_list.to(device=dev, dtype=torch.int32).contiguous()
+    G = grid_keys.numel()
+    if randv is None and active.numel() == G and G > 0:
+        # Every key active once (the trainer's randperm list with drop ratio 0): the table-order
+        # form gives each key the same terms as the per-key scatter, with the random sample
+        # offsets drawn per table entry instead of per list position, and sums each voxel's
+        # contributions before touching the gradient.
+        plan = _table_order_plan(grid_keys, grid2voxel, grid_voxel_coord, grid_voxel_size, grid_res, vox_key)
+        if plan is not None and int(torch.bincount(active.long(), minlength=G).max()) == 1:
+            eik = name == "svr_grid_eikonal_bw"
+            plan.weights.copy_(torch.tensor([float(weight) if eik else 0.0, 0.0 if eik else float(weight)]))
+            n_coef = (7 if eik else 3) * G
+            if plan.coef is None or plan.coef.numel() < n_coef:
+                plan.coef = torch.empty(n_coef, dtype=torch.float32, device=dev)
+            mask = grid_mask.to(torch.uint8).contiguous() if grid_mask.dtype != torch.uint8 else grid_mask.contiguous()
+            plan.run(_f32(grid_pts, "grid_pts"), grid_pts_grad, mask, 2 if eik else 1, plan.weights,
+                     float(vox_size_inv), int(seed), plan.step0, plan.coef)
+            return
     rv = None
     if randv is not None:
         rv = _f32(randv.to(dev), "randv")

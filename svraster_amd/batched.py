@@ -394,6 +394,89 @@ def grid_voxel_coords(vox_center, vox_size, scene_center, inside_extent: float, 
     return coord, size
 
 
+def grid_dense_index(keys: torch.Tensor, res: int) -> torch.Tensor:
+    """Cell -> index into the sorted keys for all res^3 cells (-1 where absent)."""
+    lib = _lib.load()
+    dense = torch.empty(res ** 3, dtype=torch.int32, device=keys.device)
+    _lib.check(lib.svr_grid_dense_index(_p(keys), int(keys.numel()), int(res), _p(dense),
+                                        ctypes.c_void_p(torch.cuda.current_stream(keys.device).cuda_stream)),
+               "svr_grid_dense_index")
+    return dense
+
+
+class TableOrderPlan:
+    """Set-up of the table-order ("pull") form of the grid eikonal / Laplacian backward
+    (svr_grid_smoothness_bw_table_order) for one valid-gradient table, given as its sorted keys
+    and their voxels (the form train.py passes to grid_loss_bw).  The entries are put in the
+    canonical table order -- voxel by voxel in voxel order, each voxel's cells x fastest -- and
+    `ok` tells whether the table has the layout the form needs: unique cells, every voxel's
+    entries exactly its integer cell box, power-of-two box sizes (octree voxels have them).
+    Entry e of that order draws the random sample offset with index e."""
+
+    def __init__(self, keys_sorted, g2v_sorted, coord, size, res: int, vox_key):
+        self.lib = _lib.load()
+        self.dev = keys_sorted.device
+        self.res, self.G = int(res), int(keys_sorted.numel())
+        self.coord, self.size, self.vox_key = coord, size, vox_key
+        self.N = int(vox_key.shape[0])
+        self.dense = grid_dense_index(keys_sorted, self.res)
+        self.ok = self._canonical_order(keys_sorted.long(), g2v_sorted.long())
+        if not self.ok:
+            return
+        n3 = self.res ** 3
+        tk = self.table_keys.long()
+        self.cell_entry = torch.full((n3,), -1, dtype=torch.int32, device=self.dev)
+        self.cell_entry[tk] = torch.arange(self.G, dtype=torch.int32, device=self.dev)
+        # compact numbering of the table's voxels, in table order
+        tvox, table_tv = torch.unique_consecutive(self.table_vox, return_inverse=True)
+        self.tvox = tvox.to(torch.int32).contiguous()
+        self.table_tv = table_tv.to(torch.int32).contiguous()
+        self.T = int(self.tvox.numel())
+        self.cell_tv = torch.full((n3,), -1, dtype=torch.int32, device=self.dev)
+        self.cell_tv[tk] = self.table_tv
+        self.rec = torch.empty(12 * self.T, dtype=torch.float32, device=self.dev)
+
+    def _canonical_order(self, k, v) -> bool:
+        if self.G == 0 or int((self.dense >= 0).sum().item()) != self.G:
+            return False                               # empty table or repeated cells
+        res = self.res
+        x, y, z = k % res, (k // res) % res, k // (res * res)
+        lo = self.coord.long()[v]
+        s = self.size.long()[v]
+        if not bool(((s > 0) & ((s & (s - 1)) == 0)).all()):
+            return False                               # sizes must be powers of two
+        dx, dy, dz = x - lo[:, 0], y - lo[:, 1], z - lo[:, 2]
+        if not bool(((dx >= 0) & (dx < s) & (dy >= 0) & (dy < s) & (dz >= 0) & (dz < s)).all()):
+            return False
+        local = (dz * s + dy) * s + dx                 # x fastest inside the voxel's box
+        order = torch.argsort(v * (1 << 30) + local)
+        v, local, s = v[order], local[order], s[order]
+        same = v[1:] == v[:-1]
+        # each voxel's entries run through local index 0 .. s^3 - 1, one each
+        if int(local[0]) != 0 or not bool((local[1:][same] == local[:-1][same] + 1).all()):
+            return False
+        if not bool((local[1:][~same] == 0).all()):
+            return False
+        last = torch.cat([~same, torch.ones(1, dtype=torch.bool, device=self.dev)])
+        if not bool((local[last] == s[last] ** 3 - 1).all()):
+            return False
+        self.table_keys = k[order].to(torch.int32).contiguous()
+        self.table_vox = v.to(torch.int32).contiguous()
+        return True
+
+    def run(self, grid_pts, grid_grad, mask, terms: int, weights_dev, vox_size_inv: float, seed: int, step_dev,
+            coef):
+        """Accumulate the enabled terms' gradient (terms: 1 Laplacian, 2 eikonal, 3 both; weights
+        in weights_dev = [eikonal, Laplacian]) into grid_grad; coef: float[3G] or float[7G]."""
+        st = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+        _lib.check(self.lib.svr_grid_smoothness_bw_table_order(
+            _p(grid_pts), grid_pts.numel(), _p(self.vox_key), self.N, _p(self.coord), _p(self.size), self.res,
+            _p(mask), _p(self.table_keys), _p(self.table_tv), self.G, _p(self.cell_entry), _p(self.cell_tv),
+            _p(self.tvox), self.T, int(terms), _p(weights_dev), ctypes.c_float(vox_size_inv),
+            ctypes.c_uint(int(seed) & 0xFFFFFFFF), _p(step_dev), _p(self.rec), _p(coef), _p(grid_grad), st),
+            "svr_grid_smoothness_bw_table_order")
+
+
 class GridSmoothness:
     """Laplacian smoothness (and, with eikonal_weight != 0, grid eikonal) backward on the
     valid-gradient grid table, as the reference trainer calls them each iteration
@@ -425,35 +508,23 @@ class GridSmoothness:
                                                   self.max_level, is_leaf.reshape(-1).bool())
         self.keys, order = keys.sort(stable=True)
         self.g2v = g2v[order].contiguous()
-        del order
-        self.table_keys, self.table_vox = keys, g2v          # table order: voxel by voxel
-        del keys, g2v
+        del order, keys, g2v
         self.mask = mask
         self.G = int(self.keys.numel())
         self.coord, self.size = grid_voxel_coords(vox_center, vox_size, scene_center, inside_extent,
                                                   self.max_level)
         self.vox_key = vox_key.contiguous()
         self.N = int(vox_center.shape[0])
-        n3 = self.res ** 3
-        self.dense = torch.empty(n3, dtype=torch.int32, device=self.dev)
-        _lib.check(self.lib.svr_grid_dense_index(_p(self.keys), self.G, self.res, _p(self.dense), self._stream()),
-                   "svr_grid_dense_index")
-        layout_ok = self._table_layout_ok()
-        # Active list = every key in table order, as indices into the sorted keys.
-        self.active = self.dense[self.table_keys.long()].contiguous()
-        self.pull = bool(pull) and layout_ok
-        if self.pull:
-            tk = self.table_keys.long()
-            self.cell_entry = torch.full((n3,), -1, dtype=torch.int32, device=self.dev)
-            self.cell_entry[tk] = torch.arange(self.G, dtype=torch.int32, device=self.dev)
-            # compact numbering of the table's voxels, in table order
-            self.tvox, self.table_tv = torch.unique_consecutive(self.table_vox, return_inverse=True)
-            self.tvox = self.tvox.to(torch.int32).contiguous()
-            self.table_tv = self.table_tv.to(torch.int32).contiguous()
-            self.T = int(self.tvox.numel())
-            self.cell_tv = torch.full((n3,), -1, dtype=torch.int32, device=self.dev)
-            self.cell_tv[tk] = self.table_tv
-            del tk
+        self.plan = TableOrderPlan(self.keys, self.g2v, self.coord, self.size, self.res, self.vox_key)
+        self.dense = self.plan.dense
+        self.pull = bool(pull) and self.plan.ok
+        # Active list = every key once, in the canonical table order when the table has its built
+        # layout, as indices into the sorted keys.
+        if self.plan.ok:
+            self.table_keys = self.plan.table_keys
+            self.active = self.dense[self.table_keys.long()].contiguous()
+        else:
+            self.active = torch.arange(self.G, dtype=torch.int32, device=self.dev)
         # Term weights (each already * G / A with A = G): eikonal, Laplacian.  The eikonal term is
         # on when its weight is non-zero at construction; the kernels read the weights from device
         # memory, so set_weights() changes them for later replays of a captured step.
@@ -461,49 +532,9 @@ class GridSmoothness:
         self.weights = torch.tensor([float(eikonal_weight), float(weight)], dtype=torch.float32, device=self.dev)
         self.weight, self.eikonal_weight = float(weight), float(eikonal_weight)
         if self.pull:
-            self.rec = torch.empty(12 * self.T, dtype=torch.float32, device=self.dev)
             self.coef = torch.empty((7 if self.eikonal else 3) * self.G, dtype=torch.float32, device=self.dev)
         self.vox_size_inv = float(self.res / inside_extent)
         self.seed = int(seed) & 0xFFFFFFFF
-
-    def _table_layout_ok(self) -> bool:
-        """Cells unique and each voxel's entries exactly its integer cell box, of power-of-two
-        size (the table build writes a box x-major, z fastest); on success the entries of every
-        voxel are reordered in place to x fastest, the layout svr_grid_laplacian_bw_table_order
-        reads."""
-        if self.G == 0:
-            return False
-        if int((self.dense >= 0).sum().item()) != self.G:
-            return False
-        k = self.table_keys.long()
-        v = self.table_vox.long()
-        res = self.res
-        x, y, z = k % res, (k // res) % res, k // (res * res)
-        lo = self.coord.long()[v]
-        s = self.size.long()[v]
-        if not bool(((s > 0) & ((s & (s - 1)) == 0)).all()):
-            return False                               # sizes must be powers of two
-        dx, dy, dz = x - lo[:, 0], y - lo[:, 1], z - lo[:, 2]
-        inside = (dx >= 0) & (dx < s) & (dy >= 0) & (dy < s) & (dz >= 0) & (dz < s)
-        if not bool(inside.all()):
-            return False
-        e = torch.arange(self.G, device=self.dev)
-        local = (dx * s + dy) * s + dz
-        first = e - local
-        same = v[1:] == v[:-1]
-        # entries of a voxel are consecutive, start at local 0 and run through s^3 - 1
-        ok = bool((first[1:][same] == first[:-1][same]).all()) and bool((local[1:][~same] == 0).all())
-        ok = ok and int(local[0]) == 0
-        last = torch.cat([~same, torch.ones(1, dtype=torch.bool, device=self.dev)])
-        ok = ok and bool((local[last] == s[last] ** 3 - 1).all())
-        if not ok:
-            return False
-        # x fastest inside each voxel: order by (first entry of the voxel, local index z, y, x)
-        local_x = (dz * s + dy) * s + dx
-        order = torch.argsort(first * (1 << 30) + local_x)
-        self.table_keys = self.table_keys[order].contiguous()
-        self.table_vox = self.table_vox[order].contiguous()
-        return True
 
     def _stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
@@ -521,13 +552,8 @@ class GridSmoothness:
 
     def __call__(self, grid_pts: torch.Tensor, grid_grad: torch.Tensor, step: torch.Tensor):
         if self.pull:
-            terms = 3 if self.eikonal else 1
-            _lib.check(self.lib.svr_grid_smoothness_bw_table_order(
-                _p(grid_pts), grid_pts.numel(), _p(self.vox_key), self.N, _p(self.coord), _p(self.size), self.res,
-                _p(self.mask), _p(self.table_keys), _p(self.table_tv), self.G, _p(self.cell_entry),
-                _p(self.cell_tv), _p(self.tvox), self.T, terms, _p(self.weights),
-                ctypes.c_float(self.vox_size_inv), ctypes.c_uint(self.seed), _p(step), _p(self.rec), _p(self.coef),
-                _p(grid_grad), self._stream()), "svr_grid_smoothness_bw_table_order")
+            self.plan.run(grid_pts, grid_grad, self.mask, 3 if self.eikonal else 1, self.weights,
+                          self.vox_size_inv, self.seed, step, self.coef)
         else:
             terms = ((1, self.weight),) + (((0, self.eikonal_weight),) if self.eikonal else ())
             for lap, wt in terms:

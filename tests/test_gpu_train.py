@@ -166,10 +166,17 @@ def test_grid_eikonal_and_laplacian_table_order_match_api():
     svraster_cuda.grid_loss_bw.grid_eikonal(weight=w_ge, grid_pts_grad=ref, **kw)
     eik = torch.zeros_like(sc.geo_grid_pts)
     svraster_cuda.grid_loss_bw.grid_eikonal(weight=w_ge, grid_pts_grad=eik, **kw)
+    # the per-key scatter form (dense cell index, same offsets) of both terms
+    push = GridSmoothness(sc.vox_center, sc.vox_size, sc.octlevel, sc.is_leaf, sc.vox_key, sc.scene_center, 2.0,
+                          weight=w_ls, seed=11, eikonal_weight=w_ge, pull=False)
+    got_push = torch.zeros_like(sc.geo_grid_pts)
+    push(sc.geo_grid_pts, got_push, step)
     torch.cuda.synchronize()
     e = rel_l2(got, ref)
-    say(f"  rel-L2 {e:.2e}; |eikonal part| {eik.norm().item():.3e}, |total| {ref.norm().item():.3e}")
-    assert eik.norm().item() > 0 and e <= 1e-5
+    e_push = rel_l2(got_push, ref)
+    say(f"  rel-L2 {e:.2e} (drop-in calls), {e_push:.2e} (per-key scatter); |eikonal part| {eik.norm().item():.3e}, "
+        f"|total| {ref.norm().item():.3e}")
+    assert eik.norm().item() > 0 and e <= 1e-5 and e_push <= 1e-5
     # eikonal alone through the weights in device memory (Laplacian weight 0), and back
     sm.set_weights(laplacian=0.0)
     got2 = torch.zeros_like(sc.geo_grid_pts)
