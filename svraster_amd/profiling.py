@@ -1,7 +1,8 @@
-"""Summaries of rocprofv3 counter runs for the compositor roofline.
+"""Summaries of rocprofv3 runs: counters for the compositor roofline, kernel-trace statistics.
 
     python -m svraster_amd.profiling compositors --out FILE.json \
         --calib-fetch DIR --calib-write DIR --run DIR [--run DIR ...] [--command TEXT ...]
+    python -m svraster_amd.profiling stats --run DIR --out FILE.txt [--title TEXT] [--command TEXT ...]
 
 Each DIR is the output directory of one `rocprofv3 --pmc ... --output-format csv -d DIR` run.
 The counter means per dispatch are merged over the runs, per kernel.  HBM bytes per launch are
@@ -21,7 +22,7 @@ import os
 import sys
 
 COPY_BYTES = 1 << 30
-COPY_KERNEL = "vectorized_elementwise_kernel"
+COPY_KERNEL = "MulFunctor"   # the copy launches of `bench.py --copy-calibration` (not its fill)
 
 
 def counter_means(run_dir: str, name_filter: str = "") -> dict:
@@ -69,6 +70,27 @@ def compositors(args) -> dict:
             "write_scale_bytes_per_kib": wscale * 1024.0, "kernels": kernels}
 
 
+def kernel_stats(args) -> str:
+    """Text table of a `rocprofv3 --kernel-trace --stats --output-format csv` run: per kernel the
+    number of dispatches, total and mean duration (ms) and share of the kernel time."""
+    paths = glob.glob(os.path.join(args.run, "**", "*kernel_stats.csv"), recursive=True)
+    if not paths:
+        raise RuntimeError(f"no kernel_stats.csv under {args.run}")
+    rows = []
+    with open(paths[0]) as f:
+        for r in csv.DictReader(f):
+            rows.append((r["Name"], int(r["Calls"]), float(r["TotalDurationNs"]) * 1e-6,
+                         float(r["AverageNs"]) * 1e-6, float(r["Percentage"])))
+    rows.sort(key=lambda r: -r[2])
+    lines = [f"# rocprofv3 kernel statistics: {args.title}"]
+    lines += [f"# command: {c}" for c in args.command]
+    lines.append("")
+    lines.append(f"{'kernel':<96} {'calls':>7} {'total_ms':>10} {'avg_ms':>9} {'pct':>6}")
+    for name, calls, tot, avg, pct in rows[:args.top]:
+        lines.append(f"{name[:96]:<96} {calls:>7} {tot:>10.3f} {avg:>9.4f} {pct:>6.2f}")
+    return "\n".join(lines) + "\n"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="what", required=True)
@@ -78,7 +100,19 @@ def main(argv=None):
     c.add_argument("--calib-write", required=True)
     c.add_argument("--run", action="append", required=True)
     c.add_argument("--command", action="append", default=[])
+    t = sub.add_parser("stats")
+    t.add_argument("--run", required=True)
+    t.add_argument("--out", required=True)
+    t.add_argument("--title", default="")
+    t.add_argument("--top", type=int, default=40)
+    t.add_argument("--command", action="append", default=[])
     args = ap.parse_args(argv)
+    if args.what == "stats":
+        text = kernel_stats(args)
+        with open(args.out, "w") as f:
+            f.write(text)
+        print(f"wrote {args.out}", file=sys.stderr)
+        return
     res = compositors(args)
     with open(args.out, "w") as f:
         json.dump(res, f, indent=1)
