@@ -120,15 +120,20 @@ __device__ inline Ray pair_ray(const Ray& r, const float4 d, const float4 inv) {
   return pr;
 }
 
-// Slab test of a ray against a cube of centre c and half size h (auxiliary.h:251-264).
-__device__ inline void ray_box(const Ray& r, float cx, float cy, float cz, float h,
-                               float& a, float& b) {
-  const float dx = cx - r.ox, dy = cy - r.oy, dz = cz - r.oz;
+// Slab test of a ray against a cube of half size h whose centre relative to the ray origin is
+// (dx, dy, dz) (auxiliary.h:251-264, where dir = vox_c - ro).  The compositors stage the relative
+// centre once per entry, since every ray of a view starts at the camera centre.
+__device__ inline void ray_box_rel(const Ray& r, float dx, float dy, float dz, float h, float& a, float& b) {
   const float x0 = (dx - h) * r.ix, x1 = (dx + h) * r.ix;
   const float y0 = (dy - h) * r.iy, y1 = (dy + h) * r.iy;
   const float z0 = (dz - h) * r.iz, z1 = (dz + h) * r.iz;
   a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
   b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+}
+
+// Slab test against a cube of centre c and half size h.
+__device__ inline void ray_box(const Ray& r, float cx, float cy, float cz, float h, float& a, float& b) {
+  ray_box_rel(r, cx - r.ox, cy - r.oy, cz - r.oz, h, a, b);
 }
 
 __device__ inline bool pixel_in_bbox(int px, int py, uint32_t b0, uint32_t b1) {
@@ -323,22 +328,22 @@ __device__ inline float2 wave_comp_sum(float hi, float lo) {
   return make_float2(hi, lo);
 }
 
-// Position of the k-th (0-based) set bit of m; k < popcount(m).
+// Position of the k-th (0-based) set bit of m; k < popcount(m).  After picking the 32-bit half,
+// the position p is the largest with popcount(x & (2^p - 1)) <= k, found by a binary search on
+// the bits of p that counts the bits below each candidate with one bit-field extract (no
+// running remainder, no shifts of x).
 __device__ inline int select_set_bit(uint64_t m, int k) {
-  int pos = 0;
-  const int c32 = __popc((uint32_t)m);
-  if (k >= c32) { k -= c32; m >>= 32; pos += 32; }
-  uint32_t x = (uint32_t)m;
-  int c = __popc(x & 0xFFFFu);
-  if (k >= c) { k -= c; x >>= 16; pos += 16; }
-  c = __popc(x & 0xFFu);
-  if (k >= c) { k -= c; x >>= 8; pos += 8; }
-  c = __popc(x & 0xFu);
-  if (k >= c) { k -= c; x >>= 4; pos += 4; }
-  c = __popc(x & 0x3u);
-  if (k >= c) { k -= c; x >>= 2; pos += 2; }
-  if (k >= (int)(x & 1u)) pos += 1;
-  return pos;
+  const uint32_t lo = (uint32_t)m;
+  const int c32 = __popc(lo);
+  const bool up = k >= c32;
+  const uint32_t x = up ? (uint32_t)(m >> 32) : lo;
+  const int kk = up ? k - c32 : k;
+  int p = (__popc(x & 0xFFFFu) <= kk) ? 16 : 0;
+  p = (__popc(__builtin_amdgcn_ubfe(x, 0u, (uint32_t)p + 8u)) <= kk) ? p + 8 : p;
+  p = (__popc(__builtin_amdgcn_ubfe(x, 0u, (uint32_t)p + 4u)) <= kk) ? p + 4 : p;
+  p = (__popc(__builtin_amdgcn_ubfe(x, 0u, (uint32_t)p + 2u)) <= kk) ? p + 2 : p;
+  p = (__popc(__builtin_amdgcn_ubfe(x, 0u, (uint32_t)p + 1u)) <= kk) ? p + 1 : p;
+  return up ? p + 32 : p;
 }
 
 // Conservative per-voxel test: false only if no ray segment through the voxel can reach

@@ -48,9 +48,9 @@ __device__ inline void fetch_entry_b(EntryRegsB& e, const BwdArgs& a, uint32_t v
   e.r = rgbs[3 * vox]; e.g = rgbs[3 * vox + 1]; e.b = rgbs[3 * vox + 2];
 }
 
-__device__ inline void store_entry_b(const EntryRegsB& e, bool need_normal, float4* s_ch, float4* s_ml, float4* s_g0,
-                                     float4* s_g1, float4* s_rgb, float4* s_nrm, int slot) {
-  s_ch[slot] = make_float4(e.cx, e.cy, e.cz, 0.5f * e.l);
+__device__ inline void store_entry_b(const EntryRegsB& e, bool need_normal, const Ray& r, float4* s_ch, float4* s_ml,
+                                     float4* s_g0, float4* s_g1, float4* s_rgb, float4* s_nrm, int slot) {
+  s_ch[slot] = make_float4(e.cx - r.ox, e.cy - r.oy, e.cz - r.oz, 0.5f * e.l);   // centre relative to the camera
   s_ml[slot] = make_float4(e.cx - 0.5f * e.l, e.cy - 0.5f * e.l, e.cz - 0.5f * e.l, 1.f / e.l);
   s_g0[slot] = e.g0;
   s_g1[slot] = e.g1;
@@ -465,7 +465,7 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
       __syncthreads();
       const int cnt = min(kStage, n - base);
       if (t < cnt) {
-        store_entry_b(er, kNormal, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, t);
+        store_entry_b(er, kNormal, r, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, t);
         s_used[t] = 0u;
       }
       for (int k = t; k < kStage * 12; k += 256) (&s_acc[0][0])[k] = 0.f;
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
               const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
               const float4 ch = s_ch[j];
               float ta, tb;
-              ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
+              ray_box_rel(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
               const float4 ml = s_ml[j];
               const float4 g0 = s_g0[j], g1 = s_g1[j];
               const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
@@ -598,7 +598,7 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
               const Ray pr = pair_ray(r, rdd, s_ri[w][bit]);
               const float4 ch = s_ch[j];
               float ta, tb;
-              ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
+              ray_box_rel(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
               const float4 ml = s_ml[j];
               const float4 g0 = s_g0[j], g1 = s_g1[j];
               const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};

@@ -94,10 +94,10 @@ __device__ inline void fetch_entry(EntryRegs& e, uint32_t val, int view, long lo
 // LDS copy of a fetched entry.  With kCull, an entry that cannot reach the minimum alpha gets an
 // empty bbox (no pixel masks); the multi-view path has left such voxels out of binning already.
 template <bool kCull>
-__device__ inline void store_entry(const EntryRegs& e, bool need_normal, float s, float cam_scale, float4* s_ch,
-                                   float4* s_ml, float4* s_g0, float4* s_g1, float4* s_rgb, float4* s_nrm, uint2* s_bb,
-                                   int slot) {
-  s_ch[slot] = make_float4(e.cx, e.cy, e.cz, 0.5f * e.l);
+__device__ inline void store_entry(const EntryRegs& e, bool need_normal, float s, float cam_scale, const Ray& r,
+                                   float4* s_ch, float4* s_ml, float4* s_g0, float4* s_g1, float4* s_rgb, float4* s_nrm,
+                                   uint2* s_bb, int slot) {
+  s_ch[slot] = make_float4(e.cx - r.ox, e.cy - r.oy, e.cz - r.oz, 0.5f * e.l);   // centre relative to the camera
   s_ml[slot] = make_float4(e.cx - 0.5f * e.l, e.cy - 0.5f * e.l, e.cz - 0.5f * e.l, 1.f / e.l);
   s_g0[slot] = e.g0;
   s_g1[slot] = e.g1;
@@ -373,8 +373,8 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
   __shared__ uint16_t s_hd[4][kPairWin];      // their entry | pixel << 7
   __shared__ float s_pa[4][kPairWin];         // alpha per position (0 when the ray misses the box)
   __shared__ float s_pd[4][kDepth ? kPairWin : 1];
-  __shared__ uint64_t s_u[4][2];              // per wave: entries of the last round it composited
-  __shared__ uint64_t s_pix[4][kHalves][64];  // per pixel, in quadrant order: entries it composited
+  __shared__ unsigned long long s_em[kStage][4];      // per staged entry and 8x8 quadrant: pixels that composited it
+  __shared__ uint64_t s_uu[kHalves];          // per staging wave: its entries that some pixel composited
   __shared__ uint32_t s_red[4];
   __shared__ int s_tile;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -433,45 +433,52 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
     }
 
     // A round's contributing entries are known once every wave has finished it, so they are
-    // written at the top of the next round: packed after the tile's earlier ones (cc so far), each
-    // wave one quadrant's pixel masks, the staging lane the value.  At the end of every round each
-    // wave leaves in s_pix the entry sets of its pixels, at their quadrant positions, and in s_u
-    // the union of its pixels' sets.
-    int qw, qb;   // quadrant and bit of this lane's pixel
+    // written at the top of the next round: packed after the tile's earlier ones (cc so far), the
+    // staging lane writing its entry's value and its four quadrant pixel masks.  The masks are
+    // built during the composite: every pixel ORs its bit into s_em[entry][its quadrant] for each
+    // entry it composites.  Between the two barriers at the top of a round the staging lanes read
+    // and clear their rows (no wave composites then), and each staging wave leaves the ballot of
+    // its composited entries in s_uu.
+    uint64_t qbit;   // this lane's pixel as a bit of its 8x8 quadrant's mask
+    int qw;          // that quadrant
     {
       const int xt = px - tx * V.tile_w, yt = py - ty * V.tile_h;
       qw = (xt >> 3) + (V.tile_w >> 3) * (yt >> 3);
-      qb = (xt & 7) + 8 * (yt & 7);
+      qbit = 1ull << ((xt & 7) + 8 * (yt & 7));
     }
+    unsigned long long* const s_em_q = &s_em[0][qw];
     uint32_t cc = 0;
-    if (lane == 0) { s_u[w][0] = 0ull; s_u[w][1] = 0ull; }
+    uint32_t sval = 0;   // value of the entry this lane staged in the last round
+    if (t < kStage) { s_em[t][0] = 0ull; s_em[t][1] = 0ull; s_em[t][2] = 0ull; s_em[t][3] = 0ull; }
     for (int base = 0;; base += kStage) {
       const int ndone = __syncthreads_count(done || base >= n);
       const bool fin = ndone == 256;
-      const uint64_t u0 = (s_u[0][0] | s_u[1][0]) | (s_u[2][0] | s_u[3][0]);
-      const uint64_t u1 = kHalves > 1 ? (s_u[0][1] | s_u[1][1]) | (s_u[2][1] | s_u[3][1]) : 0ull;
-      const uint32_t n0 = (uint32_t)__popcll(u0);
-      const long long o = (long long)rg.x + cc;
-      if (t < kStage) {   // waves 0 and 1 staged entries 0-63 and 64-127
-        const uint64_t u = w == 0 ? u0 : u1;
-        if ((u >> lane) & 1ull)
-          a.cvals[o + (w == 0 ? 0u : n0) + __popcll(u & lt_mask)] = __float_as_uint(s_rgb[t].w);
+      uint64_t e0 = 0ull, e1 = 0ull, e2 = 0ull, e3 = 0ull;
+      if (t < kStage) {
+        e0 = s_em[t][0]; e1 = s_em[t][1]; e2 = s_em[t][2]; e3 = s_em[t][3];
+        s_em[t][0] = 0ull; s_em[t][1] = 0ull; s_em[t][2] = 0ull; s_em[t][3] = 0ull;
       }
-      cc += n0 + (uint32_t)__popcll(u1);
-      if (!fin && t < kStage && base + t < n)
-        store_entry<!kQueue>(er, kNormal, s, cam_scale, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t);
-      // The backward takes the pixels by quadrant: wave w transposes quadrant w's pixel sets into
-      // per-entry masks.
-      // (A half whose entries no pixel composited has nothing to hand over: no transpose.)
-      if (u0 != 0ull) {
-        const uint64_t q_lo = transpose64(s_pix[w][0][lane], lane);
-        if ((u0 >> lane) & 1ull) a.pmask[4 * (o + __popcll(u0 & lt_mask)) + w] = q_lo;
-      }
-      if (kHalves > 1 && u1 != 0ull) {
-        const uint64_t q_hi = transpose64(s_pix[w][kHalves - 1][lane], lane);
-        if ((u1 >> lane) & 1ull) a.pmask[4 * (o + n0 + __popcll(u1 & lt_mask)) + w] = q_hi;
+      const bool comp = ((e0 | e1) | (e2 | e3)) != 0ull;
+      const uint64_t ub = __ballot(comp);
+      if (w < kHalves && lane == 0) s_uu[w] = ub;
+      const uint32_t pval = sval;
+      if (!fin && t < kStage && base + t < n) {
+        store_entry<!kQueue>(er, kNormal, s, cam_scale, r, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, s_bb, t);
+        sval = er.val;
       }
       __syncthreads();
+      {
+        const uint64_t u0 = s_uu[0], u1 = kHalves > 1 ? s_uu[kHalves - 1] : 0ull;
+        const uint32_t n0 = (uint32_t)__popcll(u0);
+        if (comp) {   // waves 0 and 1 staged entries 0-63 and 64-127
+          const long long idx = (long long)rg.x + cc + (w == 0 ? 0u : n0) + (uint32_t)__popcll(ub & lt_mask);
+          a.cvals[idx] = pval;
+          ulonglong2* pm = reinterpret_cast<ulonglong2*>(a.pmask + 4 * idx);
+          pm[0] = make_ulonglong2(e0, e1);
+          pm[1] = make_ulonglong2(e2, e3);
+        }
+        cc += n0 + (uint32_t)__popcll(u1);
+      }
       if (fin) break;
       if (t < kStage) {
         if (base + kStage + t < n) fetch_entry(er, nval, vi, a.P, a.bboxes, a.centers, a.lengths, a.geos, rgbs_v);
@@ -479,7 +486,6 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
       }
       const int cnt = min(kStage, n - base);
       const uint64_t live = __ballot(!done);
-      uint64_t pr_lo = 0ull, pr_hi = 0ull;   // entries this pixel composited in this round
 
       // 1. entry masks (empty once every pixel is saturated) and the entry-major run offsets.
       const uint64_t m_lo = lane < cnt ? entry_block_mask(s_bb, s_rgb, lane, L, s_om[w], live) : 0ull;
@@ -527,7 +533,7 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
               const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
               const float4 ch = s_ch[j];
               float ta, tb;
-              ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
+              ray_box_rel(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
               hit = !(ta > tb);
               desc = (uint32_t)j | ((uint32_t)bit << 7);
               s_pa[w][k] = 0.f;
@@ -549,7 +555,7 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
             const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
             const float4 ch = s_ch[j];
             float ta, tb;
-            ray_box(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
+            ray_box_rel(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
             const float4 ml = s_ml[j];
             const float4 g0 = s_g0[j], g1 = s_g1[j];
             const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
@@ -585,20 +591,12 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
             T *= (1.f - alpha);
             done = T < kEarlyStopT;
             last = (uint32_t)(base + j + 1);
+            atomicOr(s_em_q + 4 * j, qbit);
             if (kMaxW)
               atomicMax(reinterpret_cast<uint32_t*>(a.max_w) + (__float_as_uint(rgb.w) & kVoxMask), __float_as_uint(wgt));
           }
-          pr_lo |= cs0_lo & ~cs_lo;     // composited: the set minus what was left when T saturated
-          pr_hi |= cs0_hi & ~cs_hi;
           wave_lds_sync();
         }
-      }
-      s_pix[qw][0][qb] = pr_lo;
-      if (kHalves > 1) s_pix[qw][kHalves - 1][qb] = pr_hi;
-      {
-        const uint64_t u_lo = wave_or_u64(pr_lo, lane);
-        const uint64_t u_hi = kHalves > 1 ? wave_or_u64(pr_hi, lane) : 0ull;
-        if (lane == 0) { s_u[w][0] = u_lo; s_u[w][1] = u_hi; }
       }
     }
 
