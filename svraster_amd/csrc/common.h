@@ -263,16 +263,30 @@ __device__ inline uint32_t wave_incl_max_u32(uint32_t x) {
   return x;
 }
 
-// Segmented inclusive sum over the wave of N floats per lane; lane l's segment starts at lane
-// seg_start <= l.  After the call the last lane of each segment holds the segment's sums.
-// DPP row shifts by 1, 2, 4, 8 inside 16-lane rows, then the last lane of the previous row
-// (rows 1 and 3) and lane 31 (rows 2 and 3) for segments that started in an earlier row.
+// Segmented inclusive sum over the wave of N floats per lane (N a multiple of 4); lane l's
+// segment starts at lane seg_start <= l.  After the call the last lane of each segment holds the
+// segment's sums.  DPP row shifts by 1, 2, 4, 8 inside 16-lane rows, then the last lane of the
+// previous row (rows 1 and 3) and lane 31 (rows 2 and 3) for segments that started in an earlier
+// row.  Each step adds the shifted partial sum times 1 or 0 (whether the source lane is in the
+// lane's segment): x = fma(dpp(x), c, x) is one v_fmac_f32 with the DPP shift on its first source.
+// The compiler does not fold a DPP move into fmac, so the steps are written as v_fmac_f32_dpp
+// directly, four values per statement (a value's DPP read follows its previous write by at least
+// three other instructions, and each statement starts with two wait states for the write before
+// it).  Rows outside row_mask keep their value, as do lanes whose source is out of the row
+// (bound_ctrl: source reads 0).  Sums of finite values equal those of a select-and-add form
+// (adding +0 leaves a non-zero sum unchanged).
+#define SVR_FMAC_DPP4(CTRL, V, Q, C)                                                       \
+  asm volatile("s_nop 1\n\t"                                                               \
+               "v_fmac_f32_dpp %0, %0, %4 " CTRL "\n\t"                                     \
+               "v_fmac_f32_dpp %1, %1, %4 " CTRL "\n\t"                                     \
+               "v_fmac_f32_dpp %2, %2, %4 " CTRL "\n\t"                                     \
+               "v_fmac_f32_dpp %3, %3, %4 " CTRL                                            \
+               : "+v"(V[Q]), "+v"(V[Q + 1]), "+v"(V[Q + 2]), "+v"(V[Q + 3])                  \
+               : "v"(C))
+
 template <int N>
 __device__ inline void wave_segmented_sum(float (&v)[N], int lane, int seg_start) {
-  // Each step adds the shifted partial sum times 1 or 0 (whether the source lane is in the
-  // lane's segment): x = fma(dpp(x), c, x), one VOP2 fmac with the DPP shift folded into its
-  // source operand, instead of a DPP move, a select and an add.  Sums of finite values equal
-  // those of the select form (adding +0 leaves a non-zero sum unchanged).
+  static_assert(N % 4 == 0, "values are summed in groups of four");
   const int lr = lane & 15;
   const float c1 = (lr >= 1 && lane - 1 >= seg_start) ? 1.f : 0.f;
   const float c2 = (lr >= 2 && lane - 2 >= seg_start) ? 1.f : 0.f;
@@ -281,17 +295,19 @@ __device__ inline void wave_segmented_sum(float (&v)[N], int lane, int seg_start
   const float cr = (((lane >> 4) & 1) && seg_start < (lane & ~15)) ? 1.f : 0.f;
   const float cw = (lane >= 32 && seg_start < 32) ? 1.f : 0.f;
 #pragma unroll
-  for (int q = 0; q < N; ++q) {
-    float x = v[q];
-    x = fmaf(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xF, 0xF, true)), c1, x);
-    x = fmaf(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x112, 0xF, 0xF, true)), c2, x);
-    x = fmaf(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xF, 0xF, true)), c4, x);
-    x = fmaf(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x118, 0xF, 0xF, true)), c8, x);
-    x = fmaf(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x142, 0xA, 0xF, false)), cr, x);
-    x = fmaf(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x143, 0xC, 0xF, false)), cw, x);
-    v[q] = x;
-  }
+  for (int q = 0; q < N; q += 4) SVR_FMAC_DPP4("row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1", v, q, c1);
+#pragma unroll
+  for (int q = 0; q < N; q += 4) SVR_FMAC_DPP4("row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1", v, q, c2);
+#pragma unroll
+  for (int q = 0; q < N; q += 4) SVR_FMAC_DPP4("row_shr:4 row_mask:0xf bank_mask:0xf bound_ctrl:1", v, q, c4);
+#pragma unroll
+  for (int q = 0; q < N; q += 4) SVR_FMAC_DPP4("row_shr:8 row_mask:0xf bank_mask:0xf bound_ctrl:1", v, q, c8);
+#pragma unroll
+  for (int q = 0; q < N; q += 4) SVR_FMAC_DPP4("row_bcast:15 row_mask:0xa bank_mask:0xf", v, q, cr);
+#pragma unroll
+  for (int q = 0; q < N; q += 4) SVR_FMAC_DPP4("row_bcast:31 row_mask:0xc bank_mask:0xf", v, q, cw);
 }
+#undef SVR_FMAC_DPP4
 
 // Compensated FP32 summation.  two_sum is Knuth's error-free transformation: s = fl(a + b) and
 // e = (a + b) - s exactly.  A running (hi, lo) pair adds each term to hi and its rounding error to
