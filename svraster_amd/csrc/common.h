@@ -167,6 +167,42 @@ __device__ inline float fast_sigmoid(float x) {
   return __builtin_amdgcn_rcpf(1.f + __expf(-x));
 }
 
+// Per-entry constants of the SDF segment as the compositors stage them, once per entry and view.
+// Every ray of a view starts at the camera centre o, so the box corners are kept relative to it,
+// and the corner SDFs with the differences the z lerps take:
+//   box = (lo.x, lo.y, lo.z, 1/l), hi = (hi.x, hi.y, hi.z, 0), lo/hi = (c - o) -/+ l/2;
+//   gd0 = (g0, g1 - g0, g2, g3 - g2), gd1 = (g4, g5 - g4, g6, g7 - g6).
+// ray_box_st and trilerp_d on these give the same values as ray_box and trilerp on the raw data.
+__device__ inline void stage_segment(float cx, float cy, float cz, float l, float4 g0, float4 g1, const Ray& r,
+                                     float4& box, float4& hi, float4& gd0, float4& gd1) {
+#pragma clang fp contract(off)
+  const float h = 0.5f * l;
+  const float dx = cx - r.ox, dy = cy - r.oy, dz = cz - r.oz;
+  box = make_float4(dx - h, dy - h, dz - h, 1.f / l);
+  hi = make_float4(dx + h, dy + h, dz + h, 0.f);
+  gd0 = make_float4(g0.x, g0.y - g0.x, g0.z, g0.w - g0.z);
+  gd1 = make_float4(g1.x, g1.y - g1.x, g1.z, g1.w - g1.z);
+}
+
+__device__ inline void ray_box_st(const Ray& r, const float4 box, const float4 hi, float& a, float& b) {
+  const float x0 = box.x * r.ix, x1 = hi.x * r.ix;
+  const float y0 = box.y * r.iy, y1 = hi.y * r.iy;
+  const float z0 = box.z * r.iz, z1 = hi.z * r.iz;
+  a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
+  b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+}
+
+__device__ inline float trilerp_d(const float4 gd0, const float4 gd1, float x, float y, float z) {
+#pragma clang fp contract(off)
+  const float c00 = fmaf(z, gd0.y, gd0.x);
+  const float c01 = fmaf(z, gd0.w, gd0.z);
+  const float c10 = fmaf(z, gd1.y, gd1.x);
+  const float c11 = fmaf(z, gd1.w, gd1.z);
+  const float c0 = fmaf(y, c01 - c00, c00);
+  const float c1 = fmaf(y, c11 - c10, c10);
+  return fmaf(x, c1 - c0, c0);
+}
+
 // The SDF segment of one (ray, voxel) pair: SDF at the entry point and three equal steps to
 // the exit point, their logistic CDFs, and the NeuS alpha (forward.cu:255-306).
 struct Segment {
@@ -190,6 +226,26 @@ __device__ inline void eval_segment(const Ray& r, float a, float b, float minx, 
   for (int k = 0; k < 4; ++k) {
     const float fk = (float)k;
     seg.sdf[k] = trilerp(g, fmaf(seg.sx, fk, seg.qx), fmaf(seg.sy, fk, seg.qy), fmaf(seg.sz, fk, seg.qz));
+    seg.phi[k] = fast_sigmoid(s * seg.sdf[k]);
+  }
+  const float raw = (seg.phi[0] - seg.phi[3]) * __builtin_amdgcn_rcpf(seg.phi[0] + kPhiEps);
+  seg.alpha = fmaxf(raw, 0.f);
+}
+
+// eval_segment on staged constants: the local coordinate of the entry point is
+// (a d - lo) / l, the ray origin having been folded into lo.
+__device__ inline void eval_segment_st(const Ray& r, float a, float b, const float4 box, const float4 gd0,
+                                       const float4 gd1, float s, Segment& seg) {
+#pragma clang fp contract(off)
+  seg.qx = fmaf(a, r.dx, -box.x) * box.w;
+  seg.qy = fmaf(a, r.dy, -box.y) * box.w;
+  seg.qz = fmaf(a, r.dz, -box.z) * box.w;
+  const float st = ((b - a) * (1.f / 3.f)) * box.w;
+  seg.sx = st * r.dx; seg.sy = st * r.dy; seg.sz = st * r.dz;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float fk = (float)k;
+    seg.sdf[k] = trilerp_d(gd0, gd1, fmaf(seg.sx, fk, seg.qx), fmaf(seg.sy, fk, seg.qy), fmaf(seg.sz, fk, seg.qz));
     seg.phi[k] = fast_sigmoid(s * seg.sdf[k]);
   }
   const float raw = (seg.phi[0] - seg.phi[3]) * __builtin_amdgcn_rcpf(seg.phi[0] + kPhiEps);

@@ -50,10 +50,7 @@ __device__ inline void fetch_entry_b(EntryRegsB& e, const BwdArgs& a, uint32_t v
 
 __device__ inline void store_entry_b(const EntryRegsB& e, bool need_normal, const Ray& r, float4* s_ch, float4* s_ml,
                                      float4* s_g0, float4* s_g1, float4* s_rgb, float4* s_nrm, int slot) {
-  s_ch[slot] = make_float4(e.cx - r.ox, e.cy - r.oy, e.cz - r.oz, 0.5f * e.l);   // centre relative to the camera
-  s_ml[slot] = make_float4(e.cx - 0.5f * e.l, e.cy - 0.5f * e.l, e.cz - 0.5f * e.l, 1.f / e.l);
-  s_g0[slot] = e.g0;
-  s_g1[slot] = e.g1;
+  stage_segment(e.cx, e.cy, e.cz, e.l, e.g0, e.g1, r, s_ch[slot], s_ml[slot], s_g0[slot], s_g1[slot]);
   s_rgb[slot] = make_float4(e.r, e.g, e.b, __uint_as_float(e.val));
   if (need_normal) {
     const float g[8] = {e.g0.x, e.g0.y, e.g0.z, e.g0.w, e.g1.x, e.g1.y, e.g1.z, e.g1.w};
@@ -190,12 +187,9 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
         s_bb[t] = a.bboxes[(long long)vi * a.P + vox];
         const float cx = a.centers[3 * vox], cy = a.centers[3 * vox + 1], cz = a.centers[3 * vox + 2];
         const float l = a.lengths[vox];
-        s_ch[t] = make_float4(cx, cy, cz, 0.5f * l);
-        s_ml[t] = make_float4(cx - 0.5f * l, cy - 0.5f * l, cz - 0.5f * l, 1.f / l);
         const float4 g0 = reinterpret_cast<const float4*>(a.geos)[2 * vox];
         const float4 g1 = reinterpret_cast<const float4*>(a.geos)[2 * vox + 1];
-        s_g0[t] = g0;
-        s_g1[t] = g1;
+        stage_segment(cx, cy, cz, l, g0, g1, r, s_ch[t], s_ml[t], s_g0[t], s_g1[t]);
         const float* rgbv = a.rgbs + (long long)vi * a.rgb_view_stride;
         s_rgb[t] = make_float4(rgbv[3 * vox], rgbv[3 * vox + 1], rgbv[3 * vox + 2], __uint_as_float(val));
         if (kNormal) {
@@ -221,16 +215,12 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
         }
         float ta = 0.f, tb = 0.f;
         if (act) {
-          const float4 ch = s_ch[j];
-          ray_box(r, ch.x, ch.y, ch.z, ch.w, ta, tb);
+          ray_box_st(r, s_ch[j], s_ml[j], ta, tb);
           act = !(ta > tb);
         }
         if (act) {
-          const float4 ml = s_ml[j];
-          const float4 g0 = s_g0[j], g1 = s_g1[j];
-          const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
           Segment seg;
-          eval_segment(r, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
+          eval_segment_st(r, ta, tb, s_ch[j], s_g0[j], s_g1[j], s, seg);
           const float alpha = seg.alpha;
           act = !(alpha < kMinAlpha);
           if (act) {
@@ -518,14 +508,11 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
               const int j = (int)h - 1;
               const int bit = select_set_bit(s_m[w][j], (int)(w0 + k - (uint32_t)s_pref[w][j]));
               const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
-              const float4 ch = s_ch[j];
+              const float4 box = s_ch[j];
               float ta, tb;
-              ray_box_rel(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
-              const float4 ml = s_ml[j];
-              const float4 g0 = s_g0[j], g1 = s_g1[j];
-              const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+              ray_box_st(pr, box, s_ml[j], ta, tb);
               Segment seg;
-              eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
+              eval_segment_st(pr, ta, tb, box, s_g0[j], s_g1[j], s, seg);
               s_pa[w][k] = seg.alpha;
               if (kDepth) {
                 float sub[3];
@@ -596,14 +583,11 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
               const float dLda = s_pl[w][k];
               const float4 rdd = s_rd[w][bit];
               const Ray pr = pair_ray(r, rdd, s_ri[w][bit]);
-              const float4 ch = s_ch[j];
+              const float4 box = s_ch[j];
               float ta, tb;
-              ray_box_rel(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
-              const float4 ml = s_ml[j];
-              const float4 g0 = s_g0[j], g1 = s_g1[j];
-              const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+              ray_box_st(pr, box, s_ml[j], ta, tb);
               Segment seg;
-              eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
+              eval_segment_st(pr, ta, tb, box, s_g0[j], s_g1[j], s, seg);
               const float alpha = seg.alpha;
               const float ptw = alpha * Tp;
               const float p0v = seg.phi[0], p3v = seg.phi[3];

@@ -4,8 +4,8 @@
 //
 // One 256-thread workgroup per 256-pixel tile (16x16, 32x8 or 8x32), four wavefronts of 64
 // pixels.  The tile's sorted entries are staged into LDS with the per-entry constants the pixel
-// work needs (box centre / half size, box minimum and 1/size, 8 corner SDFs, colour, packed bbox,
-// normalised linear normal).  Two kernels:
+// work needs (box corners relative to the camera and 1/size, corner SDFs with their z differences,
+// colour, packed bbox, normalised linear normal).  Two kernels:
 //   render_fwd_kernel        one entry at a time, wave-uniform, wave w on the 8x8 quadrant w
 //                            (lane l: pixel (l&7, l>>3) of it); 256 entries per round;
 //   render_fwd_pairs_kernel  the default, pair-compacted (below), wave w on an interleaved 8x8
@@ -39,19 +39,16 @@ __device__ inline void stage_entry(const uint32_t* __restrict__ vals, uint32_t e
                                    const float* __restrict__ lengths, const float* __restrict__ geos,
                                    const float* __restrict__ rgbs, bool need_normal,
                                    float4* s_ch, float4* s_ml, float4* s_g0, float4* s_g1,
-                                   float4* s_rgb, float4* s_nrm, uint2* s_bb, int slot,
+                                   float4* s_rgb, float4* s_nrm, uint2* s_bb, int slot, const Ray& r,
                                    bool cull = false, float s = 0.f, float cam_scale = 0.f) {
   const uint32_t val = vals[e];
   const uint32_t vox = val & kVoxMask;
   uint2 bb = bboxes[(long long)view * P + vox];
   const float cx = centers[3 * vox + 0], cy = centers[3 * vox + 1], cz = centers[3 * vox + 2];
   const float l = lengths[vox];
-  s_ch[slot] = make_float4(cx, cy, cz, 0.5f * l);
-  s_ml[slot] = make_float4(cx - 0.5f * l, cy - 0.5f * l, cz - 0.5f * l, 1.f / l);
   const float4 g0 = reinterpret_cast<const float4*>(geos)[2 * vox];
   const float4 g1 = reinterpret_cast<const float4*>(geos)[2 * vox + 1];
-  s_g0[slot] = g0;
-  s_g1[slot] = g1;
+  stage_segment(cx, cy, cz, l, g0, g1, r, s_ch[slot], s_ml[slot], s_g0[slot], s_g1[slot]);
   if (cull) {
     // An entry that cannot reach the minimum alpha gets an empty bbox (no pixel masks).
     const float g[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
@@ -97,10 +94,7 @@ template <bool kCull>
 __device__ inline void store_entry(const EntryRegs& e, bool need_normal, float s, float cam_scale, const Ray& r,
                                    float4* s_ch, float4* s_ml, float4* s_g0, float4* s_g1, float4* s_rgb, float4* s_nrm,
                                    uint2* s_bb, int slot) {
-  s_ch[slot] = make_float4(e.cx - r.ox, e.cy - r.oy, e.cz - r.oz, 0.5f * e.l);   // centre relative to the camera
-  s_ml[slot] = make_float4(e.cx - 0.5f * e.l, e.cy - 0.5f * e.l, e.cz - 0.5f * e.l, 1.f / e.l);
-  s_g0[slot] = e.g0;
-  s_g1[slot] = e.g1;
+  stage_segment(e.cx, e.cy, e.cz, e.l, e.g0, e.g1, r, s_ch[slot], s_ml[slot], s_g0[slot], s_g1[slot]);
   const float g[8] = {e.g0.x, e.g0.y, e.g0.z, e.g0.w, e.g1.x, e.g1.y, e.g1.z, e.g1.w};
   const float ps = cam_scale + fmaxf(fabsf(e.cx), fmaxf(fabsf(e.cy), fabsf(e.cz)));
   s_bb[slot] = (!kCull || voxel_may_contribute(g, s, ps, e.l)) ? e.bb : make_uint2(0xFFFFFFFFu, 0u);
@@ -213,7 +207,7 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
       if (base + t < n) {
         stage_entry(a.vals, rg.x + base + t, vi, a.P, a.bboxes, a.centers, a.lengths, a.geos,
                     a.rgbs + (long long)vi * a.rgb_view_stride, kNormal, s_ch, s_ml, s_g0, s_g1, s_rgb,
-                    s_nrm, s_bb, t);
+                    s_nrm, s_bb, t, r);
         a.cvals[rg.x + base + t] = a.vals[rg.x + base + t];
       }
       __syncthreads();
@@ -226,15 +220,12 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
         const float4 rgb = s_rgb[j];
         const uint32_t val = __float_as_uint(rgb.w);
         if (!pixel_in_bbox(px, py, bb.x, bb.y) || (val >> 29) != r.octant) continue;
-        const float4 ch = s_ch[j];
+        const float4 box = s_ch[j];
         float ta, tb;
-        ray_box(r, ch.x, ch.y, ch.z, ch.w, ta, tb);
+        ray_box_st(r, box, s_ml[j], ta, tb);
         if (ta > tb) continue;
-        const float4 ml = s_ml[j];
-        const float4 g0 = s_g0[j], g1 = s_g1[j];
-        const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
         Segment seg;
-        eval_segment(r, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
+        eval_segment_st(r, ta, tb, box, s_g0[j], s_g1[j], s, seg);
         if (seg.alpha < kMinAlpha) continue;
         const float wgt = T * seg.alpha;
         C0 += wgt * rgb.x; C1 += wgt * rgb.y; C2 += wgt * rgb.z;
@@ -303,7 +294,7 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
 // identical bit for bit.  Each round's contributing entries (and, per contributing entry and 8x8
 // quadrant, the pixels that composited it) are written for the backward.
 // ------------------------------------------------------------------------------------------
-constexpr int kPairWin = 192;     // pair positions held per wave at once
+constexpr int kPairWin = 256;     // pair positions held per wave at once
 
 // Pixels of the wave's 8x8 lattice (bit = 8 * row + column) inside a packed bbox.
 __device__ inline uint64_t bbox_block_mask(uint32_t b0, uint32_t b1, const WaveLattice& L) {
@@ -369,9 +360,7 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
   __shared__ uint16_t s_pref[4][kStage];  // per entry: first position of its run
   __shared__ unsigned long long s_cs[4][2][64];   // per pixel: contributing entries (lo, hi halves)
   __shared__ uint8_t s_own[4][kPairWin];      // 1 + entry whose run starts at the position, else 0
-  __shared__ uint16_t s_hit[4][kPairWin];     // positions whose ray hits the box
-  __shared__ uint16_t s_hd[4][kPairWin];      // their entry | pixel << 7
-  __shared__ float s_pa[4][kPairWin];         // alpha per position (0 when the ray misses the box)
+  __shared__ float s_pa[4][kPairWin];         // alpha per position (read for contributing pairs only)
   __shared__ float s_pd[4][kDepth ? kPairWin : 1];
   __shared__ unsigned long long s_em[kStage][4];      // per staged entry and 8x8 quadrant: pixels that composited it
   __shared__ uint64_t s_uu[kHalves];          // per staging wave: its entries that some pixel composited
@@ -517,57 +506,34 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
           if (c_hi && p_hi >= w0 && p_hi < w1) s_own[w][p_hi - w0] = (uint8_t)(lane + 65);
           wave_lds_sync();
 
-          // 2a. ray/box test per position; hits listed with their (entry, pixel).
-          uint32_t nh = 0;
+          // 2. + 3. per position: ray/box test, alpha (and depth value) of the segment; contributing
+          // pairs into the pixel's set.  Misses are evaluated too (their alpha is not used): about
+          // half of the positions miss, and listing the hits for a second pass cost more than the
+          // segments it saved.
           for (uint32_t c0 = 0; c0 < np; c0 += 64) {
             const uint32_t k = c0 + lane;
             const bool valid = k < np;
             uint32_t h = valid ? (uint32_t)s_own[w][k] : 0u;
             h = max(wave_incl_max_u32(h), carry);
             carry = (uint32_t)__builtin_amdgcn_readlane((int)h, 63);
-            bool hit = false;
-            uint32_t desc = 0;
             if (valid) {
               const int j = (int)h - 1;
               const int bit = select_set_bit(s_m[w][j], (int)(w0 + k - (uint32_t)s_pref[w][j]));
               const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
-              const float4 ch = s_ch[j];
+              const float4 box = s_ch[j];
               float ta, tb;
-              ray_box_rel(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
-              hit = !(ta > tb);
-              desc = (uint32_t)j | ((uint32_t)bit << 7);
-              s_pa[w][k] = 0.f;
+              ray_box_st(pr, box, s_ml[j], ta, tb);
+              const bool hit = !(ta > tb);
+              Segment seg;
+              eval_segment_st(pr, ta, tb, box, s_g0[j], s_g1[j], s, seg);
+              s_pa[w][k] = seg.alpha;
+              if (kDepth) {
+                float sub[3];
+                sub_alphas(seg, sub);
+                s_pd[w][k] = depth_value(sub, ta, tb);
+              }
+              if (hit && seg.alpha >= kMinAlpha) atomicOr(&s_cs[w][j >> 6][bit], 1ull << (j & 63));
             }
-            const uint64_t bal = __ballot(hit);
-            if (hit) {
-              const uint32_t idx = nh + (uint32_t)__popcll(bal & lt_mask);
-              s_hit[w][idx] = (uint16_t)k;
-              s_hd[w][idx] = (uint16_t)desc;
-            }
-            nh += (uint32_t)__popcll(bal);
-          }
-          wave_lds_sync();
-          // 2b. + 3. alpha (and depth value) per hit; contributing pairs into the pixel's set.
-          for (uint32_t h = lane; h < nh; h += 64) {
-            const uint32_t k = s_hit[w][h];
-            const uint32_t d = s_hd[w][h];
-            const int j = (int)(d & 127u), bit = (int)(d >> 7);
-            const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
-            const float4 ch = s_ch[j];
-            float ta, tb;
-            ray_box_rel(pr, ch.x, ch.y, ch.z, ch.w, ta, tb);
-            const float4 ml = s_ml[j];
-            const float4 g0 = s_g0[j], g1 = s_g1[j];
-            const float gg[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
-            Segment seg;
-            eval_segment(pr, ta, tb, ml.x, ml.y, ml.z, ml.w, gg, s, seg);
-            s_pa[w][k] = seg.alpha;
-            if (kDepth) {
-              float sub[3];
-              sub_alphas(seg, sub);
-              s_pd[w][k] = depth_value(sub, ta, tb);
-            }
-            if (seg.alpha >= kMinAlpha) atomicOr(&s_cs[w][j >> 6][bit], 1ull << (j & 63));
           }
           wave_lds_sync();
           // 4. composite this pixel's contributing pairs of the window, in entry order.
