@@ -342,9 +342,10 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
 // composited it.  Those (pixel, entry) pairs are exactly the ones the reverse walk visits: alpha
 // at least the minimum, list position before the pixel's last contributor.  Per staged batch
 // (entries in back-to-front order), each wave:
-//   1. reads its block's pixel masks of the batch's entries and prefix-sums their counts: entry j
-//      owns the run [pref_j, pref_j + |m_j|) of an entry-major pair list; the transposed masks
-//      are each pixel's entry set;
+//   1. reads its block's pixel masks of the batch's entries and prefix-sums their counts: entry j owns
+//      the run [pref_j, pref_j + |m_j|) of an entry-major pair list, cut into windows of at most
+//      128 positions and 64 entries; each pixel's entry set in a window is ORed together in LDS
+//      while step 2 visits the pairs;
 //   2. evaluates alpha (and the depth value) of every pair, 64 per instruction;
 //   3. walks each pixel's set in back-to-front order: T <- T / (1 - a), dL/dalpha and the running
 //      dL/dT -- the only sequential part, a few FMAs per pair;
@@ -354,27 +355,28 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
 // The per-voxel global atomics fall from 12 per (pixel, entry) in the reference to 12 per
 // (tile, entry).
 // ------------------------------------------------------------------------------------------
-constexpr int kBwdWin = 128;
+constexpr int kBwdWin = 512;   // pair positions per window (the four waves together)
 
 template <bool kDepth, bool kNormal, bool kConcen, bool kQueue, int kHalves>
-__global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
+__global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : 4) void render_bwd_pairs_kernel(BwdArgs a) {
   constexpr int kStage = 64 * kHalves;   // entries staged per round (two 64-entry halves or one)
   __shared__ float4 s_ch[kStage], s_ml[kStage], s_g0[kStage], s_g1[kStage], s_rgb[kStage];
   __shared__ float4 s_nrm[kNormal ? kStage : 1];
   __shared__ float s_acc[kStage][12];
   __shared__ uint32_t s_used[kStage];
   __shared__ float4 s_rd[4][64];           // ray direction xyz, dL/ddepth
-  __shared__ float4 s_ri[4][64];           // 1/direction xyz
-  __shared__ float4 s_dc[4][64];           // dL/dcolor xyz
+  __shared__ float4 s_ri[4][64];           // 1/direction xyz, dL/dcolor x
+  __shared__ float2 s_dc[4][64];           // dL/dcolor y, z
+  __shared__ unsigned long long s_cs[4][64];   // per pixel: its entries (relative to the window base) in the window
   __shared__ float4 s_dn[4][kNormal ? 64 : 1];
   __shared__ float4 s_gt[4][kConcen ? 64 : 1];
-  __shared__ uint64_t s_m[4][kStage];      // per entry: the block's pixels it contributed to
-  __shared__ uint16_t s_pref[4][kStage];   // per entry: first position of its run
-  __shared__ uint8_t s_own[4][kBwdWin];    // 1 + entry whose run starts at the position, else 0
-  __shared__ uint16_t s_hd[4][kBwdWin];    // entry | pixel << 7 per position
-  __shared__ float s_pa[4][kBwdWin];       // alpha per position; after the walk, T before the entry
-  __shared__ float s_pd[4][kDepth ? kBwdWin : 1];
-  __shared__ float s_pl[4][kBwdWin];       // dL/dalpha per position
+  __shared__ uint64_t s_m[4][kStage];      // per block and entry: the block's pixels it contributed to
+  __shared__ uint16_t s_pref[kStage + 1];  // per entry: first position of its run; [kStage] = total
+  __shared__ uint32_t s_qo[kStage];        // per entry: offsets of blocks 1, 2, 3 in its run (8 bits each)
+  __shared__ uint16_t s_hd[kBwdWin];       // entry | block << 7 | pixel << 9 per window position
+  __shared__ float s_pa[kBwdWin];          // alpha per position; after the walk, T before the entry
+  __shared__ float s_pd[kDepth ? kBwdWin : 1];
+  __shared__ float s_pl[kBwdWin];          // dL/dalpha per position
   __shared__ float2 s_red[4];
   __shared__ int s_tile;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -434,8 +436,8 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
     }
     const float wR = kConcen ? a.lambda_R_concen / (float)(W * H) : 0.f;
     s_rd[w][lane] = make_float4(r.dx, r.dy, r.dz, dD);
-    s_ri[w][lane] = make_float4(r.ix, r.iy, r.iz, 0.f);
-    s_dc[w][lane] = make_float4(dC0, dC1, dC2, 0.f);
+    s_ri[w][lane] = make_float4(r.ix, r.iy, r.iz, dC0);
+    s_dc[w][lane] = make_float2(dC1, dC2);
     if (kNormal) s_dn[w][lane] = make_float4(dN0, dN1, dN2, 0.f);
     if (kConcen) s_gt[w][lane] = make_float4(gt0, gt1, gt2, 0.f);
 
@@ -459,208 +461,235 @@ __global__ __launch_bounds__(256) void render_bwd_pairs_kernel(BwdArgs a) {
         s_used[t] = 0u;
       }
       for (int k = t; k < kStage * 12; k += 256) (&s_acc[0][0])[k] = 0.f;
+      // This wave's block masks of the batch (loaded during the previous batch).
+      s_m[w][lane] = nm_lo;
+      if (kHalves > 1) s_m[w][lane + 64] = nm_hi;
       __syncthreads();
       if (t < kStage) {
         if (base + kStage + t < n) fetch_entry_b(er, a, nval, rgbs_v);
         if (base + 2 * kStage + t < n) nval = a.cvals[top - (base + 2 * kStage + t)];
       }
-      const uint64_t m_lo = nm_lo, m_hi = nm_hi;
       {
         const int nb = base + kStage;
         nm_lo = nb + lane < n ? a.pmask[4 * (top - nb - lane) + w] : 0ull;
         nm_hi = (kHalves > 1 && nb + 64 + lane < n) ? a.pmask[4 * (top - nb - 64 - lane) + w] : 0ull;
       }
-      const uint32_t c_lo = (uint32_t)__popcll(m_lo), c_hi = (uint32_t)__popcll(m_hi);
-      if (__ballot((c_lo | c_hi) != 0u) != 0ull) {
+      // 1. The batch's pair list, shared by the four waves: entry e owns the positions
+      // [pref_e, pref_e + c_e), its block-0 pairs first, then blocks 1, 2, 3, pixels ascending.
+      if (w == 0) {
+        uint32_t c_lo, qo_lo, c_hi = 0u, qo_hi = 0u;
+        {
+          const uint32_t a0 = (uint32_t)__popcll(s_m[0][lane]), a1 = (uint32_t)__popcll(s_m[1][lane]);
+          const uint32_t a2 = (uint32_t)__popcll(s_m[2][lane]), a3 = (uint32_t)__popcll(s_m[3][lane]);
+          qo_lo = a0 | ((a0 + a1) << 8) | ((a0 + a1 + a2) << 16);
+          c_lo = a0 + a1 + a2 + a3;
+        }
+        if (kHalves > 1) {
+          const uint32_t a0 = (uint32_t)__popcll(s_m[0][lane + 64]), a1 = (uint32_t)__popcll(s_m[1][lane + 64]);
+          const uint32_t a2 = (uint32_t)__popcll(s_m[2][lane + 64]), a3 = (uint32_t)__popcll(s_m[3][lane + 64]);
+          qo_hi = a0 | ((a0 + a1) << 8) | ((a0 + a1 + a2) << 16);
+          c_hi = a0 + a1 + a2 + a3;
+        }
         const uint32_t i_lo = wave_incl_scan_u32(c_lo);
         const uint32_t i_hi = kHalves > 1 ? wave_incl_scan_u32(c_hi) : 0u;
         const uint32_t t_lo = (uint32_t)__builtin_amdgcn_readlane((int)i_lo, 63);
-        const uint32_t total = t_lo + (uint32_t)__builtin_amdgcn_readlane((int)i_hi, 63);
-        const uint32_t p_lo = i_lo - c_lo, p_hi = t_lo + i_hi - c_hi;
-        s_m[w][lane] = m_lo;
-        s_pref[w][lane] = (uint16_t)p_lo;
+        s_pref[lane] = (uint16_t)(i_lo - c_lo);
+        s_qo[lane] = qo_lo;
         if (kHalves > 1) {
-          s_m[w][lane + 64] = m_hi;
-          s_pref[w][lane + 64] = (uint16_t)p_hi;
+          s_pref[lane + 64] = (uint16_t)(t_lo + i_hi - c_hi);
+          s_qo[lane + 64] = qo_hi;
         }
-        // This pixel's entries of the batch (ascending slot = back to front along the list).
-        uint64_t cs_lo = transpose64(m_lo, lane);
-        uint64_t cs_hi = kHalves > 1 ? transpose64(m_hi, lane) : 0ull;
-        uint32_t carry = 0;   // 1 + entry owning the last position handled so far
+        if (lane == 63) s_pref[kStage] = (uint16_t)(t_lo + i_hi);
+      }
+      __syncthreads();
+      const uint32_t total = s_pref[kStage];
+      // A window holds at most kBwdWin positions of at most 64 consecutive entries [ja, ja + 64),
+      // so each pixel's entries in it form one 64-bit set (ORed together while the alphas are
+      // evaluated).  With 128 entries per batch, ja is 0 or 64.  The window bounds come from
+      // shared data, so every wave takes the same windows and meets the same barriers.
+      int ja = 0;
+      const int qsh = 8 * (w - 1);   // this block's offset field in s_qo (w > 0)
+      for (uint32_t w0 = 0, w1 = 0; w0 < total; w0 = w1) {
+        if (kHalves > 1 && ja == 0 && (uint32_t)s_pref[64] <= w0) ja = 64;
+        const uint32_t lim = (kHalves > 1 && ja == 0) ? (uint32_t)s_pref[64] : total;
+        w1 = min(min(total, lim), w0 + (uint32_t)kBwdWin);
+        const uint32_t np = w1 - w0;
+        s_cs[w][lane] = 0ull;
+        __syncthreads();
 
-        for (uint32_t w0 = 0; w0 < total; w0 += kBwdWin) {
-          const uint32_t w1 = min(total, w0 + (uint32_t)kBwdWin);
-          const uint32_t np = w1 - w0;
-          for (uint32_t k = lane; k < np; k += 64) s_own[w][k] = 0;
-          wave_lds_sync_b();
-          if (c_lo && p_lo >= w0 && p_lo < w1) s_own[w][p_lo - w0] = (uint8_t)(lane + 1);
-          if (c_hi && p_hi >= w0 && p_hi < w1) s_own[w][p_hi - w0] = (uint8_t)(lane + 65);
-          wave_lds_sync_b();
-
-          // 2. alpha (and depth value) per position.
-          for (uint32_t c0 = 0; c0 < np; c0 += 64) {
-            const uint32_t k = c0 + lane;
-            const bool valid = k < np;
-            uint32_t h = valid ? (uint32_t)s_own[w][k] : 0u;
-            h = max(wave_incl_max_u32(h), carry);
-            carry = (uint32_t)__builtin_amdgcn_readlane((int)h, 63);
-            if (valid) {
-              const int j = (int)h - 1;
-              const int bit = select_set_bit(s_m[w][j], (int)(w0 + k - (uint32_t)s_pref[w][j]));
-              const Ray pr = pair_ray(r, s_rd[w][bit], s_ri[w][bit]);
-              const float4 box = s_ch[j];
-              float ta, tb;
-              ray_box_st(pr, box, s_ml[j], ta, tb);
-              Segment seg;
-              eval_segment_st(pr, ta, tb, box, s_g0[j], s_g1[j], s, seg);
-              s_pa[w][k] = seg.alpha;
-              if (kDepth) {
-                float sub[3];
-                sub_alphas(seg, sub);
-                s_pd[w][k] = depth_value(sub, ta, tb);
-              }
-              s_hd[w][k] = (uint16_t)((uint32_t)j | ((uint32_t)bit << 7));
-            }
-          }
-          wave_lds_sync_b();
-
-          // 3. sequential reverse walk per pixel over its entries whose pair lies in the window,
-          // software-pipelined: the next entry's position is looked up while this one is walked.
-          int jc = -1;
-          uint32_t kc = 0;
-          if ((cs_lo | cs_hi) != 0ull) {
-            jc = cs_lo ? __builtin_ctzll(cs_lo) : 64 + __builtin_ctzll(cs_hi);
-            kc = (uint32_t)s_pref[w][jc] + (uint32_t)__popcll(s_m[w][jc] & lt_mask);
-          }
-          while (jc >= 0 && kc < w1) {
-            const int j = jc;
-            const uint32_t kk = kc - w0;
-            if (cs_lo) cs_lo &= cs_lo - 1ull; else cs_hi &= cs_hi - 1ull;
-            const float alpha = s_pa[w][kk];
-            const float4 rgb = s_rgb[j];
-            jc = -1;
-            if ((cs_lo | cs_hi) != 0ull) {
-              jc = cs_lo ? __builtin_ctzll(cs_lo) : 64 + __builtin_ctzll(cs_hi);
-              kc = (uint32_t)s_pref[w][jc] + (uint32_t)__popcll(s_m[w][jc] & lt_mask);
-            }
-            T = T / (1.f - alpha);
-            float dptw = dC0 * rgb.x + dC1 * rgb.y + dC2 * rgb.z;
-            if (kNormal) {
-              const float4 nn = s_nrm[j];
-              dptw += dN0 * nn.x + dN1 * nn.y + dN2 * nn.z;
-            }
-            const float dLda = T * (dptw - last_dT);
-            last_dT += alpha * (dptw - last_dT);
-            if (kDepth) last_dT += dD * s_pd[w][kk];
-            s_pl[w][kk] = dLda;
-            s_pa[w][kk] = T;
-          }
-          wave_lds_sync_b();
-
-          // 4. per-pair derivatives, 64 pairs per instruction; each entry's run is summed across
-          // lanes (segmented DPP sum) and its last lane adds the 12 sums to the workgroup's LDS
-          // accumulators.
-          uint32_t jprev = 0xFFFFFFFFu;   // entry of the previous chunk's last pair
-          for (uint32_t c0 = 0; c0 < np; c0 += 64) {
-            const uint32_t k = c0 + lane;
-            const bool valid = k < np;
-            const uint32_t d = valid ? (uint32_t)s_hd[w][k] : 0u;
-            const uint32_t jj = valid ? (d & 127u) : 0xFFFFFFFEu;
-            uint32_t jp = (uint32_t)__shfl_up((int)jj, 1, 64);
-            if (lane == 0) jp = jprev;
-            const uint32_t jn = (uint32_t)__shfl_down((int)jj, 1, 64);
-            jprev = (uint32_t)__builtin_amdgcn_readlane((int)jj, 63);
-            const bool head = jj != jp;
-            const int seg_start = (int)wave_incl_max_u32(head ? (uint32_t)lane : 0u);
-            const bool tail = valid && (lane == 63 || k + 1 == np || jn != jj);
-            float gv[12];
+        // 2. alpha (and depth value) per position, 64-position chunks dealt to the waves in turn:
+        // the owning entry (binary search of the run starts), its block and pixel.
+        for (uint32_t c0 = 64u * (uint32_t)w; c0 < np; c0 += 256u) {
+          const uint32_t k = c0 + lane;
+          if (k < np) {
+            const uint32_t p = w0 + k;
+            int j = ja;
 #pragma unroll
-            for (int q = 0; q < 12; ++q) gv[q] = 0.f;
-            const int j = (int)(d & 127u);
-            if (valid) {
-              const int bit = (int)(d >> 7);
-              const float Tp = s_pa[w][k];
-              const float dLda = s_pl[w][k];
-              const float4 rdd = s_rd[w][bit];
-              const Ray pr = pair_ray(r, rdd, s_ri[w][bit]);
-              const float4 box = s_ch[j];
-              float ta, tb;
-              ray_box_st(pr, box, s_ml[j], ta, tb);
-              Segment seg;
-              eval_segment_st(pr, ta, tb, box, s_g0[j], s_g1[j], s, seg);
-              const float alpha = seg.alpha;
-              const float ptw = alpha * Tp;
-              const float p0v = seg.phi[0], p3v = seg.phi[3];
-              const float rden = 1.f / (p0v + kPhiEps);
-              const float q0 = p0v * (1.f - p0v), q3 = p3v * (1.f - p3v);
-              float c[4];
-              c[0] = dLda * s * q0 * (1.f - alpha) * rden;
-              c[1] = 0.f;
-              c[2] = 0.f;
-              c[3] = -dLda * s * q3 * rden;
-              comp_add(ds_hi, ds_lo, dLda * (q0 * seg.sdf[0] * (1.f - alpha) - q3 * seg.sdf[3]) * rden);
-              if (kDepth) {
-                float sub[3];
-                sub_alphas(seg, sub);
-                const float step = kThird * (tb - ta);
-                const float t0 = ta + 0.5f * step, t1 = ta + 1.5f * step, t2 = ta + 2.5f * step;
-                const float a0 = sub[0], a1 = sub[1], a2 = sub[2];
-                const float kd = rdd.w * Tp;
-                const float gk[3] = {kd * (t0 + a1 * a2 * t2 - a1 * t1 - a2 * t2),
-                                     kd * (t1 + a0 * a2 * t2 - a0 * t1 - a2 * t2),
-                                     kd * (t2 + a0 * a1 * t2 - a0 * t2 - a1 * t2)};
+            for (int st = 32; st >= 1; st >>= 1)
+              if ((uint32_t)s_pref[j + st] <= p) j += st;
+            const uint32_t rr = p - (uint32_t)s_pref[j];
+            const uint32_t qo = s_qo[j];
+            const uint32_t o1 = qo & 255u, o2 = (qo >> 8) & 255u, o3 = qo >> 16;
+            const int q = (int)(rr >= o1) + (int)(rr >= o2) + (int)(rr >= o3);
+            const uint32_t off = q == 0 ? 0u : (q == 1 ? o1 : (q == 2 ? o2 : o3));
+            const int bit = select_set_bit(s_m[q][j], (int)(rr - off));
+            const Ray pr = pair_ray(r, s_rd[q][bit], s_ri[q][bit]);
+            const float4 box = s_ch[j];
+            float ta, tb;
+            ray_box_st(pr, box, s_ml[j], ta, tb);
+            Segment seg;
+            eval_segment_st(pr, ta, tb, box, s_g0[j], s_g1[j], s, seg);
+            s_pa[k] = seg.alpha;
+            if (kDepth) {
+              float sub[3];
+              sub_alphas(seg, sub);
+              s_pd[k] = depth_value(sub, ta, tb);
+            }
+            s_hd[k] = (uint16_t)((uint32_t)j | ((uint32_t)q << 7) | ((uint32_t)bit << 9));
+            atomicOr(&s_cs[q][bit], 1ull << (j - ja));
+          }
+        }
+        __syncthreads();
+
+        // 3. sequential reverse walk per pixel (wave w: block w) over its entries in the window
+        // (ascending slot = back to front along the list), software-pipelined: the next entry's
+        // position is looked up while this one is walked.
+        uint64_t cs = s_cs[w][lane];
+        int jc = -1;
+        uint32_t kc = 0;
+        if (cs != 0ull) {
+          jc = ja + __builtin_ctzll(cs);
+          kc = (uint32_t)s_pref[jc] + (w == 0 ? 0u : ((s_qo[jc] >> qsh) & 255u)) +
+               (uint32_t)__popcll(s_m[w][jc] & lt_mask);
+        }
+        while (jc >= 0) {
+          const int j = jc;
+          const uint32_t kk = kc - w0;
+          cs &= cs - 1ull;
+          const float alpha = s_pa[kk];
+          const float4 rgb = s_rgb[j];
+          jc = -1;
+          if (cs != 0ull) {
+            jc = ja + __builtin_ctzll(cs);
+            kc = (uint32_t)s_pref[jc] + (w == 0 ? 0u : ((s_qo[jc] >> qsh) & 255u)) +
+                 (uint32_t)__popcll(s_m[w][jc] & lt_mask);
+          }
+          T = T / (1.f - alpha);
+          float dptw = dC0 * rgb.x + dC1 * rgb.y + dC2 * rgb.z;
+          if (kNormal) {
+            const float4 nn = s_nrm[j];
+            dptw += dN0 * nn.x + dN1 * nn.y + dN2 * nn.z;
+          }
+          const float dLda = T * (dptw - last_dT);
+          last_dT += alpha * (dptw - last_dT);
+          if (kDepth) last_dT += dD * s_pd[kk];
+          s_pl[kk] = dLda;
+          s_pa[kk] = T;
+        }
+        __syncthreads();
+
+        // 4. per-pair derivatives, chunks dealt to the waves in turn; each entry's run (all its
+        // blocks) is summed across lanes (segmented DPP sum) and its last lane in the chunk adds
+        // the 12 sums to the workgroup's LDS accumulators.
+        for (uint32_t c0 = 64u * (uint32_t)w; c0 < np; c0 += 256u) {
+          const uint32_t k = c0 + lane;
+          const bool valid = k < np;
+          const uint32_t d = valid ? (uint32_t)s_hd[k] : 0u;
+          const uint32_t jj = valid ? (d & 127u) : 0xFFFFFFFEu;
+          uint32_t jp = (uint32_t)__shfl_up((int)jj, 1, 64);
+          if (lane == 0) jp = 0xFFFFFFFFu;
+          const uint32_t jn = (uint32_t)__shfl_down((int)jj, 1, 64);
+          const bool head = jj != jp;
+          const int seg_start = (int)wave_incl_max_u32(head ? (uint32_t)lane : 0u);
+          const bool tail = valid && (lane == 63 || k + 1 == np || jn != jj);
+          float gv[12];
 #pragma unroll
-                for (int mm = 0; mm < 3; ++mm) {
-                  const float pk = seg.phi[mm], pk1 = seg.phi[mm + 1];
-                  const float dk = 1.f / (pk + kPhiEps);
-                  if ((pk - pk1) * dk > 0.f) {
-                    c[mm] += gk[mm] * pk1 * dk * dk * s * pk * (1.f - pk);
-                    c[mm + 1] += -gk[mm] * dk * s * pk1 * (1.f - pk1);
-                  }
+          for (int q = 0; q < 12; ++q) gv[q] = 0.f;
+          const int j = (int)(d & 127u);
+          if (valid) {
+            const int qb = (int)((d >> 7) & 3u), bit = (int)(d >> 9);
+            const float Tp = s_pa[k];
+            const float dLda = s_pl[k];
+            const float4 rdd = s_rd[qb][bit];
+            const float4 rid = s_ri[qb][bit];
+            const Ray pr = pair_ray(r, rdd, rid);
+            const float4 box = s_ch[j];
+            float ta, tb;
+            ray_box_st(pr, box, s_ml[j], ta, tb);
+            Segment seg;
+            eval_segment_st(pr, ta, tb, box, s_g0[j], s_g1[j], s, seg);
+            const float alpha = seg.alpha;
+            const float ptw = alpha * Tp;
+            const float p0v = seg.phi[0], p3v = seg.phi[3];
+            const float rden = 1.f / (p0v + kPhiEps);
+            const float q0 = p0v * (1.f - p0v), q3 = p3v * (1.f - p3v);
+            float c[4];
+            c[0] = dLda * s * q0 * (1.f - alpha) * rden;
+            c[1] = 0.f;
+            c[2] = 0.f;
+            c[3] = -dLda * s * q3 * rden;
+            comp_add(ds_hi, ds_lo, dLda * (q0 * seg.sdf[0] * (1.f - alpha) - q3 * seg.sdf[3]) * rden);
+            if (kDepth) {
+              float sub[3];
+              sub_alphas(seg, sub);
+              const float step = kThird * (tb - ta);
+              const float t0 = ta + 0.5f * step, t1 = ta + 1.5f * step, t2 = ta + 2.5f * step;
+              const float a0 = sub[0], a1 = sub[1], a2 = sub[2];
+              const float kd = rdd.w * Tp;
+              const float gk[3] = {kd * (t0 + a1 * a2 * t2 - a1 * t1 - a2 * t2),
+                                   kd * (t1 + a0 * a2 * t2 - a0 * t1 - a2 * t2),
+                                   kd * (t2 + a0 * a1 * t2 - a0 * t2 - a1 * t2)};
+#pragma unroll
+              for (int mm = 0; mm < 3; ++mm) {
+                const float pk = seg.phi[mm], pk1 = seg.phi[mm + 1];
+                const float dk = 1.f / (pk + kPhiEps);
+                if ((pk - pk1) * dk > 0.f) {
+                  c[mm] += gk[mm] * pk1 * dk * dk * s * pk * (1.f - pk);
+                  c[mm + 1] += -gk[mm] * dk * s * pk1 * (1.f - pk1);
                 }
               }
-#pragma unroll
-              for (int mm = 0; mm < 4; ++mm) {
-                if (!kDepth && (mm == 1 || mm == 2)) continue;   // c[1] = c[2] = 0 without depth
-                const float fm = (float)mm;
-                float wts[8];
-                trilinear_weights(seg.qx + seg.sx * fm, seg.qy + seg.sy * fm, seg.qz + seg.sz * fm, wts);
-#pragma unroll
-                for (int i = 0; i < 8; ++i) gv[i] += c[mm] * wts[i];
-              }
-              const float4 dcv = s_dc[w][bit];
-              if (kNormal) {
-                const float4 nn = s_nrm[j];
-                const float4 dnv = s_dn[w][bit];
-                const float sx = ptw * dnv.x, sy = ptw * dnv.y, sz = ptw * dnv.z;
-                const float pr2 = sx * nn.x + sy * nn.y + sz * nn.z;
-                const float lx = nn.w * (sx - pr2 * nn.x);
-                const float ly = nn.w * (sy - pr2 * nn.y);
-                const float lz = nn.w * (sz - pr2 * nn.z);
-#pragma unroll
-                for (int i = 0; i < 8; ++i)
-                  gv[i] += ((i & 4) ? lx : -lx) + ((i & 2) ? ly : -ly) + ((i & 1) ? lz : -lz);
-              }
-              gv[8] = ptw * dcv.x;
-              gv[9] = ptw * dcv.y;
-              gv[10] = ptw * dcv.z;
-              if (kConcen) {
-                const float4 gtv = s_gt[w][bit];
-                const float4 rgb = s_rgb[j];
-                gv[8] += wR * ptw * 2.f * (rgb.x - gtv.x);
-                gv[9] += wR * ptw * 2.f * (rgb.y - gtv.y);
-                gv[10] += wR * ptw * 2.f * (rgb.z - gtv.z);
-              }
-              gv[11] = fabsf(dLda * alpha);
             }
-            wave_segmented_sum<12>(gv, lane, seg_start);
-            if (tail) {
 #pragma unroll
-              for (int q = 0; q < 12; ++q) atomicAdd(&s_acc[j][q], gv[q]);
-              s_used[j] = 1u;
+            for (int mm = 0; mm < 4; ++mm) {
+              if (!kDepth && (mm == 1 || mm == 2)) continue;   // c[1] = c[2] = 0 without depth
+              const float fm = (float)mm;
+              float wts[8];
+              trilinear_weights(seg.qx + seg.sx * fm, seg.qy + seg.sy * fm, seg.qz + seg.sz * fm, wts);
+#pragma unroll
+              for (int i = 0; i < 8; ++i) gv[i] += c[mm] * wts[i];
             }
+            const float2 dcyz = s_dc[qb][bit];
+            const float4 dcv = make_float4(rid.w, dcyz.x, dcyz.y, 0.f);
+            if (kNormal) {
+              const float4 nn = s_nrm[j];
+              const float4 dnv = s_dn[qb][bit];
+              const float sx = ptw * dnv.x, sy = ptw * dnv.y, sz = ptw * dnv.z;
+              const float pr2 = sx * nn.x + sy * nn.y + sz * nn.z;
+              const float lx = nn.w * (sx - pr2 * nn.x);
+              const float ly = nn.w * (sy - pr2 * nn.y);
+              const float lz = nn.w * (sz - pr2 * nn.z);
+#pragma unroll
+              for (int i = 0; i < 8; ++i)
+                gv[i] += ((i & 4) ? lx : -lx) + ((i & 2) ? ly : -ly) + ((i & 1) ? lz : -lz);
+            }
+            gv[8] = ptw * dcv.x;
+            gv[9] = ptw * dcv.y;
+            gv[10] = ptw * dcv.z;
+            if (kConcen) {
+              const float4 gtv = s_gt[qb][bit];
+              const float4 rgb = s_rgb[j];
+              gv[8] += wR * ptw * 2.f * (rgb.x - gtv.x);
+              gv[9] += wR * ptw * 2.f * (rgb.y - gtv.y);
+              gv[10] += wR * ptw * 2.f * (rgb.z - gtv.z);
+            }
+            gv[11] = fabsf(dLda * alpha);
           }
-          wave_lds_sync_b();
+          wave_segmented_sum<12>(gv, lane, seg_start);
+          if (tail) {
+#pragma unroll
+            for (int q = 0; q < 12; ++q) atomicAdd(&s_acc[j][q], gv[q]);
+            s_used[j] = 1u;
+          }
         }
       }
       __syncthreads();
