@@ -801,6 +801,9 @@ int svr_batch_setup(void* ws, int n_views, int P, long long n_pixels, long long 
     d.tile_base = (int)tile_base;
     d.pix_base = pix_base;
     SVR_TRY(launch_init_view(bp.g.views + v, d, w2c + 16 * v, c2w + 16 * v, background, st));
+    if (tile_base + (long long)d.tiles_x * d.tiles_y > n_tiles)
+      return fail(1, "tile/pixel totals do not match the views%s (tiles over %lld)", "", n_tiles);
+    SVR_TRY(launch_fill_u32(bp.im.tile_view + tile_base, (long long)d.tiles_x * d.tiles_y, (uint32_t)v, st));
     tile_base += (long long)d.tiles_x * d.tiles_y;
     pix_base += (long long)W[v] * H[v];
   }
@@ -902,6 +905,7 @@ int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long l
   a.tile_last = bp.im.tile_last; a.n_contrib = bp.im.n_contrib;
   a.color = out_color; a.depth = out_depth; a.normal = out_normal; a.T = out_T; a.max_w = nullptr;
   a.work_counter = bp.im.misc;
+  a.tile_view = bp.im.tile_view;
   a.cvals = bp.c.cvals; a.pmask = bp.c.pmask; a.cranges = bp.im.cranges;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;
@@ -943,6 +947,7 @@ int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long lo
   a.dL_dvox = dL_dvox; a.dL_drgb_views = dL_drgb_views; a.dL_ds = dL_ds; a.ds_part = bp.im.ds_part;
   a.vox_key = vox_key; a.dL_dgrid = dL_dgrid_pts;
   a.work_counter = bp.im.misc + 1;
+  a.tile_view = bp.im.tile_view;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;
   a.halves = g_batch_depth / 64;

@@ -378,7 +378,7 @@ __global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : 4) void ren
   __shared__ float s_pd[kDepth ? kBwdWin : 1];
   __shared__ float s_pl[kBwdWin];          // dL/dalpha per position
   __shared__ float2 s_red[4];
-  __shared__ int s_tile;
+  __shared__ int s_tile[2];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const float s = *a.s_val;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -387,16 +387,17 @@ __global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : 4) void ren
   for (int iter = 0;; ++iter) {
     int g;
     if (kQueue) {
-      if (t == 0) s_tile = (int)atomicAdd(a.work_counter, 1u);
+      // Two slots by iteration parity: a slot is rewritten two tiles later, after every wave has
+      // passed the barriers of the tile in between, so one barrier per tile suffices.
+      if (t == 0) s_tile[iter & 1] = (int)atomicAdd(a.work_counter, 1u);
       __syncthreads();
-      g = __builtin_amdgcn_readfirstlane(s_tile);   // uniform: tile and view data in scalar registers
-      __syncthreads();
+      g = __builtin_amdgcn_readfirstlane(s_tile[iter & 1]);   // uniform: tile and view data in scalar registers
       if (g >= a.total_tiles) break;
     } else {
       if (iter > 0) break;
       g = blockIdx.x;
     }
-    const int vi = find_view_b(a.views, a.n_views, g);
+    const int vi = kQueue ? (int)a.tile_view[g] : find_view_b(a.views, a.n_views, g);
     const ViewDesc& V = a.views[vi];
     const int W = V.W, H = V.H;
     const int lt = g - V.tile_base;

@@ -323,18 +323,6 @@ __device__ inline uint64_t entry_block_mask(const uint2* s_bb, const float4* s_r
   return bbox_block_mask(bb.x, bb.y, L) & live & s_om[q];
 }
 
-// OR of a 64-bit value over the wave.
-__device__ inline uint64_t wave_or_u64(uint64_t v, int lane) {
-  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-  lo |= xor_lane<32>(lo, lane); hi |= xor_lane<32>(hi, lane);
-  lo |= xor_lane<16>(lo, lane); hi |= xor_lane<16>(hi, lane);
-  lo |= xor_lane<8>(lo, lane);  hi |= xor_lane<8>(hi, lane);
-  lo |= xor_lane<4>(lo, lane);  hi |= xor_lane<4>(hi, lane);
-  lo |= xor_lane<2>(lo, lane);  hi |= xor_lane<2>(hi, lane);
-  lo |= xor_lane<1>(lo, lane);  hi |= xor_lane<1>(hi, lane);
-  return ((uint64_t)hi << 32) | lo;
-}
-
 // Pops the lowest entry of a 128-bit entry set.
 __device__ inline int pop_entry(uint64_t& lo, uint64_t& hi) {
   if (lo) {
@@ -365,7 +353,7 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
   __shared__ unsigned long long s_em[kStage][4];      // per staged entry and 8x8 quadrant: pixels that composited it
   __shared__ uint64_t s_uu[kHalves];          // per staging wave: its entries that some pixel composited
   __shared__ uint32_t s_red[4];
-  __shared__ int s_tile;
+  __shared__ int s_tile[2];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const float s = *a.s_val;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -373,16 +361,17 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
   for (int iter = 0;; ++iter) {
     int g;
     if (kQueue) {
-      if (t == 0) s_tile = (int)atomicAdd(a.work_counter, 1u);
+      // Two slots by iteration parity: a slot is rewritten two tiles later, after every wave has
+      // passed the barriers of the tile in between, so one barrier per tile suffices.
+      if (t == 0) s_tile[iter & 1] = (int)atomicAdd(a.work_counter, 1u);
       __syncthreads();
-      g = __builtin_amdgcn_readfirstlane(s_tile);   // uniform: tile and view data in scalar registers
-      __syncthreads();
+      g = __builtin_amdgcn_readfirstlane(s_tile[iter & 1]);   // uniform: tile and view data in scalar registers
       if (g >= a.total_tiles) return;
     } else {
       if (iter > 0) return;
       g = blockIdx.x;
     }
-    const int vi = find_view(a.views, a.n_views, g);
+    const int vi = kQueue ? (int)a.tile_view[g] : find_view(a.views, a.n_views, g);
     const ViewDesc& V = a.views[vi];
     const int W = V.W, H = V.H;
     const int lt = g - V.tile_base;

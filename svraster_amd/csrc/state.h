@@ -135,7 +135,7 @@ inline size_t raster_bin_bytes(long long K) { return BinLayout(K).total + Contri
 
 // Image state.
 struct ImageLayout {
-  size_t views, misc, ranges, tile_last, n_contrib, cranges, ds_part, total;
+  size_t views, misc, ranges, tile_last, n_contrib, cranges, ds_part, tile_view, total;
   ImageLayout(long long n_views, long long n_pix, long long n_tiles) {
     Carver c;
     views = c.take<ViewDesc>(n_views);
@@ -145,6 +145,7 @@ struct ImageLayout {
     n_contrib = c.take<uint32_t>(n_pix);
     cranges = c.take<uint2>(n_tiles);
     ds_part = c.take<float2>(n_tiles);
+    tile_view = c.take<uint32_t>(n_tiles);
     total = c.off;
   }
 };
@@ -157,6 +158,7 @@ struct ImagePtrs {
   uint32_t* n_contrib;
   uint2* cranges;      // per tile: its contributing entries [x, y) in the contributing-entry arrays
   float2* ds_part;     // per backward workgroup: its dL/ds partial sum as an unevaluated (hi, lo) pair
+  uint32_t* tile_view; // per tile: its view (filled at multi-view set-up)
   ImagePtrs(void* base, long long n_views, long long n_pix, long long n_tiles) {
     ImageLayout L(n_views, n_pix, n_tiles);
     char* b = (char*)base;
@@ -167,6 +169,7 @@ struct ImagePtrs {
     n_contrib = (uint32_t*)(b + L.n_contrib);
     cranges = (uint2*)(b + L.cranges);
     ds_part = (float2*)(b + L.ds_part);
+    tile_view = (uint32_t*)(b + L.tile_view);
   }
 };
 
@@ -256,6 +259,7 @@ struct FwdArgs {
   uint32_t* tile_last; uint32_t* n_contrib;
   float* color; float* depth; float* normal; float* T; float* max_w;
   uint32_t* work_counter;          // null: one workgroup per tile
+  const uint32_t* tile_view = nullptr;   // per tile: its view (work-queue launches)
   uint32_t* cvals; uint64_t* pmask; uint2* cranges;   // contributing entries (written)
   int need_depth, need_normal;
   int variant;                     // 0: pair-compacted (default), 1: one entry at a time
@@ -282,6 +286,7 @@ struct BwdArgs {
   // flush; null: into dL_dvox slots 0-7).
   const int64_t* vox_key; float* dL_dgrid;
   uint32_t* work_counter;
+  const uint32_t* tile_view = nullptr;   // per tile: its view (work-queue launches)
   // Contributing entries per tile and their pixel masks, as the forward wrote them.
   const uint32_t* cvals; const uint64_t* pmask; const uint2* cranges;
   int need_depth, need_normal;
