@@ -302,11 +302,10 @@ __device__ inline uint64_t bbox_block_mask(uint32_t b0, uint32_t b1, const WaveL
   const int x0 = max(((int)(b0 >> 16) - L.ox + mx) >> L.kx, 0), x1 = min(((int)(b1 >> 16) - L.ox) >> L.kx, 7);
   const int y0 = max(((int)(b0 & 0xFFFFu) - L.oy + my) >> L.ky, 0), y1 = min(((int)(b1 & 0xFFFFu) - L.oy) >> L.ky, 7);
   if (x0 > x1 || y0 > y1) return 0ull;
-  const uint64_t cols = (uint64_t)(((1u << (x1 + 1)) - 1u) & ~((1u << x0) - 1u));
-  const uint64_t hi = (y1 == 7) ? ~0ull : ((1ull << (8 * (y1 + 1))) - 1ull);
-  const uint64_t lo = (1ull << (8 * y0)) - 1ull;
-  const uint64_t rows = 0x0101010101010101ull & hi & ~lo;
-  return cols * rows;
+  const uint32_t cols = (0xFFu >> (7 - x1)) & (0xFFu << x0);     // columns x0..x1 of a row
+  const uint32_t rep = __builtin_amdgcn_perm(0u, cols, 0u);       // that byte in all four bytes
+  const uint64_t rows = (~0ull << (8 * y0)) & (~0ull >> (8 * (7 - y1)));   // bytes of rows y0..y1
+  return ((uint64_t)rep << 32 | rep) & rows;
 }
 
 __device__ inline void wave_lds_sync() {
@@ -321,18 +320,6 @@ __device__ inline uint64_t entry_block_mask(const uint2* s_bb, const float4* s_r
   const uint2 bb = s_bb[j];
   const uint32_t q = __float_as_uint(s_rgb[j].w) >> 29;
   return bbox_block_mask(bb.x, bb.y, L) & live & s_om[q];
-}
-
-// Pops the lowest entry of a 128-bit entry set.
-__device__ inline int pop_entry(uint64_t& lo, uint64_t& hi) {
-  if (lo) {
-    const int j = __builtin_ctzll(lo);
-    lo &= lo - 1ull;
-    return j;
-  }
-  const int j = 64 + __builtin_ctzll(hi);
-  hi &= hi - 1ull;
-  return j;
 }
 
 template <bool kDepth, bool kNormal, bool kMaxW, bool kQueue, int kHalves>
@@ -526,10 +513,13 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
           }
           wave_lds_sync();
           // 4. composite this pixel's contributing pairs of the window, in entry order.
-          const uint64_t cs0_lo = s_cs[w][0][lane], cs0_hi = s_cs[w][1][lane];
-          uint64_t cs_lo = cs0_lo, cs_hi = cs0_hi;
-          while ((cs_lo | cs_hi) != 0ull && !done) {
-            const int j = pop_entry(cs_lo, cs_hi);
+          // The 128-bit set as the word being consumed and the one after it (entries jb + bit).
+          uint64_t cur = s_cs[w][0][lane], nxt = s_cs[w][1][lane];
+          int jb = 0;
+          while ((cur | nxt) != 0ull && !done) {
+            if (cur == 0ull) { cur = nxt; nxt = 0ull; jb = 64; }
+            const int j = jb + __builtin_ctzll(cur);
+            cur &= cur - 1ull;
             const uint32_t k = (uint32_t)s_pref[w][j] + (uint32_t)__popcll(s_m[w][j] & lt_mask) - w0;
             const float alpha = s_pa[w][k];
             const float4 rgb = s_rgb[j];
