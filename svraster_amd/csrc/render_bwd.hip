@@ -445,6 +445,7 @@ __global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : 4) void ren
     // Pipelined staging: entry data one batch ahead in registers, its value two batches ahead;
     // each wave's block masks of the next batch (lane: slots lane and lane + 64).
     const float* rgbs_v = a.rgbs + (long long)vi * a.rgb_view_stride;
+    const uint64_t rgb_row0 = (uint64_t)vi * (uint64_t)a.P;   // this view's first row of dL_drgb_views
     EntryRegsB er;
     uint32_t nval = 0;
     if (t < kStage) {
@@ -695,16 +696,26 @@ __global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : 4) void ren
       }
       __syncthreads();
       // Flush the batch's per-entry sums: one global atomic per (entry, slot) that received any.
-      for (int k = t; k < cnt * 12; k += 256) {
-        const int j = k / 12, slot = k - 12 * j;
-        if (!s_used[j]) continue;
-        const float v = s_acc[j][slot];
-        const uint32_t vox = __float_as_uint(s_rgb[j].w) & kVoxMask;
-        float* dst = a.dL_dvox + (long long)vox * 12 + slot;
-        if (a.dL_drgb_views && slot >= 8 && slot < 11)
-          dst = a.dL_drgb_views + ((long long)vi * a.P + vox) * 3 + (slot - 8);
-        if (a.dL_dgrid && slot < 8) dst = a.dL_dgrid + a.vox_key[(long long)vox * 8 + slot];
-        atomicAdd(dst, v);
+      // Thread t takes the sums t, t + 256, ... of the cnt x 12; as 256 = 21 * 12 + 4, each step
+      // advances 21 entries and 4 slots (no division per step).
+      {
+        int j = t / 12, slot = t - 12 * j;
+        for (int k = t; k < cnt * 12; k += 256) {
+          if (s_used[j]) {
+            const float v = s_acc[j][slot];
+            const uint64_t vox = __float_as_uint(s_rgb[j].w) & kVoxMask;
+            float* dst = a.dL_dvox + ((vox << 3) + (vox << 2)) + slot;
+            if (a.dL_drgb_views && slot >= 8 && slot < 11) {
+              const uint64_t row = rgb_row0 + vox;
+              dst = a.dL_drgb_views + ((row << 1) + row) + (slot - 8);
+            }
+            if (a.dL_dgrid && slot < 8) dst = a.dL_dgrid + a.vox_key[(vox << 3) + slot];
+            atomicAdd(dst, v);
+          }
+          j += 21;
+          slot += 4;
+          if (slot >= 12) { slot -= 12; ++j; }
+        }
       }
     }
     flush_tile_ds(ds_hi, ds_lo, s_red, a.ds_part, g, t, lane, w);
