@@ -209,11 +209,12 @@ __global__ __launch_bounds__(256) void gather_geo_bw_kernel(int n, const int64_t
 // Batched views: colour of every (view, visible voxel) pair, direction from that view's camera.
 // One voxel per thread, looping over the views, so each voxel's coefficients are read once.
 // The SH gradient rows of a block of 256 voxels are contiguous ([P, M-1, 3], at most 45 floats
-// for M <= 16); the backward passes them through LDS in three chunks of kShChunk = 15 floats per
-// row so that the global accesses are coalesced while the block holds only 15 KB of LDS (the odd
-// stride keeps the per-thread LDS writes free of bank conflicts).
+// for M <= 16); the backward passes them through LDS kShVox voxels at a time, whole rows, so each
+// pass reads and writes one contiguous span of the row arrays (gradient, or parameter and Adam
+// moments) exactly once, coalesced, with 11.5 KB of LDS (the odd row stride keeps the per-thread
+// LDS writes free of bank conflicts).
 constexpr int kShRowMax = 45;
-constexpr int kShChunk = 15;
+constexpr int kShVox = 64;
 
 // Bit k set when view v0 + k sees voxel i (non-empty octant set, i.e. n_dup > 0); the loads are
 // independent, so they are all in flight at once instead of one dependent load per view.
@@ -314,7 +315,7 @@ __global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, co
                                                                 float* __restrict__ dsh0,
                                                                 float* __restrict__ dshs, ShAdam A) {
   constexpr int NB = (kDeg + 1) * (kDeg + 1);
-  __shared__ float s_rows[256 * kShChunk];
+  __shared__ float s_rows[kShVox * kShRowMax];
   const int t = threadIdx.x;
   const long long i0 = (long long)blockIdx.x * 256;
   const int nv = (int)min(256LL, (long long)P - i0);
@@ -372,36 +373,30 @@ __global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, co
       dsh0[3 * i] = acc[0][0]; dsh0[3 * i + 1] = acc[0][1]; dsh0[3 * i + 2] = acc[0][2];
     }
   }
-  // Rows through LDS, 15 floats of each row at a time (element q of a row is coefficient
-  // 1 + q / 3, channel q % 3; elements past the active degree are zero).
+  // Rows through LDS, kShVox voxels at a time (element q of a row is coefficient 1 + q / 3,
+  // channel q % 3; elements past the active degree are zero).
+  for (int p0 = 0; p0 < nv; p0 += kShVox) {
+    const int pn = min(kShVox, nv - p0);
+    if (t >= p0 && t < p0 + pn) {
+      float* c = s_rows + (t - p0) * row;
 #pragma unroll
-  for (int q0 = 0; q0 < kShRowMax; q0 += kShChunk) {
-    if (q0 >= row) break;
-    const int qn = min(kShChunk, row - q0);
-    if (t < nv) {
-      float* c = s_rows + t * kShChunk;
-#pragma unroll
-      for (int j = 0; j < kShChunk; ++j) {
-        const int q = q0 + j;
-        c[j] = (1 + q / 3 < NB) ? acc[1 + q / 3 < NB ? 1 + q / 3 : 0][q % 3] : 0.f;
-      }
+      for (int q = 0; q < kShRowMax; ++q)
+        if (q < row) c[q] = (1 + q / 3 < NB) ? acc[1 + q / 3 < NB ? 1 + q / 3 : 0][q % 3] : 0.f;
     }
     __syncthreads();
+    const long long e0 = (i0 + p0) * row;
+    const int n = pn * row;
     if (kAdam) {
-      for (int k = t; k < nv * qn; k += 256) {
-        const int r = k / qn, col = k - r * qn;
-        const float g = s_rows[r * kShChunk + col];
+      for (int k = t; k < n; k += 256) {
+        const float g = s_rows[k];
         if (A.sparse && g == 0.0f) continue;
-        const long long e = (i0 + r) * row + q0 + col;
+        const long long e = e0 + k;
         float pv = A.shs[e], mv = A.ms[e], vv = A.vs[e];
         adam_update<false>(pv, mv, vv, g, hs.x, A.ts.b1, A.ts.b2, hs.y, A.ts.eps);
         A.shs[e] = pv; A.ms[e] = mv; A.vs[e] = vv;
       }
     } else {
-      for (int k = t; k < nv * qn; k += 256) {
-        const int r = k / qn, col = k - r * qn;
-        dshs[(i0 + r) * row + q0 + col] = s_rows[r * kShChunk + col];
-      }
+      for (int k = t; k < n; k += 256) dshs[e0 + k] = s_rows[k];
     }
     __syncthreads();
   }
