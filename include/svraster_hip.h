@@ -287,6 +287,13 @@ int svr_is_in_cone(float tanfovx, float tanfovy, float near_plane, const float* 
  * row q = octree_path XOR (q repeated in every 3-bit level field). */
 int svr_voxel_order_rank(const int64_t* octree_paths, int P, int64_t* out_ranks, void* stream);
 
+/* Device self-check (no counterpart in the reference): the preprocess takes 1/z of corner depths
+ * in [2^-100, 2^100] as the hardware reciprocal plus one Newton step instead of the IEEE division
+ * sequence, which must give the same bits.  This compares the two on every float32 significand of
+ * the exponents [e_lo, e_hi] (within [-126, 127]), both signs, and adds the number of mismatches
+ * to mismatches (device uint64[1], zeroed by the caller). */
+int svr_selfcheck_rcp_rn(int e_lo, int e_hi, unsigned long long* mismatches, void* stream);
+
 /* Replaces UTILS::valid_gradient_table (src/utils.cu:190-252, 362-408) in two calls: count
  * (synchronises `stream`; workspace of svr_valid_gradient_table_workspace_bytes(N)), then
  * fill.  grid_mask uint8[(2^grid_res_pow2)^3] zeroed by the caller; keys / voxel ids int32[count]

@@ -88,9 +88,28 @@ __global__ __launch_bounds__(256) void voxel_order_rank_kernel(int P, const int6
   for (uint32_t q = 0; q < 8; ++q) out[(long long)q * P + i] = (int64_t)(path ^ rep3(q));
 }
 
+// Device check of rcp_rn against the IEEE division 1.0f / x: every significand of the
+// exponents [e_lo, e_lo + gridDim.y), both signs; mismatches counted into out[0].
+__global__ __launch_bounds__(256) void selfcheck_rcp_rn_kernel(int e_lo, unsigned long long* __restrict__ out) {
+  const uint32_t m = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t bits = ((uint32_t)(e_lo + (int)blockIdx.y + 127) << 23) | m;
+  uint32_t bad = 0;
+#pragma unroll
+  for (int sgn = 0; sgn < 2; ++sgn) {
+    const float x = __uint_as_float(bits | ((uint32_t)sgn << 31));
+    bad += (uint32_t)(__float_as_uint(rcp_rn(x)) != __float_as_uint(1.0f / x));
+  }
+  if (bad != 0u) atomicAdd(out, (unsigned long long)bad);
+}
+
 inline unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
+
+hipError_t launch_selfcheck_rcp_rn(int e_lo, int e_hi, unsigned long long* out, hipStream_t st) {
+  selfcheck_rcp_rn_kernel<<<dim3((1u << 23) / 256u, (unsigned)(e_hi - e_lo + 1)), 256, 0, st>>>(e_lo, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_compute_rd(int W, int H, float cx, float cy, float tfx, float tfy, const float* c2w, float* rd,
                              hipStream_t st) {

@@ -745,6 +745,15 @@ int svr_voxel_order_rank(const int64_t* octree_paths, int P, int64_t* out_ranks,
   return 0;
 }
 
+int svr_selfcheck_rcp_rn(int e_lo, int e_hi, unsigned long long* mismatches, void* stream) {
+  g_err.clear();
+  if (e_lo < -126 || e_hi > 127 || e_lo > e_hi)
+    return fail(1, "exponent range must lie in [-126, 127]%s (got %lld)", "", (long long)e_lo);
+  if (!mismatches) return fail(1, "mismatches is required%s", "", 0);
+  SVR_TRY(launch_selfcheck_rcp_rn(e_lo, e_hi, mismatches, as_stream(stream)));
+  return 0;
+}
+
 // ------------------------------------------------------------------------------------------
 // Batched views: one workspace holding the geometry state of V views x P voxels, a binning
 // state of fixed capacity and the image state of all views' tiles and pixels.  Every launch

@@ -167,6 +167,18 @@ __device__ inline float fast_sigmoid(float x) {
   return __builtin_amdgcn_rcpf(1.f + __expf(-x));
 }
 
+// Range in which rcp_rn is the correctly rounded reciprocal: |x| in [2^-100, 2^100].
+constexpr float kRcpRnLo = 0x1p-100f, kRcpRnHi = 0x1p100f;
+
+// 1/x rounded to nearest for |x| in [kRcpRnLo, kRcpRnHi]: the hardware reciprocal (within 1 ulp)
+// and one Newton step, whose residual 1 - x y is exact in an FMA.  The result equals the IEEE
+// quotient 1.0f / x for every float32 of that range on gfx950; svr_selfcheck_rcp_rn checks all
+// 2^23 significands of any exponent range on the device, against the IEEE division.
+__device__ inline float rcp_rn(float x) {
+  const float y = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(__builtin_fmaf(-x, y, 1.f), y, y);
+}
+
 // Per-entry constants of the SDF segment as the compositors stage them, once per entry and view.
 // Every ray of a view starts at the camera centre o, so the box corners are kept relative to it,
 // and the corner SDFs with the differences the z lerps take:

@@ -50,20 +50,55 @@ __device__ inline void preprocess_one(const ViewDesc& v, bool leaf, float vcx, f
     if (!(d2 < v.near_ * v.near_)) {
       float mnx = 1e9f, mny = 1e9f, mxx = -1e9f, mxy = -1e9f;
       const float* m = v.w2c;
+      // Corner c = (x, y, z) bits (c & 4, c & 2, c & 1): coordinate centre -/+ r per axis
+      // (centre + r * (-1) is centre - r exactly), in camera space with the operation order
+      // ((m0 x + m1 y) + m2 z) + m3.
+      const float wx[2] = {vcx + r * -1.f, vcx + r * 1.f};
+      const float wy[2] = {vcy + r * -1.f, vcy + r * 1.f};
+      const float wz[2] = {vcz + r * -1.f, vcz + r * 1.f};
+      float ccx[8], ccy[8], ccz[8];
+      bool front = true;
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
-        const float wx = vcx + r * (float)(((c & 4) >> 2) * 2 - 1);
-        const float wy = vcy + r * (float)(((c & 2) >> 1) * 2 - 1);
-        const float wz = vcz + r * (float)((c & 1) * 2 - 1);
-        const float cx = m[0] * wx + m[1] * wy + m[2] * wz + m[3];
-        const float cy = m[4] * wx + m[5] * wy + m[6] * wz + m[7];
-        const float cz = m[8] * wx + m[9] * wy + m[10] * wz + m[11];
-        if (cz < v.near_) continue;
-        const float inv_z = 1.0f / cz;
-        const float ux = cx * inv_z, uy = cy * inv_z;
-        mnx = fminf(mnx, ux); mny = fminf(mny, uy);
-        mxx = fmaxf(mxx, ux); mxy = fmaxf(mxy, uy);
-        qbits |= 1u << ((uint32_t(wx < rox) << 2) | (uint32_t(wy < roy) << 1) | uint32_t(wz < roz));
+        const float x = wx[(c >> 2) & 1], y = wy[(c >> 1) & 1], z = wz[c & 1];
+        ccx[c] = m[0] * x + m[1] * y + m[2] * z + m[3];
+        ccy[c] = m[4] * x + m[5] * y + m[6] * z + m[7];
+        ccz[c] = m[8] * x + m[9] * y + m[10] * z + m[11];
+        front = front && !(ccz[c] < v.near_) && ccz[c] >= kRcpRnLo && ccz[c] <= kRcpRnHi;
+      }
+      if (front) {
+        // Every corner in front of the near plane with depth in rcp_rn's range (the common
+        // case): the reciprocal without the division sequence (same bits), straight-line min /
+        // max (no NaN can arise) and the octant set as the product of the per-axis sides, the
+        // same values the per-corner loop below gives.
+        float ux[8], uy[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const float inv_z = rcp_rn(ccz[c]);
+          ux[c] = ccx[c] * inv_z;
+          uy[c] = ccy[c] * inv_z;
+        }
+        mnx = fminf(mnx, fminf(fminf(fminf(ux[0], ux[1]), fminf(ux[2], ux[3])), fminf(fminf(ux[4], ux[5]), fminf(ux[6], ux[7]))));
+        mny = fminf(mny, fminf(fminf(fminf(uy[0], uy[1]), fminf(uy[2], uy[3])), fminf(fminf(uy[4], uy[5]), fminf(uy[6], uy[7]))));
+        mxx = fmaxf(mxx, fmaxf(fmaxf(fmaxf(ux[0], ux[1]), fmaxf(ux[2], ux[3])), fmaxf(fmaxf(ux[4], ux[5]), fmaxf(ux[6], ux[7]))));
+        mxy = fmaxf(mxy, fmaxf(fmaxf(fmaxf(uy[0], uy[1]), fmaxf(uy[2], uy[3])), fmaxf(fmaxf(uy[4], uy[5]), fmaxf(uy[6], uy[7]))));
+        // Bit b of xs: some corner has x-octant bit b (x < camera x); likewise ys, zs.
+        const uint32_t xs = (1u << uint32_t(wx[0] < rox)) | (1u << uint32_t(wx[1] < rox));
+        const uint32_t ys = (1u << uint32_t(wy[0] < roy)) | (1u << uint32_t(wy[1] < roy));
+        const uint32_t zs = (1u << uint32_t(wz[0] < roz)) | (1u << uint32_t(wz[1] < roz));
+        const uint32_t yz = ((ys & 1u) ? zs : 0u) | ((ys & 2u) ? (zs << 2) : 0u);
+        qbits = ((xs & 1u) ? yz : 0u) | ((xs & 2u) ? (yz << 4) : 0u);
+      } else {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          if (ccz[c] < v.near_) continue;
+          const float inv_z = 1.0f / ccz[c];
+          const float ux = ccx[c] * inv_z, uy = ccy[c] * inv_z;
+          mnx = fminf(mnx, ux); mny = fminf(mny, uy);
+          mxx = fmaxf(mxx, ux); mxy = fmaxf(mxy, uy);
+          qbits |= 1u << ((uint32_t(wx[(c >> 2) & 1] < rox) << 2) | (uint32_t(wy[(c >> 1) & 1] < roy) << 1) |
+                          uint32_t(wz[c & 1] < roz));
+        }
       }
       const float cxh = v.cx - 0.5f, cyh = v.cy - 0.5f;
       const float bx0 = fmaxf(v.focal_x * mnx + cxh, 0.0f);

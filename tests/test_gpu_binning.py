@@ -120,3 +120,17 @@ def test_pair_rank_sort_stable(P, kind):
     keys = (octpaths.astype(np.uint64)[:, None] ^ REP3[None, :]).reshape(-1) & np.uint64((1 << 48) - 1)
     expect = np.argsort(keys, kind="stable").astype(np.uint32)
     np.testing.assert_array_equal(_pair_order_gpu(octpaths), expect)
+
+
+@pytest.mark.gpu
+def test_preprocess_reciprocal_matches_ieee_division():
+    """The preprocess takes 1/z of corner depths in [2^-100, 2^100] as the hardware reciprocal plus
+    one Newton step; the bboxes are bit-exact only if that equals the IEEE quotient.  Every
+    float32 significand of every exponent in that range, both signs, must agree."""
+    from svraster_amd import _lib
+    lib = _lib.load()
+    out = torch.zeros(1, dtype=torch.int64, device="cuda")
+    say("[binning] rcp + Newton step vs IEEE 1/x, exponents -100..100")
+    _lib.check(lib.svr_selfcheck_rcp_rn(-100, 100, out.data_ptr(), None), "selfcheck_rcp_rn")
+    torch.cuda.synchronize()
+    assert int(out.item()) == 0
