@@ -343,28 +343,37 @@ int svr_batch_preprocess(void* ws, int n_views, int P, long long n_pixels, long 
 int svr_voxel_activity(int P, const uint8_t* is_leaf, const float* geos, const float* s_val,
                        const float* vox_centers, const float* vox_lengths, float cam_scale, uint8_t* out,
                        void* stream);
+/* Voxel records of the multi-view compositors, out_records float [P,16] (16-byte aligned), per
+ * voxel: centre xyz, size, the eight corner SDFs gathered from grid_pts [M] through vox_key [P,8]
+ * (GatherGeoParams, geo_params_gather.cu:134-152 / renderer.py:370-406, fused here with the
+ * record build: no [P,8] corner-SDF array is written), 1/size and three zeros.  With out_bin_leaf
+ * [P] it also writes what svr_voxel_activity would for those corner SDFs (cull != 0) or is_leaf
+ * itself (cull = 0). */
+int svr_batch_voxel_records(int P, const int64_t* vox_key, const float* grid_pts, const float* vox_centers,
+                            const float* vox_lengths, const uint8_t* is_leaf, const float* s_val, float cam_scale,
+                            int cull, float* out_records, uint8_t* out_bin_leaf, void* stream);
 /* SH colour of every (view, visible voxel) pair into rgbs_views. */
 int svr_batch_sh_forward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                          const float* vox_centers, const float* sh0, const float* shs, int degree, int M,
                          float* rgbs_views, void* stream);
 /* Keys written pair by pair in rank order, a stable sort by global (view, tile) id only (each
  * tile's entries then follow the reference's order, forward.cu:749-829), tile ranges, and one
- * persistent compositor launch over all (view, tile) pairs.  Colour and T need no clearing;
- * depth is zeroed by the caller. */
+ * persistent compositor launch over all (view, tile) pairs, the entries' geometry read from
+ * svr_batch_voxel_records' records.  Colour and T need no clearing; depth is zeroed by the
+ * caller. */
 int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
-                        const float* vox_centers, const float* vox_lengths, const float* geos,
-                        const float* rgbs_views, const float* s_val, const uint32_t* pair_order, int need_depth,
+                        const float* vox_records, const float* rgbs_views, const float* s_val, const uint32_t* pair_order, int need_depth,
                         int need_normal, float* out_color, float* out_depth, float* out_normal, float* out_T,
                         void* stream);
-/* One persistent reverse compositor launch over all (view, tile) pairs.  dL_dvox [P,12]
+/* One persistent reverse compositor launch over all (view, tile) pairs (geometry from the same
+ * records as the forward).  dL_dvox [P,12]
  * (colour slots unused), dL_drgb_views [V,P,3] and dL_ds [1] zeroed by the caller.  With
  * vox_key [P,8] and dL_dgrid_pts [M] (zeroed by the caller) the corner-SDF gradients are added
  * straight to the grid points as they are flushed -- the transpose of GatherGeoParams
  * (geo_params_gather.cu:22-72) fused into the compositor -- and dL_dvox slots 0-7 stay
  * untouched. */
 int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
-                       const float* vox_centers, const float* vox_lengths, const float* geos,
-                       const float* rgbs_views, const float* s_val, const float* out_T,
+                       const float* vox_records, const float* rgbs_views, const float* s_val, const float* out_T,
                        const float* dL_dcolor, const float* dL_ddepth, const float* dL_dnormal,
                        const float* dL_dT, int need_depth, int need_normal, float* dL_dvox,
                        float* dL_drgb_views, float* dL_ds, const int64_t* vox_key, float* dL_dgrid_pts,

@@ -81,7 +81,8 @@ class BatchedRasterizer:
         self.c2w = torch.stack([c.c2w for c in cameras]).float().contiguous().to(self.dev)
         self.bg = background.float().contiguous().to(self.dev)
         # Voxels whose corner SDFs cannot reach the minimum alpha are left out of binning
-        # (svr_voxel_activity); images and gradients are unchanged, the key count drops.
+        # (the activity test of svr_batch_voxel_records); images and gradients are unchanged, the key
+        # count drops.
         self.cull = bool(cull)
         self.cam_scale = float(self.c2w[:, :3, 3].abs().max().item()) if self.V else 0.0
         self.bin_leaf = self.is_leaf.clone()
@@ -114,7 +115,9 @@ class BatchedRasterizer:
                                             self.tile_w, self.tile_h, st),
                    "svr_batch_setup")
         f = dict(dtype=torch.float32, device=dev)
-        self.geos = torch.empty(P, 8, **f)
+        # Per-voxel records of the compositors: centre, size, corner SDFs (gathered from the grid
+        # points each forward), 1/size -- 64 bytes per voxel.
+        self.vrec = torch.empty(P, 16, **f)
         self.rgbs_views = torch.zeros(V, P, 3, **f)
         self.color = torch.zeros(3 * self.n_pix, **f)
         self.T = torch.zeros(self.n_pix, **f)
@@ -127,6 +130,11 @@ class BatchedRasterizer:
         self.dshs = torch.zeros(P, self.M - 1, 3, **f)
         self.dgrid = None
         self.zero_T = torch.zeros(self.n_pix, **f)
+
+    @property
+    def geos(self) -> torch.Tensor:
+        """Corner SDFs [P,8] of the last forward (a copy out of the voxel records)."""
+        return self.vrec[:, 4:12].contiguous()
 
     def _stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
@@ -185,16 +193,12 @@ class BatchedRasterizer:
     def forward(self, grid_pts: torch.Tensor, sh0: torch.Tensor, shs: torch.Tensor, s_val: torch.Tensor):
         """Render every view.  Returns the concatenated colour buffer (views [3,H,W] back to back)."""
         st = self._stream()
-        _lib.check(self.lib.svr_gather_triinterp_geo_params(_p(self.vox_key), self.P, None, self.P,
-                                                            _p(grid_pts), _p(self.geos), st), "gather")
-        if self.cull:
-            _lib.check(self.lib.svr_voxel_activity(self.P, _p(self.is_leaf), _p(self.geos), _p(s_val),
-                                                   _p(self.vox_center), _p(self.vox_size),
-                                                   ctypes.c_float(self.cam_scale), _p(self.bin_leaf), st),
-                       "svr_voxel_activity")
-            self.preprocess(self.bin_leaf)
-        else:
-            self.preprocess()
+        _lib.check(self.lib.svr_batch_voxel_records(self.P, _p(self.vox_key), _p(grid_pts), _p(self.vox_center),
+                                                    _p(self.vox_size), _p(self.is_leaf), _p(s_val),
+                                                    ctypes.c_float(self.cam_scale), int(self.cull), _p(self.vrec),
+                                                    _p(self.bin_leaf) if self.cull else None, st),
+                   "svr_batch_voxel_records")
+        self.preprocess(self.bin_leaf if self.cull else None)
         if self.check_eager and not torch.cuda.is_current_stream_capturing():
             self.check_capacity()
         _lib.check(self.lib.svr_batch_sh_forward(_p(self.ws), *self._dims(), _p(self.vox_center), _p(sh0), _p(shs),
@@ -205,8 +209,8 @@ class BatchedRasterizer:
         if self.normal is not None:
             self.normal.zero_()
         self.s_val = s_val
-        _lib.check(self.lib.svr_batch_rasterize(_p(self.ws), *self._dims(), _p(self.vox_center), _p(self.vox_size),
-                                                _p(self.geos), _p(self.rgbs_views), _p(s_val), _p(self.pair_order),
+        _lib.check(self.lib.svr_batch_rasterize(_p(self.ws), *self._dims(), _p(self.vrec), _p(self.rgbs_views),
+                                                _p(s_val), _p(self.pair_order),
                                                 int(self.need_depth), int(self.need_normal),
                                                 _p(self.color), _p(self.depth), _p(self.normal), _p(self.T), st),
                    "svr_batch_rasterize")
@@ -227,8 +231,8 @@ class BatchedRasterizer:
         self.dgrid.zero_()
         dT = self.zero_T if dL_dT is None else dL_dT
         fuse = self.fuse_gather_bw
-        _lib.check(self.lib.svr_batch_backward(_p(self.ws), *self._dims(), _p(self.vox_center), _p(self.vox_size),
-                                               _p(self.geos), _p(self.rgbs_views), _p(self.s_val), _p(self.T),
+        _lib.check(self.lib.svr_batch_backward(_p(self.ws), *self._dims(), _p(self.vrec), _p(self.rgbs_views),
+                                               _p(self.s_val), _p(self.T),
                                                _p(dL_dcolor), _p(dL_ddepth), _p(dL_dnormal), _p(dT),
                                                int(self.need_depth), int(self.need_normal), _p(self.dvox),
                                                _p(self.drgb_views), _p(self.ds),

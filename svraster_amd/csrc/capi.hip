@@ -199,7 +199,7 @@ int svr_timing_read(int kind, int reset, double* total_ms, long long* launches) 
   }
   return 0;
 }
-int svr_abi_version(void) { return 2; }
+int svr_abi_version(void) { return 3; }
 
 int svr_set_option(const char* name, int value) {
   g_err.clear();
@@ -868,6 +868,21 @@ int svr_voxel_activity(int P, const uint8_t* is_leaf, const float* geos, const f
   return 0;
 }
 
+int svr_batch_voxel_records(int P, const int64_t* vox_key, const float* grid_pts, const float* vox_centers,
+                            const float* vox_lengths, const uint8_t* is_leaf, const float* s_val, float cam_scale,
+                            int cull, float* out_records, uint8_t* out_bin_leaf, void* stream) {
+  g_err.clear();
+  if (P < 0) return fail(1, "bad voxel count%s %lld", "", P);
+  if (P > 0 && (!vox_key || !grid_pts || !vox_centers || !vox_lengths || !out_records))
+    return fail(1, "vox_key, grid_pts, centres, sizes and out_records are required%s%lld", "", 0);
+  if (reinterpret_cast<uintptr_t>(out_records) % 16 != 0) return fail(1, "out_records must be 16-byte aligned%s%lld", "", 0);
+  if (out_bin_leaf && (!is_leaf || !s_val)) return fail(1, "out_bin_leaf needs is_leaf and s_val%s%lld", "", 0);
+  if (!(cam_scale >= 0.f)) return fail(1, "cam_scale must be >= 0%s %lld", "", 0);
+  SVR_TRY(launch_voxel_records(P, vox_key, grid_pts, vox_centers, vox_lengths, is_leaf, s_val, cam_scale, cull,
+                               reinterpret_cast<VoxelRecord*>(out_records), out_bin_leaf, as_stream(stream)));
+  return 0;
+}
+
 int svr_batch_sh_forward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                          const float* vox_centers, const float* sh0, const float* shs, int degree, int M,
                          float* rgbs_views, void* stream) {
@@ -882,8 +897,7 @@ int svr_batch_sh_forward(void* ws, int n_views, int P, long long n_pixels, long 
 }
 
 int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
-                        const float* vox_centers, const float* vox_lengths, const float* geos,
-                        const float* rgbs_views, const float* s_val, const uint32_t* pair_order, int need_depth,
+                        const float* vox_records, const float* rgbs_views, const float* s_val, const uint32_t* pair_order, int need_depth,
                         int need_normal, float* out_color, float* out_depth, float* out_normal, float* out_T,
                         void* stream) {
   g_err.clear();
@@ -908,7 +922,7 @@ int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long l
   a.views = bp.g.views; a.n_views = n_views; a.total_tiles = (int)n_tiles;
   a.ranges = bp.im.ranges; a.vals = bp.b.vals;
   a.bboxes = bp.g.bbox; a.P = P;
-  a.centers = vox_centers; a.lengths = vox_lengths; a.geos = geos; a.rgbs = rgbs_views;
+  a.vrec = reinterpret_cast<const VoxelRecord*>(vox_records); a.rgbs = rgbs_views;
   a.rgb_view_stride = 3LL * P;
   a.s_val = s_val;
   a.tile_last = bp.im.tile_last; a.n_contrib = bp.im.n_contrib;
@@ -928,8 +942,7 @@ int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long l
 }
 
 int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
-                       const float* vox_centers, const float* vox_lengths, const float* geos,
-                       const float* rgbs_views, const float* s_val, const float* out_T, const float* dL_dcolor,
+                       const float* vox_records, const float* rgbs_views, const float* s_val, const float* out_T, const float* dL_dcolor,
                        const float* dL_ddepth, const float* dL_dnormal, const float* dL_dT, int need_depth,
                        int need_normal, float* dL_dvox, float* dL_drgb_views, float* dL_ds,
                        const int64_t* vox_key, float* dL_dgrid_pts, void* stream) {
@@ -947,7 +960,7 @@ int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long lo
   a.views = bp.g.views; a.n_views = n_views; a.total_tiles = (int)n_tiles;
   a.ranges = bp.im.ranges; a.tile_last = bp.im.tile_last; a.vals = bp.b.vals;
   a.bboxes = bp.g.bbox; a.P = P;
-  a.centers = vox_centers; a.lengths = vox_lengths; a.geos = geos; a.rgbs = rgbs_views;
+  a.vrec = reinterpret_cast<const VoxelRecord*>(vox_records); a.rgbs = rgbs_views;
   a.rgb_view_stride = 3LL * P;
   a.s_val = s_val;
   a.T = out_T; a.n_contrib = bp.im.n_contrib;

@@ -167,6 +167,40 @@ __device__ inline float fast_sigmoid(float x) {
   return __builtin_amdgcn_rcpf(1.f + __expf(-x));
 }
 
+// Voxel record of the multi-view path, 64 bytes per voxel (one aligned segment per staged
+// entry): centre xyz and size, the eight corner SDFs (gathered from the grid points), 1/size and
+// three unused floats.
+struct VoxelRecord {
+  float4 cl;    // centre xyz, size
+  float4 g0, g1;
+  float4 aux;   // 1/size (IEEE quotient), 0, 0, 0
+};
+
+// Geometry of voxel v for the compositors: from the 64-byte record when there is one, else from
+// the separate centre / size / corner-SDF arrays (the one-view drop-in path).
+struct VoxelGeom {
+  float cx, cy, cz, l, l_inv;
+  float4 g0, g1;
+};
+
+__device__ inline void load_voxel_geom(VoxelGeom& e, const VoxelRecord* __restrict__ rec,
+                                       const float* __restrict__ centers, const float* __restrict__ lengths,
+                                       const float* __restrict__ geos, uint32_t v) {
+  if (rec) {
+    const VoxelRecord& R = rec[v];
+    const float4 cl = R.cl;
+    e.cx = cl.x; e.cy = cl.y; e.cz = cl.z; e.l = cl.w;
+    e.g0 = R.g0; e.g1 = R.g1;
+    e.l_inv = R.aux.x;
+  } else {
+    e.cx = centers[3 * v]; e.cy = centers[3 * v + 1]; e.cz = centers[3 * v + 2];
+    e.l = lengths[v];
+    e.g0 = reinterpret_cast<const float4*>(geos)[2 * v];
+    e.g1 = reinterpret_cast<const float4*>(geos)[2 * v + 1];
+    e.l_inv = 1.f / e.l;
+  }
+}
+
 // Range in which rcp_rn is the correctly rounded reciprocal: |x| in [2^-100, 2^100].
 constexpr float kRcpRnLo = 0x1p-100f, kRcpRnHi = 0x1p100f;
 
@@ -185,12 +219,13 @@ __device__ inline float rcp_rn(float x) {
 //   box = (lo.x, lo.y, lo.z, 1/l), hi = (hi.x, hi.y, hi.z, 0), lo/hi = (c - o) -/+ l/2;
 //   gd0 = (g0, g1 - g0, g2, g3 - g2), gd1 = (g4, g5 - g4, g6, g7 - g6).
 // ray_box_st and trilerp_d on these give the same values as ray_box and trilerp on the raw data.
-__device__ inline void stage_segment(float cx, float cy, float cz, float l, float4 g0, float4 g1, const Ray& r,
-                                     float4& box, float4& hi, float4& gd0, float4& gd1) {
+// l_inv is 1.f / l (IEEE quotient), from the caller or the voxel record.
+__device__ inline void stage_segment(float cx, float cy, float cz, float l, float l_inv, float4 g0, float4 g1,
+                                     const Ray& r, float4& box, float4& hi, float4& gd0, float4& gd1) {
 #pragma clang fp contract(off)
   const float h = 0.5f * l;
   const float dx = cx - r.ox, dy = cy - r.oy, dz = cz - r.oz;
-  box = make_float4(dx - h, dy - h, dz - h, 1.f / l);
+  box = make_float4(dx - h, dy - h, dz - h, l_inv);
   hi = make_float4(dx + h, dy + h, dz + h, 0.f);
   gd0 = make_float4(g0.x, g0.y - g0.x, g0.z, g0.w - g0.z);
   gd1 = make_float4(g1.x, g1.y - g1.x, g1.z, g1.w - g1.z);

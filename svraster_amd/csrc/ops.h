@@ -12,6 +12,7 @@ hipError_t launch_sh_backward(int n, const int64_t* idx, int deg, int M, const f
                               const float* cam, const float* dL_drgb, float* dsh0, float* dshs,
                               hipStream_t st);
 struct ViewDesc;
+struct VoxelRecord;
 hipError_t launch_sh_views_forward(int P, int V, const uint8_t* quads, const ViewDesc* views, int deg, int M,
                                    const float* centers, const float* sh0, const float* shs, float* rgbs_views,
                                    hipStream_t st);
@@ -32,6 +33,9 @@ struct ShAdam {
 hipError_t launch_sh_views_backward(int P, int V, const uint8_t* quads, const ViewDesc* views, int deg, int M,
                                     const float* centers, float* drgb_views, float* dsh0, float* dshs,
                                     const ShAdam* adam, hipStream_t st);
+hipError_t launch_voxel_records(int P, const int64_t* vox_key, const float* grid_pts, const float* centers,
+                                const float* lengths, const uint8_t* is_leaf, const float* s_val, float cam_scale,
+                                int cull, VoxelRecord* rec, uint8_t* bin_leaf, hipStream_t st);
 hipError_t launch_voxel_activity(int P, const uint8_t* is_leaf, const float* geos, const float* s_val,
                                  const float* centers, const float* lengths, float cam_scale, uint8_t* out,
                                  hipStream_t st);

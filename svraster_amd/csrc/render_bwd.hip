@@ -33,27 +33,24 @@ __device__ inline int find_view_b(const ViewDesc* views, int n_views, int g) {
 // of the next batch are in flight while the current batch is processed.
 struct EntryRegsB {
   uint32_t val;
-  float cx, cy, cz, l;
-  float4 g0, g1;
+  VoxelGeom v;
   float r, g, b;
 };
 
 __device__ inline void fetch_entry_b(EntryRegsB& e, const BwdArgs& a, uint32_t val, const float* __restrict__ rgbs) {
   const uint32_t vox = val & kVoxMask;
   e.val = val;
-  e.cx = a.centers[3 * vox]; e.cy = a.centers[3 * vox + 1]; e.cz = a.centers[3 * vox + 2];
-  e.l = a.lengths[vox];
-  e.g0 = reinterpret_cast<const float4*>(a.geos)[2 * vox];
-  e.g1 = reinterpret_cast<const float4*>(a.geos)[2 * vox + 1];
+  load_voxel_geom(e.v, a.vrec, a.centers, a.lengths, a.geos, vox);
   e.r = rgbs[3 * vox]; e.g = rgbs[3 * vox + 1]; e.b = rgbs[3 * vox + 2];
 }
 
 __device__ inline void store_entry_b(const EntryRegsB& e, bool need_normal, const Ray& r, float4* s_ch, float4* s_ml,
                                      float4* s_g0, float4* s_g1, float4* s_rgb, float4* s_nrm, int slot) {
-  stage_segment(e.cx, e.cy, e.cz, e.l, e.g0, e.g1, r, s_ch[slot], s_ml[slot], s_g0[slot], s_g1[slot]);
+  const VoxelGeom& v = e.v;
+  stage_segment(v.cx, v.cy, v.cz, v.l, v.l_inv, v.g0, v.g1, r, s_ch[slot], s_ml[slot], s_g0[slot], s_g1[slot]);
   s_rgb[slot] = make_float4(e.r, e.g, e.b, __uint_as_float(e.val));
   if (need_normal) {
-    const float g[8] = {e.g0.x, e.g0.y, e.g0.z, e.g0.w, e.g1.x, e.g1.y, e.g1.z, e.g1.w};
+    const float g[8] = {v.g0.x, v.g0.y, v.g0.z, v.g0.w, v.g1.x, v.g1.y, v.g1.z, v.g1.w};
     float nx, ny, nz, rn;
     lin_normal(g, nx, ny, nz, rn);
     s_nrm[slot] = make_float4(nx * rn, ny * rn, nz * rn, rn);
@@ -185,11 +182,10 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
         const uint32_t val = a.vals[e];
         const uint32_t vox = val & kVoxMask;
         s_bb[t] = a.bboxes[(long long)vi * a.P + vox];
-        const float cx = a.centers[3 * vox], cy = a.centers[3 * vox + 1], cz = a.centers[3 * vox + 2];
-        const float l = a.lengths[vox];
-        const float4 g0 = reinterpret_cast<const float4*>(a.geos)[2 * vox];
-        const float4 g1 = reinterpret_cast<const float4*>(a.geos)[2 * vox + 1];
-        stage_segment(cx, cy, cz, l, g0, g1, r, s_ch[t], s_ml[t], s_g0[t], s_g1[t]);
+        VoxelGeom vg;
+        load_voxel_geom(vg, a.vrec, a.centers, a.lengths, a.geos, vox);
+        const float4 g0 = vg.g0, g1 = vg.g1;
+        stage_segment(vg.cx, vg.cy, vg.cz, vg.l, vg.l_inv, g0, g1, r, s_ch[t], s_ml[t], s_g0[t], s_g1[t]);
         const float* rgbv = a.rgbs + (long long)vi * a.rgb_view_stride;
         s_rgb[t] = make_float4(rgbv[3 * vox], rgbv[3 * vox + 1], rgbv[3 * vox + 2], __uint_as_float(val));
         if (kNormal) {

@@ -34,21 +34,18 @@ struct StagedEntry {
   float4 nrm;   // normalised linear normal, 1/|lin_n|
 };
 
-__device__ inline void stage_entry(const uint32_t* __restrict__ vals, uint32_t e, int view, long long P,
-                                   const uint2* __restrict__ bboxes, const float* __restrict__ centers,
-                                   const float* __restrict__ lengths, const float* __restrict__ geos,
-                                   const float* __restrict__ rgbs, bool need_normal,
-                                   float4* s_ch, float4* s_ml, float4* s_g0, float4* s_g1,
+__device__ inline void stage_entry(const FwdArgs& a, uint32_t e, int view, const float* __restrict__ rgbs,
+                                   bool need_normal, float4* s_ch, float4* s_ml, float4* s_g0, float4* s_g1,
                                    float4* s_rgb, float4* s_nrm, uint2* s_bb, int slot, const Ray& r,
                                    bool cull = false, float s = 0.f, float cam_scale = 0.f) {
-  const uint32_t val = vals[e];
+  const uint32_t val = a.vals[e];
   const uint32_t vox = val & kVoxMask;
-  uint2 bb = bboxes[(long long)view * P + vox];
-  const float cx = centers[3 * vox + 0], cy = centers[3 * vox + 1], cz = centers[3 * vox + 2];
-  const float l = lengths[vox];
-  const float4 g0 = reinterpret_cast<const float4*>(geos)[2 * vox];
-  const float4 g1 = reinterpret_cast<const float4*>(geos)[2 * vox + 1];
-  stage_segment(cx, cy, cz, l, g0, g1, r, s_ch[slot], s_ml[slot], s_g0[slot], s_g1[slot]);
+  uint2 bb = a.bboxes[(long long)view * a.P + vox];
+  VoxelGeom v;
+  load_voxel_geom(v, a.vrec, a.centers, a.lengths, a.geos, vox);
+  const float cx = v.cx, cy = v.cy, cz = v.cz, l = v.l;
+  const float4 g0 = v.g0, g1 = v.g1;
+  stage_segment(cx, cy, cz, l, v.l_inv, g0, g1, r, s_ch[slot], s_ml[slot], s_g0[slot], s_g1[slot]);
   if (cull) {
     // An entry that cannot reach the minimum alpha gets an empty bbox (no pixel masks).
     const float g[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
@@ -70,21 +67,16 @@ __device__ inline void stage_entry(const uint32_t* __restrict__ vals, uint32_t e
 struct EntryRegs {
   uint32_t val;
   uint2 bb;
-  float cx, cy, cz, l;
-  float4 g0, g1;
+  VoxelGeom v;
   float r, g, b;
 };
 
-__device__ inline void fetch_entry(EntryRegs& e, uint32_t val, int view, long long P, const uint2* __restrict__ bboxes,
-                                   const float* __restrict__ centers, const float* __restrict__ lengths,
-                                   const float* __restrict__ geos, const float* __restrict__ rgbs) {
+__device__ inline void fetch_entry(EntryRegs& e, uint32_t val, int view, const FwdArgs& a,
+                                   const float* __restrict__ rgbs) {
   const uint32_t vox = val & kVoxMask;
   e.val = val;
-  e.bb = bboxes[(long long)view * P + vox];
-  e.cx = centers[3 * vox + 0]; e.cy = centers[3 * vox + 1]; e.cz = centers[3 * vox + 2];
-  e.l = lengths[vox];
-  e.g0 = reinterpret_cast<const float4*>(geos)[2 * vox];
-  e.g1 = reinterpret_cast<const float4*>(geos)[2 * vox + 1];
+  e.bb = a.bboxes[(long long)view * a.P + vox];
+  load_voxel_geom(e.v, a.vrec, a.centers, a.lengths, a.geos, vox);
   e.r = rgbs[3 * vox + 0]; e.g = rgbs[3 * vox + 1]; e.b = rgbs[3 * vox + 2];
 }
 
@@ -94,10 +86,11 @@ template <bool kCull>
 __device__ inline void store_entry(const EntryRegs& e, bool need_normal, float s, float cam_scale, const Ray& r,
                                    float4* s_ch, float4* s_ml, float4* s_g0, float4* s_g1, float4* s_rgb, float4* s_nrm,
                                    uint2* s_bb, int slot) {
-  stage_segment(e.cx, e.cy, e.cz, e.l, e.g0, e.g1, r, s_ch[slot], s_ml[slot], s_g0[slot], s_g1[slot]);
-  const float g[8] = {e.g0.x, e.g0.y, e.g0.z, e.g0.w, e.g1.x, e.g1.y, e.g1.z, e.g1.w};
-  const float ps = cam_scale + fmaxf(fabsf(e.cx), fmaxf(fabsf(e.cy), fabsf(e.cz)));
-  s_bb[slot] = (!kCull || voxel_may_contribute(g, s, ps, e.l)) ? e.bb : make_uint2(0xFFFFFFFFu, 0u);
+  const VoxelGeom& v = e.v;
+  stage_segment(v.cx, v.cy, v.cz, v.l, v.l_inv, v.g0, v.g1, r, s_ch[slot], s_ml[slot], s_g0[slot], s_g1[slot]);
+  const float g[8] = {v.g0.x, v.g0.y, v.g0.z, v.g0.w, v.g1.x, v.g1.y, v.g1.z, v.g1.w};
+  const float ps = cam_scale + fmaxf(fabsf(v.cx), fmaxf(fabsf(v.cy), fabsf(v.cz)));
+  s_bb[slot] = (!kCull || voxel_may_contribute(g, s, ps, v.l)) ? e.bb : make_uint2(0xFFFFFFFFu, 0u);
   s_rgb[slot] = make_float4(e.r, e.g, e.b, __uint_as_float(e.val));
   if (need_normal) {
     float nx, ny, nz, rn;
@@ -108,13 +101,13 @@ __device__ inline void store_entry(const EntryRegs& e, bool need_normal, float s
 
 // Median depth refined with 16 sub-steps inside the median voxel (forward.cu:450-521).
 __device__ inline float median_depth(const FwdArgs& a, const Ray& r, int v, float dmed_T, float s) {
-  const float cx = a.centers[3 * v], cy = a.centers[3 * v + 1], cz = a.centers[3 * v + 2];
-  const float l = a.lengths[v];
-  float gg[8];
-  for (int k = 0; k < 8; ++k) gg[k] = a.geos[8 * v + k];
+  VoxelGeom e;
+  load_voxel_geom(e, a.vrec, a.centers, a.lengths, a.geos, (uint32_t)v);
+  const float cx = e.cx, cy = e.cy, cz = e.cz, l = e.l;
+  const float gg[8] = {e.g0.x, e.g0.y, e.g0.z, e.g0.w, e.g1.x, e.g1.y, e.g1.z, e.g1.w};
   float ta, tb;
   ray_box(r, cx, cy, cz, 0.5f * l, ta, tb);
-  const float l_inv = 1.f / l;
+  const float l_inv = e.l_inv;
   const float step = (tb - ta) * (1.f / 16.f);
   float Dmed = ta - 0.5f * step;
   float qx = (r.ox + ta * r.dx - (cx - 0.5f * l)) * l_inv;
@@ -205,9 +198,8 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
     for (int base = 0; base < n; base += 256) {
       if (__syncthreads_count(done) == 256) break;
       if (base + t < n) {
-        stage_entry(a.vals, rg.x + base + t, vi, a.P, a.bboxes, a.centers, a.lengths, a.geos,
-                    a.rgbs + (long long)vi * a.rgb_view_stride, kNormal, s_ch, s_ml, s_g0, s_g1, s_rgb,
-                    s_nrm, s_bb, t, r);
+        stage_entry(a, rg.x + base + t, vi, a.rgbs + (long long)vi * a.rgb_view_stride, kNormal, s_ch, s_ml,
+                    s_g0, s_g1, s_rgb, s_nrm, s_bb, t, r);
         a.cvals[rg.x + base + t] = a.vals[rg.x + base + t];
       }
       __syncthreads();
@@ -393,7 +385,7 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
     EntryRegs er;
     uint32_t nval = 0;
     if (t < kStage) {
-      if (t < n) fetch_entry(er, a.vals[rg.x + t], vi, a.P, a.bboxes, a.centers, a.lengths, a.geos, rgbs_v);
+      if (t < n) fetch_entry(er, a.vals[rg.x + t], vi, a, rgbs_v);
       if (kStage + t < n) nval = a.vals[rg.x + kStage + t];
     }
 
@@ -446,7 +438,7 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
       }
       if (fin) break;
       if (t < kStage) {
-        if (base + kStage + t < n) fetch_entry(er, nval, vi, a.P, a.bboxes, a.centers, a.lengths, a.geos, rgbs_v);
+        if (base + kStage + t < n) fetch_entry(er, nval, vi, a, rgbs_v);
         if (base + 2 * kStage + t < n) nval = a.vals[rg.x + base + 2 * kStage + t];
       }
       const int cnt = min(kStage, n - base);

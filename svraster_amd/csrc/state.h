@@ -254,6 +254,7 @@ struct FwdArgs {
   const uint2* ranges; const uint32_t* vals;
   const uint2* bboxes; long long P;
   const float* centers; const float* lengths; const float* geos; const float* rgbs;
+  const VoxelRecord* vrec;         // per-voxel records (multi-view path); null: centers / lengths / geos
   long long rgb_view_stride;       // floats between the colour arrays of consecutive views (0: shared)
   const float* s_val;
   uint32_t* tile_last; uint32_t* n_contrib;
@@ -273,6 +274,7 @@ struct BwdArgs {
   const uint2* ranges; const uint32_t* tile_last; const uint32_t* vals;
   const uint2* bboxes; long long P;
   const float* centers; const float* lengths; const float* geos; const float* rgbs;
+  const VoxelRecord* vrec;         // per-voxel records (multi-view path); null: centers / lengths / geos
   long long rgb_view_stride;
   const float* s_val;
   const float* T; const uint32_t* n_contrib;

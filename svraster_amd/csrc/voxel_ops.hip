@@ -591,6 +591,52 @@ __global__ __launch_bounds__(256) void voxel_activity_kernel(int P, const uint8_
   out[i] = keep;
 }
 
+// Voxel records of the multi-view compositors (VoxelRecord): corner SDFs gathered from the grid
+// points through vox_key (the reference's corner gather, geo_params_gather.cu:134-152), centre,
+// size and 1/size, one 64-byte record per voxel; and, when bin_leaf is given, the voxels that
+// enter binning: leaves whose corner SDFs can reach the minimum alpha (with cull; every leaf
+// without).
+__global__ __launch_bounds__(256) void voxel_records_kernel(int P, const int64_t* __restrict__ vox_key,
+                                                            const float* __restrict__ grid_pts,
+                                                            const float* __restrict__ centers,
+                                                            const float* __restrict__ lengths,
+                                                            const uint8_t* __restrict__ is_leaf,
+                                                            const float* __restrict__ s_val, float cam_scale, int cull,
+                                                            VoxelRecord* __restrict__ rec,
+                                                            uint8_t* __restrict__ bin_leaf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const int64_t* k = vox_key + 8LL * i;
+  const int64_t k0 = k[0], k1 = k[1], k2 = k[2], k3 = k[3], k4 = k[4], k5 = k[5], k6 = k[6], k7 = k[7];
+  const float4 g0 = make_float4(grid_pts[k0], grid_pts[k1], grid_pts[k2], grid_pts[k3]);
+  const float4 g1 = make_float4(grid_pts[k4], grid_pts[k5], grid_pts[k6], grid_pts[k7]);
+  const float cx = centers[3 * i], cy = centers[3 * i + 1], cz = centers[3 * i + 2];
+  const float l = lengths[i];
+  VoxelRecord* R = rec + i;
+  R->cl = make_float4(cx, cy, cz, l);
+  R->g0 = g0;
+  R->g1 = g1;
+  R->aux = make_float4(1.f / l, 0.f, 0.f, 0.f);
+  if (bin_leaf) {
+    uint8_t keep = 0;
+    if (is_leaf[i]) {
+      const float g[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+      const float c = fmaxf(fabsf(cx), fmaxf(fabsf(cy), fabsf(cz)));
+      keep = (!cull || voxel_may_contribute(g, *s_val, cam_scale + c, l)) ? 1 : 0;
+    }
+    bin_leaf[i] = keep;
+  }
+}
+
+hipError_t launch_voxel_records(int P, const int64_t* vox_key, const float* grid_pts, const float* centers,
+                                const float* lengths, const uint8_t* is_leaf, const float* s_val, float cam_scale,
+                                int cull, VoxelRecord* rec, uint8_t* bin_leaf, hipStream_t st) {
+  if (P <= 0) return hipSuccess;
+  voxel_records_kernel<<<blocks_for(P), 256, 0, st>>>(P, vox_key, grid_pts, centers, lengths, is_leaf, s_val,
+                                                       cam_scale, cull, rec, bin_leaf);
+  return hipGetLastError();
+}
+
 hipError_t launch_voxel_activity(int P, const uint8_t* is_leaf, const float* geos, const float* s_val,
                                  const float* centers, const float* lengths, float cam_scale, uint8_t* out,
                                  hipStream_t st) {

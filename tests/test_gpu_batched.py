@@ -199,3 +199,42 @@ def test_batched_key_capacity_is_enforced():
     big.vox_size.mul_(2.0)
     with pytest.raises(RuntimeError, match="capacity"):
         big.forward(m._geo_grid_pts.detach(), m._sh0.detach(), m._shs.detach(), s_val)
+
+
+@pytest.mark.gpu
+def test_voxel_records_gather_corners_and_activity():
+    """svr_batch_voxel_records: per voxel the centre, size, the corner SDFs of GatherGeoParams
+    (grid_pts[vox_key]) and 1/size, bit for bit; its binning mask equals svr_voxel_activity's on the
+    separately gathered corners (cull) and is_leaf itself (no cull)."""
+    import ctypes
+    from svraster_amd import _lib
+    say("[batched] voxel records vs the corner gather and the activity test")
+    w, sc, cams, targets = _setup()
+    lib = _lib.load()
+    P = sc.num_voxels
+    m = VoxelModel(sc)
+    grid = m._geo_grid_pts.detach().reshape(-1).contiguous()
+    leaf = sc.is_leaf.reshape(-1).to(torch.uint8).contiguous()
+    s_val = torch.tensor([float(w.s_val)], device="cuda")
+    rec = torch.empty(P, 16, device="cuda")
+    cam_scale = 3.0
+    for cull in (1, 0):
+        bl = torch.empty(P, dtype=torch.uint8, device="cuda")
+        _lib.check(lib.svr_batch_voxel_records(P, sc.vox_key.data_ptr(), grid.data_ptr(), sc.vox_center.data_ptr(),
+                                               sc.vox_size.data_ptr(), leaf.data_ptr(), s_val.data_ptr(),
+                                               ctypes.c_float(cam_scale), cull, rec.data_ptr(), bl.data_ptr(), None),
+                   "svr_batch_voxel_records")
+        geos = grid[sc.vox_key]
+        size = sc.vox_size.reshape(-1)
+        assert torch.equal(rec[:, 0:3], sc.vox_center) and torch.equal(rec[:, 3], size)
+        assert torch.equal(rec[:, 4:12], geos)
+        assert torch.equal(rec[:, 12].cpu(), 1.0 / size.cpu()) and not rec[:, 13:16].any()
+        if cull:
+            act = torch.empty(P, dtype=torch.uint8, device="cuda")
+            _lib.check(lib.svr_voxel_activity(P, leaf.data_ptr(), geos.contiguous().data_ptr(), s_val.data_ptr(),
+                                              sc.vox_center.data_ptr(), sc.vox_size.data_ptr(),
+                                              ctypes.c_float(cam_scale), act.data_ptr(), None), "svr_voxel_activity")
+            say(f"  cull: {int(bl.sum())} of {int(leaf.sum())} leaves enter binning")
+            assert torch.equal(bl, act) and 0 < int(bl.sum()) <= int(leaf.sum())
+        else:
+            assert torch.equal(bl, leaf)
