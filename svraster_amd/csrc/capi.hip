@@ -249,7 +249,7 @@ int svr_rasterize_preprocess(int W, int H, float tanfovx, float tanfovy, float c
   SVR_TRY(launch_init_view(gp.views, v, w2c, c2w, nullptr, st));
   SVR_TRY(launch_fill_u32(gp.misc, 1, (uint32_t)kRankBits, st));
   SVR_TRY(launch_preprocess(gp.views, 1, P, vox_centers, vox_lengths, is_leaf, octree_paths, out_n_dup,
-                            gp.n_dup, gp.bbox, gp.quad, gp.misc, st));
+                            gp.n_dup, gp.bbox, gp.tbox, gp.quad, gp.misc, st));
   return 0;
 }
 
@@ -316,7 +316,7 @@ int svr_rasterize_forward(int vox_geo_mode, int density_mode, int W, int H, floa
   memset(&a, 0, sizeof(a));
   a.views = ip.views; a.n_views = 1; a.total_tiles = (int)n_tiles;
   a.ranges = ip.ranges; a.vals = bp.vals;
-  a.bboxes = gp.bbox; a.P = P;
+  a.bboxes = gp.bbox; a.tboxes = gp.tbox; a.P = P;
   a.centers = vox_centers; a.lengths = vox_lengths; a.geos = geos; a.rgbs = rgbs;
   a.s_val = s_val;
   a.tile_last = ip.tile_last; a.n_contrib = ip.n_contrib;
@@ -850,7 +850,7 @@ int svr_batch_preprocess(void* ws, int n_views, int P, long long n_pixels, long 
   BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
   const long long n = 8LL * P;
   SVR_TRY(launch_preprocess_views(bp.g.views, n_views, P, vox_centers, vox_lengths, is_leaf, nullptr, bp.g.bbox,
-                                  bp.g.quad, bp.pr.pair_cnt, bp.pr.vmask, st));
+                                  bp.g.tbox, bp.g.quad, bp.pr.pair_cnt, bp.pr.vmask, st));
   SVR_TRY(launch_pair_gather(n, pair_order, bp.pr.pair_cnt, bp.pr.cnt_ranked, st));
   SVR_TRY(launch_scan(bp.pr.cnt_ranked, nullptr, n, bp.pr.partial, bp.pr.scan_ranked, true, st));
   SVR_TRY(launch_key_peak(bp.pr.scan_ranked + (n - 1), bp.g.misc + 8, st));
@@ -921,7 +921,7 @@ int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long l
   memset(&a, 0, sizeof(a));
   a.views = bp.g.views; a.n_views = n_views; a.total_tiles = (int)n_tiles;
   a.ranges = bp.im.ranges; a.vals = bp.b.vals;
-  a.bboxes = bp.g.bbox; a.P = P;
+  a.bboxes = bp.g.bbox; a.tboxes = bp.g.tbox; a.P = P;
   a.vrec = reinterpret_cast<const VoxelRecord*>(vox_records); a.rgbs = rgbs_views;
   a.rgb_view_stride = 3LL * P;
   a.s_val = s_val;

@@ -35,12 +35,21 @@ __global__ void init_view_kernel(ViewDesc* out, ViewDesc base, const float* w2c,
 // ------------------------------------------------------------------------------------------
 // bbox, octant bits and key count of voxel (c, half size r) in view v (preprocess.cu:60-153);
 // shared by the per-view and the voxel-major batched kernels so both give identical bits.
+// tb: the pixels whose centre lies in the projected corners' rectangle widened by kTboxMargin,
+// clipped to bb (packed like bb; empty when x0 > x1 or y0 > y1).  A ray through a pixel centre
+// outside it cannot meet the cube when every corner is in front of the near plane (the cube's
+// projection is the convex hull of its corners' projections); the margin covers FP32 rounding of
+// the projection and of the compositors' ray / box test (a few 1e-3 pixel at most).  With a
+// corner behind the near plane the projection is unbounded and tb = bb.
+constexpr float kTboxMargin = 0.03125f;
+
 __device__ inline void preprocess_one(const ViewDesc& v, bool leaf, float vcx, float vcy, float vcz, float vlen,
-                                      uint32_t& nd, uint2& bb, uint32_t& qbits, uint32_t& ntiles) {
+                                      uint32_t& nd, uint2& bb, uint2& tb, uint32_t& qbits, uint32_t& ntiles) {
 #pragma clang fp contract(off)
   nd = 0;
   ntiles = 0;
   bb = make_uint2(0u, 0u);
+  tb = bb;
   qbits = 0;
   if (leaf) {
     const float r = 0.5f * vlen;
@@ -106,8 +115,18 @@ __device__ inline void preprocess_one(const ViewDesc& v, bool leaf, float vcx, f
       const float bx1 = fminf(v.focal_x * mxx + cxh, (float)v.W);
       const float by1 = fminf(v.focal_y * mxy + cyh, (float)v.H);
       if (!(bx0 > bx1 || by0 > by1)) {
-        bb.x = ((uint32_t)__builtin_rintf(bx0) << 16) | (uint32_t)__builtin_rintf(by0);
-        bb.y = ((uint32_t)__builtin_rintf(bx1) << 16) | (uint32_t)__builtin_rintf(by1);
+        const uint32_t rx0 = (uint32_t)__builtin_rintf(bx0), ry0 = (uint32_t)__builtin_rintf(by0);
+        const uint32_t rx1 = (uint32_t)__builtin_rintf(bx1), ry1 = (uint32_t)__builtin_rintf(by1);
+        bb.x = (rx0 << 16) | ry0;
+        bb.y = (rx1 << 16) | ry1;
+        tb = bb;
+        if (front) {
+          const uint32_t tx0 = max(rx0, (uint32_t)ceilf(bx0 - kTboxMargin));
+          const uint32_t ty0 = max(ry0, (uint32_t)ceilf(by0 - kTboxMargin));
+          const uint32_t tx1 = min(rx1, (uint32_t)floorf(bx1 + kTboxMargin));
+          const uint32_t ty1 = min(ry1, (uint32_t)floorf(by1 + kTboxMargin));
+          tb = make_uint2((tx0 << 16) | ty0, (tx1 << 16) | ty1);
+        }
         int tx0, ty0, tx1, ty1;
         bbox_tiles(bb.x, bb.y, v, tx0, ty0, tx1, ty1);
         ntiles = (uint32_t)((1 + ty1 - ty0) * (1 + tx1 - tx0));
@@ -115,15 +134,15 @@ __device__ inline void preprocess_one(const ViewDesc& v, bool leaf, float vcx, f
       }
     }
   }
-  if (nd == 0) { bb = make_uint2(0u, 0u); qbits = 0; ntiles = 0; }
+  if (nd == 0) { bb = make_uint2(0u, 0u); tb = bb; qbits = 0; ntiles = 0; }
 }
 
 __global__ __launch_bounds__(256) void preprocess_kernel(
     const ViewDesc* __restrict__ views, int P, const float* __restrict__ centers,
     const float* __restrict__ lengths, const uint8_t* __restrict__ is_leaf,
     const int64_t* __restrict__ octpaths, int32_t* __restrict__ out_n_dup,
-    uint32_t* __restrict__ n_dup, uint2* __restrict__ bboxes, uint8_t* __restrict__ quads,
-    uint32_t* __restrict__ tz_min) {
+    uint32_t* __restrict__ n_dup, uint2* __restrict__ bboxes, uint2* __restrict__ tboxes,
+    uint8_t* __restrict__ quads, uint32_t* __restrict__ tz_min) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int view = blockIdx.y;
@@ -131,13 +150,14 @@ __global__ __launch_bounds__(256) void preprocess_kernel(
   const ViewDesc& v = views[view];
   const long long o = (long long)view * P + i;
   uint32_t nd, ntiles;
-  uint2 bb;
+  uint2 bb, tb;
   uint32_t qbits;
   preprocess_one(v, is_leaf[i] != 0, centers[3 * i + 0], centers[3 * i + 1], centers[3 * i + 2], lengths[i], nd, bb,
-                 qbits, ntiles);
+                 tb, qbits, ntiles);
   if (out_n_dup) out_n_dup[o] = (int32_t)nd;
   n_dup[o] = nd;
   bboxes[o] = bb;
+  tboxes[o] = tb;
   quads[o] = (uint8_t)qbits;
   if (tz_min) {
     // Number of low rank bits (a multiple of 3) that are zero in this voxel's octpath; reduced
@@ -537,8 +557,8 @@ __global__ __launch_bounds__(256) void pair_rank_kernel(long long n_pairs, const
 __global__ __launch_bounds__(256) void preprocess_views_kernel(
     const ViewDesc* __restrict__ views, int V, int P, const float* __restrict__ centers,
     const float* __restrict__ lengths, const uint8_t* __restrict__ is_leaf, uint32_t* __restrict__ n_dup,
-    uint2* __restrict__ bboxes, uint8_t* __restrict__ quads, uint32_t* __restrict__ pair_cnt,
-    uint32_t* __restrict__ pair_vmask) {
+    uint2* __restrict__ bboxes, uint2* __restrict__ tboxes, uint8_t* __restrict__ quads,
+    uint32_t* __restrict__ pair_cnt, uint32_t* __restrict__ pair_vmask) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P) return;
   const bool leaf = is_leaf[i] != 0;
@@ -549,13 +569,14 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(
   uint32_t m8[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};   // views (of the current 32) that see octant q
   for (int v = 0; v < V; ++v) {
     uint32_t nd, qbits, nt;   // nt: tiles of the bbox, the keys per octant
-    uint2 bb;
-    preprocess_one(views[v], leaf, cx, cy, cz, len, nd, bb, qbits, nt);
+    uint2 bb, tb;
+    preprocess_one(views[v], leaf, cx, cy, cz, len, nd, bb, tb, qbits, nt);
     const long long o = (long long)v * P + i;
     if (n_dup) n_dup[o] = nd;
     quads[o] = (uint8_t)qbits;
     if (nd) {
       bboxes[o] = bb;        // read only where n_dup > 0
+      tboxes[o] = tb;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         c8[q] += ((qbits >> q) & 1u) ? nt : 0u;
@@ -698,12 +719,12 @@ hipError_t launch_init_view(ViewDesc* out, const ViewDesc& base, const float* w2
 
 hipError_t launch_preprocess(const ViewDesc* views, int n_views, int P, const float* centers,
                              const float* lengths, const uint8_t* is_leaf, const int64_t* octpaths,
-                             int32_t* out_n_dup, uint32_t* n_dup, uint2* bboxes, uint8_t* quads,
+                             int32_t* out_n_dup, uint32_t* n_dup, uint2* bboxes, uint2* tboxes, uint8_t* quads,
                              uint32_t* tz_min, hipStream_t st) {
   if (P <= 0 || n_views <= 0) return hipSuccess;
   dim3 grid(ceil_div(P, 256), n_views);
   preprocess_kernel<<<grid, 256, 0, st>>>(views, P, centers, lengths, is_leaf, octpaths, out_n_dup,
-                                           n_dup, bboxes, quads, tz_min);
+                                           n_dup, bboxes, tboxes, quads, tz_min);
   return hipGetLastError();
 }
 
@@ -820,11 +841,11 @@ hipError_t launch_pair_rank(long long n_pairs, const int64_t* octpaths, uint64_t
 }
 
 hipError_t launch_preprocess_views(const ViewDesc* views, int V, int P, const float* centers, const float* lengths,
-                                   const uint8_t* is_leaf, uint32_t* n_dup, uint2* bboxes, uint8_t* quads,
-                                   uint32_t* pair_cnt, uint32_t* pair_vmask, hipStream_t st) {
+                                   const uint8_t* is_leaf, uint32_t* n_dup, uint2* bboxes, uint2* tboxes,
+                                   uint8_t* quads, uint32_t* pair_cnt, uint32_t* pair_vmask, hipStream_t st) {
   if (P <= 0 || V <= 0) return hipSuccess;
   preprocess_views_kernel<<<ceil_div(P, 256), 256, 0, st>>>(views, V, P, centers, lengths, is_leaf, n_dup, bboxes,
-                                                            quads, pair_cnt, pair_vmask);
+                                                            tboxes, quads, pair_cnt, pair_vmask);
   return hipGetLastError();
 }
 

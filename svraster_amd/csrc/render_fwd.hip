@@ -40,7 +40,7 @@ __device__ inline void stage_entry(const FwdArgs& a, uint32_t e, int view, const
                                    bool cull = false, float s = 0.f, float cam_scale = 0.f) {
   const uint32_t val = a.vals[e];
   const uint32_t vox = val & kVoxMask;
-  uint2 bb = a.bboxes[(long long)view * a.P + vox];
+  uint2 bb = a.tboxes[(long long)view * a.P + vox];
   VoxelGeom v;
   load_voxel_geom(v, a.vrec, a.centers, a.lengths, a.geos, vox);
   const float cx = v.cx, cy = v.cy, cz = v.cz, l = v.l;
@@ -75,7 +75,7 @@ __device__ inline void fetch_entry(EntryRegs& e, uint32_t val, int view, const F
                                    const float* __restrict__ rgbs) {
   const uint32_t vox = val & kVoxMask;
   e.val = val;
-  e.bb = a.bboxes[(long long)view * a.P + vox];
+  e.bb = a.tboxes[(long long)view * a.P + vox];
   load_voxel_geom(e.v, a.vrec, a.centers, a.lengths, a.geos, vox);
   e.r = rgbs[3 * vox + 0]; e.g = rgbs[3 * vox + 1]; e.b = rgbs[3 * vox + 2];
 }
@@ -273,8 +273,10 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
 // not the entries times the pixels.  Wave w takes the interleaved lattice of the tile's pixels
 // (wave_lattice: strides 2x2 on 16x16 tiles), so every entry's footprint falls about evenly on the
 // four waves and their work per round is balanced.  Per staged batch of 128 entries each wave:
-//   1. builds the 64-bit mask of every entry's pixels in its lattice (bbox, octant, pixels
-//      still unsaturated; empty for an entry whose corner SDFs cannot reach the minimum alpha),
+//   1. builds the 64-bit mask of every entry's pixels in its lattice (candidate rectangle --
+//      the pixel centres inside the projected corners' rectangle, within the reference's bbox --,
+//      octant, pixels still unsaturated; empty for an entry whose corner SDFs cannot reach the
+//      minimum alpha),
 //      lane l holding entries l and l + 64, and prefix-sums the counts: entry j owns the run
 //      [pref_j, pref_j + |m_j|) of an entry-major pair list;
 //   2. walks that list 64 pairs per instruction (the entry of a position from a prefix maximum

@@ -33,7 +33,7 @@ struct Carver {
 
 // Geometry state.
 struct GeomLayout {
-  size_t views, misc, n_dup, scan, bbox, quad, partial, total;
+  size_t views, misc, n_dup, scan, bbox, quad, partial, tbox, total;
   GeomLayout(long long n_views, long long P) {
     Carver c;
     const long long VP = n_views * P;
@@ -44,6 +44,7 @@ struct GeomLayout {
     bbox = c.take<uint2>(VP);
     quad = c.take<uint8_t>(VP);
     partial = c.take<uint32_t>((long long)scan_partial_count(VP) + 1);
+    tbox = c.take<uint2>(VP);
     total = c.off;
   }
 };
@@ -57,6 +58,10 @@ struct GeomPtrs {
   uint2* bbox;
   uint8_t* quad;
   uint32_t* partial;
+  // Per (view, voxel): the pixels whose centre lies inside the projected corners' rectangle (with
+  // a 1/32-pixel margin), packed like bbox and inside it.  Only such pixels can hit the voxel, so
+  // the compositors take their candidate pixels from it.
+  uint2* tbox;
   GeomPtrs(void* base, long long n_views, long long P) {
     GeomLayout L(n_views, P);
     char* b = (char*)base;
@@ -67,6 +72,7 @@ struct GeomPtrs {
     bbox = (uint2*)(b + L.bbox);
     quad = (uint8_t*)(b + L.quad);
     partial = (uint32_t*)(b + L.partial);
+    tbox = (uint2*)(b + L.tbox);
   }
 };
 
@@ -211,7 +217,7 @@ hipError_t launch_init_view(ViewDesc* out, const ViewDesc& base, const float* w2
                             const float* bg, hipStream_t st);
 hipError_t launch_preprocess(const ViewDesc* views, int n_views, int P, const float* centers,
                              const float* lengths, const uint8_t* is_leaf, const int64_t* octpaths,
-                             int32_t* out_n_dup, uint32_t* n_dup, uint2* bboxes, uint8_t* quads,
+                             int32_t* out_n_dup, uint32_t* n_dup, uint2* bboxes, uint2* tboxes, uint8_t* quads,
                              uint32_t* tz_min, hipStream_t st);
 hipError_t launch_scan(const uint32_t* in, const uint32_t* n_dev, long long n_cap, uint32_t* partial,
                        uint32_t* out, bool inclusive, hipStream_t st);
@@ -236,8 +242,8 @@ hipError_t launch_tile_sort(uint32_t* tiles, uint32_t* vals, uint32_t* tiles_alt
 hipError_t launch_pair_rank(long long n_pairs, const int64_t* octpaths, uint64_t* keys, uint32_t* vals,
                             hipStream_t st);
 hipError_t launch_preprocess_views(const ViewDesc* views, int V, int P, const float* centers, const float* lengths,
-                                   const uint8_t* is_leaf, uint32_t* n_dup, uint2* bboxes, uint8_t* quads,
-                                   uint32_t* pair_cnt, uint32_t* pair_vmask, hipStream_t st);
+                                   const uint8_t* is_leaf, uint32_t* n_dup, uint2* bboxes, uint2* tboxes,
+                                   uint8_t* quads, uint32_t* pair_cnt, uint32_t* pair_vmask, hipStream_t st);
 // *peak = max(*peak, *total): the largest key count asked of a fixed-capacity binning state.
 hipError_t launch_key_peak(const uint32_t* total, uint32_t* peak, hipStream_t st);
 hipError_t launch_pair_gather(long long n_pairs, const uint32_t* order, const uint32_t* pair_cnt,
@@ -253,6 +259,7 @@ struct FwdArgs {
   const ViewDesc* views; int n_views; int total_tiles;
   const uint2* ranges; const uint32_t* vals;
   const uint2* bboxes; long long P;
+  const uint2* tboxes;             // candidate pixel rectangles (GeomPtrs::tbox), indexed like bboxes
   const float* centers; const float* lengths; const float* geos; const float* rgbs;
   const VoxelRecord* vrec;         // per-voxel records (multi-view path); null: centers / lengths / geos
   long long rgb_view_stride;       // floats between the colour arrays of consecutive views (0: shared)
