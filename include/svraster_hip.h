@@ -330,7 +330,11 @@ int svr_batch_setup(void* ws, int n_views, int P, long long n_pixels, long long 
 size_t svr_batch_pair_order_bytes(int P);
 int svr_batch_pair_order(int P, const int64_t* octree_paths, void* tmp, uint32_t* order_out, void* stream);
 /* Preprocess (bbox and octant set; preprocess.cu:60-153) of every (view, voxel), voxel-major,
- * and the key count of every (voxel, octant) pair over all views, scanned in rank order. */
+ * and the key count of every (voxel, octant) pair over all views, scanned in rank order.  Keys
+ * cover the tiles of each (view, voxel)'s candidate rectangle -- the pixels whose centre lies in
+ * the projected corners' rectangle (1/32-pixel margin) -- rather than its bbox, and a (view,
+ * voxel) with no such pixel emits none: each tile's list is the reference's list less entries
+ * whose ray / box test fails at every pixel of the tile, in the reference's order. */
 int svr_batch_preprocess(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                          const float* vox_centers, const float* vox_lengths, const uint8_t* is_leaf,
                          const uint32_t* pair_order, void* stream);

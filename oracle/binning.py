@@ -46,6 +46,15 @@ class Preprocessed:
     quad_bits: np.ndarray    # uint32 [P]
     tiles_x: int
     tiles_y: int
+    tbox: np.ndarray = None  # uint32 [P,2]: candidate pixels (see TBOX_MARGIN), packed like bbox
+
+
+# Candidate-pixel rectangle of the build's compositors (not a reference quantity): when every
+# corner is in front of the near plane (depth also within [2^-100, 2^100]), the pixels whose
+# centre lies in the projected corners' rectangle widened by TBOX_MARGIN, clipped to bbox
+# (empty when x0 > x1 or y0 > y1); otherwise bbox itself.  In pixel-index units the rectangle
+# is [bmin, bmax] (the pre-rounding bbox values), so the range is [ceil(bmin - m), floor(bmax + m)].
+TBOX_MARGIN = f32(0.03125)
 
 
 def focal(W: int, tanfov: float) -> np.float32:
@@ -72,6 +81,7 @@ def preprocess(W, H, tanfovx, tanfovy, cx, cy, w2c, c2w, near,
     alive = (is_leaf != 0) & ~(d2 < near * near)
 
     qbits = np.zeros(P, dtype=np.uint32)
+    front = np.ones(P, dtype=bool)
     cmin = np.full((P, 2), f32(1e9), dtype=f32)
     cmax = np.full((P, 2), f32(-1e9), dtype=f32)
     for i in range(8):
@@ -84,6 +94,7 @@ def preprocess(W, H, tanfovx, tanfovy, cx, cy, w2c, c2w, near,
             m[4] * wc[:, 0] + m[5] * wc[:, 1] + m[6] * wc[:, 2] + m[7],
             m[8] * wc[:, 0] + m[9] * wc[:, 1] + m[10] * wc[:, 2] + m[11]], -1)
         ok = ~(cam[:, 2] < near)
+        front &= ok & (cam[:, 2] >= f32(2.0 ** -100)) & (cam[:, 2] <= f32(2.0 ** 100))
         with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
             inv_z = f32(1.0) / cam[:, 2]
             co = np.stack([cam[:, 0] * inv_z, cam[:, 1] * inv_z], -1)
@@ -115,7 +126,41 @@ def preprocess(W, H, tanfovx, tanfovy, cx, cy, w2c, c2w, near,
     tiles = (1 + tmax[:, 1] - tmin[:, 1]) * (1 + tmax[:, 0] - tmin[:, 0])
     popc = np.array([bin(b).count("1") for b in range(256)], dtype=np.int64)[qbits & 0xFF]
     n_dup = np.where(alive, tiles * popc, 0).astype(np.int32)
-    return Preprocessed(n_dup, bbox, qbits, tiles_x, tiles_y)
+
+    def tight(v, up):
+        with np.errstate(over="ignore", invalid="ignore"):
+            c = np.ceil(v - TBOX_MARGIN) if up else np.floor(v + TBOX_MARGIN)
+        return np.where(alive, c, f32(0.0)).astype(np.int64)
+
+    bx = [rint_u(v).astype(np.int64) for v in (bmin_x, bmin_y, bmax_x, bmax_y)]
+    t0x = np.maximum(bx[0], tight(bmin_x, True))
+    t0y = np.maximum(bx[1], tight(bmin_y, True))
+    t1x = np.minimum(bx[2], tight(bmax_x, False))
+    t1y = np.minimum(bx[3], tight(bmax_y, False))
+    tbox = np.zeros((P, 2), dtype=np.uint32)
+    tbox[:, 0] = ((t0x << 16) | t0y).astype(np.uint32)
+    tbox[:, 1] = ((t1x << 16) | t1y).astype(np.uint32)
+    tbox = np.where((alive & front)[:, None], tbox, bbox)
+    return Preprocessed(n_dup, bbox, qbits, tiles_x, tiles_y, tbox)
+
+
+def tbox_empty(tbox: np.ndarray) -> np.ndarray:
+    return ((tbox[:, 0] >> 16) > (tbox[:, 1] >> 16)) | ((tbox[:, 0] & 0xFFFF) > (tbox[:, 1] & 0xFFFF))
+
+
+def tight_binning_input(pre: Preprocessed) -> Preprocessed:
+    """The preprocess output the build's multi-view binning uses: each (view, voxel) binned over
+    the tiles of its candidate rectangle instead of its bbox, and left out when that rectangle
+    holds no pixel (its keys could only land on pixels whose ray misses the voxel)."""
+    empty = tbox_empty(pre.tbox)
+    alive = (pre.n_dup > 0) & ~empty
+    tmin, tmax = tile_rect(pre.tbox, pre.tiles_x, pre.tiles_y)
+    tiles = (1 + tmax[:, 1] - tmin[:, 1]) * (1 + tmax[:, 0] - tmin[:, 0])
+    popc = np.array([bin(b).count("1") for b in range(256)], dtype=np.int64)[pre.quad_bits & 0xFF]
+    n_dup = np.where(alive, tiles * popc, 0).astype(np.int32)
+    box = np.where(alive[:, None], pre.tbox, 0).astype(np.uint32)
+    q = np.where(alive, pre.quad_bits, 0).astype(np.uint32)
+    return Preprocessed(n_dup, box, q, pre.tiles_x, pre.tiles_y, box)
 
 
 def tile_rect(bbox: np.ndarray, tiles_x: int, tiles_y: int):

@@ -41,8 +41,12 @@ __global__ void init_view_kernel(ViewDesc* out, ViewDesc base, const float* w2c,
 // projection is the convex hull of its corners' projections); the margin covers FP32 rounding of
 // the projection and of the compositors' ray / box test (a few 1e-3 pixel at most).  With a
 // corner behind the near plane the projection is unbounded and tb = bb.
+// kTight (the multi-view path): bin over tb's tiles instead of bb's, and leave the (view, voxel)
+// out (no keys, no octants) when tb holds no pixel; bb is then tb.  The per-view path keeps the
+// reference's keys (kTight = false).
 constexpr float kTboxMargin = 0.03125f;
 
+template <bool kTight>
 __device__ inline void preprocess_one(const ViewDesc& v, bool leaf, float vcx, float vcy, float vcz, float vlen,
                                       uint32_t& nd, uint2& bb, uint2& tb, uint32_t& qbits, uint32_t& ntiles) {
 #pragma clang fp contract(off)
@@ -126,6 +130,10 @@ __device__ inline void preprocess_one(const ViewDesc& v, bool leaf, float vcx, f
           const uint32_t tx1 = min(rx1, (uint32_t)floorf(bx1 + kTboxMargin));
           const uint32_t ty1 = min(ry1, (uint32_t)floorf(by1 + kTboxMargin));
           tb = make_uint2((tx0 << 16) | ty0, (tx1 << 16) | ty1);
+          if (kTight) {
+            bb = tb;
+            if (tx0 > tx1 || ty0 > ty1) qbits = 0;
+          }
         }
         int tx0, ty0, tx1, ty1;
         bbox_tiles(bb.x, bb.y, v, tx0, ty0, tx1, ty1);
@@ -152,8 +160,8 @@ __global__ __launch_bounds__(256) void preprocess_kernel(
   uint32_t nd, ntiles;
   uint2 bb, tb;
   uint32_t qbits;
-  preprocess_one(v, is_leaf[i] != 0, centers[3 * i + 0], centers[3 * i + 1], centers[3 * i + 2], lengths[i], nd, bb,
-                 tb, qbits, ntiles);
+  preprocess_one<false>(v, is_leaf[i] != 0, centers[3 * i + 0], centers[3 * i + 1], centers[3 * i + 2], lengths[i], nd,
+                        bb, tb, qbits, ntiles);
   if (out_n_dup) out_n_dup[o] = (int32_t)nd;
   n_dup[o] = nd;
   bboxes[o] = bb;
@@ -553,7 +561,9 @@ __global__ __launch_bounds__(256) void pair_rank_kernel(long long n_pairs, const
 
 // Voxel-major preprocess of all views: the voxel is read once, its bbox / octants / n_dup are
 // written per view, and per octant q the number of keys it emits over all views (tiles of the
-// views whose octant set holds q) goes to pair_cnt[8 * voxel + q].
+// views whose octant set holds q) goes to pair_cnt[8 * voxel + q].  Each (view, voxel) is binned
+// over the tiles of its candidate-pixel rectangle (preprocess_one<true>): every tile list is the
+// reference's list without the entries that cannot meet any pixel of the tile, in the same order.
 __global__ __launch_bounds__(256) void preprocess_views_kernel(
     const ViewDesc* __restrict__ views, int V, int P, const float* __restrict__ centers,
     const float* __restrict__ lengths, const uint8_t* __restrict__ is_leaf, uint32_t* __restrict__ n_dup,
@@ -570,7 +580,7 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(
   for (int v = 0; v < V; ++v) {
     uint32_t nd, qbits, nt;   // nt: tiles of the bbox, the keys per octant
     uint2 bb, tb;
-    preprocess_one(views[v], leaf, cx, cy, cz, len, nd, bb, tb, qbits, nt);
+    preprocess_one<true>(views[v], leaf, cx, cy, cz, len, nd, bb, tb, qbits, nt);
     const long long o = (long long)v * P + i;
     if (n_dup) n_dup[o] = nd;
     quads[o] = (uint8_t)qbits;

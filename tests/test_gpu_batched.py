@@ -114,10 +114,13 @@ def test_batched_voxel_skip_keeps_results():
 @pytest.mark.gpu
 @pytest.mark.parametrize("n_views", [4, 40])
 def test_batched_tile_lists_match_per_view(n_views):
-    """Rank-order binning of all views at once gives every tile the per-view list (same values in
-    the same order) and the same per-pixel contributor counts.  With 40 views each (voxel, octant)
-    pair is seen by about 5 of them, so a workgroup's 256 pairs own more (pair, view) rectangles than
-    the key emission's LDS list holds at once (1024) and it runs in several batches."""
+    """Rank-order binning of all views at once, over each (view, voxel)'s candidate rectangle, gives
+    every tile the per-view (reference) list less the entries whose candidate rectangle (the
+    oracle's) misses the tile, in the same order, and the same per-pixel last contributors.  With
+    40 views each (voxel, octant) pair is seen by about 5 of them, so a workgroup's 256 pairs own
+    more (pair, view) rectangles than the key emission's LDS list holds at once (1024) and it runs
+    in several batches."""
+    from oracle import binning as ob
     from svraster_amd import _C
     say(f"[batched] per-tile lists and contributor counts vs the per-view binning, {n_views} views")
     w, sc, cams, targets = _setup(n_views)
@@ -129,6 +132,7 @@ def test_batched_tile_lists_match_per_view(n_views):
     torch.cuda.synchronize()
     ranges_b, vals_b, ncon_b, _ = r.state_views()
     tile_base = 0
+    dropped = 0
     for v, cam in enumerate(cams):
         W, H = cam.image_width, cam.image_height
         n_dup, geom = _C.rasterize_preprocess(W, H, cam.tanfovx, cam.tanfovy, cam.cx, cam.cy, cam.w2c, cam.c2w, 0,
@@ -140,21 +144,40 @@ def test_batched_tile_lists_match_per_view(n_views):
             torch.empty(sc.num_voxels, 0, device="cuda"), geom, False, s_val)
         _, vals_v = _C.unpack_BinningState(K, binning)
         rng_v, _, ncon_v = _C.unpack_ImageState(W, H, img)
-        nt = rng_v.shape[0] * rng_v.shape[1]
-        rb = ranges_b[tile_base:tile_base + nt].long()
-        rv = rng_v.reshape(nt, 2).long()
-        assert torch.equal(rb[:, 1] - rb[:, 0], rv[:, 1] - rv[:, 0]), f"view {v}: tile list lengths"
-        # Same values in the same order in every tile.
-        lens = (rv[:, 1] - rv[:, 0])
-        idx_b = torch.repeat_interleave(rb[:, 0], lens) + (torch.arange(int(lens.sum()), device="cuda")
-                                                          - torch.repeat_interleave(lens.cumsum(0) - lens, lens))
-        idx_v = torch.repeat_interleave(rv[:, 0], lens) + (torch.arange(int(lens.sum()), device="cuda")
-                                                          - torch.repeat_interleave(lens.cumsum(0) - lens, lens))
-        assert torch.equal(vals_b[idx_b], vals_v[idx_v]), f"view {v}: tile list contents/order"
-        nb = ncon_b[int(r.pix_base[v]):int(r.pix_base[v]) + W * H].view(H, W)
-        assert torch.equal(nb, ncon_v), f"view {v}: contributor counts"
-        say(f"  view {v} {W}x{H}: {int(K)} keys, lists and contributor counts identical")
+        tx_n, ty_n = rng_v.shape[1], rng_v.shape[0]
+        nt = tx_n * ty_n
+        pre = ob.preprocess(W, H, cam.tanfovx, cam.tanfovy, cam.cx, cam.cy, cam.w2c.cpu().numpy(),
+                            cam.c2w.cpu().numpy(), cam.near, sc.vox_center.cpu().numpy(), sc.vox_size.cpu().numpy(),
+                            sc.is_leaf.cpu().numpy())
+        tb = pre.tbox.astype(np.int64)
+        x0, y0, x1, y1 = tb[:, 0] >> 16, tb[:, 0] & 0xFFFF, tb[:, 1] >> 16, tb[:, 1] & 0xFFFF
+        cx0, cx1 = np.clip(x0 >> 4, 0, tx_n - 1), np.clip(x1 >> 4, 0, tx_n - 1)
+        cy0, cy1 = np.clip(y0 >> 4, 0, ty_n - 1), np.clip(y1 >> 4, 0, ty_n - 1)
+        rv = rng_v.reshape(nt, 2).long().cpu().numpy()
+        rb = ranges_b[tile_base:tile_base + nt].long().cpu().numpy()
+        vv = vals_v.cpu().numpy().view(np.uint32)
+        vb = vals_b.cpu().numpy().view(np.uint32)
+        nc_v = ncon_v.cpu().numpy().astype(np.int64)
+        nc_b = ncon_b[int(r.pix_base[v]):int(r.pix_base[v]) + W * H].view(H, W).cpu().numpy().astype(np.int64)
+        want_nc = np.zeros_like(nc_v)
+        for t in range(nt):
+            tx, ty = t % tx_n, t // tx_n
+            lst = vv[rv[t, 0]:rv[t, 1]]
+            vox = (lst & ((1 << 29) - 1)).astype(np.int64)
+            keep = ((x0[vox] <= x1[vox]) & (y0[vox] <= y1[vox]) & (cx0[vox] <= tx) & (cx1[vox] >= tx)
+                    & (cy0[vox] <= ty) & (cy1[vox] >= ty))
+            got = vb[rb[t, 0]:rb[t, 1]]
+            assert np.array_equal(got, lst[keep]), f"view {v} tile {t}: list contents/order"
+            dropped += int((~keep).sum())
+            # last contributor: the same entry, at its position in the shorter list
+            kept_before = np.concatenate([[0], np.cumsum(keep)])
+            blk = nc_v[16 * ty:16 * ty + 16, 16 * tx:16 * tx + 16]
+            want_nc[16 * ty:16 * ty + 16, 16 * tx:16 * tx + 16] = kept_before[blk]
+        assert np.array_equal(nc_b, want_nc), f"view {v}: contributor counts"
+        say(f"  view {v} {W}x{H}: {int(K)} reference keys, lists and contributor counts as expected")
         tile_base += nt
+    say(f"  {dropped} reference entries left out over all views")
+    assert dropped > 0
 
 
 @pytest.mark.gpu

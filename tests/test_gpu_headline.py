@@ -4,7 +4,9 @@ multi_32: 4M voxels (24 spheres, levels 6-12, SH degree 3), 32 views of mixed re
 batched step.  For views 0, 8, 16 and 24 (one per resolution) the 64x64 centre crop matches the
 oracle: colour and T max-abs <= 1e-4; with cotangents on those four crops only, the corner-SDF,
 sh0 and shs gradients match to relative L2 <= 1e-3 and dL/ds to relative 1e-3; the tile lists of
-the four views (binning without the activity cull) equal the oracle's sorted lists bit for bit.
+the four views (binning without the activity cull) equal, bit for bit, the oracle's sorted lists
+binned over the candidate-pixel rectangles (the reference's lists less the entries whose rectangle
+misses the tile).
 
 train_16: 16M leaves (+ non-leaf parents kept from level 9), 16 views at 1920x1080.  One captured
 training iteration (render, MSE, backward, grid eikonal + Laplacian smoothness, dense Adam) is
@@ -100,7 +102,8 @@ def test_multi_32_batched_step_vs_oracle():
         f"({e_ds:.2e})")
     assert e_g <= 1e-3 and e_0 <= 1e-3 and e_s <= 1e-3 and e_ds <= 1e-3
 
-    # tile lists bit-exact: the same batched binning without the activity cull
+    # tile lists bit-exact: the same batched binning without the activity cull, against the oracle's
+    # binning over candidate rectangles
     del r
     torch.cuda.empty_cache()
     r2 = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key, cams,
@@ -109,18 +112,23 @@ def test_multi_32_batched_step_vs_oracle():
     torch.cuda.synchronize()
     ranges_b, vals_b, _, _ = r2.state_views()
     tile_base = np.concatenate([[0], np.cumsum(r2.tiles)])
+    octpath = sc.octpath.cpu().numpy()
     for v in views:
         bn = bins[v]
+        tpre = ob.tight_binning_input(bn.pre)
+        tkeys, tvals = ob.sort_pairs(*ob.duplicate_with_keys(tpre, octpath))
+        tranges = ob.tile_ranges(tkeys, tpre.tiles_x * tpre.tiles_y)
         nt = r2.tiles[v]
         rb = ranges_b[int(tile_base[v]):int(tile_base[v]) + nt].cpu().numpy().astype(np.int64)
-        lb, lo_ = rb[:, 1] - rb[:, 0], bn.ranges[:, 1].astype(np.int64) - bn.ranges[:, 0].astype(np.int64)
+        lb, lo_ = rb[:, 1] - rb[:, 0], tranges[:, 1].astype(np.int64) - tranges[:, 0].astype(np.int64)
         assert np.array_equal(lb, lo_), f"view {v}: tile list lengths"
         ne = np.nonzero(lb)[0]
         start, end = int(rb[ne[0], 0]), int(rb[ne[-1], 1])
         got = vals_b[start:end].cpu().numpy().astype(np.uint32)
-        assert got.shape[0] == bn.vals.shape[0]
-        assert np.array_equal(got, bn.vals.astype(np.uint32)), f"view {v}: tile list contents"
-        say(f"  view {v}: {got.shape[0]} keys, every tile list equal to the oracle's")
+        assert got.shape[0] == tvals.shape[0]
+        assert np.array_equal(got, tvals.astype(np.uint32)), f"view {v}: tile list contents"
+        say(f"  view {v}: {got.shape[0]} keys ({bn.K} in the reference's binning), every tile list equal to "
+            f"the oracle's")
 
 
 @pytest.mark.gpu

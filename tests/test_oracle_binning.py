@@ -99,3 +99,78 @@ def test_ragged_and_tiny_images():
         assert bn.ranges.shape[0] == tiles_x * tiles_y
         tiles = bn.keys >> np.uint64(48)
         assert tiles.size == 0 or int(tiles.max()) < tiles_x * tiles_y
+
+
+def _tbox_ring_misses(sc, cam):
+    """Every pixel of a voxel's bbox outside its candidate rectangle (tbox): the FP64 ray through
+    its centre misses the voxel's box.  Returns the number of such pixels checked."""
+    bn = ob.binning_for_camera(cam, sc)
+    pre = bn.pre
+    bb = pre.bbox.astype(np.int64)
+    tb = pre.tbox.astype(np.int64)
+    bx0, by0, bx1, by1 = bb[:, 0] >> 16, bb[:, 0] & 0xFFFF, bb[:, 1] >> 16, bb[:, 1] & 0xFFFF
+    tx0, ty0, tx1, ty1 = tb[:, 0] >> 16, tb[:, 0] & 0xFFFF, tb[:, 1] >> 16, tb[:, 1] & 0xFFFF
+    alive = pre.n_dup > 0
+    # inside bbox (or empty)
+    assert np.all(~alive | (tx0 >= bx0) & (ty0 >= by0) & (tx1 <= bx1) & (ty1 <= by1))
+    rc = orc.camera_from(cam)
+    W, H = cam.image_width, cam.image_height
+    c = sc.vox_center.double()
+    l = sc.vox_size.double().reshape(-1)
+    pv, px, py = [], [], []
+    for v in np.nonzero(alive)[0]:
+        for y in range(by0[v], min(by1[v], H - 1) + 1):
+            for x in range(bx0[v], min(bx1[v], W - 1) + 1):
+                if not (tx0[v] <= x <= tx1[v] and ty0[v] <= y <= ty1[v]):
+                    pv.append(v); px.append(x); py.append(y)
+    if not pv:
+        return 0
+    pv = torch.tensor(pv)
+    ro, rd, _ = orc.pixel_rays(rc, torch.tensor(px), torch.tensor(py))
+    inv = 1.0 / rd
+    lo = (c[pv] - 0.5 * l[pv, None] - ro) * inv
+    hi = (c[pv] + 0.5 * l[pv, None] - ro) * inv
+    a = torch.minimum(lo, hi).max(dim=1).values
+    b = torch.maximum(lo, hi).min(dim=1).values
+    assert bool((a > b).all()), int((a <= b).sum())
+    return len(pv)
+
+
+def test_candidate_rectangle_holds_every_hit():
+    """The candidate rectangle drops only pixels whose ray misses the box: checked pixel by pixel
+    (FP64 rays) on every voxel of two views, one with many sub-pixel voxels."""
+    w = make_workload("small_128")
+    n1 = _tbox_ring_misses(w.scene, w.cameras[0])
+    far = look_at_camera(np.array([3.5, -4.0, 2.0]), 97, 61, 60.0, 60.0)
+    n2 = _tbox_ring_misses(w.scene, far)
+    assert n1 > 1000 and n2 > 1000, (n1, n2)
+
+
+def test_tight_binning_is_the_reference_list_filtered():
+    """Binning over candidate rectangles gives each tile the reference list without the entries
+    whose rectangle misses the tile (or is empty), in the reference order."""
+    w = make_workload("small_128")
+    for cam in (w.cameras[0], look_at_camera(np.array([3.5, -4.0, 2.0]), 97, 61, 60.0, 60.0)):
+        _check_tight_lists(w.scene, cam)
+
+
+def _check_tight_lists(sc, cam):
+    bn = ob.binning_for_camera(cam, sc)
+    tpre = ob.tight_binning_input(bn.pre)
+    keys, vals = ob.duplicate_with_keys(tpre, sc.octpath.numpy())
+    skeys, svals = ob.sort_pairs(keys, vals)
+    rt = ob.tile_ranges(skeys, tpre.tiles_x * tpre.tiles_y)
+    tb = bn.pre.tbox.astype(np.int64)
+    x0, y0, x1, y1 = tb[:, 0] >> 16, tb[:, 0] & 0xFFFF, tb[:, 1] >> 16, tb[:, 1] & 0xFFFF
+    cx0, cx1 = np.clip(x0 >> 4, 0, tpre.tiles_x - 1), np.clip(x1 >> 4, 0, tpre.tiles_x - 1)
+    cy0, cy1 = np.clip(y0 >> 4, 0, tpre.tiles_y - 1), np.clip(y1 >> 4, 0, tpre.tiles_y - 1)
+    dropped = 0
+    for t in range(tpre.tiles_x * tpre.tiles_y):
+        tx, ty = t % tpre.tiles_x, t // tpre.tiles_x
+        lst = bn.vals[bn.ranges[t, 0]:bn.ranges[t, 1]]
+        vox = (lst & ((1 << 29) - 1)).astype(np.int64)
+        keep = ((x0[vox] <= x1[vox]) & (y0[vox] <= y1[vox]) & (cx0[vox] <= tx) & (cx1[vox] >= tx)
+                & (cy0[vox] <= ty) & (cy1[vox] >= ty))
+        assert np.array_equal(svals[rt[t, 0]:rt[t, 1]], lst[keep]), t
+        dropped += int((~keep).sum())
+    assert 0 < dropped < bn.K
