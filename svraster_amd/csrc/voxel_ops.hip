@@ -387,7 +387,41 @@ __global__ __launch_bounds__(256) void sh_views_backward_kernel(int P, int V, co
     const long long e0 = (i0 + p0) * row;
     const int n = pn * row;
     if (kAdam) {
-      for (int k = t; k < n; k += 256) {
+      // The group's elements are contiguous in sh / m / v: whole float4s (the group starts
+      // 16-byte aligned when the arrays do, as (i0 + p0) * row is a multiple of 4), with the
+      // loads of three float4 steps issued before their updates; a tail of n % 4 elements, and
+      // unaligned arrays, take the scalar loop.
+      const bool vec = ((reinterpret_cast<uintptr_t>(A.shs + e0) | reinterpret_cast<uintptr_t>(A.ms + e0) |
+                         reinterpret_cast<uintptr_t>(A.vs + e0)) & 15u) == 0;
+      const int n4 = vec ? (n >> 2) : 0;
+      float4* P4 = reinterpret_cast<float4*>(A.shs + e0);
+      float4* M4 = reinterpret_cast<float4*>(A.ms + e0);
+      float4* V4 = reinterpret_cast<float4*>(A.vs + e0);
+      const float4* G4 = reinterpret_cast<const float4*>(s_rows);
+      for (int k0 = t; k0 < n4; k0 += 3 * 256) {
+        float4 pv[3], mv[3], vv[3];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+          const int k = k0 + u * 256;
+          if (k < n4) { pv[u] = P4[k]; mv[u] = M4[k]; vv[u] = V4[k]; }
+        }
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+          const int k = k0 + u * 256;
+          if (k >= n4) continue;
+          const float4 g = G4[k];
+          float* pp = &pv[u].x;
+          float* mm = &mv[u].x;
+          float* qq = &vv[u].x;
+          const float gg[4] = {g.x, g.y, g.z, g.w};
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            if (!A.sparse || gg[c] != 0.0f)
+              adam_update<false>(pp[c], mm[c], qq[c], gg[c], hs.x, A.ts.b1, A.ts.b2, hs.y, A.ts.eps);
+          P4[k] = pv[u]; M4[k] = mv[u]; V4[k] = vv[u];
+        }
+      }
+      for (int k = 4 * n4 + t; k < n; k += 256) {
         const float g = s_rows[k];
         if (A.sparse && g == 0.0f) continue;
         const long long e = e0 + k;
