@@ -257,7 +257,9 @@ def kernel_times(lib, fn):
 def train_iteration_bytes(it, smooth):
     """Algorithmic HBM bytes of the non-render stages of one training iteration.
     loss: colour and the scaled target read, dL/dcolour written (3 x 12 B per ray; with views of
-    different sizes a per-element scale is read too, 4 x 12 B).
+    different sizes a per-element scale is read too, 4 x 12 B); 0 when the compositor writes
+    dL/dcolour itself (fused loss, the default), whose 24 B per ray (target read, gradient
+    written) are then part of the render stage.
     smooth: every grid key once (key 4, active index 4, voxel id 4), the mask and dense index of
     the res^3 grid once (1 + 4 B per cell), the voxel data of the table voxels once (corner ids
     64, grid coordinate 12, size 4), corner SDFs read once and the gradient read + written
@@ -265,7 +267,7 @@ def train_iteration_bytes(it, smooth):
     adam: corner SDFs, their moments and gradient (p, m, v read + written, g read: 28 B each)."""
     R = it.r.n_pix
     M = it.grid_pts.numel()
-    out = {"loss": (36.0 if it.loss_grad.scale is None else 48.0) * R, "adam": 28.0 * M}
+    out = {"loss": 0.0 if it.fuse_loss else (36.0 if it.loss_grad.scale is None else 48.0) * R, "adam": 28.0 * M}
     if smooth is not None:
         n_tab_vox = int(smooth.g2v.unique().numel())
         out["smooth"] = 12.0 * smooth.G + 5.0 * smooth.res ** 3 + 80.0 * n_tab_vox + 12.0 * M
@@ -388,7 +390,8 @@ def bench_train(args):
     log(f"[bench] batched graph iteration: {ms_b:.2f} ms; stages (eager) {st_b}")
 
     nbytes = train_iteration_bytes(it, smooth)
-    stage_gbs = {k: round(nbytes[k] / (st_b[k] / 1000.0) / 1e9, 1) for k in nbytes if st_b.get(k, 0) > 0}
+    stage_gbs = {k: round(nbytes[k] / (st_b[k] / 1000.0) / 1e9, 1) for k in nbytes
+                 if nbytes[k] > 0 and st_b.get(k, 0) > 0}
     roof = roofline_block(rast, kt_b, args.workload)
     it.check_capacity()
     value = rays / (ms_b / 1000.0) / 1e6
@@ -424,8 +427,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="multi_32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-full-frames", action="store_true",
-                    help="CPU leg also times the 256x256 fd_256 frame (about a minute of host time)")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
     ap.add_argument("--no-autotune", action="store_true", help="default launch configuration, no tuning")
     ap.add_argument("--batched-only", action="store_true",
@@ -569,7 +570,8 @@ def main():
     }
     if not args.no_cpu_baseline and not args.profile:
         log("[bench] CPU oracle baseline ...")
-        frames = ("small_128", "fd_256") if args.cpu_full_frames else ("small_128",)
+        # whole frames of the two CPU-sized workloads (128x128 and 256x256, about 15 s together)
+        frames = ("small_128", "fd_256")
         result["cpu_baseline"], crop_err = cpu_baseline(args.workload, rast.view_slice(rast.color, 3, 0), frames)
         result["paths"]["image_max_abs_vs_fp64_oracle_crop"] = crop_err
     print(json.dumps(result), flush=True)

@@ -364,11 +364,13 @@ int svr_batch_sh_forward(void* ws, int n_views, int P, long long n_pixels, long 
  * tile's entries then follow the reference's order, forward.cu:749-829), tile ranges, and one
  * persistent compositor launch over all (view, tile) pairs, the entries' geometry read from
  * svr_batch_voxel_records' records.  Colour and T need no clearing; depth is zeroed by the
- * caller. */
+ * caller.  With mse_target (laid out like out_color) and out_dL_dcolor, each pixel write also
+ * writes the gradient of its view's mean squared colour error, (color - target) * 2 / (3 W H):
+ * the training step's loss fused into the compositor (both null: not written). */
 int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                         const float* vox_records, const float* rgbs_views, const float* s_val, const uint32_t* pair_order, int need_depth,
                         int need_normal, float* out_color, float* out_depth, float* out_normal, float* out_T,
-                        void* stream);
+                        const float* mse_target, float* out_dL_dcolor, void* stream);
 /* One persistent reverse compositor launch over all (view, tile) pairs (geometry from the same
  * records as the forward).  dL_dvox [P,12]
  * (colour slots unused), dL_drgb_views [V,P,3] and dL_ds [1] zeroed by the caller.  With

@@ -190,8 +190,13 @@ class BatchedRasterizer:
                    "svr_batch_key_count")
         return int(k.value)
 
-    def forward(self, grid_pts: torch.Tensor, sh0: torch.Tensor, shs: torch.Tensor, s_val: torch.Tensor):
-        """Render every view.  Returns the concatenated colour buffer (views [3,H,W] back to back)."""
+    def forward(self, grid_pts: torch.Tensor, sh0: torch.Tensor, shs: torch.Tensor, s_val: torch.Tensor,
+                mse_target: Optional[torch.Tensor] = None, out_dL_dcolor: Optional[torch.Tensor] = None):
+        """Render every view.  Returns the concatenated colour buffer (views [3,H,W] back to back).
+        With mse_target (laid out like the colour buffer), the compositor also writes the gradient of
+        every view's mean squared colour error into out_dL_dcolor."""
+        if (mse_target is None) != (out_dL_dcolor is None):
+            raise ValueError("mse_target and out_dL_dcolor go together")
         st = self._stream()
         _lib.check(self.lib.svr_batch_voxel_records(self.P, _p(self.vox_key), _p(grid_pts), _p(self.vox_center),
                                                     _p(self.vox_size), _p(self.is_leaf), _p(s_val),
@@ -212,7 +217,8 @@ class BatchedRasterizer:
         _lib.check(self.lib.svr_batch_rasterize(_p(self.ws), *self._dims(), _p(self.vrec), _p(self.rgbs_views),
                                                 _p(s_val), _p(self.pair_order),
                                                 int(self.need_depth), int(self.need_normal),
-                                                _p(self.color), _p(self.depth), _p(self.normal), _p(self.T), st),
+                                                _p(self.color), _p(self.depth), _p(self.normal), _p(self.T),
+                                                _p(mse_target), _p(out_dL_dcolor), st),
                    "svr_batch_rasterize")
         return self.color
 
@@ -284,8 +290,12 @@ class BatchedTrainStep:
     """fwd + MSE-to-target + bwd of all views as one step, optionally captured as a HIP graph.
     Gradients land in `grid_grad`, `sh0_grad`, `shs_grad`, `s_grad`."""
 
-    def __init__(self, rast: BatchedRasterizer, grid_pts, sh0, shs, log_s, targets: List[torch.Tensor]):
+    def __init__(self, rast: BatchedRasterizer, grid_pts, sh0, shs, log_s, targets: List[torch.Tensor],
+                 fuse_loss: bool = True):
         self.r = rast
+        # fuse_loss: the compositor writes dL/dcolor of the MSE as it writes the colour (one pass
+        # fewer); otherwise MseColorGrad after the forward.
+        self.fuse_loss = bool(fuse_loss)
         self.grid_pts, self.sh0, self.shs, self.log_s = grid_pts, sh0, shs, log_s
         self.loss_grad = MseColorGrad(targets, rast.pix, rast.dev)
         self.target = self.loss_grad.target
@@ -295,8 +305,11 @@ class BatchedTrainStep:
 
     def _run(self):
         torch.exp(self.log_s.reshape(1) * 10.0, out=self.s_val)
-        color = self.r.forward(self.grid_pts, self.sh0, self.shs, self.s_val)
-        self.loss_grad(color, self.dC)
+        if self.fuse_loss:
+            self.r.forward(self.grid_pts, self.sh0, self.shs, self.s_val, self.target, self.dC)
+        else:
+            color = self.r.forward(self.grid_pts, self.sh0, self.shs, self.s_val)
+            self.loss_grad(color, self.dC)
         self.grid_grad, self.sh0_grad, self.shs_grad, self.s_grad, _ = self.r.backward(
             self.dC, self.grid_pts.shape[0])
 
@@ -583,9 +596,13 @@ class BatchedTrainIteration:
 
     def __init__(self, rast: BatchedRasterizer, grid_pts, sh0, shs, log_s, targets: List[torch.Tensor],
                  smooth: Optional[GridSmoothness], lr: float = 1e-2, betas=(0.9, 0.99), eps: float = 1e-15,
-                 lr_sh0: Optional[float] = None, lr_shs: Optional[float] = None, sparse: bool = False):
+                 lr_sh0: Optional[float] = None, lr_shs: Optional[float] = None, sparse: bool = False,
+                 fuse_loss: bool = True):
         self.r = rast
         self.sparse = bool(sparse)
+        # fuse_loss: the render stage's compositor writes dL/dcolor of the MSE (the loss stage is
+        # then empty); otherwise the loss stage runs MseColorGrad.
+        self.fuse_loss = bool(fuse_loss)
         self.dev = rast.dev
         self.grid_pts, self.sh0, self.shs, self.log_s = grid_pts, sh0, shs, log_s
         self.loss_grad = MseColorGrad(targets, rast.pix, self.dev)
@@ -609,10 +626,14 @@ class BatchedTrainIteration:
     def stage_render(self):
         self.step.add_(1)
         torch.exp(self.log_s.reshape(1) * 10.0, out=self.s_val)
-        self.color = self.r.forward(self.grid_pts, self.sh0, self.shs, self.s_val)
+        if self.fuse_loss:
+            self.color = self.r.forward(self.grid_pts, self.sh0, self.shs, self.s_val, self.target, self.dC)
+        else:
+            self.color = self.r.forward(self.grid_pts, self.sh0, self.shs, self.s_val)
 
     def stage_loss(self):
-        self.loss_grad(self.color, self.dC)
+        if not self.fuse_loss:
+            self.loss_grad(self.color, self.dC)
 
     def stage_backward(self):
         self.grid_grad, _, _, self.s_grad, _ = self.r.backward(self.dC, self.grid_pts.shape[0],

@@ -899,9 +899,11 @@ int svr_batch_sh_forward(void* ws, int n_views, int P, long long n_pixels, long 
 int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                         const float* vox_records, const float* rgbs_views, const float* s_val, const uint32_t* pair_order, int need_depth,
                         int need_normal, float* out_color, float* out_depth, float* out_normal, float* out_T,
-                        void* stream) {
+                        const float* mse_target, float* out_dL_dcolor, void* stream) {
   g_err.clear();
   if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
+  if ((mse_target == nullptr) != (out_dL_dcolor == nullptr))
+    return fail(1, "mse_target and out_dL_dcolor go together%s%lld", "", 0);
   hipStream_t st = as_stream(stream);
   BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
   const long long n = 8LL * P;
@@ -927,6 +929,7 @@ int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long l
   a.s_val = s_val;
   a.tile_last = bp.im.tile_last; a.n_contrib = bp.im.n_contrib;
   a.color = out_color; a.depth = out_depth; a.normal = out_normal; a.T = out_T; a.max_w = nullptr;
+  a.mse_target = mse_target; a.dL_dcolor = out_dL_dcolor;
   a.work_counter = bp.im.misc;
   a.tile_view = bp.im.tile_view;
   a.cvals = bp.c.cvals; a.pmask = bp.c.pmask; a.cranges = bp.im.cranges;

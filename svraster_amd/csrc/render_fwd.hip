@@ -126,17 +126,44 @@ __device__ inline float median_depth(const FwdArgs& a, const Ray& r, int v, floa
   return Dmed;
 }
 
+// The MSE target of pixel (px, py) (fused colour gradient), loaded at the start of the tile so
+// that the loads are in flight while the tile is composited.
+struct PixelTarget {
+  float t0 = 0.f, t1 = 0.f, t2 = 0.f;
+};
+
+__device__ inline PixelTarget load_target(const FwdArgs& a, const ViewDesc& V, bool inside, int px, int py) {
+  PixelTarget g;
+  if (a.dL_dcolor && inside) {
+    const long long HW = (long long)V.W * V.H;
+    const long long o = V.pix_base * 3 + (long long)py * V.W + px;
+    g.t0 = a.mse_target[o];
+    g.t1 = a.mse_target[o + HW];
+    g.t2 = a.mse_target[o + 2 * HW];
+  }
+  return g;
+}
+
 template <bool kDepth, bool kNormal>
 __device__ inline void write_pixel(const FwdArgs& a, const ViewDesc& V, const Ray& r, bool inside, int px, int py,
                                    float T, float C0, float C1, float C2, float D, float Dmed, float N0,
-                                   float N1, float N2, uint32_t last) {
+                                   float N1, float N2, uint32_t last, const PixelTarget& tg) {
   if (!inside) return;
   const long long HW = (long long)V.W * V.H;
   const long long pid = (long long)py * V.W + px;
   const long long pb = V.pix_base;
-  a.color[pb * 3 + 0 * HW + pid] = C0 + T * V.bg[0];
-  a.color[pb * 3 + 1 * HW + pid] = C1 + T * V.bg[1];
-  a.color[pb * 3 + 2 * HW + pid] = C2 + T * V.bg[2];
+  const float c0 = C0 + T * V.bg[0], c1 = C1 + T * V.bg[1], c2 = C2 + T * V.bg[2];
+  a.color[pb * 3 + 0 * HW + pid] = c0;
+  a.color[pb * 3 + 1 * HW + pid] = c1;
+  a.color[pb * 3 + 2 * HW + pid] = c2;
+  if (a.dL_dcolor) {
+    // gradient of the view's mean squared error over its 3 W H colour values
+    const float sc = 2.f / (3.f * (float)HW);
+    const long long o = pb * 3 + pid;
+    a.dL_dcolor[o] = (c0 - tg.t0) * sc;
+    a.dL_dcolor[o + HW] = (c1 - tg.t1) * sc;
+    a.dL_dcolor[o + 2 * HW] = (c2 - tg.t2) * sc;
+  }
   a.T[pb + pid] = T;
   a.n_contrib[pb + pid] = last;
   if (kDepth) {
@@ -186,6 +213,7 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
     const Ray r = pixel_ray(V, px, py);
     const uint2 rg = a.ranges[g];
     const int n = (int)(rg.y - rg.x);
+    const PixelTarget tg = load_target(a, V, inside, px, py);
 
     float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f, D = 0.f, N0 = 0.f, N1 = 0.f, N2 = 0.f;
     uint32_t last = 0;
@@ -253,7 +281,7 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(FwdArgs a) {
 
     float Dmed = 0.f;
     if (kDepth && inside && dmed_vox >= 0) Dmed = median_depth(a, r, dmed_vox, dmed_T, s);
-    write_pixel<kDepth, kNormal>(a, V, r, inside, px, py, T, C0, C1, C2, D, Dmed, N0, N1, N2, last);
+    write_pixel<kDepth, kNormal>(a, V, r, inside, px, py, T, C0, C1, C2, D, Dmed, N0, N1, N2, last, tg);
     // tile_last = max over the tile's pixels of (range start + contributors) (forward.cu:561).
     const uint32_t tl = wave_max_u32(rg.x + (inside ? last : 0u));
     if (lane == 0) s_red[w] = tl;
@@ -361,6 +389,7 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
     const int px = lattice_px(L, lane), py = lattice_py(L, lane);
     const bool inside = px < W && py < H;
     const Ray r = pixel_ray(V, px, py);
+    const PixelTarget tg = load_target(a, V, inside, px, py);
     const float cam_scale = fmaxf(fabsf(r.ox), fmaxf(fabsf(r.oy), fabsf(r.oz)));
     s_rd[w][lane] = make_float4(r.dx, r.dy, r.dz, 0.f);
     s_ri[w][lane] = make_float4(r.ix, r.iy, r.iz, 0.f);
@@ -541,7 +570,7 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
 
     float Dmed = 0.f;
     if (kDepth && inside && dmed_vox >= 0) Dmed = median_depth(a, r, dmed_vox, dmed_T, s);
-    write_pixel<kDepth, kNormal>(a, V, r, inside, px, py, T, C0, C1, C2, D, Dmed, N0, N1, N2, last);
+    write_pixel<kDepth, kNormal>(a, V, r, inside, px, py, T, C0, C1, C2, D, Dmed, N0, N1, N2, last, tg);
     const uint32_t tl = wave_max_u32(rg.x + (inside ? last : 0u));
     if (lane == 0) s_red[w] = tl;
     __syncthreads();

@@ -266,6 +266,9 @@ struct FwdArgs {
   const float* s_val;
   uint32_t* tile_last; uint32_t* n_contrib;
   float* color; float* depth; float* normal; float* T; float* max_w;
+  // Optional fused MSE colour gradient: dL_dcolor = (color - mse_target) * 2 / (3 W H) per view,
+  // both laid out like color (null: not written).
+  const float* mse_target; float* dL_dcolor;
   uint32_t* work_counter;          // null: one workgroup per tile
   const uint32_t* tile_view = nullptr;   // per tile: its view (work-queue launches)
   uint32_t* cvals; uint64_t* pmask; uint2* cranges;   // contributing entries (written)

@@ -61,6 +61,36 @@ def test_batched_matches_per_view():
 
 
 @pytest.mark.gpu
+def test_fused_mse_gradient():
+    """The compositor's fused dL/dcolour equals (colour - target) * 2 / (3 W H) per view, computed in
+    FP64 from its own colour output (a few FP32 roundings: 1e-6 relative), and the step's gradients
+    equal the unfused step's (MseColorGrad after the forward) to FP32 rounding."""
+    say("[batched] fused MSE colour gradient vs FP64 and vs the separate loss pass")
+    w, sc, cams, targets = _setup()
+    m = VoxelModel(sc)
+    grads = []
+    for fuse in (True, False):
+        r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key, cams,
+                              m.bg_color, sc.sh_degree)
+        step = BatchedTrainStep(r, m._geo_grid_pts.detach(), m._sh0.detach(), m._shs.detach(), m._log_s.detach(),
+                                targets, fuse_loss=fuse)
+        step()
+        torch.cuda.synchronize()
+        if fuse:
+            for v, tgt in enumerate(targets):
+                c = r.view_slice(r.color, 3, v).double()
+                want = (c - tgt.double()) * (2.0 / (3 * r.pix[v]))
+                got = r.view_slice(step.dC, 3, v).double()
+                err = ((got - want).abs() / want.abs().clamp_min(1e-30)).max().item()
+                say(f"  view {v}: fused dL/dcolour max relative error {err:.2e}")
+                assert err <= 1e-6
+        grads.append((step.grid_grad.clone(), step.sh0_grad.clone(), step.s_grad.clone()))
+    e = [rel_l2(a, b) for a, b in zip(grads[0], grads[1])]
+    say(f"  fused vs unfused step gradients rel-L2 {e}")
+    assert max(e) <= 1e-5
+
+
+@pytest.mark.gpu
 def test_batched_graph_replays_match_eager():
     say("[batched] graph capture: replays vs eager")
     w, sc, cams, targets = _setup()
