@@ -371,6 +371,16 @@ int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long l
                         const float* vox_records, const float* rgbs_views, const float* s_val, const uint32_t* pair_order, int need_depth,
                         int need_normal, float* out_color, float* out_depth, float* out_normal, float* out_T,
                         const float* mse_target, float* out_dL_dcolor, void* stream);
+/* svr_batch_rasterize in two calls: the keys, tile sort and tile ranges (svr_batch_bin, after
+ * svr_batch_preprocess), then the compositor (svr_batch_composite, after svr_batch_sh_forward).  The
+ * two calls between them do not touch each other's data, so a caller can run svr_batch_sh_forward on
+ * a second stream beside svr_batch_bin and join before svr_batch_composite. */
+int svr_batch_bin(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                  const uint32_t* pair_order, void* stream);
+int svr_batch_composite(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                        const float* vox_records, const float* rgbs_views, const float* s_val, int need_depth,
+                        int need_normal, float* out_color, float* out_depth, float* out_normal, float* out_T,
+                        const float* mse_target, float* out_dL_dcolor, void* stream);
 /* One persistent reverse compositor launch over all (view, tile) pairs (geometry from the same
  * records as the forward).  dL_dvox [P,12]
  * (colour slots unused), dL_drgb_views [V,P,3] and dL_ds [1] zeroed by the caller.  With

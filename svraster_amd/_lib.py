@@ -64,6 +64,8 @@ _SIGS = {
     "svr_batch_sh_forward": (I, [P, I, I, LL, LL, LL, P, P, P, I, I, P, P]),
     "svr_batch_voxel_records": (I, [I, P, P, P, P, P, P, F, I, P, P, P]),
     "svr_batch_rasterize": (I, [P, I, I, LL, LL, LL, P, P, P, P, I, I, P, P, P, P, P, P, P]),
+    "svr_batch_bin": (I, [P, I, I, LL, LL, LL, P, P]),
+    "svr_batch_composite": (I, [P, I, I, LL, LL, LL, P, P, P, I, I, P, P, P, P, P, P, P]),
     "svr_batch_backward": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, P, P, I, I, P, P, P, P, P, P]),
     "svr_batch_sh_backward": (I, [P, I, I, LL, LL, LL, P, P, I, I, P, P, P]),
     "svr_batch_sh_backward_adam": (I, [P, I, I, LL, LL, LL, P, P, I, I, P, P, P, P, P, P, P, P, I, P, F, F, F, I, P]),

@@ -36,8 +36,13 @@ class BatchedRasterizer:
     def __init__(self, octpath, octlevel, vox_center, vox_size, is_leaf, vox_key, cameras: List,
                  background: torch.Tensor, sh_degree: int, n_sh_coeffs: int = 16,
                  key_capacity: Optional[int] = None, need_depth=False, need_normal=False, cull: bool = True,
-                 tile_shape=(16, 16), fuse_gather_bw: bool = False, check_eager: bool = True):
+                 tile_shape=(16, 16), fuse_gather_bw: bool = False, check_eager: bool = True,
+                 overlap_streams: bool = True):
         self.dev = vox_center.device
+        # overlap_streams: the SH colour pass runs on a second stream beside the key emission and
+        # tile sort (and the SH transpose beside the corner scatter), joined before the compositor;
+        # the second stream has the default priority.
+        self.side = torch.cuda.Stream(device=self.dev) if overlap_streams else None
         # Eager forwards read the key peak back (one sync) and raise on overflow; graph replays
         # are checked by the caller with check_capacity().
         self.check_eager = bool(check_eager)
@@ -206,20 +211,32 @@ class BatchedRasterizer:
         self.preprocess(self.bin_leaf if self.cull else None)
         if self.check_eager and not torch.cuda.is_current_stream_capturing():
             self.check_capacity()
-        _lib.check(self.lib.svr_batch_sh_forward(_p(self.ws), *self._dims(), _p(self.vox_center), _p(sh0), _p(shs),
-                                                 self.deg, self.M, _p(self.rgbs_views), st), "sh_forward")
+        def sh_forward():
+            _lib.check(self.lib.svr_batch_sh_forward(_p(self.ws), *self._dims(), _p(self.vox_center), _p(sh0),
+                                                     _p(shs), self.deg, self.M, _p(self.rgbs_views), self._stream()),
+                       "sh_forward")
+
+        if self.side is not None:
+            main = torch.cuda.current_stream(self.dev)
+            self.side.wait_stream(main)
+            with torch.cuda.stream(self.side):
+                sh_forward()
+        else:
+            sh_forward()
+        _lib.check(self.lib.svr_batch_bin(_p(self.ws), *self._dims(), _p(self.pair_order), st), "svr_batch_bin")
+        if self.side is not None:
+            main.wait_stream(self.side)
         # Every pixel of every tile is written by the compositor: colour and T need no clearing.
         if self.depth is not None:
             self.depth.zero_()
         if self.normal is not None:
             self.normal.zero_()
         self.s_val = s_val
-        _lib.check(self.lib.svr_batch_rasterize(_p(self.ws), *self._dims(), _p(self.vrec), _p(self.rgbs_views),
-                                                _p(s_val), _p(self.pair_order),
-                                                int(self.need_depth), int(self.need_normal),
+        _lib.check(self.lib.svr_batch_composite(_p(self.ws), *self._dims(), _p(self.vrec), _p(self.rgbs_views),
+                                                _p(s_val), int(self.need_depth), int(self.need_normal),
                                                 _p(self.color), _p(self.depth), _p(self.normal), _p(self.T),
                                                 _p(mse_target), _p(out_dL_dcolor), st),
-                   "svr_batch_rasterize")
+                   "svr_batch_composite")
         return self.color
 
     def backward(self, dL_dcolor: torch.Tensor, n_grid_pts: int, dL_dT: Optional[torch.Tensor] = None,
@@ -244,23 +261,36 @@ class BatchedRasterizer:
                                                _p(self.drgb_views), _p(self.ds),
                                                _p(self.vox_key) if fuse else None, _p(self.dgrid) if fuse else None,
                                                st), "svr_batch_backward")
-        if sh_adam is None:
-            _lib.check(self.lib.svr_batch_sh_backward(_p(self.ws), *self._dims(), _p(self.vox_center),
-                                                      _p(self.drgb_views), self.deg, self.M, _p(self.dsh0),
-                                                      _p(self.dshs), st), "sh_backward")
-            dsh0, dshs = self.dsh0, self.dshs
+        def sh_backward():
+            s2 = self._stream()
+            if sh_adam is None:
+                _lib.check(self.lib.svr_batch_sh_backward(_p(self.ws), *self._dims(), _p(self.vox_center),
+                                                          _p(self.drgb_views), self.deg, self.M, _p(self.dsh0),
+                                                          _p(self.dshs), s2), "sh_backward")
+            else:
+                a = sh_adam
+                _lib.check(self.lib.svr_batch_sh_backward_adam(
+                    _p(self.ws), *self._dims(), _p(self.vox_center), _p(self.drgb_views), self.deg, self.M,
+                    _p(a.sh0), _p(a.m0), _p(a.v0), _p(a.table0), _p(a.shs), _p(a.ms), _p(a.vs), _p(a.tables),
+                    a.n_table, _p(a.step), ctypes.c_float(a.b1), ctypes.c_float(a.b2), ctypes.c_float(a.eps),
+                    int(a.sparse), s2), "sh_backward_adam")
+
+        # The SH transpose (colour gradients) and the corner scatter (corner-SDF gradients) read
+        # disjoint outputs of the reverse compositor: with a second stream they run side by side.
+        if self.side is not None and not fuse:
+            main = torch.cuda.current_stream(self.dev)
+            self.side.wait_stream(main)
+            with torch.cuda.stream(self.side):
+                sh_backward()
         else:
-            a = sh_adam
-            _lib.check(self.lib.svr_batch_sh_backward_adam(
-                _p(self.ws), *self._dims(), _p(self.vox_center), _p(self.drgb_views), self.deg, self.M,
-                _p(a.sh0), _p(a.m0), _p(a.v0), _p(a.table0), _p(a.shs), _p(a.ms), _p(a.vs), _p(a.tables),
-                a.n_table, _p(a.step), ctypes.c_float(a.b1), ctypes.c_float(a.b2), ctypes.c_float(a.eps),
-                int(a.sparse), st), "sh_backward_adam")
-            dsh0 = dshs = None
+            sh_backward()
+        dsh0, dshs = (self.dsh0, self.dshs) if sh_adam is None else (None, None)
         if not fuse:
             dgeo = self.dvox[:, :8].contiguous()
             _lib.check(self.lib.svr_gather_triinterp_geo_params_bw(_p(self.vox_key), self.P, None, self.P,
                                                                    _p(dgeo), _p(self.dgrid), st), "gather_bw")
+            if self.side is not None:
+                main.wait_stream(self.side)
         return self.dgrid, dsh0, dshs, self.ds, self.dvox[:, 11]
 
 

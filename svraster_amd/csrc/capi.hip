@@ -896,14 +896,10 @@ int svr_batch_sh_forward(void* ws, int n_views, int P, long long n_pixels, long 
   return 0;
 }
 
-int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
-                        const float* vox_records, const float* rgbs_views, const float* s_val, const uint32_t* pair_order, int need_depth,
-                        int need_normal, float* out_color, float* out_depth, float* out_normal, float* out_T,
-                        const float* mse_target, float* out_dL_dcolor, void* stream) {
+int svr_batch_bin(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                  const uint32_t* pair_order, void* stream) {
   g_err.clear();
   if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
-  if ((mse_target == nullptr) != (out_dL_dcolor == nullptr))
-    return fail(1, "mse_target and out_dL_dcolor go together%s%lld", "", 0);
   hipStream_t st = as_stream(stream);
   BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
   const long long n = 8LL * P;
@@ -918,6 +914,19 @@ int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long l
                            bp.b.partial, st));
   SVR_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(bp.im.ranges), 2 * n_tiles, 0u, st));
   SVR_TRY(launch_tile_ranges_u32(tiles, n_dev, K_cap, bp.im.ranges, st));
+  return 0;
+}
+
+int svr_batch_composite(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                        const float* vox_records, const float* rgbs_views, const float* s_val, int need_depth,
+                        int need_normal, float* out_color, float* out_depth, float* out_normal, float* out_T,
+                        const float* mse_target, float* out_dL_dcolor, void* stream) {
+  g_err.clear();
+  if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
+  if ((mse_target == nullptr) != (out_dL_dcolor == nullptr))
+    return fail(1, "mse_target and out_dL_dcolor go together%s%lld", "", 0);
+  hipStream_t st = as_stream(stream);
+  BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
   SVR_TRY(launch_fill_u32(bp.im.misc, 2, 0u, st));
   FwdArgs a;
   memset(&a, 0, sizeof(a));
@@ -942,6 +951,15 @@ int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long l
     SVR_TRY(launch_render_forward(a, st));
   }
   return 0;
+}
+
+int svr_batch_rasterize(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
+                        const float* vox_records, const float* rgbs_views, const float* s_val, const uint32_t* pair_order, int need_depth,
+                        int need_normal, float* out_color, float* out_depth, float* out_normal, float* out_T,
+                        const float* mse_target, float* out_dL_dcolor, void* stream) {
+  if (int e = svr_batch_bin(ws, n_views, P, n_pixels, n_tiles, K_cap, pair_order, stream)) return e;
+  return svr_batch_composite(ws, n_views, P, n_pixels, n_tiles, K_cap, vox_records, rgbs_views, s_val, need_depth,
+                             need_normal, out_color, out_depth, out_normal, out_T, mse_target, out_dL_dcolor, stream);
 }
 
 int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
