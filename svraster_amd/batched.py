@@ -627,8 +627,13 @@ class BatchedTrainIteration:
     def __init__(self, rast: BatchedRasterizer, grid_pts, sh0, shs, log_s, targets: List[torch.Tensor],
                  smooth: Optional[GridSmoothness], lr: float = 1e-2, betas=(0.9, 0.99), eps: float = 1e-15,
                  lr_sh0: Optional[float] = None, lr_shs: Optional[float] = None, sparse: bool = False,
-                 fuse_loss: bool = True):
+                 fuse_loss: bool = True, overlap_smooth: bool = True):
         self.r = rast
+        # overlap_smooth: the grid smoothness depends on the corner SDFs only, so it runs on a stream
+        # of its own (default priority) into a gradient buffer of its own while the views render and
+        # back-propagate, and is added to the render gradient before the Adam step.
+        self.smooth_stream = torch.cuda.Stream(device=rast.dev) if (overlap_smooth and smooth is not None) else None
+        self.sgrad = torch.zeros_like(grid_pts) if self.smooth_stream is not None else None
         self.sparse = bool(sparse)
         # fuse_loss: the render stage's compositor writes dL/dcolor of the MSE (the loss stage is
         # then empty); otherwise the loss stage runs MseColorGrad.
@@ -682,11 +687,31 @@ class BatchedTrainIteration:
     STAGES = ("render", "loss", "backward", "smooth", "adam")
 
     def _run(self):
-        for name in self.STAGES:
-            getattr(self, "stage_" + name)()
+        if self.smooth_stream is None:
+            for name in self.STAGES:
+                getattr(self, "stage_" + name)()
+            return
+        # render prelude (step counter, s), then the smoothness forks off before the forward
+        main = torch.cuda.current_stream(self.dev)
+        self.step.add_(1)
+        torch.exp(self.log_s.reshape(1) * 10.0, out=self.s_val)
+        self.smooth_stream.wait_stream(main)
+        with torch.cuda.stream(self.smooth_stream):
+            self.sgrad.zero_()
+            self.smooth(self.grid_pts, self.sgrad, self.step)
+        if self.fuse_loss:
+            self.color = self.r.forward(self.grid_pts, self.sh0, self.shs, self.s_val, self.target, self.dC)
+        else:
+            self.color = self.r.forward(self.grid_pts, self.sh0, self.shs, self.s_val)
+        self.stage_loss()
+        self.stage_backward()
+        main.wait_stream(self.smooth_stream)
+        self.grid_grad.add_(self.sgrad)
+        self.stage_adam()
 
     def stage_times(self):
-        """Milliseconds per stage of one eager iteration, from HIP events on the current stream."""
+        """Milliseconds per stage of one eager iteration (stages one after another, no overlap),
+        from HIP events on the current stream."""
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(self.STAGES) + 1)]
         ev[0].record()
         for i, name in enumerate(self.STAGES):
