@@ -162,9 +162,13 @@ __device__ inline void trilinear_weights(float x, float y, float z, float w[8]) 
   w[4] = a10 * z0; w[5] = a10 * z; w[6] = a11 * z0; w[7] = a11 * z;
 }
 
-__device__ inline float fast_sigmoid(float x) {
+// The logistic CDF of s * sdf, 1 / (1 + 2^(sdf * k)) with k = sdf_cdf_scale(s) = -s log2(e): one
+// multiply (k is loop-invariant, hoisted out of the callers' pair loops), the hardware exp2 and
+// reciprocal.  Every alpha of the compositors (forward, backward, median depth) comes from it.
+__device__ inline float sdf_cdf_scale(float s) { return s * -1.44269504088896341f; }
+__device__ inline float sdf_cdf(float sdf, float k) {
 #pragma clang fp contract(off)
-  return __builtin_amdgcn_rcpf(1.f + __expf(-x));
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(sdf * k));
 }
 
 // Voxel record of the multi-view path, 64 bytes per voxel (one aligned segment per staged
@@ -269,11 +273,13 @@ __device__ inline void eval_segment(const Ray& r, float a, float b, float minx, 
   seg.qz = (fmaf(a, r.dz, r.oz) - minz) * l_inv;
   const float st = ((b - a) * (1.f / 3.f)) * l_inv;
   seg.sx = st * r.dx; seg.sy = st * r.dy; seg.sz = st * r.dz;
+  const float ks = sdf_cdf_scale(s);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const float fk = (float)k;
-    seg.sdf[k] = trilerp(g, fmaf(seg.sx, fk, seg.qx), fmaf(seg.sy, fk, seg.qy), fmaf(seg.sz, fk, seg.qz));
-    seg.phi[k] = fast_sigmoid(s * seg.sdf[k]);
+    seg.sdf[k] = k == 0 ? trilerp(g, seg.qx, seg.qy, seg.qz)
+                        : trilerp(g, fmaf(seg.sx, fk, seg.qx), fmaf(seg.sy, fk, seg.qy), fmaf(seg.sz, fk, seg.qz));
+    seg.phi[k] = sdf_cdf(seg.sdf[k], ks);
   }
   const float raw = (seg.phi[0] - seg.phi[3]) * __builtin_amdgcn_rcpf(seg.phi[0] + kPhiEps);
   seg.alpha = fmaxf(raw, 0.f);
@@ -289,11 +295,14 @@ __device__ inline void eval_segment_st(const Ray& r, float a, float b, const flo
   seg.qz = fmaf(a, r.dz, -box.z) * box.w;
   const float st = ((b - a) * (1.f / 3.f)) * box.w;
   seg.sx = st * r.dx; seg.sy = st * r.dy; seg.sz = st * r.dz;
+  const float ks = sdf_cdf_scale(s);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const float fk = (float)k;
-    seg.sdf[k] = trilerp_d(gd0, gd1, fmaf(seg.sx, fk, seg.qx), fmaf(seg.sy, fk, seg.qy), fmaf(seg.sz, fk, seg.qz));
-    seg.phi[k] = fast_sigmoid(s * seg.sdf[k]);
+    seg.sdf[k] = k == 0 ? trilerp_d(gd0, gd1, seg.qx, seg.qy, seg.qz)
+                        : trilerp_d(gd0, gd1, fmaf(seg.sx, fk, seg.qx), fmaf(seg.sy, fk, seg.qy),
+                                    fmaf(seg.sz, fk, seg.qz));
+    seg.phi[k] = sdf_cdf(seg.sdf[k], ks);
   }
   const float raw = (seg.phi[0] - seg.phi[3]) * __builtin_amdgcn_rcpf(seg.phi[0] + kPhiEps);
   seg.alpha = fmaxf(raw, 0.f);

@@ -114,12 +114,13 @@ __device__ inline float median_depth(const FwdArgs& a, const Ray& r, int v, floa
   float qy = (r.oy + ta * r.dy - (cy - 0.5f * l)) * l_inv;
   float qz = (r.oz + ta * r.dz - (cz - 0.5f * l)) * l_inv;
   const float sx = step * r.dx * l_inv, sy = step * r.dy * l_inv, sz = step * r.dz * l_inv;
-  float phi_prev = fast_sigmoid(s * trilerp(gg, qx, qy, qz));
+  const float ks = sdf_cdf_scale(s);
+  float phi_prev = sdf_cdf(trilerp(gg, qx, qy, qz), ks);
   float Tm = dmed_T;
   for (int k = 0; k < 16 && Tm > 0.5f; ++k) {
     Dmed += step;
     qx += sx; qy += sy; qz += sz;
-    const float phi = fast_sigmoid(s * trilerp(gg, qx, qy, qz));
+    const float phi = sdf_cdf(trilerp(gg, qx, qy, qz), ks);
     Tm *= 1.f - (phi_prev - phi) / (phi_prev + kPhiEps);
     phi_prev = phi;
   }
