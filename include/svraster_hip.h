@@ -115,6 +115,9 @@ int svr_gather_triinterp_geo_params(const int64_t* vox_key, int P, const int64_t
 int svr_gather_triinterp_geo_params_bw(const int64_t* vox_key, int P, const int64_t* care_idx,
                                        int n_care, const float* dL_dgeos, float* dL_dgrid_pts,
                                        void* stream);
+/* The same scatter of every voxel's corner-SDF gradients, read from the batched backward's
+ * dL_dvox [P,12] rows (slots 0-7) in place (no reference counterpart; saves the [P,8] copy). */
+int svr_batch_gather_bw(int P, const int64_t* vox_key, const float* dL_dvox, float* dL_dgrid_pts, void* stream);
 
 /* Replaces ADAM_STEP::unbiased_adam_step (src/adam_step.cu:106-135).  In place. */
 int svr_unbiased_adam_step(int sparse, float* param, const float* grad, float* exp_avg,
@@ -382,7 +385,7 @@ int svr_batch_composite(void* ws, int n_views, int P, long long n_pixels, long l
                         int need_normal, float* out_color, float* out_depth, float* out_normal, float* out_T,
                         const float* mse_target, float* out_dL_dcolor, void* stream);
 /* One persistent reverse compositor launch over all (view, tile) pairs (geometry from the same
- * records as the forward).  dL_dvox [P,12]
+ * records as the forward).  dL_dT may be NULL (no transmittance cotangent).  dL_dvox [P,12]
  * (colour slots unused), dL_drgb_views [V,P,3] and dL_ds [1] zeroed by the caller.  With
  * vox_key [P,8] and dL_dgrid_pts [M] (zeroed by the caller) the corner-SDF gradients are added
  * straight to the grid points as they are flushed -- the transpose of GatherGeoParams

@@ -41,6 +41,7 @@ _SIGS = {
     "svr_sh_compute_bw": (I, [I, I, P, I, P, I, P, P, P, P, P]),
     "svr_gather_triinterp_geo_params": (I, [P, I, P, I, P, P, P]),
     "svr_gather_triinterp_geo_params_bw": (I, [P, I, P, I, P, P, P]),
+    "svr_batch_gather_bw": (I, [I, P, P, P, P]),
     "svr_unbiased_adam_step": (I, [I, P, P, P, P, LL, D, D, D, D, F, P]),
     "svr_biased_adam_step": (I, [I, P, P, P, P, LL, F, F, F, F, P]),
     "svr_grid_eikonal_bw": (I, [P, LL, P, I, P, P, I, P, P, P, I, P, I, F, F, U, P, P, P]),

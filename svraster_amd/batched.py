@@ -134,7 +134,6 @@ class BatchedRasterizer:
         self.dsh0 = torch.zeros(P, 3, **f)
         self.dshs = torch.zeros(P, self.M - 1, 3, **f)
         self.dgrid = None
-        self.zero_T = torch.zeros(self.n_pix, **f)
 
     @property
     def geos(self) -> torch.Tensor:
@@ -252,7 +251,7 @@ class BatchedRasterizer:
         # drgb_views is zero here: it starts at zero and svr_batch_sh_backward clears what it reads.
         self.ds.zero_()
         self.dgrid.zero_()
-        dT = self.zero_T if dL_dT is None else dL_dT
+        dT = dL_dT   # None: no transmittance cotangent (read as zero)
         fuse = self.fuse_gather_bw
         _lib.check(self.lib.svr_batch_backward(_p(self.ws), *self._dims(), _p(self.vrec), _p(self.rgbs_views),
                                                _p(self.s_val), _p(self.T),
@@ -286,9 +285,8 @@ class BatchedRasterizer:
             sh_backward()
         dsh0, dshs = (self.dsh0, self.dshs) if sh_adam is None else (None, None)
         if not fuse:
-            dgeo = self.dvox[:, :8].contiguous()
-            _lib.check(self.lib.svr_gather_triinterp_geo_params_bw(_p(self.vox_key), self.P, None, self.P,
-                                                                   _p(dgeo), _p(self.dgrid), st), "gather_bw")
+            _lib.check(self.lib.svr_batch_gather_bw(self.P, _p(self.vox_key), _p(self.dvox), _p(self.dgrid), st),
+                       "svr_batch_gather_bw")
             if self.side is not None:
                 main.wait_stream(self.side)
         return self.dgrid, dsh0, dshs, self.ds, self.dvox[:, 11]

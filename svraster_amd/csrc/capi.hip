@@ -425,6 +425,15 @@ int svr_gather_triinterp_geo_params_bw(const int64_t* vox_key, int P, const int6
   return 0;
 }
 
+int svr_batch_gather_bw(int P, const int64_t* vox_key, const float* dL_dvox, float* dL_dgrid_pts, void* stream) {
+  g_err.clear();
+  if (P < 0) return fail(1, "bad voxel count%s %lld", "", P);
+  if (P > 0 && (!vox_key || !dL_dvox || !dL_dgrid_pts))
+    return fail(1, "vox_key, dL_dvox and dL_dgrid_pts are required%s%lld", "", 0);
+  SVR_TRY(launch_gather_geo_bw(P, vox_key, nullptr, dL_dvox, dL_dgrid_pts, as_stream(stream), 12));
+  return 0;
+}
+
 int svr_unbiased_adam_step(int sparse, float* param, const float* grad, float* exp_avg,
                            float* exp_avg_sq, long long numel, double step, double lr,
                            double beta1, double beta2, float eps, void* stream) {

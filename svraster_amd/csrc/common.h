@@ -26,6 +26,18 @@ constexpr int kVoxGeoTriinterp = 2, kVoxGeoTriinterp1 = 1, kVoxGeoTriinterp3 = 3
 constexpr int kDensityExpLinear11 = 0, kDensitySDF = 1;
 constexpr int kCamPersp = 0, kCamOrtho = 10;
 
+// Range in which rcp_rn is the correctly rounded reciprocal: |x| in [2^-100, 2^100].
+constexpr float kRcpRnLo = 0x1p-100f, kRcpRnHi = 0x1p100f;
+
+// 1/x rounded to nearest for |x| in [kRcpRnLo, kRcpRnHi]: the hardware reciprocal (within 1 ulp)
+// and one Newton step, whose residual 1 - x y is exact in an FMA.  The result equals the IEEE
+// quotient 1.0f / x for every float32 of that range on gfx950; svr_selfcheck_rcp_rn checks all
+// 2^23 significands of any exponent range on the device, against the IEEE division.
+__device__ inline float rcp_rn(float x) {
+  const float y = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(__builtin_fmaf(-x, y, 1.f), y, y);
+}
+
 // One camera view as the kernels see it.  Built on the device from the caller's matrices so
 // that no host synchronisation is needed, and stored in the per-view state buffers.
 struct alignas(16) ViewDesc {
@@ -100,13 +112,16 @@ __device__ inline Ray pixel_ray(const ViewDesc& v, int px, int py) {
   camera_ray(v, px, py, cx_, cy_);
   const float n = sqrtf(cx_ * cx_ + cy_ * cy_ + 1.f);
   Ray r;
-  r.norm_inv = 1.f / n;
+  r.norm_inv = n <= kRcpRnHi ? rcp_rn(n) : 1.f / n;   // n >= 1: the IEEE quotient 1 / n
   const float* m = v.c2w;
   r.dx = (m[0] * cx_ + m[1] * cy_ + m[2]) * r.norm_inv;
   r.dy = (m[4] * cx_ + m[5] * cy_ + m[6]) * r.norm_inv;
   r.dz = (m[8] * cx_ + m[9] * cy_ + m[10]) * r.norm_inv;
   r.ox = m[3]; r.oy = m[7]; r.oz = m[11];
-  r.ix = 1.f / r.dx; r.iy = 1.f / r.dy; r.iz = 1.f / r.dz;
+  // |d| <= 1: the IEEE quotient from rcp_rn unless a component is below its range (or zero)
+  r.ix = fabsf(r.dx) >= kRcpRnLo ? rcp_rn(r.dx) : 1.f / r.dx;
+  r.iy = fabsf(r.dy) >= kRcpRnLo ? rcp_rn(r.dy) : 1.f / r.dy;
+  r.iz = fabsf(r.dz) >= kRcpRnLo ? rcp_rn(r.dz) : 1.f / r.dz;
   r.octant = ray_octant(r.dx, r.dy, r.dz);
   return r;
 }
@@ -205,17 +220,6 @@ __device__ inline void load_voxel_geom(VoxelGeom& e, const VoxelRecord* __restri
   }
 }
 
-// Range in which rcp_rn is the correctly rounded reciprocal: |x| in [2^-100, 2^100].
-constexpr float kRcpRnLo = 0x1p-100f, kRcpRnHi = 0x1p100f;
-
-// 1/x rounded to nearest for |x| in [kRcpRnLo, kRcpRnHi]: the hardware reciprocal (within 1 ulp)
-// and one Newton step, whose residual 1 - x y is exact in an FMA.  The result equals the IEEE
-// quotient 1.0f / x for every float32 of that range on gfx950; svr_selfcheck_rcp_rn checks all
-// 2^23 significands of any exponent range on the device, against the IEEE division.
-__device__ inline float rcp_rn(float x) {
-  const float y = __builtin_amdgcn_rcpf(x);
-  return __builtin_fmaf(__builtin_fmaf(-x, y, 1.f), y, y);
-}
 
 // Per-entry constants of the SDF segment as the compositors stage them, once per entry and view.
 // Every ray of a view starts at the camera centre o, so the box corners are kept relative to it,

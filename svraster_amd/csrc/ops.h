@@ -42,7 +42,7 @@ hipError_t launch_voxel_activity(int P, const uint8_t* is_leaf, const float* geo
 hipError_t launch_gather_geo(int n, const int64_t* vox_key, const int64_t* care, const float* grid_pts,
                              float* geos, hipStream_t st);
 hipError_t launch_gather_geo_bw(int n, const int64_t* vox_key, const int64_t* care, const float* dgeos,
-                                float* dgrid, hipStream_t st);
+                                float* dgrid, hipStream_t st, int stride = 8);
 hipError_t launch_adam(bool sparse, bool biased, long long n, float* p, const float* g, float* m, float* v,
                        float step_size, float b1, float b2, float rsqrt_bc2, float eps, hipStream_t st);
 hipError_t launch_adam_table(bool sparse, long long n, float* p, const float* g, float* m, float* v,

@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
       dC0 = a.dL_dcolor[pb * 3 + 0 * HW + pid];
       dC1 = a.dL_dcolor[pb * 3 + 1 * HW + pid];
       dC2 = a.dL_dcolor[pb * 3 + 2 * HW + pid];
-      last_dT = a.dL_dT[pb + pid] + (V.bg[0] * dC0 + V.bg[1] * dC1 + V.bg[2] * dC2);
+      last_dT = (a.dL_dT ? a.dL_dT[pb + pid] : 0.f) + (V.bg[0] * dC0 + V.bg[1] * dC1 + V.bg[2] * dC2);
       if (kDepth) dD = a.dL_ddepth[pb * 4 + pid] * r.norm_inv;
       if (kNormal) {
         dN0 = a.dL_dnormal[pb * 3 + 0 * HW + pid];
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : 4) void ren
       dC0 = a.dL_dcolor[pb * 3 + 0 * HW + pid];
       dC1 = a.dL_dcolor[pb * 3 + 1 * HW + pid];
       dC2 = a.dL_dcolor[pb * 3 + 2 * HW + pid];
-      last_dT = a.dL_dT[pb + pid] + (V.bg[0] * dC0 + V.bg[1] * dC1 + V.bg[2] * dC2);
+      last_dT = (a.dL_dT ? a.dL_dT[pb + pid] : 0.f) + (V.bg[0] * dC0 + V.bg[1] * dC1 + V.bg[2] * dC2);
       if (kDepth) dD = a.dL_ddepth[pb * 4 + pid] * r.norm_inv;
       if (kNormal) {
         dN0 = a.dL_dnormal[pb * 3 + 0 * HW + pid];

@@ -194,14 +194,14 @@ __global__ __launch_bounds__(256) void gather_geo_kernel(int n, const int64_t* _
 
 __global__ __launch_bounds__(256) void gather_geo_bw_kernel(int n, const int64_t* __restrict__ vox_key,
                                                             const int64_t* __restrict__ care,
-                                                            const float* __restrict__ dgeos,
+                                                            const float* __restrict__ dgeos, int stride,
                                                             float* __restrict__ dgrid) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const long long v = care ? care[t] : t;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    const float g = dgeos[v * 8 + i];
+    const float g = dgeos[v * stride + i];
     if (g != 0.f) atomicAdd(dgrid + vox_key[v * 8 + i], g);
   }
 }
@@ -687,9 +687,9 @@ hipError_t launch_gather_geo(int n, const int64_t* vox_key, const int64_t* care,
 }
 
 hipError_t launch_gather_geo_bw(int n, const int64_t* vox_key, const int64_t* care, const float* dgeos,
-                                float* dgrid, hipStream_t st) {
+                                float* dgrid, hipStream_t st, int stride) {
   if (n <= 0) return hipSuccess;
-  gather_geo_bw_kernel<<<blocks_for(n), 256, 0, st>>>(n, vox_key, care, dgeos, dgrid);
+  gather_geo_bw_kernel<<<blocks_for(n), 256, 0, st>>>(n, vox_key, care, dgeos, stride, dgrid);
   return hipGetLastError();
 }
 
