@@ -366,7 +366,8 @@ def bench_train(args):
 
     def make_step(shape):
         r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key,
-                              cams, model.bg_color, sc.sh_degree, tile_shape=shape)
+                              cams, model.bg_color, sc.sh_degree, tile_shape=shape,
+                              overlap_streams=not args.serial_streams)
         return BatchedTrainStep(r, model._geo_grid_pts.detach(), model._sh0.detach(), model._shs.detach(),
                                 model._log_s.detach(), targets)
 
@@ -376,9 +377,10 @@ def bench_train(args):
     t0 = time.perf_counter()
     rast = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key,
                              cams, model.bg_color, sc.sh_degree, tile_shape=best["tile_shape"],
-                             fuse_gather_bw=best["fuse_gather_bw"])
+                             fuse_gather_bw=best["fuse_gather_bw"], overlap_streams=not args.serial_streams)
     it = BatchedTrainIteration(rast, model._geo_grid_pts.detach(), model._sh0.detach(), model._shs.detach(),
-                               model._log_s.detach(), targets, smooth, lr=lr, betas=betas, eps=eps)
+                               model._log_s.detach(), targets, smooth, lr=lr, betas=betas, eps=eps,
+                               overlap_smooth=not args.serial_streams)
     it()
     sync()
     log(f"[bench] batched set-up {time.perf_counter() - t0:.1f} s, keys {rast.key_count()} / capacity {rast.cap}")
@@ -428,6 +430,8 @@ def main():
     ap.add_argument("--workload", default="multi_32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
+    ap.add_argument("--serial-streams", action="store_true",
+                    help="every stage on one stream (per-kernel times without concurrent kernels, for profiles)")
     ap.add_argument("--no-autotune", action="store_true", help="default launch configuration, no tuning")
     ap.add_argument("--batched-only", action="store_true",
                     help="profiling aid: skip the per-view leg (the JSON then carries no per-view numbers)")
@@ -500,7 +504,8 @@ def main():
 
     def make_step(shape):
         r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key,
-                              cams, model.bg_color, sc.sh_degree, tile_shape=shape)
+                              cams, model.bg_color, sc.sh_degree, tile_shape=shape,
+                              overlap_streams=not args.serial_streams)
         return BatchedTrainStep(r, model._geo_grid_pts.detach(), model._sh0.detach(), model._shs.detach(),
                                 model._log_s.detach(), targets)
 
