@@ -297,6 +297,14 @@ int svr_voxel_order_rank(const int64_t* octree_paths, int P, int64_t* out_ranks,
  * to mismatches (device uint64[1], zeroed by the caller). */
 int svr_selfcheck_rcp_rn(int e_lo, int e_hi, unsigned long long* mismatches, void* stream);
 
+/* Device self-check (no counterpart in the reference): the reverse compositor recovers the
+ * transmittance before an entry as T / (1 - alpha) with div_rn (correctly rounded reciprocal,
+ * product, one FMA correction) instead of the IEEE division sequence, which must give the same
+ * bits.  This compares the two on 256 * n_blocks random operand pairs (y exponents in [-30, 0], x
+ * exponents in [-60, 0] or x = 0; n_blocks in 1..2^22, `seed` picks the sample) and adds the
+ * number of mismatches to mismatches (device uint64[1], zeroed by the caller). */
+int svr_selfcheck_div_rn(int n_blocks, unsigned seed, unsigned long long* mismatches, void* stream);
+
 /* Replaces UTILS::valid_gradient_table (src/utils.cu:190-252, 362-408) in two calls: count
  * (synchronises `stream`; workspace of svr_valid_gradient_table_workspace_bytes(N)), then
  * fill.  grid_mask uint8[(2^grid_res_pow2)^3] zeroed by the caller; keys / voxel ids int32[count]

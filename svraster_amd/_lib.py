@@ -53,6 +53,7 @@ _SIGS = {
     "svr_is_in_cone": (I, [F, F, F, P, P, LL, P, P]),
     "svr_voxel_order_rank": (I, [P, I, P, P]),
     "svr_selfcheck_rcp_rn": (I, [I, I, P, P]),
+    "svr_selfcheck_div_rn": (I, [I, U, P, P]),
     "svr_valid_gradient_table_workspace_bytes": (SZ, [I]),
     "svr_valid_gradient_table_count": (I, [P, P, P, F, I, P, I, P, ctypes.POINTER(LL), P]),
     "svr_valid_gradient_table_fill": (I, [P, P, P, F, I, P, I, P, P, P, P, P]),

@@ -102,12 +102,33 @@ __global__ __launch_bounds__(256) void selfcheck_rcp_rn_kernel(int e_lo, unsigne
   if (bad != 0u) atomicAdd(out, (unsigned long long)bad);
 }
 
+// Operands of div_rn's self-check: y with exponent in [-30, 0] and x with exponent in [-60, 0],
+// random significands (a hash of the sample index and the seed), every 4096th x zero.
+__device__ inline uint32_t mix32(uint32_t h) {
+  h ^= h >> 16; h *= 0x7FEB352Du; h ^= h >> 15; h *= 0x846CA68Bu; h ^= h >> 16;
+  return h;
+}
+
+__global__ __launch_bounds__(256) void selfcheck_div_rn_kernel(uint32_t seed, unsigned long long* __restrict__ out) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t h0 = mix32(i ^ mix32(seed)), h1 = mix32(h0 + 0x9E3779B9u), h2 = mix32(h1 + 0x9E3779B9u);
+  const float y = __uint_as_float(((127u - (h0 >> 27) % 31u) << 23) | (h1 & 0x7FFFFFu));
+  const float x = (i & 4095u) == 0u ? 0.f : __uint_as_float(((127u - (h2 >> 26) % 61u) << 23) | (h2 & 0x7FFFFFu));
+  const uint32_t bad = (uint32_t)(__float_as_uint(div_rn(x, y)) != __float_as_uint(x / y));
+  if (bad != 0u) atomicAdd(out, (unsigned long long)bad);
+}
+
 inline unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
 hipError_t launch_selfcheck_rcp_rn(int e_lo, int e_hi, unsigned long long* out, hipStream_t st) {
   selfcheck_rcp_rn_kernel<<<dim3((1u << 23) / 256u, (unsigned)(e_hi - e_lo + 1)), 256, 0, st>>>(e_lo, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_selfcheck_div_rn(int n_blocks, uint32_t seed, unsigned long long* out, hipStream_t st) {
+  selfcheck_div_rn_kernel<<<(unsigned)n_blocks, 256, 0, st>>>(seed, out);
   return hipGetLastError();
 }
 

@@ -199,7 +199,7 @@ int svr_timing_read(int kind, int reset, double* total_ms, long long* launches) 
   }
   return 0;
 }
-int svr_abi_version(void) { return 3; }
+int svr_abi_version(void) { return 4; }
 
 int svr_set_option(const char* name, int value) {
   g_err.clear();
@@ -760,6 +760,14 @@ int svr_selfcheck_rcp_rn(int e_lo, int e_hi, unsigned long long* mismatches, voi
     return fail(1, "exponent range must lie in [-126, 127]%s (got %lld)", "", (long long)e_lo);
   if (!mismatches) return fail(1, "mismatches is required%s", "", 0);
   SVR_TRY(launch_selfcheck_rcp_rn(e_lo, e_hi, mismatches, as_stream(stream)));
+  return 0;
+}
+
+int svr_selfcheck_div_rn(int n_blocks, unsigned seed, unsigned long long* mismatches, void* stream) {
+  g_err.clear();
+  if (n_blocks < 1 || n_blocks > (1 << 22)) return fail(1, "n_blocks must be 1..2^22%s (got %lld)", "", (long long)n_blocks);
+  if (!mismatches) return fail(1, "mismatches is required%s", "", 0);
+  SVR_TRY(launch_selfcheck_div_rn(n_blocks, (uint32_t)seed, mismatches, as_stream(stream)));
   return 0;
 }
 

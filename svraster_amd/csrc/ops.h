@@ -102,5 +102,6 @@ hipError_t launch_is_in_cone(long long N, float tfx, float tfy, float near_plane
                              const float* pts, uint8_t* out, hipStream_t st);
 hipError_t launch_voxel_order_rank(int P, const int64_t* paths, int64_t* out, hipStream_t st);
 hipError_t launch_selfcheck_rcp_rn(int e_lo, int e_hi, unsigned long long* out, hipStream_t st);
+hipError_t launch_selfcheck_div_rn(int n_blocks, uint32_t seed, unsigned long long* out, hipStream_t st);
 
 }  // namespace svr

@@ -572,7 +572,7 @@ __global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : 4) void ren
             kc = (uint32_t)s_pref[jc] + (w == 0 ? 0u : ((s_qo[jc] >> qsh) & 255u)) +
                  (uint32_t)__popcll(s_m[w][jc] & lt_mask);
           }
-          T = T / (1.f - alpha);
+          T = div_rn(T, 1.f - alpha);   // 1 - alpha > 2^-21, T normal: the IEEE quotient
           float dptw = dC0 * rgb.x + dC1 * rgb.y + dC2 * rgb.z;
           if (kNormal) {
             const float4 nn = s_nrm[j];
@@ -619,7 +619,7 @@ __global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : 4) void ren
             const float alpha = seg.alpha;
             const float ptw = alpha * Tp;
             const float p0v = seg.phi[0], p3v = seg.phi[3];
-            const float rden = 1.f / (p0v + kPhiEps);
+            const float rden = rcp_rn(p0v + kPhiEps);   // p0 + eps in [1e-6, 2]: the IEEE quotient
             const float q0 = p0v * (1.f - p0v), q3 = p3v * (1.f - p3v);
             float c[4];
             c[0] = dLda * s * q0 * (1.f - alpha) * rden;
@@ -640,7 +640,7 @@ __global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : 4) void ren
 #pragma unroll
               for (int mm = 0; mm < 3; ++mm) {
                 const float pk = seg.phi[mm], pk1 = seg.phi[mm + 1];
-                const float dk = 1.f / (pk + kPhiEps);
+                const float dk = rcp_rn(pk + kPhiEps);
                 if ((pk - pk1) * dk > 0.f) {
                   c[mm] += gk[mm] * pk1 * dk * dk * s * pk * (1.f - pk);
                   c[mm + 1] += -gk[mm] * dk * s * pk1 * (1.f - pk1);
@@ -652,7 +652,8 @@ __global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : 4) void ren
               if (!kDepth && (mm == 1 || mm == 2)) continue;   // c[1] = c[2] = 0 without depth
               const float fm = (float)mm;
               float wts[8];
-              trilinear_weights(seg.qx + seg.sx * fm, seg.qy + seg.sy * fm, seg.qz + seg.sz * fm, wts);
+              if (mm == 0) trilinear_weights(seg.qx, seg.qy, seg.qz, wts);   // the entry point itself
+              else trilinear_weights(seg.qx + seg.sx * fm, seg.qy + seg.sy * fm, seg.qz + seg.sz * fm, wts);
 #pragma unroll
               for (int i = 0; i < 8; ++i) gv[i] += c[mm] * wts[i];
             }

@@ -134,3 +134,18 @@ def test_preprocess_reciprocal_matches_ieee_division():
     _lib.check(lib.svr_selfcheck_rcp_rn(-100, 100, out.data_ptr(), None), "selfcheck_rcp_rn")
     torch.cuda.synchronize()
     assert int(out.item()) == 0
+
+
+@pytest.mark.gpu
+def test_reverse_walk_division_matches_ieee_division():
+    """The reverse compositor recovers T / (1 - alpha) with the correctly rounded reciprocal, a
+    product and one FMA correction; the backward equals the IEEE-division form bit for bit only if
+    that is the IEEE quotient.  2 x 2^28 random operand pairs must agree."""
+    from svraster_amd import _lib
+    lib = _lib.load()
+    out = torch.zeros(1, dtype=torch.int64, device="cuda")
+    for seed in (1, 2):
+        say(f"[binning] FMA-corrected quotient vs IEEE x / y, 2^28 pairs, seed {seed}")
+        _lib.check(lib.svr_selfcheck_div_rn(1 << 20, seed, out.data_ptr(), None), "selfcheck_div_rn")
+    torch.cuda.synchronize()
+    assert int(out.item()) == 0
