@@ -373,12 +373,14 @@ __global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : 4) void ren
   __shared__ float s_pa[kBwdWin];          // alpha per position; after the walk, T before the entry
   __shared__ float s_pd[kDepth ? kBwdWin : 1];
   __shared__ float s_pl[kBwdWin];          // dL/dalpha per position
+  __shared__ uint64_t s_rsb[kBwdWin / 64];  // run starts in the window, one bit per position
   __shared__ float2 s_red[4];
   __shared__ int s_tile[2];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const float s = *a.s_val;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
   float ds_hi = 0.f, ds_lo = 0.f;
+  if (t < kBwdWin / 64) s_rsb[t] = 0ull;   // wave 0 alone writes it from here on (marks, clears)
 
   for (int iter = 0;; ++iter) {
     int g;
@@ -513,18 +515,23 @@ __global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : 4) void ren
         w1 = min(min(total, lim), w0 + (uint32_t)kBwdWin);
         const uint32_t np = w1 - w0;
         s_cs[w][lane] = 0ull;
+        // Run starts of the window's entries as a bitmap over its positions (wave 0; the bitmap
+        // was cleared after the previous window's alpha pass).  Every entry has at least one
+        // pair, so the starts are distinct.
+        const uint32_t pl = (uint32_t)s_pref[ja + lane];   // run start of entry ja + lane
+        if (w == 0 && pl > w0 && pl < w1) atomicOr(&s_rsb[(pl - w0) >> 6], 1ull << ((pl - w0) & 63u));
         __syncthreads();
 
         // 2. alpha (and depth value) per position, 64-position chunks dealt to the waves in turn:
-        // the owning entry (binary search of the run starts), its block and pixel.
+        // the owning entry (the entries starting at or before the chunk's first position, plus
+        // the run starts between it and the position), its block and pixel.
         for (uint32_t c0 = 64u * (uint32_t)w; c0 < np; c0 += 256u) {
           const uint32_t k = c0 + lane;
+          const int j0 = ja - 1 + (int)__popcll(__ballot(pl <= w0 + c0));
+          const uint64_t starts = s_rsb[c0 >> 6] & ~1ull;   // run starts after the chunk's first position
           if (k < np) {
             const uint32_t p = w0 + k;
-            int j = ja;
-#pragma unroll
-            for (int st = 32; st >= 1; st >>= 1)
-              if ((uint32_t)s_pref[j + st] <= p) j += st;
+            const int j = j0 + (int)__popcll(starts & (~0ull >> (63 - lane)));
             const uint32_t rr = p - (uint32_t)s_pref[j];
             const uint32_t qo = s_qo[j];
             const uint32_t o1 = qo & 255u, o2 = (qo >> 8) & 255u, o3 = qo >> 16;
@@ -552,6 +559,7 @@ __global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : 4) void ren
         // 3. sequential reverse walk per pixel (wave w: block w) over its entries in the window
         // (ascending slot = back to front along the list), software-pipelined: the next entry's
         // position is looked up while this one is walked.
+        if (w == 0 && lane < kBwdWin / 64) s_rsb[lane] = 0ull;   // no wave reads it until the next window
         uint64_t cs = s_cs[w][lane];
         int jc = -1;
         uint32_t kc = 0;
