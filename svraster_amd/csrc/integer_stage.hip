@@ -71,13 +71,15 @@ __device__ inline void preprocess_one(const ViewDesc& v, bool leaf, float vcx, f
       const float wz[2] = {vcz + r * -1.f, vcz + r * 1.f};
       float ccx[8], ccy[8], ccz[8];
       bool front = true;
+      // !(z < near) && z >= 2^-100 is z >= max(near, 2^-100) (both false for a NaN z)
+      const float zlo = fmaxf(v.near_, kRcpRnLo);
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
         const float x = wx[(c >> 2) & 1], y = wy[(c >> 1) & 1], z = wz[c & 1];
         ccx[c] = m[0] * x + m[1] * y + m[2] * z + m[3];
         ccy[c] = m[4] * x + m[5] * y + m[6] * z + m[7];
         ccz[c] = m[8] * x + m[9] * y + m[10] * z + m[11];
-        front = front && !(ccz[c] < v.near_) && ccz[c] >= kRcpRnLo && ccz[c] <= kRcpRnHi;
+        front = front && ccz[c] >= zlo && ccz[c] <= kRcpRnHi;
       }
       if (front) {
         // Every corner in front of the near plane with depth in rcp_rn's range (the common
@@ -119,20 +121,27 @@ __device__ inline void preprocess_one(const ViewDesc& v, bool leaf, float vcx, f
       const float bx1 = fminf(v.focal_x * mxx + cxh, (float)v.W);
       const float by1 = fminf(v.focal_y * mxy + cyh, (float)v.H);
       if (!(bx0 > bx1 || by0 > by1)) {
-        const uint32_t rx0 = (uint32_t)__builtin_rintf(bx0), ry0 = (uint32_t)__builtin_rintf(by0);
-        const uint32_t rx1 = (uint32_t)__builtin_rintf(bx1), ry1 = (uint32_t)__builtin_rintf(by1);
-        bb.x = (rx0 << 16) | ry0;
-        bb.y = (rx1 << 16) | ry1;
-        tb = bb;
-        if (front) {
-          const uint32_t tx0 = max(rx0, (uint32_t)ceilf(bx0 - kTboxMargin));
-          const uint32_t ty0 = max(ry0, (uint32_t)ceilf(by0 - kTboxMargin));
-          const uint32_t tx1 = min(rx1, (uint32_t)floorf(bx1 + kTboxMargin));
-          const uint32_t ty1 = min(ry1, (uint32_t)floorf(by1 + kTboxMargin));
+        if (kTight && front) {
+          // The candidate rectangle alone: for 0 <= x < 2^18 (x -/+ 1/32 exact), ceil(x - 1/32)
+          // >= rint(x) and floor(x + 1/32) <= rint(x) (ties to even included), so it lies within
+          // the reference bbox without clamping to it.
+          const uint32_t tx0 = (uint32_t)ceilf(bx0 - kTboxMargin), ty0 = (uint32_t)ceilf(by0 - kTboxMargin);
+          const uint32_t tx1 = (uint32_t)floorf(bx1 + kTboxMargin), ty1 = (uint32_t)floorf(by1 + kTboxMargin);
           tb = make_uint2((tx0 << 16) | ty0, (tx1 << 16) | ty1);
-          if (kTight) {
-            bb = tb;
-            if (tx0 > tx1 || ty0 > ty1) qbits = 0;
+          bb = tb;
+          if (tx0 > tx1 || ty0 > ty1) qbits = 0;
+        } else {
+          const uint32_t rx0 = (uint32_t)__builtin_rintf(bx0), ry0 = (uint32_t)__builtin_rintf(by0);
+          const uint32_t rx1 = (uint32_t)__builtin_rintf(bx1), ry1 = (uint32_t)__builtin_rintf(by1);
+          bb.x = (rx0 << 16) | ry0;
+          bb.y = (rx1 << 16) | ry1;
+          tb = bb;
+          if (front) {
+            const uint32_t tx0 = max(rx0, (uint32_t)ceilf(bx0 - kTboxMargin));
+            const uint32_t ty0 = max(ry0, (uint32_t)ceilf(by0 - kTboxMargin));
+            const uint32_t tx1 = min(rx1, (uint32_t)floorf(bx1 + kTboxMargin));
+            const uint32_t ty1 = min(ry1, (uint32_t)floorf(by1 + kTboxMargin));
+            tb = make_uint2((tx0 << 16) | ty0, (tx1 << 16) | ty1);
           }
         }
         int tx0, ty0, tx1, ty1;
@@ -587,10 +596,12 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(
     if (nd) {
       bboxes[o] = bb;        // read only where n_dup > 0
       tboxes[o] = tb;
+      const uint32_t vbit = 1u << (v & 31);
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        c8[q] += ((qbits >> q) & 1u) ? nt : 0u;
-        m8[q] |= ((qbits >> q) & 1u) << (v & 31);
+        const uint32_t sq = 0u - ((qbits >> q) & 1u);   // all ones when the view sees octant q
+        c8[q] += nt & sq;
+        m8[q] |= vbit & sq;
       }
     }
     if ((v & 31) == 31 || v == V - 1) {
