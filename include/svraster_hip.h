@@ -340,15 +340,20 @@ int svr_batch_setup(void* ws, int n_views, int P, long long n_pixels, long long 
  * (auxiliary.h:40-59, forward.cu:749-802).  tmp: svr_batch_pair_order_bytes(P) bytes. */
 size_t svr_batch_pair_order_bytes(int P);
 int svr_batch_pair_order(int P, const int64_t* octree_paths, void* tmp, uint32_t* order_out, void* stream);
+/* Inverse of the pair rank order (no reference counterpart; once per voxel set):
+ * slots_out[pair_order[i]] = i for the 8P pairs, the rank-order slot of pair voxel << 3 | octant. */
+int svr_batch_pair_slots(int P, const uint32_t* pair_order, uint32_t* slots_out, void* stream);
 /* Preprocess (bbox and octant set; preprocess.cu:60-153) of every (view, voxel), voxel-major,
  * and the key count of every (voxel, octant) pair over all views, scanned in rank order.  Keys
  * cover the tiles of each (view, voxel)'s candidate rectangle -- the pixels whose centre lies in
  * the projected corners' rectangle (1/32-pixel margin) -- rather than its bbox, and a (view,
  * voxel) with no such pixel emits none: each tile's list is the reference's list less entries
- * whose ray / box test fails at every pixel of the tile, in the reference's order. */
+ * whose ray / box test fails at every pixel of the tile, in the reference's order.  With pair_slots
+ * (svr_batch_pair_slots of pair_order) the counts are stored straight at their rank-order slots;
+ * with pair_slots = NULL they are gathered through pair_order in a pass of their own. */
 int svr_batch_preprocess(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                          const float* vox_centers, const float* vox_lengths, const uint8_t* is_leaf,
-                         const uint32_t* pair_order, void* stream);
+                         const uint32_t* pair_order, const uint32_t* pair_slots, void* stream);
 /* out[i] = is_leaf[i] and voxel i can still produce a pair alpha >= 1e-5 (MIN_ALPHA,
  * config.h:19) for the corner SDFs `geos` [P,8] and s: a conservative bound, so a voxel with
  * out[i] = 0 contributes to no pixel (forward.cu:303-305 skips every such pair).  Passing `out`

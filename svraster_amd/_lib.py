@@ -61,7 +61,8 @@ _SIGS = {
     "svr_batch_setup": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P, P, P, P, P, I, I, P]),
     "svr_batch_pair_order_bytes": (SZ, [I]),
     "svr_batch_pair_order": (I, [I, P, P, P, P]),
-    "svr_batch_preprocess": (I, [P, I, I, LL, LL, LL, P, P, P, P, P]),
+    "svr_batch_pair_slots": (I, [I, P, P, P]),
+    "svr_batch_preprocess": (I, [P, I, I, LL, LL, LL, P, P, P, P, P, P]),
     "svr_voxel_activity": (I, [I, P, P, P, P, P, F, P, P]),
     "svr_batch_sh_forward": (I, [P, I, I, LL, LL, LL, P, P, P, I, I, P, P]),
     "svr_batch_voxel_records": (I, [I, P, P, P, P, P, P, F, I, P, P, P]),

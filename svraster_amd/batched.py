@@ -66,6 +66,10 @@ class BatchedRasterizer:
         _lib.check(self.lib.svr_batch_pair_order(self.P, _p(self.octpath), _p(tmp), _p(self.pair_order),
                                                  self._stream()), "svr_batch_pair_order")
         del tmp
+        # Rank-order slot of every pair (the inverse order): the preprocess stores key counts there.
+        self.pair_slots = torch.empty(8 * self.P, dtype=torch.int32, device=self.dev)
+        _lib.check(self.lib.svr_batch_pair_slots(self.P, _p(self.pair_order), _p(self.pair_slots), self._stream()),
+                   "svr_batch_pair_slots")
         self.Ws = [c.image_width for c in cameras]
         self.Hs = [c.image_height for c in cameras]
         self.tile_w, self.tile_h = int(tile_shape[0]), int(tile_shape[1])
@@ -152,7 +156,8 @@ class BatchedRasterizer:
     def preprocess(self, leaf: Optional[torch.Tensor] = None):
         leaf = self.is_leaf if leaf is None else leaf
         _lib.check(self.lib.svr_batch_preprocess(_p(self.ws), *self._dims(), _p(self.vox_center), _p(self.vox_size),
-                                                 _p(leaf), _p(self.pair_order), self._stream()),
+                                                 _p(leaf), _p(self.pair_order), _p(self.pair_slots),
+                                                 self._stream()),
                    "svr_batch_preprocess")
 
     def state_views(self):

@@ -199,7 +199,7 @@ int svr_timing_read(int kind, int reset, double* total_ms, long long* launches) 
   }
   return 0;
 }
-int svr_abi_version(void) { return 4; }
+int svr_abi_version(void) { return 5; }
 
 int svr_set_option(const char* name, int value) {
   g_err.clear();
@@ -858,17 +858,31 @@ int svr_batch_pair_order(int P, const int64_t* octree_paths, void* tmp, uint32_t
   return 0;
 }
 
+int svr_batch_pair_slots(int P, const uint32_t* pair_order, uint32_t* slots_out, void* stream) {
+  g_err.clear();
+  if (P <= 0 || P >= (1 << 29)) return fail(1, "voxel count must be 1..2^29%s (got %lld)", "", P);
+  if (!pair_order || !slots_out) return fail(1, "pair_order and slots_out are required%s%lld", "", 0);
+  SVR_TRY(launch_pair_slots(8LL * P, pair_order, slots_out, as_stream(stream)));
+  return 0;
+}
+
 int svr_batch_preprocess(void* ws, int n_views, int P, long long n_pixels, long long n_tiles, long long K_cap,
                          const float* vox_centers, const float* vox_lengths, const uint8_t* is_leaf,
-                         const uint32_t* pair_order, void* stream) {
+                         const uint32_t* pair_order, const uint32_t* pair_slots, void* stream) {
   g_err.clear();
   if (int e = check_batch(n_views, P, n_pixels, n_tiles, K_cap)) return e;
+  if (!pair_order && !pair_slots) return fail(1, "pair_order or pair_slots is required%s%lld", "", 0);
   hipStream_t st = as_stream(stream);
   BatchPtrs bp(ws, n_views, P, n_pixels, n_tiles, K_cap);
   const long long n = 8LL * P;
-  SVR_TRY(launch_preprocess_views(bp.g.views, n_views, P, vox_centers, vox_lengths, is_leaf, nullptr, bp.g.bbox,
-                                  bp.g.tbox, bp.g.quad, bp.pr.pair_cnt, bp.pr.vmask, st));
-  SVR_TRY(launch_pair_gather(n, pair_order, bp.pr.pair_cnt, bp.pr.cnt_ranked, st));
+  if (pair_slots) {
+    SVR_TRY(launch_preprocess_views(bp.g.views, n_views, P, vox_centers, vox_lengths, is_leaf, nullptr, bp.g.bbox,
+                                    bp.g.tbox, bp.g.quad, pair_slots, bp.pr.cnt_ranked, bp.pr.vmask, st));
+  } else {
+    SVR_TRY(launch_preprocess_views(bp.g.views, n_views, P, vox_centers, vox_lengths, is_leaf, nullptr, bp.g.bbox,
+                                    bp.g.tbox, bp.g.quad, nullptr, bp.pr.pair_cnt, bp.pr.vmask, st));
+    SVR_TRY(launch_pair_gather(n, pair_order, bp.pr.pair_cnt, bp.pr.cnt_ranked, st));
+  }
   SVR_TRY(launch_scan(bp.pr.cnt_ranked, nullptr, n, bp.pr.partial, bp.pr.scan_ranked, true, st));
   SVR_TRY(launch_key_peak(bp.pr.scan_ranked + (n - 1), bp.g.misc + 8, st));
   return 0;

@@ -577,7 +577,7 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(
     const ViewDesc* __restrict__ views, int V, int P, const float* __restrict__ centers,
     const float* __restrict__ lengths, const uint8_t* __restrict__ is_leaf, uint32_t* __restrict__ n_dup,
     uint2* __restrict__ bboxes, uint2* __restrict__ tboxes, uint8_t* __restrict__ quads,
-    uint32_t* __restrict__ pair_cnt, uint32_t* __restrict__ pair_vmask) {
+    const uint32_t* __restrict__ pair_slots, uint32_t* __restrict__ pair_cnt, uint32_t* __restrict__ pair_vmask) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P) return;
   const bool leaf = is_leaf[i] != 0;
@@ -612,9 +612,23 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(
       }
     }
   }
-  uint4* out = reinterpret_cast<uint4*>(pair_cnt + 8LL * i);
-  out[0] = make_uint4(c8[0], c8[1], c8[2], c8[3]);
-  out[1] = make_uint4(c8[4], c8[5], c8[6], c8[7]);
+  if (pair_slots) {
+    // Straight into rank order: eight scattered stores instead of a gather pass over all pairs.
+    const uint4* sl = reinterpret_cast<const uint4*>(pair_slots + 8LL * i);
+    const uint4 s0 = sl[0], s1 = sl[1];
+    pair_cnt[s0.x] = c8[0]; pair_cnt[s0.y] = c8[1]; pair_cnt[s0.z] = c8[2]; pair_cnt[s0.w] = c8[3];
+    pair_cnt[s1.x] = c8[4]; pair_cnt[s1.y] = c8[5]; pair_cnt[s1.z] = c8[6]; pair_cnt[s1.w] = c8[7];
+  } else {
+    uint4* out = reinterpret_cast<uint4*>(pair_cnt + 8LL * i);
+    out[0] = make_uint4(c8[0], c8[1], c8[2], c8[3]);
+    out[1] = make_uint4(c8[4], c8[5], c8[6], c8[7]);
+  }
+}
+
+__global__ __launch_bounds__(256) void pair_slots_kernel(long long n_pairs, const uint32_t* __restrict__ order,
+                                                         uint32_t* __restrict__ slots) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_pairs) slots[order[i]] = (uint32_t)i;
 }
 
 // Key counts in rank order.
@@ -863,10 +877,17 @@ hipError_t launch_pair_rank(long long n_pairs, const int64_t* octpaths, uint64_t
 
 hipError_t launch_preprocess_views(const ViewDesc* views, int V, int P, const float* centers, const float* lengths,
                                    const uint8_t* is_leaf, uint32_t* n_dup, uint2* bboxes, uint2* tboxes,
-                                   uint8_t* quads, uint32_t* pair_cnt, uint32_t* pair_vmask, hipStream_t st) {
+                                   uint8_t* quads, const uint32_t* pair_slots, uint32_t* pair_cnt,
+                                   uint32_t* pair_vmask, hipStream_t st) {
   if (P <= 0 || V <= 0) return hipSuccess;
   preprocess_views_kernel<<<ceil_div(P, 256), 256, 0, st>>>(views, V, P, centers, lengths, is_leaf, n_dup, bboxes,
-                                                            tboxes, quads, pair_cnt, pair_vmask);
+                                                            tboxes, quads, pair_slots, pair_cnt, pair_vmask);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_slots(long long n_pairs, const uint32_t* order, uint32_t* slots, hipStream_t st) {
+  if (n_pairs <= 0) return hipSuccess;
+  pair_slots_kernel<<<(unsigned)((n_pairs + 255) / 256), 256, 0, st>>>(n_pairs, order, slots);
   return hipGetLastError();
 }
 

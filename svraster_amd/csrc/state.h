@@ -241,9 +241,14 @@ hipError_t launch_tile_sort(uint32_t* tiles, uint32_t* vals, uint32_t* tiles_alt
                             uint32_t* scan_partial, hipStream_t st);
 hipError_t launch_pair_rank(long long n_pairs, const int64_t* octpaths, uint64_t* keys, uint32_t* vals,
                             hipStream_t st);
+// pair_slots (may be null): rank-order slot of every (voxel, octant) pair; then the key counts go
+// straight to pair_cnt[pair_slots[8 v + q]] (rank order), else to pair_cnt[8 v + q].
 hipError_t launch_preprocess_views(const ViewDesc* views, int V, int P, const float* centers, const float* lengths,
                                    const uint8_t* is_leaf, uint32_t* n_dup, uint2* bboxes, uint2* tboxes,
-                                   uint8_t* quads, uint32_t* pair_cnt, uint32_t* pair_vmask, hipStream_t st);
+                                   uint8_t* quads, const uint32_t* pair_slots, uint32_t* pair_cnt,
+                                   uint32_t* pair_vmask, hipStream_t st);
+// slots[order[i]] = i: the inverse of the pair rank order.
+hipError_t launch_pair_slots(long long n_pairs, const uint32_t* order, uint32_t* slots, hipStream_t st);
 // *peak = max(*peak, *total): the largest key count asked of a fixed-capacity binning state.
 hipError_t launch_key_peak(const uint32_t* total, uint32_t* peak, hipStream_t st);
 hipError_t launch_pair_gather(long long n_pairs, const uint32_t* order, const uint32_t* pair_cnt,

@@ -13,7 +13,7 @@ def test_library_exports_every_header_symbol():
 
 def test_abi_version_and_state_sizes():
     lib = _lib.load()
-    assert lib.svr_abi_version() == 4
+    assert lib.svr_abi_version() == 5
     g1 = lib.svr_geom_state_bytes(1, 1000)
     g2 = lib.svr_geom_state_bytes(1, 2000)
     assert g2 > g1 > 17 * 1000

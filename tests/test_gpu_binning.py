@@ -119,7 +119,18 @@ def test_pair_rank_sort_stable(P, kind):
     from oracle.binning import REP3
     keys = (octpaths.astype(np.uint64)[:, None] ^ REP3[None, :]).reshape(-1) & np.uint64((1 << 48) - 1)
     expect = np.argsort(keys, kind="stable").astype(np.uint32)
-    np.testing.assert_array_equal(_pair_order_gpu(octpaths), expect)
+    order = _pair_order_gpu(octpaths)
+    np.testing.assert_array_equal(order, expect)
+    # svr_batch_pair_slots: the rank-order slot of every pair, the inverse permutation
+    from svraster_amd import _lib
+    lib = _lib.load()
+    o = torch.from_numpy(order.view(np.int32)).cuda()
+    slots = torch.full((8 * P,), -1, dtype=torch.int32, device="cuda")
+    _lib.check(lib.svr_batch_pair_slots(P, o.data_ptr(), slots.data_ptr(), None), "svr_batch_pair_slots")
+    torch.cuda.synchronize()
+    inv = np.empty(8 * P, dtype=np.uint32)
+    inv[expect] = np.arange(8 * P, dtype=np.uint32)
+    np.testing.assert_array_equal(slots.cpu().numpy().view(np.uint32), inv)
 
 
 @pytest.mark.gpu
