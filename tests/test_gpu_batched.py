@@ -291,3 +291,29 @@ def test_voxel_records_gather_corners_and_activity():
             assert torch.equal(bl, act) and 0 < int(bl.sum()) <= int(leaf.sum())
         else:
             assert torch.equal(bl, leaf)
+
+
+@pytest.mark.gpu
+def test_rank_order_count_stores_match_gather():
+    """The preprocess stores each (voxel, octant) key count at its rank-order slot
+    (svr_batch_pair_slots); passing no slots gathers the counts through the rank order in a pass of
+    its own instead.  Both give the same key lists, contributor counts and images, bit for bit."""
+    say("[batched] key counts stored at rank-order slots vs gathered through the order, 40 views")
+    w, sc, cams, targets = _setup(40)
+    m = VoxelModel(sc)
+    r = BatchedRasterizer(sc.octpath, sc.octlevel, sc.vox_center, sc.vox_size, sc.is_leaf, sc.vox_key, cams,
+                          m.bg_color, sc.sh_degree)
+    s_val = torch.exp(m._log_s.detach().reshape(1) * 10.0)
+    args = (m._geo_grid_pts.detach(), m._sh0.detach(), m._shs.detach(), s_val)
+    r.forward(*args)
+    torch.cuda.synchronize()
+    want = [t.clone() for t in r.state_views()] + [r.color.clone()]
+    slots, r.pair_slots = r.pair_slots, None
+    try:
+        r.forward(*args)
+        torch.cuda.synchronize()
+        got = list(r.state_views()) + [r.color]
+    finally:
+        r.pair_slots = slots
+    for name, a, b in zip(("ranges", "vals", "n_contrib", "tile_last", "color"), got, want):
+        assert torch.equal(a, b), name
