@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 def bwd_kernel_name():
     """Name of the multi-view backward compositor instantiation the current options launch."""
     from svraster_amd import _lib
-    halves = _lib.get_option("batch_depth") // 64
+    halves = _lib.get_option("bwd_batch_depth") // 64
     return f"render_bwd_pairs_kernel<false, false, false, true, {halves}>"
 
 

@@ -36,9 +36,11 @@ int svr_timing_read(int kind, int reset, double* total_ms, long long* launches);
 /* Runtime options (no counterpart in the reference), the knobs the autotuner
  * (svraster_amd/autotune.py) sets.  "compositor_variant": 0 = pair-compacted compositor kernels
  * (default), 1 = one-entry-at-a-time kernels.  "batch_depth": entries staged in LDS per round by
- * the multi-view pair-compacted compositors, 64 or 128 (default).  "persistent_blocks_per_cu":
- * workgroups per CU of the multi-view persistent launches, 1..32 (default 8).  None of them
- * changes an image or a gradient.  svr_get_option returns -1 for an unknown name. */
+ * the multi-view pair-compacted forward compositor, 64 or 128 (default).  "bwd_batch_depth": the
+ * same for the pair-compacted reverse compositors (one-view and multi-view), 64 (default; five
+ * workgroups per CU) or 128.  "persistent_blocks_per_cu": workgroups per CU of the multi-view
+ * persistent launches, 1..32 (default 8).  None of them changes an image; gradients change only by
+ * the order of float additions.  svr_get_option returns -1 for an unknown name. */
 int svr_set_option(const char* name, int value);
 int svr_get_option(const char* name);
 

@@ -35,18 +35,20 @@ from . import _lib
 
 TILE_SHAPES: Tuple[Tuple[int, int], ...] = ((16, 16), (32, 8), (8, 32))
 BATCH_DEPTHS: Tuple[int, ...] = (128, 64)
+BWD_BATCH_DEPTHS: Tuple[int, ...] = (64, 128)
 BLOCKS_PER_CU: Tuple[int, ...] = (8, 4, 16)
 VARIANTS: Tuple[int, ...] = (0, 1)
 FUSE_GATHER_BW: Tuple[bool, ...] = (False, True)
 
-DEFAULT = {"tile_shape": (16, 16), "batch_depth": 128, "persistent_blocks_per_cu": 8, "compositor_variant": 0,
-           "fuse_gather_bw": False}
+DEFAULT = {"tile_shape": (16, 16), "batch_depth": 128, "bwd_batch_depth": 64, "persistent_blocks_per_cu": 8,
+           "compositor_variant": 0, "fuse_gather_bw": False}
 
 
 def apply_options(cfg: Dict, step=None) -> None:
     """Set the library options of a configuration (the tile shape belongs to the rasterizer,
     the gradient route to the step's rasterizer when one is given)."""
     _lib.set_option("batch_depth", int(cfg["batch_depth"]))
+    _lib.set_option("bwd_batch_depth", int(cfg.get("bwd_batch_depth", DEFAULT["bwd_batch_depth"])))
     _lib.set_option("persistent_blocks_per_cu", int(cfg["persistent_blocks_per_cu"]))
     _lib.set_option("compositor_variant", int(cfg["compositor_variant"]))
     rast = getattr(step, "r", None)
@@ -148,9 +150,10 @@ def autotune(make_step: Callable[[Tuple[int, int]], Callable[[], None]], signatu
         if ms < best_ms:
             best_ms, best = ms, cfg
     step = make_step(best["tile_shape"])
-    # 2.-5. the launch options and the gradient route, one at a time
-    for key, values in (("batch_depth", BATCH_DEPTHS), ("persistent_blocks_per_cu", BLOCKS_PER_CU),
-                        ("compositor_variant", VARIANTS), ("fuse_gather_bw", FUSE_GATHER_BW)):
+    # 2.-6. the launch options and the gradient route, one at a time
+    for key, values in (("batch_depth", BATCH_DEPTHS), ("bwd_batch_depth", BWD_BATCH_DEPTHS),
+                        ("persistent_blocks_per_cu", BLOCKS_PER_CU), ("compositor_variant", VARIANTS),
+                        ("fuse_gather_bw", FUSE_GATHER_BW)):
         for val in values:
             if val == best[key]:
                 continue

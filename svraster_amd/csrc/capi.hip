@@ -77,6 +77,9 @@ struct LaunchTimer {
 // persistent_blocks_per_cu = workgroups per CU of the multi-view persistent launches.
 int g_compositor_variant = 0;
 int g_batch_depth = 128;
+// Entries staged per round by the reverse compositor (64 or 128): with 64 its LDS and register
+// budget fit five workgroups per CU instead of four.
+int g_bwd_batch_depth = 64;
 int g_blocks_per_cu = 8;
 
 int tile_bits_for(long long n_tiles) {
@@ -213,6 +216,11 @@ int svr_set_option(const char* name, int value) {
     g_batch_depth = value;
     return 0;
   }
+  if (name && strcmp(name, "bwd_batch_depth") == 0) {
+    if (value != 64 && value != 128) return fail(1, "bwd_batch_depth must be 64 or 128%s (got %lld)", "", value);
+    g_bwd_batch_depth = value;
+    return 0;
+  }
   if (name && strcmp(name, "persistent_blocks_per_cu") == 0) {
     if (value < 1 || value > 32) return fail(1, "persistent_blocks_per_cu must be 1..32%s (got %lld)", "", value);
     g_blocks_per_cu = value;
@@ -224,6 +232,7 @@ int svr_set_option(const char* name, int value) {
 int svr_get_option(const char* name) {
   if (name && strcmp(name, "compositor_variant") == 0) return g_compositor_variant;
   if (name && strcmp(name, "batch_depth") == 0) return g_batch_depth;
+  if (name && strcmp(name, "bwd_batch_depth") == 0) return g_bwd_batch_depth;
   if (name && strcmp(name, "persistent_blocks_per_cu") == 0) return g_blocks_per_cu;
   return -1;
 }
@@ -375,7 +384,7 @@ int svr_rasterize_backward(long long K, int vox_geo_mode, int density_mode, int 
   a.work_counter = nullptr;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;
-  a.halves = g_batch_depth / 64;
+  a.halves = g_bwd_batch_depth / 64;
   a.blocks_per_cu = g_blocks_per_cu;
   const ContribPtrs cp((char*)binning_buffer + BinLayout(K).total, K);
   a.cvals = cp.cvals; a.pmask = cp.pmask; a.cranges = ip.cranges;
@@ -1024,7 +1033,7 @@ int svr_batch_backward(void* ws, int n_views, int P, long long n_pixels, long lo
   a.tile_view = bp.im.tile_view;
   a.need_depth = need_depth; a.need_normal = need_normal;
   a.variant = g_compositor_variant;
-  a.halves = g_batch_depth / 64;
+  a.halves = g_bwd_batch_depth / 64;
   a.blocks_per_cu = g_blocks_per_cu;
   a.cvals = bp.c.cvals; a.pmask = bp.c.pmask; a.cranges = bp.im.cranges;
   {

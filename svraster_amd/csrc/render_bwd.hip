@@ -354,7 +354,7 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(BwdArgs a) {
 constexpr int kBwdWin = 512;   // pair positions per window (the four waves together)
 
 template <bool kDepth, bool kNormal, bool kConcen, bool kQueue, int kHalves>
-__global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : 4) void render_bwd_pairs_kernel(BwdArgs a) {
+__global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : (kHalves == 1 ? 5 : 4)) void render_bwd_pairs_kernel(BwdArgs a) {
   constexpr int kStage = 64 * kHalves;   // entries staged per round (two 64-entry halves or one)
   __shared__ float4 s_ch[kStage], s_ml[kStage], s_g0[kStage], s_g1[kStage], s_rgb[kStage];
   __shared__ float4 s_nrm[kNormal ? kStage : 1];
@@ -741,6 +741,7 @@ static hipError_t launch_bwd_c(const BwdArgs& a, hipStream_t st, unsigned grid) 
       else render_bwd_pairs_kernel<D, N, false, true, 2><<<grid, 256, 0, st>>>(a);
     } else {
       if (concen) render_bwd_pairs_kernel<D, N, true, false, 2><<<grid, 256, 0, st>>>(a);
+      else if (a.halves == 1) render_bwd_pairs_kernel<D, N, false, false, 1><<<grid, 256, 0, st>>>(a);
       else render_bwd_pairs_kernel<D, N, false, false, 2><<<grid, 256, 0, st>>>(a);
     }
   }

@@ -33,7 +33,7 @@ def default_options():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("cfg", [
-    {"tile_shape": (32, 8)}, {"tile_shape": (8, 32)}, {"batch_depth": 64},
+    {"tile_shape": (32, 8)}, {"tile_shape": (8, 32)}, {"batch_depth": 64}, {"bwd_batch_depth": 128},
     {"persistent_blocks_per_cu": 4}, {"tile_shape": (32, 8), "batch_depth": 64, "persistent_blocks_per_cu": 16},
 ])
 def test_launch_configuration_keeps_results(cfg, default_options):
@@ -69,10 +69,11 @@ def test_autotune_picks_and_caches(tmp_path, default_options):
         best, table = autotune.autotune(make_step, sig, steps=2, warmup=1, log=say)
         assert best["tile_shape"] in autotune.TILE_SHAPES
         assert best["batch_depth"] in autotune.BATCH_DEPTHS
+        assert best["bwd_batch_depth"] in autotune.BWD_BATCH_DEPTHS
         assert best["persistent_blocks_per_cu"] in autotune.BLOCKS_PER_CU
         assert best["compositor_variant"] in autotune.VARIANTS
         assert best["fuse_gather_bw"] in autotune.FUSE_GATHER_BW
-        assert len(table) == 8
+        assert len(table) == 9
         assert _lib.get_option("batch_depth") == best["batch_depth"]
         cached, rows = autotune.autotune(make_step, sig)
         assert cached == best and rows[0][0] == "cached"
