@@ -440,15 +440,24 @@ __global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : (kHalves ==
     if (kNormal) s_dn[w][lane] = make_float4(dN0, dN1, dN2, 0.f);
     if (kConcen) s_gt[w][lane] = make_float4(gt0, gt1, gt2, 0.f);
 
-    // Pipelined staging: entry data one batch ahead in registers, its value two batches ahead;
-    // each wave's block masks of the next batch (lane: slots lane and lane + 64).
+    // Each wave's block masks of the next batch are loaded a batch ahead (lane: slots lane and
+    // lane + 64).
     const float* rgbs_v = a.rgbs + (long long)vi * a.rgb_view_stride;
     const uint64_t rgb_row0 = (uint64_t)vi * (uint64_t)a.P;   // this view's first row of dL_drgb_views
+    // Staging (threads t < kStage, one entry each).  With 128-entry batches (four workgroups per
+    // CU) the entry data is loaded a batch ahead into registers and its value two batches ahead;
+    // with 64-entry batches the registers go to occupancy (five workgroups per CU, no spills): the
+    // value is loaded a batch ahead and the entry data when the batch is staged.
+    constexpr bool kPrefetch = kHalves > 1;
     EntryRegsB er;
     uint32_t nval = 0;
     if (t < kStage) {
-      if (t < n) fetch_entry_b(er, a, a.cvals[top - t], rgbs_v);
-      if (kStage + t < n) nval = a.cvals[top - kStage - t];
+      if (kPrefetch) {
+        if (t < n) fetch_entry_b(er, a, a.cvals[top - t], rgbs_v);
+        if (kStage + t < n) nval = a.cvals[top - kStage - t];
+      } else if (t < n) {
+        nval = a.cvals[top - t];
+      }
     }
     uint64_t nm_lo = lane < n ? a.pmask[4 * (top - lane) + w] : 0ull;
     uint64_t nm_hi = (kHalves > 1 && 64 + lane < n) ? a.pmask[4 * (top - 64 - lane) + w] : 0ull;
@@ -457,6 +466,7 @@ __global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : (kHalves ==
       __syncthreads();
       const int cnt = min(kStage, n - base);
       if (t < cnt) {
+        if (!kPrefetch) fetch_entry_b(er, a, nval, rgbs_v);
         store_entry_b(er, kNormal, r, s_ch, s_ml, s_g0, s_g1, s_rgb, s_nrm, t);
         s_used[t] = 0u;
       }
@@ -466,8 +476,12 @@ __global__ __launch_bounds__(256, kDepth || kNormal || kConcen ? 3 : (kHalves ==
       if (kHalves > 1) s_m[w][lane + 64] = nm_hi;
       __syncthreads();
       if (t < kStage) {
-        if (base + kStage + t < n) fetch_entry_b(er, a, nval, rgbs_v);
-        if (base + 2 * kStage + t < n) nval = a.cvals[top - (base + 2 * kStage + t)];
+        if (kPrefetch) {
+          if (base + kStage + t < n) fetch_entry_b(er, a, nval, rgbs_v);
+          if (base + 2 * kStage + t < n) nval = a.cvals[top - (base + 2 * kStage + t)];
+        } else if (base + kStage + t < n) {
+          nval = a.cvals[top - (base + kStage + t)];
+        }
       }
       {
         const int nb = base + kStage;
