@@ -337,6 +337,15 @@ __device__ inline void wave_lds_sync() {
 }
 
 
+// p + popcount(m): two v_bcnt with accumulator.  Written as such because the compiler otherwise
+// distributes a following scale-by-4 of the sum over its three terms (three more shifts).
+__device__ inline uint32_t pref_plus_rank(uint32_t p, uint64_t m) {
+  uint32_t r, k;
+  asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"((uint32_t)m), "v"(p));
+  asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(k) : "v"((uint32_t)(m >> 32)), "v"(r));
+  return k;
+}
+
 // Mask of entry slot j for the wave's pixels: bbox, octant of the pixel rays, live pixels.
 __device__ inline uint64_t entry_block_mask(const uint2* s_bb, const float4* s_rgb, int j, const WaveLattice& L,
                                             const uint64_t* s_om, uint64_t live) {
@@ -538,19 +547,24 @@ __global__ __launch_bounds__(256, kDepth ? 3 : 4) void render_fwd_pairs_kernel(F
           wave_lds_sync();
           // 4. composite this pixel's contributing pairs of the window, in entry order.
           // The 128-bit set as the word being consumed and the one after it (entries jb + bit).
+          // The word being consumed is kept non-zero while entries remain: it takes the next word
+          // as soon as it empties, so the loop tests one word.
           uint64_t cur = s_cs[w][0][lane], nxt = s_cs[w][1][lane];
           int jb = 0;
-          while ((cur | nxt) != 0ull && !done) {
-            if (cur == 0ull) { cur = nxt; nxt = 0ull; jb = 64; }
+          if (cur == 0ull) { cur = nxt; nxt = 0ull; jb = 64; }
+          const float* const pa = &s_pa[w][0] - w0;   // indexed by absolute position
+          const float* const pd = &s_pd[w][0] - w0;
+          while (cur != 0ull && !done) {
             const int j = jb + __builtin_ctzll(cur);
             cur &= cur - 1ull;
-            const uint32_t k = (uint32_t)s_pref[w][j] + (uint32_t)__popcll(s_m[w][j] & lt_mask) - w0;
-            const float alpha = s_pa[w][k];
+            if (cur == 0ull) { cur = nxt; nxt = 0ull; jb = 64; }
+            const uint32_t k = pref_plus_rank(s_pref[w][j], s_m[w][j] & lt_mask);
+            const float alpha = pa[k];
             const float4 rgb = s_rgb[j];
             const float wgt = T * alpha;
             C0 += wgt * rgb.x; C1 += wgt * rgb.y; C2 += wgt * rgb.z;
             if (kDepth) {
-              D += T * s_pd[w][k];
+              D += T * pd[k];
               if (T > 0.5f) { dmed_vox = (int)(__float_as_uint(rgb.w) & kVoxMask); dmed_T = T; }
             }
             if (kNormal) {
